@@ -1,0 +1,119 @@
+"""ctypes binding of libfealpy_amd.so (declared in include/fealpy_amd.h).
+
+The product path has no CPU fallback: every numerical routine of the package
+calls into this library, and ``lib()`` raises if the library is missing or no
+GPU is visible.  torch is imported first so the library binds to the same HIP
+runtime (libamdhip64.so.7) torch already loaded.
+"""
+import ctypes
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfealpy_amd.so")
+
+FA_OK, FA_EINVAL, FA_ENOMEM, FA_EHIP, FA_ENOCONV, FA_ECAPACITY = range(6)
+FA_F64, FA_F32 = 0, 1
+COEF_NONE, COEF_CONST, COEF_CELL, COEF_CELLQ = 0, 1, 2, 3
+
+_p = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_f64 = ctypes.c_double
+_sz = ctypes.c_size_t
+_pi32 = ctypes.POINTER(ctypes.c_int32)
+_pi64 = ctypes.POINTER(ctypes.c_int64)
+_pf64 = ctypes.POINTER(ctypes.c_double)
+
+_SIGNATURES = {
+    "fa_last_error": (ctypes.c_char_p, []),
+    "fa_version": (ctypes.c_int, []),
+    "fa_adjacency_work_bytes": (_sz, [_i64, _i32, _i64]),
+    "fa_adjacency_build": (ctypes.c_int, [_p, _i64, _i32, _i64, _p, _p, _pi32, _p, _sz, _p]),
+    "fa_pattern_work_bytes": (_sz, [_i64]),
+    "fa_pattern_count": (ctypes.c_int, [_p, _i64, _i32, _i64, _p, _p, _i32, _p, _pi64, _pi32, _p, _sz, _p]),
+    "fa_pattern_fill": (ctypes.c_int, [_p, _i64, _i32, _i64, _p, _p, _i32, _p, _p, _p]),
+    "fa_element_matrices": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _p, _p, _i64,
+                                           _p, _p, _p, ctypes.c_int, _f64, _p, ctypes.c_int, _f64, _p, _p, _p]),
+    "fa_element_vectors": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _p, _p, _i64,
+                                          _p, _p, ctypes.c_int, _f64, _p, _p, _p]),
+    "fa_gather_matrix": (ctypes.c_int, [ctypes.c_int, _i64, _i32, _p, _p, _p, _p, _p, _i32, _p, _p, _p]),
+    "fa_gather_vector": (ctypes.c_int, [ctypes.c_int, _i64, _i32, _p, _p, _p, _p, _p]),
+    "fa_dirichlet_work_bytes": (_sz, [_i64]),
+    "fa_dirichlet_count": (ctypes.c_int, [_i64, _p, _p, _p, _p, _pi64, _p, _sz, _p]),
+    "fa_dirichlet_fill": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p, _p, _p, _p, _p]),
+    "fa_dirichlet_rhs": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p, _p, _p, _p, _p]),
+    "fa_spmv": (ctypes.c_int, [ctypes.c_int, _i64, _i64, _p, _p, _p, _p, _p, _p]),
+    "fa_reduce_work_bytes": (_sz, []),
+    "fa_dot": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p, _sz, _p]),
+    "fa_axpby": (ctypes.c_int, [ctypes.c_int, _i64, _f64, _p, _f64, _p, _p]),
+    "fa_cast": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64, _p, _p, _p]),
+    "fa_cg_work_bytes": (_sz, [ctypes.c_int, _i64]),
+    "fa_cg": (ctypes.c_int, [ctypes.c_int, _i64, _i64, _p, _p, _p, _p, _p, _f64, _f64, _i64, _i32, _pi64, _pf64,
+                             _p, _sz, _p]),
+}
+
+_lib = None
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def load(path=LIB_PATH):
+    """Load the shared library and declare every C-ABI signature (no GPU needed)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise NativeError(f"{path} is missing: build it with `python -m fealpy_amd.build`")
+        lib = ctypes.CDLL(path)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def lib():
+    """The library, for compute calls: requires a visible GPU."""
+    if not torch.cuda.is_available():
+        raise NativeError("fealpy_amd needs a ROCm GPU (MI355X): torch.cuda.is_available() is False")
+    return load()
+
+
+def check(status, what=""):
+    if status == FA_OK:
+        return
+    msg = _lib.fa_last_error().decode() if _lib is not None else ""
+    if status == FA_EINVAL:
+        raise ValueError(f"{what}: {msg}")
+    if status == FA_ENOMEM:
+        raise MemoryError(f"{what}: {msg}")
+    raise NativeError(f"{what} failed (status {status}): {msg}")
+
+
+def ptr(t):
+    """Device pointer of a tensor (None -> NULL)."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("fealpy_amd kernels need device (cuda) tensors")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def dtype_code(dt):
+    if dt == torch.float64:
+        return FA_F64
+    if dt == torch.float32:
+        return FA_F32
+    raise ValueError(f"unsupported value dtype {dt}; use float64 or float32")
+
+
+def workspace(nbytes, device):
+    return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)

@@ -1,0 +1,93 @@
+"""Dirichlet boundary condition (fealpy/fem/dirichlet_bc.py:13-234)."""
+import torch
+
+from .. import _native as nat
+from ..sparse.tensors import COOTensor, CSRTensor
+
+
+class DirichletBC:
+    def __init__(self, space, gd=None, *, threshold=None, method=None):
+        if isinstance(space, tuple):
+            raise NotImplementedError("tuples of spaces are not provided")
+        self.space = space
+        self.gd = gd
+        self.threshold = threshold
+        self.method = method
+        self.bctype = 'Dirichlet'
+        self.gdof = space.number_of_global_dofs()
+        if torch.is_tensor(threshold):
+            self.is_boundary_dof = threshold
+        else:
+            self.is_boundary_dof = space.is_boundary_dof(threshold=threshold, method=method)
+        self.boundary_dof_index = torch.nonzero(self.is_boundary_dof)[:, 0]
+        self._bd_u8 = self.is_boundary_dof.to(torch.uint8).contiguous()
+
+    def check_matrix(self, matrix):
+        if not isinstance(matrix, (COOTensor, CSRTensor)):
+            raise ValueError('The type of matrix must be COOTensor or CSRTensor.')
+        if len(matrix.shape) != 2:
+            raise ValueError('The matrix must be a 2-D sparse COO matrix.')
+        if matrix.shape[0] != matrix.shape[1]:
+            raise ValueError('The matrix must be a square matrix.')
+        if matrix.shape[0] != self.gdof:
+            raise ValueError('The matrix size must match the gdof of the space.')
+        return matrix
+
+    def check_vector(self, vector):
+        if not torch.is_tensor(vector):
+            raise ValueError('The type of vector must be a tensor.')
+        if vector.ndim not in (1, 2):
+            raise ValueError('The vector must be 1-D or 2-D.')
+        if vector.shape[0] != self.gdof:
+            raise ValueError('The vector size must match the gdof of the space.')
+        return vector
+
+    def apply(self, A, f, uh=None, gd=None, *, check=True):
+        """-> (A with boundary rows/columns removed and a unit boundary
+        diagonal, f - A g with f[bd] = g[bd])."""
+        f = self.apply_vector(f, A, uh, gd, check=check)
+        A = self.apply_matrix(A, check=check)
+        return A, f
+
+    def _csr(self, A):
+        return A.coalesce().tocsr() if isinstance(A, COOTensor) else A
+
+    def apply_matrix(self, matrix, *, check=True):
+        A = self.check_matrix(matrix) if check else matrix
+        was_coo = isinstance(A, COOTensor)
+        A = self._csr(A)
+        L = nat.lib()
+        n = A.shape[0]
+        dev = A.device
+        new_ptr = torch.empty(n + 1, dtype=torch.int32, device=dev)
+        ws = nat.workspace(L.fa_dirichlet_work_bytes(n), dev)
+        nnz = nat.ctypes.c_int64(0)
+        nat.check(L.fa_dirichlet_count(n, nat.ptr(A.crow), nat.ptr(A.col), nat.ptr(self._bd_u8), nat.ptr(new_ptr),
+                                       nat.ctypes.byref(nnz), nat.ptr(ws), ws.numel(), nat.stream()),
+                  "fa_dirichlet_count")
+        new_idx = torch.empty(int(nnz.value), dtype=torch.int32, device=dev)
+        new_val = torch.empty(int(nnz.value), dtype=A.dtype, device=dev)
+        nat.check(L.fa_dirichlet_fill(nat.dtype_code(A.dtype), n, nat.ptr(A.crow), nat.ptr(A.col),
+                                      nat.ptr(A.values), nat.ptr(self._bd_u8), nat.ptr(new_ptr), nat.ptr(new_idx),
+                                      nat.ptr(new_val), nat.stream()), "fa_dirichlet_fill")
+        out = CSRTensor(new_ptr, new_idx, new_val, A.shape)
+        return out.tocoo() if was_coo else out
+
+    def apply_vector(self, vector, matrix, uh=None, gd=None, *, check=True):
+        A = self.check_matrix(matrix) if check else matrix
+        f = self.check_vector(vector) if check else vector
+        gd = self.gd if gd is None else gd
+        if gd is None:
+            raise RuntimeError("The boundary condition is None.")
+        if uh is None:
+            uh = torch.zeros_like(f)
+        uh, _ = self.space.boundary_interpolate(gd=gd, uh=uh, threshold=self.threshold, method=self.method)
+        g = uh.array if hasattr(uh, 'array') else uh
+        A = self._csr(A)
+        g = g.to(A.dtype).contiguous()
+        f = f.to(A.dtype).contiguous()
+        out = torch.empty_like(f)
+        nat.check(nat.lib().fa_dirichlet_rhs(nat.dtype_code(A.dtype), A.shape[0], nat.ptr(A.crow), nat.ptr(A.col),
+                                             nat.ptr(A.values), nat.ptr(self._bd_u8), nat.ptr(g), nat.ptr(f),
+                                             nat.ptr(out), nat.stream()), "fa_dirichlet_rhs")
+        return out

@@ -1,0 +1,178 @@
+/*
+ * fealpy_amd.h -- C-ABI of libfealpy_amd.so, the MI355X (gfx950) kernels of the
+ * Lagrange finite-element assemble-and-solve path.
+ *
+ * Conventions
+ *   - Every array argument is a DEVICE pointer (HBM), except where noted.
+ *   - Integer index arrays are int32 (cells, DOFs, CSR indptr/indices); sizes
+ *     are int64.  CSR indices inside a row are sorted ascending.
+ *   - `dtype` selects the floating type of values/vectors: FA_F64 or FA_F32.
+ *     Geometry (node coordinates) and quadrature tables are always FP64.
+ *   - `stream` is a hipStream_t passed as an opaque pointer (NULL = default
+ *     stream).  All work is enqueued on it; functions that must read a result
+ *     back to the host say so and synchronise the stream.
+ *   - Scratch memory is caller-owned: `*_work_bytes` reports the size a call
+ *     needs and the call takes `work`/`work_bytes`.  No entry point allocates
+ *     device memory.
+ *   - Return value: FA_OK or an error code; fa_last_error() gives the message
+ *     (thread-local).
+ *   - Sums of element contributions into one global entry are formed in a
+ *     fixed order (ascending cell id, as the reference's stable lexsort +
+ *     sequential np.add.at), with no floating-point atomics, so results are
+ *     bit-for-bit reproducible run to run.
+ *
+ * Reference interfaces replaced (weihuayi/fealpy, paths relative to its root)
+ * are cited on each declaration.
+ */
+#ifndef FEALPY_AMD_H
+#define FEALPY_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* fa_stream_t;
+
+typedef enum {
+    FA_OK = 0,
+    FA_EINVAL = 1,     /* bad argument (shape, size, dtype)                      */
+    FA_ENOMEM = 2,     /* caller workspace too small                             */
+    FA_EHIP = 3,       /* HIP runtime / launch error                              */
+    FA_ENOCONV = 4,    /* iterative solver stopped at maxit                       */
+    FA_ECAPACITY = 5   /* a row exceeds a kernel's per-row capacity              */
+} fa_status;
+
+enum { FA_F64 = 0, FA_F32 = 1 };
+
+/* coefficient kinds for element kernels */
+enum {
+    FA_COEF_NONE = 0,   /* operator absent                                        */
+    FA_COEF_CONST = 1,  /* scalar `scale`                                         */
+    FA_COEF_CELL = 2,   /* scale * coef[c],      coef shape (NC,)                 */
+    FA_COEF_CELLQ = 3   /* scale * coef[c*NQ+q], coef shape (NC, NQ)              */
+};
+
+const char* fa_last_error(void);
+int fa_version(void);
+
+/* ---------------------------------------------------------------- pattern --
+ * Sparsity of the global matrix of a cell->DOF map (kernel "b").
+ * Replaces COOTensor.coalesce().tocsr() on the index part
+ * (fealpy/sparse/coo_tensor.py:137-157,184-213) called from
+ * BilinearForm.assembly (fealpy/fem/bilinear_form.py:83-105).
+ *
+ * Step 1 fa_adjacency_build: DOF -> (cell, local) incidence, a CSR
+ *   adj_ptr[gdof+1], adj[nc*ldof] whose entries e = cell*ldof + local are
+ *   sorted ascending within each DOF.  Reports the largest incidence count.
+ * Step 2 fa_pattern_count: per-row count of distinct columns, exclusive scan
+ *   into indptr[gdof+1]; synchronises and returns nnz and the longest row.
+ * Step 3 fa_pattern_fill: writes indices[nnz] (sorted per row).
+ */
+size_t fa_adjacency_work_bytes(int64_t nc, int32_t ldof, int64_t gdof);
+int fa_adjacency_build(const int32_t* cell2dof, int64_t nc, int32_t ldof, int64_t gdof,
+                       int32_t* adj_ptr, int32_t* adj, int32_t* max_degree_host,
+                       void* work, size_t work_bytes, fa_stream_t stream);
+size_t fa_pattern_work_bytes(int64_t gdof);
+int fa_pattern_count(const int32_t* cell2dof, int64_t nc, int32_t ldof, int64_t gdof,
+                     const int32_t* adj_ptr, const int32_t* adj, int32_t max_degree,
+                     int32_t* indptr, int64_t* nnz_host, int32_t* max_row_host,
+                     void* work, size_t work_bytes, fa_stream_t stream);
+int fa_pattern_fill(const int32_t* cell2dof, int64_t nc, int32_t ldof, int64_t gdof,
+                    const int32_t* adj_ptr, const int32_t* adj, int32_t max_degree,
+                    const int32_t* indptr, int32_t* indices, fa_stream_t stream);
+
+/* ------------------------------------------------------- element matrices --
+ * Scalar Lagrange element matrices of ScalarDiffusionIntegrator (default
+ * variant, fealpy/fem/scalar_diffusion_integrator.py:54-63) plus
+ * ScalarMassIntegrator (fealpy/fem/scalar_mass_integrator.py:48-54), summed
+ * as a GroupIntegrator (fealpy/fem/integrator.py:361-378):
+ *   K_c[i][j] = |T_c| sum_q w_q ( kd_cq grad(phi_i).grad(phi_j) + km_cq phi_i phi_j )
+ * Geometry per cell (grad lambda, |T|) follows numpy_backend.py:586-629 and
+ * tetrahedron_mesh.py:165-175.  Tables (FP64, host-prepared):
+ *   qw[nq], phi[nq*ldof] (mass only), gphi_u[nq*ldof*(td+1)] (diffusion only).
+ * Output elem[nc*ldof*ldof] in `dtype`.
+ */
+int fa_element_matrices(int dtype, int td, int ldof, int nq,
+                        const double* node, const int32_t* cell, int64_t nc,
+                        const double* qw, const double* phi, const double* gphi_u,
+                        int diff_kind, double diff_scale, const double* diff_coef,
+                        int mass_kind, double mass_scale, const double* mass_coef,
+                        void* elem, fa_stream_t stream);
+
+/* Element load vectors of ScalarSourceIntegrator
+ * (fealpy/fem/scalar_source_integrator.py:51-58 -> functional.py:27-65):
+ *   b_c[i] = |T_c| sum_q w_q f_cq phi_i(q);  output elem[nc*ldof] in `dtype`. */
+int fa_element_vectors(int dtype, int td, int ldof, int nq,
+                       const double* node, const int32_t* cell, int64_t nc,
+                       const double* qw, const double* phi,
+                       int src_kind, double src_scale, const double* src,
+                       void* elem, fa_stream_t stream);
+
+/* ----------------------------------------------------------- value gather --
+ * Kernel "c": CSR values from element matrices by a fixed-order segmented
+ * sum over the DOF->cell incidence (no atomics).  Replaces the value part of
+ * coalesce (np.add.at in stable-sorted COO order, coo_tensor.py:197-200).
+ * values[nnz] in `dtype`; elem as written by fa_element_matrices (`dtype`). */
+int fa_gather_matrix(int dtype, int64_t gdof, int32_t ldof, const int32_t* cell2dof,
+                     const int32_t* adj_ptr, const int32_t* adj,
+                     const int32_t* indptr, const int32_t* indices, int32_t max_row,
+                     const void* elem, void* values, fa_stream_t stream);
+/* Dense global vector from element vectors: LinearForm.assembly(format='dense')
+ * (fealpy/fem/linear_form.py:66-86 -> COOTensor.to_dense, coo_tensor.py:116-130). */
+int fa_gather_vector(int dtype, int64_t gdof, int32_t ldof,
+                     const int32_t* adj_ptr, const int32_t* adj,
+                     const void* elem, void* out, fa_stream_t stream);
+
+/* ------------------------------------------------------------ Dirichlet BC --
+ * DirichletBC.apply (fealpy/fem/dirichlet_bc.py:101-234), reference-compatible
+ * form: the new matrix drops every entry in a boundary row or boundary column
+ * (_mul, :213-229) and gets a unit diagonal on boundary rows (spdiags + add,
+ * :123-147); f <- f - A uh, f[bd] <- uh[bd] (apply_vector :149-198).
+ * fa_dirichlet_count writes new_indptr[n+1] and returns the new nnz (syncs).
+ * is_bd: uint8[n]; uh: boundary values (zeros off the boundary), `dtype`. */
+size_t fa_dirichlet_work_bytes(int64_t n);
+int fa_dirichlet_count(int64_t n, const int32_t* indptr, const int32_t* indices, const uint8_t* is_bd,
+                       int32_t* new_indptr, int64_t* new_nnz_host,
+                       void* work, size_t work_bytes, fa_stream_t stream);
+int fa_dirichlet_fill(int dtype, int64_t n, const int32_t* indptr, const int32_t* indices, const void* values,
+                      const uint8_t* is_bd, const int32_t* new_indptr, int32_t* new_indices, void* new_values,
+                      fa_stream_t stream);
+int fa_dirichlet_rhs(int dtype, int64_t n, const int32_t* indptr, const int32_t* indices, const void* values,
+                     const uint8_t* is_bd, const void* uh, const void* f, void* f_out, fa_stream_t stream);
+
+/* ------------------------------------------------------------------ SpMV --
+ * y = A x.  Replaces CSRTensor.matmul -> bm.csr_spmm -> scipy csr_matvec
+ * (fealpy/sparse/csr_tensor.py:411-452, backend/numpy_backend.py:181-199). */
+int fa_spmv(int dtype, int64_t n, int64_t nnz, const int32_t* indptr, const int32_t* indices,
+            const void* values, const void* x, void* y, fa_stream_t stream);
+
+/* ------------------------------------------------------------ vector ops --
+ * Building blocks of fealpy/solver/cg.py's vector algebra (bm.sum(r*z),
+ * x + alpha p, ...).  Reductions use a fixed grid and a fixed tree order
+ * (deterministic) and write their FP64 result to device memory `out`. */
+size_t fa_reduce_work_bytes(void);
+int fa_dot(int dtype, int64_t n, const void* x, const void* y, double* out, void* work, size_t work_bytes,
+           fa_stream_t stream);
+int fa_axpby(int dtype, int64_t n, double a, const void* x, double b, void* y, fa_stream_t stream);
+int fa_cast(int dtype_in, int dtype_out, int64_t n, const void* x, void* y, fa_stream_t stream);
+
+/* -------------------------------------------------------------------- CG --
+ * cg(A, b, x0, M=None, atol, rtol, maxit) of fealpy/solver/cg.py:14-123 for
+ * one right-hand side, with the reference's stopping rule
+ * (||r|| < atol or ||r|| < rtol ||b|| or niter >= maxit, checked after each
+ * update).  x holds x0 on entry and the solution on exit.  The host checks
+ * convergence every `check_every` iterations; iterations past convergence are
+ * skipped on the device.  Returns FA_OK or FA_ENOCONV (maxit reached);
+ * niter/residual are written to host memory. */
+size_t fa_cg_work_bytes(int dtype, int64_t n);
+int fa_cg(int dtype, int64_t n, int64_t nnz, const int32_t* indptr, const int32_t* indices, const void* values,
+          const void* b, void* x, double atol, double rtol, int64_t maxit, int32_t check_every,
+          int64_t* niter_host, double* residual_host, void* work, size_t work_bytes, fa_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FEALPY_AMD_H */

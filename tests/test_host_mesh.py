@@ -1,0 +1,117 @@
+"""Host logic of the product (torch, CPU here): mesh topology, DOF numbering,
+reference tables -- bit-exact against the oracle and the reference's golden
+vectors; and the C-ABI library exports every symbol include/fealpy_amd.h declares."""
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+from fealpy_amd.functionspace import LagrangeFESpace
+from fealpy_amd.functionspace.basis import grad_shape_function, multi_index_matrix, shape_function
+from fealpy_amd.mesh import TetrahedronMesh, TriangleMesh
+from fealpy_amd.quadrature import simplex_rule
+from oracle import basis as ob
+from oracle.mesh import tet_from_box, tri_from_box
+
+CPU = torch.device("cpu")
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.mark.parametrize("n", [(1, 1, 1), (3, 2, 1), (4, 4, 4)])
+def test_tet_topology_matches_oracle(n):
+    m = TetrahedronMesh.from_box(nx=n[0], ny=n[1], nz=n[2], device=CPU)
+    o = tet_from_box(nx=n[0], ny=n[1], nz=n[2])
+    np.testing.assert_array_equal(_np(m.cell), o.cell)
+    np.testing.assert_allclose(_np(m.node), o.node, atol=0)
+    for k in ("face", "edge", "face2cell", "cell2face", "cell2edge"):
+        np.testing.assert_array_equal(_np(getattr(m, k)), getattr(o, k), err_msg=k)
+    for p in (1, 2, 3, 4):
+        np.testing.assert_array_equal(_np(m.cell_to_ipoint(p)), o.cell_to_ipoint(p), err_msg=f"c2d p={p}")
+        np.testing.assert_array_equal(_np(m.face_to_ipoint(p)), o.face_to_ipoint(p), err_msg=f"f2d p={p}")
+        np.testing.assert_allclose(_np(m.interpolation_points(p)), o.interpolation_points(p), atol=1e-15)
+        np.testing.assert_array_equal(_np(LagrangeFESpace(m, p).is_boundary_dof()), o.is_boundary_dof(p))
+        assert m.number_of_global_ipoints(p) == o.number_of_global_ipoints(p)
+
+
+@pytest.mark.parametrize("n", [(1, 1), (2, 3), (5, 4)])
+def test_tri_topology_matches_oracle(n):
+    m = TriangleMesh.from_box(nx=n[0], ny=n[1], device=CPU)
+    o = tri_from_box(nx=n[0], ny=n[1])
+    np.testing.assert_array_equal(_np(m.cell), o.cell)
+    for k in ("edge", "face2cell", "cell2edge"):
+        np.testing.assert_array_equal(_np(getattr(m, k)), getattr(o, k), err_msg=k)
+    for p in (1, 2, 3, 4):
+        np.testing.assert_array_equal(_np(m.cell_to_ipoint(p)), o.cell_to_ipoint(p), err_msg=f"c2d p={p}")
+        np.testing.assert_allclose(_np(m.interpolation_points(p)), o.interpolation_points(p), atol=1e-15)
+        np.testing.assert_array_equal(_np(LagrangeFESpace(m, p).is_boundary_dof()), o.is_boundary_dof(p))
+
+
+def test_golden_tet_dof_maps(golden):
+    T = golden("ref_tetrahedron_mesh.npz")
+    m = TetrahedronMesh.from_box(nx=3, ny=2, nz=1, device=CPU)
+    np.testing.assert_array_equal(_np(m.face_to_ipoint(4)), T["face_to_ipoint::0::f2p"])
+    np.testing.assert_array_equal(_np(m.cell_to_ipoint(4)), T["cell_to_ipoint::0::cell2ipoint"])
+    np.testing.assert_allclose(_np(m.grad_lambda()), T["grad_lambda::0::glambda"], atol=1e-14)
+    np.testing.assert_allclose(_np(m.entity_measure('cell')), T["entity_measure::0::cm"], atol=1e-14)
+    th = lambda p: (p[..., 0] < 0.5) & (p[..., 1] < 0.5) & (p[..., 2] < 0.5)
+    mt = TetrahedronMesh.from_box(nx=3, ny=2, nz=1, threshold=th, device=CPU)
+    for k in ("edge", "face", "cell", "face2cell"):
+        np.testing.assert_array_equal(_np(getattr(mt, k)), T["from_box::0::" + k])
+
+
+def test_golden_tri_and_space(golden):
+    R = golden("ref_triangle_mesh.npz")
+    L = golden("ref_lagrange_fe_space.npz")
+    m = TriangleMesh.from_box(nx=2, ny=2, device=CPU)
+    np.testing.assert_array_equal(_np(m.cell_to_ipoint(4)), R["interpolation_point_data::0::cip"])
+    np.testing.assert_allclose(_np(TriangleMesh.from_box((-1, 1, -1, 1), 2, 2, device=CPU).grad_lambda()),
+                               R["grad_lambda_data::0::val"], atol=1e-14)
+    s = LagrangeFESpace(TriangleMesh.from_box((0, 1, 0, 1), 1, 1, device=CPU), 2)
+    np.testing.assert_array_equal(_np(s.cell_to_dof()), L["triangle_mesh_one_box::0::cell_to_dof"])
+    np.testing.assert_array_equal(_np(s.is_boundary_dof()), L["triangle_mesh_one_box::0::is_boundary_dof"])
+
+
+@pytest.mark.parametrize("TD,p", [(2, p) for p in (1, 2, 3, 4)] + [(3, p) for p in (1, 2, 3, 4)])
+def test_reference_tables(TD, p):
+    np.testing.assert_array_equal(multi_index_matrix(p, TD), ob.multi_index_matrix(p, TD))
+    for q in ((1, 2, 3, 5, 9, 10, 11) if TD == 2 else (1, 2, 4, 6, 7, 8)):
+        bcs, ws = simplex_rule(TD, q).get_quadrature_points_and_weights()
+        ob_b, ob_w = ob.quadrature(TD, q)
+        np.testing.assert_allclose(_np(bcs), ob_b, atol=1e-15)
+        np.testing.assert_allclose(_np(ws), ob_w, atol=1e-15)
+        np.testing.assert_allclose(shape_function(ob_b, p), ob.simplex_shape_function(ob_b, p), atol=1e-13)
+        np.testing.assert_allclose(grad_shape_function(ob_b, p), ob.simplex_grad_shape_function(ob_b, p),
+                                   atol=1e-12)
+
+
+def test_cabi_exports_every_declared_symbol():
+    """libfealpy_amd.so loads without a GPU and exports each function of include/*.h."""
+    import ctypes
+    from fealpy_amd import _native
+    lib = _native.load()
+    names = set()
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        src = open(os.path.join(ROOT, "include", h)).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        names |= set(re.findall(r"\b(fa_[a-z0-9_]+)\s*\(", src))
+    assert len(names) > 10
+    for n in sorted(names):
+        assert hasattr(lib, n), n
+        assert n in _native._SIGNATURES, f"{n} has no ctypes signature"
+    assert lib.fa_version() >= 1
+    assert lib.fa_cg_work_bytes(0, 1000) > 3 * 8000
+
+
+def test_product_refuses_cpu_tensors():
+    """The product path has no CPU fallback: compute entry points raise."""
+    from fealpy_amd import _native
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    with pytest.raises(_native.NativeError):
+        _native.lib()
