@@ -211,23 +211,22 @@ __global__ void __launch_bounds__(RED_BLOCK) k_cg_init(int64_t n, const int32_t*
     }
     rr = block_sum(rr);
     bb = block_sum(bb);
-    if (publish_partial(rr, partials, &st->counter)) {
-        double t = sum_partials(partials, gridDim.x);
-        if (threadIdx.x == 0) st->rz = t;
-    }
-    // second value: reuse partials after the first sum is consumed -> separate slots
-    __shared__ bool last2;
+    // one arrival counter for both sums: partials[0..G) = r.r, partials[G..2G) = b.b
+    __shared__ bool last;
     if (threadIdx.x == 0) {
+        partials[blockIdx.x] = rr;
         partials[gridDim.x + blockIdx.x] = bb;
         __threadfence();
         unsigned prev = atomicAdd(&st->counter, 1u);
-        last2 = (prev == 2 * gridDim.x - 1);
+        last = (prev == gridDim.x - 1);
     }
     __syncthreads();
-    if (last2) {
-        double t = sum_partials(partials + gridDim.x, gridDim.x);
+    if (last) {
+        const double trr = sum_partials(partials, gridDim.x);
+        const double tbb = sum_partials(partials + gridDim.x, gridDim.x);
         if (threadIdx.x == 0) {
-            st->bnorm = sqrt(t);
+            st->rz = trr;
+            st->bnorm = sqrt(tbb);
             st->niter = 0;
             st->done = (st->bnorm < 1e-15) ? 2 : 0;
             st->counter = 0u;
