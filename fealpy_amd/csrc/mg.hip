@@ -1,0 +1,834 @@
+// Kernel "d": Galerkin coarse operators, multigrid V-cycle and the PCG driver.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+
+#include "common.hpp"
+#include "reduce.hpp"
+#include "sell.hpp"
+
+namespace fa {
+
+// ---------------------------------------------------------------------------
+// R A P: symbolic (count / fill) with one wavefront per coarse row.
+// Candidates J are enumerated R-entry by R-entry (sequential), lanes over the
+// entries of A's row, each lane appending the columns of its P row through a
+// wave prefix sum; then bitonic sort + unique in LDS (as the pattern kernel).
+template <int CAP>
+__global__ void __launch_bounds__(WAVE) k_rap_rows(int64_t nrows, const int32_t* __restrict__ rp,
+                                                   const int32_t* __restrict__ ri, const int32_t* __restrict__ ap,
+                                                   const int32_t* __restrict__ ai, const int32_t* __restrict__ pp,
+                                                   const int32_t* __restrict__ pi, int32_t* __restrict__ rowcount,
+                                                   const int32_t* __restrict__ cptr, int32_t* __restrict__ cidx,
+                                                   int32_t* __restrict__ max_row, int32_t* __restrict__ overflow) {
+    __shared__ int32_t buf[CAP];
+    const int lane = threadIdx.x;
+    int32_t row_max = 0;
+    for (int64_t I = blockIdx.x; I < nrows; I += gridDim.x) {
+        int m = 0;
+        bool ovf = false;
+        for (int32_t u = rp[I]; u < rp[I + 1]; ++u) {
+            const int32_t i = ri[u];
+            const int32_t a0 = ap[i], a1 = ap[i + 1];
+            for (int32_t t0 = a0; t0 < a1; t0 += WAVE) {
+                const int32_t t = t0 + lane;
+                int32_t j = -1, cnt = 0;
+                if (t < a1) {
+                    j = ai[t];
+                    cnt = pp[j + 1] - pp[j];
+                }
+                const int inc = wave_inclusive_scan(cnt);
+                const int tot = __shfl(inc, WAVE - 1);
+                if (m + tot <= CAP) {
+                    for (int c = 0; c < cnt; ++c) buf[m + inc - cnt + c] = pi[pp[j] + c];
+                } else {
+                    ovf = true;
+                }
+                m += tot;
+            }
+        }
+        __syncthreads();
+        if (ovf) {
+            if (lane == 0) atomicOr(overflow, 1);
+            m = 0;
+        }
+        int P = 2;
+        while (P < m) P <<= 1;
+        for (int t = m + lane; t < P; t += WAVE) buf[t] = INT_MAX;
+        __syncthreads();
+        for (int k = 2; k <= P; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int t = lane; t < P; t += WAVE) {
+                    int p = t ^ j;
+                    if (p > t) {
+                        int32_t a = buf[t], b = buf[p];
+                        if ((a > b) == ((t & k) == 0)) {
+                            buf[t] = b;
+                            buf[p] = a;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        int32_t count = 0;
+        const int32_t base = cidx ? cptr[I] : 0;
+        for (int t0 = 0; t0 < P; t0 += WAVE) {
+            const int t = t0 + lane;
+            const int32_t v = t < P ? buf[t] : INT_MAX;
+            const bool keep = v != INT_MAX && (t == 0 || buf[t - 1] != v);
+            const int inc = wave_inclusive_scan(keep ? 1 : 0);
+            if (cidx && keep) cidx[base + count + inc - 1] = v;
+            count += __shfl(inc, WAVE - 1);
+        }
+        if (count == 0) {  // empty coarse row (Dirichlet node): unit diagonal
+            if (cidx && lane == 0) cidx[base] = static_cast<int32_t>(I);
+            count = 1;
+        }
+        if (!cidx && lane == 0) rowcount[I] = count;
+        row_max = max(row_max, count);
+        __syncthreads();
+    }
+    if (max_row && lane == 0) atomicMax(max_row, row_max);
+}
+
+// R A P numeric: one wavefront per coarse row; lane l takes A entries l, l+64,
+// ... of each R entry (in R order) and accumulates r*a*p into its private LDS
+// row; the row value is then the sum over lanes in lane order.
+__global__ void __launch_bounds__(WAVE) k_rap_values(int64_t nrows, const int32_t* __restrict__ rp,
+                                                     const int32_t* __restrict__ ri, const double* __restrict__ rv,
+                                                     const int32_t* __restrict__ ap, const int32_t* __restrict__ ai,
+                                                     const double* __restrict__ av, const int32_t* __restrict__ pp,
+                                                     const int32_t* __restrict__ pi, const double* __restrict__ pv,
+                                                     const int32_t* __restrict__ cptr, const int32_t* __restrict__ cidx,
+                                                     int lmax, double* __restrict__ cval) {
+    extern __shared__ char smem[];
+    int32_t* cols = reinterpret_cast<int32_t*>(smem);
+    double* acc = reinterpret_cast<double*>(smem + align_up(sizeof(int32_t) * lmax, 16));  // [WAVE][lmax]
+    const int lane = threadIdx.x;
+    for (int64_t I = blockIdx.x; I < nrows; I += gridDim.x) {
+        const int32_t c0 = cptr[I];
+        const int L = cptr[I + 1] - c0;
+        for (int t = lane; t < L; t += WAVE) cols[t] = cidx[c0 + t];
+        for (int t = 0; t < L; ++t) acc[lane * lmax + t] = 0.0;
+        __syncthreads();
+        for (int32_t u = rp[I]; u < rp[I + 1]; ++u) {
+            const int32_t i = ri[u];
+            const double r = rv[u];
+            for (int32_t t = ap[i] + lane; t < ap[i + 1]; t += WAVE) {
+                const int32_t j = ai[t];
+                const double ra = r * av[t];
+                for (int32_t q = pp[j]; q < pp[j + 1]; ++q) {
+                    const int32_t J = pi[q];
+                    int lo = 0, hi = L;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (cols[mid] < J) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    if (lo < L && cols[lo] == J) acc[lane * lmax + lo] += ra * pv[q];
+                }
+            }
+        }
+        __syncthreads();
+        const bool empty = (rp[I] == rp[I + 1]);
+        for (int t = lane; t < L; t += WAVE) {
+            double s = 0.0;
+            for (int l = 0; l < WAVE; ++l) s += acc[l * lmax + t];
+            cval[c0 + t] = empty ? 1.0 : s;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Chebyshev smoothing of D^-1 A (recurrence of Saad, Iterative Methods, 12.3):
+//   FROM_ZERO: x = c2 * D^-1 b, d = x
+//   otherwise: res = b - A x_in; d = c1 d + c2 D^-1 res; x_out = x_in + d
+//   (FIRST: the c1 d term is absent, so d need not be initialised)
+template <class T, bool FROM_ZERO, bool FIRST>
+__global__ void __launch_bounds__(256) k_cheb(int64_t n, const int32_t* __restrict__ sp, const int32_t* __restrict__ sc,
+                                              const T* __restrict__ sv, const T* __restrict__ dinv,
+                                              const T* __restrict__ b, const T* __restrict__ xin, T* __restrict__ xout,
+                                              T* __restrict__ d, T c1, T c2, const int* __restrict__ done) {
+    if (done && *done) return;
+    for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
+         row += (int64_t)gridDim.x * blockDim.x) {
+        if (FROM_ZERO) {
+            if (row < n) {
+                const T v = c2 * dinv[row] * b[row];
+                d[row] = v;
+                xout[row] = v;
+            }
+        } else {
+            const T ax = sell_row_dot<T, T>(sp, sc, sv, row, xin);
+            if (row < n) {
+                const T res = b[row] - ax;
+                const T dn = (FIRST ? T(0) : c1 * d[row]) + c2 * dinv[row] * res;
+                d[row] = dn;
+                xout[row] = xin[row] + dn;
+            }
+        }
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_residual(int64_t n, const int32_t* __restrict__ sp,
+                                                 const int32_t* __restrict__ sc, const T* __restrict__ sv,
+                                                 const T* __restrict__ b, const T* __restrict__ x, T* __restrict__ r,
+                                                 const int* __restrict__ done) {
+    if (done && *done) return;
+    for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
+         row += (int64_t)gridDim.x * blockDim.x) {
+        const T ax = sell_row_dot<T, T>(sp, sc, sv, row, x);
+        if (row < n) r[row] = b[row] - ax;
+    }
+}
+
+// y = A x (ADD: y += A x)
+template <class T, bool ADD>
+__global__ void __launch_bounds__(256) k_sell_apply(int64_t n, const int32_t* __restrict__ sp,
+                                                   const int32_t* __restrict__ sc, const T* __restrict__ sv,
+                                                   const T* __restrict__ x, T* __restrict__ y,
+                                                   const int* __restrict__ done) {
+    if (done && *done) return;
+    for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
+         row += (int64_t)gridDim.x * blockDim.x) {
+        const T v = sell_row_dot<T, T>(sp, sc, sv, row, x);
+        if (row < n) y[row] = ADD ? y[row] + v : v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// coarsest level: dense Cholesky (one workgroup)
+__global__ void __launch_bounds__(256) k_csr_to_dense(int64_t n, const int32_t* __restrict__ ptr,
+                                                     const int32_t* __restrict__ idx, const double* __restrict__ val,
+                                                     double* __restrict__ dense) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * n; e += (int64_t)gridDim.x * blockDim.x)
+        dense[e] = 0.0;
+}
+
+__global__ void __launch_bounds__(256) k_csr_scatter_dense(int64_t n, const int32_t* __restrict__ ptr,
+                                                          const int32_t* __restrict__ idx,
+                                                          const double* __restrict__ val, double* __restrict__ dense) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+        for (int32_t k = ptr[r]; k < ptr[r + 1]; ++k) dense[r * n + idx[k]] += val[k];
+}
+
+__global__ void __launch_bounds__(256) k_cholesky(int n, double* __restrict__ a) {
+    for (int k = 0; k < n; ++k) {
+        if (threadIdx.x == 0) a[(int64_t)k * n + k] = sqrt(a[(int64_t)k * n + k]);
+        __syncthreads();
+        const double dkk = a[(int64_t)k * n + k];
+        for (int i = k + 1 + threadIdx.x; i < n; i += blockDim.x) a[(int64_t)i * n + k] /= dkk;
+        __syncthreads();
+        const int m = n - k - 1;
+        for (int64_t t = threadIdx.x; t < (int64_t)m * m; t += blockDim.x) {
+            const int i = k + 1 + static_cast<int>(t / m), j = k + 1 + static_cast<int>(t % m);
+            if (j <= i) a[(int64_t)i * n + j] -= a[(int64_t)i * n + k] * a[(int64_t)j * n + k];
+        }
+        __syncthreads();
+    }
+}
+
+constexpr int COARSE_MAX = 4096;
+
+template <class T>
+__global__ void __launch_bounds__(256) k_chol_solve(int n, const double* __restrict__ L, const T* __restrict__ b,
+                                                   T* __restrict__ x, const int* __restrict__ done) {
+    if (done && *done) return;
+    __shared__ double y[COARSE_MAX];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) y[i] = static_cast<double>(b[i]);
+    __syncthreads();
+    for (int j = 0; j < n; ++j) {
+        if (threadIdx.x == 0) y[j] /= L[(int64_t)j * n + j];
+        __syncthreads();
+        const double yj = y[j];
+        for (int i = j + 1 + threadIdx.x; i < n; i += blockDim.x) y[i] -= L[(int64_t)i * n + j] * yj;
+        __syncthreads();
+    }
+    for (int j = n - 1; j >= 0; --j) {
+        if (threadIdx.x == 0) y[j] /= L[(int64_t)j * n + j];
+        __syncthreads();
+        const double yj = y[j];
+        for (int i = threadIdx.x; i < j; i += blockDim.x) y[i] -= L[(int64_t)j * n + i] * yj;
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = static_cast<T>(y[i]);
+}
+
+// ---------------------------------------------------------------------------
+// power iteration on D^-1 A: w = D^-1 A v; lambda ~ (v, A v) / (v, D v); v <- w / ||w||
+template <class T>
+__global__ void __launch_bounds__(RED_BLOCK) k_power(int64_t n, const int32_t* __restrict__ sp,
+                                                    const int32_t* __restrict__ sc, const T* __restrict__ sv,
+                                                    const T* __restrict__ dinv, const T* __restrict__ v,
+                                                    T* __restrict__ w, double* __restrict__ partials,
+                                                    unsigned* __restrict__ counter, double* __restrict__ out) {
+    double vav = 0.0, vdv = 0.0, ww = 0.0;
+    for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
+         row += (int64_t)gridDim.x * blockDim.x) {
+        const T av = sell_row_dot<T, T>(sp, sc, sv, row, v);
+        if (row < n) {
+            const double vi = v[row], di = dinv[row];
+            const double wi = di * static_cast<double>(av);
+            w[row] = static_cast<T>(wi);
+            vav += vi * static_cast<double>(av);
+            vdv += vi * vi / di;
+            ww += wi * wi;
+        }
+    }
+    double s[3] = {block_sum(vav), block_sum(vdv), block_sum(ww)};
+    if (publish_partials<3>(s, partials, counter)) {
+        const double a = sum_partials(partials, gridDim.x);
+        const double b = sum_partials(partials + gridDim.x, gridDim.x);
+        const double c = sum_partials(partials + 2 * gridDim.x, gridDim.x);
+        if (threadIdx.x == 0) {
+            out[0] = a / b;
+            out[1] = 1.0 / sqrt(c);
+            *counter = 0u;
+        }
+    }
+}
+
+template <class T>
+__global__ void k_scale(int64_t n, const T* __restrict__ w, T* __restrict__ v, const double* __restrict__ s) {
+    const T f = static_cast<T>(s[1]);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        v[i] = w[i] * f;
+}
+
+template <class T>
+__global__ void k_power_init(int64_t n, T* __restrict__ v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        v[i] = static_cast<T>(1.0 + 0.5 * sin(0.7 * static_cast<double>(i)));
+}
+
+__global__ void k_zero_counter(unsigned* c) { *c = 0u; }
+
+// ---------------------------------------------------------------------------
+// V-cycle driver
+template <class T>
+struct Cycle {
+    const fa_mg_t* mg;
+    hipStream_t s;
+    const int* done;
+
+    static unsigned grid(int64_t n) { return grid_for(((n + 63) / 64) * 64, 256, 1 << 16); }
+
+    int coarse(const fa_mg_level_t& L, const T* b, T* x) {
+        hipLaunchKernelGGL(k_chol_solve<T>, dim3(1), dim3(256), 0, s, (int)L.A.n, mg->coarse_chol, b, x, done);
+        FA_LAUNCH("k_chol_solve");
+        return FA_OK;
+    }
+
+    // Chebyshev sweep of `deg` steps on level L: from zero (pre) or from *cur (post).
+    int smooth(const fa_mg_level_t& L, const T* b, T*& cur, T*& nxt, bool from_zero) {
+        const double lo = mg->lower * L.lmax, hi = mg->upper * L.lmax;
+        const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+        double rho = 1.0 / sigma;
+        const T* dinv = static_cast<const T*>(L.dinv);
+        const T* sv = static_cast<const T*>(L.A.vals);
+        T* d = static_cast<T*>(L.d);
+        const unsigned g = grid(L.A.n);
+        for (int k = 0; k < mg->degree; ++k) {
+            T c1, c2;
+            double rho_new = rho;
+            if (k == 0) {
+                c1 = T(0);
+                c2 = static_cast<T>(1.0 / theta);
+            } else {
+                rho_new = 1.0 / (2.0 * sigma - rho);
+                c1 = static_cast<T>(rho_new * rho);
+                c2 = static_cast<T>(2.0 * rho_new / delta);
+            }
+            if (k == 0 && from_zero) {
+                hipLaunchKernelGGL((k_cheb<T, true, true>), dim3(g), dim3(256), 0, s, L.A.n, L.A.slice_ptr, L.A.cols,
+                                   sv, dinv, b, cur, cur, d, c1, c2, done);
+                continue;  // result written in place into cur
+            }
+            if (k == 0)
+                hipLaunchKernelGGL((k_cheb<T, false, true>), dim3(g), dim3(256), 0, s, L.A.n, L.A.slice_ptr,
+                                   L.A.cols, sv, dinv, b, cur, nxt, d, c1, c2, done);
+            else
+                hipLaunchKernelGGL((k_cheb<T, false, false>), dim3(g), dim3(256), 0, s, L.A.n, L.A.slice_ptr,
+                                   L.A.cols, sv, dinv, b, cur, nxt, d, c1, c2, done);
+            std::swap(cur, nxt);
+            rho = rho_new;
+        }
+        FA_LAUNCH("k_cheb");
+        return FA_OK;
+    }
+
+    // returns (through out) the level-l vector that holds the result
+    int run(int l, const T* b, T** out) {
+        const fa_mg_level_t& L = mg->levels[l];
+        T* cur = static_cast<T*>(L.x);
+        T* nxt = static_cast<T*>(L.x2);
+        if (l == mg->nlevels - 1) {
+            int st = coarse(L, b, cur);
+            *out = cur;
+            return st;
+        }
+        int st = smooth(L, b, cur, nxt, true);
+        if (st) return st;
+        const fa_mg_level_t& C = mg->levels[l + 1];
+        T* r = static_cast<T*>(L.r);
+        const unsigned g = grid(L.A.n);
+        hipLaunchKernelGGL(k_residual<T>, dim3(g), dim3(256), 0, s, L.A.n, L.A.slice_ptr, L.A.cols,
+                           static_cast<const T*>(L.A.vals), b, cur, r, done);
+        hipLaunchKernelGGL((k_sell_apply<T, false>), dim3(grid(L.R.n)), dim3(256), 0, s, L.R.n, L.R.slice_ptr,
+                           L.R.cols, static_cast<const T*>(L.R.vals), r, static_cast<T*>(C.b), done);
+        FA_LAUNCH("restrict");
+        T* xc = nullptr;
+        st = run(l + 1, static_cast<const T*>(C.b), &xc);
+        if (st) return st;
+        hipLaunchKernelGGL((k_sell_apply<T, true>), dim3(grid(L.P.n)), dim3(256), 0, s, L.P.n, L.P.slice_ptr,
+                           L.P.cols, static_cast<const T*>(L.P.vals), xc, cur, done);
+        FA_LAUNCH("prolongate");
+        st = smooth(L, b, cur, nxt, false);
+        *out = cur;
+        return st;
+    }
+};
+
+template <class T>
+__global__ void k_copy(int64_t n, const T* __restrict__ x, T* __restrict__ y, const int* __restrict__ done) {
+    if (done && *done) return;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = x[i];
+}
+
+template <class T>
+static int vcycle_t(const fa_mg_t* mg, const void* r, void* z, const int* done, hipStream_t s) {
+    Cycle<T> c{mg, s, done};
+    T* out = nullptr;
+    int st = c.run(0, static_cast<const T*>(r), &out);
+    if (st) return st;
+    if (out != z)
+        hipLaunchKernelGGL(k_copy<T>, dim3(grid_for(mg->levels[0].A.n, 256, 1 << 16)), dim3(256), 0, s,
+                           mg->levels[0].A.n, out, static_cast<T*>(z), done);
+    FA_LAUNCH("k_copy");
+    return FA_OK;
+}
+
+static int vcycle_any(const fa_mg_t* mg, const void* r, void* z, const int* done, hipStream_t s) {
+    if (mg->dtype == FA_F64) return vcycle_t<double>(mg, r, z, done, s);
+    return vcycle_t<float>(mg, r, z, done, s);
+}
+
+// ---------------------------------------------------------------------------
+// PCG (outer dtype T, cycle dtype U)
+struct PcgState {
+    double rz, pAp, alpha, beta, rnorm, bnorm, atol, rtol;
+    long long niter, maxit;
+    int done;  // 0 running, 1 converged, 2 ||b|| < 1e-15, 3 maxit, 4 breakdown
+    unsigned counter;
+};
+
+__device__ inline void pcg_check(PcgState* st, double rz_new) {
+    const double rn = sqrt(rz_new);
+    st->rnorm = rn;
+    st->niter += 1;
+    if (rn < st->atol || rn < st->rtol * st->bnorm) st->done = 1;
+    else if (st->niter >= st->maxit) st->done = 3;
+    else if (!isfinite(rn) || !isfinite(st->alpha)) st->done = 4;
+    st->beta = rz_new / st->rz;
+    st->rz = rz_new;
+}
+
+template <class T, class U>
+__global__ void __launch_bounds__(RED_BLOCK) k_pcg_init(int64_t n, const int32_t* __restrict__ sp,
+                                                       const int32_t* __restrict__ sc, const T* __restrict__ sv,
+                                                       const T* __restrict__ b, const T* __restrict__ x,
+                                                       T* __restrict__ r, U* __restrict__ ru, T* __restrict__ p,
+                                                       double* __restrict__ partials, PcgState* st) {
+    double rr = 0.0, bb = 0.0;
+    for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
+         row += (int64_t)gridDim.x * blockDim.x) {
+        const T ax = sell_row_dot<T, T>(sp, sc, sv, row, x);
+        if (row < n) {
+            const T bv = b[row];
+            const T rv = bv - ax;
+            r[row] = rv;
+            if (ru) ru[row] = static_cast<U>(rv);
+            else p[row] = rv;
+            rr += double(rv) * double(rv);
+            bb += double(bv) * double(bv);
+        }
+    }
+    double v[2] = {block_sum(rr), block_sum(bb)};
+    if (publish_partials<2>(v, partials, &st->counter)) {
+        const double trr = sum_partials(partials, gridDim.x);
+        const double tbb = sum_partials(partials + gridDim.x, gridDim.x);
+        if (threadIdx.x == 0) {
+            st->rz = trr;  // overwritten by r.z when preconditioned
+            st->bnorm = sqrt(tbb);
+            st->niter = 0;
+            st->done = (st->bnorm < 1e-15) ? 2 : 0;
+            st->counter = 0u;
+        }
+    }
+}
+
+template <class T>
+__global__ void k_pcg_trivial(int64_t n, T* __restrict__ x, const PcgState* st) {
+    if (st->done != 2) return;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = T(0);
+}
+
+// rz = r.z (after the first preconditioner application), p = z
+template <class T, class U>
+__global__ void __launch_bounds__(RED_BLOCK) k_pcg_rz0(int64_t n, const T* __restrict__ r, const U* __restrict__ z,
+                                                      T* __restrict__ p, double* __restrict__ partials,
+                                                      PcgState* st) {
+    if (st->done) return;
+    double acc = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const T zi = static_cast<T>(z[i]);
+        p[i] = zi;
+        acc += double(r[i]) * double(zi);
+    }
+    acc = block_sum(acc);
+    if (publish_partial(acc, partials, &st->counter)) {
+        const double t = sum_partials(partials, gridDim.x);
+        if (threadIdx.x == 0) {
+            st->rz = t;
+            st->counter = 0u;
+        }
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(RED_BLOCK) k_pcg_spmv(int64_t n, const int32_t* __restrict__ sp,
+                                                       const int32_t* __restrict__ sc, const T* __restrict__ sv,
+                                                       const T* __restrict__ p, T* __restrict__ Ap,
+                                                       double* __restrict__ partials, PcgState* st) {
+    if (st->done) return;
+    double acc = 0.0;
+    for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
+         row += (int64_t)gridDim.x * blockDim.x) {
+        const T v = sell_row_dot<T, T>(sp, sc, sv, row, p);
+        if (row < n) {
+            Ap[row] = v;
+            acc += double(p[row]) * double(v);
+        }
+    }
+    acc = block_sum(acc);
+    if (publish_partial(acc, partials, &st->counter)) {
+        const double t = sum_partials(partials, gridDim.x);
+        if (threadIdx.x == 0) {
+            st->pAp = t;
+            st->alpha = st->rz / t;
+            st->counter = 0u;
+        }
+    }
+}
+
+// x += alpha p; r -= alpha Ap; then either r.r + stopping test (no
+// preconditioner) or the cast of r for the cycle
+template <class T, class U, bool PREC>
+__global__ void __launch_bounds__(RED_BLOCK) k_pcg_update(int64_t n, T* __restrict__ x, T* __restrict__ r,
+                                                         U* __restrict__ ru, const T* __restrict__ p,
+                                                         const T* __restrict__ Ap, double* __restrict__ partials,
+                                                         PcgState* st) {
+    if (st->done) return;
+    const T alpha = static_cast<T>(st->alpha);
+    double acc = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        x[i] += alpha * p[i];
+        const T rv = r[i] - alpha * Ap[i];
+        r[i] = rv;
+        if (PREC) ru[i] = static_cast<U>(rv);
+        else acc += double(rv) * double(rv);
+    }
+    if (PREC) return;
+    acc = block_sum(acc);
+    if (publish_partial(acc, partials, &st->counter)) {
+        const double t = sum_partials(partials, gridDim.x);
+        if (threadIdx.x == 0) {
+            pcg_check(st, t);
+            st->counter = 0u;
+        }
+    }
+}
+
+template <class T, class U>
+__global__ void __launch_bounds__(RED_BLOCK) k_pcg_rz(int64_t n, const T* __restrict__ r, const U* __restrict__ z,
+                                                     double* __restrict__ partials, PcgState* st) {
+    if (st->done) return;
+    double acc = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        acc += double(r[i]) * double(z[i]);
+    acc = block_sum(acc);
+    if (publish_partial(acc, partials, &st->counter)) {
+        const double t = sum_partials(partials, gridDim.x);
+        if (threadIdx.x == 0) {
+            pcg_check(st, t);
+            st->counter = 0u;
+        }
+    }
+}
+
+template <class T, class U>
+__global__ void k_pcg_pupdate(int64_t n, T* __restrict__ p, const U* __restrict__ z, const PcgState* st) {
+    if (st->done) return;
+    const T beta = static_cast<T>(st->beta);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = static_cast<T>(z[i]) + beta * p[i];
+}
+
+__global__ void k_pcg_set(PcgState* st, double atol, double rtol, long long maxit) {
+    st->atol = atol;
+    st->rtol = rtol;
+    st->maxit = maxit;
+    st->counter = 0u;
+    st->done = 0;
+    st->niter = 0;
+}
+
+template <class T, class U>
+static int pcg_run(const fa_sell_t* A, const fa_mg_t* M, const T* b, T* x, double atol, double rtol, int64_t maxit,
+                   int check_every, int64_t* niter_host, double* res_host, void* work, size_t work_bytes,
+                   hipStream_t s) {
+    const int64_t n = A->n;
+    Arena ar(work, work_bytes);
+    T* r = ar.take<T>(n);
+    T* p = ar.take<T>(n);
+    T* Ap = ar.take<T>(n);
+    U* ru = M ? ar.take<U>(n) : nullptr;
+    U* z = M ? ar.take<U>(n) : nullptr;
+    double* partials = ar.take<double>(2 * RED_GRID);
+    PcgState* st = ar.take<PcgState>(1);
+    if (!r || !p || !Ap || (M && (!ru || !z)) || !partials || !st) {
+        set_error("fa_pcg: workspace too small (%zu bytes)", work_bytes);
+        return FA_ENOMEM;
+    }
+    const int* done = &st->done;
+    const dim3 grid(RED_GRID), block(RED_BLOCK);
+    const T* sv = static_cast<const T*>(A->vals);
+    const int64_t cap = maxit > 0 ? maxit : (1LL << 62);
+    hipLaunchKernelGGL(k_pcg_set, dim3(1), dim3(1), 0, s, st, atol, rtol, (long long)cap);
+    hipLaunchKernelGGL((k_pcg_init<T, U>), grid, block, 0, s, n, A->slice_ptr, A->cols, sv, b, x, r, ru, p,
+                       partials, st);
+    hipLaunchKernelGGL(k_pcg_trivial<T>, grid, block, 0, s, n, x, st);
+    FA_LAUNCH("k_pcg_init");
+    if (M) {
+        int e = vcycle_any(M, ru, z, done, s);
+        if (e) return e;
+        hipLaunchKernelGGL((k_pcg_rz0<T, U>), grid, block, 0, s, n, r, z, p, partials, st);
+        FA_LAUNCH("k_pcg_rz0");
+    }
+    PcgState h{};
+    int64_t it = 0;
+    for (;;) {
+        const int64_t chunk = std::min<int64_t>(check_every, cap - it);
+        for (int64_t k = 0; k < chunk; ++k) {
+            hipLaunchKernelGGL(k_pcg_spmv<T>, grid, block, 0, s, n, A->slice_ptr, A->cols, sv, p, Ap, partials, st);
+            if (M) {
+                hipLaunchKernelGGL((k_pcg_update<T, U, true>), grid, block, 0, s, n, x, r, ru, p, Ap, partials, st);
+                int e = vcycle_any(M, ru, z, done, s);
+                if (e) return e;
+                hipLaunchKernelGGL((k_pcg_rz<T, U>), grid, block, 0, s, n, r, z, partials, st);
+                hipLaunchKernelGGL((k_pcg_pupdate<T, U>), grid, block, 0, s, n, p, z, st);
+            } else {
+                hipLaunchKernelGGL((k_pcg_update<T, T, false>), grid, block, 0, s, n, x, r, (T*)nullptr, p, Ap,
+                                   partials, st);
+                hipLaunchKernelGGL((k_pcg_pupdate<T, T>), grid, block, 0, s, n, p, r, st);
+            }
+        }
+        FA_LAUNCH("pcg iteration");
+        it += chunk;
+        FA_HIP(hipMemcpyAsync(&h, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+        FA_HIP(hipStreamSynchronize(s));
+        if (h.done || it >= cap) break;
+    }
+    if (niter_host) *niter_host = h.niter;
+    if (res_host) *res_host = h.rnorm;
+    return (h.done == 3 || h.done == 4) ? FA_ENOCONV : FA_OK;
+}
+
+}  // namespace fa
+
+using namespace fa;
+
+extern "C" size_t fa_rap_work_bytes(int64_t nrows) {
+    return align_up((nrows + 1) * sizeof(int32_t)) + exclusive_scan_work_bytes(nrows) + 2048;
+}
+
+template <int CAP>
+static void launch_rap_rows(int64_t nrows, const int32_t* rp, const int32_t* ri, const int32_t* ap,
+                            const int32_t* ai, const int32_t* pp, const int32_t* pi, int32_t* rowcount,
+                            const int32_t* cptr, int32_t* cidx, int32_t* max_row, int32_t* overflow, hipStream_t s) {
+    hipLaunchKernelGGL(k_rap_rows<CAP>, dim3(grid_for(nrows, 1, 1 << 22)), dim3(WAVE), 0, s, nrows, rp, ri, ap, ai, pp,
+                       pi, rowcount, cptr, cidx, max_row, overflow);
+}
+
+static int rap_rows(int32_t max_cand, int64_t nrows, const int32_t* rp, const int32_t* ri, const int32_t* ap,
+                    const int32_t* ai, const int32_t* pp, const int32_t* pi, int32_t* rowcount, const int32_t* cptr,
+                    int32_t* cidx, int32_t* max_row, int32_t* overflow, hipStream_t s) {
+    if (max_cand <= 512) launch_rap_rows<512>(nrows, rp, ri, ap, ai, pp, pi, rowcount, cptr, cidx, max_row, overflow, s);
+    else if (max_cand <= 2048)
+        launch_rap_rows<2048>(nrows, rp, ri, ap, ai, pp, pi, rowcount, cptr, cidx, max_row, overflow, s);
+    else if (max_cand <= 8192)
+        launch_rap_rows<8192>(nrows, rp, ri, ap, ai, pp, pi, rowcount, cptr, cidx, max_row, overflow, s);
+    else {
+        set_error("fa_rap: %d candidates per coarse row exceed the kernel capacity (8192)", max_cand);
+        return FA_ECAPACITY;
+    }
+    FA_LAUNCH("k_rap_rows");
+    return FA_OK;
+}
+
+__global__ void k_zero_i32(int32_t* p, int n) {
+    if (threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+extern "C" int fa_rap_count(int64_t nrows, int64_t ncols, const int32_t* r_ptr, const int32_t* r_idx,
+                            const int32_t* a_ptr, const int32_t* a_idx, const int32_t* p_ptr, const int32_t* p_idx,
+                            int32_t max_cand, int32_t* c_ptr, int64_t* nnz_host, int32_t* max_row_host, void* work,
+                            size_t work_bytes, fa_stream_t stream) {
+    (void)ncols;
+    FA_REQUIRE(nrows > 0, "fa_rap_count: empty operator");
+    hipStream_t s = S(stream);
+    Arena ar(work, work_bytes);
+    int32_t* cnt = ar.take<int32_t>(nrows + 1);
+    int32_t* flags = ar.take<int32_t>(4);
+    const size_t sb = exclusive_scan_work_bytes(nrows);
+    void* sw = ar.take<char>(sb);
+    if (!cnt || !flags || !sw) {
+        set_error("fa_rap_count: workspace too small");
+        return FA_ENOMEM;
+    }
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, flags, 4);
+    int st = rap_rows(max_cand, nrows, r_ptr, r_idx, a_ptr, a_idx, p_ptr, p_idx, cnt, nullptr, nullptr, flags,
+                      flags + 1, s);
+    if (st) return st;
+    st = exclusive_scan_i32(cnt, c_ptr, nrows, sw, sb, s);
+    if (st) return st;
+    int32_t h[2] = {0, 0}, nnz = 0;
+    FA_HIP(hipMemcpyAsync(h, flags, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    FA_HIP(hipMemcpyAsync(&nnz, c_ptr + nrows, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    FA_HIP(hipStreamSynchronize(s));
+    if (h[1]) {
+        set_error("fa_rap_count: max_cand %d is smaller than a coarse row's candidate count", max_cand);
+        return FA_ECAPACITY;
+    }
+    if (nnz_host) *nnz_host = nnz;
+    if (max_row_host) *max_row_host = h[0];
+    return FA_OK;
+}
+
+extern "C" int fa_rap_fill(int64_t nrows, const int32_t* r_ptr, const int32_t* r_idx, const int32_t* a_ptr,
+                           const int32_t* a_idx, const int32_t* p_ptr, const int32_t* p_idx, int32_t max_cand,
+                           const int32_t* c_ptr, int32_t* c_idx, fa_stream_t stream) {
+    return rap_rows(max_cand, nrows, r_ptr, r_idx, a_ptr, a_idx, p_ptr, p_idx, nullptr, c_ptr, c_idx, nullptr,
+                    nullptr, S(stream));
+}
+
+extern "C" int fa_rap_values(int64_t nrows, const int32_t* r_ptr, const int32_t* r_idx, const double* r_val,
+                             const int32_t* a_ptr, const int32_t* a_idx, const double* a_val, const int32_t* p_ptr,
+                             const int32_t* p_idx, const double* p_val, const int32_t* c_ptr, const int32_t* c_idx,
+                             int32_t max_row, double* c_val, fa_stream_t stream) {
+    FA_REQUIRE(max_row > 0, "fa_rap_values: bad max_row");
+    const size_t lds = align_up(sizeof(int32_t) * max_row, 16) + sizeof(double) * WAVE * max_row;
+    if (lds > 64 * 1024) {
+        set_error("fa_rap_values: coarse rows of %d entries exceed the LDS capacity", max_row);
+        return FA_ECAPACITY;
+    }
+    hipLaunchKernelGGL(k_rap_values, dim3(grid_for(nrows, 1, 1 << 22)), dim3(WAVE), lds, S(stream), nrows, r_ptr,
+                       r_idx, r_val, a_ptr, a_idx, a_val, p_ptr, p_idx, p_val, c_ptr, c_idx, max_row, c_val);
+    FA_LAUNCH("k_rap_values");
+    return FA_OK;
+}
+
+extern "C" int fa_csr_to_dense(int64_t n, const int32_t* indptr, const int32_t* indices, const double* values,
+                               double* dense, fa_stream_t stream) {
+    hipStream_t s = S(stream);
+    hipLaunchKernelGGL(k_csr_to_dense, dim3(grid_for(n * n, 256, 4096)), dim3(256), 0, s, n, indptr, indices, values,
+                       dense);
+    hipLaunchKernelGGL(k_csr_scatter_dense, dim3(grid_for(n, 256, 4096)), dim3(256), 0, s, n, indptr, indices,
+                       values, dense);
+    FA_LAUNCH("k_csr_to_dense");
+    return FA_OK;
+}
+
+extern "C" int fa_dense_cholesky(int64_t n, double* a, fa_stream_t stream) {
+    FA_REQUIRE(n > 0 && n <= COARSE_MAX, "fa_dense_cholesky: n = %lld outside 1..%d", (long long)n, COARSE_MAX);
+    hipLaunchKernelGGL(k_cholesky, dim3(1), dim3(256), 0, S(stream), (int)n, a);
+    FA_LAUNCH("k_cholesky");
+    return FA_OK;
+}
+
+extern "C" int fa_mg_lmax(const fa_mg_t* mg, int level, int iters, double* lmax_host, void* work, size_t work_bytes,
+                          fa_stream_t stream) {
+    FA_REQUIRE(mg && level >= 0 && level < mg->nlevels && iters > 0, "fa_mg_lmax: bad arguments");
+    hipStream_t s = S(stream);
+    const fa_mg_level_t& L = mg->levels[level];
+    Arena ar(work, work_bytes);
+    double* partials = ar.take<double>(3 * RED_GRID);
+    unsigned* counter = ar.take<unsigned>(1);
+    double* out = ar.take<double>(2);
+    if (!partials || !counter || !out) {
+        set_error("fa_mg_lmax: workspace too small");
+        return FA_ENOMEM;
+    }
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, counter);
+    const unsigned g = grid_for(L.A.n, 256, 1 << 16);
+    double lam = 0.0;
+#define FA_PW(T)                                                                                                   \
+    {                                                                                                              \
+        T* v = static_cast<T*>(L.x);                                                                               \
+        T* w = static_cast<T*>(L.x2);                                                                              \
+        hipLaunchKernelGGL(k_power_init<T>, dim3(g), dim3(256), 0, s, L.A.n, v);                                  \
+        for (int k = 0; k < iters; ++k) {                                                                          \
+            hipLaunchKernelGGL(k_power<T>, dim3(RED_GRID), dim3(RED_BLOCK), 0, s, L.A.n, L.A.slice_ptr, L.A.cols, \
+                               static_cast<const T*>(L.A.vals), static_cast<const T*>(L.dinv), v, w, partials,    \
+                               counter, out);                                                                      \
+            hipLaunchKernelGGL(k_scale<T>, dim3(g), dim3(256), 0, s, L.A.n, w, v, out);                           \
+        }                                                                                                          \
+    }
+    if (mg->dtype == FA_F64) FA_PW(double) else FA_PW(float)
+#undef FA_PW
+    FA_LAUNCH("k_power");
+    FA_HIP(hipMemcpyAsync(&lam, out, sizeof(double), hipMemcpyDeviceToHost, s));
+    FA_HIP(hipStreamSynchronize(s));
+    if (lmax_host) *lmax_host = lam;
+    return FA_OK;
+}
+
+extern "C" int fa_mg_vcycle(const fa_mg_t* mg, const void* r, void* z, fa_stream_t stream) {
+    FA_REQUIRE(mg && mg->nlevels >= 1 && mg->levels, "fa_mg_vcycle: empty hierarchy");
+    FA_REQUIRE(mg->coarse_chol, "fa_mg_vcycle: the coarsest level needs a Cholesky factor");
+    FA_REQUIRE(mg->levels[mg->nlevels - 1].A.n <= COARSE_MAX, "fa_mg_vcycle: coarsest level too large");
+    return vcycle_any(mg, r, z, nullptr, S(stream));
+}
+
+extern "C" size_t fa_pcg_work_bytes(int dtype, int64_t n) {
+    const size_t e = dtype == FA_F32 ? 4 : 8;
+    return 3 * align_up(n * e) + 2 * align_up(n * 8) + align_up(2 * RED_GRID * sizeof(double)) +
+           align_up(sizeof(PcgState)) + 2048;
+}
+
+extern "C" int fa_pcg(int dtype, const fa_sell_t* A, const fa_mg_t* M, const void* b, void* x, double atol,
+                      double rtol, int64_t maxit, int32_t check_every, int64_t* niter_host, double* residual_host,
+                      void* work, size_t work_bytes, fa_stream_t stream) {
+    FA_REQUIRE(A && A->n > 0 && check_every > 0, "fa_pcg: bad arguments");
+    if (M) {
+        FA_REQUIRE(M->nlevels >= 1 && M->levels && M->coarse_chol, "fa_pcg: incomplete multigrid hierarchy");
+        FA_REQUIRE(M->levels[0].A.n == A->n, "fa_pcg: preconditioner size mismatch");
+        FA_REQUIRE(M->levels[M->nlevels - 1].A.n <= COARSE_MAX, "fa_pcg: coarsest level too large");
+    }
+    hipStream_t s = S(stream);
+    const int ud = M ? M->dtype : dtype;
+#define FA_PCG(T, U)                                                                                         \
+    return pcg_run<T, U>(A, M, static_cast<const T*>(b), static_cast<T*>(x), atol, rtol, maxit, check_every, \
+                         niter_host, residual_host, work, work_bytes, s)
+    if (dtype == FA_F64 && ud == FA_F64) FA_PCG(double, double);
+    if (dtype == FA_F64 && ud == FA_F32) FA_PCG(double, float);
+    if (dtype == FA_F32 && ud == FA_F32) FA_PCG(float, float);
+#undef FA_PCG
+    set_error("fa_pcg: unsupported dtype pair (%d, %d)", dtype, ud);
+    return FA_EINVAL;
+}

@@ -1,0 +1,158 @@
+// SELL-64 construction from CSR, SELL SpMV and Jacobi diagonal.
+#include "common.hpp"
+#include "sell.hpp"
+
+namespace fa {
+
+// entries per slice = 64 * (longest row of the slice); one wave per slice
+__global__ void __launch_bounds__(256) k_sell_widths(int64_t n, const int32_t* __restrict__ indptr,
+                                                    int32_t* __restrict__ entries) {
+    const int64_t nslices = (n + 63) / 64;
+    for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < nslices * 64;
+         row += (int64_t)gridDim.x * blockDim.x) {
+        int len = row < n ? indptr[row + 1] - indptr[row] : 0;
+        for (int o = WAVE / 2; o > 0; o >>= 1) len = max(len, __shfl_xor(len, o));
+        if ((row & 63) == 0) entries[row >> 6] = 64 * len;
+    }
+}
+
+template <class A, class B>
+__global__ void __launch_bounds__(256) k_sell_fill(int64_t n, const int32_t* __restrict__ indptr,
+                                                  const int32_t* __restrict__ indices, const A* __restrict__ vin,
+                                                  const int32_t* __restrict__ slice_ptr, int32_t* __restrict__ cols,
+                                                  B* __restrict__ vals) {
+    const int64_t nslices = (n + 63) / 64;
+    for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < nslices * 64;
+         row += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = row >> 6;
+        const int i = static_cast<int>(row & 63);
+        const int32_t p0 = slice_ptr[s];
+        const int w = (slice_ptr[s + 1] - p0) >> 6;
+        const int32_t r0 = row < n ? indptr[row] : 0;
+        const int len = row < n ? indptr[row + 1] - r0 : 0;
+        const int32_t pad = row < n ? static_cast<int32_t>(row) : 0;
+        for (int k = 0; k < w; ++k) {
+            const int64_t o = p0 + 64 * k + i;
+            if (cols) cols[o] = k < len ? indices[r0 + k] : pad;
+            if (vals) vals[o] = k < len ? static_cast<B>(vin[r0 + k]) : B(0);
+        }
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_sell_spmv(int64_t n, const int32_t* __restrict__ slice_ptr,
+                                                  const int32_t* __restrict__ cols, const T* __restrict__ vals,
+                                                  const T* __restrict__ x, T* __restrict__ y) {
+    for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
+         row += (int64_t)gridDim.x * blockDim.x) {
+        const T v = sell_row_dot<T, T>(slice_ptr, cols, vals, row, x);
+        if (row < n) y[row] = v;
+    }
+}
+
+template <class A, class B>
+__global__ void k_diag_inv(int64_t n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices,
+                           const A* __restrict__ vals, B* __restrict__ dinv) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        double d = 0.0;
+        for (int32_t k = indptr[r]; k < indptr[r + 1]; ++k)
+            if (indices[k] == r) d = static_cast<double>(vals[k]);
+        dinv[r] = static_cast<B>(d != 0.0 ? 1.0 / d : 1.0);
+    }
+}
+
+}  // namespace fa
+
+using namespace fa;
+
+extern "C" size_t fa_sell_work_bytes(int64_t n) {
+    const int64_t ns = (n + 63) / 64;
+    return align_up((ns + 1) * sizeof(int32_t)) + exclusive_scan_work_bytes(ns) + 1024;
+}
+
+extern "C" int fa_sell_count(int64_t n, const int32_t* indptr, int32_t* slice_ptr, int64_t* entries_host,
+                             void* work, size_t work_bytes, fa_stream_t stream) {
+    FA_REQUIRE(n > 0, "fa_sell_count: empty matrix");
+    hipStream_t s = S(stream);
+    const int64_t ns = (n + 63) / 64;
+    Arena ar(work, work_bytes);
+    int32_t* ent = ar.take<int32_t>(ns + 1);
+    const size_t sb = exclusive_scan_work_bytes(ns);
+    void* sw = ar.take<char>(sb);
+    if (!ent || !sw) {
+        set_error("fa_sell_count: workspace too small");
+        return FA_ENOMEM;
+    }
+    hipLaunchKernelGGL(k_sell_widths, dim3(grid_for(ns * 64, 256, 1 << 16)), dim3(256), 0, s, n, indptr, ent);
+    FA_LAUNCH("k_sell_widths");
+    int st = exclusive_scan_i32(ent, slice_ptr, ns, sw, sb, s);
+    if (st != FA_OK) return st;
+    int32_t tot = 0;
+    FA_HIP(hipMemcpyAsync(&tot, slice_ptr + ns, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    FA_HIP(hipStreamSynchronize(s));
+    if (entries_host) *entries_host = tot;
+    return FA_OK;
+}
+
+extern "C" int fa_sell_fill(int dtype_in, int dtype_out, int64_t n, const int32_t* indptr, const int32_t* indices,
+                            const void* values, const int32_t* slice_ptr, int32_t* cols, void* vals,
+                            fa_stream_t stream) {
+    FA_REQUIRE(n > 0, "fa_sell_fill: empty matrix");
+    FA_REQUIRE(!vals || values, "fa_sell_fill: values missing");
+    hipStream_t s = S(stream);
+    const unsigned grid = grid_for(((n + 63) / 64) * 64, 256, 1 << 16);
+#define FA_SF(A_, B_)                                                                                       \
+    hipLaunchKernelGGL((k_sell_fill<A_, B_>), dim3(grid), dim3(256), 0, s, n, indptr, indices,             \
+                       static_cast<const A_*>(values), slice_ptr, cols, static_cast<B_*>(vals))
+    if (dtype_in == FA_F64 && dtype_out == FA_F64) FA_SF(double, double);
+    else if (dtype_in == FA_F64 && dtype_out == FA_F32) FA_SF(double, float);
+    else if (dtype_in == FA_F32 && dtype_out == FA_F32) FA_SF(float, float);
+    else if (dtype_in == FA_F32 && dtype_out == FA_F64) FA_SF(float, double);
+    else {
+        set_error("fa_sell_fill: bad dtypes");
+        return FA_EINVAL;
+    }
+#undef FA_SF
+    FA_LAUNCH("k_sell_fill");
+    return FA_OK;
+}
+
+extern "C" int fa_sell_spmv(int dtype, const fa_sell_t* A, const void* x, void* y, fa_stream_t stream) {
+    FA_REQUIRE(A && A->n > 0, "fa_sell_spmv: empty matrix");
+    hipStream_t s = S(stream);
+    const unsigned grid = grid_for(((A->n + 63) / 64) * 64, 256, 1 << 16);
+    if (dtype == FA_F64)
+        hipLaunchKernelGGL(k_sell_spmv<double>, dim3(grid), dim3(256), 0, s, A->n, A->slice_ptr, A->cols,
+                           static_cast<const double*>(A->vals), static_cast<const double*>(x),
+                           static_cast<double*>(y));
+    else if (dtype == FA_F32)
+        hipLaunchKernelGGL(k_sell_spmv<float>, dim3(grid), dim3(256), 0, s, A->n, A->slice_ptr, A->cols,
+                           static_cast<const float*>(A->vals), static_cast<const float*>(x), static_cast<float*>(y));
+    else {
+        set_error("fa_sell_spmv: bad dtype %d", dtype);
+        return FA_EINVAL;
+    }
+    FA_LAUNCH("k_sell_spmv");
+    return FA_OK;
+}
+
+extern "C" int fa_diag_inv(int dtype_in, int dtype_out, int64_t n, const int32_t* indptr, const int32_t* indices,
+                           const void* values, void* dinv, fa_stream_t stream) {
+    hipStream_t s = S(stream);
+    const unsigned grid = grid_for(n, 256, 1 << 16);
+    if (dtype_in == FA_F64 && dtype_out == FA_F64)
+        hipLaunchKernelGGL((k_diag_inv<double, double>), dim3(grid), dim3(256), 0, s, n, indptr, indices,
+                           static_cast<const double*>(values), static_cast<double*>(dinv));
+    else if (dtype_in == FA_F64 && dtype_out == FA_F32)
+        hipLaunchKernelGGL((k_diag_inv<double, float>), dim3(grid), dim3(256), 0, s, n, indptr, indices,
+                           static_cast<const double*>(values), static_cast<float*>(dinv));
+    else if (dtype_in == FA_F32 && dtype_out == FA_F32)
+        hipLaunchKernelGGL((k_diag_inv<float, float>), dim3(grid), dim3(256), 0, s, n, indptr, indices,
+                           static_cast<const float*>(values), static_cast<float*>(dinv));
+    else {
+        set_error("fa_diag_inv: bad dtypes");
+        return FA_EINVAL;
+    }
+    FA_LAUNCH("k_diag_inv");
+    return FA_OK;
+}

@@ -1,51 +1,11 @@
-// Dirichlet BC, SpMV, deterministic reductions and the CG driver (kernel "d").
+// Dirichlet BC, CSR SpMV and vector kernels (kernel "d" building blocks).
 #include <algorithm>
 #include <cmath>
 
 #include "common.hpp"
+#include "reduce.hpp"
 
 namespace fa {
-
-constexpr int RED_BLOCK = 256;
-constexpr int RED_GRID = 1024;  // fixed grid => fixed reduction order
-
-// ---- block reduction + "last block finishes" ----------------------------------
-__device__ inline double block_sum(double v) {
-    __shared__ double ws[RED_BLOCK / WAVE];
-    v = wave_sum(v);
-    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
-    __syncthreads();
-    if (lane == 0) ws[wid] = v;
-    __syncthreads();
-    double t = 0.0;
-    if (threadIdx.x < WAVE) {
-        t = (threadIdx.x < blockDim.x / WAVE) ? ws[threadIdx.x] : 0.0;
-        t = wave_sum(t);
-    }
-    return t;  // valid in thread 0
-}
-
-// Publishes this block's partial and returns true in every thread of the last
-// block to arrive; that block then sums all partials in index order.
-__device__ inline bool publish_partial(double v, double* partials, unsigned* counter) {
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
-        partials[blockIdx.x] = v;
-        __threadfence();
-        unsigned prev = atomicAdd(counter, 1u);
-        last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    return last;
-}
-
-__device__ inline double sum_partials(double* partials, int n) {
-    __threadfence();
-    double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x)
-        s += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return block_sum(s);
-}
 
 // ---- SpMV -------------------------------------------------------------------
 template <class T, int G>
@@ -170,193 +130,6 @@ __global__ void k_cast(int64_t n, const A* __restrict__ x, B* __restrict__ y) {
 
 __global__ void k_zero_u32(unsigned* p, int n) {
     if (threadIdx.x < n) p[threadIdx.x] = 0u;
-}
-
-// ---- CG ---------------------------------------------------------------------------
-struct CgState {
-    double rz;       // r.r of the current residual
-    double pAp;
-    double alpha;
-    double beta;
-    double rnorm;
-    double bnorm;
-    double atol, rtol;
-    long long niter, maxit;
-    int done;        // 0 running, 1 converged, 2 trivial (||b|| < 1e-15), 3 maxit, 4 breakdown (NaN/Inf)
-    unsigned counter;
-};
-
-// r = b - A x, p = r; sums r.r and b.b
-template <class T, int G>
-__global__ void __launch_bounds__(RED_BLOCK) k_cg_init(int64_t n, const int32_t* __restrict__ indptr,
-                                                      const int32_t* __restrict__ indices, const T* __restrict__ vals,
-                                                      const T* __restrict__ b, const T* __restrict__ x,
-                                                      T* __restrict__ r, T* __restrict__ p,
-                                                      double* __restrict__ partials, CgState* st) {
-    const int lane = threadIdx.x % G;
-    double rr = 0.0, bb = 0.0;
-    for (int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G; row < n;
-         row += (int64_t)gridDim.x * blockDim.x / G) {
-        T s = T(0);
-        for (int32_t k = indptr[row] + lane; k < indptr[row + 1]; k += G) s += vals[k] * x[indices[k]];
-        s = group_sum<G>(s);
-        if (lane == 0) {
-            const T bv = b[row];
-            const T rv = bv - s;
-            r[row] = rv;
-            p[row] = rv;
-            rr += double(rv) * double(rv);
-            bb += double(bv) * double(bv);
-        }
-    }
-    rr = block_sum(rr);
-    bb = block_sum(bb);
-    // one arrival counter for both sums: partials[0..G) = r.r, partials[G..2G) = b.b
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
-        partials[blockIdx.x] = rr;
-        partials[gridDim.x + blockIdx.x] = bb;
-        __threadfence();
-        unsigned prev = atomicAdd(&st->counter, 1u);
-        last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (last) {
-        const double trr = sum_partials(partials, gridDim.x);
-        const double tbb = sum_partials(partials + gridDim.x, gridDim.x);
-        if (threadIdx.x == 0) {
-            st->rz = trr;
-            st->bnorm = sqrt(tbb);
-            st->niter = 0;
-            st->done = (st->bnorm < 1e-15) ? 2 : 0;
-            st->counter = 0u;
-        }
-    }
-}
-
-template <class T>
-__global__ void k_cg_trivial(int64_t n, T* __restrict__ x, const CgState* st) {
-    if (st->done != 2) return;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        x[i] = T(0);
-}
-
-// Ap = A p; pAp = p.Ap; alpha = rz / pAp
-template <class T, int G>
-__global__ void __launch_bounds__(RED_BLOCK) k_cg_spmv(int64_t n, const int32_t* __restrict__ indptr,
-                                                      const int32_t* __restrict__ indices, const T* __restrict__ vals,
-                                                      const T* __restrict__ p, T* __restrict__ Ap,
-                                                      double* __restrict__ partials, CgState* st) {
-    if (st->done) return;
-    const int lane = threadIdx.x % G;
-    double acc = 0.0;
-    for (int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G; row < n;
-         row += (int64_t)gridDim.x * blockDim.x / G) {
-        T s = T(0);
-        for (int32_t k = indptr[row] + lane; k < indptr[row + 1]; k += G) s += vals[k] * p[indices[k]];
-        s = group_sum<G>(s);
-        if (lane == 0) {
-            Ap[row] = s;
-            acc += double(p[row]) * double(s);
-        }
-    }
-    acc = block_sum(acc);
-    if (publish_partial(acc, partials, &st->counter)) {
-        double t = sum_partials(partials, gridDim.x);
-        if (threadIdx.x == 0) {
-            st->pAp = t;
-            st->alpha = st->rz / t;
-            st->counter = 0u;
-        }
-    }
-}
-
-// x += alpha p; r -= alpha Ap; rz_new = r.r; stopping test of cg.py:99-117; beta
-template <class T>
-__global__ void __launch_bounds__(RED_BLOCK) k_cg_update(int64_t n, T* __restrict__ x, T* __restrict__ r,
-                                                        const T* __restrict__ p, const T* __restrict__ Ap,
-                                                        double* __restrict__ partials, CgState* st) {
-    if (st->done) return;
-    const T alpha = static_cast<T>(st->alpha);
-    double acc = 0.0;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        x[i] += alpha * p[i];
-        const T rv = r[i] - alpha * Ap[i];
-        r[i] = rv;
-        acc += double(rv) * double(rv);
-    }
-    acc = block_sum(acc);
-    if (publish_partial(acc, partials, &st->counter)) {
-        double t = sum_partials(partials, gridDim.x);
-        if (threadIdx.x == 0) {
-            const double rn = sqrt(t);
-            st->rnorm = rn;
-            st->niter += 1;
-            if (rn < st->atol || rn < st->rtol * st->bnorm) st->done = 1;
-            else if (st->niter >= st->maxit) st->done = 3;
-            else if (!isfinite(rn) || !isfinite(st->alpha)) st->done = 4;  // breakdown
-            st->beta = t / st->rz;
-            st->rz = t;
-            st->counter = 0u;
-        }
-    }
-}
-
-template <class T>
-__global__ void k_cg_pupdate(int64_t n, T* __restrict__ p, const T* __restrict__ r, const CgState* st) {
-    if (st->done) return;
-    const T beta = static_cast<T>(st->beta);
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = r[i] + beta * p[i];
-}
-
-__global__ void k_cg_set(CgState* st, double atol, double rtol, long long maxit) {
-    st->atol = atol;
-    st->rtol = rtol;
-    st->maxit = maxit;
-    st->counter = 0u;
-    st->done = 0;
-    st->niter = 0;
-}
-
-template <class T, int G>
-static int cg_run(int64_t n, const int32_t* indptr, const int32_t* indices, const T* vals, const T* b, T* x,
-                  double atol, double rtol, int64_t maxit, int check_every, int64_t* niter_host, double* res_host,
-                  void* work, size_t work_bytes, hipStream_t s) {
-    Arena ar(work, work_bytes);
-    T* r = ar.take<T>(n);
-    T* p = ar.take<T>(n);
-    T* Ap = ar.take<T>(n);
-    double* partials = ar.take<double>(2 * RED_GRID);
-    CgState* st = ar.take<CgState>(1);
-    if (!r || !p || !Ap || !partials || !st) {
-        set_error("fa_cg: workspace too small (%zu bytes)", work_bytes);
-        return FA_ENOMEM;
-    }
-    const dim3 grid(RED_GRID), block(RED_BLOCK);
-    hipLaunchKernelGGL(k_cg_set, dim3(1), dim3(1), 0, s, st, atol, rtol, (long long)(maxit > 0 ? maxit : (1LL << 62)));
-    hipLaunchKernelGGL((k_cg_init<T, G>), grid, block, 0, s, n, indptr, indices, vals, b, x, r, p, partials, st);
-    hipLaunchKernelGGL(k_cg_trivial<T>, grid, block, 0, s, n, x, st);
-    FA_LAUNCH("k_cg_init");
-    CgState h{};
-    const int64_t cap = maxit > 0 ? maxit : (1LL << 62);
-    int64_t it = 0;
-    for (;;) {
-        const int64_t chunk = std::min<int64_t>(check_every, cap - it);
-        for (int64_t k = 0; k < chunk; ++k) {
-            hipLaunchKernelGGL((k_cg_spmv<T, G>), grid, block, 0, s, n, indptr, indices, vals, p, Ap, partials, st);
-            hipLaunchKernelGGL(k_cg_update<T>, grid, block, 0, s, n, x, r, p, Ap, partials, st);
-            hipLaunchKernelGGL(k_cg_pupdate<T>, grid, block, 0, s, n, p, r, st);
-        }
-        FA_LAUNCH("cg iteration");
-        it += chunk;
-        FA_HIP(hipMemcpyAsync(&h, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
-        FA_HIP(hipStreamSynchronize(s));
-        if (h.done || it >= cap) break;
-    }
-    if (niter_host) *niter_host = h.niter;
-    if (res_host) *res_host = h.rnorm;
-    return (h.done == 3 || h.done == 4) ? FA_ENOCONV : FA_OK;
 }
 
 }  // namespace fa
@@ -494,30 +267,3 @@ extern "C" int fa_cast(int dtype_in, int dtype_out, int64_t n, const void* x, vo
     return FA_OK;
 }
 
-extern "C" size_t fa_cg_work_bytes(int dtype, int64_t n) {
-    const size_t e = dtype == FA_F32 ? 4 : 8;
-    return 3 * align_up(n * e) + align_up(2 * RED_GRID * sizeof(double)) + align_up(sizeof(CgState)) + 1024;
-}
-
-extern "C" int fa_cg(int dtype, int64_t n, int64_t nnz, const int32_t* indptr, const int32_t* indices,
-                     const void* values, const void* b, void* x, double atol, double rtol, int64_t maxit,
-                     int32_t check_every, int64_t* niter_host, double* residual_host, void* work, size_t work_bytes,
-                     fa_stream_t stream) {
-    FA_REQUIRE(dtype == FA_F64 || dtype == FA_F32, "fa_cg: bad dtype %d", dtype);
-    FA_REQUIRE(n > 0 && check_every > 0, "fa_cg: bad sizes");
-    hipStream_t s = S(stream);
-    const int G = lanes_for(n, nnz);
-#define FA_CG_CASE(T, G_)                                                                                        \
-    if (G == G_)                                                                                                 \
-        return cg_run<T, G_>(n, indptr, indices, static_cast<const T*>(values), static_cast<const T*>(b),        \
-                             static_cast<T*>(x), atol, rtol, maxit, check_every, niter_host, residual_host, work, \
-                             work_bytes, s);
-    if (dtype == FA_F64) {
-        FA_CG_CASE(double, 4) FA_CG_CASE(double, 8) FA_CG_CASE(double, 16) FA_CG_CASE(double, 32)
-        FA_CG_CASE(double, 64)
-    } else {
-        FA_CG_CASE(float, 4) FA_CG_CASE(float, 8) FA_CG_CASE(float, 16) FA_CG_CASE(float, 32) FA_CG_CASE(float, 64)
-    }
-#undef FA_CG_CASE
-    return FA_EINVAL;
-}

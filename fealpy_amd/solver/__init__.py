@@ -1,3 +1,6 @@
 from .cg import cg
 
 __all__ = ["cg"]
+from .mg import GeometricMG
+
+__all__.append("GeometricMG")

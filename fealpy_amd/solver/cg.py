@@ -25,7 +25,7 @@ def cg(A, b, x0=None, M=None, *, batch_first=False, atol=1e-12, rtol=1e-8, maxit
         raise ValueError("x0 and b must have the same shape")
     if b.ndim == 2:
         raise NotImplementedError("multiple right-hand sides: solve them one by one or use fealpy_amd.serving")
-    if M is not None and hasattr(M, 'pcg'):
+    if M is not None and hasattr(M, 'pcg') and isinstance(A, CSRTensor):
         x, info = M.pcg(A, b, x0, atol=atol, rtol=rtol, maxit=maxit)
     elif M is None and isinstance(A, CSRTensor):
         x, info = _cg_native(A, b, x0, atol, rtol, maxit, check_every)
@@ -35,19 +35,24 @@ def cg(A, b, x0=None, M=None, *, batch_first=False, atol=1e-12, rtol=1e-8, maxit
 
 
 def _cg_native(A, b, x0, atol, rtol, maxit, check_every):
+    """Plain CG on the SELL copy of A (fa_pcg with no preconditioner)."""
+    import ctypes
+    from ..sparse.sell import sell_of
     L = nat.lib()
     dt = A.dtype
+    S = sell_of(A)
     b = b.to(dt).contiguous()
     x = torch.zeros_like(b) if x0 is None else x0.to(dt).clone().contiguous()
     n = b.shape[0]
-    ws = nat.workspace(L.fa_cg_work_bytes(nat.dtype_code(dt), n), b.device)
-    niter = nat.ctypes.c_int64(0)
-    res = nat.ctypes.c_double(0.0)
-    st = L.fa_cg(nat.dtype_code(dt), n, A.nnz, nat.ptr(A.crow), nat.ptr(A.col), nat.ptr(A.values), nat.ptr(b),
-                 nat.ptr(x), float(atol), float(rtol), int(maxit if maxit is not None else 0), int(check_every),
-                 nat.ctypes.byref(niter), nat.ctypes.byref(res), nat.ptr(ws), ws.numel(), nat.stream())
+    ws = nat.workspace(L.fa_pcg_work_bytes(nat.dtype_code(dt), n), b.device)
+    niter = ctypes.c_int64(0)
+    res = ctypes.c_double(0.0)
+    d = S.desc(dt)
+    st = L.fa_pcg(nat.dtype_code(dt), ctypes.byref(d), None, nat.ptr(b), nat.ptr(x), float(atol), float(rtol),
+                  int(maxit if maxit is not None else 0), int(check_every), ctypes.byref(niter), ctypes.byref(res),
+                  nat.ptr(ws), ws.numel(), nat.stream())
     if st != nat.FA_ENOCONV:
-        nat.check(st, "fa_cg")
+        nat.check(st, "fa_pcg")
     return x, {'residual': float(res.value), 'niter': int(niter.value)}
 
 

@@ -159,18 +159,108 @@ int fa_dot(int dtype, int64_t n, const void* x, const void* y, double* out, void
 int fa_axpby(int dtype, int64_t n, double a, const void* x, double b, void* y, fa_stream_t stream);
 int fa_cast(int dtype_in, int dtype_out, int64_t n, const void* x, void* y, fa_stream_t stream);
 
-/* -------------------------------------------------------------------- CG --
- * cg(A, b, x0, M=None, atol, rtol, maxit) of fealpy/solver/cg.py:14-123 for
- * one right-hand side, with the reference's stopping rule
- * (||r|| < atol or ||r|| < rtol ||b|| or niter >= maxit, checked after each
- * update).  x holds x0 on entry and the solution on exit.  The host checks
- * convergence every `check_every` iterations; iterations past convergence are
- * skipped on the device.  Returns FA_OK or FA_ENOCONV (maxit reached);
- * niter/residual are written to host memory. */
-size_t fa_cg_work_bytes(int dtype, int64_t n);
-int fa_cg(int dtype, int64_t n, int64_t nnz, const int32_t* indptr, const int32_t* indices, const void* values,
-          const void* b, void* x, double atol, double rtol, int64_t maxit, int32_t check_every,
-          int64_t* niter_host, double* residual_host, void* work, size_t work_bytes, fa_stream_t stream);
+/* ----------------------------------------------------------------- SELL-64 --
+ * Solver-side matrix format: sliced ELLPACK whose slice is one 64-lane
+ * wavefront of consecutive rows.  Row r = 64 s + i keeps its k-th entry at
+ * slice_ptr[s] + 64 k + i (coalesced across the wave); the slice width is its
+ * longest row; padding entries have value 0.  Built from CSR once per pattern
+ * (fa_sell_count + fa_sell_fill with cols), values refreshed per assembly
+ * (fa_sell_fill with cols = NULL), optionally cast FP64 -> FP32 on the way. */
+typedef struct {
+    int64_t n;                 /* rows                                          */
+    int64_t ncols;             /* columns                                       */
+    const int32_t* slice_ptr;  /* ceil(n/64) + 1 entry offsets                  */
+    const int32_t* cols;
+    const void* vals;          /* FP64 or FP32, as the call's dtype says        */
+} fa_sell_t;
+
+size_t fa_sell_work_bytes(int64_t n);
+int fa_sell_count(int64_t n, const int32_t* indptr, int32_t* slice_ptr, int64_t* entries_host,
+                  void* work, size_t work_bytes, fa_stream_t stream);
+int fa_sell_fill(int dtype_in, int dtype_out, int64_t n, const int32_t* indptr, const int32_t* indices,
+                 const void* values, const int32_t* slice_ptr, int32_t* cols, void* vals, fa_stream_t stream);
+int fa_sell_spmv(int dtype, const fa_sell_t* A, const void* x, void* y, fa_stream_t stream);
+/* 1 / diag(A) of a CSR matrix (rows without a diagonal get 1). */
+int fa_diag_inv(int dtype_in, int dtype_out, int64_t n, const int32_t* indptr, const int32_t* indices,
+                const void* values, void* dinv, fa_stream_t stream);
+
+/* ------------------------------------------------------- Galerkin R A P --
+ * Coarse operator C = R A P of the multigrid hierarchy, the role of
+ * `R @ A @ P` with SciPy SpGEMM in GAMGSolver.setup
+ * (fealpy/solver/gamg_solver.py:85-101).  All operands CSR (int32 indices,
+ * FP64 values).  Symbolic phase (count + fill) once per hierarchy; numeric
+ * phase per assembly, summed in a fixed order (no atomics).  A coarse row
+ * with no entries (a Dirichlet node outside R's range) gets a unit diagonal.
+ * max_cand bounds sum_{i in R row} sum_{j in A row i} |P row j|. */
+size_t fa_rap_work_bytes(int64_t nrows);
+int fa_rap_count(int64_t nrows, int64_t ncols, const int32_t* r_ptr, const int32_t* r_idx,
+                 const int32_t* a_ptr, const int32_t* a_idx, const int32_t* p_ptr, const int32_t* p_idx,
+                 int32_t max_cand, int32_t* c_ptr, int64_t* nnz_host, int32_t* max_row_host,
+                 void* work, size_t work_bytes, fa_stream_t stream);
+int fa_rap_fill(int64_t nrows, const int32_t* r_ptr, const int32_t* r_idx, const int32_t* a_ptr,
+                const int32_t* a_idx, const int32_t* p_ptr, const int32_t* p_idx, int32_t max_cand,
+                const int32_t* c_ptr, int32_t* c_idx, fa_stream_t stream);
+int fa_rap_values(int64_t nrows, const int32_t* r_ptr, const int32_t* r_idx, const double* r_val,
+                  const int32_t* a_ptr, const int32_t* a_idx, const double* a_val,
+                  const int32_t* p_ptr, const int32_t* p_idx, const double* p_val,
+                  const int32_t* c_ptr, const int32_t* c_idx, int32_t max_row, double* c_val,
+                  fa_stream_t stream);
+
+/* ------------------------------------------------------- multigrid cycle --
+ * Geometric multigrid V-cycle used as a CG preconditioner: Chebyshev
+ * polynomial smoothing of D^-1 A (pre and post, same polynomial, so the cycle
+ * is symmetric), restriction R, prolongation P, dense Cholesky on the
+ * coarsest level.  Replaces GAMGSolver.vcycle (fealpy/solver/gamg_solver.py:
+ * 256-301), whose Gauss-Seidel smoothing is a sequential triangular solve.
+ * All level arrays are caller-owned device memory of the cycle's dtype. */
+typedef struct {
+    fa_sell_t A;               /* operator of this level                        */
+    const void* dinv;          /* 1 / diag(A)                                   */
+    fa_sell_t R;               /* restriction to the next coarser level         */
+    fa_sell_t P;               /* prolongation from the next coarser level      */
+    double lmax;               /* largest-eigenvalue estimate of D^-1 A         */
+    void* b;                   /* level vectors of length A.n                   */
+    void* x;
+    void* x2;
+    void* d;
+    void* r;
+} fa_mg_level_t;
+
+typedef struct {
+    int dtype;                 /* precision of the cycle                        */
+    int nlevels;
+    const fa_mg_level_t* levels;   /* host array, finest level first            */
+    int degree;                /* Chebyshev degree of each smoothing sweep      */
+    double lower, upper;       /* smoothing interval [lower, upper] * lmax      */
+    const double* coarse_chol; /* lower Cholesky factor of the coarsest A,
+                                  row-major n x n FP64                          */
+} fa_mg_t;
+
+/* Dense row-major copy of a small CSR matrix (FP64) and its in-place
+ * Cholesky factorisation (lower triangle), single workgroup. */
+int fa_csr_to_dense(int64_t n, const int32_t* indptr, const int32_t* indices, const double* values,
+                    double* dense, fa_stream_t stream);
+int fa_dense_cholesky(int64_t n, double* a, fa_stream_t stream);
+/* Power iteration for the largest eigenvalue of D^-1 A (level `level`);
+ * uses the level's x, x2 as scratch; synchronises. */
+int fa_mg_lmax(const fa_mg_t* mg, int level, int iters, double* lmax_host, void* work, size_t work_bytes,
+               fa_stream_t stream);
+/* z = V(r) on the finest level (r, z: dtype of the cycle). */
+int fa_mg_vcycle(const fa_mg_t* mg, const void* r, void* z, fa_stream_t stream);
+
+/* -------------------------------------------------------------------- PCG --
+ * cg(A, b, x0, M, atol, rtol, maxit) of fealpy/solver/cg.py:14-123 for one
+ * right-hand side with the reference's stopping rule (sqrt(r.z) < atol or
+ * < rtol ||b||, or niter >= maxit, tested after each update).  A and the
+ * vectors b, x are in `dtype`; M = NULL (plain CG) or a V-cycle of any dtype
+ * (FP64 CG with an FP32 cycle is the mixed-precision solver).  x holds x0 on
+ * entry.  The host reads the device convergence flag every `check_every`
+ * iterations; iterations past convergence are skipped on the device.
+ * Returns FA_OK, or FA_ENOCONV (maxit reached or a NaN/Inf breakdown). */
+size_t fa_pcg_work_bytes(int dtype, int64_t n);
+int fa_pcg(int dtype, const fa_sell_t* A, const fa_mg_t* M, const void* b, void* x, double atol, double rtol,
+           int64_t maxit, int32_t check_every, int64_t* niter_host, double* residual_host,
+           void* work, size_t work_bytes, fa_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -105,7 +105,7 @@ def test_cabi_exports_every_declared_symbol():
         assert hasattr(lib, n), n
         assert n in _native._SIGNATURES, f"{n} has no ctypes signature"
     assert lib.fa_version() >= 1
-    assert lib.fa_cg_work_bytes(0, 1000) > 3 * 8000
+    assert lib.fa_pcg_work_bytes(0, 1000) > 3 * 8000
 
 
 def test_product_refuses_cpu_tensors():
