@@ -51,7 +51,7 @@ _SIGNATURES = {
     "fa_cast": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64, _p, _p, _p]),
     "fa_sell_work_bytes": (_sz, [_i64]),
     "fa_sell_count": (ctypes.c_int, [_i64, _p, _p, _pi64, _p, _sz, _p]),
-    "fa_sell_fill": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64, _p, _p, _p, _p, _p, _p, _p]),
+    "fa_sell_fill": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64, _i64, _p, _p, _p, _p, _p, _p, _p]),
     "fa_sell_spmv": (ctypes.c_int, [ctypes.c_int, _p, _p, _p, _p]),
     "fa_diag_inv": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64, _p, _p, _p, _p, _p]),
     "fa_rap_work_bytes": (_sz, [_i64]),

@@ -16,8 +16,11 @@ __global__ void __launch_bounds__(256) k_sell_widths(int64_t n, const int32_t* _
     }
 }
 
+// Padding entries get value 0 and a column that is valid for x: the row itself
+// when it is a column index (square matrices: same cache line as the
+// diagonal), else column 0 (e.g. a prolongation with fewer columns than rows).
 template <class A, class B>
-__global__ void __launch_bounds__(256) k_sell_fill(int64_t n, const int32_t* __restrict__ indptr,
+__global__ void __launch_bounds__(256) k_sell_fill(int64_t n, int64_t ncols, const int32_t* __restrict__ indptr,
                                                   const int32_t* __restrict__ indices, const A* __restrict__ vin,
                                                   const int32_t* __restrict__ slice_ptr, int32_t* __restrict__ cols,
                                                   B* __restrict__ vals) {
@@ -30,7 +33,7 @@ __global__ void __launch_bounds__(256) k_sell_fill(int64_t n, const int32_t* __r
         const int w = (slice_ptr[s + 1] - p0) >> 6;
         const int32_t r0 = row < n ? indptr[row] : 0;
         const int len = row < n ? indptr[row + 1] - r0 : 0;
-        const int32_t pad = row < n ? static_cast<int32_t>(row) : 0;
+        const int32_t pad = row < ncols ? static_cast<int32_t>(row) : 0;
         for (int k = 0; k < w; ++k) {
             const int64_t o = p0 + 64 * k + i;
             if (cols) cols[o] = k < len ? indices[r0 + k] : pad;
@@ -94,15 +97,15 @@ extern "C" int fa_sell_count(int64_t n, const int32_t* indptr, int32_t* slice_pt
     return FA_OK;
 }
 
-extern "C" int fa_sell_fill(int dtype_in, int dtype_out, int64_t n, const int32_t* indptr, const int32_t* indices,
-                            const void* values, const int32_t* slice_ptr, int32_t* cols, void* vals,
-                            fa_stream_t stream) {
-    FA_REQUIRE(n > 0, "fa_sell_fill: empty matrix");
+extern "C" int fa_sell_fill(int dtype_in, int dtype_out, int64_t n, int64_t ncols, const int32_t* indptr,
+                            const int32_t* indices, const void* values, const int32_t* slice_ptr, int32_t* cols,
+                            void* vals, fa_stream_t stream) {
+    FA_REQUIRE(n > 0 && ncols > 0, "fa_sell_fill: empty matrix");
     FA_REQUIRE(!vals || values, "fa_sell_fill: values missing");
     hipStream_t s = S(stream);
     const unsigned grid = grid_for(((n + 63) / 64) * 64, 256, 1 << 16);
 #define FA_SF(A_, B_)                                                                                       \
-    hipLaunchKernelGGL((k_sell_fill<A_, B_>), dim3(grid), dim3(256), 0, s, n, indptr, indices,             \
+    hipLaunchKernelGGL((k_sell_fill<A_, B_>), dim3(grid), dim3(256), 0, s, n, ncols, indptr, indices,      \
                        static_cast<const A_*>(values), slice_ptr, cols, static_cast<B_*>(vals))
     if (dtype_in == FA_F64 && dtype_out == FA_F64) FA_SF(double, double);
     else if (dtype_in == FA_F64 && dtype_out == FA_F32) FA_SF(double, float);

@@ -28,9 +28,11 @@ class SellMatrix:
         nat.check(L.fa_sell_count(n, nat.ptr(crow), nat.ptr(self.slice_ptr), ctypes.byref(ent), nat.ptr(ws),
                                   ws.numel(), nat.stream()), "fa_sell_count")
         self.entries = int(ent.value)
-        self.cols = torch.empty(max(self.entries, 1), dtype=torch.int32, device=dev)
-        nat.check(L.fa_sell_fill(nat.FA_F64, nat.FA_F64, n, nat.ptr(crow), nat.ptr(col), None,
-                                 nat.ptr(self.slice_ptr), nat.ptr(self.cols), None, nat.stream()), "fa_sell_fill")
+        self.cols = torch.zeros(max(self.entries, 1), dtype=torch.int32, device=dev)
+        if self.entries > 0:
+            nat.check(L.fa_sell_fill(nat.FA_F64, nat.FA_F64, n, self.shape[1], nat.ptr(crow), nat.ptr(col), None,
+                                     nat.ptr(self.slice_ptr), nat.ptr(self.cols), None, nat.stream()),
+                      "fa_sell_fill")
         self.vals = {}
 
     def set_values(self, values, dtype=None):
@@ -38,10 +40,12 @@ class SellMatrix:
         dtype = values.dtype if dtype is None else dtype
         out = self.vals.get(dtype)
         if out is None:
-            out = torch.empty(max(self.entries, 1), dtype=dtype, device=values.device)
+            out = torch.zeros(max(self.entries, 1), dtype=dtype, device=values.device)
             self.vals[dtype] = out
+        if self.entries == 0 or values.numel() == 0:      # no stored entries (e.g. an empty transfer)
+            return out
         nat.check(nat.lib().fa_sell_fill(nat.dtype_code(values.dtype), nat.dtype_code(dtype), self.shape[0],
-                                         nat.ptr(self.crow), nat.ptr(self.col), nat.ptr(values.contiguous()),
+                                         self.shape[1], nat.ptr(self.crow), nat.ptr(self.col), nat.ptr(values.contiguous()),
                                          nat.ptr(self.slice_ptr), None, nat.ptr(out), nat.stream()), "fa_sell_fill")
         return out
 

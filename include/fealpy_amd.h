@@ -163,7 +163,8 @@ int fa_cast(int dtype_in, int dtype_out, int64_t n, const void* x, void* y, fa_s
  * Solver-side matrix format: sliced ELLPACK whose slice is one 64-lane
  * wavefront of consecutive rows.  Row r = 64 s + i keeps its k-th entry at
  * slice_ptr[s] + 64 k + i (coalesced across the wave); the slice width is its
- * longest row; padding entries have value 0.  Built from CSR once per pattern
+ * longest row; padding entries have value 0 and a column index < ncols.
+ * Built from CSR once per pattern
  * (fa_sell_count + fa_sell_fill with cols), values refreshed per assembly
  * (fa_sell_fill with cols = NULL), optionally cast FP64 -> FP32 on the way. */
 typedef struct {
@@ -177,8 +178,9 @@ typedef struct {
 size_t fa_sell_work_bytes(int64_t n);
 int fa_sell_count(int64_t n, const int32_t* indptr, int32_t* slice_ptr, int64_t* entries_host,
                   void* work, size_t work_bytes, fa_stream_t stream);
-int fa_sell_fill(int dtype_in, int dtype_out, int64_t n, const int32_t* indptr, const int32_t* indices,
-                 const void* values, const int32_t* slice_ptr, int32_t* cols, void* vals, fa_stream_t stream);
+int fa_sell_fill(int dtype_in, int dtype_out, int64_t n, int64_t ncols, const int32_t* indptr,
+                 const int32_t* indices, const void* values, const int32_t* slice_ptr, int32_t* cols, void* vals,
+                 fa_stream_t stream);
 int fa_sell_spmv(int dtype, const fa_sell_t* A, const void* x, void* y, fa_stream_t stream);
 /* 1 / diag(A) of a CSR matrix (rows without a diagonal get 1). */
 int fa_diag_inv(int dtype_in, int dtype_out, int64_t n, const int32_t* indptr, const int32_t* indices,

@@ -79,6 +79,17 @@ def test_rap_matches_scipy_triple_product():
     assert diff.max() < 1e-12 * abs(ref).max()
 
 
+def test_sell_padding_columns_in_range():
+    """Padding entries of rectangular transfers (P: fine x coarse) must point at
+    a valid column of the operand (regression: they used the row index)."""
+    mesh, space, bc, A, b = _system(16)
+    mg = GeometricMG(space, bc.is_boundary_dof, dtype=torch.float64)
+    for lv in mg.levels[:-1]:
+        for S in (lv.Ps, lv.Rs):
+            assert int(S.cols[:max(S.entries, 1)].max()) < S.shape[1]
+            assert int(S.cols[:max(S.entries, 1)].min()) >= 0
+
+
 def test_vcycle_is_symmetric_and_spd():
     mesh, space, bc, A, b = _system(16)
     k = _lognormal(mesh)
