@@ -68,14 +68,14 @@ __global__ void k_bc_fill(int64_t n, const int32_t* __restrict__ indptr, const i
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
         int32_t o = nptr[r];
         if (bd[r]) {
-            nidx[o] = static_cast<int32_t>(r);
+            if (nidx) nidx[o] = static_cast<int32_t>(r);
             nval[o] = T(1);
             continue;
         }
         for (int32_t k = indptr[r]; k < indptr[r + 1]; ++k) {
             const int32_t c = indices[k];
             if (!bd[c]) {
-                nidx[o] = c;
+                if (nidx) nidx[o] = c;
                 nval[o] = vals[k];
                 ++o;
             }

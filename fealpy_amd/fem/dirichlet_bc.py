@@ -59,6 +59,26 @@ class DirichletBC:
         L = nat.lib()
         n = A.shape[0]
         dev = A.device
+        new_ptr, new_idx, fresh = self._filtered_pattern(A)
+        new_val = torch.empty(new_idx.shape[0], dtype=A.dtype, device=dev)
+        nat.check(L.fa_dirichlet_fill(nat.dtype_code(A.dtype), n, nat.ptr(A.crow), nat.ptr(A.col),
+                                      nat.ptr(A.values), nat.ptr(self._bd_u8), nat.ptr(new_ptr),
+                                      nat.ptr(new_idx) if fresh else None, nat.ptr(new_val), nat.stream()),
+                  "fa_dirichlet_fill")
+        out = CSRTensor(new_ptr, new_idx, new_val, A.shape)
+        return out.tocoo() if was_coo else out
+
+    def _filtered_pattern(self, A):
+        """(indptr, indices, fresh) of the filtered pattern.  The pattern depends
+        only on A's pattern and the boundary flags, so it is kept on the space
+        and reused by later assemblies (fresh=False: indices already written)."""
+        cache = self.space.__dict__.setdefault('_bc_patterns', [])
+        for (crow, col, bd), val in cache:
+            if crow is A.crow and col is A.col and torch.equal(bd, self._bd_u8):
+                return val + (False,)
+        L = nat.lib()
+        n = A.shape[0]
+        dev = A.device
         new_ptr = torch.empty(n + 1, dtype=torch.int32, device=dev)
         ws = nat.workspace(L.fa_dirichlet_work_bytes(n), dev)
         nnz = nat.ctypes.c_int64(0)
@@ -66,12 +86,9 @@ class DirichletBC:
                                        nat.ctypes.byref(nnz), nat.ptr(ws), ws.numel(), nat.stream()),
                   "fa_dirichlet_count")
         new_idx = torch.empty(int(nnz.value), dtype=torch.int32, device=dev)
-        new_val = torch.empty(int(nnz.value), dtype=A.dtype, device=dev)
-        nat.check(L.fa_dirichlet_fill(nat.dtype_code(A.dtype), n, nat.ptr(A.crow), nat.ptr(A.col),
-                                      nat.ptr(A.values), nat.ptr(self._bd_u8), nat.ptr(new_ptr), nat.ptr(new_idx),
-                                      nat.ptr(new_val), nat.stream()), "fa_dirichlet_fill")
-        out = CSRTensor(new_ptr, new_idx, new_val, A.shape)
-        return out.tocoo() if was_coo else out
+        del cache[:-3]                      # keep a few patterns
+        cache.append(((A.crow, A.col, self._bd_u8), (new_ptr, new_idx)))
+        return new_ptr, new_idx, True
 
     def apply_vector(self, vector, matrix, uh=None, gd=None, *, check=True):
         A = self.check_matrix(matrix) if check else matrix

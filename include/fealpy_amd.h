@@ -131,8 +131,9 @@ int fa_gather_vector(int dtype, int64_t gdof, int32_t ldof,
  * form: the new matrix drops every entry in a boundary row or boundary column
  * (_mul, :213-229) and gets a unit diagonal on boundary rows (spdiags + add,
  * :123-147); f <- f - A uh, f[bd] <- uh[bd] (apply_vector :149-198).
- * fa_dirichlet_count writes new_indptr[n+1] and returns the new nnz (syncs).
- * is_bd: uint8[n]; uh: boundary values (zeros off the boundary), `dtype`. */
+ * fa_dirichlet_count writes new_indptr[n+1] and returns the new nnz (syncs);
+ * fa_dirichlet_fill writes new_indices (skipped when NULL: pattern reuse) and
+ * new_values.  is_bd: uint8[n]; uh: boundary values (zeros off the boundary). */
 size_t fa_dirichlet_work_bytes(int64_t n);
 int fa_dirichlet_count(int64_t n, const int32_t* indptr, const int32_t* indices, const uint8_t* is_bd,
                        int32_t* new_indptr, int64_t* new_nnz_host,
