@@ -298,10 +298,55 @@ __global__ void k_scale(int64_t n, const T* __restrict__ w, T* __restrict__ v, c
         v[i] = w[i] * f;
 }
 
+// pseudo-random start vector (integer hash -> [0.5, 1.5)): excites every mode,
+// unlike a smooth start, which converges slowly to the top of the spectrum
 template <class T>
 __global__ void k_power_init(int64_t n, T* __restrict__ v) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        v[i] = static_cast<T>(1.0 + 0.5 * sin(0.7 * static_cast<double>(i)));
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t h = static_cast<uint32_t>(i) * 2654435761u;
+        h ^= h >> 16;
+        h *= 2246822519u;
+        h ^= h >> 13;
+        v[i] = static_cast<T>(0.5 + (h & 0xffffff) / 16777216.0);
+    }
+}
+
+// Gershgorin bound of D^-1 A: max_i sum_j |a_ij| / |a_ii|  (>= lambda_max)
+__global__ void __launch_bounds__(RED_BLOCK) k_gershgorin(int64_t n, const int32_t* __restrict__ ptr,
+                                                         const int32_t* __restrict__ idx,
+                                                         const double* __restrict__ val,
+                                                         double* __restrict__ partials,
+                                                         unsigned* __restrict__ counter, double* __restrict__ out) {
+    __shared__ double ws[RED_BLOCK / WAVE];
+    __shared__ bool last;
+    double m = 0.0;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        double s = 0.0, d = 0.0;
+        for (int32_t k = ptr[r]; k < ptr[r + 1]; ++k) {
+            s += fabs(val[k]);
+            if (idx[k] == r) d = fabs(val[k]);
+        }
+        if (d > 0.0) m = fmax(m, s / d);
+    }
+    for (int o = WAVE / 2; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if ((threadIdx.x & (WAVE - 1)) == 0) ws[threadIdx.x / WAVE] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = 0.0;
+        for (int w = 0; w < RED_BLOCK / WAVE; ++w) b = fmax(b, ws[w]);
+        partials[blockIdx.x] = b;
+        __threadfence();
+        last = atomicAdd(counter, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        __threadfence();
+        double b = 0.0;
+        for (unsigned i = 0; i < gridDim.x; ++i)
+            b = fmax(b, __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        *out = b;
+        *counter = 0u;
+    }
 }
 
 __global__ void k_zero_counter(unsigned* c) { *c = 0u; }
@@ -795,6 +840,29 @@ extern "C" int fa_mg_lmax(const fa_mg_t* mg, int level, int iters, double* lmax_
     FA_HIP(hipMemcpyAsync(&lam, out, sizeof(double), hipMemcpyDeviceToHost, s));
     FA_HIP(hipStreamSynchronize(s));
     if (lmax_host) *lmax_host = lam;
+    return FA_OK;
+}
+
+extern "C" int fa_gershgorin(int64_t n, const int32_t* indptr, const int32_t* indices, const double* values,
+                             double* bound_host, void* work, size_t work_bytes, fa_stream_t stream) {
+    FA_REQUIRE(n > 0, "fa_gershgorin: empty matrix");
+    hipStream_t s = S(stream);
+    Arena ar(work, work_bytes);
+    double* partials = ar.take<double>(RED_GRID);
+    unsigned* counter = ar.take<unsigned>(1);
+    double* out = ar.take<double>(1);
+    if (!partials || !counter || !out) {
+        set_error("fa_gershgorin: workspace too small");
+        return FA_ENOMEM;
+    }
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, counter);
+    hipLaunchKernelGGL(k_gershgorin, dim3(RED_GRID), dim3(RED_BLOCK), 0, s, n, indptr, indices, values, partials,
+                       counter, out);
+    FA_LAUNCH("k_gershgorin");
+    double h = 0.0;
+    FA_HIP(hipMemcpyAsync(&h, out, sizeof(double), hipMemcpyDeviceToHost, s));
+    FA_HIP(hipStreamSynchronize(s));
+    if (bound_host) *bound_host = h;
     return FA_OK;
 }
 

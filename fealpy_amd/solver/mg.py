@@ -13,7 +13,8 @@ the preconditioner of CG.  Designed for the GPU:
 * Dirichlet rows stay decoupled: P has no rows for fine Dirichlet nodes and
   no columns for coarse Dirichlet nodes; coarse Dirichlet rows become unit rows;
 * smoothing is a Chebyshev polynomial in D^-1 A (degree ``degree``, interval
-  [lower, upper] * lambda_max from a power iteration), the same polynomial
+  [lower, upper] * the Gershgorin bound of D^-1 A -- an upper bound of the
+  spectrum, so no mode is amplified), the same polynomial
   before and after the coarse correction, so the V-cycle is symmetric and
   CG-compatible; the coarsest level is solved by a dense Cholesky factor;
 * the cycle runs in ``dtype`` (FP32 for the mixed-precision solver: FP64 CG
@@ -110,7 +111,7 @@ class _Level:
 
 
 class GeometricMG:
-    def __init__(self, space, is_bd, *, dtype=torch.float32, degree=2, lower=0.1, upper=1.1, coarse_max=512,
+    def __init__(self, space, is_bd, *, dtype=torch.float32, degree=2, lower=0.1, upper=1.0, coarse_max=512,
                  power_iters=12):
         if space.p != 1:
             raise NotImplementedError("GeometricMG covers P1 spaces")
@@ -207,12 +208,22 @@ class GeometricMG:
         self._setup_A = A
         ws = nat.workspace(8 * 4 * 1024 + 4096, self.device)
         for l in range(len(self.levels) - 1):
+            lv = self.levels[l]
             lam = ctypes.c_double(0.0)
-            nat.check(L.fa_mg_lmax(ctypes.byref(self._desc), l, self.power_iters, ctypes.byref(lam), nat.ptr(ws),
-                                   ws.numel(), nat.stream()), "fa_mg_lmax")
-            self.levels[l].lmax = float(lam.value)
-            self._lv_array[l].lmax = float(lam.value)
+            nat.check(L.fa_gershgorin(lv.n, nat.ptr(lv.A.crow), nat.ptr(lv.A.col), nat.ptr(lv.A.values),
+                                      ctypes.byref(lam), nat.ptr(ws), ws.numel(), nat.stream()), "fa_gershgorin")
+            lv.lmax = float(lam.value)
+            self._lv_array[l].lmax = lv.lmax
         return self
+
+    def estimate_lmax(self, level, iters=None):
+        """Power-iteration estimate of lambda_max(D^-1 A) on a level (diagnostics)."""
+        L = nat.lib()
+        ws = nat.workspace(8 * 4 * 1024 + 4096, self.device)
+        lam = ctypes.c_double(0.0)
+        nat.check(L.fa_mg_lmax(ctypes.byref(self._desc), level, iters or self.power_iters, ctypes.byref(lam),
+                               nat.ptr(ws), ws.numel(), nat.stream()), "fa_mg_lmax")
+        return float(lam.value)
 
     def _build_desc(self):
         n = len(self.levels)

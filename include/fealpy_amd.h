@@ -244,8 +244,12 @@ typedef struct {
 int fa_csr_to_dense(int64_t n, const int32_t* indptr, const int32_t* indices, const double* values,
                     double* dense, fa_stream_t stream);
 int fa_dense_cholesky(int64_t n, double* a, fa_stream_t stream);
-/* Power iteration for the largest eigenvalue of D^-1 A (level `level`);
- * uses the level's x, x2 as scratch; synchronises. */
+/* Bounds for the Chebyshev interval.  fa_gershgorin: max_i sum_j |a_ij|/|a_ii|
+ * of a CSR FP64 matrix, an upper bound of the spectrum of D^-1 A (the bound the
+ * smoother uses); syncs.  fa_mg_lmax: power-iteration estimate of the largest
+ * eigenvalue of D^-1 A on level `level` (uses the level's x, x2); syncs. */
+int fa_gershgorin(int64_t n, const int32_t* indptr, const int32_t* indices, const double* values,
+                  double* bound_host, void* work, size_t work_bytes, fa_stream_t stream);
 int fa_mg_lmax(const fa_mg_t* mg, int level, int iters, double* lmax_host, void* work, size_t work_bytes,
                fa_stream_t stream);
 /* z = V(r) on the finest level (r, z: dtype of the cycle). */
