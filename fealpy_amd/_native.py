@@ -40,6 +40,7 @@ _SIGNATURES = {
                                           _p, _p, ctypes.c_int, _f64, _p, _p, _p]),
     "fa_gather_matrix": (ctypes.c_int, [ctypes.c_int, _i64, _i32, _p, _p, _p, _p, _p, _i32, _p, _p, _p]),
     "fa_gather_vector": (ctypes.c_int, [ctypes.c_int, _i64, _i32, _p, _p, _p, _p, _p]),
+    "fa_bc_to_point": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64, ctypes.c_int, _p, _p, _p, _p, _p]),
     "fa_dirichlet_work_bytes": (_sz, [_i64]),
     "fa_dirichlet_count": (ctypes.c_int, [_i64, _p, _p, _p, _p, _pi64, _p, _sz, _p]),
     "fa_dirichlet_fill": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p, _p, _p, _p, _p]),
@@ -53,16 +54,15 @@ _SIGNATURES = {
     "fa_sell_count": (ctypes.c_int, [_i64, _p, _p, _pi64, _p, _sz, _p]),
     "fa_sell_fill": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64, _i64, _p, _p, _p, _p, _p, _p, _p]),
     "fa_sell_spmv": (ctypes.c_int, [ctypes.c_int, _p, _p, _p, _p]),
-    "fa_diag_inv": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64, _p, _p, _p, _p, _p]),
     "fa_rap_work_bytes": (_sz, [_i64]),
     "fa_rap_count": (ctypes.c_int, [_i64, _i64, _p, _p, _p, _p, _p, _p, _i32, _p, _pi64, _pi32, _p, _sz, _p]),
     "fa_rap_fill": (ctypes.c_int, [_i64, _p, _p, _p, _p, _p, _p, _i32, _p, _p, _p]),
     "fa_rap_values": (ctypes.c_int, [_i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i32, _p, _p]),
     "fa_csr_to_dense": (ctypes.c_int, [_i64, _p, _p, _p, _p, _p]),
-    "fa_dense_cholesky": (ctypes.c_int, [_i64, _p, _p]),
+    "fa_dense_spd_inverse": (ctypes.c_int, [_i64, _p, _p]),
     "fa_mg_lmax": (ctypes.c_int, [_p, ctypes.c_int, ctypes.c_int, _pf64, _p, _sz, _p]),
     "fa_mg_vcycle": (ctypes.c_int, [_p, _p, _p, _p]),
-    "fa_gershgorin": (ctypes.c_int, [_i64, _p, _p, _p, _pf64, _p, _sz, _p]),
+    "fa_jacobi_setup": (ctypes.c_int, [ctypes.c_int, _i64, _i64, _p, _p, _p, _p, _pf64, _p, _sz, _p]),
     "fa_pcg_work_bytes": (_sz, [ctypes.c_int, _i64]),
     "fa_pcg": (ctypes.c_int, [ctypes.c_int, _p, _p, _p, _p, _f64, _f64, _i64, _i32, _pi64, _pf64, _p, _sz, _p]),
 }

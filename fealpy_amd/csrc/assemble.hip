@@ -228,15 +228,48 @@ __global__ void k_gather_matrix(int64_t gdof, int ldof, const int32_t* __restric
     }
 }
 
+// 16 lanes per DOF, lane l adds incidences l, l+16, ...; the lane partials
+// are combined by a fixed tree (deterministic)
 template <class T>
 __global__ void __launch_bounds__(256) k_gather_vector(int64_t gdof, const int32_t* __restrict__ ptr,
                                                       const int32_t* __restrict__ adj, const T* __restrict__ elem,
                                                       T* __restrict__ out) {
-    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < gdof; r += (int64_t)gridDim.x * blockDim.x) {
+    constexpr int G = 16;
+    const int lane = threadIdx.x % G;
+    for (int64_t r = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G; r < gdof;
+         r += (int64_t)gridDim.x * blockDim.x / G) {
         T s = T(0);
         const int32_t k1 = ptr[r + 1];
-        for (int32_t k = ptr[r]; k < k1; ++k) s += elem[adj[k]];
-        out[r] = s;
+        for (int32_t k = ptr[r] + lane; k < k1; k += G) s += elem[adj[k]];
+        s = group_sum<G>(s);
+        if (lane == 0) out[r] = s;
+    }
+}
+
+// physical coordinates of barycentric points on every cell: one thread per
+// cell reads its vertices once and writes its nq*gd contiguous outputs
+template <int GD, int NV>
+__global__ void __launch_bounds__(256) k_bc_to_point(int64_t nc, int nq, const double* __restrict__ node,
+                                                    const int32_t* __restrict__ cell, const double* __restrict__ bcs,
+                                                    double* __restrict__ out) {
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
+        double v[NV][GD];
+#pragma unroll
+        for (int a = 0; a < NV; ++a) {
+            const int64_t nd = cell[c * NV + a];
+#pragma unroll
+            for (int d = 0; d < GD; ++d) v[a][d] = node[nd * GD + d];
+        }
+        double* o = out + c * nq * GD;
+        for (int q = 0; q < nq; ++q) {
+#pragma unroll
+            for (int d = 0; d < GD; ++d) {
+                double x = 0.0;
+#pragma unroll
+                for (int a = 0; a < NV; ++a) x += bcs[q * NV + a] * v[a][d];
+                o[q * GD + d] = x;
+            }
+        }
     }
 }
 
@@ -365,7 +398,7 @@ extern "C" int fa_gather_vector(int dtype, int64_t gdof, int32_t ldof, const int
     (void)ldof;
     FA_REQUIRE(dtype == FA_F64 || dtype == FA_F32, "fa_gather_vector: bad dtype %d", dtype);
     hipStream_t s = S(stream);
-    const unsigned grid = grid_for(gdof, 256, 1 << 16);
+    const unsigned grid = grid_for(gdof * 16, 256, 1 << 16);
     if (dtype == FA_F64)
         hipLaunchKernelGGL(k_gather_vector<double>, dim3(grid), dim3(256), 0, s, gdof, adj_ptr, adj,
                            static_cast<const double*>(elem), static_cast<double*>(out));
@@ -373,5 +406,25 @@ extern "C" int fa_gather_vector(int dtype, int64_t gdof, int32_t ldof, const int
         hipLaunchKernelGGL(k_gather_vector<float>, dim3(grid), dim3(256), 0, s, gdof, adj_ptr, adj,
                            static_cast<const float*>(elem), static_cast<float*>(out));
     FA_LAUNCH("k_gather_vector");
+    return FA_OK;
+}
+
+extern "C" int fa_bc_to_point(int gd, int nv, int64_t nc, int nq, const double* node, const int32_t* cell,
+                              const double* bcs, double* out, fa_stream_t stream) {
+    FA_REQUIRE(nq >= 1 && nc >= 0, "fa_bc_to_point: bad sizes");
+    if (nc == 0) return FA_OK;
+    hipStream_t s = S(stream);
+    const unsigned grid = grid_for(nc, 256, 1 << 16);
+#define FA_BP(GD, NV) \
+    hipLaunchKernelGGL((k_bc_to_point<GD, NV>), dim3(grid), dim3(256), 0, s, nc, nq, node, cell, bcs, out)
+    if (gd == 2 && nv == 3) FA_BP(2, 3);
+    else if (gd == 3 && nv == 3) FA_BP(3, 3);
+    else if (gd == 3 && nv == 4) FA_BP(3, 4);
+    else {
+        set_error("fa_bc_to_point: unsupported (gd, nv) = (%d, %d)", gd, nv);
+        return FA_EINVAL;
+    }
+#undef FA_BP
+    FA_LAUNCH("k_bc_to_point");
     return FA_OK;
 }

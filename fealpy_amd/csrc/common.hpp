@@ -94,6 +94,18 @@ __device__ inline T group_sum(T v) {
     return v;
 }
 
+// inclusive scan inside an aligned group of G lanes
+template <int G>
+__device__ inline int group_inclusive_scan(int v) {
+    const int lane = threadIdx.x % G;
+#pragma unroll
+    for (int o = 1; o < G; o <<= 1) {
+        int t = __shfl_up(v, o, G);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
 // scan of int32 counts: out[i] = sum_{j<i} in[j], out[n] = total (out has n+1 slots)
 int exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, void* work, size_t work_bytes,
                        hipStream_t s);

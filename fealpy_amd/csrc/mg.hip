@@ -215,46 +215,63 @@ __global__ void __launch_bounds__(256) k_csr_scatter_dense(int64_t n, const int3
         for (int32_t k = ptr[r]; k < ptr[r + 1]; ++k) dense[r * n + idx[k]] += val[k];
 }
 
-__global__ void __launch_bounds__(256) k_cholesky(int n, double* __restrict__ a) {
-    for (int k = 0; k < n; ++k) {
-        if (threadIdx.x == 0) a[(int64_t)k * n + k] = sqrt(a[(int64_t)k * n + k]);
-        __syncthreads();
-        const double dkk = a[(int64_t)k * n + k];
-        for (int i = k + 1 + threadIdx.x; i < n; i += blockDim.x) a[(int64_t)i * n + k] /= dkk;
-        __syncthreads();
-        const int m = n - k - 1;
-        for (int64_t t = threadIdx.x; t < (int64_t)m * m; t += blockDim.x) {
-            const int i = k + 1 + static_cast<int>(t / m), j = k + 1 + static_cast<int>(t % m);
-            if (j <= i) a[(int64_t)i * n + j] -= a[(int64_t)i * n + k] * a[(int64_t)j * n + k];
-        }
-        __syncthreads();
+// In-place Gauss-Jordan inversion of an SPD matrix (no pivoting needed), one
+// workgroup.  Step k: scale row k by 1/a_kk, stage row k and column k, then the
+// rank-1 update of every other row.
+constexpr int COARSE_MAX = 2048;
+constexpr int COARSE_LDS_MAX = 89;  // (n*n + 2n) doubles fit in 64 KiB of LDS
+
+__device__ inline void gj_step(int n, int k, double* m, double* rowk, double* colk) {
+    const double piv = m[k * n + k];
+    __syncthreads();
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        const double v = ((j == k) ? 1.0 : m[k * n + j]) / piv;
+        m[k * n + j] = v;
+        rowk[j] = v;
     }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) colk[i] = (i == k) ? 0.0 : m[i * n + k];
+    __syncthreads();
+    for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+        const int i = t / n, j = t - i * n;
+        if (i == k) continue;
+        const double f = colk[i];
+        m[t] = (j == k) ? -f * rowk[j] : m[t] - f * rowk[j];
+    }
+    __syncthreads();
 }
 
-constexpr int COARSE_MAX = 4096;
-
-template <class T>
-__global__ void __launch_bounds__(256) k_chol_solve(int n, const double* __restrict__ L, const T* __restrict__ b,
-                                                   T* __restrict__ x, const int* __restrict__ done) {
-    if (done && *done) return;
-    __shared__ double y[COARSE_MAX];
-    for (int i = threadIdx.x; i < n; i += blockDim.x) y[i] = static_cast<double>(b[i]);
+// whole matrix in LDS (n <= COARSE_LDS_MAX)
+__global__ void __launch_bounds__(256) k_spd_inverse_lds(int n, double* __restrict__ a) {
+    extern __shared__ double lds[];
+    double* m = lds;
+    double* rowk = lds + n * n;
+    double* colk = rowk + n;
+    for (int t = threadIdx.x; t < n * n; t += blockDim.x) m[t] = a[t];
     __syncthreads();
-    for (int j = 0; j < n; ++j) {
-        if (threadIdx.x == 0) y[j] /= L[(int64_t)j * n + j];
-        __syncthreads();
-        const double yj = y[j];
-        for (int i = j + 1 + threadIdx.x; i < n; i += blockDim.x) y[i] -= L[(int64_t)i * n + j] * yj;
-        __syncthreads();
+    for (int k = 0; k < n; ++k) gj_step(n, k, m, rowk, colk);
+    for (int t = threadIdx.x; t < n * n; t += blockDim.x) a[t] = m[t];
+}
+
+// matrix in global memory (n <= COARSE_MAX), row and column k in LDS
+__global__ void __launch_bounds__(1024) k_spd_inverse(int n, double* __restrict__ a) {
+    __shared__ double rowk[COARSE_MAX], colk[COARSE_MAX];
+    for (int k = 0; k < n; ++k) gj_step(n, k, a, rowk, colk);
+}
+
+// x = Ainv b on the coarsest level (dense matvec, one workgroup)
+template <class T>
+__global__ void __launch_bounds__(256) k_coarse_apply(int n, const double* __restrict__ Ainv, const T* __restrict__ b,
+                                                     T* __restrict__ x, const int* __restrict__ done) {
+    if (done && *done) return;
+    __shared__ double bs[COARSE_MAX];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) bs[i] = static_cast<double>(b[i]);
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double* row = Ainv + (int64_t)i * n;
+        double s = 0.0;
+        for (int j = 0; j < n; ++j) s += row[j] * bs[j];
+        x[i] = static_cast<T>(s);
     }
-    for (int j = n - 1; j >= 0; --j) {
-        if (threadIdx.x == 0) y[j] /= L[(int64_t)j * n + j];
-        __syncthreads();
-        const double yj = y[j];
-        for (int i = threadIdx.x; i < j; i += blockDim.x) y[i] -= L[(int64_t)j * n + i] * yj;
-        __syncthreads();
-    }
-    for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = static_cast<T>(y[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -263,8 +280,7 @@ template <class T>
 __global__ void __launch_bounds__(RED_BLOCK) k_power(int64_t n, const int32_t* __restrict__ sp,
                                                     const int32_t* __restrict__ sc, const T* __restrict__ sv,
                                                     const T* __restrict__ dinv, const T* __restrict__ v,
-                                                    T* __restrict__ w, double* __restrict__ partials,
-                                                    unsigned* __restrict__ counter, double* __restrict__ out) {
+                                                    T* __restrict__ w, double* __restrict__ partials) {
     double vav = 0.0, vdv = 0.0, ww = 0.0;
     for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
          row += (int64_t)gridDim.x * blockDim.x) {
@@ -278,16 +294,19 @@ __global__ void __launch_bounds__(RED_BLOCK) k_power(int64_t n, const int32_t* _
             ww += wi * wi;
         }
     }
-    double s[3] = {block_sum(vav), block_sum(vdv), block_sum(ww)};
-    if (publish_partials<3>(s, partials, counter)) {
-        const double a = sum_partials(partials, gridDim.x);
-        const double b = sum_partials(partials + gridDim.x, gridDim.x);
-        const double c = sum_partials(partials + 2 * gridDim.x, gridDim.x);
-        if (threadIdx.x == 0) {
-            out[0] = a / b;
-            out[1] = 1.0 / sqrt(c);
-            *counter = 0u;
-        }
+    const double s[3] = {block_sum(vav), block_sum(vdv), block_sum(ww)};
+    store_partials<3>(s, partials);
+}
+
+// out[0] = (v, A v) / (v, D v), out[1] = 1 / ||w||
+__global__ void __launch_bounds__(RED_BLOCK) k_power_finish(const double* __restrict__ partials, int nparts,
+                                                           double* __restrict__ out) {
+    const double a = finish_sum(partials, nparts);
+    const double b = finish_sum(partials + nparts, nparts);
+    const double c = finish_sum(partials + 2 * nparts, nparts);
+    if (threadIdx.x == 0) {
+        out[0] = a / b;
+        out[1] = 1.0 / sqrt(c);
     }
 }
 
@@ -311,22 +330,31 @@ __global__ void k_power_init(int64_t n, T* __restrict__ v) {
     }
 }
 
-// Gershgorin bound of D^-1 A: max_i sum_j |a_ij| / |a_ii|  (>= lambda_max)
-__global__ void __launch_bounds__(RED_BLOCK) k_gershgorin(int64_t n, const int32_t* __restrict__ ptr,
-                                                         const int32_t* __restrict__ idx,
-                                                         const double* __restrict__ val,
-                                                         double* __restrict__ partials,
-                                                         unsigned* __restrict__ counter, double* __restrict__ out) {
+// Jacobi data of a CSR FP64 operator, G lanes per row: dinv = 1/a_ii (1 where
+// the diagonal is absent or zero) and the Gershgorin bound of D^-1 A,
+// max_i sum_j |a_ij| / |a_ii|, which bounds its spectrum from above.
+template <class T, int G>
+__global__ void __launch_bounds__(RED_BLOCK) k_jacobi_setup(int64_t n, const int32_t* __restrict__ ptr,
+                                                           const int32_t* __restrict__ idx,
+                                                           const double* __restrict__ val, T* __restrict__ dinv,
+                                                           double* __restrict__ partials) {
     __shared__ double ws[RED_BLOCK / WAVE];
-    __shared__ bool last;
+    const int lane = threadIdx.x % G;
     double m = 0.0;
-    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t r = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G; r < n;
+         r += (int64_t)gridDim.x * blockDim.x / G) {
         double s = 0.0, d = 0.0;
-        for (int32_t k = ptr[r]; k < ptr[r + 1]; ++k) {
-            s += fabs(val[k]);
-            if (idx[k] == r) d = fabs(val[k]);
+        for (int32_t k = ptr[r] + lane; k < ptr[r + 1]; k += G) {
+            const double v = val[k];
+            s += fabs(v);
+            if (idx[k] == r) d = v;
         }
-        if (d > 0.0) m = fmax(m, s / d);
+        s = group_sum<G>(s);
+        d = group_sum<G>(d);
+        if (lane == 0) {
+            dinv[r] = static_cast<T>(d != 0.0 ? 1.0 / d : 1.0);
+            if (d != 0.0) m = fmax(m, s / fabs(d));
+        }
     }
     for (int o = WAVE / 2; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
     if ((threadIdx.x & (WAVE - 1)) == 0) ws[threadIdx.x / WAVE] = m;
@@ -335,21 +363,16 @@ __global__ void __launch_bounds__(RED_BLOCK) k_gershgorin(int64_t n, const int32
         double b = 0.0;
         for (int w = 0; w < RED_BLOCK / WAVE; ++w) b = fmax(b, ws[w]);
         partials[blockIdx.x] = b;
-        __threadfence();
-        last = atomicAdd(counter, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (last && threadIdx.x == 0) {
-        __threadfence();
-        double b = 0.0;
-        for (unsigned i = 0; i < gridDim.x; ++i)
-            b = fmax(b, __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        *out = b;
-        *counter = 0u;
     }
 }
 
-__global__ void k_zero_counter(unsigned* c) { *c = 0u; }
+__global__ void __launch_bounds__(WAVE) k_max_finish(const double* __restrict__ partials, int nparts,
+                                                    double* __restrict__ out) {
+    double m = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += WAVE) m = fmax(m, partials[i]);
+    for (int o = WAVE / 2; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if (threadIdx.x == 0) *out = m;
+}
 
 // ---------------------------------------------------------------------------
 // V-cycle driver
@@ -362,8 +385,8 @@ struct Cycle {
     static unsigned grid(int64_t n) { return grid_for(((n + 63) / 64) * 64, 256, 1 << 16); }
 
     int coarse(const fa_mg_level_t& L, const T* b, T* x) {
-        hipLaunchKernelGGL(k_chol_solve<T>, dim3(1), dim3(256), 0, s, (int)L.A.n, mg->coarse_chol, b, x, done);
-        FA_LAUNCH("k_chol_solve");
+        hipLaunchKernelGGL(k_coarse_apply<T>, dim3(1), dim3(256), 0, s, (int)L.A.n, mg->coarse_chol, b, x, done);
+        FA_LAUNCH("k_coarse_apply");
         return FA_OK;
     }
 
@@ -468,18 +491,41 @@ struct PcgState {
     double rz, pAp, alpha, beta, rnorm, bnorm, atol, rtol;
     long long niter, maxit;
     int done;  // 0 running, 1 converged, 2 ||b|| < 1e-15, 3 maxit, 4 breakdown
-    unsigned counter;
 };
 
-__device__ inline void pcg_check(PcgState* st, double rz_new) {
-    const double rn = sqrt(rz_new);
-    st->rnorm = rn;
-    st->niter += 1;
-    if (rn < st->atol || rn < st->rtol * st->bnorm) st->done = 1;
-    else if (st->niter >= st->maxit) st->done = 3;
-    else if (!isfinite(rn) || !isfinite(st->alpha)) st->done = 4;
-    st->beta = rz_new / st->rz;
-    st->rz = rz_new;
+// One PCG iteration = spmv(+p.Ap partials) | alpha | update | [cycle] |
+// r.z partials | check | p update.  The one-block *_finish kernels own
+// every write to the state; the grid kernels only read it.
+enum PcgFinish { PCG_INIT = 0, PCG_RZ0 = 1, PCG_ALPHA = 2, PCG_CHECK = 3 };
+
+template <int WHAT>
+__global__ void __launch_bounds__(RED_BLOCK) k_pcg_finish(const double* __restrict__ partials, int nparts,
+                                                         PcgState* st) {
+    if (WHAT != PCG_INIT && st->done) return;
+    const double t = finish_sum(partials, nparts);
+    double t2 = 0.0;
+    if (WHAT == PCG_INIT) t2 = finish_sum(partials + nparts, nparts);
+    if (threadIdx.x != 0) return;
+    if (WHAT == PCG_INIT) {
+        st->rz = t;  // replaced by r.z when preconditioned
+        st->bnorm = sqrt(t2);
+        st->niter = 0;
+        st->done = (st->bnorm < 1e-15) ? 2 : 0;
+    } else if (WHAT == PCG_RZ0) {
+        st->rz = t;
+    } else if (WHAT == PCG_ALPHA) {
+        st->pAp = t;
+        st->alpha = st->rz / t;
+    } else {
+        const double rn = sqrt(t);
+        st->rnorm = rn;
+        st->niter += 1;
+        if (rn < st->atol || rn < st->rtol * st->bnorm) st->done = 1;
+        else if (st->niter >= st->maxit) st->done = 3;
+        else if (!isfinite(rn) || !isfinite(st->alpha)) st->done = 4;
+        st->beta = t / st->rz;
+        st->rz = t;
+    }
 }
 
 template <class T, class U>
@@ -487,7 +533,7 @@ __global__ void __launch_bounds__(RED_BLOCK) k_pcg_init(int64_t n, const int32_t
                                                        const int32_t* __restrict__ sc, const T* __restrict__ sv,
                                                        const T* __restrict__ b, const T* __restrict__ x,
                                                        T* __restrict__ r, U* __restrict__ ru, T* __restrict__ p,
-                                                       double* __restrict__ partials, PcgState* st) {
+                                                       double* __restrict__ partials) {
     double rr = 0.0, bb = 0.0;
     for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
          row += (int64_t)gridDim.x * blockDim.x) {
@@ -502,18 +548,8 @@ __global__ void __launch_bounds__(RED_BLOCK) k_pcg_init(int64_t n, const int32_t
             bb += double(bv) * double(bv);
         }
     }
-    double v[2] = {block_sum(rr), block_sum(bb)};
-    if (publish_partials<2>(v, partials, &st->counter)) {
-        const double trr = sum_partials(partials, gridDim.x);
-        const double tbb = sum_partials(partials + gridDim.x, gridDim.x);
-        if (threadIdx.x == 0) {
-            st->rz = trr;  // overwritten by r.z when preconditioned
-            st->bnorm = sqrt(tbb);
-            st->niter = 0;
-            st->done = (st->bnorm < 1e-15) ? 2 : 0;
-            st->counter = 0u;
-        }
-    }
+    const double v[2] = {block_sum(rr), block_sum(bb)};
+    store_partials<2>(v, partials);
 }
 
 template <class T>
@@ -523,11 +559,11 @@ __global__ void k_pcg_trivial(int64_t n, T* __restrict__ x, const PcgState* st) 
         x[i] = T(0);
 }
 
-// rz = r.z (after the first preconditioner application), p = z
+// partials of r.z (after the first preconditioner application), p = z
 template <class T, class U>
 __global__ void __launch_bounds__(RED_BLOCK) k_pcg_rz0(int64_t n, const T* __restrict__ r, const U* __restrict__ z,
                                                       T* __restrict__ p, double* __restrict__ partials,
-                                                      PcgState* st) {
+                                                      const PcgState* st) {
     if (st->done) return;
     double acc = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -535,21 +571,15 @@ __global__ void __launch_bounds__(RED_BLOCK) k_pcg_rz0(int64_t n, const T* __res
         p[i] = zi;
         acc += double(r[i]) * double(zi);
     }
-    acc = block_sum(acc);
-    if (publish_partial(acc, partials, &st->counter)) {
-        const double t = sum_partials(partials, gridDim.x);
-        if (threadIdx.x == 0) {
-            st->rz = t;
-            st->counter = 0u;
-        }
-    }
+    store_partial(block_sum(acc), partials);
 }
 
+// Ap = A p and partials of p.Ap
 template <class T>
 __global__ void __launch_bounds__(RED_BLOCK) k_pcg_spmv(int64_t n, const int32_t* __restrict__ sp,
                                                        const int32_t* __restrict__ sc, const T* __restrict__ sv,
                                                        const T* __restrict__ p, T* __restrict__ Ap,
-                                                       double* __restrict__ partials, PcgState* st) {
+                                                       double* __restrict__ partials, const PcgState* st) {
     if (st->done) return;
     double acc = 0.0;
     for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
@@ -560,24 +590,16 @@ __global__ void __launch_bounds__(RED_BLOCK) k_pcg_spmv(int64_t n, const int32_t
             acc += double(p[row]) * double(v);
         }
     }
-    acc = block_sum(acc);
-    if (publish_partial(acc, partials, &st->counter)) {
-        const double t = sum_partials(partials, gridDim.x);
-        if (threadIdx.x == 0) {
-            st->pAp = t;
-            st->alpha = st->rz / t;
-            st->counter = 0u;
-        }
-    }
+    store_partial(block_sum(acc), partials);
 }
 
-// x += alpha p; r -= alpha Ap; then either r.r + stopping test (no
+// x += alpha p; r -= alpha Ap; then either partials of r.r (no
 // preconditioner) or the cast of r for the cycle
 template <class T, class U, bool PREC>
 __global__ void __launch_bounds__(RED_BLOCK) k_pcg_update(int64_t n, T* __restrict__ x, T* __restrict__ r,
                                                          U* __restrict__ ru, const T* __restrict__ p,
                                                          const T* __restrict__ Ap, double* __restrict__ partials,
-                                                         PcgState* st) {
+                                                         const PcgState* st) {
     if (st->done) return;
     const T alpha = static_cast<T>(st->alpha);
     double acc = 0.0;
@@ -588,32 +610,17 @@ __global__ void __launch_bounds__(RED_BLOCK) k_pcg_update(int64_t n, T* __restri
         if (PREC) ru[i] = static_cast<U>(rv);
         else acc += double(rv) * double(rv);
     }
-    if (PREC) return;
-    acc = block_sum(acc);
-    if (publish_partial(acc, partials, &st->counter)) {
-        const double t = sum_partials(partials, gridDim.x);
-        if (threadIdx.x == 0) {
-            pcg_check(st, t);
-            st->counter = 0u;
-        }
-    }
+    if (!PREC) store_partial(block_sum(acc), partials);
 }
 
 template <class T, class U>
 __global__ void __launch_bounds__(RED_BLOCK) k_pcg_rz(int64_t n, const T* __restrict__ r, const U* __restrict__ z,
-                                                     double* __restrict__ partials, PcgState* st) {
+                                                     double* __restrict__ partials, const PcgState* st) {
     if (st->done) return;
     double acc = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         acc += double(r[i]) * double(z[i]);
-    acc = block_sum(acc);
-    if (publish_partial(acc, partials, &st->counter)) {
-        const double t = sum_partials(partials, gridDim.x);
-        if (threadIdx.x == 0) {
-            pcg_check(st, t);
-            st->counter = 0u;
-        }
-    }
+    store_partial(block_sum(acc), partials);
 }
 
 template <class T, class U>
@@ -628,7 +635,6 @@ __global__ void k_pcg_set(PcgState* st, double atol, double rtol, long long maxi
     st->atol = atol;
     st->rtol = rtol;
     st->maxit = maxit;
-    st->counter = 0u;
     st->done = 0;
     st->niter = 0;
 }
@@ -651,18 +657,22 @@ static int pcg_run(const fa_sell_t* A, const fa_mg_t* M, const T* b, T* x, doubl
         return FA_ENOMEM;
     }
     const int* done = &st->done;
-    const dim3 grid(RED_GRID), block(RED_BLOCK);
+    const unsigned g = red_grid(n);
+    const int np = static_cast<int>(g);
+    const dim3 grid(g), block(RED_BLOCK), one(1);
     const T* sv = static_cast<const T*>(A->vals);
     const int64_t cap = maxit > 0 ? maxit : (1LL << 62);
-    hipLaunchKernelGGL(k_pcg_set, dim3(1), dim3(1), 0, s, st, atol, rtol, (long long)cap);
+    hipLaunchKernelGGL(k_pcg_set, one, one, 0, s, st, atol, rtol, (long long)cap);
     hipLaunchKernelGGL((k_pcg_init<T, U>), grid, block, 0, s, n, A->slice_ptr, A->cols, sv, b, x, r, ru, p,
-                       partials, st);
+                       partials);
+    hipLaunchKernelGGL(k_pcg_finish<PCG_INIT>, one, block, 0, s, partials, np, st);
     hipLaunchKernelGGL(k_pcg_trivial<T>, grid, block, 0, s, n, x, st);
     FA_LAUNCH("k_pcg_init");
     if (M) {
         int e = vcycle_any(M, ru, z, done, s);
         if (e) return e;
         hipLaunchKernelGGL((k_pcg_rz0<T, U>), grid, block, 0, s, n, r, z, p, partials, st);
+        hipLaunchKernelGGL(k_pcg_finish<PCG_RZ0>, one, block, 0, s, partials, np, st);
         FA_LAUNCH("k_pcg_rz0");
     }
     PcgState h{};
@@ -671,15 +681,18 @@ static int pcg_run(const fa_sell_t* A, const fa_mg_t* M, const T* b, T* x, doubl
         const int64_t chunk = std::min<int64_t>(check_every, cap - it);
         for (int64_t k = 0; k < chunk; ++k) {
             hipLaunchKernelGGL(k_pcg_spmv<T>, grid, block, 0, s, n, A->slice_ptr, A->cols, sv, p, Ap, partials, st);
+            hipLaunchKernelGGL(k_pcg_finish<PCG_ALPHA>, one, block, 0, s, partials, np, st);
             if (M) {
                 hipLaunchKernelGGL((k_pcg_update<T, U, true>), grid, block, 0, s, n, x, r, ru, p, Ap, partials, st);
                 int e = vcycle_any(M, ru, z, done, s);
                 if (e) return e;
                 hipLaunchKernelGGL((k_pcg_rz<T, U>), grid, block, 0, s, n, r, z, partials, st);
+                hipLaunchKernelGGL(k_pcg_finish<PCG_CHECK>, one, block, 0, s, partials, np, st);
                 hipLaunchKernelGGL((k_pcg_pupdate<T, U>), grid, block, 0, s, n, p, z, st);
             } else {
                 hipLaunchKernelGGL((k_pcg_update<T, T, false>), grid, block, 0, s, n, x, r, (T*)nullptr, p, Ap,
                                    partials, st);
+                hipLaunchKernelGGL(k_pcg_finish<PCG_CHECK>, one, block, 0, s, partials, np, st);
                 hipLaunchKernelGGL((k_pcg_pupdate<T, T>), grid, block, 0, s, n, p, r, st);
             }
         }
@@ -799,10 +812,14 @@ extern "C" int fa_csr_to_dense(int64_t n, const int32_t* indptr, const int32_t* 
     return FA_OK;
 }
 
-extern "C" int fa_dense_cholesky(int64_t n, double* a, fa_stream_t stream) {
-    FA_REQUIRE(n > 0 && n <= COARSE_MAX, "fa_dense_cholesky: n = %lld outside 1..%d", (long long)n, COARSE_MAX);
-    hipLaunchKernelGGL(k_cholesky, dim3(1), dim3(256), 0, S(stream), (int)n, a);
-    FA_LAUNCH("k_cholesky");
+extern "C" int fa_dense_spd_inverse(int64_t n, double* a, fa_stream_t stream) {
+    FA_REQUIRE(n > 0 && n <= COARSE_MAX, "fa_dense_spd_inverse: n = %lld outside 1..%d", (long long)n, COARSE_MAX);
+    if (n <= COARSE_LDS_MAX)
+        hipLaunchKernelGGL(k_spd_inverse_lds, dim3(1), dim3(256), (n * n + 2 * n) * sizeof(double), S(stream), (int)n,
+                           a);
+    else
+        hipLaunchKernelGGL(k_spd_inverse, dim3(1), dim3(1024), 0, S(stream), (int)n, a);
+    FA_LAUNCH("k_spd_inverse");
     return FA_OK;
 }
 
@@ -813,14 +830,13 @@ extern "C" int fa_mg_lmax(const fa_mg_t* mg, int level, int iters, double* lmax_
     const fa_mg_level_t& L = mg->levels[level];
     Arena ar(work, work_bytes);
     double* partials = ar.take<double>(3 * RED_GRID);
-    unsigned* counter = ar.take<unsigned>(1);
     double* out = ar.take<double>(2);
-    if (!partials || !counter || !out) {
+    if (!partials || !out) {
         set_error("fa_mg_lmax: workspace too small");
         return FA_ENOMEM;
     }
-    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, counter);
     const unsigned g = grid_for(L.A.n, 256, 1 << 16);
+    const unsigned gr = red_grid(L.A.n);
     double lam = 0.0;
 #define FA_PW(T)                                                                                                   \
     {                                                                                                              \
@@ -828,9 +844,9 @@ extern "C" int fa_mg_lmax(const fa_mg_t* mg, int level, int iters, double* lmax_
         T* w = static_cast<T*>(L.x2);                                                                              \
         hipLaunchKernelGGL(k_power_init<T>, dim3(g), dim3(256), 0, s, L.A.n, v);                                  \
         for (int k = 0; k < iters; ++k) {                                                                          \
-            hipLaunchKernelGGL(k_power<T>, dim3(RED_GRID), dim3(RED_BLOCK), 0, s, L.A.n, L.A.slice_ptr, L.A.cols, \
-                               static_cast<const T*>(L.A.vals), static_cast<const T*>(L.dinv), v, w, partials,    \
-                               counter, out);                                                                      \
+            hipLaunchKernelGGL(k_power<T>, dim3(gr), dim3(RED_BLOCK), 0, s, L.A.n, L.A.slice_ptr, L.A.cols,        \
+                               static_cast<const T*>(L.A.vals), static_cast<const T*>(L.dinv), v, w, partials);   \
+            hipLaunchKernelGGL(k_power_finish, dim3(1), dim3(RED_BLOCK), 0, s, partials, (int)gr, out);           \
             hipLaunchKernelGGL(k_scale<T>, dim3(g), dim3(256), 0, s, L.A.n, w, v, out);                           \
         }                                                                                                          \
     }
@@ -843,22 +859,31 @@ extern "C" int fa_mg_lmax(const fa_mg_t* mg, int level, int iters, double* lmax_
     return FA_OK;
 }
 
-extern "C" int fa_gershgorin(int64_t n, const int32_t* indptr, const int32_t* indices, const double* values,
-                             double* bound_host, void* work, size_t work_bytes, fa_stream_t stream) {
-    FA_REQUIRE(n > 0, "fa_gershgorin: empty matrix");
+extern "C" int fa_jacobi_setup(int dtype_out, int64_t n, int64_t nnz, const int32_t* indptr, const int32_t* indices,
+                               const double* values, void* dinv, double* bound_host, void* work, size_t work_bytes,
+                               fa_stream_t stream) {
+    FA_REQUIRE(n > 0, "fa_jacobi_setup: empty matrix");
     hipStream_t s = S(stream);
     Arena ar(work, work_bytes);
     double* partials = ar.take<double>(RED_GRID);
-    unsigned* counter = ar.take<unsigned>(1);
     double* out = ar.take<double>(1);
-    if (!partials || !counter || !out) {
-        set_error("fa_gershgorin: workspace too small");
+    if (!partials || !out) {
+        set_error("fa_jacobi_setup: workspace too small");
         return FA_ENOMEM;
     }
-    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, s, counter);
-    hipLaunchKernelGGL(k_gershgorin, dim3(RED_GRID), dim3(RED_BLOCK), 0, s, n, indptr, indices, values, partials,
-                       counter, out);
-    FA_LAUNCH("k_gershgorin");
+    const double avg = double(nnz) / double(n);
+    const unsigned grid = grid_for(n * 16, RED_BLOCK, RED_GRID);
+#define FA_JS(T, G)                                                                                               \
+    hipLaunchKernelGGL((k_jacobi_setup<T, G>), dim3(grid), dim3(RED_BLOCK), 0, s, n, indptr, indices, values,   \
+                       static_cast<T*>(dinv), partials)
+    if (dtype_out == FA_F64) {
+        if (avg <= 12) FA_JS(double, 8); else if (avg <= 24) FA_JS(double, 16); else FA_JS(double, 32);
+    } else {
+        if (avg <= 12) FA_JS(float, 8); else if (avg <= 24) FA_JS(float, 16); else FA_JS(float, 32);
+    }
+#undef FA_JS
+    hipLaunchKernelGGL(k_max_finish, dim3(1), dim3(WAVE), 0, s, partials, (int)grid, out);
+    FA_LAUNCH("k_jacobi_setup");
     double h = 0.0;
     FA_HIP(hipMemcpyAsync(&h, out, sizeof(double), hipMemcpyDeviceToHost, s));
     FA_HIP(hipStreamSynchronize(s));
@@ -868,7 +893,7 @@ extern "C" int fa_gershgorin(int64_t n, const int32_t* indptr, const int32_t* in
 
 extern "C" int fa_mg_vcycle(const fa_mg_t* mg, const void* r, void* z, fa_stream_t stream) {
     FA_REQUIRE(mg && mg->nlevels >= 1 && mg->levels, "fa_mg_vcycle: empty hierarchy");
-    FA_REQUIRE(mg->coarse_chol, "fa_mg_vcycle: the coarsest level needs a Cholesky factor");
+    FA_REQUIRE(mg->coarse_chol, "fa_mg_vcycle: the coarsest level needs its inverse");
     FA_REQUIRE(mg->levels[mg->nlevels - 1].A.n <= COARSE_MAX, "fa_mg_vcycle: coarsest level too large");
     return vcycle_any(mg, r, z, nullptr, S(stream));
 }

@@ -1,12 +1,18 @@
-// Deterministic block / grid reductions: fixed grid, fixed tree order, the
-// last block to arrive sums the per-block partials in index order.
+// Deterministic block / grid reductions: a grid that depends on n only,
+// fixed tree order inside a block, per-block partials summed in index order.
 #pragma once
 #include "common.hpp"
 
 namespace fa {
 
 constexpr int RED_BLOCK = 256;
-constexpr int RED_GRID = 1024;
+constexpr int RED_GRID = 8192;  // upper bound of a reduction grid (partials capacity)
+
+// reduction grid for n items: depends on n only, so the summation order is fixed
+inline unsigned red_grid(int64_t n) {
+    int64_t g = (n + RED_BLOCK - 1) / RED_BLOCK;
+    return static_cast<unsigned>(g < 1 ? 1 : (g > RED_GRID ? RED_GRID : g));
+}
 
 __device__ inline double block_sum(double v) {
     __shared__ double ws[RED_BLOCK / WAVE];
@@ -23,33 +29,28 @@ __device__ inline double block_sum(double v) {
     return t;  // valid in thread 0
 }
 
-// Thread 0 stores this block's partial(s); every thread of the last block to
-// arrive gets true.  `k` partial arrays of length gridDim.x are laid out
-// back to back.
+// Thread 0 stores the block's partial sums; partial array i (length
+// gridDim.x) starts at partials + i * gridDim.x.  A one-block finishing kernel
+// launched next on the same stream adds them in index order (finish_sum).
+// Two kernels instead of a last-block-arrives counter: a device-scope atomic
+// per workgroup on one address serialises across the XCDs and cost more than
+// the reduction itself at thousands of workgroups.
 template <int K>
-__device__ inline bool publish_partials(const double (&v)[K], double* partials, unsigned* counter) {
-    __shared__ bool last;
+__device__ inline void store_partials(const double (&v)[K], double* partials) {
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int i = 0; i < K; ++i) partials[i * gridDim.x + blockIdx.x] = v[i];
-        __threadfence();
-        unsigned prev = atomicAdd(counter, 1u);
-        last = (prev == gridDim.x - 1);
     }
-    __syncthreads();
-    return last;
 }
 
-__device__ inline bool publish_partial(double v, double* partials, unsigned* counter) {
-    const double a[1] = {v};
-    return publish_partials<1>(a, partials, counter);
+__device__ inline void store_partial(double v, double* partials) {
+    if (threadIdx.x == 0) partials[blockIdx.x] = v;
 }
 
-__device__ inline double sum_partials(double* partials, int n) {
-    __threadfence();
+// sum of partials[0..n) in a fixed order by one block; valid in thread 0
+__device__ inline double finish_sum(const double* partials, int n) {
     double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x)
-        s += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += partials[i];
     return block_sum(s);
 }
 

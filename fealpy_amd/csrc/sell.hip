@@ -1,4 +1,4 @@
-// SELL-64 construction from CSR, SELL SpMV and Jacobi diagonal.
+// SELL-64 construction from CSR and SELL SpMV.
 #include "common.hpp"
 #include "sell.hpp"
 
@@ -50,17 +50,6 @@ __global__ void __launch_bounds__(256) k_sell_spmv(int64_t n, const int32_t* __r
          row += (int64_t)gridDim.x * blockDim.x) {
         const T v = sell_row_dot<T, T>(slice_ptr, cols, vals, row, x);
         if (row < n) y[row] = v;
-    }
-}
-
-template <class A, class B>
-__global__ void k_diag_inv(int64_t n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices,
-                           const A* __restrict__ vals, B* __restrict__ dinv) {
-    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
-        double d = 0.0;
-        for (int32_t k = indptr[r]; k < indptr[r + 1]; ++k)
-            if (indices[k] == r) d = static_cast<double>(vals[k]);
-        dinv[r] = static_cast<B>(d != 0.0 ? 1.0 / d : 1.0);
     }
 }
 
@@ -139,23 +128,3 @@ extern "C" int fa_sell_spmv(int dtype, const fa_sell_t* A, const void* x, void* 
     return FA_OK;
 }
 
-extern "C" int fa_diag_inv(int dtype_in, int dtype_out, int64_t n, const int32_t* indptr, const int32_t* indices,
-                           const void* values, void* dinv, fa_stream_t stream) {
-    hipStream_t s = S(stream);
-    const unsigned grid = grid_for(n, 256, 1 << 16);
-    if (dtype_in == FA_F64 && dtype_out == FA_F64)
-        hipLaunchKernelGGL((k_diag_inv<double, double>), dim3(grid), dim3(256), 0, s, n, indptr, indices,
-                           static_cast<const double*>(values), static_cast<double*>(dinv));
-    else if (dtype_in == FA_F64 && dtype_out == FA_F32)
-        hipLaunchKernelGGL((k_diag_inv<double, float>), dim3(grid), dim3(256), 0, s, n, indptr, indices,
-                           static_cast<const double*>(values), static_cast<float*>(dinv));
-    else if (dtype_in == FA_F32 && dtype_out == FA_F32)
-        hipLaunchKernelGGL((k_diag_inv<float, float>), dim3(grid), dim3(256), 0, s, n, indptr, indices,
-                           static_cast<const float*>(values), static_cast<float*>(dinv));
-    else {
-        set_error("fa_diag_inv: bad dtypes");
-        return FA_EINVAL;
-    }
-    FA_LAUNCH("k_diag_inv");
-    return FA_OK;
-}

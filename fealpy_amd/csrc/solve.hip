@@ -47,73 +47,90 @@ static int lanes_for(int64_t n, int64_t nnz) {
     return 64;
 }
 
-// ---- Dirichlet BC -------------------------------------------------------------
-__global__ void k_bc_count(int64_t n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices,
-                           const uint8_t* __restrict__ bd, int32_t* __restrict__ cnt) {
-    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
-        if (bd[r]) {
-            cnt[r] = 1;
-            continue;
-        }
-        int32_t c = 0;
-        for (int32_t k = indptr[r]; k < indptr[r + 1]; ++k) c += bd[indices[k]] ? 0 : 1;
-        cnt[r] = c;
+// ---- Dirichlet BC (G lanes per row) ------------------------------------------------
+constexpr int BCG = 16;
+
+__global__ void __launch_bounds__(256) k_bc_count(int64_t n, const int32_t* __restrict__ indptr,
+                                                 const int32_t* __restrict__ indices, const uint8_t* __restrict__ bd,
+                                                 int32_t* __restrict__ cnt) {
+    const int lane = threadIdx.x % BCG;
+    for (int64_t r = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / BCG; r < n;
+         r += (int64_t)gridDim.x * blockDim.x / BCG) {
+        int c = 0;
+        if (!bd[r])
+            for (int32_t k = indptr[r] + lane; k < indptr[r + 1]; k += BCG) c += bd[indices[k]] ? 0 : 1;
+        c = group_sum<BCG>(c);
+        if (lane == 0) cnt[r] = bd[r] ? 1 : c;
     }
 }
 
+// kept entries keep their order; nidx == NULL refills values only
 template <class T>
-__global__ void k_bc_fill(int64_t n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices,
-                          const T* __restrict__ vals, const uint8_t* __restrict__ bd,
-                          const int32_t* __restrict__ nptr, int32_t* __restrict__ nidx, T* __restrict__ nval) {
-    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
-        int32_t o = nptr[r];
+__global__ void __launch_bounds__(256) k_bc_fill(int64_t n, const int32_t* __restrict__ indptr,
+                                                const int32_t* __restrict__ indices, const T* __restrict__ vals,
+                                                const uint8_t* __restrict__ bd, const int32_t* __restrict__ nptr,
+                                                int32_t* __restrict__ nidx, T* __restrict__ nval) {
+    const int lane = threadIdx.x % BCG;
+    for (int64_t r = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / BCG; r < n;
+         r += (int64_t)gridDim.x * blockDim.x / BCG) {
+        const int32_t o0 = nptr[r];
         if (bd[r]) {
-            if (nidx) nidx[o] = static_cast<int32_t>(r);
-            nval[o] = T(1);
-            continue;
-        }
-        for (int32_t k = indptr[r]; k < indptr[r + 1]; ++k) {
-            const int32_t c = indices[k];
-            if (!bd[c]) {
-                if (nidx) nidx[o] = c;
-                nval[o] = vals[k];
-                ++o;
+            if (lane == 0) {
+                if (nidx) nidx[o0] = static_cast<int32_t>(r);
+                nval[o0] = T(1);
             }
+            continue;
+        }
+        int32_t base = o0;
+        const int32_t k1 = indptr[r + 1];
+        for (int32_t k0 = indptr[r]; k0 < k1; k0 += BCG) {
+            const int32_t k = k0 + lane;
+            int32_t c = 0;
+            bool keep = false;
+            if (k < k1) {
+                c = indices[k];
+                keep = !bd[c];
+            }
+            const int inc = group_inclusive_scan<BCG>(keep ? 1 : 0);
+            if (keep) {
+                if (nidx) nidx[base + inc - 1] = c;
+                nval[base + inc - 1] = vals[k];
+            }
+            base += __shfl(inc, BCG - 1, BCG);  // group total
         }
     }
 }
 
 template <class T>
-__global__ void k_bc_rhs(int64_t n, const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices,
-                         const T* __restrict__ vals, const uint8_t* __restrict__ bd, const T* __restrict__ uh,
-                         const T* __restrict__ f, T* __restrict__ out) {
-    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
-        if (bd[r]) {
-            out[r] = uh[r];
-            continue;
-        }
+__global__ void __launch_bounds__(256) k_bc_rhs(int64_t n, const int32_t* __restrict__ indptr,
+                                               const int32_t* __restrict__ indices, const T* __restrict__ vals,
+                                               const uint8_t* __restrict__ bd, const T* __restrict__ uh,
+                                               const T* __restrict__ f, T* __restrict__ out) {
+    const int lane = threadIdx.x % BCG;
+    for (int64_t r = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / BCG; r < n;
+         r += (int64_t)gridDim.x * blockDim.x / BCG) {
         T s = T(0);
-        for (int32_t k = indptr[r]; k < indptr[r + 1]; ++k) s += vals[k] * uh[indices[k]];
-        out[r] = f[r] - s;
+        if (!bd[r])
+            for (int32_t k = indptr[r] + lane; k < indptr[r + 1]; k += BCG) s += vals[k] * uh[indices[k]];
+        s = group_sum<BCG>(s);
+        if (lane == 0) out[r] = bd[r] ? uh[r] : f[r] - s;
     }
 }
 
 // ---- vector kernels --------------------------------------------------------------
 template <class T>
 __global__ void __launch_bounds__(RED_BLOCK) k_dot(int64_t n, const T* __restrict__ x, const T* __restrict__ y,
-                                                  double* __restrict__ partials, unsigned* __restrict__ counter,
-                                                  double* __restrict__ out) {
+                                                  double* __restrict__ partials) {
     double s = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         s += double(x[i]) * double(y[i]);
-    s = block_sum(s);
-    if (publish_partial(s, partials, counter)) {
-        double t = sum_partials(partials, gridDim.x);
-        if (threadIdx.x == 0) {
-            *out = t;
-            *counter = 0u;
-        }
-    }
+    store_partial(block_sum(s), partials);
+}
+
+__global__ void __launch_bounds__(RED_BLOCK) k_dot_finish(const double* __restrict__ partials, int nparts,
+                                                         double* __restrict__ out) {
+    const double t = finish_sum(partials, nparts);
+    if (threadIdx.x == 0) *out = t;
 }
 
 template <class T>
@@ -126,10 +143,6 @@ template <class A, class B>
 __global__ void k_cast(int64_t n, const A* __restrict__ x, B* __restrict__ y) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         y[i] = static_cast<B>(x[i]);
-}
-
-__global__ void k_zero_u32(unsigned* p, int n) {
-    if (threadIdx.x < n) p[threadIdx.x] = 0u;
 }
 
 }  // namespace fa
@@ -165,7 +178,8 @@ extern "C" int fa_dirichlet_count(int64_t n, const int32_t* indptr, const int32_
         set_error("fa_dirichlet_count: workspace too small");
         return FA_ENOMEM;
     }
-    hipLaunchKernelGGL(k_bc_count, dim3(grid_for(n, 256, 1 << 16)), dim3(256), 0, s, n, indptr, indices, is_bd, cnt);
+    hipLaunchKernelGGL(k_bc_count, dim3(grid_for(n * BCG, 256, 1 << 16)), dim3(256), 0, s, n, indptr, indices, is_bd,
+                       cnt);
     FA_LAUNCH("k_bc_count");
     int st = exclusive_scan_i32(cnt, new_indptr, n, sw, sb, s);
     if (st != FA_OK) return st;
@@ -181,7 +195,7 @@ extern "C" int fa_dirichlet_fill(int dtype, int64_t n, const int32_t* indptr, co
                                  int32_t* new_indices, void* new_values, fa_stream_t stream) {
     FA_REQUIRE(dtype == FA_F64 || dtype == FA_F32, "fa_dirichlet_fill: bad dtype %d", dtype);
     hipStream_t s = S(stream);
-    const unsigned grid = grid_for(n, 256, 1 << 16);
+    const unsigned grid = grid_for(n * BCG, 256, 1 << 16);
     if (dtype == FA_F64)
         hipLaunchKernelGGL(k_bc_fill<double>, dim3(grid), dim3(256), 0, s, n, indptr, indices,
                            static_cast<const double*>(values), is_bd, new_indptr, new_indices,
@@ -199,7 +213,7 @@ extern "C" int fa_dirichlet_rhs(int dtype, int64_t n, const int32_t* indptr, con
                                 fa_stream_t stream) {
     FA_REQUIRE(dtype == FA_F64 || dtype == FA_F32, "fa_dirichlet_rhs: bad dtype %d", dtype);
     hipStream_t s = S(stream);
-    const unsigned grid = grid_for(n, 256, 1 << 16);
+    const unsigned grid = grid_for(n * BCG, 256, 1 << 16);
     if (dtype == FA_F64)
         hipLaunchKernelGGL(k_bc_rhs<double>, dim3(grid), dim3(256), 0, s, n, indptr, indices,
                            static_cast<const double*>(values), is_bd, static_cast<const double*>(uh),
@@ -212,26 +226,26 @@ extern "C" int fa_dirichlet_rhs(int dtype, int64_t n, const int32_t* indptr, con
     return FA_OK;
 }
 
-extern "C" size_t fa_reduce_work_bytes(void) { return align_up(RED_GRID * sizeof(double)) + 256; }
+extern "C" size_t fa_reduce_work_bytes(void) { return 3 * align_up(RED_GRID * sizeof(double)) + 1024; }
 
 extern "C" int fa_dot(int dtype, int64_t n, const void* x, const void* y, double* out, void* work,
                       size_t work_bytes, fa_stream_t stream) {
     FA_REQUIRE(dtype == FA_F64 || dtype == FA_F32, "fa_dot: bad dtype %d", dtype);
     Arena ar(work, work_bytes);
     double* partials = ar.take<double>(RED_GRID);
-    unsigned* counter = ar.take<unsigned>(1);
-    if (!partials || !counter) {
+    if (!partials) {
         set_error("fa_dot: workspace too small");
         return FA_ENOMEM;
     }
     hipStream_t s = S(stream);
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, counter, 1);
+    const unsigned g = red_grid(n);
     if (dtype == FA_F64)
-        hipLaunchKernelGGL(k_dot<double>, dim3(RED_GRID), dim3(RED_BLOCK), 0, s, n, static_cast<const double*>(x),
-                           static_cast<const double*>(y), partials, counter, out);
+        hipLaunchKernelGGL(k_dot<double>, dim3(g), dim3(RED_BLOCK), 0, s, n, static_cast<const double*>(x),
+                           static_cast<const double*>(y), partials);
     else
-        hipLaunchKernelGGL(k_dot<float>, dim3(RED_GRID), dim3(RED_BLOCK), 0, s, n, static_cast<const float*>(x),
-                           static_cast<const float*>(y), partials, counter, out);
+        hipLaunchKernelGGL(k_dot<float>, dim3(g), dim3(RED_BLOCK), 0, s, n, static_cast<const float*>(x),
+                           static_cast<const float*>(y), partials);
+    hipLaunchKernelGGL(k_dot_finish, dim3(1), dim3(RED_BLOCK), 0, s, partials, (int)g, out);
     FA_LAUNCH("k_dot");
     return FA_OK;
 }

@@ -17,10 +17,21 @@ class DirichletBC:
         self.gdof = space.number_of_global_dofs()
         if torch.is_tensor(threshold):
             self.is_boundary_dof = threshold
+            self._bd_u8 = threshold.to(torch.uint8).contiguous()
+        elif threshold is None and hasattr(space, '_boundary'):
+            # boundary flags depend only on the mesh: shared by every BC of the space
+            flag = space._boundary()[0]
+            self.is_boundary_dof = flag
+            if getattr(space, '_bd_u8', None) is None:
+                space._bd_u8 = flag.to(torch.uint8).contiguous()
+            self._bd_u8 = space._bd_u8
         else:
             self.is_boundary_dof = space.is_boundary_dof(threshold=threshold, method=method)
-        self.boundary_dof_index = torch.nonzero(self.is_boundary_dof)[:, 0]
-        self._bd_u8 = self.is_boundary_dof.to(torch.uint8).contiguous()
+            self._bd_u8 = self.is_boundary_dof.to(torch.uint8).contiguous()
+
+    @property
+    def boundary_dof_index(self):
+        return torch.nonzero(self.is_boundary_dof)[:, 0]
 
     def check_matrix(self, matrix):
         if not isinstance(matrix, (COOTensor, CSRTensor)):
@@ -74,7 +85,7 @@ class DirichletBC:
         and reused by later assemblies (fresh=False: indices already written)."""
         cache = self.space.__dict__.setdefault('_bc_patterns', [])
         for (crow, col, bd), val in cache:
-            if crow is A.crow and col is A.col and torch.equal(bd, self._bd_u8):
+            if crow is A.crow and col is A.col and (bd is self._bd_u8 or torch.equal(bd, self._bd_u8)):
                 return val + (False,)
         L = nat.lib()
         n = A.shape[0]

@@ -65,6 +65,7 @@ class LagrangeFESpace:
         self._c2d = None
         self._tables = {}
         self._pattern = None
+        self._bd = None
 
     def __str__(self):
         return "Lagrange finite element space on linear mesh!"
@@ -95,9 +96,21 @@ class LagrangeFESpace:
     def top_dimension(self):
         return self.TD
 
+    def _boundary(self):
+        """(flags, indices, interpolation points) of the DOFs on boundary faces,
+        computed once per space: Dirichlet data is re-imposed on every assembly
+        and these depend only on the mesh."""
+        if self._bd is None:
+            flag = self.mesh.is_boundary_dof(self.p)
+            idx = torch.nonzero(flag)[:, 0]
+            self._bd = (flag, idx, self.interpolation_points()[idx].contiguous())
+        return self._bd
+
     def is_boundary_dof(self, threshold=None, method=None):
         """DOFs on boundary faces (fealpy/functionspace/dofs.py:23-55)."""
         gdof = self.number_of_global_dofs()
+        if threshold is None and self._bd is not None:
+            return self._bd[0].clone()
         if torch.is_tensor(threshold):
             if threshold.dtype == torch.bool and threshold.shape[0] == gdof:
                 return threshold
@@ -121,10 +134,21 @@ class LagrangeFESpace:
     def boundary_interpolate(self, gd, uh=None, *, threshold=None, method=None):
         """uh[bd] = gd(interpolation points on the boundary)
         (lagrange_fe_space.py:109-142).  Returns (uh, isDDof)."""
-        isDDof = self.is_boundary_dof(threshold=threshold, method='interp')
         if uh is None:
             uh = torch.zeros(self.number_of_global_dofs(), dtype=self.ftype, device=self.device)
         arr = uh.array if isinstance(uh, Function) else uh
+        if threshold is None:
+            isDDof, idx, pts = self._boundary()
+            if torch.is_tensor(gd):
+                arr[idx] = gd[idx].to(arr.dtype)
+            elif callable(gd):
+                arr[idx] = torch.as_tensor(gd(pts), dtype=arr.dtype, device=arr.device)
+            elif isinstance(gd, (int, float)):
+                arr[idx] = gd
+            else:
+                raise TypeError("gd must be a tensor or a callable function")
+            return (uh if isinstance(uh, Function) else Function(self, arr)), isDDof.clone()
+        isDDof = self.is_boundary_dof(threshold=threshold, method='interp')
         if torch.is_tensor(gd):
             arr[isDDof] = gd[isDDof].to(arr.dtype)
         elif callable(gd):

@@ -189,9 +189,20 @@ class SimplexMesh:
         return torch.einsum('cbm, qjb -> cqjm', gl, R)
 
     def bc_to_point(self, bcs, index=_S):
+        """(NC, NQ, GD) physical points of barycentric points bcs (NQ, TD+1);
+        on the GPU through fa_bc_to_point."""
         bcs = torch.as_tensor(bcs, dtype=self.ftype, device=self.device)
-        pts = self.node[self.cell[index].long()]                 # (NC, TD+1, GD)
-        return torch.einsum('cjk, qj -> cqk', pts, bcs)
+        cell = self.cell[index]
+        if not self.node.is_cuda:
+            return torch.einsum('cjk, qj -> cqk', self.node[cell.long()], bcs)
+        from .. import _native as nat
+        cell = cell.to(torch.int32).contiguous()
+        nc, nv = cell.shape
+        nq = bcs.shape[0]
+        out = torch.empty((nc, nq, self.GD), dtype=torch.float64, device=self.device)
+        nat.check(nat.lib().fa_bc_to_point(self.GD, nv, nc, nq, nat.ptr(self.node.contiguous()), nat.ptr(cell),
+                                           nat.ptr(bcs.contiguous()), nat.ptr(out), nat.stream()), "fa_bc_to_point")
+        return out
 
     def entity_barycenter(self, etype='cell', index=_S):
         e = self.entity(etype, index)

@@ -16,7 +16,7 @@ the preconditioner of CG.  Designed for the GPU:
   [lower, upper] * the Gershgorin bound of D^-1 A -- an upper bound of the
   spectrum, so no mode is amplified), the same polynomial
   before and after the coarse correction, so the V-cycle is symmetric and
-  CG-compatible; the coarsest level is solved by a dense Cholesky factor;
+  CG-compatible; the coarsest level is solved with its dense inverse;
 * the cycle runs in ``dtype`` (FP32 for the mixed-precision solver: FP64 CG
   with an FP32 V-cycle).
 """
@@ -111,7 +111,7 @@ class _Level:
 
 
 class GeometricMG:
-    def __init__(self, space, is_bd, *, dtype=torch.float32, degree=2, lower=0.1, upper=1.0, coarse_max=512,
+    def __init__(self, space, is_bd, *, dtype=torch.float32, degree=2, lower=0.1, upper=1.0, coarse_max=64,
                  power_iters=12):
         if space.p != 1:
             raise NotImplementedError("GeometricMG covers P1 spaces")
@@ -143,8 +143,8 @@ class GeometricMG:
             nxt.dims, nxt.n, nxt.bd = dims_c, nc, bd_c
             self.levels.append(nxt)
             lev, dims, bd = nxt, dims_c, bd_c
-        if self.levels[-1].n > 4096:
-            raise ValueError(f"coarsest level has {self.levels[-1].n} unknowns; the dense solver holds 4096")
+        if self.levels[-1].n > 2048:
+            raise ValueError(f"coarsest level has {self.levels[-1].n} unknowns; the dense coarse solver holds 2048")
         for lv in self.levels:
             z = lambda: torch.zeros(lv.n, dtype=dtype, device=dev)
             lv.b, lv.x, lv.x2, lv.d, lv.r, lv.dinv = z(), z(), z(), z(), z(), z()
@@ -196,30 +196,26 @@ class GeometricMG:
                                       nat.ptr(f.P.col), nat.ptr(f.P.values), nat.ptr(c.A.crow), nat.ptr(c.A.col),
                                       c.max_row, nat.ptr(c.A.values), nat.stream()), "fa_rap_values")
         code = nat.dtype_code(self.dtype)
+        ws = nat.workspace(L.fa_reduce_work_bytes(), self.device)
         for lv in self.levels:
             lv.As.set_values(lv.A.values, self.dtype)
-            nat.check(L.fa_diag_inv(nat.FA_F64, code, lv.n, nat.ptr(lv.A.crow), nat.ptr(lv.A.col),
-                                    nat.ptr(lv.A.values), nat.ptr(lv.dinv), nat.stream()), "fa_diag_inv")
+            lam = ctypes.c_double(0.0)
+            nat.check(L.fa_jacobi_setup(code, lv.n, lv.A.nnz, nat.ptr(lv.A.crow), nat.ptr(lv.A.col),
+                                        nat.ptr(lv.A.values), nat.ptr(lv.dinv), ctypes.byref(lam), nat.ptr(ws),
+                                        ws.numel(), nat.stream()), "fa_jacobi_setup")
+            lv.lmax = float(lam.value)
         last = self.levels[-1]
         nat.check(L.fa_csr_to_dense(last.n, nat.ptr(last.A.crow), nat.ptr(last.A.col), nat.ptr(last.A.values),
                                     nat.ptr(last.dense), nat.stream()), "fa_csr_to_dense")
-        nat.check(L.fa_dense_cholesky(last.n, nat.ptr(last.dense), nat.stream()), "fa_dense_cholesky")
+        nat.check(L.fa_dense_spd_inverse(last.n, nat.ptr(last.dense), nat.stream()), "fa_dense_spd_inverse")
         self._build_desc()
         self._setup_A = A
-        ws = nat.workspace(8 * 4 * 1024 + 4096, self.device)
-        for l in range(len(self.levels) - 1):
-            lv = self.levels[l]
-            lam = ctypes.c_double(0.0)
-            nat.check(L.fa_gershgorin(lv.n, nat.ptr(lv.A.crow), nat.ptr(lv.A.col), nat.ptr(lv.A.values),
-                                      ctypes.byref(lam), nat.ptr(ws), ws.numel(), nat.stream()), "fa_gershgorin")
-            lv.lmax = float(lam.value)
-            self._lv_array[l].lmax = lv.lmax
         return self
 
     def estimate_lmax(self, level, iters=None):
         """Power-iteration estimate of lambda_max(D^-1 A) on a level (diagnostics)."""
         L = nat.lib()
-        ws = nat.workspace(8 * 4 * 1024 + 4096, self.device)
+        ws = nat.workspace(L.fa_reduce_work_bytes(), self.device)
         lam = ctypes.c_double(0.0)
         nat.check(L.fa_mg_lmax(ctypes.byref(self._desc), level, iters or self.power_iters, ctypes.byref(lam),
                                nat.ptr(ws), ws.numel(), nat.stream()), "fa_mg_lmax")

@@ -126,6 +126,15 @@ int fa_gather_vector(int dtype, int64_t gdof, int32_t ldof,
                      const int32_t* adj_ptr, const int32_t* adj,
                      const void* elem, void* out, fa_stream_t stream);
 
+/* Physical points of barycentric quadrature points on every cell:
+ * out[c][q][d] = sum_v bcs[q][v] node[cell[c][v]][d]  (FP64; (gd, nv) in
+ * {(2, 3), (3, 3), (3, 4)}).
+ * Replaces mesh.bc_to_point (fealpy/mesh/mesh_base.py:454-481 ->
+ * backend/numpy_backend.py:401-407), the input of every cartesian
+ * coefficient / source callable. */
+int fa_bc_to_point(int gd, int nv, int64_t nc, int nq, const double* node, const int32_t* cell,
+                   const double* bcs, double* out, fa_stream_t stream);
+
 /* ------------------------------------------------------------ Dirichlet BC --
  * DirichletBC.apply (fealpy/fem/dirichlet_bc.py:101-234), reference-compatible
  * form: the new matrix drops every entry in a boundary row or boundary column
@@ -183,9 +192,6 @@ int fa_sell_fill(int dtype_in, int dtype_out, int64_t n, int64_t ncols, const in
                  const int32_t* indices, const void* values, const int32_t* slice_ptr, int32_t* cols, void* vals,
                  fa_stream_t stream);
 int fa_sell_spmv(int dtype, const fa_sell_t* A, const void* x, void* y, fa_stream_t stream);
-/* 1 / diag(A) of a CSR matrix (rows without a diagonal get 1). */
-int fa_diag_inv(int dtype_in, int dtype_out, int64_t n, const int32_t* indptr, const int32_t* indices,
-                const void* values, void* dinv, fa_stream_t stream);
 
 /* ------------------------------------------------------- Galerkin R A P --
  * Coarse operator C = R A P of the multigrid hierarchy, the role of
@@ -212,7 +218,7 @@ int fa_rap_values(int64_t nrows, const int32_t* r_ptr, const int32_t* r_idx, con
 /* ------------------------------------------------------- multigrid cycle --
  * Geometric multigrid V-cycle used as a CG preconditioner: Chebyshev
  * polynomial smoothing of D^-1 A (pre and post, same polynomial, so the cycle
- * is symmetric), restriction R, prolongation P, dense Cholesky on the
+ * is symmetric), restriction R, prolongation P, dense inverse on the
  * coarsest level.  Replaces GAMGSolver.vcycle (fealpy/solver/gamg_solver.py:
  * 256-301), whose Gauss-Seidel smoothing is a sequential triangular solve.
  * All level arrays are caller-owned device memory of the cycle's dtype. */
@@ -235,21 +241,25 @@ typedef struct {
     const fa_mg_level_t* levels;   /* host array, finest level first            */
     int degree;                /* Chebyshev degree of each smoothing sweep      */
     double lower, upper;       /* smoothing interval [lower, upper] * lmax      */
-    const double* coarse_chol; /* lower Cholesky factor of the coarsest A,
-                                  row-major n x n FP64                          */
+    const double* coarse_chol; /* inverse of the coarsest A, row-major n x n
+                                  FP64 (fa_dense_spd_inverse)                   */
 } fa_mg_t;
 
-/* Dense row-major copy of a small CSR matrix (FP64) and its in-place
- * Cholesky factorisation (lower triangle), single workgroup. */
+/* Dense row-major copy of a small CSR matrix (FP64) and its in-place inverse
+ * (Gauss-Jordan, SPD so no pivoting; n <= 2048, single workgroup).  The
+ * coarsest-level solve of the cycle is then one dense matrix-vector product. */
 int fa_csr_to_dense(int64_t n, const int32_t* indptr, const int32_t* indices, const double* values,
                     double* dense, fa_stream_t stream);
-int fa_dense_cholesky(int64_t n, double* a, fa_stream_t stream);
-/* Bounds for the Chebyshev interval.  fa_gershgorin: max_i sum_j |a_ij|/|a_ii|
- * of a CSR FP64 matrix, an upper bound of the spectrum of D^-1 A (the bound the
- * smoother uses); syncs.  fa_mg_lmax: power-iteration estimate of the largest
- * eigenvalue of D^-1 A on level `level` (uses the level's x, x2); syncs. */
-int fa_gershgorin(int64_t n, const int32_t* indptr, const int32_t* indices, const double* values,
-                  double* bound_host, void* work, size_t work_bytes, fa_stream_t stream);
+int fa_dense_spd_inverse(int64_t n, double* a, fa_stream_t stream);
+/* Smoother data of a level.  fa_jacobi_setup: dinv = 1/diag(A) (1 where the
+ * diagonal is absent) in `dtype_out`, and the Gershgorin bound of D^-1 A,
+ * max_i sum_j |a_ij|/|a_ii| -- an upper bound of its spectrum, the top of the
+ * Chebyshev interval; syncs.  fa_mg_lmax: power-iteration estimate of the
+ * largest eigenvalue of D^-1 A on level `level` (uses the level's x, x2;
+ * diagnostics); syncs.  Work: fa_reduce_work_bytes(). */
+int fa_jacobi_setup(int dtype_out, int64_t n, int64_t nnz, const int32_t* indptr, const int32_t* indices,
+                    const double* values, void* dinv, double* bound_host, void* work, size_t work_bytes,
+                    fa_stream_t stream);
 int fa_mg_lmax(const fa_mg_t* mg, int level, int iters, double* lmax_host, void* work, size_t work_bytes,
                fa_stream_t stream);
 /* z = V(r) on the finest level (r, z: dtype of the cycle). */

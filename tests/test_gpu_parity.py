@@ -199,3 +199,34 @@ def test_fp32_values_match_fp64():
     np.testing.assert_allclose(_np(A32.values), _np(A64.values), rtol=0, atol=2e-6 * float(A64.values.abs().max()))
     x = torch.rand(A64.shape[0], dtype=torch.float64, device=DEV)
     np.testing.assert_allclose(_np(A32 @ x.float()), _np(A64 @ x), rtol=0, atol=1e-5 * float((A64 @ x).abs().max()))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_bc_to_point_matches_host(dim):
+    """fa_bc_to_point == the host evaluation sum_v bcs[q, v] node[cell[c, v]]."""
+    if dim == 2:
+        mesh = fa.TriangleMesh.from_box(nx=7, ny=5, device=DEV)
+    else:
+        mesh = fa.TetrahedronMesh.from_box(nx=4, ny=3, nz=5, device=DEV)
+    bcs, _ = mesh.quadrature_formula(4).get_quadrature_points_and_weights()
+    got = mesh.bc_to_point(bcs).cpu().numpy()
+    node = mesh.node.cpu().numpy()
+    cell = mesh.cell.cpu().numpy()
+    ref = np.einsum('qv, cvd -> cqd', np.asarray(bcs.cpu() if torch.is_tensor(bcs) else bcs), node[cell])
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-15)
+
+
+def test_boundary_data_cached_per_space():
+    """Second DirichletBC on the same space reuses the filtered pattern and
+    gives identical results."""
+    mesh = fa.TetrahedronMesh.from_box(nx=6, ny=6, nz=6, device=DEV)
+    space = fa.LagrangeFESpace(mesh, 2)
+    u = fa.cartesian(lambda x: x[..., 0] * x[..., 1] + x[..., 2])
+    A = fa.BilinearForm(space).add_integrator(fa.ScalarDiffusionIntegrator(q=3)).assembly()
+    F = fa.LinearForm(space).add_integrator(fa.ScalarSourceIntegrator(1.0, q=3)).assembly()
+    A1, F1 = fa.DirichletBC(space, gd=u).apply(A, F)
+    A2, F2 = fa.DirichletBC(space, gd=u).apply(A, F)
+    assert A2.col is A1.col and torch.equal(A1.values, A2.values) and torch.equal(F1, F2)
+    ref_flag = space.is_boundary_dof(threshold=lambda p: torch.ones(p.shape[0], dtype=torch.bool, device=p.device),
+                                     method='interp')
+    assert torch.equal(space.is_boundary_dof(), ref_flag)
