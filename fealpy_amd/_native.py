@@ -40,6 +40,11 @@ _SIGNATURES = {
                                           _p, _p, ctypes.c_int, _f64, _p, _p, _p]),
     "fa_gather_matrix": (ctypes.c_int, [ctypes.c_int, _i64, _i32, _p, _p, _p, _p, _p, _i32, _p, _p, _p]),
     "fa_gather_vector": (ctypes.c_int, [ctypes.c_int, _i64, _i32, _p, _p, _p, _p, _p]),
+    "fa_assemble_p1_matrix": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64, ctypes.c_int, _p, _p, _p, _p,
+                                             ctypes.c_int, ctypes.c_double, _p, ctypes.c_int, ctypes.c_double, _p,
+                                             _p, _p, _p, _p, _i32, _p, _p]),
+    "fa_assemble_p1_vector": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64, ctypes.c_int, _p, _p, _p, _p,
+                                             ctypes.c_int, ctypes.c_double, _p, _p, _p, _p, _p]),
     "fa_bc_to_point": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64, ctypes.c_int, _p, _p, _p, _p, _p]),
     "fa_dirichlet_work_bytes": (_sz, [_i64]),
     "fa_dirichlet_count": (ctypes.c_int, [_i64, _p, _p, _p, _p, _pi64, _p, _sz, _p]),

@@ -247,30 +247,271 @@ __global__ void __launch_bounds__(256) k_gather_vector(int64_t gdof, const int32
 }
 
 // physical coordinates of barycentric points on every cell: one thread per
-// cell reads its vertices once and writes its nq*gd contiguous outputs
+// cell reads its vertices once; the block's nq*gd outputs per cell are staged
+// in LDS and written back as one contiguous, coalesced range
 template <int GD, int NV>
-__global__ void __launch_bounds__(256) k_bc_to_point(int64_t nc, int nq, const double* __restrict__ node,
-                                                    const int32_t* __restrict__ cell, const double* __restrict__ bcs,
-                                                    double* __restrict__ out) {
-    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
-        double v[NV][GD];
+__global__ void k_bc_to_point(int64_t nc, int nq, const double* __restrict__ node, const int32_t* __restrict__ cell,
+                              const double* __restrict__ bcs, double* __restrict__ out) {
+    extern __shared__ double stage[];
+    const int per = nq * GD;
+    for (int64_t c0 = (int64_t)blockIdx.x * blockDim.x; c0 < nc; c0 += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = c0 + threadIdx.x;
+        if (c < nc) {
+            double v[NV][GD];
 #pragma unroll
-        for (int a = 0; a < NV; ++a) {
-            const int64_t nd = cell[c * NV + a];
+            for (int a = 0; a < NV; ++a) {
+                const int64_t nd = cell[c * NV + a];
 #pragma unroll
-            for (int d = 0; d < GD; ++d) v[a][d] = node[nd * GD + d];
-        }
-        double* o = out + c * nq * GD;
-        for (int q = 0; q < nq; ++q) {
+                for (int d = 0; d < GD; ++d) v[a][d] = node[nd * GD + d];
+            }
+            double* o = stage + threadIdx.x * per;
+            for (int q = 0; q < nq; ++q) {
 #pragma unroll
-            for (int d = 0; d < GD; ++d) {
-                double x = 0.0;
+                for (int d = 0; d < GD; ++d) {
+                    double x = 0.0;
 #pragma unroll
-                for (int a = 0; a < NV; ++a) x += bcs[q * NV + a] * v[a][d];
-                o[q * GD + d] = x;
+                    for (int a = 0; a < NV; ++a) x += bcs[q * NV + a] * v[a][d];
+                    o[q * GD + d] = x;
+                }
             }
         }
+        __syncthreads();
+        const int64_t rem = nc - c0;
+        const int64_t cnt = (rem < (int64_t)blockDim.x ? rem : (int64_t)blockDim.x) * per;
+        double* o = out + c0 * per;
+        for (int64_t t = threadIdx.x; t < cnt; t += blockDim.x) o[t] = stage[t];
+        __syncthreads();
     }
+}
+
+// ---- fused P1 assembly: element rows computed where they are summed ---------
+// P1 gradients are constant on a cell, so the element row i of cell c is
+//   A_ij = kd |T| (gl_i . gl_j) + |T| sum_q w_q k_q phi_i(q) phi_j(q),
+// kd = sum_q w_q k_q, gl_i = n_i / (6|T|) with n_i the face normals of
+// Geom<3> (tetrahedron) or the rotated edges of Geom<2> (triangle).
+template <int TD>
+struct P1Cell {
+    double n[TD + 1][TD];  // grad lambda times the scale `den` (den = 6|T| or 2|T|)
+    double den, meas;
+    __device__ void compute(const double p[TD + 1][TD]);
+};
+
+template <>
+__device__ inline void P1Cell<3>::compute(const double p[4][3]) {
+    double a[3], b[3], c[3], t[3];
+    for (int d = 0; d < 3; ++d) {
+        a[d] = p[1][d] - p[0][d];
+        b[d] = p[2][d] - p[1][d];
+        c[d] = p[3][d] - p[2][d];
+    }
+    cross3(b, c, t);
+    den = a[0] * t[0] + a[1] * t[1] + a[2] * t[2];
+    const int lf[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        double vjk[3], vjm[3];
+        for (int d = 0; d < 3; ++d) {
+            vjk[d] = p[lf[i][1]][d] - p[lf[i][0]][d];
+            vjm[d] = p[lf[i][2]][d] - p[lf[i][0]][d];
+        }
+        cross3(vjm, vjk, n[i]);
+    }
+    double v01[3], v02[3], v03[3], x[3];
+    for (int d = 0; d < 3; ++d) {
+        v01[d] = p[1][d] - p[0][d];
+        v02[d] = p[2][d] - p[0][d];
+        v03[d] = p[3][d] - p[0][d];
+    }
+    cross3(v01, v02, x);
+    meas = (v03[0] * x[0] + v03[1] * x[1] + v03[2] * x[2]) / 6.0;
+}
+
+template <>
+__device__ inline void P1Cell<2>::compute(const double p[3][2]) {
+    double e[3][2];
+    for (int d = 0; d < 2; ++d) {
+        e[0][d] = p[2][d] - p[1][d];
+        e[1][d] = p[0][d] - p[2][d];
+        e[2][d] = p[1][d] - p[0][d];
+    }
+    den = e[0][0] * e[1][1] - e[0][1] * e[1][0];
+    for (int i = 0; i < 3; ++i) {
+        n[i][0] = -e[i][1];
+        n[i][1] = e[i][0];
+    }
+    const double a0 = p[1][0] - p[0][0], a1 = p[1][1] - p[0][1];
+    const double b0 = p[2][0] - p[1][0], b1 = p[2][1] - p[1][1];
+    meas = (a0 * b1 - a1 * b0) / 2.0;
+}
+
+template <int TD>
+__device__ inline void p1_row(const P1Cell<TD>& g, int i, int64_t c, int nq, const double* __restrict__ qw,
+                              const double* __restrict__ phi, int dk, double ds, const double* __restrict__ dc, int mk,
+                              double ms, const double* __restrict__ mc, double out[TD + 1]) {
+#pragma unroll
+    for (int j = 0; j < TD + 1; ++j) out[j] = 0.0;
+    if (dk) {
+        double kd = 0.0;
+        for (int q = 0; q < nq; ++q) kd += qw[q] * coef_value(dk, ds, dc, c, q, nq);
+        const double s = kd * g.meas / (g.den * g.den);
+        double ni[TD];  // row i selected without a runtime index into g.n (keeps it in registers)
+#pragma unroll
+        for (int d = 0; d < TD; ++d) ni[d] = g.n[0][d];
+#pragma unroll
+        for (int a = 1; a < TD + 1; ++a)
+            if (i == a) {
+#pragma unroll
+                for (int d = 0; d < TD; ++d) ni[d] = g.n[a][d];
+            }
+#pragma unroll
+        for (int j = 0; j < TD + 1; ++j) {
+            double dot = 0.0;
+#pragma unroll
+            for (int d = 0; d < TD; ++d) dot += ni[d] * g.n[j][d];
+            out[j] = s * dot;
+        }
+    }
+    if (mk) {
+        for (int q = 0; q < nq; ++q) {
+            const double* ph = phi + q * (TD + 1);
+            const double w = qw[q] * g.meas * coef_value(mk, ms, mc, c, q, nq) * ph[i];
+#pragma unroll
+            for (int j = 0; j < TD + 1; ++j) out[j] += w * ph[j];
+        }
+    }
+}
+
+// G lanes per CSR row, K row positions owned per lane.  Incidences of row r
+// (cell c, local i; ascending c) are taken G at a time: lane l computes the
+// element row of its incidence and parks value j at tab[pos(col_j)][l]; the
+// owner of each position then adds tab[pos][0..G) in lane order.  Every CSR
+// entry is therefore the sequential sum of its cell contributions in
+// ascending cell order, the order of the reference's stable lexsort followed
+// by sequential np.add.at (coo_tensor.py:184-213).
+template <int TD, int G, int K, class T>
+__global__ void __launch_bounds__(256) k_assemble_p1(int64_t gdof, const double* __restrict__ node,
+                                                    const int32_t* __restrict__ cell, int nq,
+                                                    const double* __restrict__ qw, const double* __restrict__ phi,
+                                                    int dk, double ds, const double* __restrict__ dc, int mk, double ms,
+                                                    const double* __restrict__ mc, const int32_t* __restrict__ ptr,
+                                                    const int32_t* __restrict__ adj, const int32_t* __restrict__ indptr,
+                                                    const int32_t* __restrict__ indices, int lmax,
+                                                    T* __restrict__ values) {
+    constexpr int NV = TD + 1;
+    extern __shared__ char smem[];
+    const int rpb = blockDim.x / G;
+    const int sub = threadIdx.x / G;
+    const int lane = threadIdx.x % G;
+    int32_t* cols = reinterpret_cast<int32_t*>(smem) + sub * lmax;
+    double* tab = reinterpret_cast<double*>(smem + align_up(sizeof(int32_t) * rpb * lmax, 16)) + (int64_t)sub * lmax * G;
+    for (int64_t r = (int64_t)blockIdx.x * rpb + sub; r < gdof; r += (int64_t)gridDim.x * rpb) {
+        const int32_t r0 = indptr[r];
+        const int L = indptr[r + 1] - r0;
+        for (int t = lane; t < L; t += G) cols[t] = indices[r0 + t];
+        for (int t = 0; t < L; ++t) tab[t * G + lane] = 0.0;
+        double acc[K];
+#pragma unroll
+        for (int m = 0; m < K; ++m) acc[m] = 0.0;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const int32_t k1 = ptr[r + 1];
+        for (int32_t kb = ptr[r]; kb < k1; kb += G) {
+            const int32_t k = kb + lane;
+            int pos[NV];
+            if (k < k1) {
+                const int32_t e = adj[k];
+                const int64_t c = e / NV;
+                const int i = e - static_cast<int>(c) * NV;
+                double p[NV][TD];
+                load_cell<TD>(node, cell, c, p);
+                P1Cell<TD> g;
+                g.compute(p);
+                double v[NV];
+                p1_row<TD>(g, i, c, nq, qw, phi, dk, ds, dc, mk, ms, mc, v);
+#pragma unroll
+                for (int j = 0; j < NV; ++j) {
+                    pos[j] = lower_bound_lds(cols, L, cell[c * NV + j]);
+                    tab[pos[j] * G + lane] = v[j];
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+            for (int m = 0; m < K; ++m) {
+                const int t = lane + m * G;
+                if (t < L) {
+#pragma unroll
+                    for (int l = 0; l < G; ++l) acc[m] += tab[t * G + l];
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            if (k < k1) {
+#pragma unroll
+                for (int j = 0; j < NV; ++j) tab[pos[j] * G + lane] = 0.0;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < K; ++m) {
+            const int t = lane + m * G;
+            if (t < L) values[r0 + t] = static_cast<T>(acc[m]);
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+}
+
+// load vector, P1: b_r = sum over incidences (c, i) of |T_c| sum_q w_q f_cq phi_i(q);
+// G lanes per DOF, lane partials combined by a fixed tree
+template <int TD, int G, class T>
+__global__ void __launch_bounds__(256) k_assemble_p1_vec(int64_t gdof, const double* __restrict__ node,
+                                                        const int32_t* __restrict__ cell, int nq,
+                                                        const double* __restrict__ qw,
+                                                        const double* __restrict__ phi, int sk, double ss,
+                                                        const double* __restrict__ src,
+                                                        const int32_t* __restrict__ ptr,
+                                                        const int32_t* __restrict__ adj, T* __restrict__ out) {
+    constexpr int NV = TD + 1;
+    const int lane = threadIdx.x % G;
+    for (int64_t r = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G; r < gdof;
+         r += (int64_t)gridDim.x * blockDim.x / G) {
+        double s = 0.0;
+        const int32_t k1 = ptr[r + 1];
+        for (int32_t k = ptr[r] + lane; k < k1; k += G) {
+            const int32_t e = adj[k];
+            const int64_t c = e / NV;
+            const int i = e - static_cast<int>(c) * NV;
+            double p[NV][TD];
+            load_cell<TD>(node, cell, c, p);
+            P1Cell<TD> g;
+            g.compute(p);
+            double v = 0.0;
+            for (int q = 0; q < nq; ++q) v += qw[q] * g.meas * coef_value(sk, ss, src, c, q, nq) * phi[q * NV + i];
+            s += v;
+        }
+        s = group_sum<G>(s);
+        if (lane == 0) out[r] = static_cast<T>(s);
+    }
+}
+
+template <int TD, int G, int K, class T>
+static int launch_p1(int64_t gdof, const double* node, const int32_t* cell, int nq, const double* qw,
+                     const double* phi, int dk, double ds, const double* dc, int mk, double ms, const double* mc,
+                     const int32_t* ptr, const int32_t* adj, const int32_t* indptr, const int32_t* indices, int lmax,
+                     void* values, hipStream_t s) {
+    const size_t per_row = sizeof(int32_t) * lmax + sizeof(double) * lmax * G;
+    int rpb = 256 / G;
+    while (rpb > 1 && rpb * per_row + 16 > 48 * 1024) rpb >>= 1;
+    const size_t lds = align_up(sizeof(int32_t) * rpb * lmax, 16) + sizeof(double) * rpb * lmax * G;
+    if (lds > 64 * 1024) {
+        set_error("fa_assemble_p1_matrix: row length %d exceeds the LDS capacity", lmax);
+        return FA_ECAPACITY;
+    }
+    const unsigned grid = grid_for(gdof, rpb, 1 << 20);
+    hipLaunchKernelGGL((k_assemble_p1<TD, G, K, T>), dim3(grid), dim3(rpb * G), lds, s, gdof, node, cell, nq, qw, phi,
+                       dk, ds, dc, mk, ms, mc, ptr, adj, indptr, indices, lmax, static_cast<T*>(values));
+    FA_LAUNCH("k_assemble_p1");
+    return FA_OK;
 }
 
 template <int TD, int LDOF, class T>
@@ -414,9 +655,13 @@ extern "C" int fa_bc_to_point(int gd, int nv, int64_t nc, int nq, const double* 
     FA_REQUIRE(nq >= 1 && nc >= 0, "fa_bc_to_point: bad sizes");
     if (nc == 0) return FA_OK;
     hipStream_t s = S(stream);
-    const unsigned grid = grid_for(nc, 256, 1 << 16);
+    const size_t per = static_cast<size_t>(nq) * gd * sizeof(double);
+    FA_REQUIRE(16 * per <= 64 * 1024, "fa_bc_to_point: %d points per cell exceed the staging capacity", nq);
+    unsigned blk = 256;
+    while (blk > 16 && blk * per > 48 * 1024) blk >>= 1;
+    const unsigned grid = grid_for(nc, blk, 1 << 16);
 #define FA_BP(GD, NV) \
-    hipLaunchKernelGGL((k_bc_to_point<GD, NV>), dim3(grid), dim3(256), 0, s, nc, nq, node, cell, bcs, out)
+    hipLaunchKernelGGL((k_bc_to_point<GD, NV>), dim3(grid), dim3(blk), blk * per, s, nc, nq, node, cell, bcs, out)
     if (gd == 2 && nv == 3) FA_BP(2, 3);
     else if (gd == 3 && nv == 3) FA_BP(3, 3);
     else if (gd == 3 && nv == 4) FA_BP(3, 4);
@@ -426,5 +671,64 @@ extern "C" int fa_bc_to_point(int gd, int nv, int64_t nc, int nq, const double* 
     }
 #undef FA_BP
     FA_LAUNCH("k_bc_to_point");
+    return FA_OK;
+}
+
+extern "C" int fa_assemble_p1_matrix(int dtype, int td, int64_t gdof, int nq, const double* node, const int32_t* cell,
+                                     const double* qw, const double* phi, int diff_kind, double diff_scale,
+                                     const double* diff_coef, int mass_kind, double mass_scale,
+                                     const double* mass_coef, const int32_t* adj_ptr, const int32_t* adj,
+                                     const int32_t* indptr, const int32_t* indices, int32_t max_row, void* values,
+                                     fa_stream_t stream) {
+    FA_REQUIRE(dtype == FA_F64 || dtype == FA_F32, "fa_assemble_p1_matrix: bad dtype %d", dtype);
+    FA_REQUIRE(td == 2 || td == 3, "fa_assemble_p1_matrix: td must be 2 or 3");
+    FA_REQUIRE(gdof > 0 && nq > 0 && max_row > 0, "fa_assemble_p1_matrix: bad sizes");
+    FA_REQUIRE(!mass_kind || phi, "fa_assemble_p1_matrix: mass needs phi");
+    FA_REQUIRE(!(diff_kind >= FA_COEF_CELL && !diff_coef) && !(mass_kind >= FA_COEF_CELL && !mass_coef),
+               "fa_assemble_p1_matrix: coefficient array missing");
+    hipStream_t s = S(stream);
+#define FA_P1(TD_, G_, K_)                                                                                     \
+    return dtype == FA_F64                                                                                     \
+               ? launch_p1<TD_, G_, K_, double>(gdof, node, cell, nq, qw, phi, diff_kind, diff_scale, diff_coef, \
+                                                mass_kind, mass_scale, mass_coef, adj_ptr, adj, indptr, indices,   \
+                                                max_row, values, s)                                              \
+               : launch_p1<TD_, G_, K_, float>(gdof, node, cell, nq, qw, phi, diff_kind, diff_scale, diff_coef,  \
+                                               mass_kind, mass_scale, mass_coef, adj_ptr, adj, indptr, indices,    \
+                                               max_row, values, s)
+    if (td == 3) {
+        if (max_row <= 32) FA_P1(3, 16, 2);
+        if (max_row <= 64) FA_P1(3, 16, 4);
+        if (max_row <= 128) FA_P1(3, 32, 4);
+    } else {
+        if (max_row <= 16) FA_P1(2, 8, 2);
+        if (max_row <= 32) FA_P1(2, 8, 4);
+        if (max_row <= 128) FA_P1(2, 32, 4);
+    }
+#undef FA_P1
+    set_error("fa_assemble_p1_matrix: rows of %d entries exceed the kernel capacity (128)", max_row);
+    return FA_ECAPACITY;
+}
+
+extern "C" int fa_assemble_p1_vector(int dtype, int td, int64_t gdof, int nq, const double* node, const int32_t* cell,
+                                     const double* qw, const double* phi, int src_kind, double src_scale,
+                                     const double* src, const int32_t* adj_ptr, const int32_t* adj, void* out,
+                                     fa_stream_t stream) {
+    FA_REQUIRE(dtype == FA_F64 || dtype == FA_F32, "fa_assemble_p1_vector: bad dtype %d", dtype);
+    FA_REQUIRE(td == 2 || td == 3, "fa_assemble_p1_vector: td must be 2 or 3");
+    FA_REQUIRE(gdof > 0 && nq > 0 && phi, "fa_assemble_p1_vector: bad sizes");
+    FA_REQUIRE(src_kind < FA_COEF_CELL || src, "fa_assemble_p1_vector: source array missing");
+    hipStream_t s = S(stream);
+    constexpr int G = 8;
+    const unsigned grid = grid_for(gdof * G, 256, 1 << 16);
+#define FA_V(TD_, T_)                                                                                             \
+    hipLaunchKernelGGL((k_assemble_p1_vec<TD_, G, T_>), dim3(grid), dim3(256), 0, s, gdof, node, cell, nq, qw, phi, \
+                       src_kind, src_scale, src, adj_ptr, adj, static_cast<T_*>(out))
+    if (td == 3) {
+        if (dtype == FA_F64) FA_V(3, double); else FA_V(3, float);
+    } else {
+        if (dtype == FA_F64) FA_V(2, double); else FA_V(2, float);
+    }
+#undef FA_V
+    FA_LAUNCH("k_assemble_p1_vec");
     return FA_OK;
 }

@@ -92,9 +92,11 @@ __global__ void __launch_bounds__(WAVE) k_rap_rows(int64_t nrows, const int32_t*
     if (max_row && lane == 0) atomicMax(max_row, row_max);
 }
 
-// R A P numeric: one wavefront per coarse row; lane l takes A entries l, l+64,
-// ... of each R entry (in R order) and accumulates r*a*p into its private LDS
-// row; the row value is then the sum over lanes in lane order.
+// R A P numeric: one wavefront per coarse row I.  The (R entry u, A entry t)
+// pairs of the row are numbered flat (R order, then A order) and dealt to the
+// lanes round-robin; lane l accumulates r_u a_t p_q into its private LDS row at
+// the position of column J = col(p_q).  The row value is the sum over lanes by
+// a fixed binary tree, so the result does not depend on timing.
 __global__ void __launch_bounds__(WAVE) k_rap_values(int64_t nrows, const int32_t* __restrict__ rp,
                                                      const int32_t* __restrict__ ri, const double* __restrict__ rv,
                                                      const int32_t* __restrict__ ap, const int32_t* __restrict__ ai,
@@ -105,6 +107,8 @@ __global__ void __launch_bounds__(WAVE) k_rap_values(int64_t nrows, const int32_
     extern __shared__ char smem[];
     int32_t* cols = reinterpret_cast<int32_t*>(smem);
     double* acc = reinterpret_cast<double*>(smem + align_up(sizeof(int32_t) * lmax, 16));  // [WAVE][lmax]
+    __shared__ int32_t s_off[WAVE + 1], s_a0[WAVE];
+    __shared__ double s_r[WAVE];
     const int lane = threadIdx.x;
     for (int64_t I = blockIdx.x; I < nrows; I += gridDim.x) {
         const int32_t c0 = cptr[I];
@@ -112,31 +116,55 @@ __global__ void __launch_bounds__(WAVE) k_rap_values(int64_t nrows, const int32_
         for (int t = lane; t < L; t += WAVE) cols[t] = cidx[c0 + t];
         for (int t = 0; t < L; ++t) acc[lane * lmax + t] = 0.0;
         __syncthreads();
-        for (int32_t u = rp[I]; u < rp[I + 1]; ++u) {
-            const int32_t i = ri[u];
-            const double r = rv[u];
-            for (int32_t t = ap[i] + lane; t < ap[i + 1]; t += WAVE) {
+        const int32_t r0 = rp[I], r1 = rp[I + 1];
+        for (int32_t u0 = r0; u0 < r1; u0 += WAVE) {
+            const int32_t u = u0 + lane;
+            int len = 0;
+            if (u < r1) {
+                const int32_t i = ri[u];
+                const int32_t a0 = ap[i];
+                s_a0[lane] = a0;
+                s_r[lane] = rv[u];
+                len = ap[i + 1] - a0;
+            }
+            const int inc = wave_inclusive_scan(len);
+            s_off[lane + 1] = inc;
+            if (lane == 0) s_off[0] = 0;
+            const int total = __shfl(inc, WAVE - 1);
+            const int nu = min(WAVE, static_cast<int>(r1 - u0));
+            __syncthreads();
+            for (int f = lane; f < total; f += WAVE) {
+                int lo = 0, hi = nu - 1;  // last R entry whose offset is <= f
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_off[mid] <= f) lo = mid;
+                    else hi = mid - 1;
+                }
+                const int32_t t = s_a0[lo] + (f - s_off[lo]);
                 const int32_t j = ai[t];
-                const double ra = r * av[t];
+                const double ra = s_r[lo] * av[t];
                 for (int32_t q = pp[j]; q < pp[j + 1]; ++q) {
                     const int32_t J = pi[q];
-                    int lo = 0, hi = L;
-                    while (lo < hi) {
-                        const int mid = (lo + hi) >> 1;
-                        if (cols[mid] < J) lo = mid + 1;
-                        else hi = mid;
+                    int a = 0, b = L;
+                    while (a < b) {
+                        const int mid = (a + b) >> 1;
+                        if (cols[mid] < J) a = mid + 1;
+                        else b = mid;
                     }
-                    if (lo < L && cols[lo] == J) acc[lane * lmax + lo] += ra * pv[q];
+                    if (a < L && cols[a] == J) acc[lane * lmax + a] += ra * pv[q];
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
-        const bool empty = (rp[I] == rp[I + 1]);
-        for (int t = lane; t < L; t += WAVE) {
-            double s = 0.0;
-            for (int l = 0; l < WAVE; ++l) s += acc[l * lmax + t];
-            cval[c0 + t] = empty ? 1.0 : s;
+        for (int h = WAVE / 2; h > 0; h >>= 1) {  // lanes [0, h) += lanes [h, 2h)
+            for (int w = lane; w < h * L; w += WAVE) {
+                const int l = w / L, t = w - l * L;
+                acc[l * lmax + t] += acc[(l + h) * lmax + t];
+            }
+            __syncthreads();
         }
+        const bool empty = (r0 == r1);
+        for (int t = lane; t < L; t += WAVE) cval[c0 + t] = empty ? 1.0 : acc[t];
         __syncthreads();
     }
 }
@@ -366,12 +394,140 @@ __global__ void __launch_bounds__(RED_BLOCK) k_jacobi_setup(int64_t n, const int
     }
 }
 
-__global__ void __launch_bounds__(WAVE) k_max_finish(const double* __restrict__ partials, int nparts,
-                                                    double* __restrict__ out) {
+__global__ void __launch_bounds__(RED_BLOCK) k_max_finish(const double* __restrict__ partials, int nparts,
+                                                         double* __restrict__ out) {
+    __shared__ double ws[RED_BLOCK / WAVE];
     double m = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += WAVE) m = fmax(m, partials[i]);
+    for (int i = threadIdx.x; i < nparts; i += RED_BLOCK) m = fmax(m, partials[i]);
     for (int o = WAVE / 2; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
-    if (threadIdx.x == 0) *out = m;
+    if ((threadIdx.x & (WAVE - 1)) == 0) ws[threadIdx.x / WAVE] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < RED_BLOCK / WAVE; ++w) m = fmax(m, ws[w]);
+        *out = m;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused coarse cycle: the levels with at most FUSE_ROWS rows run as one
+// workgroup, phases separated by block barriers, instead of ~7 launches per
+// level whose cost on these sizes is the launch itself.  Same operations and
+// order as the per-level kernels below.
+constexpr int FUSE_MAX_LEVELS = 16;
+constexpr int64_t FUSE_ROWS = 1024;  // one workgroup is latency-bound above ~1 row per thread
+constexpr int FUSE_BLOCK = 1024;
+
+struct FusedLevel {
+    fa_sell_t A, R, P;
+    const void* dinv;
+    void *b, *x, *x2, *d, *r;
+    double lmax;
+};
+
+struct FusedCycle {
+    int nlev, degree;
+    double lower, upper;
+    const double* ainv;  // inverse of the coarsest operator, row-major
+    FusedLevel lv[FUSE_MAX_LEVELS];
+};
+
+template <class T>
+__device__ inline void blk_smooth(const FusedCycle& fc, const FusedLevel& L, const T* b, T*& cur, T*& nxt,
+                                  bool from_zero) {
+    const int64_t n = L.A.n, npad = (n + 63) & ~63LL;
+    const T* sv = static_cast<const T*>(L.A.vals);
+    const T* dinv = static_cast<const T*>(L.dinv);
+    T* d = static_cast<T*>(L.d);
+    const double lo = fc.lower * L.lmax, hi = fc.upper * L.lmax;
+    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
+    double rho = 1.0 / sigma;
+    for (int k = 0; k < fc.degree; ++k) {
+        T c1, c2;
+        double rho_new = rho;
+        if (k == 0) {
+            c1 = T(0);
+            c2 = static_cast<T>(1.0 / theta);
+        } else {
+            rho_new = 1.0 / (2.0 * sigma - rho);
+            c1 = static_cast<T>(rho_new * rho);
+            c2 = static_cast<T>(2.0 * rho_new / delta);
+        }
+        if (k == 0 && from_zero) {
+            for (int64_t row = threadIdx.x; row < n; row += blockDim.x) {
+                const T v = c2 * dinv[row] * b[row];
+                d[row] = v;
+                cur[row] = v;
+            }
+        } else {
+            for (int64_t row = threadIdx.x; row < npad; row += blockDim.x) {
+                const T ax = sell_row_dot<T, T>(L.A.slice_ptr, L.A.cols, sv, row, cur);
+                if (row < n) {
+                    const T res = b[row] - ax;
+                    const T dn = (k == 0 ? T(0) : c1 * d[row]) + c2 * dinv[row] * res;
+                    d[row] = dn;
+                    nxt[row] = cur[row] + dn;
+                }
+            }
+            T* t = cur;
+            cur = nxt;
+            nxt = t;
+        }
+        rho = rho_new;
+        __syncthreads();
+    }
+}
+
+template <class T, bool ADD>
+__device__ inline void blk_apply(const fa_sell_t& M, const T* x, T* y) {
+    const int64_t npad = (M.n + 63) & ~63LL;
+    const T* mv = static_cast<const T*>(M.vals);
+    for (int64_t row = threadIdx.x; row < npad; row += blockDim.x) {
+        const T v = sell_row_dot<T, T>(M.slice_ptr, M.cols, mv, row, x);
+        if (row < M.n) y[row] = ADD ? y[row] + v : v;
+    }
+    __syncthreads();
+}
+
+template <class T>
+__global__ void __launch_bounds__(FUSE_BLOCK) k_fused_cycle(FusedCycle fc, const int* __restrict__ done) {
+    if (done && *done) return;
+    T* cur[FUSE_MAX_LEVELS];
+    T* nxt[FUSE_MAX_LEVELS];
+    const int nl = fc.nlev;
+    for (int l = 0; l < nl - 1; ++l) {
+        const FusedLevel& L = fc.lv[l];
+        const T* b = static_cast<const T*>(L.b);
+        cur[l] = static_cast<T*>(L.x);
+        nxt[l] = static_cast<T*>(L.x2);
+        blk_smooth<T>(fc, L, b, cur[l], nxt[l], true);
+        T* r = static_cast<T*>(L.r);
+        const int64_t n = L.A.n, npad = (n + 63) & ~63LL;
+        for (int64_t row = threadIdx.x; row < npad; row += blockDim.x) {
+            const T ax = sell_row_dot<T, T>(L.A.slice_ptr, L.A.cols, static_cast<const T*>(L.A.vals), row, cur[l]);
+            if (row < n) r[row] = b[row] - ax;
+        }
+        __syncthreads();
+        blk_apply<T, false>(L.R, r, static_cast<T*>(fc.lv[l + 1].b));
+    }
+    {
+        const FusedLevel& C = fc.lv[nl - 1];
+        const int n = static_cast<int>(C.A.n);
+        const T* b = static_cast<const T*>(C.b);
+        T* x = static_cast<T*>(C.x);
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            const double* row = fc.ainv + (int64_t)i * n;
+            double s = 0.0;
+            for (int j = 0; j < n; ++j) s += row[j] * static_cast<double>(b[j]);
+            x[i] = static_cast<T>(s);
+        }
+        cur[nl - 1] = x;
+        __syncthreads();
+    }
+    for (int l = nl - 2; l >= 0; --l) {
+        const FusedLevel& L = fc.lv[l];
+        blk_apply<T, true>(L.P, cur[l + 1], cur[l]);
+        blk_smooth<T>(fc, L, static_cast<const T*>(L.b), cur[l], nxt[l], false);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -428,11 +584,40 @@ struct Cycle {
         return FA_OK;
     }
 
+    // levels l..nlevels-1 in one workgroup when all are small; the result of
+    // level l lands in x2 (an odd number of smoother swaps: degree-1 before,
+    // degree after), or in x when l is the coarsest level
+    int fused(int l, T** out) {
+        FusedCycle fc{};
+        fc.nlev = mg->nlevels - l;
+        fc.degree = mg->degree;
+        fc.lower = mg->lower;
+        fc.upper = mg->upper;
+        fc.ainv = mg->coarse_chol;
+        for (int k = 0; k < fc.nlev; ++k) {
+            const fa_mg_level_t& L = mg->levels[l + k];
+            fc.lv[k] = FusedLevel{L.A, L.R, L.P, L.dinv, L.b, L.x, L.x2, L.d, L.r, L.lmax};
+        }
+        hipLaunchKernelGGL(k_fused_cycle<T>, dim3(1), dim3(FUSE_BLOCK), 0, s, fc, done);
+        FA_LAUNCH("k_fused_cycle");
+        const fa_mg_level_t& L = mg->levels[l];
+        *out = static_cast<T*>(fc.nlev == 1 ? L.x : L.x2);
+        return FA_OK;
+    }
+
+    bool fusable(int l) const {
+        if (mg->nlevels - l > FUSE_MAX_LEVELS || mg->degree < 1) return false;
+        for (int k = l; k < mg->nlevels; ++k)
+            if (mg->levels[k].A.n > FUSE_ROWS) return false;
+        return true;
+    }
+
     // returns (through out) the level-l vector that holds the result
     int run(int l, const T* b, T** out) {
         const fa_mg_level_t& L = mg->levels[l];
         T* cur = static_cast<T*>(L.x);
         T* nxt = static_cast<T*>(L.x2);
+        if (l > 0 && fusable(l)) return fused(l, out);
         if (l == mg->nlevels - 1) {
             int st = coarse(L, b, cur);
             *out = cur;
@@ -882,7 +1067,7 @@ extern "C" int fa_jacobi_setup(int dtype_out, int64_t n, int64_t nnz, const int3
         if (avg <= 12) FA_JS(float, 8); else if (avg <= 24) FA_JS(float, 16); else FA_JS(float, 32);
     }
 #undef FA_JS
-    hipLaunchKernelGGL(k_max_finish, dim3(1), dim3(WAVE), 0, s, partials, (int)grid, out);
+    hipLaunchKernelGGL(k_max_finish, dim3(1), dim3(RED_BLOCK), 0, s, partials, (int)grid, out);
     FA_LAUNCH("k_jacobi_setup");
     double h = 0.0;
     FA_HIP(hipMemcpyAsync(&h, out, sizeof(double), hipMemcpyDeviceToHost, s));

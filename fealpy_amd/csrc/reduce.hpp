@@ -6,7 +6,7 @@
 namespace fa {
 
 constexpr int RED_BLOCK = 256;
-constexpr int RED_GRID = 8192;  // upper bound of a reduction grid (partials capacity)
+constexpr int RED_GRID = 2048;  // upper bound of a reduction grid (partials capacity): 8 waves per SIMD
 
 // reduction grid for n items: depends on n only, so the summation order is fixed
 inline unsigned red_grid(int64_t n) {
@@ -49,8 +49,16 @@ __device__ inline void store_partial(double v, double* partials) {
 
 // sum of partials[0..n) in a fixed order by one block; valid in thread 0
 __device__ inline double finish_sum(const double* partials, int n) {
+    constexpr int U = RED_GRID / RED_BLOCK;
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {  // all loads in flight before the adds
+        const int i = threadIdx.x + u * RED_BLOCK;
+        v[u] = i < n ? partials[i] : 0.0;
+    }
     double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s += partials[i];
+#pragma unroll
+    for (int u = 0; u < U; ++u) s += v[u];
     return block_sum(s);
 }
 

@@ -9,7 +9,8 @@ import torch
 
 from .. import _native as nat
 from ..sparse.tensors import COOTensor, CSRTensor
-from .integrators import GroupIntegrator, Integrator, _axpby_sum, _call_assembly
+from .integrators import GroupIntegrator, Integrator, _axpby_sum, _call_assembly, p1_fusable_ops, \
+    p1_matrix_values, p1_source_fusable, p1_vector_values
 from .pattern import pattern_of
 
 
@@ -108,6 +109,10 @@ class BilinearForm(Form):
         if format not in ('csr', 'coo'):
             raise ValueError(f"Unsupported format {format}.")
         space = self._spaces[0]
+        fused = self._p1_fused(space, dtype)
+        if fused is not None:
+            self._M = fused if format == 'csr' else fused.tocoo()
+            return self._M
         full_map = space.cell_to_dof()
         total = None
         values = None
@@ -128,6 +133,24 @@ class BilinearForm(Form):
             raise RuntimeError("no integrator was added to the form")
         self._M = values if format == 'csr' else values.tocoo()
         return self._M
+
+    def _p1_fused(self, space, dtype):
+        """P1 scalar diffusion/mass groups over all cells: one fused kernel per
+        group (fa_assemble_p1_matrix), no element buffer."""
+        if space.p != 1 or getattr(space, 'TD', None) not in (2, 3):
+            return None
+        groups = [p1_fusable_ops(it) for it in self.integrators.values()]
+        if not groups or any(g is None for g in groups):
+            return None
+        pat = pattern_of(space)
+        if pat.max_row > 128:
+            return None
+        dtype = torch.float64 if dtype is None else dtype
+        values = None
+        for ops in groups:
+            v = p1_matrix_values(space, ops, pat, dtype)
+            values = v if values is None else _axpby_sum(values, v)
+        return CSRTensor(pat.indptr, pat.indices, values, self.sparse_shape)
 
     def __matmul__(self, u):
         if self._M is None:
@@ -158,7 +181,15 @@ class LinearForm(Form):
         space = self._spaces[0]
         full_map = space.cell_to_dof()
         out = None
-        for elem, e2d in self._groups(dtype):
+        ints = list(self.integrators.values())
+        if space.p == 1 and getattr(space, 'TD', None) in (2, 3) and ints and all(map(p1_source_fusable, ints)):
+            pat = pattern_of(space)
+            dt = torch.float64 if dtype is None else dtype
+            for it in ints:                         # fused P1 load vector, no element buffer
+                v = p1_vector_values(space, it, pat, dt)
+                out = v if out is None else _axpby_sum(out, v)
+            ints = []
+        for elem, e2d in (self._groups(dtype) if ints else []):
             pat = pattern_of(space) if (e2d.shape == full_map.shape and e2d.data_ptr() == full_map.data_ptr()) \
                 else pattern_of(space, e2d)
             v = pat.gather_vector(elem)

@@ -280,3 +280,60 @@ class ScalarSourceIntegrator(_ScalarCellInt):
                                        nat.ptr(cell), nc, nat.ptr(qw), nat.ptr(phi), kind, scale, nat.ptr(src),
                                        nat.ptr(elem), nat.stream()), "fa_element_vectors")
         return elem
+
+
+# ---- fused P1 path (element rows computed inside the CSR-row kernel) ------------
+def p1_fusable_ops(it):
+    """[(op, integrator)] when `it` is a scalar diffusion/mass integrator (or a
+    group of them) over all cells, so its P1 matrix can be assembled by
+    fa_assemble_p1_matrix; else None."""
+    ints = it.ints if isinstance(it, GroupIntegrator) else [it]
+    if any(i.get_region() is not None or getattr(i, 'batched', False) for i in ints):
+        return None
+    if isinstance(it, GroupIntegrator):
+        return _fused_scalar_group(ints)
+    if type(it) is ScalarDiffusionIntegrator:
+        return [('diff', it)]
+    if type(it) is ScalarMassIntegrator:
+        return [('mass', it)]
+    return None
+
+
+def p1_matrix_values(space, ops, pat, dtype):
+    """CSR values of one fusable group on the space's pattern (P1)."""
+    L = nat.lib()
+    mesh = space.mesh
+    q = ops[0][1].q if ops[0][1].q is not None else space.p + 3
+    _, qw, phi, _ = space.tables(q)
+    cell = space.cell_to_dof()
+    spec = {'diff': (nat.COEF_NONE, 0.0, None), 'mass': (nat.COEF_NONE, 0.0, None)}
+    for op, it in ops:
+        spec[op] = coef_spec(it.coef, space, q)
+    (dk, ds, dc), (mk, ms, mc) = spec['diff'], spec['mass']
+    values = torch.empty(pat.nnz, dtype=dtype, device=space.device)
+    nat.check(L.fa_assemble_p1_matrix(nat.dtype_code(dtype), space.TD, pat.gdof, qw.shape[0], nat.ptr(mesh.node),
+                                      nat.ptr(cell), nat.ptr(qw), nat.ptr(phi), dk, ds, nat.ptr(dc), mk, ms,
+                                      nat.ptr(mc), nat.ptr(pat.adj_ptr), nat.ptr(pat.adj), nat.ptr(pat.indptr),
+                                      nat.ptr(pat.indices), max(pat.max_row, 1), nat.ptr(values), nat.stream()),
+              "fa_assemble_p1_matrix")
+    return values
+
+
+def p1_source_fusable(it):
+    return type(it) is ScalarSourceIntegrator and it.get_region() is None and not it.batched
+
+
+def p1_vector_values(space, it, pat, dtype):
+    L = nat.lib()
+    mesh = space.mesh
+    q = it.q if it.q is not None else space.p + 3
+    _, qw, phi, _ = space.tables(q)
+    kind, scale, src = coef_spec(it.source, space, q)
+    if it.source is None:
+        kind, scale, src = nat.COEF_CONST, 1.0, None
+    out = torch.empty(pat.gdof, dtype=dtype, device=space.device)
+    nat.check(L.fa_assemble_p1_vector(nat.dtype_code(dtype), space.TD, pat.gdof, qw.shape[0], nat.ptr(mesh.node),
+                                      nat.ptr(space.cell_to_dof()), nat.ptr(qw), nat.ptr(phi), kind, scale,
+                                      nat.ptr(src), nat.ptr(pat.adj_ptr), nat.ptr(pat.adj), nat.ptr(out),
+                                      nat.stream()), "fa_assemble_p1_vector")
+    return out

@@ -126,6 +126,30 @@ int fa_gather_vector(int dtype, int64_t gdof, int32_t ldof,
                      const int32_t* adj_ptr, const int32_t* adj,
                      const void* elem, void* out, fa_stream_t stream);
 
+/* Fused P1 matrix assembly: geometry, element rows and the fixed-order CSR sum
+ * in one kernel, no element buffer.  Diffusion and/or mass terms with the
+ * coefficient kinds of fa_element_matrices; `phi` is the P1 basis at the nq
+ * points (needed for mass).  Every CSR entry is the sequential sum of its cell
+ * contributions in ascending cell order.  Replaces, for p = 1,
+ * BilinearForm.assembly (fealpy/fem/bilinear_form.py:46-105) over
+ * ScalarDiffusionIntegrator / ScalarMassIntegrator.assembly
+ * (scalar_diffusion_integrator.py:54-63, scalar_mass_integrator.py:41-54)
+ * and COOTensor.coalesce().tocsr() (sparse/coo_tensor.py:137-213).
+ * max_row <= 128. */
+int fa_assemble_p1_matrix(int dtype, int td, int64_t gdof, int nq, const double* node,
+                          const int32_t* cell, const double* qw, const double* phi,
+                          int diff_kind, double diff_scale, const double* diff_coef,
+                          int mass_kind, double mass_scale, const double* mass_coef,
+                          const int32_t* adj_ptr, const int32_t* adj,
+                          const int32_t* indptr, const int32_t* indices, int32_t max_row,
+                          void* values, fa_stream_t stream);
+/* Fused P1 load vector (ScalarSourceIntegrator + LinearForm.assembly,
+ * fealpy/fem/scalar_source_integrator.py:44-58, linear_form.py:36-86). */
+int fa_assemble_p1_vector(int dtype, int td, int64_t gdof, int nq, const double* node,
+                          const int32_t* cell, const double* qw, const double* phi,
+                          int src_kind, double src_scale, const double* src,
+                          const int32_t* adj_ptr, const int32_t* adj, void* out, fa_stream_t stream);
+
 /* Physical points of barycentric quadrature points on every cell:
  * out[c][q][d] = sum_v bcs[q][v] node[cell[c][v]][d]  (FP64; (gd, nv) in
  * {(2, 3), (3, 3), (3, 4)}).

@@ -230,3 +230,36 @@ def test_boundary_data_cached_per_space():
     ref_flag = space.is_boundary_dof(threshold=lambda p: torch.ones(p.shape[0], dtype=torch.bool, device=p.device),
                                      method='interp')
     assert torch.equal(space.is_boundary_dof(), ref_flag)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_fused_p1_matches_element_gather(dim):
+    """The fused P1 kernels (element rows computed inside the CSR-row sum)
+    against the element-tensor + fixed-order gather path, diffusion + mass with
+    a per-(cell, point) coefficient and a per-cell source."""
+    from fealpy_amd.fem.pattern import pattern_of
+    if dim == 2:
+        mesh = fa.TriangleMesh.from_box(nx=9, ny=7, device=DEV)
+    else:
+        mesh = fa.TetrahedronMesh.from_box(nx=5, ny=4, nz=6, device=DEV)
+    space = fa.LagrangeFESpace(mesh, 1)
+    nc = mesh.number_of_cells()
+    g = np.random.default_rng(3)
+    nq = space.tables(3)[1].shape[0]
+    kq = torch.as_tensor(np.exp(g.standard_normal((nc, nq))), device=DEV)
+    kc = torch.as_tensor(np.exp(g.standard_normal(nc)), device=DEV)
+    di, ma = fa.ScalarDiffusionIntegrator(kq, q=3), fa.ScalarMassIntegrator(kc, q=3)
+    A = fa.BilinearForm(space).add_integrator(di, ma).assembly()
+    pat = pattern_of(space)
+    elem = fa.fem.integrators.GroupIntegrator(di, ma).assembly(space)
+    ref = pat.gather_matrix(elem)
+    assert torch.equal(A.crow, pat.indptr) and torch.equal(A.col, pat.indices)
+    scale = float(ref.abs().max())
+    assert float((A.values - ref).abs().max()) <= 1e-13 * scale
+    src = fa.ScalarSourceIntegrator(kc, q=3)
+    F = fa.LinearForm(space).add_integrator(src).assembly()
+    Fref = pat.gather_vector(src.assembly(space))
+    assert float((F - Fref).abs().max()) <= 1e-13 * float(Fref.abs().max())
+    A32 = fa.BilinearForm(space).add_integrator(di, ma).assembly(dtype=torch.float32)
+    assert A32.values.dtype == torch.float32
+    assert float((A32.values.double() - ref).abs().max()) <= 1e-6 * scale
