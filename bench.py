@@ -28,6 +28,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of k_sell_spmv<float> on the 128^3 operator: FETCH_SIZE
+# and WRITE_SIZE counter passes over `bench.py --spmv-only`, each scaled by the
+# same counter's reading for a 256 MiB device copy (profiles/sell_spmv_f32_128cube_counters.md)
+SPMV_F32_TRAFFIC_128 = 294.3e6
 
 
 def parse():
@@ -246,6 +250,8 @@ def main():
     phases_d = phase_breakdown(space, k, u, f, mg64, args.rtol)
     err_l2 = float(mesh.error(u, space.function(x_m)))
     roof = spmv_roofline(A2, torch.float32)
+    if args.n == 128:
+        roof["traffic"] = SPMV_F32_TRAFFIC_128
     roof64 = spmv_roofline(A2, torch.float64)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n)
 
