@@ -1,15 +1,17 @@
 """fealpy_amd -- MI355X-native Lagrange finite-element assemble-and-solve.
 
 Drop-in for the path of weihuayi/fealpy that goes
-LagrangeFESpace -> BilinearForm/LinearForm (diffusion, mass, source) ->
+LagrangeFESpace / TensorFunctionSpace -> BilinearForm/LinearForm (diffusion,
+mass, source, linear elasticity, vector source) ->
 DirichletBC -> CSRTensor -> cg, on torch device tensors, with every numerical
 step in hand-written HIP kernels for gfx950 (libfealpy_amd.so, C-ABI in
 include/fealpy_amd.h).
 """
 from .decorator import barycentric, cartesian
-from .fem import BilinearForm, DirichletBC, GroupIntegrator, LinearForm, ScalarDiffusionIntegrator, \
-    ScalarMassIntegrator, ScalarSourceIntegrator
-from .functionspace import Function, LagrangeFESpace
+from .fem import BilinearForm, DirichletBC, GroupIntegrator, LinearElasticityIntegrator, LinearForm, \
+    ScalarDiffusionIntegrator, ScalarMassIntegrator, ScalarSourceIntegrator, VectorSourceIntegrator
+from .functionspace import Function, LagrangeFESpace, TensorFunctionSpace
+from .material import LinearElasticMaterial
 from .mesh import TetrahedronMesh, TriangleMesh
 from .solver import cg
 from .sparse import COOTensor, CSRTensor

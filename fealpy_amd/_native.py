@@ -40,6 +40,11 @@ _SIGNATURES = {
                                           _p, _p, ctypes.c_int, _f64, _p, _p, _p]),
     "fa_gather_matrix": (ctypes.c_int, [ctypes.c_int, _i64, _i32, _p, _p, _p, _p, _p, _i32, _p, _p, _p]),
     "fa_gather_vector": (ctypes.c_int, [ctypes.c_int, _i64, _i32, _p, _p, _p, _p, _p]),
+    "fa_element_elasticity": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _p, _p, _i64, _p,
+                                             ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_int, _p, _p]),
+    "fa_element_vectors_tensor": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                 ctypes.c_int, ctypes.c_int, _p, _p, _i64, _p, _p, _p, _p, _p]),
     "fa_assemble_p1_matrix": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64, ctypes.c_int, _p, _p, _p, _p,
                                              ctypes.c_int, ctypes.c_double, _p, ctypes.c_int, ctypes.c_double, _p,
                                              _p, _p, _p, _p, _i32, _p, _p]),

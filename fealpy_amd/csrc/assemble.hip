@@ -284,6 +284,101 @@ __global__ void k_bc_to_point(int64_t nc, int nq, const double* __restrict__ nod
     }
 }
 
+// ---- linear elasticity (vector Lagrange, isotropic) --------------------------
+// A_ab[i][j] = |T| sum_q w_q d_a phi_i d_b phi_j
+//            = |T| sum_{m,n} gl[m][a] gl[n][b] M[i][j][m][n],
+// M[i][j][m][n] = sum_q w_q R[q][i][m] R[q][j][n] precontracted on the host
+// (R = d phi / d lambda).  Element entry of (i, a), (j, b):
+//   a == b: cA A_aa + cB (sum of the other diagonal blocks, in axis order)
+//   a != b: cC A_ab + cD A_ba
+// laid out as i*TD + a (dof_priority = 0) or a*LDOF + i (dof_priority = 1).
+// One thread per (cell, i) writes the TD rows of scalar DOF i.
+template <int TD, int LDOF, class T>
+__global__ void __launch_bounds__(256) k_element_elasticity(const double* __restrict__ node,
+                                                           const int32_t* __restrict__ cell, int64_t nc,
+                                                           const double* __restrict__ M, double cA, double cB,
+                                                           double cC, double cD, int dof_priority,
+                                                           T* __restrict__ elem) {
+    constexpr int NV = TD + 1;
+    constexpr int N = TD * LDOF;
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= nc * LDOF) return;
+    const int64_t c = t / LDOF;
+    const int i = static_cast<int>(t - c * LDOF);
+    double p[NV][TD];
+    load_cell<TD>(node, cell, c, p);
+    Geom<TD> g;
+    g.compute(p);
+    T* out = elem + c * (int64_t)N * N;
+    for (int j = 0; j < LDOF; ++j) {
+        const double* Mij = M + ((int64_t)i * LDOF + j) * NV * NV;
+        double tm[NV][TD];
+#pragma unroll
+        for (int m = 0; m < NV; ++m)
+#pragma unroll
+            for (int b = 0; b < TD; ++b) {
+                double s = 0.0;
+#pragma unroll
+                for (int n = 0; n < NV; ++n) s += Mij[m * NV + n] * g.gl[n][b];
+                tm[m][b] = s;
+            }
+        double A[TD][TD];
+#pragma unroll
+        for (int a = 0; a < TD; ++a)
+#pragma unroll
+            for (int b = 0; b < TD; ++b) {
+                double s = 0.0;
+#pragma unroll
+                for (int m = 0; m < NV; ++m) s += g.gl[m][a] * tm[m][b];
+                A[a][b] = g.meas * s;
+            }
+#pragma unroll
+        for (int a = 0; a < TD; ++a) {
+            const int row = dof_priority ? a * LDOF + i : i * TD + a;
+#pragma unroll
+            for (int b = 0; b < TD; ++b) {
+                double v;
+                if (a == b) {
+                    double others = 0.0;
+#pragma unroll
+                    for (int e = 0; e < TD; ++e)
+                        if (e != a) others += A[e][e];
+                    v = cA * A[a][a] + cB * others;
+                } else {
+                    v = cC * A[a][b] + cD * A[b][a];
+                }
+                const int col = dof_priority ? b * LDOF + j : j * TD + b;
+                out[(int64_t)row * N + col] = static_cast<T>(v);
+            }
+        }
+    }
+}
+
+// vector load: out[c][I(i, a)] = |T| sum_q w_q phi_i(q) f[c][q][a], f (NC, NQ, ncomp)
+template <int TD, class T>
+__global__ void __launch_bounds__(256) k_element_vectors_tensor(const double* __restrict__ node,
+                                                               const int32_t* __restrict__ cell, int64_t nc,
+                                                               int ldof, int ncomp, int dof_priority, int nq,
+                                                               const double* __restrict__ qw,
+                                                               const double* __restrict__ phi,
+                                                               const double* __restrict__ src,
+                                                               T* __restrict__ elem) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= nc * ldof) return;
+    const int64_t c = t / ldof;
+    const int i = static_cast<int>(t - c * ldof);
+    double p[TD + 1][TD];
+    load_cell<TD>(node, cell, c, p);
+    Geom<TD> g;
+    g.compute(p);
+    T* out = elem + c * (int64_t)ldof * ncomp;
+    for (int a = 0; a < ncomp; ++a) {
+        double acc = 0.0;
+        for (int q = 0; q < nq; ++q) acc += qw[q] * g.meas * src[(c * nq + q) * ncomp + a] * phi[(int64_t)q * ldof + i];
+        out[dof_priority ? a * ldof + i : i * ncomp + a] = static_cast<T>(acc);
+    }
+}
+
 // ---- fused P1 assembly: element rows computed where they are summed ---------
 // P1 gradients are constant on a cell, so the element row i of cell c is
 //   A_ij = kd |T| (gl_i . gl_j) + |T| sum_q w_q k_q phi_i(q) phi_j(q),
@@ -730,5 +825,52 @@ extern "C" int fa_assemble_p1_vector(int dtype, int td, int64_t gdof, int nq, co
     }
 #undef FA_V
     FA_LAUNCH("k_assemble_p1_vec");
+    return FA_OK;
+}
+
+extern "C" int fa_element_elasticity(int dtype, int td, int ldof, const double* node, const int32_t* cell, int64_t nc,
+                                     const double* M, double cA, double cB, double cC, double cD, int dof_priority,
+                                     void* elem, fa_stream_t stream) {
+    FA_REQUIRE(dtype == FA_F64 || dtype == FA_F32, "fa_element_elasticity: bad dtype %d", dtype);
+    FA_REQUIRE(M && nc >= 0, "fa_element_elasticity: bad arguments");
+    if (nc == 0) return FA_OK;
+    hipStream_t s = S(stream);
+    const unsigned grid = (unsigned)((nc * ldof + 255) / 256);
+#define FA_EL(TD_, L_)                                                                                         \
+    if (td == TD_ && ldof == L_) {                                                                             \
+        if (dtype == FA_F64)                                                                                   \
+            hipLaunchKernelGGL((k_element_elasticity<TD_, L_, double>), dim3(grid), dim3(256), 0, s, node, cell, \
+                               nc, M, cA, cB, cC, cD, dof_priority, static_cast<double*>(elem));               \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_element_elasticity<TD_, L_, float>), dim3(grid), dim3(256), 0, s, node, cell,  \
+                               nc, M, cA, cB, cC, cD, dof_priority, static_cast<float*>(elem));                \
+        FA_LAUNCH("k_element_elasticity");                                                                     \
+        return FA_OK;                                                                                          \
+    }
+    FA_EL(2, 3) FA_EL(2, 6) FA_EL(2, 10) FA_EL(3, 4) FA_EL(3, 10) FA_EL(3, 20)
+#undef FA_EL
+    set_error("fa_element_elasticity: unsupported (td=%d, ldof=%d); p must be 1..3", td, ldof);
+    return FA_EINVAL;
+}
+
+extern "C" int fa_element_vectors_tensor(int dtype, int td, int ldof, int ncomp, int dof_priority, int nq,
+                                         const double* node, const int32_t* cell, int64_t nc, const double* qw,
+                                         const double* phi, const double* src, void* elem, fa_stream_t stream) {
+    FA_REQUIRE(dtype == FA_F64 || dtype == FA_F32, "fa_element_vectors_tensor: bad dtype %d", dtype);
+    FA_REQUIRE(td == 2 || td == 3, "fa_element_vectors_tensor: td must be 2 or 3");
+    FA_REQUIRE(src && ncomp >= 1 && nq >= 1, "fa_element_vectors_tensor: bad arguments");
+    if (nc == 0) return FA_OK;
+    hipStream_t s = S(stream);
+    const unsigned grid = (unsigned)((nc * ldof + 255) / 256);
+#define FA_VT(TD_, T_)                                                                                          \
+    hipLaunchKernelGGL((k_element_vectors_tensor<TD_, T_>), dim3(grid), dim3(256), 0, s, node, cell, nc, ldof,   \
+                       ncomp, dof_priority, nq, qw, phi, src, static_cast<T_*>(elem))
+    if (td == 3) {
+        if (dtype == FA_F64) FA_VT(3, double); else FA_VT(3, float);
+    } else {
+        if (dtype == FA_F64) FA_VT(2, double); else FA_VT(2, float);
+    }
+#undef FA_VT
+    FA_LAUNCH("k_element_vectors_tensor");
     return FA_OK;
 }

@@ -126,6 +126,24 @@ int fa_gather_vector(int dtype, int64_t gdof, int32_t ldof,
                      const int32_t* adj_ptr, const int32_t* adj,
                      const void* elem, void* out, fa_stream_t stream);
 
+/* Element matrices of LinearElasticityIntegrator on a vector Lagrange space
+ * (fealpy/fem/linear_elasticity_integrator.py:60-186, isotropic D of
+ * material/elastic_material.py:236-249).  `M` is the precontracted
+ * reference tensor M[i][j][m][n] = sum_q w_q R[q][i][m] R[q][j][n]
+ * (ldof*ldof*(td+1)^2 doubles, R = d phi / d lambda).  Entry of rows (i, a),
+ * columns (j, b): cA A_aa + cB sum_{e != a} A_ee on diagonal blocks,
+ * cC A_ab + cD A_ba off them, with A_ab = |T| sum_q w_q d_a phi_i d_b phi_j.
+ * Layout i*td + a (dof_priority 0, TensorFunctionSpace shape (-1, td)) or
+ * a*ldof + i (dof_priority 1).  elem: (nc, td*ldof, td*ldof).  p <= 3. */
+int fa_element_elasticity(int dtype, int td, int ldof, const double* node, const int32_t* cell, int64_t nc,
+                          const double* M, double cA, double cB, double cC, double cD, int dof_priority,
+                          void* elem, fa_stream_t stream);
+/* Element vectors of VectorSourceIntegrator (fealpy/fem/vector_source_integrator.py:45-52):
+ * elem[c][I(i, a)] = |T| sum_q w_q phi_i(q) src[c][q][a], src (nc, nq, ncomp) FP64. */
+int fa_element_vectors_tensor(int dtype, int td, int ldof, int ncomp, int dof_priority, int nq,
+                              const double* node, const int32_t* cell, int64_t nc, const double* qw,
+                              const double* phi, const double* src, void* elem, fa_stream_t stream);
+
 /* Fused P1 matrix assembly: geometry, element rows and the fixed-order CSR sum
  * in one kernel, no element buffer.  Diffusion and/or mass terms with the
  * coefficient kinds of fa_element_matrices; `phi` is the P1 basis at the nq

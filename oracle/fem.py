@@ -102,6 +102,71 @@ def source_element(mesh, p, source, q=None):
     return linear_integral(phi, ws, cm, f)
 
 
+def tensor_cell2dof(cell2dof, numel, gdof, dof_priority):
+    """to_tensor_dof: fealpy/functionspace/utils.py:65-95 (array branch)."""
+    idx = np.arange(gdof * numel)
+    if dof_priority:
+        idx = idx.reshape(numel, gdof)[:, cell2dof].swapaxes(0, 1)
+    else:
+        idx = idx.reshape(gdof, numel)[cell2dof, :]
+    return idx.reshape(cell2dof.shape[0], -1)
+
+
+def elasticity_element(mesh, p, lam, mu, q=None, dof_priority=False):
+    """LinearElasticityIntegrator.assembly for simplices:
+    fealpy/fem/linear_elasticity_integrator.py:60-156, isotropic D of
+    fealpy/material/elastic_material.py:236-249 ('3D' / 'plane_strain').
+    Blocks A_ab[c,i,j] = sum_q w_q |T| d_a phi_i d_b phi_j; the entry of
+    (i, a), (j, b) is c_aa(A_aa, others) on diagonal blocks and
+    lam A_ab + mu A_ba off the diagonal."""
+    q = p + 3 if q is None else q
+    bcs, ws = mesh.quadrature_formula(q)
+    cm = mesh.entity_measure('cell')
+    gphi = mesh.grad_shape_function(bcs, p, 'x')            # (NC, NQ, ldof, GD)
+    NC, _, ldof, GD = gphi.shape
+    A = [[np.einsum('q, cqi, cqj, c -> cij', ws, gphi[..., a], gphi[..., b], cm, optimize=True)
+          for b in range(GD)] for a in range(GD)]
+    KK = np.zeros((NC, GD * ldof, GD * ldof))
+
+    def put(a, b, val):
+        if dof_priority:
+            KK[:, a * ldof:(a + 1) * ldof, b * ldof:(b + 1) * ldof] = val
+        else:
+            KK[:, a::GD, b::GD] = val
+    if GD == 3:
+        D00, D01, D55 = 2 * mu + lam, lam, mu
+        dd = D00 if dof_priority else 2 * D55 + D01       # linear_elasticity_integrator.py:130-165
+        put(0, 0, dd * A[0][0] + D55 * (A[1][1] + A[2][2]))
+        put(1, 1, dd * A[1][1] + D55 * (A[0][0] + A[2][2]))
+        put(2, 2, dd * A[2][2] + D55 * (A[0][0] + A[1][1]))
+        for a in range(3):
+            for b in range(3):
+                if a != b:
+                    put(a, b, D01 * A[a][b] + D55 * A[b][a])
+    else:
+        D00, D01, D22 = 2 * mu + lam, lam, mu
+        put(0, 0, D00 * A[0][0] + D22 * A[1][1])
+        put(1, 1, D00 * A[1][1] + D22 * A[0][0])
+        put(0, 1, D01 * A[0][1] + D22 * A[1][0])
+        put(1, 0, D01 * A[1][0] + D22 * A[0][1])
+    return KK
+
+
+def vector_source_element(mesh, p, source, q=None, dof_priority=False):
+    """VectorSourceIntegrator.assembly: fealpy/fem/vector_source_integrator.py:45-52
+    -> linear_integral with a tensor basis (functional.py:27-65,
+    functionspace/functional.py:13-40); source values (NC, NQ, GD)."""
+    q = p + 3 if q is None else q
+    bcs, ws = mesh.quadrature_formula(q)
+    cm = mesh.entity_measure('cell')
+    phi = mesh.shape_function(bcs, p)                       # (NQ, ldof)
+    f = process_coef(source, bcs, mesh)                     # (NC, NQ, GD)
+    r = np.einsum('c, q, qi, cqd -> cid', cm, ws, phi, f, optimize=True)
+    if dof_priority:
+        r = r.swapaxes(1, 2)
+    return r.reshape(r.shape[0], -1)
+
+
 class CSR:
     """Minimal CSR triple (int64 indptr, int32/int64 indices, float64 values)."""
 
