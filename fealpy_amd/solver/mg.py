@@ -1,10 +1,14 @@
-"""Geometric multigrid preconditioner for P1 problems on nested ``from_box`` meshes.
+"""Geometric multigrid preconditioner on nested ``from_box`` meshes.
 
 Role of the reference's ``GAMGSolver`` (fealpy/solver/gamg_solver.py:66-301):
 a hierarchy of operators A_l, transfers P_l / R_l = P_l^T, V-cycles used as
 the preconditioner of CG.  Designed for the GPU:
 
-* levels are the ``from_box`` meshes with every cell count halved; the Kuhn /
+* for a degree-p space the first coarse level is P1 on the same mesh
+  (interpolation of P1 into Pp, exact; the p-transfer of
+  lagrange_fe_space.py:237-258); vector spaces apply every transfer per
+  component;
+* geometric levels are the ``from_box`` meshes with every cell count halved; the Kuhn /
   diagonal-split triangulations are nested under halving, so P1 prolongation
   is closed form: a fine node with grid index g takes 1/2 of the coarse
   nodes floor(g/2) and ceil(g/2) (one node with weight 1 when g is even);
@@ -64,8 +68,10 @@ def _grid_index(dims, ids):
     return torch.stack(out[::-1], dim=-1)
 
 
-def _prolongation(dims_f, bd_f, bd_c, device):
-    """CSR (int32/FP64) of P1 prolongation coarse -> fine for nested grids."""
+def _grid_prolongation(dims_f, device):
+    """Scalar P1 prolongation coarse -> fine for nested grids as (rows, cols,
+    vals): a fine node at grid index g takes coarse nodes floor(g/2) and
+    ceil(g/2) per axis with weight 1/2 each (1 when g is even)."""
     dims_c = [n // 2 for n in dims_f]
     nf = 1
     for n in dims_f:
@@ -79,14 +85,58 @@ def _prolongation(dims_f, bd_f, bd_c, device):
     cols = torch.cat([c_lo, c_hi[~same]])
     vals = torch.cat([torch.where(same, 1.0, 0.5).to(torch.float64), torch.full((int((~same).sum()),), 0.5,
                                                                              dtype=torch.float64, device=device)])
+    return rows, cols, vals
+
+
+def _order_prolongation(space):
+    """Scalar P1 -> Pp interpolation on one mesh as (rows, cols, vals): the
+    Pp DOF at barycentric point alpha/p of a cell takes alpha_v/p of the cell's
+    vertex v (P1 is contained in Pp, so the transfer is exact).  Each DOF's row
+    comes from its first cell (every cell gives the same row)."""
+    mesh = space.mesh
+    c2d = space.cell_to_dof().long()
+    cell = mesh.cell.long()
+    nc, ldof = c2d.shape
+    gdof = space.number_of_global_dofs()
+    dev = c2d.device
+    first = torch.full((gdof,), nc * ldof, dtype=torch.int64, device=dev)
+    first.scatter_reduce_(0, c2d.reshape(-1), torch.arange(nc * ldof, device=dev), reduce='amin')
+    rc, ri = first // ldof, first % ldof
+    mi = torch.as_tensor(space.multi_index_matrix(), dtype=torch.float64, device=dev)     # (ldof, TD+1)
+    w = mi[ri] / space.p                                                                  # (gdof, TD+1)
+    rows = torch.arange(gdof, device=dev)[:, None].expand_as(w).reshape(-1)
+    cols = cell[rc].reshape(-1)
+    vals = w.reshape(-1)
+    keep = vals != 0
+    return rows[keep], cols[keep], vals[keep]
+
+
+def _block_prolongation(rows, cols, vals, k, prio, nf, nc, bd_f, bd_c):
+    """Expand a scalar transfer to k components (component-interleaved, or
+    component blocks when prio) and drop fine Dirichlet rows / coarse
+    Dirichlet columns; -> CSR sorted by (row, col)."""
+    if k > 1:
+        comp = torch.arange(k, device=rows.device)
+        if prio:
+            rows = (comp[:, None] * nf + rows[None, :]).reshape(-1)
+            cols = (comp[:, None] * nc + cols[None, :]).reshape(-1)
+        else:
+            rows = (rows[None, :] * k + comp[:, None]).reshape(-1)
+            cols = (cols[None, :] * k + comp[:, None]).reshape(-1)
+        vals = vals[None, :].expand(k, -1).reshape(-1)
     keep = ~bd_f[rows] & ~bd_c[cols]
     rows, cols, vals = rows[keep], cols[keep], vals[keep]
-    order = torch.argsort(rows * (cols.max() + 1 if cols.numel() else 1) + cols)
-    rows, cols, vals = rows[order], cols[order], vals[order]
-    nc = 1
-    for n in dims_c:
-        nc *= n + 1
-    return _csr(rows, cols, vals, nf, nc)
+    order = torch.argsort(rows * (k * nc) + cols)
+    return _csr(rows[order], cols[order], vals[order], k * nf, k * nc)
+
+
+def _select_nodes(bd, k, prio, n_scalar, ids):
+    """Flags of the scalar nodes `ids` for every component."""
+    if k == 1:
+        return bd[ids]
+    b = bd.reshape(k, n_scalar) if prio else bd.reshape(n_scalar, k).T
+    sel = b[:, ids]
+    return sel.reshape(-1) if prio else sel.T.reshape(-1)
 
 
 def _csr(rows, cols, vals, nrows, ncols):
@@ -111,11 +161,19 @@ class _Level:
 
 
 class GeometricMG:
+    """V-cycle preconditioner.  ``space`` is a Lagrange space (any p) or a
+    TensorFunctionSpace over one, on a ``from_box`` mesh; ``is_bd`` flags the
+    Dirichlet DOFs of that space.  For p > 1 the first transfer goes to P1 on
+    the same mesh; vector spaces coarsen every component alike."""
+
     def __init__(self, space, is_bd, *, dtype=torch.float32, degree=2, lower=0.1, upper=1.0, coarse_max=64,
                  power_iters=12):
-        if space.p != 1:
-            raise NotImplementedError("GeometricMG covers P1 spaces")
-        mesh = space.mesh
+        ss = getattr(space, 'scalar_space', space)
+        k = int(getattr(space, 'dof_numel', 1))
+        prio = bool(getattr(space, 'dof_priority', False))
+        if getattr(space, 'dof_ndim', 1) != 1:
+            raise NotImplementedError("GeometricMG covers scalar and vector spaces")
+        mesh = ss.mesh
         dims = _grid_shape(mesh)
         dev = space.device
         self.dtype, self.degree, self.lower, self.upper = dtype, degree, lower, upper
@@ -123,24 +181,33 @@ class GeometricMG:
         self.device = dev
         self.levels = []
         bd = is_bd.to(torch.bool)
+        if bd.shape[0] != space.number_of_global_dofs():
+            raise ValueError("is_bd must flag every DOF of the space")
         lev = _Level()
-        lev.dims, lev.n, lev.bd = dims, space.number_of_global_dofs(), bd
+        lev.n, lev.bd = space.number_of_global_dofs(), bd
         self.levels.append(lev)
+        if ss.p > 1:                                    # Pp -> P1 on the same mesh
+            nn = mesh.number_of_nodes()
+            ns = ss.number_of_global_dofs()
+            bd_c = _select_nodes(bd, k, prio, ns, torch.arange(nn, device=dev))
+            self._link(lev, _block_prolongation(*_order_prolongation(ss), k, prio, ns, nn, bd, bd_c))
+            nxt = _Level()
+            nxt.n, nxt.bd = k * nn, bd_c
+            self.levels.append(nxt)
+            lev, bd = nxt, bd_c
+        lev.dims = dims
         while all(n % 2 == 0 and n >= 2 for n in dims) and lev.n > coarse_max:
             dims_c = [n // 2 for n in dims]
-            nc = 1
+            nf_s, nc_s = 1, 1
+            for n in dims:
+                nf_s *= n + 1
             for n in dims_c:
-                nc *= n + 1
-            g = _grid_index(dims_c, torch.arange(nc, device=dev))
-            bd_c = bd[_node_ids(dims, 2 * g)]
-            lev.P = _prolongation(dims, bd, bd_c, dev)
-            lev.R = _transpose(lev.P)
-            lev.Ps = SellMatrix(lev.P.crow, lev.P.col, lev.P.shape)
-            lev.Rs = SellMatrix(lev.R.crow, lev.R.col, lev.R.shape)
-            lev.Ps.set_values(lev.P.values, dtype)
-            lev.Rs.set_values(lev.R.values, dtype)
+                nc_s *= n + 1
+            g = _grid_index(dims_c, torch.arange(nc_s, device=dev))
+            bd_c = _select_nodes(bd, k, prio, nf_s, _node_ids(dims, 2 * g))
+            self._link(lev, _block_prolongation(*_grid_prolongation(dims, dev), k, prio, nf_s, nc_s, bd, bd_c))
             nxt = _Level()
-            nxt.dims, nxt.n, nxt.bd = dims_c, nc, bd_c
+            nxt.dims, nxt.n, nxt.bd = dims_c, k * nc_s, bd_c
             self.levels.append(nxt)
             lev, dims, bd = nxt, dims_c, bd_c
         if self.levels[-1].n > 2048:
@@ -151,6 +218,14 @@ class GeometricMG:
         self._pattern_key = None
         self._desc = None
         self._setup_A = None
+
+    def _link(self, lev, P):
+        lev.P = P
+        lev.R = _transpose(P)
+        lev.Ps = SellMatrix(P.crow, P.col, P.shape)
+        lev.Rs = SellMatrix(lev.R.crow, lev.R.col, lev.R.shape)
+        lev.Ps.set_values(P.values, self.dtype)
+        lev.Rs.set_values(lev.R.values, self.dtype)
 
     # -- per-matrix setup -----------------------------------------------------
     def _symbolic(self, A):

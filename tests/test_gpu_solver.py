@@ -142,3 +142,38 @@ def test_cg_maxit_and_trivial_rhs():
     assert info["niter"] == 3
     z = fa.cg(A, torch.zeros_like(b))
     assert float(z.abs().max()) == 0.0
+
+
+def test_mg_pcg_p2_scalar():
+    """P2 -> P1 (same mesh) -> geometric levels."""
+    mesh, space, bc, A, b = _system(12, p=2)
+    k = _lognormal(mesh, seed=7)
+    _, _, _, A, b = _system(12, k=k, p=2)
+    x_ref = fa.cg(A, b, atol=1e-30, rtol=1e-13, maxit=50000)
+    mg = GeometricMG(space, bc.is_boundary_dof, dtype=torch.float32)
+    assert mg.levels[1].n == mesh.number_of_nodes()
+    x, info = fa.cg(A, b, M=mg, atol=1e-30, rtol=1e-11, maxit=300, returninfo=True)
+    err = float(torch.linalg.norm(x - x_ref)) / float(torch.linalg.norm(x_ref))
+    assert err < 1e-8, (err, info)
+    assert info["niter"] < 80, info
+
+
+@pytest.mark.parametrize("shape", [(-1, 3), (3, -1)])
+def test_mg_pcg_elasticity_p2(shape):
+    """Vector P2 elasticity, clamped: P2 -> P1 -> geometric, per component."""
+    n = 8
+    mesh = fa.TetrahedronMesh.from_box([0, 1, 0, 1, 0, 1], nx=n, ny=n, nz=n, device=DEV)
+    ts = fa.TensorFunctionSpace(fa.LagrangeFESpace(mesh, 2), shape)
+    mat = fa.LinearElasticMaterial("E", lame_lambda=1.0, shear_modulus=0.5)
+    A = fa.BilinearForm(ts).add_integrator(fa.LinearElasticityIntegrator(mat, q=3)).assembly()
+    F = fa.LinearForm(ts).add_integrator(fa.VectorSourceIntegrator(
+        torch.tensor([1.0, -2.0, 0.5], dtype=torch.float64, device=DEV), q=3)).assembly()
+    bc = fa.DirichletBC(ts, gd=0.0)
+    A, F = bc.apply(A, F)
+    x_ref = fa.cg(A, F, atol=1e-30, rtol=1e-13, maxit=50000)
+    mg = GeometricMG(ts, bc.is_boundary_dof, dtype=torch.float32)
+    assert mg.levels[1].n == 3 * mesh.number_of_nodes()
+    x, info = fa.cg(A, F, M=mg, atol=1e-30, rtol=1e-11, maxit=300, returninfo=True)
+    err = float(torch.linalg.norm(x - x_ref)) / float(torch.linalg.norm(x_ref))
+    assert err < 1e-8, (err, info)
+    assert info["niter"] < 120, info

@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 import torch
 
+import fealpy_amd as fa
 from fealpy_amd.functionspace import LagrangeFESpace
 from fealpy_amd.functionspace.basis import grad_shape_function, multi_index_matrix, shape_function
 from fealpy_amd.mesh import TetrahedronMesh, TriangleMesh
@@ -115,3 +116,38 @@ def test_product_refuses_cpu_tensors():
         pytest.skip("GPU present")
     with pytest.raises(_native.NativeError):
         _native.lib()
+
+
+@pytest.mark.parametrize("TD,p", [(2, 2), (3, 2), (3, 3)])
+def test_order_prolongation_interpolates_linears(TD, p):
+    """P1 -> Pp transfer of the multigrid: exact for linear functions."""
+    from fealpy_amd.solver.mg import _order_prolongation
+    from fealpy_amd.sparse.tensors import CSRTensor  # noqa: F401
+    mesh = (fa.TetrahedronMesh.from_box(nx=2, ny=3, nz=2, device=torch.device("cpu")) if TD == 3
+            else fa.TriangleMesh.from_box(nx=3, ny=2, device=torch.device("cpu")))
+    space = fa.LagrangeFESpace(mesh, p)
+    rows, cols, vals = _order_prolongation(space)
+    lin = lambda x: 0.3 + 1.7 * x[..., 0] - 0.4 * x[..., 1] + (2.1 * x[..., 2] if TD == 3 else 0.0)
+    u1 = lin(mesh.node)
+    up = torch.zeros(space.number_of_global_dofs(), dtype=torch.float64)
+    up.index_add_(0, rows, vals * u1[cols])
+    ref = lin(space.interpolation_points())
+    assert float((up - ref).abs().max()) < 1e-14
+    sums = torch.zeros_like(up).index_add_(0, rows, vals)
+    assert float((sums - 1).abs().max()) < 1e-15
+
+
+@pytest.mark.parametrize("prio", [False, True])
+def test_block_prolongation_layout(prio):
+    from fealpy_amd.solver.mg import _block_prolongation
+    rows = torch.tensor([0, 1, 1, 2]); cols = torch.tensor([0, 0, 1, 1])
+    vals = torch.tensor([1.0, 0.5, 0.5, 1.0], dtype=torch.float64)
+    k, nf, nc = 3, 3, 2
+    bd_f = torch.zeros(k * nf, dtype=torch.bool); bd_c = torch.zeros(k * nc, dtype=torch.bool)
+    P = _block_prolongation(rows, cols, vals, k, prio, nf, nc, bd_f, bd_c)
+    D = torch.zeros(k * nf, k * nc, dtype=torch.float64)
+    r = torch.repeat_interleave(torch.arange(k * nf), (P.crow[1:] - P.crow[:-1]).long())
+    D[r, P.col.long()] = P.values
+    S = torch.zeros(nf, nc, dtype=torch.float64); S[rows, cols] = vals
+    ref = torch.kron(torch.eye(k, dtype=torch.float64), S) if prio else torch.kron(S, torch.eye(k, dtype=torch.float64))
+    assert torch.equal(D, ref)
