@@ -44,6 +44,9 @@ def parse():
                     help="PCG relative residual; every run checks the solution error against 1e-8")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-n", type=int, default=40)
+    ap.add_argument("--workload", choices=("diffusion", "elasticity"), default="diffusion",
+                    help="diffusion: 3D P1 lognormal diffusion (the headline line); elasticity: vector P2 "
+                         "linear elasticity on from_box n^3 (n defaults to 40)")
     ap.add_argument("--spmv-only", action="store_true",
                     help="assemble once, then run only the FP32 SELL SpMV and a plain copy kernel of known size "
                          "(for hardware-counter runs)")
@@ -206,6 +209,91 @@ def cpu_baseline(n):
                       f"{t_solve:.2f}s (oracle, 1 thread; mesh build {t_mesh:.2f}s not counted)"}
 
 
+def make_elasticity(n, dev, lam=1.0, mu=0.5):
+    """Vector P2 on TetrahedronMesh.from_box n^3, lam = 1, mu = 0.5; body force
+    of the manufactured displacement u_i = (i+1) sin(pi x) sin(pi y) sin(pi z):
+    f = 3 pi^2 mu s c - (lam + mu) H(s) c with c = (1, 2, 3), H the Hessian of s.
+    u vanishes on the boundary (clamped)."""
+    import torch
+    import fealpy_amd as fa
+    mesh = fa.TetrahedronMesh.from_box([0, 1, 0, 1, 0, 1], nx=n, ny=n, nz=n, device=dev)
+    ts = fa.TensorFunctionSpace(fa.LagrangeFESpace(mesh, 2), (-1, 3))
+    mat = fa.LinearElasticMaterial("E", lame_lambda=lam, shear_modulus=mu)
+    c = torch.tensor([1.0, 2.0, 3.0], dtype=torch.float64, device=dev)
+    pi = np.pi
+
+    @fa.cartesian
+    def f(p):
+        sn, cs = torch.sin(pi * p), torch.cos(pi * p)
+        s = sn.prod(dim=-1)
+        hxy = pi ** 2 * cs[..., 0] * cs[..., 1] * sn[..., 2]
+        hxz = pi ** 2 * cs[..., 0] * sn[..., 1] * cs[..., 2]
+        hyz = pi ** 2 * sn[..., 0] * cs[..., 1] * cs[..., 2]
+        hc = torch.stack([-pi ** 2 * s * c[0] + hxy * c[1] + hxz * c[2],
+                          hxy * c[0] - pi ** 2 * s * c[1] + hyz * c[2],
+                          hxz * c[0] + hyz * c[1] - pi ** 2 * s * c[2]], dim=-1)
+        return 3 * pi ** 2 * mu * s[..., None] * c - (lam + mu) * hc
+
+    @fa.cartesian
+    def u(p):
+        return torch.sin(pi * p).prod(dim=-1)[..., None] * c
+
+    return mesh, ts, mat, f, u
+
+
+def elasticity_step(ts, mat, f, mg, rtol, maxit=1000):
+    import fealpy_amd as fa
+    A = fa.BilinearForm(ts).add_integrator(fa.LinearElasticityIntegrator(mat, q=5)).assembly()
+    F = fa.LinearForm(ts).add_integrator(fa.VectorSourceIntegrator(f, q=5)).assembly()
+    A2, F2 = fa.DirichletBC(ts, gd=0.0).apply(A, F)
+    mg.setup(A2)
+    x, info = fa.cg(A2, F2, M=mg, atol=0.0, rtol=rtol, maxit=maxit, returninfo=True)
+    return x, info, A2, F2
+
+
+def elasticity_main(args):
+    import torch
+    import fealpy_amd as fa
+    from fealpy_amd.solver import GeometricMG
+    n = args.n if args.n != 128 else 40
+    dev = torch.device("cuda:0")
+    mesh, ts, mat, f, u = make_elasticity(n, dev)
+    gdof = ts.number_of_global_dofs()
+    from fealpy_amd.fem.pattern import pattern_of
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pat = pattern_of(ts)
+    torch.cuda.synchronize()
+    t_pattern = time.perf_counter() - t0
+    bd = fa.DirichletBC(ts, gd=0.0).is_boundary_dof
+    mg32 = GeometricMG(ts, bd, dtype=torch.float32)
+    mg64 = GeometricMG(ts, bd, dtype=torch.float64)
+    x_ref, info_ref, A2, _ = elasticity_step(ts, mat, f, mg64, 1e-14)
+    nref = float(torch.linalg.norm(x_ref))
+    t_m, (x_m, info_m, _, _) = timed(lambda: elasticity_step(ts, mat, f, mg32, args.rtol), args.steps, args.warmup)
+    t_d, (x_d, info_d, _, _) = timed(lambda: elasticity_step(ts, mat, f, mg64, args.rtol), args.steps, args.warmup)
+    err_m = float(torch.linalg.norm(x_m - x_ref)) / nref
+    err_d = float(torch.linalg.norm(x_d - x_ref)) / nref
+    err_l2 = float(mesh.error(lambda p: u(p), ts.function(x_m)))
+    print(json.dumps({
+        "metric": "DOF/s assembled+solved to 1e-8 rel err, mixed vs FP64 GPU; serving p99",
+        "value": round(gdof / t_m, 1), "unit": "DOF/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(t_m * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64+f32", "data": "synthetic",
+        "config": {"workload": f"3D linear elasticity, vector P2, TetrahedronMesh.from_box {n}^3, lam=1, mu=0.5, "
+                               f"q=5, clamped, multigrid-preconditioned CG", "gdof": gdof,
+                   "ncell": mesh.number_of_cells(), "nnz": int(A2.nnz), "rtol": args.rtol},
+        "rel_err_ok": bool(max(err_m, err_d) <= 1e-8),
+        "fp64": {"value": round(gdof / t_d, 1), "ms_per_step": round(t_d * 1e3, 3), "rel_err": err_d,
+                 "pcg_iterations": info_d["niter"]},
+        "mixed": {"rel_err": err_m, "pcg_iterations": info_m["niter"], "speedup_vs_fp64": round(t_d / t_m, 3)},
+        "l2_error_vs_manufactured": err_l2,
+        "cold": {"pattern_ms": round(t_pattern * 1e3, 3)},
+        "reference_solve": {"pcg_iterations": info_ref["niter"]},
+        "levels": [lv.n for lv in mg32.levels],
+    }))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -214,6 +302,9 @@ def main():
         sys.exit(2)
     if args.spmv_only:
         spmv_only(args)
+        return
+    if args.workload == "elasticity":
+        elasticity_main(args)
         return
     import torch
     import fealpy_amd as fa

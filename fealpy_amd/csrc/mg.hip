@@ -248,6 +248,7 @@ __global__ void __launch_bounds__(256) k_csr_scatter_dense(int64_t n, const int3
 // rank-1 update of every other row.
 constexpr int COARSE_MAX = 2048;
 constexpr int COARSE_LDS_MAX = 89;  // (n*n + 2n) doubles fit in 64 KiB of LDS
+constexpr int COARSE_ONE_BLOCK = 256;  // dense coarse apply by one workgroup up to this size
 
 __device__ inline void gj_step(int n, int k, double* m, double* rowk, double* colk) {
     const double piv = m[k * n + k];
@@ -280,10 +281,80 @@ __global__ void __launch_bounds__(256) k_spd_inverse_lds(int n, double* __restri
     for (int t = threadIdx.x; t < n * n; t += blockDim.x) a[t] = m[t];
 }
 
-// matrix in global memory (n <= COARSE_MAX), row and column k in LDS
-__global__ void __launch_bounds__(1024) k_spd_inverse(int n, double* __restrict__ a) {
-    __shared__ double rowk[COARSE_MAX], colk[COARSE_MAX];
-    for (int k = 0; k < n; ++k) gj_step(n, k, a, rowk, colk);
+// Blocked Gauss-Jordan for COARSE_LDS_MAX < n <= COARSE_MAX, pivot blocks of
+// GJB columns K = [k0, k0+b), four launches per block:
+//   (1) A_KK <- P = A_KK^-1              (one workgroup, LDS)
+//   (2) A_Kj <- P A_Kj,   j outside K    (row panel)
+//   (3) A_ij <- A_ij - A_iK A_Kj, i, j outside K (old A_iK, new A_Kj)
+//   (4) A_iK <- -A_iK P,  i outside K    (column panel)
+// which is the scalar step of gj_step applied to blocks; after the last
+// block the matrix holds its inverse.  SPD: every pivot block is invertible.
+constexpr int GJB = 32;
+
+__global__ void __launch_bounds__(256) k_gj_pivot(int n, int k0, int b, double* __restrict__ a) {
+    __shared__ double m[GJB * GJB], rowk[GJB], colk[GJB];
+    for (int t = threadIdx.x; t < b * b; t += blockDim.x) m[t] = a[(int64_t)(k0 + t / b) * n + k0 + t % b];
+    __syncthreads();
+    for (int k = 0; k < b; ++k) gj_step(b, k, m, rowk, colk);
+    for (int t = threadIdx.x; t < b * b; t += blockDim.x) a[(int64_t)(k0 + t / b) * n + k0 + t % b] = m[t];
+}
+
+__global__ void __launch_bounds__(256) k_gj_rowpanel(int n, int k0, int b, double* __restrict__ a) {
+    __shared__ double P[GJB * GJB];
+    for (int t = threadIdx.x; t < b * b; t += blockDim.x) P[t] = a[(int64_t)(k0 + t / b) * n + k0 + t % b];
+    __syncthreads();
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        if (j >= k0 && j < k0 + b) continue;
+        double col[GJB];
+        for (int t = 0; t < b; ++t) col[t] = a[(int64_t)(k0 + t) * n + j];
+        for (int r = 0; r < b; ++r) {
+            double s = 0.0;
+            for (int t = 0; t < b; ++t) s += P[r * b + t] * col[t];
+            a[(int64_t)(k0 + r) * n + j] = s;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_gj_trailing(int n, int k0, int b, double* __restrict__ a) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < (int64_t)n * n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int i = static_cast<int>(e / n), j = static_cast<int>(e - (int64_t)i * n);
+        if ((i >= k0 && i < k0 + b) || (j >= k0 && j < k0 + b)) continue;
+        double s = 0.0;
+        for (int t = 0; t < b; ++t) s += a[(int64_t)i * n + k0 + t] * a[(int64_t)(k0 + t) * n + j];
+        a[e] -= s;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_gj_colpanel(int n, int k0, int b, double* __restrict__ a) {
+    __shared__ double P[GJB * GJB];
+    for (int t = threadIdx.x; t < b * b; t += blockDim.x) P[t] = a[(int64_t)(k0 + t / b) * n + k0 + t % b];
+    __syncthreads();
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (i >= k0 && i < k0 + b) continue;
+        double row[GJB];
+        for (int t = 0; t < b; ++t) row[t] = a[(int64_t)i * n + k0 + t];
+        for (int c = 0; c < b; ++c) {
+            double s = 0.0;
+            for (int t = 0; t < b; ++t) s += row[t] * P[t * b + c];
+            a[(int64_t)i * n + k0 + c] = -s;
+        }
+    }
+}
+
+// x = Ainv b on a large coarsest level: one wavefront per row
+template <class T>
+__global__ void __launch_bounds__(256) k_dense_gemv(int n, const double* __restrict__ Ainv, const T* __restrict__ b,
+                                                   T* __restrict__ x, const int* __restrict__ done) {
+    if (done && *done) return;
+    const int lane = threadIdx.x & (WAVE - 1);
+    for (int i = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE; i < n; i += gridDim.x * blockDim.x / WAVE) {
+        const double* row = Ainv + (int64_t)i * n;
+        double s = 0.0;
+        for (int j = lane; j < n; j += WAVE) s += row[j] * static_cast<double>(b[j]);
+        s = wave_sum(s);
+        if (lane == 0) x[i] = static_cast<T>(s);
+    }
 }
 
 // x = Ainv b on the coarsest level (dense matvec, one workgroup)
@@ -541,8 +612,13 @@ struct Cycle {
     static unsigned grid(int64_t n) { return grid_for(((n + 63) / 64) * 64, 256, 1 << 16); }
 
     int coarse(const fa_mg_level_t& L, const T* b, T* x) {
-        hipLaunchKernelGGL(k_coarse_apply<T>, dim3(1), dim3(256), 0, s, (int)L.A.n, mg->coarse_chol, b, x, done);
-        FA_LAUNCH("k_coarse_apply");
+        const int n = static_cast<int>(L.A.n);
+        if (n <= COARSE_ONE_BLOCK)
+            hipLaunchKernelGGL(k_coarse_apply<T>, dim3(1), dim3(256), 0, s, n, mg->coarse_chol, b, x, done);
+        else
+            hipLaunchKernelGGL(k_dense_gemv<T>, dim3(grid_for((int64_t)n * WAVE, 256, 4096)), dim3(256), 0, s, n,
+                               mg->coarse_chol, b, x, done);
+        FA_LAUNCH("coarse solve");
         return FA_OK;
     }
 
@@ -607,6 +683,7 @@ struct Cycle {
 
     bool fusable(int l) const {
         if (mg->nlevels - l > FUSE_MAX_LEVELS || mg->degree < 1) return false;
+        if (mg->levels[mg->nlevels - 1].A.n > COARSE_ONE_BLOCK) return false;
         for (int k = l; k < mg->nlevels; ++k)
             if (mg->levels[k].A.n > FUSE_ROWS) return false;
         return true;
@@ -999,11 +1076,20 @@ extern "C" int fa_csr_to_dense(int64_t n, const int32_t* indptr, const int32_t* 
 
 extern "C" int fa_dense_spd_inverse(int64_t n, double* a, fa_stream_t stream) {
     FA_REQUIRE(n > 0 && n <= COARSE_MAX, "fa_dense_spd_inverse: n = %lld outside 1..%d", (long long)n, COARSE_MAX);
-    if (n <= COARSE_LDS_MAX)
-        hipLaunchKernelGGL(k_spd_inverse_lds, dim3(1), dim3(256), (n * n + 2 * n) * sizeof(double), S(stream), (int)n,
-                           a);
-    else
-        hipLaunchKernelGGL(k_spd_inverse, dim3(1), dim3(1024), 0, S(stream), (int)n, a);
+    hipStream_t s = S(stream);
+    if (n <= COARSE_LDS_MAX) {
+        hipLaunchKernelGGL(k_spd_inverse_lds, dim3(1), dim3(256), (n * n + 2 * n) * sizeof(double), s, (int)n, a);
+    } else {
+        const int nn = static_cast<int>(n);
+        const unsigned gp = grid_for(nn, 256, 1024), gt = grid_for((int64_t)nn * nn, 256, 1 << 14);
+        for (int k0 = 0; k0 < nn; k0 += GJB) {
+            const int b = std::min(GJB, nn - k0);
+            hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(256), 0, s, nn, k0, b, a);
+            hipLaunchKernelGGL(k_gj_rowpanel, dim3(gp), dim3(256), 0, s, nn, k0, b, a);
+            hipLaunchKernelGGL(k_gj_trailing, dim3(gt), dim3(256), 0, s, nn, k0, b, a);
+            hipLaunchKernelGGL(k_gj_colpanel, dim3(gp), dim3(256), 0, s, nn, k0, b, a);
+        }
+    }
     FA_LAUNCH("k_spd_inverse");
     return FA_OK;
 }

@@ -177,3 +177,17 @@ def test_mg_pcg_elasticity_p2(shape):
     err = float(torch.linalg.norm(x - x_ref)) / float(torch.linalg.norm(x_ref))
     assert err < 1e-8, (err, info)
     assert info["niter"] < 120, info
+
+
+@pytest.mark.parametrize("n", [40, 89, 90, 300, 648])
+def test_dense_spd_inverse(n):
+    """fa_dense_spd_inverse (LDS Gauss-Jordan up to 89 rows, blocked above)
+    against numpy.linalg.inv of the same SPD matrix."""
+    from fealpy_amd import _native as nat
+    rng = np.random.default_rng(n)
+    X = rng.standard_normal((n, n))
+    A = X @ X.T + n * np.eye(n)
+    a = torch.as_tensor(A, device=DEV).contiguous()
+    nat.check(nat.lib().fa_dense_spd_inverse(n, nat.ptr(a), nat.stream()), "fa_dense_spd_inverse")
+    ref = np.linalg.inv(A)
+    assert np.abs(a.cpu().numpy() - ref).max() < 1e-12 * np.abs(ref).max()
