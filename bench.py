@@ -44,9 +44,10 @@ def parse():
                     help="PCG relative residual; every run checks the solution error against 1e-8")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-n", type=int, default=40)
-    ap.add_argument("--workload", choices=("diffusion", "elasticity"), default="diffusion",
+    ap.add_argument("--workload", choices=("diffusion", "elasticity", "highorder"), default="diffusion",
                     help="diffusion: 3D P1 lognormal diffusion (the headline line); elasticity: vector P2 "
-                         "linear elasticity on from_box n^3 (n defaults to 40)")
+                         "linear elasticity on from_box n^3 (n defaults to 40); highorder: P3/P4 diffusion + "
+                         "mass element matrices on the matrix cores, from_box n^3 (n defaults to 48)")
     ap.add_argument("--spmv-only", action="store_true",
                     help="assemble once, then run only the FP32 SELL SpMV and a plain copy kernel of known size "
                          "(for hardware-counter runs)")
@@ -294,6 +295,78 @@ def elasticity_main(args):
     }))
 
 
+MFMA_PEAK_TFLOPS = {"bf16": 2500.0, "fp32": 157.3}
+FP64_PEAK_TFLOPS = 78.6
+
+
+def highorder_main(args):
+    """Element matrices of diffusion k(x) = 1 + 0.5 sin(2 pi x) cos(2 pi y) z
+    plus mass 1 on TetrahedronMesh.from_box n^3, P3 (q=5, 35 points) and P4
+    (q=6, 56 points): FP64 element kernel vs the MFMA contraction in FP32 and
+    BF16 (FP32 accumulation).  Kernel time by HIP events on the launching
+    stream; FLOPs = 2 NC NQ 7 ldof^2."""
+    import torch
+    import fealpy_amd as fa
+    from fealpy_amd.fem.highorder import element_matrices_mfma, mfma_flops
+    from fealpy_amd.fem.integrators import GroupIntegrator, coef_spec
+    n = args.n if args.n != 128 else 48
+    dev = torch.device("cuda:0")
+    mesh = fa.TetrahedronMesh.from_box([0, 1, 0, 1, 0, 1], nx=n, ny=n, nz=n, device=dev)
+    pi = np.pi
+
+    @fa.cartesian
+    def kx(p):
+        return 1 + 0.5 * torch.sin(2 * pi * p[..., 0]) * torch.cos(2 * pi * p[..., 1]) * p[..., 2]
+
+    def ev_time(fn, reps=5):
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            out = fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps, out
+
+    results = []
+    for p, q in ((3, 5), (4, 6)):
+        space = fa.LagrangeFESpace(mesh, p)
+        nq = space.tables(q)[1].shape[0]
+        ldof = space.number_of_local_dofs()
+        nc = mesh.number_of_cells()
+        diff, mass = coef_spec(kx, space, q), coef_spec(1.0, space, q)      # coefficient sampled once
+        kq = diff[2]
+        g64 = GroupIntegrator(fa.ScalarDiffusionIntegrator(kq, q=q), fa.ScalarMassIntegrator(1.0, q=q))
+        ms64, ref = ev_time(lambda: g64.assembly(space), reps=2)
+        flops = mfma_flops(nc, nq, ldof)
+        row = {"p": p, "q": q, "nq": nq, "ldof": ldof, "ncell": nc, "gflop": round(flops / 1e9, 2),
+               "fp64_element_ms": round(ms64, 3),
+               "fp64_tflops": round(flops / (ms64 * 1e-3) / 1e12, 2)}
+        scale = float(ref.abs().max())
+        for prec in ("fp32", "bf16"):
+            ms, got = ev_time(lambda: element_matrices_mfma(space, diff, mass, q, prec))
+            tf = flops / (ms * 1e-3) / 1e12
+            out_gbs = nc * ldof * ldof * 4 / (ms * 1e-3) / 1e9
+            row[prec] = {"ms": round(ms, 3), "tflops": round(tf, 1), "peak_tflops": MFMA_PEAK_TFLOPS[prec],
+                         "frac": round(tf / MFMA_PEAK_TFLOPS[prec], 4),
+                         "output_write_gbs": round(out_gbs, 1),
+                         "max_rel_err_vs_fp64": float((got.double() - ref).abs().max()) / scale}
+            del got
+        del ref
+        results.append(row)
+        torch.cuda.empty_cache()
+    best = results[-1]["bf16"]
+    print(json.dumps({
+        "metric": "MFMA element contraction TFLOP/s (high-order diffusion + mass)", "value": best["tflops"],
+        "unit": "TFLOP/s", "n_gpus": 1, "higher_is_better": True, "dtype": "bf16 in, f32 acc", "data": "synthetic",
+        "config": {"workload": f"TetrahedronMesh.from_box {n}^3, P3 (q=5) and P4 (q=6), k(x)=1+0.5 sin(2 pi x) "
+                               f"cos(2 pi y) z + mass 1"},
+        "roofline": {"bound": "mfma", "achieved": best["tflops"], "peak": MFMA_PEAK_TFLOPS["bf16"], "unit": "TFLOP/s",
+                     "frac": best["frac"], "traffic": None},
+        "results": results}))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -305,6 +378,9 @@ def main():
         return
     if args.workload == "elasticity":
         elasticity_main(args)
+        return
+    if args.workload == "highorder":
+        highorder_main(args)
         return
     import torch
     import fealpy_amd as fa

@@ -193,10 +193,23 @@ def _fused_scalar_group(ints):
     return ops
 
 
+MFMA_METHODS = {'mfma_bf16': 'bf16', 'mfma_fp32': 'fp32'}
+
+
 def _scalar_elements(space, ops, *, indices=None, selection=_S, dtype=None):
     L = nat.lib()
     mesh = space.mesh
     q = ops[0][1].q if ops[0][1].q is not None else space.p + 3
+    methods = {getattr(it, 'method', None) for _, it in ops}
+    if len(methods) == 1 and next(iter(methods)) in MFMA_METHODS:
+        if not (isinstance(selection, slice) and selection == _S):
+            raise NotImplementedError("the MFMA element path assembles all cells")
+        from .highorder import element_matrices_mfma
+        spec = {'diff': None, 'mass': None}
+        for op, it in ops:
+            spec[op] = coef_spec(it.coef, space, q)
+        elem = element_matrices_mfma(space, spec['diff'], spec['mass'], q, MFMA_METHODS[next(iter(methods))])
+        return elem if dtype in (None, torch.float32) else elem.to(dtype)
     bcs, qw, phi, gphi = space.tables(q)
     cell = mesh.cell[selection].to(torch.int32).contiguous()
     nc = cell.shape[0]

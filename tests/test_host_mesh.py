@@ -151,3 +151,40 @@ def test_block_prolongation_layout(prio):
     S = torch.zeros(nf, nc, dtype=torch.float64); S[rows, cols] = vals
     ref = torch.kron(torch.eye(k, dtype=torch.float64), S) if prio else torch.kron(S, torch.eye(k, dtype=torch.float64))
     assert torch.equal(D, ref)
+
+
+@pytest.mark.parametrize("p", [2, 3, 4])
+def test_mfma_reference_tensors_rebuild_the_gradient_product(p):
+    """sum_s G_s K_s[q] == grad phi_i . grad phi_j at every point for a random
+    tetrahedron (G from grad lambda_1..3), and K_6 == phi_i phi_j."""
+    from fealpy_amd.fem.highorder import reference_tensors
+    mesh = fa.TetrahedronMesh(torch.tensor([[0.1, 0.0, 0.2], [1.3, 0.1, -0.2], [0.2, 0.9, 0.3], [0.4, 0.3, 1.1]],
+                                           dtype=torch.float64), torch.tensor([[0, 1, 2, 3]], dtype=torch.int32))
+    space = fa.LagrangeFESpace(mesh, p)
+    q = p + 1
+    K = reference_tensors(space, q)                      # (7, NQ, ldof^2)
+    gl = mesh.grad_lambda()[0]                           # (4, 3)
+    pairs = ((1, 1), (2, 2), (3, 3), (1, 2), (1, 3), (2, 3))
+    G = torch.tensor([float(gl[m] @ gl[n]) for m, n in pairs], dtype=torch.float64)
+    got = torch.einsum('s, sqk -> qk', G, K[:6])
+    bcs = space.tables(q)[0]
+    gphi = mesh.grad_shape_function(bcs, p, variables='x')[0]              # (NQ, ldof, 3)
+    ref = torch.einsum('qid, qjd -> qij', gphi, gphi).reshape(got.shape)
+    assert float((got - ref).abs().max()) < 1e-12 * float(ref.abs().max())
+    phi = space.tables(q)[2]
+    assert torch.allclose(K[6], (phi[:, :, None] * phi[:, None, :]).reshape(K[6].shape), atol=0, rtol=0)
+
+
+@pytest.mark.parametrize("per_lane,ks,tpg,groups", [(8, 3, 2, 7), (1, 18, 1, 13)])
+def test_mfma_fragment_order(per_lane, ks, tpg, groups):
+    from fealpy_amd.fem.highorder import fragment_order
+    S, nq, nn = 7, ks * 2 * per_lane - 3, groups * tpg * 32 - 17
+    K = torch.arange(S * nq * nn, dtype=torch.float64).reshape(S, nq, nn) + 1
+    flat = fragment_order(K, ks, per_lane, tpg, groups).reshape(groups, S, tpg, ks, 64, per_lane)
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        g, s, T, j, l, t = (int(rng.integers(0, m)) for m in (groups, S, tpg, ks, 64, per_lane))
+        k = 2 * per_lane * j + per_lane * (l // 32) + t
+        col = (g * tpg + T) * 32 + l % 32
+        want = float(K[s, k, col]) if (k < nq and col < nn) else 0.0
+        assert float(flat[g, s, T, j, l, t]) == want
