@@ -44,7 +44,11 @@ def parse():
                     help="PCG relative residual; every run checks the solution error against 1e-8")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-n", type=int, default=40)
-    ap.add_argument("--workload", choices=("diffusion", "elasticity", "highorder"), default="diffusion",
+    ap.add_argument("--requests", type=int, default=4096, help="serving: number of requests")
+    ap.add_argument("--slots", type=int, default=8, help="serving: concurrent request slots (streams)")
+    ap.add_argument("--rate", type=float, default=0.0,
+                    help="serving: Poisson arrival rate per second (0: 0.8 x the measured server throughput)")
+    ap.add_argument("--workload", choices=("diffusion", "elasticity", "highorder", "serving"), default="diffusion",
                     help="diffusion: 3D P1 lognormal diffusion (the headline line); elasticity: vector P2 "
                          "linear elasticity on from_box n^3 (n defaults to 40); highorder: P3/P4 diffusion + "
                          "mass element matrices on the matrix cores, from_box n^3 (n defaults to 48)")
@@ -367,6 +371,92 @@ def highorder_main(args):
         "results": results}))
 
 
+def _pcts(lat):
+    lat = np.sort(np.asarray(lat))
+    return {"p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+            "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
+            "mean_ms": round(float(lat.mean()) * 1e3, 2)}
+
+
+def serving_main(args):
+    """Mixed-size serving: `requests` 2D diffusion requests (TriangleMesh.from_box
+    nx^2, nx log-uniform in [16, 512], P1/P2 50:50, k = exp(N(0,1)) per cell,
+    8 Fourier modes, u = 0 on the boundary), solved to rtol 1e-8 with
+    multigrid-preconditioned CG.  Resident per-size data is built first
+    (reported as cold time).  Then: (1) the plain one-at-a-time loop on one
+    stream, (2) the continuous-batching server with `slots` streams, both
+    saturated (every request queued at t=0) for solves/s, (3) the server under
+    Poisson arrivals at `rate` for p50/p99 latency (arrival to result); the
+    plain loop's latency at that rate follows from its measured service times
+    (a FIFO single server: start = max(arrival, previous done))."""
+    import torch
+    from fealpy_amd.serving import SolveServer, make_requests
+    n = args.requests
+    seed = 20261202
+    srv = SolveServer(slots=args.slots, rtol=args.rtol)
+    plain = make_requests(n, seed=seed)
+    sizes = sorted({(r.nx, r.p) for r in plain})
+    t_warm = 0.0
+    for i in range(0, len(sizes), 32):
+        t_warm += srv.warm(sizes[i:i + 32])
+        print(f"# warm {min(i + 32, len(sizes))}/{len(sizes)} sizes, {t_warm:.1f}s", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in plain:
+        srv.solve(r)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    print(f"# plain loop {n / t_plain:.1f} solves/s", file=sys.stderr, flush=True)
+    mix = make_requests(n, seed=seed)
+    srv.start()
+    t0 = time.perf_counter()
+    for r in mix:
+        srv.submit(r)
+    for r in mix:
+        r.event.wait()
+    t_srv = time.perf_counter() - t0
+    thr = n / t_srv
+    print(f"# server {thr:.1f} solves/s", file=sys.stderr, flush=True)
+    same = sum(int(torch.equal(a.x, b.x)) for a, b in zip(plain, mix) if b.error is None)
+    errors = sum(1 for r in mix if r.error is not None)
+    rate = args.rate if args.rate > 0 else 0.8 * thr
+    timed_reqs = make_requests(n, seed=seed, rate=rate)
+    t0 = time.perf_counter()
+    for r in timed_reqs:
+        while True:
+            now = time.perf_counter() - t0
+            if now >= r.arrival:
+                break
+            time.sleep(min(r.arrival - now, 0.002))
+        r.arrival += t0
+        srv.submit(r)
+    for r in timed_reqs:
+        r.event.wait()
+    srv.close()
+    lat_srv = [r.done - r.arrival for r in timed_reqs]
+    service = [r.done - r.start for r in plain]
+    done, lat_plain = 0.0, []
+    for r, sv in zip(timed_reqs, service):
+        arr = r.arrival - t0
+        done = max(arr, done) + sv
+        lat_plain.append(done - arr)
+    gdofs = [(r.p * r.nx + 1) ** 2 for r in plain]
+    print(json.dumps({
+        "metric": "serving solves/s and p99 latency, mixed-size 2D diffusion to rtol 1e-8",
+        "value": round(thr, 2), "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f64+f32", "data": "synthetic",
+        "config": {"workload": f"{n} requests, TriangleMesh.from_box nx^2 with nx log-uniform in [16, 512], P1/P2, "
+                               f"k=exp(N(0,1)) per cell, 8 Fourier modes, u=0 on the boundary",
+                   "slots": args.slots, "rtol": args.rtol, "distinct_sizes": len(sizes),
+                   "mean_gdof": round(float(np.mean(gdofs)), 1), "max_gdof": int(max(gdofs))},
+        "plain_loop": {"solves_per_s": round(n / t_plain, 2), **_pcts(lat_plain)},
+        "server": {"solves_per_s": round(thr, 2), "speedup_vs_plain": round(thr * t_plain / n, 3),
+                   "arrival_rate": round(rate, 2), **_pcts(lat_srv)},
+        "results_identical_alone_and_in_mix": f"{same}/{n}", "request_errors": errors,
+        "cold": {"warm_s": round(t_warm, 2)},
+        "dof_per_s_server": round(sum(gdofs) / t_srv, 1)}))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -381,6 +471,9 @@ def main():
         return
     if args.workload == "highorder":
         highorder_main(args)
+        return
+    if args.workload == "serving":
+        serving_main(args)
         return
     import torch
     import fealpy_amd as fa

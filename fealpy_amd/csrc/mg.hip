@@ -624,6 +624,10 @@ struct Cycle {
 
     // Chebyshev sweep of `deg` steps on level L: from zero (pre) or from *cur (post).
     int smooth(const fa_mg_level_t& L, const T* b, T*& cur, T*& nxt, bool from_zero) {
+        return smooth_deg(L, b, cur, nxt, from_zero, mg->degree);
+    }
+
+    int smooth_deg(const fa_mg_level_t& L, const T* b, T*& cur, T*& nxt, bool from_zero, int degree) {
         const double lo = mg->lower * L.lmax, hi = mg->upper * L.lmax;
         const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
         double rho = 1.0 / sigma;
@@ -631,7 +635,7 @@ struct Cycle {
         const T* sv = static_cast<const T*>(L.A.vals);
         T* d = static_cast<T*>(L.d);
         const unsigned g = grid(L.A.n);
-        for (int k = 0; k < mg->degree; ++k) {
+        for (int k = 0; k < degree; ++k) {
             T c1, c2;
             double rho_new = rho;
             if (k == 0) {
@@ -683,7 +687,7 @@ struct Cycle {
 
     bool fusable(int l) const {
         if (mg->nlevels - l > FUSE_MAX_LEVELS || mg->degree < 1) return false;
-        if (mg->levels[mg->nlevels - 1].A.n > COARSE_ONE_BLOCK) return false;
+        if (!mg->coarse_chol || mg->levels[mg->nlevels - 1].A.n > COARSE_ONE_BLOCK) return false;
         for (int k = l; k < mg->nlevels; ++k)
             if (mg->levels[k].A.n > FUSE_ROWS) return false;
         return true;
@@ -696,7 +700,7 @@ struct Cycle {
         T* nxt = static_cast<T*>(L.x2);
         if (l > 0 && fusable(l)) return fused(l, out);
         if (l == mg->nlevels - 1) {
-            int st = coarse(L, b, cur);
+            int st = mg->coarse_chol ? coarse(L, b, cur) : smooth_deg(L, b, cur, nxt, true, mg->coarse_degree);
             *out = cur;
             return st;
         }
@@ -1164,8 +1168,10 @@ extern "C" int fa_jacobi_setup(int dtype_out, int64_t n, int64_t nnz, const int3
 
 extern "C" int fa_mg_vcycle(const fa_mg_t* mg, const void* r, void* z, fa_stream_t stream) {
     FA_REQUIRE(mg && mg->nlevels >= 1 && mg->levels, "fa_mg_vcycle: empty hierarchy");
-    FA_REQUIRE(mg->coarse_chol, "fa_mg_vcycle: the coarsest level needs its inverse");
-    FA_REQUIRE(mg->levels[mg->nlevels - 1].A.n <= COARSE_MAX, "fa_mg_vcycle: coarsest level too large");
+    FA_REQUIRE(mg->coarse_chol || mg->coarse_degree >= 1,
+               "fa_mg_vcycle: the coarsest level needs its inverse or a smoothing degree");
+    FA_REQUIRE(!mg->coarse_chol || mg->levels[mg->nlevels - 1].A.n <= COARSE_MAX,
+               "fa_mg_vcycle: coarsest level too large for the dense inverse");
     return vcycle_any(mg, r, z, nullptr, S(stream));
 }
 
@@ -1180,9 +1186,11 @@ extern "C" int fa_pcg(int dtype, const fa_sell_t* A, const fa_mg_t* M, const voi
                       void* work, size_t work_bytes, fa_stream_t stream) {
     FA_REQUIRE(A && A->n > 0 && check_every > 0, "fa_pcg: bad arguments");
     if (M) {
-        FA_REQUIRE(M->nlevels >= 1 && M->levels && M->coarse_chol, "fa_pcg: incomplete multigrid hierarchy");
+        FA_REQUIRE(M->nlevels >= 1 && M->levels && (M->coarse_chol || M->coarse_degree >= 1),
+                   "fa_pcg: incomplete multigrid hierarchy");
         FA_REQUIRE(M->levels[0].A.n == A->n, "fa_pcg: preconditioner size mismatch");
-        FA_REQUIRE(M->levels[M->nlevels - 1].A.n <= COARSE_MAX, "fa_pcg: coarsest level too large");
+        FA_REQUIRE(!M->coarse_chol || M->levels[M->nlevels - 1].A.n <= COARSE_MAX,
+                   "fa_pcg: coarsest level too large for the dense inverse");
     }
     hipStream_t s = S(stream);
     const int ud = M ? M->dtype : dtype;

@@ -1,8 +1,13 @@
 """Dirichlet boundary condition (fealpy/fem/dirichlet_bc.py:13-234)."""
+import threading
+
 import torch
 
 from .. import _native as nat
 from ..sparse.tensors import COOTensor, CSRTensor
+
+
+_PATTERN_LOCK = threading.Lock()
 
 
 class DirichletBC:
@@ -70,36 +75,42 @@ class DirichletBC:
         L = nat.lib()
         n = A.shape[0]
         dev = A.device
-        new_ptr, new_idx, fresh = self._filtered_pattern(A)
+        new_ptr, new_idx = self._filtered_pattern(A)
         new_val = torch.empty(new_idx.shape[0], dtype=A.dtype, device=dev)
         nat.check(L.fa_dirichlet_fill(nat.dtype_code(A.dtype), n, nat.ptr(A.crow), nat.ptr(A.col),
-                                      nat.ptr(A.values), nat.ptr(self._bd_u8), nat.ptr(new_ptr),
-                                      nat.ptr(new_idx) if fresh else None, nat.ptr(new_val), nat.stream()),
-                  "fa_dirichlet_fill")
+                                      nat.ptr(A.values), nat.ptr(self._bd_u8), nat.ptr(new_ptr), None,
+                                      nat.ptr(new_val), nat.stream()), "fa_dirichlet_fill")
         out = CSRTensor(new_ptr, new_idx, new_val, A.shape)
         return out.tocoo() if was_coo else out
 
     def _filtered_pattern(self, A):
-        """(indptr, indices, fresh) of the filtered pattern.  The pattern depends
-        only on A's pattern and the boundary flags, so it is kept on the space
-        and reused by later assemblies (fresh=False: indices already written)."""
-        cache = self.space.__dict__.setdefault('_bc_patterns', [])
-        for (crow, col, bd), val in cache:
-            if crow is A.crow and col is A.col and (bd is self._bd_u8 or torch.equal(bd, self._bd_u8)):
-                return val + (False,)
-        L = nat.lib()
-        n = A.shape[0]
-        dev = A.device
-        new_ptr = torch.empty(n + 1, dtype=torch.int32, device=dev)
-        ws = nat.workspace(L.fa_dirichlet_work_bytes(n), dev)
-        nnz = nat.ctypes.c_int64(0)
-        nat.check(L.fa_dirichlet_count(n, nat.ptr(A.crow), nat.ptr(A.col), nat.ptr(self._bd_u8), nat.ptr(new_ptr),
-                                       nat.ctypes.byref(nnz), nat.ptr(ws), ws.numel(), nat.stream()),
-                  "fa_dirichlet_count")
-        new_idx = torch.empty(int(nnz.value), dtype=torch.int32, device=dev)
-        del cache[:-3]                      # keep a few patterns
-        cache.append(((A.crow, A.col, self._bd_u8), (new_ptr, new_idx)))
-        return new_ptr, new_idx, True
+        """(indptr, indices) of the filtered pattern.  It depends only on A's
+        pattern and the boundary flags, so it is built once, completed on the
+        building stream, and kept on the space for later assemblies (also from
+        other threads and streams)."""
+        with _PATTERN_LOCK:
+            cache = self.space.__dict__.setdefault('_bc_patterns', [])
+            for (crow, col, bd), val in cache:
+                if crow is A.crow and col is A.col and (bd is self._bd_u8 or torch.equal(bd, self._bd_u8)):
+                    return val
+            L = nat.lib()
+            n = A.shape[0]
+            dev = A.device
+            new_ptr = torch.empty(n + 1, dtype=torch.int32, device=dev)
+            ws = nat.workspace(L.fa_dirichlet_work_bytes(n), dev)
+            nnz = nat.ctypes.c_int64(0)
+            nat.check(L.fa_dirichlet_count(n, nat.ptr(A.crow), nat.ptr(A.col), nat.ptr(self._bd_u8),
+                                           nat.ptr(new_ptr), nat.ctypes.byref(nnz), nat.ptr(ws), ws.numel(),
+                                           nat.stream()), "fa_dirichlet_count")
+            new_idx = torch.empty(int(nnz.value), dtype=torch.int32, device=dev)
+            scratch = torch.empty(int(nnz.value), dtype=A.dtype, device=dev)
+            nat.check(L.fa_dirichlet_fill(nat.dtype_code(A.dtype), n, nat.ptr(A.crow), nat.ptr(A.col),
+                                          nat.ptr(A.values), nat.ptr(self._bd_u8), nat.ptr(new_ptr),
+                                          nat.ptr(new_idx), nat.ptr(scratch), nat.stream()), "fa_dirichlet_fill")
+            torch.cuda.current_stream(dev).synchronize()
+            del cache[:-3]                      # keep a few patterns
+            cache.append(((A.crow, A.col, self._bd_u8), (new_ptr, new_idx)))
+            return new_ptr, new_idx
 
     def apply_vector(self, vector, matrix, uh=None, gd=None, *, check=True):
         A = self.check_matrix(matrix) if check else matrix

@@ -3,6 +3,8 @@
 Holds the DOF->(cell, local) incidence (used by every value gather) and the
 CSR pattern (int32 indptr/indices, columns sorted in each row).
 """
+import threading
+
 import torch
 
 from .. import _native as nat
@@ -68,11 +70,20 @@ class Pattern:
         return out
 
 
+_LOCK = threading.Lock()
+
+
 def pattern_of(space, cell2dof=None):
-    """Cached pattern of ``space.cell_to_dof()`` (or of an explicit map)."""
+    """Cached pattern of ``space.cell_to_dof()`` (or of an explicit map).  A
+    new pattern is completed on the building stream before it is published,
+    so other threads and streams may use it at once."""
     if cell2dof is None:
         if getattr(space, "_pattern", None) is None:
-            space._pattern = Pattern(space.cell_to_dof(), space.number_of_global_dofs())
+            with _LOCK:
+                if getattr(space, "_pattern", None) is None:
+                    pat = Pattern(space.cell_to_dof(), space.number_of_global_dofs())
+                    torch.cuda.current_stream(pat.indptr.device).synchronize()
+                    space._pattern = pat
         return space._pattern
     key = (cell2dof.data_ptr(), tuple(cell2dof.shape))
     cache = space.__dict__.setdefault("_patterns", {})

@@ -42,7 +42,7 @@ class FaMgLevel(ctypes.Structure):
 class FaMg(ctypes.Structure):
     _fields_ = [("dtype", ctypes.c_int), ("nlevels", ctypes.c_int), ("levels", ctypes.POINTER(FaMgLevel)),
                 ("degree", ctypes.c_int), ("lower", ctypes.c_double), ("upper", ctypes.c_double),
-                ("coarse_chol", ctypes.c_void_p)]
+                ("coarse_chol", ctypes.c_void_p), ("coarse_degree", ctypes.c_int)]
 
 
 def _grid_shape(mesh):
@@ -167,7 +167,7 @@ class GeometricMG:
     the same mesh; vector spaces coarsen every component alike."""
 
     def __init__(self, space, is_bd, *, dtype=torch.float32, degree=2, lower=0.1, upper=1.0, coarse_max=64,
-                 power_iters=12):
+                 power_iters=12, coarse_degree=12):
         ss = getattr(space, 'scalar_space', space)
         k = int(getattr(space, 'dof_numel', 1))
         prio = bool(getattr(space, 'dof_priority', False))
@@ -210,8 +210,11 @@ class GeometricMG:
             nxt.dims, nxt.n, nxt.bd = dims_c, k * nc_s, bd_c
             self.levels.append(nxt)
             lev, dims, bd = nxt, dims_c, bd_c
-        if self.levels[-1].n > 2048:
-            raise ValueError(f"coarsest level has {self.levels[-1].n} unknowns; the dense coarse solver holds 2048")
+        # the coarsest level is solved exactly (dense inverse) up to 2048 rows;
+        # above that (an odd grid dimension stops the halving early) it is
+        # smoothed from zero with a Chebyshev polynomial of degree coarse_degree
+        self.coarse_dense = self.levels[-1].n <= 2048
+        self.coarse_degree = coarse_degree
         for lv in self.levels:
             z = lambda: torch.zeros(lv.n, dtype=dtype, device=dev)
             lv.b, lv.x, lv.x2, lv.d, lv.r, lv.dinv = z(), z(), z(), z(), z(), z()
@@ -252,7 +255,8 @@ class GeometricMG:
             c.max_row = int(mrow.value)
             c.As = SellMatrix(c.A.crow, c.A.col, c.A.shape)
         last = self.levels[-1]
-        last.dense = torch.zeros((last.n, last.n), dtype=torch.float64, device=self.device)
+        last.dense = (torch.zeros((last.n, last.n), dtype=torch.float64, device=self.device) if self.coarse_dense
+                      else None)
 
     def setup(self, A):
         """Numeric setup for the (Dirichlet-applied, FP64) fine matrix A."""
@@ -280,9 +284,10 @@ class GeometricMG:
                                         ws.numel(), nat.stream()), "fa_jacobi_setup")
             lv.lmax = float(lam.value)
         last = self.levels[-1]
-        nat.check(L.fa_csr_to_dense(last.n, nat.ptr(last.A.crow), nat.ptr(last.A.col), nat.ptr(last.A.values),
-                                    nat.ptr(last.dense), nat.stream()), "fa_csr_to_dense")
-        nat.check(L.fa_dense_spd_inverse(last.n, nat.ptr(last.dense), nat.stream()), "fa_dense_spd_inverse")
+        if self.coarse_dense:
+            nat.check(L.fa_csr_to_dense(last.n, nat.ptr(last.A.crow), nat.ptr(last.A.col), nat.ptr(last.A.values),
+                                        nat.ptr(last.dense), nat.stream()), "fa_csr_to_dense")
+            nat.check(L.fa_dense_spd_inverse(last.n, nat.ptr(last.dense), nat.stream()), "fa_dense_spd_inverse")
         self._build_desc()
         self._setup_A = A
         return self
@@ -310,8 +315,9 @@ class GeometricMG:
             e.b, e.x, e.x2, e.d, e.r = (lv.b.data_ptr(), lv.x.data_ptr(), lv.x2.data_ptr(), lv.d.data_ptr(),
                                         lv.r.data_ptr())
         self._lv_array = arr
+        dense = self.levels[-1].dense
         self._desc = FaMg(nat.dtype_code(self.dtype), n, arr, self.degree, self.lower, self.upper,
-                          self.levels[-1].dense.data_ptr())
+                          dense.data_ptr() if dense is not None else None, self.coarse_degree)
 
     # -- use ----------------------------------------------------------------------
     def __matmul__(self, r):

@@ -284,7 +284,12 @@ typedef struct {
     int degree;                /* Chebyshev degree of each smoothing sweep      */
     double lower, upper;       /* smoothing interval [lower, upper] * lmax      */
     const double* coarse_chol; /* inverse of the coarsest A, row-major n x n
-                                  FP64 (fa_dense_spd_inverse)                   */
+                                  FP64 (fa_dense_spd_inverse), or NULL: then
+                                  the coarsest level is smoothed from zero
+                                  with a Chebyshev polynomial of degree
+                                  coarse_degree (any size; SPD, so the cycle
+                                  stays a valid CG preconditioner)              */
+    int coarse_degree;
 } fa_mg_t;
 
 /* Dense row-major copy of a small CSR matrix (FP64) and its in-place inverse
