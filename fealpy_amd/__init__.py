@@ -13,7 +13,7 @@ from .fem import BilinearForm, DirichletBC, GroupIntegrator, LinearElasticityInt
 from .functionspace import Function, LagrangeFESpace, TensorFunctionSpace
 from .material import LinearElasticMaterial
 from .mesh import TetrahedronMesh, TriangleMesh
-from .solver import cg
+from .solver import GeometricMG, cg
 from .sparse import COOTensor, CSRTensor
 
 __version__ = "0.1.0"

@@ -45,6 +45,7 @@ _SIGNATURES = {
                                              ctypes.c_int, _p, _p]),
     "fa_element_vectors_tensor": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                  ctypes.c_int, ctypes.c_int, _p, _p, _i64, _p, _p, _p, _p, _p]),
+    "fa_sell16_fill": (ctypes.c_int, [_i64, _p, _p, _p, _p, _p, _p, _p, ctypes.c_size_t, _p]),
     "fa_mfma_tiles_per_group": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "fa_mfma_work_bytes": (ctypes.c_size_t, [ctypes.c_int, _i64, ctypes.c_int]),
     "fa_element_matrices_mfma": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,

@@ -1,5 +1,6 @@
 // Kernel "d": Galerkin coarse operators, multigrid V-cycle and the PCG driver.
 #include <algorithm>
+#include <type_traits>
 #include <climits>
 #include <cmath>
 
@@ -197,6 +198,41 @@ __global__ void __launch_bounds__(256) k_cheb(int64_t n, const int32_t* __restri
                 xout[row] = xin[row] + dn;
             }
         }
+    }
+}
+
+// The same two steps on the compressed D^-1 A (FP32 cycles):
+//   cheb:     d = c1 d + c2 (dinv b - (D^-1 A) x_in); x_out = x_in + d
+//   residual: r = b - diag * ((D^-1 A) x)
+template <bool FIRST>
+__global__ void __launch_bounds__(256) k_cheb16(int64_t n, const int32_t* __restrict__ sp,
+                                               const int16_t* __restrict__ so, const _Float16* __restrict__ sv,
+                                               const float* __restrict__ dinv, const float* __restrict__ b,
+                                               const float* __restrict__ xin, float* __restrict__ xout,
+                                               float* __restrict__ d, float c1, float c2,
+                                               const int* __restrict__ done) {
+    if (done && *done) return;
+    for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
+         row += (int64_t)gridDim.x * blockDim.x) {
+        const float ax = sell16_row_dot(sp, so, sv, row, n, xin);
+        if (row < n) {
+            const float dn = (FIRST ? 0.f : c1 * d[row]) + c2 * (dinv[row] * b[row] - ax);
+            d[row] = dn;
+            xout[row] = xin[row] + dn;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_residual16(int64_t n, const int32_t* __restrict__ sp,
+                                                   const int16_t* __restrict__ so, const _Float16* __restrict__ sv,
+                                                   const float* __restrict__ diag, const float* __restrict__ b,
+                                                   const float* __restrict__ x, float* __restrict__ r,
+                                                   const int* __restrict__ done) {
+    if (done && *done) return;
+    for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
+         row += (int64_t)gridDim.x * blockDim.x) {
+        const float ax = sell16_row_dot(sp, so, sv, row, n, x);
+        if (row < n) r[row] = b[row] - diag[row] * ax;
     }
 }
 
@@ -611,6 +647,11 @@ struct Cycle {
 
     static unsigned grid(int64_t n) { return grid_for(((n + 63) / 64) * 64, 256, 1 << 16); }
 
+    // compressed operator present (FP32 cycles only)
+    static bool compressed(const fa_mg_level_t& L) {
+        return std::is_same<T, float>::value && L.Ah.vals != nullptr && L.diag != nullptr;
+    }
+
     int coarse(const fa_mg_level_t& L, const T* b, T* x) {
         const int n = static_cast<int>(L.A.n);
         if (n <= COARSE_ONE_BLOCK)
@@ -650,6 +691,22 @@ struct Cycle {
                 hipLaunchKernelGGL((k_cheb<T, true, true>), dim3(g), dim3(256), 0, s, L.A.n, L.A.slice_ptr, L.A.cols,
                                    sv, dinv, b, cur, cur, d, c1, c2, done);
                 continue;  // result written in place into cur
+            }
+            if (compressed(L)) {
+                const _Float16* hv = static_cast<const _Float16*>(L.Ah.vals);
+                const float* fb = reinterpret_cast<const float*>(b);
+                const float* fdinv = reinterpret_cast<const float*>(dinv);
+                if (k == 0)
+                    hipLaunchKernelGGL(k_cheb16<true>, dim3(g), dim3(256), 0, s, L.A.n, L.Ah.slice_ptr, L.Ah.offs, hv,
+                                       fdinv, fb, reinterpret_cast<const float*>(cur), reinterpret_cast<float*>(nxt),
+                                       reinterpret_cast<float*>(d), (float)c1, (float)c2, done);
+                else
+                    hipLaunchKernelGGL(k_cheb16<false>, dim3(g), dim3(256), 0, s, L.A.n, L.Ah.slice_ptr, L.Ah.offs, hv,
+                                       fdinv, fb, reinterpret_cast<const float*>(cur), reinterpret_cast<float*>(nxt),
+                                       reinterpret_cast<float*>(d), (float)c1, (float)c2, done);
+                std::swap(cur, nxt);
+                rho = rho_new;
+                continue;
             }
             if (k == 0)
                 hipLaunchKernelGGL((k_cheb<T, false, true>), dim3(g), dim3(256), 0, s, L.A.n, L.A.slice_ptr,
@@ -709,8 +766,14 @@ struct Cycle {
         const fa_mg_level_t& C = mg->levels[l + 1];
         T* r = static_cast<T*>(L.r);
         const unsigned g = grid(L.A.n);
-        hipLaunchKernelGGL(k_residual<T>, dim3(g), dim3(256), 0, s, L.A.n, L.A.slice_ptr, L.A.cols,
-                           static_cast<const T*>(L.A.vals), b, cur, r, done);
+        if (compressed(L))
+            hipLaunchKernelGGL(k_residual16, dim3(g), dim3(256), 0, s, L.A.n, L.Ah.slice_ptr, L.Ah.offs,
+                               static_cast<const _Float16*>(L.Ah.vals), static_cast<const float*>(L.diag),
+                               reinterpret_cast<const float*>(b), reinterpret_cast<const float*>(cur),
+                               reinterpret_cast<float*>(r), done);
+        else
+            hipLaunchKernelGGL(k_residual<T>, dim3(g), dim3(256), 0, s, L.A.n, L.A.slice_ptr, L.A.cols,
+                               static_cast<const T*>(L.A.vals), b, cur, r, done);
         hipLaunchKernelGGL((k_sell_apply<T, false>), dim3(grid(L.R.n)), dim3(256), 0, s, L.R.n, L.R.slice_ptr,
                            L.R.cols, static_cast<const T*>(L.R.vals), r, static_cast<T*>(C.b), done);
         FA_LAUNCH("restrict");

@@ -53,9 +53,70 @@ __global__ void __launch_bounds__(256) k_sell_spmv(int64_t n, const int32_t* __r
     }
 }
 
+// compressed fill: offs = col - row, vals = a_ij / a_ii in FP16 (one thread per row)
+__global__ void __launch_bounds__(256) k_sell16_fill(int64_t n, const int32_t* __restrict__ indptr,
+                                                    const int32_t* __restrict__ indices,
+                                                    const double* __restrict__ vin,
+                                                    const int32_t* __restrict__ slice_ptr,
+                                                    int16_t* __restrict__ offs, _Float16* __restrict__ vals,
+                                                    int32_t* __restrict__ overflow) {
+    const int64_t nslices = (n + 63) / 64;
+    for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < nslices * 64;
+         row += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = row >> 6;
+        const int i = static_cast<int>(row & 63);
+        const int32_t p0 = slice_ptr[s];
+        const int w = (slice_ptr[s + 1] - p0) >> 6;
+        const int32_t r0 = row < n ? indptr[row] : 0;
+        const int len = row < n ? indptr[row + 1] - r0 : 0;
+        double d = 0.0;
+        for (int k = 0; k < len; ++k)
+            if (indices[r0 + k] == row) d = vin[r0 + k];
+        const double inv = d != 0.0 ? 1.0 / d : 1.0;
+        bool bad = false;
+        for (int k = 0; k < w; ++k) {
+            const int64_t o = p0 + 64 * k + i;
+            int64_t off = 0;
+            double v = 0.0;
+            if (k < len) {
+                off = static_cast<int64_t>(indices[r0 + k]) - row;
+                v = vin[r0 + k] * inv;
+            }
+            if (off > 32767 || off < -32767) {
+                bad = true;
+                off = 0;
+            }
+            offs[o] = static_cast<int16_t>(off);
+            vals[o] = static_cast<_Float16>(v);
+        }
+        if (bad) overflow[0] = 1;
+    }
+}
+
 }  // namespace fa
 
 using namespace fa;
+
+extern "C" int fa_sell16_fill(int64_t n, const int32_t* indptr, const int32_t* indices, const double* values,
+                              const int32_t* slice_ptr, int16_t* offs, void* vals, void* work, size_t work_bytes,
+                              fa_stream_t stream) {
+    FA_REQUIRE(n > 0 && indptr && indices && values && slice_ptr && offs && vals, "fa_sell16_fill: bad arguments");
+    FA_REQUIRE(work && work_bytes >= sizeof(int32_t), "fa_sell16_fill: workspace too small");
+    hipStream_t s = S(stream);
+    int32_t* flag = static_cast<int32_t*>(work);
+    FA_HIP(hipMemsetAsync(flag, 0, sizeof(int32_t), s));
+    hipLaunchKernelGGL(k_sell16_fill, dim3(grid_for(((n + 63) / 64) * 64, 256, 1 << 16)), dim3(256), 0, s, n, indptr,
+                       indices, values, slice_ptr, offs, static_cast<_Float16*>(vals), flag);
+    FA_LAUNCH("k_sell16_fill");
+    int32_t h = 0;
+    FA_HIP(hipMemcpyAsync(&h, flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    FA_HIP(hipStreamSynchronize(s));
+    if (h) {
+        set_error("fa_sell16_fill: a column lies more than 32767 from its row");
+        return FA_ECAPACITY;
+    }
+    return FA_OK;
+}
 
 extern "C" size_t fa_sell_work_bytes(int64_t n) {
     const int64_t ns = (n + 63) / 64;

@@ -29,14 +29,14 @@ import ctypes
 import torch
 
 from .. import _native as nat
-from ..sparse.sell import FaSell, SellMatrix
+from ..sparse.sell import FaSell, FaSell16, SellMatrix
 from ..sparse.tensors import CSRTensor
 
 
 class FaMgLevel(ctypes.Structure):
     _fields_ = [("A", FaSell), ("dinv", ctypes.c_void_p), ("R", FaSell), ("P", FaSell), ("lmax", ctypes.c_double),
                 ("b", ctypes.c_void_p), ("x", ctypes.c_void_p), ("x2", ctypes.c_void_p), ("d", ctypes.c_void_p),
-                ("r", ctypes.c_void_p)]
+                ("r", ctypes.c_void_p), ("Ah", FaSell16), ("diag", ctypes.c_void_p)]
 
 
 class FaMg(ctypes.Structure):
@@ -167,7 +167,7 @@ class GeometricMG:
     the same mesh; vector spaces coarsen every component alike."""
 
     def __init__(self, space, is_bd, *, dtype=torch.float32, degree=2, lower=0.1, upper=1.0, coarse_max=64,
-                 power_iters=12, coarse_degree=12):
+                 power_iters=12, coarse_degree=12, compress=True):
         ss = getattr(space, 'scalar_space', space)
         k = int(getattr(space, 'dof_numel', 1))
         prio = bool(getattr(space, 'dof_priority', False))
@@ -177,6 +177,8 @@ class GeometricMG:
         dims = _grid_shape(mesh)
         dev = space.device
         self.dtype, self.degree, self.lower, self.upper = dtype, degree, lower, upper
+        # FP32 cycles smooth with FP16 D^-1 A and int16 column offsets where the band allows
+        self.compress = bool(compress) and dtype == torch.float32
         self.power_iters = power_iters
         self.device = dev
         self.levels = []
@@ -283,6 +285,9 @@ class GeometricMG:
                                         nat.ptr(lv.A.values), nat.ptr(lv.dinv), ctypes.byref(lam), nat.ptr(ws),
                                         ws.numel(), nat.stream()), "fa_jacobi_setup")
             lv.lmax = float(lam.value)
+            lv.compressed = self.compress and lv.n > 1024 and lv.As.set_compressed(lv.A.values, ws)
+            if lv.compressed:
+                lv.diag = torch.reciprocal(lv.dinv)
         last = self.levels[-1]
         if self.coarse_dense:
             nat.check(L.fa_csr_to_dense(last.n, nat.ptr(last.A.crow), nat.ptr(last.A.col), nat.ptr(last.A.values),
@@ -314,6 +319,12 @@ class GeometricMG:
             e.lmax = getattr(lv, 'lmax', 2.0)
             e.b, e.x, e.x2, e.d, e.r = (lv.b.data_ptr(), lv.x.data_ptr(), lv.x2.data_ptr(), lv.d.data_ptr(),
                                         lv.r.data_ptr())
+            if getattr(lv, 'compressed', False):
+                e.Ah = lv.As.desc16()
+                e.diag = lv.diag.data_ptr()
+            else:
+                e.Ah = FaSell16(0, None, None, None)
+                e.diag = None
         self._lv_array = arr
         dense = self.levels[-1].dense
         self._desc = FaMg(nat.dtype_code(self.dtype), n, arr, self.degree, self.lower, self.upper,

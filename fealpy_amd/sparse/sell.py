@@ -1,5 +1,6 @@
 """SELL-64 copies of CSR matrices for the solver kernels (see include/fealpy_amd.h)."""
 import ctypes
+import threading
 
 import torch
 
@@ -11,29 +12,81 @@ class FaSell(ctypes.Structure):
                 ("cols", ctypes.c_void_p), ("vals", ctypes.c_void_p)]
 
 
+class FaSell16(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("slice_ptr", ctypes.c_void_p), ("offs", ctypes.c_void_p),
+                ("vals", ctypes.c_void_p)]
+
+
+_INDEX = {}
+_INDEX_LOCK = threading.Lock()
+
+
+def _build_index(crow, col, shape):
+    L = nat.lib()
+    n = shape[0]
+    dev = crow.device
+    slice_ptr = torch.empty((n + 63) // 64 + 1, dtype=torch.int32, device=dev)
+    ws = nat.workspace(L.fa_sell_work_bytes(n), dev)
+    cnt = ctypes.c_int64(0)
+    nat.check(L.fa_sell_count(n, nat.ptr(crow), nat.ptr(slice_ptr), ctypes.byref(cnt), nat.ptr(ws), ws.numel(),
+                              nat.stream()), "fa_sell_count")
+    entries = int(cnt.value)
+    cols = torch.zeros(max(entries, 1), dtype=torch.int32, device=dev)
+    if entries > 0:
+        nat.check(L.fa_sell_fill(nat.FA_F64, nat.FA_F64, n, shape[1], nat.ptr(crow), nat.ptr(col), None,
+                                 nat.ptr(slice_ptr), nat.ptr(cols), None, nat.stream()), "fa_sell_fill")
+    return slice_ptr, cols, entries
+
+
+def _sell_index(crow, col, shape):
+    """(slice_ptr, cols, entries) of a CSR pattern, built once per pattern
+    (the entry keeps the pattern tensors alive, so their addresses are not
+    reused) and completed on the building stream before it is shared."""
+    key = (crow.data_ptr(), col.data_ptr(), int(col.numel()), tuple(shape))
+    with _INDEX_LOCK:
+        ent = _INDEX.get(key)
+        if ent is not None and ent[0] is crow and ent[1] is col:
+            return ent[2]
+        idx = _build_index(crow, col, shape)
+        torch.cuda.current_stream(crow.device).synchronize()
+        _INDEX[key] = (crow, col, idx)
+        return idx
+
+
 class SellMatrix:
     """Sliced-ELL (64-row slices) index structure of a CSR pattern plus value
-    arrays in FP64 and/or FP32.  The index part is built once per pattern;
-    ``set_values`` refreshes the values (with an optional FP64->FP32 cast)."""
+    arrays in FP64 and/or FP32.  The index part is shared by every matrix of
+    the same pattern; ``set_values`` refreshes this matrix's values (with an
+    optional FP64->FP32 cast)."""
 
-    def __init__(self, crow, col, shape):
-        L = nat.lib()
+    def __init__(self, crow, col, shape, shared=False):
         self.crow, self.col = crow, col
         self.shape = tuple(shape)
-        n = self.shape[0]
-        dev = crow.device
-        self.slice_ptr = torch.empty((n + 63) // 64 + 1, dtype=torch.int32, device=dev)
-        ws = nat.workspace(L.fa_sell_work_bytes(n), dev)
-        ent = ctypes.c_int64(0)
-        nat.check(L.fa_sell_count(n, nat.ptr(crow), nat.ptr(self.slice_ptr), ctypes.byref(ent), nat.ptr(ws),
-                                  ws.numel(), nat.stream()), "fa_sell_count")
-        self.entries = int(ent.value)
-        self.cols = torch.zeros(max(self.entries, 1), dtype=torch.int32, device=dev)
-        if self.entries > 0:
-            nat.check(L.fa_sell_fill(nat.FA_F64, nat.FA_F64, n, self.shape[1], nat.ptr(crow), nat.ptr(col), None,
-                                     nat.ptr(self.slice_ptr), nat.ptr(self.cols), None, nat.stream()),
-                      "fa_sell_fill")
+        self.slice_ptr, self.cols, self.entries = (_sell_index if shared else _build_index)(crow, col, self.shape)
         self.vals = {}
+        self.h_offs = self.h_vals = None
+
+    def set_compressed(self, values, work):
+        """Compressed FP16 D^-1 A with int16 column offsets (fa_sell16_fill)
+        from FP64 CSR values; False when a column is too far from its row."""
+        if self.entries == 0:
+            return False
+        if self.h_offs is None:
+            self.h_offs = torch.zeros(self.entries, dtype=torch.int16, device=values.device)
+            self.h_vals = torch.zeros(self.entries, dtype=torch.float16, device=values.device)
+        st = nat.lib().fa_sell16_fill(self.shape[0], nat.ptr(self.crow), nat.ptr(self.col),
+                                      nat.ptr(values.contiguous()), nat.ptr(self.slice_ptr), nat.ptr(self.h_offs),
+                                      nat.ptr(self.h_vals), nat.ptr(work), work.numel(), nat.stream())
+        if st == nat.FA_ECAPACITY:
+            self.h_offs = self.h_vals = None
+            return False
+        nat.check(st, "fa_sell16_fill")
+        return True
+
+    def desc16(self):
+        if self.h_vals is None:
+            return FaSell16(0, None, None, None)
+        return FaSell16(self.shape[0], self.slice_ptr.data_ptr(), self.h_offs.data_ptr(), self.h_vals.data_ptr())
 
     def set_values(self, values, dtype=None):
         """Copy CSR values (FP64 or FP32) into the SELL layout as `dtype`."""
@@ -67,7 +120,7 @@ def sell_of(A):
     key = (A.crow.data_ptr(), A.col.data_ptr(), A.nnz)
     S = getattr(A, "_sell", None)
     if S is None or S[0] != key:
-        S = (key, SellMatrix(A.crow, A.col, A.shape))
+        S = (key, SellMatrix(A.crow, A.col, A.shape, shared=True))
         A._sell = S
     sm = S[1]
     sm.set_values(A.values)

@@ -227,6 +227,17 @@ typedef struct {
     const void* vals;          /* FP64 or FP32, as the call's dtype says        */
 } fa_sell_t;
 
+/* Compressed SELL-64 for the FP32 smoother: the row-scaled operator
+ * D^-1 A (entries in [-1, 1]) in FP16 with int16 column offsets from the row
+ * (same slices as the full SELL; padding: offset 0, value 0).  4 bytes per
+ * entry instead of 8. */
+typedef struct {
+    int64_t n;
+    const int32_t* slice_ptr;
+    const int16_t* offs;       /* column - row                                  */
+    const void* vals;          /* FP16 (a_ij / a_ii)                            */
+} fa_sell16_t;
+
 size_t fa_sell_work_bytes(int64_t n);
 int fa_sell_count(int64_t n, const int32_t* indptr, int32_t* slice_ptr, int64_t* entries_host,
                   void* work, size_t work_bytes, fa_stream_t stream);
@@ -275,6 +286,8 @@ typedef struct {
     void* x2;
     void* d;
     void* r;
+    fa_sell16_t Ah;            /* compressed D^-1 A for an FP32 cycle, or vals = NULL */
+    const void* diag;          /* diag(A) in the cycle dtype (with Ah)          */
 } fa_mg_level_t;
 
 typedef struct {
@@ -291,6 +304,14 @@ typedef struct {
                                   stays a valid CG preconditioner)              */
     int coarse_degree;
 } fa_mg_t;
+
+/* Fill a compressed SELL (offsets and FP16 values of D^-1 A) from the FP64
+ * CSR, on the slices `slice_ptr` of the level's SELL.  Returns FA_ECAPACITY
+ * when a column lies more than 32767 from its row (the level then keeps the
+ * full format); syncs. */
+int fa_sell16_fill(int64_t n, const int32_t* indptr, const int32_t* indices, const double* values,
+                   const int32_t* slice_ptr, int16_t* offs, void* vals, void* work, size_t work_bytes,
+                   fa_stream_t stream);
 
 /* Dense row-major copy of a small CSR matrix (FP64) and its in-place inverse
  * (Gauss-Jordan, SPD so no pivoting; n <= 2048, single workgroup).  The

@@ -188,3 +188,18 @@ def test_mfma_fragment_order(per_lane, ks, tpg, groups):
         col = (g * tpg + T) * 32 + l % 32
         want = float(K[s, k, col]) if (k < nq and col < nn) else 0.0
         assert float(flat[g, s, T, j, l, t]) == want
+
+
+def test_serving_trace_is_deterministic_and_in_range():
+    from fealpy_amd.serving import make_requests, request_bytes
+    a = make_requests(64, seed=7, rate=100.0)
+    b = make_requests(64, seed=7, rate=100.0)
+    assert [(r.nx, r.p, r.arrival) for r in a] == [(r.nx, r.p, r.arrival) for r in b]
+    assert all(torch.equal(x.k, y.k) for x, y in zip(a, b))
+    assert all(16 <= r.nx <= 512 and r.p in (1, 2) for r in a)
+    assert all(r.k.shape[0] == 2 * r.nx * r.nx and float(r.k.min()) > 0 for r in a)
+    arr = [r.arrival for r in a]
+    assert arr == sorted(arr) and arr[0] > 0
+    assert request_bytes(32, 2) > request_bytes(32, 1) > 0
+    z = make_requests(8, seed=7)
+    assert all(r.arrival == 0.0 for r in z)
