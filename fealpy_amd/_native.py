@@ -38,7 +38,7 @@ _SIGNATURES = {
                                            _p, _p, _p, ctypes.c_int, _f64, _p, ctypes.c_int, _f64, _p, _p, _p]),
     "fa_element_vectors": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _p, _p, _i64,
                                           _p, _p, ctypes.c_int, _f64, _p, _p, _p]),
-    "fa_gather_matrix": (ctypes.c_int, [ctypes.c_int, _i64, _i32, _p, _p, _p, _p, _p, _i32, _p, _p, _p]),
+    "fa_gather_matrix": (ctypes.c_int, [ctypes.c_int, _i64, _i32, _p, _p, _p, _p, _p, _i32, _p, _i64, _p, _p]),
     "fa_gather_vector": (ctypes.c_int, [ctypes.c_int, _i64, _i32, _p, _p, _p, _p, _p]),
     "fa_element_elasticity": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _p, _p, _i64, _p,
                                              ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -50,7 +50,8 @@ _SIGNATURES = {
     "fa_mfma_work_bytes": (ctypes.c_size_t, [ctypes.c_int, _i64, ctypes.c_int]),
     "fa_element_matrices_mfma": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                 ctypes.c_int, _p, _p, _i64, _p, _p, ctypes.c_int, ctypes.c_double, _p,
-                                                ctypes.c_int, ctypes.c_double, _p, _p, _p, ctypes.c_size_t, _p]),
+                                                ctypes.c_int, ctypes.c_double, _p, _p, _i64, _p, ctypes.c_size_t,
+                                                _p]),
     "fa_assemble_p1_matrix": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64, ctypes.c_int, _p, _p, _p, _p,
                                              ctypes.c_int, ctypes.c_double, _p, ctypes.c_int, ctypes.c_double, _p,
                                              _p, _p, _p, _p, _i32, _p, _p]),

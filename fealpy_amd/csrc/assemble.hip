@@ -193,7 +193,7 @@ template <class T, int G>
 __global__ void k_gather_matrix(int64_t gdof, int ldof, const int32_t* __restrict__ c2d,
                                 const int32_t* __restrict__ ptr, const int32_t* __restrict__ adj,
                                 const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices, int lmax,
-                                const T* __restrict__ elem, T* __restrict__ values) {
+                                const T* __restrict__ elem, int64_t cstride, T* __restrict__ values) {
     extern __shared__ char smem[];
     const int rpb = blockDim.x / G;
     const int sub = threadIdx.x / G;
@@ -215,7 +215,7 @@ __global__ void k_gather_matrix(int64_t gdof, int ldof, const int32_t* __restric
             const int32_t c = e / ldof;
             for (int j = lane; j < ldof; j += G) {
                 const int32_t col = c2d[(int64_t)c * ldof + j];
-                const T v = elem[(int64_t)e * ldof + j];
+                const T v = elem[(int64_t)c * cstride + (e - c * ldof) * ldof + j];
                 const int pos = lower_bound_lds(cols, L, col);
                 if (pos < L && cols[pos] == col) acc[pos] += v;
             }
@@ -637,8 +637,8 @@ static int dispatch_elem(int td, int ldof, const double* node, const int32_t* ce
 
 template <class T, int G>
 static int launch_gather(int64_t gdof, int ldof, const int32_t* c2d, const int32_t* ptr, const int32_t* adj,
-                         const int32_t* indptr, const int32_t* indices, int lmax, const void* elem, void* values,
-                         hipStream_t s) {
+                         const int32_t* indptr, const int32_t* indices, int lmax, const void* elem, int64_t cstride,
+                         void* values, hipStream_t s) {
     const size_t per_row = sizeof(int32_t) * lmax + sizeof(T) * lmax;
     int rpb = 256 / G;
     while (rpb > 1 && rpb * per_row + 16 > 48 * 1024) rpb >>= 1;
@@ -649,20 +649,22 @@ static int launch_gather(int64_t gdof, int ldof, const int32_t* c2d, const int32
     }
     const unsigned grid = grid_for(gdof, rpb, 1 << 20);
     hipLaunchKernelGGL((k_gather_matrix<T, G>), dim3(grid), dim3(rpb * G), lds, s, gdof, ldof, c2d, ptr, adj, indptr,
-                       indices, lmax, static_cast<const T*>(elem), static_cast<T*>(values));
+                       indices, lmax, static_cast<const T*>(elem), cstride, static_cast<T*>(values));
     FA_LAUNCH("k_gather_matrix");
     return FA_OK;
 }
 
 template <class T>
 static int dispatch_gather(int64_t gdof, int ldof, const int32_t* c2d, const int32_t* ptr, const int32_t* adj,
-                           const int32_t* indptr, const int32_t* indices, int lmax, const void* elem, void* values,
-                           hipStream_t s) {
-    if (ldof <= 4) return launch_gather<T, 4>(gdof, ldof, c2d, ptr, adj, indptr, indices, lmax, elem, values, s);
-    if (ldof <= 8) return launch_gather<T, 8>(gdof, ldof, c2d, ptr, adj, indptr, indices, lmax, elem, values, s);
-    if (ldof <= 16) return launch_gather<T, 16>(gdof, ldof, c2d, ptr, adj, indptr, indices, lmax, elem, values, s);
-    if (ldof <= 32) return launch_gather<T, 32>(gdof, ldof, c2d, ptr, adj, indptr, indices, lmax, elem, values, s);
-    return launch_gather<T, 64>(gdof, ldof, c2d, ptr, adj, indptr, indices, lmax, elem, values, s);
+                           const int32_t* indptr, const int32_t* indices, int lmax, const void* elem, int64_t cs,
+                           void* values, hipStream_t s) {
+    if (ldof <= 4) return launch_gather<T, 4>(gdof, ldof, c2d, ptr, adj, indptr, indices, lmax, elem, cs, values, s);
+    if (ldof <= 8) return launch_gather<T, 8>(gdof, ldof, c2d, ptr, adj, indptr, indices, lmax, elem, cs, values, s);
+    if (ldof <= 16)
+        return launch_gather<T, 16>(gdof, ldof, c2d, ptr, adj, indptr, indices, lmax, elem, cs, values, s);
+    if (ldof <= 32)
+        return launch_gather<T, 32>(gdof, ldof, c2d, ptr, adj, indptr, indices, lmax, elem, cs, values, s);
+    return launch_gather<T, 64>(gdof, ldof, c2d, ptr, adj, indptr, indices, lmax, elem, cs, values, s);
 }
 
 }  // namespace fa
@@ -718,14 +720,16 @@ extern "C" int fa_element_vectors(int dtype, int td, int ldof, int nq, const dou
 
 extern "C" int fa_gather_matrix(int dtype, int64_t gdof, int32_t ldof, const int32_t* cell2dof,
                                 const int32_t* adj_ptr, const int32_t* adj, const int32_t* indptr,
-                                const int32_t* indices, int32_t max_row, const void* elem, void* values,
-                                fa_stream_t stream) {
+                                const int32_t* indices, int32_t max_row, const void* elem, int64_t elem_stride,
+                                void* values, fa_stream_t stream) {
     FA_REQUIRE(dtype == FA_F64 || dtype == FA_F32, "fa_gather_matrix: bad dtype %d", dtype);
     FA_REQUIRE(gdof > 0 && ldof > 0 && max_row > 0, "fa_gather_matrix: bad sizes");
+    const int64_t cs = elem_stride > 0 ? elem_stride : (int64_t)ldof * ldof;
+    FA_REQUIRE(cs >= (int64_t)ldof * ldof, "fa_gather_matrix: element stride %lld < ldof^2", (long long)cs);
     if (dtype == FA_F64)
-        return dispatch_gather<double>(gdof, ldof, cell2dof, adj_ptr, adj, indptr, indices, max_row, elem, values,
+        return dispatch_gather<double>(gdof, ldof, cell2dof, adj_ptr, adj, indptr, indices, max_row, elem, cs, values,
                                        S(stream));
-    return dispatch_gather<float>(gdof, ldof, cell2dof, adj_ptr, adj, indptr, indices, max_row, elem, values,
+    return dispatch_gather<float>(gdof, ldof, cell2dof, adj_ptr, adj, indptr, indices, max_row, elem, cs, values,
                                   S(stream));
 }
 

@@ -12,7 +12,7 @@
 //
 // Work split: blockIdx.y picks a slice of TPG column tiles (32 ij each) whose
 // reference tensors are staged in LDS once; the block's 4 wavefronts take
-// 32-cell tiles of the block's cell range in turn.
+// pairs of 32-cell tiles of the block's cell range in turn.
 #include "common.hpp"
 
 namespace fa {
@@ -136,7 +136,8 @@ __global__ void __launch_bounds__(256) k_element_mfma(int64_t nc, int ldof, cons
                                                      const typename MfTraits<KS, BF16>::elem* __restrict__ am,
                                                      int64_t am_stride,
                                                      const typename MfTraits<KS, BF16>::elem* __restrict__ btab,
-                                                     const float* __restrict__ metric, float* __restrict__ elem) {
+                                                     const float* __restrict__ metric, int64_t ld,
+                                                     float* __restrict__ elem) {
     typedef MfTraits<KS, BF16> Tr;
     typedef typename Tr::elem E;
     typedef typename Tr::frag F;
@@ -144,7 +145,7 @@ __global__ void __launch_bounds__(256) k_element_mfma(int64_t nc, int ldof, cons
     constexpr int SLICE = MF_NS * TPG * KS * WAVE * Tr::PER_LANE;  // elements per column slice
     extern __shared__ char smem[];
     E* bl = reinterpret_cast<E*>(smem);
-    float* gl = reinterpret_cast<float*>(smem + align_up(sizeof(E) * SLICE, 16));  // [4 waves][8][32]
+    float* gl = reinterpret_cast<float*>(smem + align_up(sizeof(E) * SLICE, 16));  // [4 waves][2][8][32]
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = threadIdx.x / WAVE;
     const int group = blockIdx.y;
@@ -156,29 +157,35 @@ __global__ void __launch_bounds__(256) k_element_mfma(int64_t nc, int ldof, cons
     }
     __syncthreads();
     const int nn = ldof * ldof;
-    float* gw = gl + wave * 8 * 32;
+    float* gw = gl + wave * 2 * 8 * 32;  // [2 tiles][8 metric rows][32 cells]
     const int64_t cbase = (int64_t)blockIdx.x * MF_CELLS_PER_BLOCK;
-    for (int ct = wave; ct < MF_CELLS_PER_BLOCK / 32; ct += 4) {
-        const int64_t c0 = cbase + ct * 32;
+    // each wavefront takes two 32-cell tiles at a time: every B fragment read
+    // from LDS feeds two MFMA chains
+    for (int pt = wave; pt < MF_CELLS_PER_BLOCK / 64; pt += 4) {
+        const int64_t c0 = cbase + pt * 64;
         if (c0 >= nc) break;
-        const int64_t cl = c0 + (lane & 31);
-        const bool valid = cl < nc;
-        {   // metric of the tile's 32 cells, transposed to [s][cell]: 8 floats per cell, two lanes per cell
-            const int cc = lane & 31, h = lane >> 5;
-            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (valid) v = reinterpret_cast<const float4*>(metric + cl * 8)[h];
-            gw[(4 * h + 0) * 32 + cc] = v.x;
-            gw[(4 * h + 1) * 32 + cc] = v.y;
-            gw[(4 * h + 2) * 32 + cc] = v.z;
-            gw[(4 * h + 3) * 32 + cc] = v.w;
-        }
-        F fa[KS], fm[KS];
-        const int64_t crow = valid ? cl : c0;
+        F fa[2][KS], fm[2][KS];
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks) {
-            const int k = ks * Tr::KSTEP + Tr::PER_LANE * (lane / 32);
-            fa[ks] = Tr::load(ad + crow * ad_stride + k);
-            fm[ks] = Tr::load(am + crow * am_stride + k);
+        for (int h2 = 0; h2 < 2; ++h2) {
+            const int64_t cl = c0 + 32 * h2 + (lane & 31);
+            const bool valid = cl < nc;
+            {   // metric of the tile's 32 cells, transposed to [s][cell]: two lanes per cell
+                const int cc = lane & 31, h = lane >> 5;
+                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (valid) v = reinterpret_cast<const float4*>(metric + cl * 8)[h];
+                float* g = gw + h2 * 8 * 32;
+                g[(4 * h + 0) * 32 + cc] = v.x;
+                g[(4 * h + 1) * 32 + cc] = v.y;
+                g[(4 * h + 2) * 32 + cc] = v.z;
+                g[(4 * h + 3) * 32 + cc] = v.w;
+            }
+            const int64_t crow = valid ? cl : c0;
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                const int k = ks * Tr::KSTEP + Tr::PER_LANE * (lane / 32);
+                fa[h2][ks] = Tr::load(ad + crow * ad_stride + k);
+                fm[h2][ks] = Tr::load(am + crow * am_stride + k);
+            }
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -186,30 +193,40 @@ __global__ void __launch_bounds__(256) k_element_mfma(int64_t nc, int ldof, cons
 #pragma unroll 1
         for (int tile = 0; tile < TPG; ++tile) {
             const int col = (group * TPG + tile) * 32 + (lane & 31);
-            floatx16 tot = {};
+            floatx16 tot0 = {}, tot1 = {};
 #pragma unroll 1
-            for (int s = 0; s < MF_NS; ++s) {  // one accumulator chain live at a time
-                floatx16 acc = {};
+            for (int s = 0; s < MF_NS; ++s) {  // one pair of accumulator chains live at a time
+                floatx16 acc0 = {}, acc1 = {};
                 const E* bs = bl + ((s * TPG + tile) * KS) * WAVE * Tr::PER_LANE + lane * Tr::PER_LANE;
                 const bool mass = s == MF_NS - 1;
 #pragma unroll
-                for (int ks = 0; ks < KS; ++ks)
-                    acc = Tr::mma(mass ? fm[ks] : fa[ks], Tr::load(bs + ks * WAVE * Tr::PER_LANE), acc);
-                const float4* gs = reinterpret_cast<const float4*>(grow + s * 32);
+                for (int ks = 0; ks < KS; ++ks) {
+                    const F b = Tr::load(bs + ks * WAVE * Tr::PER_LANE);
+                    acc0 = Tr::mma(mass ? fm[0][ks] : fa[0][ks], b, acc0);
+                    acc1 = Tr::mma(mass ? fm[1][ks] : fa[1][ks], b, acc1);
+                }
+                const float4* g0 = reinterpret_cast<const float4*>(grow + s * 32);
+                const float4* g1 = reinterpret_cast<const float4*>(grow + 8 * 32 + s * 32);
 #pragma unroll
                 for (int q4 = 0; q4 < 4; ++q4) {  // rows 8*q4 + (0..3) of this lane's half
-                    const float4 g4 = gs[2 * q4];
-                    tot[4 * q4 + 0] += g4.x * acc[4 * q4 + 0];
-                    tot[4 * q4 + 1] += g4.y * acc[4 * q4 + 1];
-                    tot[4 * q4 + 2] += g4.z * acc[4 * q4 + 2];
-                    tot[4 * q4 + 3] += g4.w * acc[4 * q4 + 3];
+                    const float4 a4 = g0[2 * q4], b4 = g1[2 * q4];
+                    tot0[4 * q4 + 0] += a4.x * acc0[4 * q4 + 0];
+                    tot0[4 * q4 + 1] += a4.y * acc0[4 * q4 + 1];
+                    tot0[4 * q4 + 2] += a4.z * acc0[4 * q4 + 2];
+                    tot0[4 * q4 + 3] += a4.w * acc0[4 * q4 + 3];
+                    tot1[4 * q4 + 0] += b4.x * acc1[4 * q4 + 0];
+                    tot1[4 * q4 + 1] += b4.y * acc1[4 * q4 + 1];
+                    tot1[4 * q4 + 2] += b4.z * acc1[4 * q4 + 2];
+                    tot1[4 * q4 + 3] += b4.w * acc1[4 * q4 + 3];
                 }
             }
             if (col < nn) {
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
-                    const int64_t c = c0 + 8 * (r / 4) + 4 * (lane / 32) + (r & 3);
-                    if (c < nc) elem[c * nn + col] = tot[r];
+                    const int row = 8 * (r / 4) + 4 * (lane / 32) + (r & 3);
+                    const int64_t ca = c0 + row, cb = c0 + 32 + row;
+                    if (ca < nc) elem[ca * ld + col] = tot0[r];
+                    if (cb < nc) elem[cb * ld + col] = tot1[r];
                 }
             }
         }
@@ -221,10 +238,10 @@ __global__ void __launch_bounds__(256) k_element_mfma(int64_t nc, int ldof, cons
 template <int KS, bool BF16, int TPG>
 static int launch_mfma(int64_t nc, int ldof, const double* node, const int32_t* cell, const void* ad,
                        int64_t ad_stride, const void* am, int64_t am_stride, const void* btab, int ngroups,
-                       const float* metric, float* elem, hipStream_t s) {
+                       const float* metric, int64_t ld, float* elem, hipStream_t s) {
     typedef typename MfTraits<KS, BF16>::elem E;
     constexpr int SLICE = MF_NS * TPG * KS * WAVE * MfTraits<KS, BF16>::PER_LANE;
-    const size_t lds = align_up(sizeof(E) * SLICE, 16) + sizeof(float) * 4 * 8 * 32;
+    const size_t lds = align_up(sizeof(E) * SLICE, 16) + sizeof(float) * 4 * 2 * 8 * 32;
     if (lds > 64 * 1024) {
         set_error("fa_element_matrices_mfma: LDS slice of %zu bytes exceeds 64 KiB", lds);
         return FA_ECAPACITY;
@@ -232,7 +249,7 @@ static int launch_mfma(int64_t nc, int ldof, const double* node, const int32_t* 
     const dim3 grid((unsigned)((nc + MF_CELLS_PER_BLOCK - 1) / MF_CELLS_PER_BLOCK), (unsigned)ngroups);
     hipLaunchKernelGGL((k_element_mfma<KS, BF16, TPG>), grid, dim3(256), lds, s, nc, ldof, node, cell,
                        static_cast<const E*>(ad), ad_stride, static_cast<const E*>(am), am_stride,
-                       static_cast<const E*>(btab), metric, elem);
+                       static_cast<const E*>(btab), metric, ld, elem);
     FA_LAUNCH("k_element_mfma");
     return FA_OK;
 }
@@ -244,7 +261,7 @@ using namespace fa;
 // Largest column-tile count per group whose LDS slice fits 64 KiB.
 extern "C" int fa_mfma_tiles_per_group(int precision, int ks) {
     const int per = MF_NS * ks * WAVE * (precision == 0 ? 8 * 2 : 4);  // bytes per tile
-    const int room = 64 * 1024 - 4 * 8 * 32 * 4 - 16;
+    const int room = 64 * 1024 - 4 * 2 * 8 * 32 * 4 - 16;
     int t = room / per;
     if (t >= 4) return 4;
     if (t >= 2) return 2;
@@ -261,12 +278,14 @@ extern "C" int fa_element_matrices_mfma(int precision, int ldof, int nq, int ks,
                                         const double* node, const int32_t* cell, int64_t nc, const double* qw,
                                         const void* btab, int diff_kind, double diff_scale, const double* diff_coef,
                                         int mass_kind, double mass_scale, const double* mass_coef, float* elem,
-                                        void* work, size_t work_bytes, fa_stream_t stream) {
+                                        int64_t elem_stride, void* work, size_t work_bytes, fa_stream_t stream) {
     FA_REQUIRE(precision == 0 || precision == 1, "fa_element_matrices_mfma: precision must be 0 (BF16) or 1 (FP32)");
     FA_REQUIRE(nc > 0 && ldof > 0 && nq > 0 && btab && elem, "fa_element_matrices_mfma: bad arguments");
     const int kstep = precision == 0 ? 16 : 2;
     FA_REQUIRE(ks * kstep >= nq, "fa_element_matrices_mfma: ks * %d < nq", kstep);
     FA_REQUIRE((int64_t)ngroups * tpg * 32 >= (int64_t)ldof * ldof, "fa_element_matrices_mfma: too few column tiles");
+    const int64_t ld = elem_stride > 0 ? elem_stride : (int64_t)ldof * ldof;
+    FA_REQUIRE(ld >= (int64_t)ldof * ldof, "fa_element_matrices_mfma: element stride below ldof^2");
     FA_REQUIRE(!(diff_kind >= FA_COEF_CELL && !diff_coef) && !(mass_kind >= FA_COEF_CELL && !mass_coef),
                "fa_element_matrices_mfma: coefficient array missing");
     hipStream_t s = S(stream);
@@ -297,7 +316,7 @@ extern "C" int fa_element_matrices_mfma(int precision, int ldof, int nq, int ks,
     FA_LAUNCH("k_mfma_coef");
     const int64_t das = dcell ? kp : 0, mas = mcell ? kp : 0;
 #define FA_MF(KS_, BF_, TPG_) \
-    return launch_mfma<KS_, BF_, TPG_>(nc, ldof, node, cell, ad, das, am, mas, btab, ngroups, metric, elem, s)
+    return launch_mfma<KS_, BF_, TPG_>(nc, ldof, node, cell, ad, das, am, mas, btab, ngroups, metric, ld, elem, s)
 #define FA_MF_TPG(KS_, BF_)                 \
     do {                                    \
         if (tpg == 4) FA_MF(KS_, BF_, 4);  \

@@ -4,7 +4,8 @@
 fa_element_matrices_mfma: a GEMM over quadrature points per metric component
 on v_mfma_f32_32x32x16_bf16 (BF16 inputs, FP32 accumulation) or
 v_mfma_f32_32x32x2f32 (FP32), the per-cell metric applied to the FP32
-accumulators.  Output is FP32 (NC, ldof, ldof).  The reference tensors
+accumulators.  Output is FP32 (NC, ldof, ldof), cells padded to a multiple
+of 32 values apart (a strided view).  The reference tensors
 K_s[q][ij] are built here in FP64 and stored in the fragment order of the
 kernel:  [group][s][tile][k-step][lane][t].
 """
@@ -90,13 +91,16 @@ def element_matrices_mfma(space, diff=None, mass=None, q=None, precision='bf16')
     (dk, ds, dc) = diff if diff is not None else (nat.COEF_NONE, 0.0, None)
     (mk, ms, mc) = mass if mass is not None else (nat.COEF_NONE, 0.0, None)
     prec = PRECISIONS[precision]
-    elem = torch.empty((nc, ldof, ldof), dtype=torch.float32, device=space.device)
+    # cells 32 floats apart (a whole number of 128-byte lines): every store of
+    # a 32-column tile then covers full cache lines
+    ld = ((ldof * ldof + 31) // 32) * 32
+    buf = torch.empty((nc, ld), dtype=torch.float32, device=space.device)
     ws = nat.workspace(L.fa_mfma_work_bytes(prec, nc, ks), space.device)
     nat.check(L.fa_element_matrices_mfma(prec, ldof, qw.shape[0], ks, tpg, groups, nat.ptr(mesh.node),
                                          nat.ptr(cell), nc, nat.ptr(qw), nat.ptr(btab), dk, ds, nat.ptr(dc),
-                                         mk, ms, nat.ptr(mc), nat.ptr(elem), nat.ptr(ws), ws.numel(),
+                                         mk, ms, nat.ptr(mc), nat.ptr(buf), ld, nat.ptr(ws), ws.numel(),
                                          nat.stream()), "fa_element_matrices_mfma")
-    return elem
+    return torch.as_strided(buf, (nc, ldof, ldof), (ld, ldof, 1))
 
 
 def mfma_flops(nc, nq, ldof):

@@ -46,15 +46,16 @@ class Pattern:
     def gather_matrix(self, elem, values=None):
         """CSR values from element matrices (NC, ldof, ldof) in ascending-cell order."""
         L = nat.lib()
-        elem = elem.contiguous()
         if tuple(elem.shape) != (self.nc, self.ldof, self.ldof):
             raise ValueError(f"element tensor shape {tuple(elem.shape)} != {(self.nc, self.ldof, self.ldof)}")
+        if not (elem.stride(2) == 1 and elem.stride(1) == self.ldof and elem.stride(0) >= self.ldof ** 2):
+            elem = elem.contiguous()                      # cells may be padded apart; rows must be dense
         if values is None:
             values = torch.empty(self.nnz, dtype=elem.dtype, device=elem.device)
         nat.check(L.fa_gather_matrix(nat.dtype_code(elem.dtype), self.gdof, self.ldof, nat.ptr(self.cell2dof),
                                      nat.ptr(self.adj_ptr), nat.ptr(self.adj), nat.ptr(self.indptr),
-                                     nat.ptr(self.indices), max(self.max_row, 1), nat.ptr(elem), nat.ptr(values),
-                                     nat.stream()), "fa_gather_matrix")
+                                     nat.ptr(self.indices), max(self.max_row, 1), nat.ptr(elem), elem.stride(0),
+                                     nat.ptr(values), nat.stream()), "fa_gather_matrix")
         return values
 
     def gather_vector(self, elem, out=None):

@@ -69,16 +69,19 @@ class SellMatrix:
     def set_compressed(self, values, work):
         """Compressed FP16 D^-1 A with int16 column offsets (fa_sell16_fill)
         from FP64 CSR values; False when a column is too far from its row."""
-        if self.entries == 0:
+        if self.entries == 0 or getattr(self, "_h_band_ok", True) is False:
             return False
-        if self.h_offs is None:
+        checked = self.h_offs is not None
+        if not checked:
             self.h_offs = torch.zeros(self.entries, dtype=torch.int16, device=values.device)
             self.h_vals = torch.zeros(self.entries, dtype=torch.float16, device=values.device)
         st = nat.lib().fa_sell16_fill(self.shape[0], nat.ptr(self.crow), nat.ptr(self.col),
                                       nat.ptr(values.contiguous()), nat.ptr(self.slice_ptr), nat.ptr(self.h_offs),
-                                      nat.ptr(self.h_vals), nat.ptr(work), work.numel(), nat.stream())
+                                      nat.ptr(self.h_vals), None if checked else nat.ptr(work),
+                                      0 if checked else work.numel(), nat.stream())
         if st == nat.FA_ECAPACITY:
             self.h_offs = self.h_vals = None
+            self._h_band_ok = False
             return False
         nat.check(st, "fa_sell16_fill")
         return True

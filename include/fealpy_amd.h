@@ -115,11 +115,12 @@ int fa_element_vectors(int dtype, int td, int ldof, int nq,
  * Kernel "c": CSR values from element matrices by a fixed-order segmented
  * sum over the DOF->cell incidence (no atomics).  Replaces the value part of
  * coalesce (np.add.at in stable-sorted COO order, coo_tensor.py:197-200).
- * values[nnz] in `dtype`; elem as written by fa_element_matrices (`dtype`). */
+ * values[nnz] in `dtype`; elem as written by fa_element_matrices (`dtype`),
+ * cell c's ldof x ldof block starting at c * elem_stride (0: ldof^2). */
 int fa_gather_matrix(int dtype, int64_t gdof, int32_t ldof, const int32_t* cell2dof,
                      const int32_t* adj_ptr, const int32_t* adj,
                      const int32_t* indptr, const int32_t* indices, int32_t max_row,
-                     const void* elem, void* values, fa_stream_t stream);
+                     const void* elem, int64_t elem_stride, void* values, fa_stream_t stream);
 /* Dense global vector from element vectors: LinearForm.assembly(format='dense')
  * (fealpy/fem/linear_form.py:66-86 -> COOTensor.to_dense, coo_tensor.py:116-130). */
 int fa_gather_vector(int dtype, int64_t gdof, int32_t ldof,
@@ -143,6 +144,27 @@ int fa_element_elasticity(int dtype, int td, int ldof, const double* node, const
 int fa_element_vectors_tensor(int dtype, int td, int ldof, int ncomp, int dof_priority, int nq,
                               const double* node, const int32_t* cell, int64_t nc, const double* qw,
                               const double* phi, const double* src, void* elem, fa_stream_t stream);
+
+/* High-order (tetrahedral P2-P4) diffusion + mass element matrices on the
+ * matrix cores: the reduced-precision variant of fa_element_matrices for
+ * ScalarDiffusionIntegrator / ScalarMassIntegrator.assembly
+ * (scalar_diffusion_integrator.py:54-63, scalar_mass_integrator.py:41-54).
+ * A_ij = sum_{s<6} |T| G_s sum_q w_q k(c,q) K_s[q][ij] + |T| sum_q w_q m(c,q) K_6[q][ij]
+ * with G_s the metric grad lambda_m . grad lambda_n (m <= n in 1..3) and K_s
+ * the reference tensors of the reduced gradients (K_6 = phi_i phi_j), built by
+ * the caller in the fragment order [group][s][tile][k-step][lane][t]
+ * (fealpy_amd/fem/highorder.py).  precision 0: BF16 inputs
+ * (v_mfma_f32_32x32x16_bf16), 1: FP32 (v_mfma_f32_32x32x2f32); FP32
+ * accumulation and output.  ks = k-steps (16 or 2 points each, zero padded);
+ * tpg = 32-column tiles per LDS slice (fa_mfma_tiles_per_group); cell c's
+ * block starts at c * elem_stride (0: ldof^2). */
+int fa_mfma_tiles_per_group(int precision, int ks);
+size_t fa_mfma_work_bytes(int precision, int64_t nc, int ks);
+int fa_element_matrices_mfma(int precision, int ldof, int nq, int ks, int tpg, int ngroups,
+                             const double* node, const int32_t* cell, int64_t nc, const double* qw,
+                             const void* btab, int diff_kind, double diff_scale, const double* diff_coef,
+                             int mass_kind, double mass_scale, const double* mass_coef, float* elem,
+                             int64_t elem_stride, void* work, size_t work_bytes, fa_stream_t stream);
 
 /* Fused P1 matrix assembly: geometry, element rows and the fixed-order CSR sum
  * in one kernel, no element buffer.  Diffusion and/or mass terms with the
@@ -306,9 +328,11 @@ typedef struct {
 } fa_mg_t;
 
 /* Fill a compressed SELL (offsets and FP16 values of D^-1 A) from the FP64
- * CSR, on the slices `slice_ptr` of the level's SELL.  Returns FA_ECAPACITY
- * when a column lies more than 32767 from its row (the level then keeps the
- * full format); syncs. */
+ * CSR, on the slices `slice_ptr` of the level's SELL.  With a workspace
+ * (>= 4 bytes) it checks the band and syncs: FA_ECAPACITY when a column lies
+ * more than 32767 from its row (the level then keeps the full format).
+ * work = NULL refreshes the values of an already checked pattern without a
+ * sync. */
 int fa_sell16_fill(int64_t n, const int32_t* indptr, const int32_t* indices, const double* values,
                    const int32_t* slice_ptr, int16_t* offs, void* vals, void* work, size_t work_bytes,
                    fa_stream_t stream);
