@@ -42,6 +42,44 @@ __global__ void __launch_bounds__(256) k_sell_fill(int64_t n, int64_t ncols, con
     }
 }
 
+// Values-only refill with the slice staged in LDS: the CSR entries of a
+// 64-row slice are one contiguous range, read with coalesced loads; each
+// lane then writes its row's entries at the slice's column stride.
+// One wavefront per slice; slices whose range exceeds SF_STAGE use the
+// per-row path above.
+constexpr int SF_STAGE = 2048;
+
+template <class A, class B>
+__global__ void __launch_bounds__(WAVE) k_sell_fill_staged(int64_t n, const int32_t* __restrict__ indptr,
+                                                          const A* __restrict__ vin,
+                                                          const int32_t* __restrict__ slice_ptr,
+                                                          B* __restrict__ vals) {
+    __shared__ A stage[SF_STAGE];
+    const int lane = threadIdx.x;
+    const int64_t nslices = (n + 63) / 64;
+    for (int64_t s = blockIdx.x; s < nslices; s += gridDim.x) {
+        const int64_t row = s * 64 + lane;
+        const int64_t rlast = (s * 64 + 64 < n) ? s * 64 + 64 : n;
+        const int32_t base = indptr[s * 64];
+        const int32_t total = indptr[rlast] - base;
+        const int32_t p0 = slice_ptr[s];
+        const int w = (slice_ptr[s + 1] - p0) >> 6;
+        const int32_t r0 = row < n ? indptr[row] - base : 0;
+        const int len = row < n ? indptr[row + 1] - base - r0 : 0;
+        if (total <= SF_STAGE) {
+            for (int t = lane; t < total; t += WAVE) stage[t] = vin[base + t];
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            for (int k = 0; k < w; ++k) vals[p0 + 64 * k + lane] = k < len ? static_cast<B>(stage[r0 + k]) : B(0);
+        } else {
+            for (int k = 0; k < w; ++k)
+                vals[p0 + 64 * k + lane] = k < len ? static_cast<B>(vin[base + r0 + k]) : B(0);
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+}
+
 template <class T>
 __global__ void __launch_bounds__(256) k_sell_spmv(int64_t n, const int32_t* __restrict__ slice_ptr,
                                                   const int32_t* __restrict__ cols, const T* __restrict__ vals,
@@ -93,6 +131,51 @@ __global__ void __launch_bounds__(256) k_sell16_fill(int64_t n, const int32_t* _
     }
 }
 
+// compressed refill of an already checked pattern, slice staged in LDS
+// (values and columns of the slice's contiguous CSR range)
+__global__ void __launch_bounds__(WAVE) k_sell16_refill(int64_t n, const int32_t* __restrict__ indptr,
+                                                       const int32_t* __restrict__ indices,
+                                                       const double* __restrict__ vin,
+                                                       const int32_t* __restrict__ slice_ptr,
+                                                       _Float16* __restrict__ vals) {
+    __shared__ double sv[SF_STAGE];
+    __shared__ int32_t si[SF_STAGE];
+    const int lane = threadIdx.x;
+    const int64_t nslices = (n + 63) / 64;
+    for (int64_t s = blockIdx.x; s < nslices; s += gridDim.x) {
+        const int64_t row = s * 64 + lane;
+        const int64_t rlast = (s * 64 + 64 < n) ? s * 64 + 64 : n;
+        const int32_t base = indptr[s * 64];
+        const int32_t total = indptr[rlast] - base;
+        const int32_t p0 = slice_ptr[s];
+        const int w = (slice_ptr[s + 1] - p0) >> 6;
+        const int32_t r0 = row < n ? indptr[row] - base : 0;
+        const int len = row < n ? indptr[row + 1] - base - r0 : 0;
+        const bool staged = total <= SF_STAGE;
+        if (staged) {
+            for (int t = lane; t < total; t += WAVE) {
+                sv[t] = vin[base + t];
+                si[t] = indices[base + t];
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        double d = 0.0;
+        for (int k = 0; k < len; ++k) {
+            const int32_t c = staged ? si[r0 + k] : indices[base + r0 + k];
+            if (c == row) d = staged ? sv[r0 + k] : vin[base + r0 + k];
+        }
+        const double inv = d != 0.0 ? 1.0 / d : 1.0;
+        for (int k = 0; k < w; ++k) {
+            double v = 0.0;
+            if (k < len) v = (staged ? sv[r0 + k] : vin[base + r0 + k]) * inv;
+            vals[p0 + 64 * k + lane] = static_cast<_Float16>(v);
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+}
+
 }  // namespace fa
 
 using namespace fa;
@@ -104,11 +187,16 @@ extern "C" int fa_sell16_fill(int64_t n, const int32_t* indptr, const int32_t* i
     FA_REQUIRE(!work || work_bytes >= sizeof(int32_t), "fa_sell16_fill: workspace too small");
     hipStream_t s = S(stream);
     int32_t* flag = static_cast<int32_t*>(work);
-    if (flag) FA_HIP(hipMemsetAsync(flag, 0, sizeof(int32_t), s));
+    if (!flag) {  // pattern already checked: values only, no sync
+        hipLaunchKernelGGL(k_sell16_refill, dim3(grid_for((n + 63) / 64, 1, 1 << 20)), dim3(WAVE), 0, s, n, indptr,
+                           indices, values, slice_ptr, static_cast<_Float16*>(vals));
+        FA_LAUNCH("k_sell16_refill");
+        return FA_OK;
+    }
+    FA_HIP(hipMemsetAsync(flag, 0, sizeof(int32_t), s));
     hipLaunchKernelGGL(k_sell16_fill, dim3(grid_for(((n + 63) / 64) * 64, 256, 1 << 16)), dim3(256), 0, s, n, indptr,
                        indices, values, slice_ptr, offs, static_cast<_Float16*>(vals), flag);
     FA_LAUNCH("k_sell16_fill");
-    if (!flag) return FA_OK;  // pattern already checked: values only, no sync
     int32_t h = 0;
     FA_HIP(hipMemcpyAsync(&h, flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     FA_HIP(hipStreamSynchronize(s));
@@ -154,6 +242,23 @@ extern "C" int fa_sell_fill(int dtype_in, int dtype_out, int64_t n, int64_t ncol
     FA_REQUIRE(n > 0 && ncols > 0, "fa_sell_fill: empty matrix");
     FA_REQUIRE(!vals || values, "fa_sell_fill: values missing");
     hipStream_t s = S(stream);
+    if (!cols && vals) {  // values-only refresh: slices staged in LDS
+        const unsigned gs = grid_for((n + 63) / 64, 1, 1 << 20);
+#define FA_SS(A_, B_)                                                                                       \
+    hipLaunchKernelGGL((k_sell_fill_staged<A_, B_>), dim3(gs), dim3(WAVE), 0, s, n, indptr,                 \
+                       static_cast<const A_*>(values), slice_ptr, static_cast<B_*>(vals))
+        if (dtype_in == FA_F64 && dtype_out == FA_F64) FA_SS(double, double);
+        else if (dtype_in == FA_F64 && dtype_out == FA_F32) FA_SS(double, float);
+        else if (dtype_in == FA_F32 && dtype_out == FA_F32) FA_SS(float, float);
+        else if (dtype_in == FA_F32 && dtype_out == FA_F64) FA_SS(float, double);
+        else {
+            set_error("fa_sell_fill: bad dtypes");
+            return FA_EINVAL;
+        }
+#undef FA_SS
+        FA_LAUNCH("k_sell_fill_staged");
+        return FA_OK;
+    }
     const unsigned grid = grid_for(((n + 63) / 64) * 64, 256, 1 << 16);
 #define FA_SF(A_, B_)                                                                                       \
     hipLaunchKernelGGL((k_sell_fill<A_, B_>), dim3(grid), dim3(256), 0, s, n, ncols, indptr, indices,      \
