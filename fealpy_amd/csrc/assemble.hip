@@ -493,17 +493,18 @@ __global__ void __launch_bounds__(256) k_assemble_p1(int64_t gdof, const double*
                                                     const int32_t* __restrict__ indices, int lmax,
                                                     T* __restrict__ values) {
     constexpr int NV = TD + 1;
+    constexpr int GS = G + 1;  // table row stride: odd, so owner lanes reading their rows hit distinct banks
     extern __shared__ char smem[];
     const int rpb = blockDim.x / G;
     const int sub = threadIdx.x / G;
     const int lane = threadIdx.x % G;
     int32_t* cols = reinterpret_cast<int32_t*>(smem) + sub * lmax;
-    double* tab = reinterpret_cast<double*>(smem + align_up(sizeof(int32_t) * rpb * lmax, 16)) + (int64_t)sub * lmax * G;
+    double* tab = reinterpret_cast<double*>(smem + align_up(sizeof(int32_t) * rpb * lmax, 16)) + (int64_t)sub * lmax * GS;
     for (int64_t r = (int64_t)blockIdx.x * rpb + sub; r < gdof; r += (int64_t)gridDim.x * rpb) {
         const int32_t r0 = indptr[r];
         const int L = indptr[r + 1] - r0;
         for (int t = lane; t < L; t += G) cols[t] = indices[r0 + t];
-        for (int t = 0; t < L; ++t) tab[t * G + lane] = 0.0;
+        for (int t = 0; t < L; ++t) tab[t * GS + lane] = 0.0;
         double acc[K];
 #pragma unroll
         for (int m = 0; m < K; ++m) acc[m] = 0.0;
@@ -526,7 +527,7 @@ __global__ void __launch_bounds__(256) k_assemble_p1(int64_t gdof, const double*
 #pragma unroll
                 for (int j = 0; j < NV; ++j) {
                     pos[j] = lower_bound_lds(cols, L, cell[c * NV + j]);
-                    tab[pos[j] * G + lane] = v[j];
+                    tab[pos[j] * GS + lane] = v[j];
                 }
             }
             __builtin_amdgcn_wave_barrier();
@@ -536,14 +537,14 @@ __global__ void __launch_bounds__(256) k_assemble_p1(int64_t gdof, const double*
                 const int t = lane + m * G;
                 if (t < L) {
 #pragma unroll
-                    for (int l = 0; l < G; ++l) acc[m] += tab[t * G + l];
+                    for (int l = 0; l < G; ++l) acc[m] += tab[t * GS + l];
                 }
             }
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             if (k < k1) {
 #pragma unroll
-                for (int j = 0; j < NV; ++j) tab[pos[j] * G + lane] = 0.0;
+                for (int j = 0; j < NV; ++j) tab[pos[j] * GS + lane] = 0.0;
             }
         }
 #pragma unroll
@@ -594,10 +595,10 @@ static int launch_p1(int64_t gdof, const double* node, const int32_t* cell, int 
                      const double* phi, int dk, double ds, const double* dc, int mk, double ms, const double* mc,
                      const int32_t* ptr, const int32_t* adj, const int32_t* indptr, const int32_t* indices, int lmax,
                      void* values, hipStream_t s) {
-    const size_t per_row = sizeof(int32_t) * lmax + sizeof(double) * lmax * G;
+    const size_t per_row = sizeof(int32_t) * lmax + sizeof(double) * lmax * (G + 1);
     int rpb = 256 / G;
     while (rpb > 1 && rpb * per_row + 16 > 48 * 1024) rpb >>= 1;
-    const size_t lds = align_up(sizeof(int32_t) * rpb * lmax, 16) + sizeof(double) * rpb * lmax * G;
+    const size_t lds = align_up(sizeof(int32_t) * rpb * lmax, 16) + sizeof(double) * rpb * lmax * (G + 1);
     if (lds > 64 * 1024) {
         set_error("fa_assemble_p1_matrix: row length %d exceeds the LDS capacity", lmax);
         return FA_ECAPACITY;

@@ -335,17 +335,23 @@ __global__ void __launch_bounds__(256) k_gj_pivot(int n, int k0, int b, double* 
     for (int t = threadIdx.x; t < b * b; t += blockDim.x) a[(int64_t)(k0 + t / b) * n + k0 + t % b] = m[t];
 }
 
+// row panel: one workgroup per block of GJB columns j outside K; the b x GJB
+// sub-panel is read into LDS before any of it is overwritten
 __global__ void __launch_bounds__(256) k_gj_rowpanel(int n, int k0, int b, double* __restrict__ a) {
-    __shared__ double P[GJB * GJB];
+    __shared__ double P[GJB * GJB], sub[GJB * GJB];
     for (int t = threadIdx.x; t < b * b; t += blockDim.x) P[t] = a[(int64_t)(k0 + t / b) * n + k0 + t % b];
-    __syncthreads();
-    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-        if (j >= k0 && j < k0 + b) continue;
-        double col[GJB];
-        for (int t = 0; t < b; ++t) col[t] = a[(int64_t)(k0 + t) * n + j];
-        for (int r = 0; r < b; ++r) {
+    for (int j0 = blockIdx.x * GJB; j0 < n; j0 += gridDim.x * GJB) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < b * GJB; t += blockDim.x) {
+            const int r = t / GJB, j = j0 + t % GJB;
+            sub[t] = j < n ? a[(int64_t)(k0 + r) * n + j] : 0.0;
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < b * GJB; t += blockDim.x) {
+            const int r = t / GJB, jj = t % GJB, j = j0 + jj;
+            if (j >= n || (j >= k0 && j < k0 + b)) continue;
             double s = 0.0;
-            for (int t = 0; t < b; ++t) s += P[r * b + t] * col[t];
+            for (int q = 0; q < b; ++q) s += P[r * b + q] * sub[q * GJB + jj];
             a[(int64_t)(k0 + r) * n + j] = s;
         }
     }
@@ -362,17 +368,22 @@ __global__ void __launch_bounds__(256) k_gj_trailing(int n, int k0, int b, doubl
     }
 }
 
+// column panel: one workgroup per block of GJB rows i outside K
 __global__ void __launch_bounds__(256) k_gj_colpanel(int n, int k0, int b, double* __restrict__ a) {
-    __shared__ double P[GJB * GJB];
+    __shared__ double P[GJB * GJB], sub[GJB * GJB];
     for (int t = threadIdx.x; t < b * b; t += blockDim.x) P[t] = a[(int64_t)(k0 + t / b) * n + k0 + t % b];
-    __syncthreads();
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        if (i >= k0 && i < k0 + b) continue;
-        double row[GJB];
-        for (int t = 0; t < b; ++t) row[t] = a[(int64_t)i * n + k0 + t];
-        for (int c = 0; c < b; ++c) {
+    for (int i0 = blockIdx.x * GJB; i0 < n; i0 += gridDim.x * GJB) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < GJB * b; t += blockDim.x) {
+            const int i = i0 + t / b, q = t % b;
+            sub[t] = i < n ? a[(int64_t)i * n + k0 + q] : 0.0;
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < GJB * b; t += blockDim.x) {
+            const int ii = t / b, c = t % b, i = i0 + ii;
+            if (i >= n || (i >= k0 && i < k0 + b)) continue;
             double s = 0.0;
-            for (int t = 0; t < b; ++t) s += row[t] * P[t * b + c];
+            for (int q = 0; q < b; ++q) s += sub[ii * b + q] * P[q * b + c];
             a[(int64_t)i * n + k0 + c] = -s;
         }
     }
@@ -1148,7 +1159,7 @@ extern "C" int fa_dense_spd_inverse(int64_t n, double* a, fa_stream_t stream) {
         hipLaunchKernelGGL(k_spd_inverse_lds, dim3(1), dim3(256), (n * n + 2 * n) * sizeof(double), s, (int)n, a);
     } else {
         const int nn = static_cast<int>(n);
-        const unsigned gp = grid_for(nn, 256, 1024), gt = grid_for((int64_t)nn * nn, 256, 1 << 14);
+        const unsigned gp = grid_for(nn, GJB, 1024), gt = grid_for((int64_t)nn * nn, 256, 1 << 14);
         for (int k0 = 0; k0 < nn; k0 += GJB) {
             const int b = std::min(GJB, nn - k0);
             hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(256), 0, s, nn, k0, b, a);
