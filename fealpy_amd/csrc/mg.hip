@@ -578,7 +578,7 @@ __device__ inline void blk_smooth(const FusedCycle& fc, const FusedLevel& L, con
             }
         } else {
             for (int64_t row = threadIdx.x; row < npad; row += blockDim.x) {
-                const T ax = sell_row_dot<T, T>(L.A.slice_ptr, L.A.cols, sv, row, cur);
+                const T ax = sell_row_dot<T, T, 8>(L.A.slice_ptr, L.A.cols, sv, row, cur);
                 if (row < n) {
                     const T res = b[row] - ax;
                     const T dn = (k == 0 ? T(0) : c1 * d[row]) + c2 * dinv[row] * res;
@@ -600,7 +600,7 @@ __device__ inline void blk_apply(const fa_sell_t& M, const T* x, T* y) {
     const int64_t npad = (M.n + 63) & ~63LL;
     const T* mv = static_cast<const T*>(M.vals);
     for (int64_t row = threadIdx.x; row < npad; row += blockDim.x) {
-        const T v = sell_row_dot<T, T>(M.slice_ptr, M.cols, mv, row, x);
+        const T v = sell_row_dot<T, T, 8>(M.slice_ptr, M.cols, mv, row, x);
         if (row < M.n) y[row] = ADD ? y[row] + v : v;
     }
     __syncthreads();
@@ -621,7 +621,7 @@ __global__ void __launch_bounds__(FUSE_BLOCK) k_fused_cycle(FusedCycle fc, const
         T* r = static_cast<T*>(L.r);
         const int64_t n = L.A.n, npad = (n + 63) & ~63LL;
         for (int64_t row = threadIdx.x; row < npad; row += blockDim.x) {
-            const T ax = sell_row_dot<T, T>(L.A.slice_ptr, L.A.cols, static_cast<const T*>(L.A.vals), row, cur[l]);
+            const T ax = sell_row_dot<T, T, 8>(L.A.slice_ptr, L.A.cols, static_cast<const T*>(L.A.vals), row, cur[l]);
             if (row < n) r[row] = b[row] - ax;
         }
         __syncthreads();

@@ -3,12 +3,23 @@
 // 64 lanes of a wave read 64 consecutive words per k (fully coalesced); the
 // slice width is the longest row of the slice, shorter rows are padded with
 // (col = own row or 0, value 0).
+//
+// Row dots load a batch of ROW_BATCH entries (columns and values), then the
+// ROW_BATCH gathered x values, then accumulate in entry order: a row of up to
+// ROW_BATCH entries costs two dependent memory latencies instead of one pair
+// per few entries, which is what bounds the small multigrid levels (few
+// waves per CU, nothing to hide latency behind).  Positions past the slice
+// width re-read the row's last entry (no branch, so the loads issue back to
+// back) and are dropped from the sum; the sum order is the entry order
+// k = 0, 1, ...
 #pragma once
 #include "common.hpp"
 
 namespace fa {
 
-template <class T, class X>
+constexpr int ROW_BATCH = 16;
+
+template <class T, class X, int B = ROW_BATCH>
 __device__ inline T sell_row_dot(const int32_t* __restrict__ slice_ptr, const int32_t* __restrict__ cols,
                                  const T* __restrict__ vals, int64_t row, const X* __restrict__ x) {
     const int64_t s = row >> 6;
@@ -18,18 +29,22 @@ __device__ inline T sell_row_dot(const int32_t* __restrict__ slice_ptr, const in
     const int32_t* c = cols + p0 + i;
     const T* v = vals + p0 + i;
     T acc = T(0);
-    int k = 0;
-    for (; k + 4 <= w; k += 4) {
-        const int32_t c0 = c[64 * k], c1 = c[64 * (k + 1)], c2 = c[64 * (k + 2)], c3 = c[64 * (k + 3)];
-        const T v0 = v[64 * k], v1 = v[64 * (k + 1)], v2 = v[64 * (k + 2)], v3 = v[64 * (k + 3)];
-        const T x0 = static_cast<T>(x[c0]), x1 = static_cast<T>(x[c1]), x2 = static_cast<T>(x[c2]),
-                x3 = static_cast<T>(x[c3]);
-        acc += v0 * x0;
-        acc += v1 * x1;
-        acc += v2 * x2;
-        acc += v3 * x3;
+    for (int k0 = 0; k0 < w; k0 += B) {
+        int32_t cc[B];
+        T vv[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const int k = min(k0 + u, w - 1);
+            cc[u] = c[64 * k];
+            vv[u] = v[64 * k];
+        }
+        T xx[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) xx[u] = static_cast<T>(x[cc[u]]);
+#pragma unroll
+        for (int u = 0; u < B; ++u)
+            if (k0 + u < w) acc += vv[u] * xx[u];
     }
-    for (; k < w; ++k) acc += v[64 * k] * static_cast<T>(x[c[64 * k]]);
     return acc;
 }
 
@@ -46,19 +61,22 @@ __device__ inline float sell16_row_dot(const int32_t* __restrict__ slice_ptr, co
     const int16_t* o = offs + p0 + i;
     const _Float16* v = vals + p0 + i;
     float acc = 0.f;
-    int k = 0;
-    for (; k + 4 <= w; k += 4) {
-        const int64_t c0 = row + o[64 * k], c1 = row + o[64 * (k + 1)], c2 = row + o[64 * (k + 2)],
-                      c3 = row + o[64 * (k + 3)];
-        const float v0 = static_cast<float>(v[64 * k]), v1 = static_cast<float>(v[64 * (k + 1)]),
-                    v2 = static_cast<float>(v[64 * (k + 2)]), v3 = static_cast<float>(v[64 * (k + 3)]);
-        const float x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
-        acc += v0 * x0;
-        acc += v1 * x1;
-        acc += v2 * x2;
-        acc += v3 * x3;
+    for (int k0 = 0; k0 < w; k0 += ROW_BATCH) {
+        int oo[ROW_BATCH];
+        float vv[ROW_BATCH];
+#pragma unroll
+        for (int u = 0; u < ROW_BATCH; ++u) {
+            const int k = min(k0 + u, w - 1);
+            oo[u] = o[64 * k];
+            vv[u] = static_cast<float>(v[64 * k]);
+        }
+        float xx[ROW_BATCH];
+#pragma unroll
+        for (int u = 0; u < ROW_BATCH; ++u) xx[u] = x[row + oo[u]];
+#pragma unroll
+        for (int u = 0; u < ROW_BATCH; ++u)
+            if (k0 + u < w) acc += vv[u] * xx[u];
     }
-    for (; k < w; ++k) acc += static_cast<float>(v[64 * k]) * x[row + o[64 * k]];
     return acc;
 }
 
