@@ -175,12 +175,31 @@ __global__ void __launch_bounds__(WAVE) k_rap_values(int64_t nrows, const int32_
 //   FROM_ZERO: x = c2 * D^-1 b, d = x
 //   otherwise: res = b - A x_in; d = c1 d + c2 D^-1 res; x_out = x_in + d
 //   (FIRST: the c1 d term is absent, so d need not be initialised)
-template <class T, bool FROM_ZERO, bool FIRST>
+//
+// The last step of the finest level's post-smoothing may also produce the
+// partial sums of r . x_out for the outer CG (DOT): r is the outer residual
+// (FP64 or FP32), one partial per workgroup; such a launch uses the
+// reduction grid red_grid(n) of RED_BLOCK threads, so the sum order is the
+// one of a separate dot kernel over the same vectors.
+struct DotOut {
+    const void* r;
+    double* partials;
+    int r_f64;
+};
+
+__device__ inline double dot_term(const DotOut& o, int64_t row, double v) {
+    const double rv = o.r_f64 ? static_cast<const double*>(o.r)[row] : static_cast<const float*>(o.r)[row];
+    return rv * v;
+}
+
+template <class T, bool FROM_ZERO, bool FIRST, bool DOT = false>
 __global__ void __launch_bounds__(256) k_cheb(int64_t n, const int32_t* __restrict__ sp, const int32_t* __restrict__ sc,
                                               const T* __restrict__ sv, const T* __restrict__ dinv,
                                               const T* __restrict__ b, const T* __restrict__ xin, T* __restrict__ xout,
-                                              T* __restrict__ d, T c1, T c2, const int* __restrict__ done) {
+                                              T* __restrict__ d, T c1, T c2, const int* __restrict__ done,
+                                              DotOut dot = DotOut{}) {
     if (done && *done) return;
+    double acc = 0.0;
     for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
          row += (int64_t)gridDim.x * blockDim.x) {
         if (FROM_ZERO) {
@@ -195,30 +214,80 @@ __global__ void __launch_bounds__(256) k_cheb(int64_t n, const int32_t* __restri
                 const T res = b[row] - ax;
                 const T dn = (FIRST ? T(0) : c1 * d[row]) + c2 * dinv[row] * res;
                 d[row] = dn;
-                xout[row] = xin[row] + dn;
+                const T xn = xin[row] + dn;
+                xout[row] = xn;
+                if (DOT) acc += dot_term(dot, row, double(xn));
             }
+        }
+    }
+    if (DOT) store_partial(block_sum(acc), dot.partials);
+}
+
+// First two steps of a sweep that starts from zero, in one pass: the first
+// step's x0 = c0 D^-1 b is formed at every gathered column instead of being
+// stored, then res = b - A x0, d = c1 x0 + c2 D^-1 res, x_out = x0 + d.  Same
+// operations and rounding as the two separate steps.
+template <class T>
+__global__ void __launch_bounds__(256) k_cheb_z(int64_t n, const int32_t* __restrict__ sp,
+                                                const int32_t* __restrict__ sc, const T* __restrict__ sv,
+                                                const T* __restrict__ dinv, const T* __restrict__ b,
+                                                T* __restrict__ xout, T* __restrict__ d, T c0, T c1, T c2,
+                                                const int* __restrict__ done) {
+    if (done && *done) return;
+    for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
+         row += (int64_t)gridDim.x * blockDim.x) {
+        const T ax = sell_row_dot_f<T, ROW_BATCH>(sp, sc, sv, row, [=](int32_t j) { return c0 * dinv[j] * b[j]; });
+        if (row < n) {
+            const T x0 = c0 * dinv[row] * b[row];
+            const T res = b[row] - ax;
+            const T dn = c1 * x0 + c2 * dinv[row] * res;
+            d[row] = dn;
+            xout[row] = x0 + dn;
         }
     }
 }
 
-// The same two steps on the compressed D^-1 A (FP32 cycles):
+// The same steps on the compressed D^-1 A (FP32 cycles):
 //   cheb:     d = c1 d + c2 (dinv b - (D^-1 A) x_in); x_out = x_in + d
+//   cheb_z:   the first two steps from zero (x0 = c0 dinv b formed per column)
 //   residual: r = b - diag * ((D^-1 A) x)
-template <bool FIRST>
+template <bool FIRST, bool DOT = false>
 __global__ void __launch_bounds__(256) k_cheb16(int64_t n, const int32_t* __restrict__ sp,
                                                const int16_t* __restrict__ so, const _Float16* __restrict__ sv,
                                                const float* __restrict__ dinv, const float* __restrict__ b,
                                                const float* __restrict__ xin, float* __restrict__ xout,
                                                float* __restrict__ d, float c1, float c2,
-                                               const int* __restrict__ done) {
+                                               const int* __restrict__ done, DotOut dot = DotOut{}) {
     if (done && *done) return;
+    double acc = 0.0;
     for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
          row += (int64_t)gridDim.x * blockDim.x) {
         const float ax = sell16_row_dot(sp, so, sv, row, n, xin);
         if (row < n) {
             const float dn = (FIRST ? 0.f : c1 * d[row]) + c2 * (dinv[row] * b[row] - ax);
             d[row] = dn;
-            xout[row] = xin[row] + dn;
+            const float xn = xin[row] + dn;
+            xout[row] = xn;
+            if (DOT) acc += dot_term(dot, row, double(xn));
+        }
+    }
+    if (DOT) store_partial(block_sum(acc), dot.partials);
+}
+
+__global__ void __launch_bounds__(256) k_cheb16_z(int64_t n, const int32_t* __restrict__ sp,
+                                                 const int16_t* __restrict__ so, const _Float16* __restrict__ sv,
+                                                 const float* __restrict__ dinv, const float* __restrict__ b,
+                                                 float* __restrict__ xout, float* __restrict__ d, float c0, float c1,
+                                                 float c2, const int* __restrict__ done) {
+    if (done && *done) return;
+    for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
+         row += (int64_t)gridDim.x * blockDim.x) {
+        const float ax = sell16_row_dot_f(sp, so, sv, row, n, [=](int64_t j) { return c0 * dinv[j] * b[j]; });
+        if (row < n) {
+            const float x0 = c0 * dinv[row] * b[row];
+            const float dn = c1 * x0 + c2 * (dinv[row] * b[row] - ax);
+            d[row] = dn;
+            xout[row] = x0 + dn;
         }
     }
 }
@@ -675,11 +744,15 @@ struct Cycle {
     }
 
     // Chebyshev sweep of `deg` steps on level L: from zero (pre) or from *cur (post).
-    int smooth(const fa_mg_level_t& L, const T* b, T*& cur, T*& nxt, bool from_zero) {
-        return smooth_deg(L, b, cur, nxt, from_zero, mg->degree);
+    int smooth(const fa_mg_level_t& L, const T* b, T*& cur, T*& nxt, bool from_zero, T* last_out = nullptr,
+               const DotOut* dot = nullptr) {
+        return smooth_deg(L, b, cur, nxt, from_zero, mg->degree, last_out, dot);
     }
 
-    int smooth_deg(const fa_mg_level_t& L, const T* b, T*& cur, T*& nxt, bool from_zero, int degree) {
+    // last_out: the last step writes there (and cur then points to it); dot:
+    // that step also stores the partials of r . x_out (reduction grid)
+    int smooth_deg(const fa_mg_level_t& L, const T* b, T*& cur, T*& nxt, bool from_zero, int degree,
+                   T* last_out = nullptr, const DotOut* dot = nullptr) {
         const double lo = mg->lower * L.lmax, hi = mg->upper * L.lmax;
         const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo), sigma = theta / delta;
         double rho = 1.0 / sigma;
@@ -687,6 +760,8 @@ struct Cycle {
         const T* sv = static_cast<const T*>(L.A.vals);
         T* d = static_cast<T*>(L.d);
         const unsigned g = grid(L.A.n);
+        const unsigned gd = red_grid(L.A.n);
+        const bool cmp = compressed(L);
         for (int k = 0; k < degree; ++k) {
             T c1, c2;
             double rho_new = rho;
@@ -699,33 +774,72 @@ struct Cycle {
                 c2 = static_cast<T>(2.0 * rho_new / delta);
             }
             if (k == 0 && from_zero) {
+                if (degree >= 2) {  // steps 0 and 1 in one pass
+                    const T c0 = c2;
+                    rho_new = 1.0 / (2.0 * sigma - rho);
+                    const T d1 = static_cast<T>(rho_new * rho), d2 = static_cast<T>(2.0 * rho_new / delta);
+                    T* out = (degree == 2 && last_out) ? last_out : nxt;
+                    if (cmp)
+                        hipLaunchKernelGGL(k_cheb16_z, dim3(g), dim3(256), 0, s, L.A.n, L.Ah.slice_ptr, L.Ah.offs,
+                                           static_cast<const _Float16*>(L.Ah.vals),
+                                           reinterpret_cast<const float*>(dinv), reinterpret_cast<const float*>(b),
+                                           reinterpret_cast<float*>(out), reinterpret_cast<float*>(d), (float)c0,
+                                           (float)d1, (float)d2, done);
+                    else
+                        hipLaunchKernelGGL(k_cheb_z<T>, dim3(g), dim3(256), 0, s, L.A.n, L.A.slice_ptr, L.A.cols, sv,
+                                           dinv, b, out, d, c0, d1, d2, done);
+                    if (out == nxt) std::swap(cur, nxt);
+                    else cur = out;
+                    rho = rho_new;
+                    ++k;
+                    continue;
+                }
                 hipLaunchKernelGGL((k_cheb<T, true, true>), dim3(g), dim3(256), 0, s, L.A.n, L.A.slice_ptr, L.A.cols,
                                    sv, dinv, b, cur, cur, d, c1, c2, done);
                 continue;  // result written in place into cur
             }
-            if (compressed(L)) {
+            const bool last = (k == degree - 1);
+            T* out = (last && last_out) ? last_out : nxt;
+            const bool with_dot = last && dot;
+            if (cmp) {
                 const _Float16* hv = static_cast<const _Float16*>(L.Ah.vals);
                 const float* fb = reinterpret_cast<const float*>(b);
                 const float* fdinv = reinterpret_cast<const float*>(dinv);
-                if (k == 0)
+                const float* fcur = reinterpret_cast<const float*>(cur);
+                float* fout = reinterpret_cast<float*>(out);
+                float* fd = reinterpret_cast<float*>(d);
+                if (with_dot) {
+                    if (k == 0)
+                        hipLaunchKernelGGL((k_cheb16<true, true>), dim3(gd), dim3(RED_BLOCK), 0, s, L.A.n,
+                                           L.Ah.slice_ptr, L.Ah.offs, hv, fdinv, fb, fcur, fout, fd, (float)c1,
+                                           (float)c2, done, *dot);
+                    else
+                        hipLaunchKernelGGL((k_cheb16<false, true>), dim3(gd), dim3(RED_BLOCK), 0, s, L.A.n,
+                                           L.Ah.slice_ptr, L.Ah.offs, hv, fdinv, fb, fcur, fout, fd, (float)c1,
+                                           (float)c2, done, *dot);
+                } else if (k == 0) {
                     hipLaunchKernelGGL(k_cheb16<true>, dim3(g), dim3(256), 0, s, L.A.n, L.Ah.slice_ptr, L.Ah.offs, hv,
-                                       fdinv, fb, reinterpret_cast<const float*>(cur), reinterpret_cast<float*>(nxt),
-                                       reinterpret_cast<float*>(d), (float)c1, (float)c2, done);
+                                       fdinv, fb, fcur, fout, fd, (float)c1, (float)c2, done, DotOut{});
+                } else {
+                    hipLaunchKernelGGL(k_cheb16<false>, dim3(g), dim3(256), 0, s, L.A.n, L.Ah.slice_ptr, L.Ah.offs,
+                                       hv, fdinv, fb, fcur, fout, fd, (float)c1, (float)c2, done, DotOut{});
+                }
+            } else if (with_dot) {
+                if (k == 0)
+                    hipLaunchKernelGGL((k_cheb<T, false, true, true>), dim3(gd), dim3(RED_BLOCK), 0, s, L.A.n,
+                                       L.A.slice_ptr, L.A.cols, sv, dinv, b, cur, out, d, c1, c2, done, *dot);
                 else
-                    hipLaunchKernelGGL(k_cheb16<false>, dim3(g), dim3(256), 0, s, L.A.n, L.Ah.slice_ptr, L.Ah.offs, hv,
-                                       fdinv, fb, reinterpret_cast<const float*>(cur), reinterpret_cast<float*>(nxt),
-                                       reinterpret_cast<float*>(d), (float)c1, (float)c2, done);
-                std::swap(cur, nxt);
-                rho = rho_new;
-                continue;
-            }
-            if (k == 0)
+                    hipLaunchKernelGGL((k_cheb<T, false, false, true>), dim3(gd), dim3(RED_BLOCK), 0, s, L.A.n,
+                                       L.A.slice_ptr, L.A.cols, sv, dinv, b, cur, out, d, c1, c2, done, *dot);
+            } else if (k == 0) {
                 hipLaunchKernelGGL((k_cheb<T, false, true>), dim3(g), dim3(256), 0, s, L.A.n, L.A.slice_ptr,
-                                   L.A.cols, sv, dinv, b, cur, nxt, d, c1, c2, done);
-            else
+                                   L.A.cols, sv, dinv, b, cur, out, d, c1, c2, done, DotOut{});
+            } else {
                 hipLaunchKernelGGL((k_cheb<T, false, false>), dim3(g), dim3(256), 0, s, L.A.n, L.A.slice_ptr,
-                                   L.A.cols, sv, dinv, b, cur, nxt, d, c1, c2, done);
-            std::swap(cur, nxt);
+                                   L.A.cols, sv, dinv, b, cur, out, d, c1, c2, done, DotOut{});
+            }
+            if (out == nxt) std::swap(cur, nxt);
+            else cur = out;
             rho = rho_new;
         }
         FA_LAUNCH("k_cheb");
@@ -761,11 +875,14 @@ struct Cycle {
         return true;
     }
 
-    // returns (through out) the level-l vector that holds the result
-    int run(int l, const T* b, T** out) {
+    // returns (through out) the level-l vector that holds the result; on the
+    // finest level a non-null `z` receives the result from the last smoothing
+    // step (and `dot` its partials), *fused tells whether that happened
+    int run(int l, const T* b, T** out, T* z = nullptr, const DotOut* dot = nullptr, bool* fused_out = nullptr) {
         const fa_mg_level_t& L = mg->levels[l];
         T* cur = static_cast<T*>(L.x);
         T* nxt = static_cast<T*>(L.x2);
+        if (fused_out) *fused_out = false;
         if (l > 0 && fusable(l)) return fused(l, out);
         if (l == mg->nlevels - 1) {
             int st = mg->coarse_chol ? coarse(L, b, cur) : smooth_deg(L, b, cur, nxt, true, mg->coarse_degree);
@@ -794,11 +911,24 @@ struct Cycle {
         hipLaunchKernelGGL((k_sell_apply<T, true>), dim3(grid(L.P.n)), dim3(256), 0, s, L.P.n, L.P.slice_ptr,
                            L.P.cols, static_cast<const T*>(L.P.vals), xc, cur, done);
         FA_LAUNCH("prolongate");
-        st = smooth(L, b, cur, nxt, false);
+        const bool direct = z && mg->degree >= 1 && z != cur && z != nxt;
+        st = smooth(L, b, cur, nxt, false, direct ? z : nullptr, direct ? dot : nullptr);
+        if (direct && fused_out) *fused_out = true;
         *out = cur;
         return st;
     }
 };
+
+// partials of r . z (one per workgroup of the reduction grid)
+template <class R, class Z>
+__global__ void __launch_bounds__(RED_BLOCK) k_dot_part(int64_t n, const R* __restrict__ r, const Z* __restrict__ z,
+                                                       double* __restrict__ partials, const int* __restrict__ done) {
+    if (done && *done) return;
+    double acc = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        acc += double(r[i]) * double(z[i]);
+    store_partial(block_sum(acc), partials);
+}
 
 template <class T>
 __global__ void k_copy(int64_t n, const T* __restrict__ x, T* __restrict__ y, const int* __restrict__ done) {
@@ -807,22 +937,37 @@ __global__ void k_copy(int64_t n, const T* __restrict__ x, T* __restrict__ y, co
         y[i] = x[i];
 }
 
+// z = V(r).  With `dot`, the partials of dot.r . z are stored too (by the
+// last smoothing step, or by a separate pass when the cycle has none).
 template <class T>
-static int vcycle_t(const fa_mg_t* mg, const void* r, void* z, const int* done, hipStream_t s) {
+static int vcycle_t(const fa_mg_t* mg, const void* r, void* z, const int* done, hipStream_t s,
+                    const DotOut* dot) {
     Cycle<T> c{mg, s, done};
     T* out = nullptr;
-    int st = c.run(0, static_cast<const T*>(r), &out);
+    bool direct = false;
+    int st = c.run(0, static_cast<const T*>(r), &out, static_cast<T*>(z), dot, &direct);
     if (st) return st;
+    const int64_t n = mg->levels[0].A.n;
     if (out != z)
-        hipLaunchKernelGGL(k_copy<T>, dim3(grid_for(mg->levels[0].A.n, 256, 1 << 16)), dim3(256), 0, s,
-                           mg->levels[0].A.n, out, static_cast<T*>(z), done);
+        hipLaunchKernelGGL(k_copy<T>, dim3(grid_for(n, 256, 1 << 16)), dim3(256), 0, s, n, out, static_cast<T*>(z),
+                           done);
     FA_LAUNCH("k_copy");
+    if (dot && !direct) {
+        if (dot->r_f64)
+            hipLaunchKernelGGL((k_dot_part<double, T>), dim3(red_grid(n)), dim3(RED_BLOCK), 0, s, n,
+                               static_cast<const double*>(dot->r), static_cast<const T*>(z), dot->partials, done);
+        else
+            hipLaunchKernelGGL((k_dot_part<float, T>), dim3(red_grid(n)), dim3(RED_BLOCK), 0, s, n,
+                               static_cast<const float*>(dot->r), static_cast<const T*>(z), dot->partials, done);
+        FA_LAUNCH("k_dot_part");
+    }
     return FA_OK;
 }
 
-static int vcycle_any(const fa_mg_t* mg, const void* r, void* z, const int* done, hipStream_t s) {
-    if (mg->dtype == FA_F64) return vcycle_t<double>(mg, r, z, done, s);
-    return vcycle_t<float>(mg, r, z, done, s);
+static int vcycle_any(const fa_mg_t* mg, const void* r, void* z, const int* done, hipStream_t s,
+                      const DotOut* dot = nullptr) {
+    if (mg->dtype == FA_F64) return vcycle_t<double>(mg, r, z, done, s, dot);
+    return vcycle_t<float>(mg, r, z, done, s, dot);
 }
 
 // ---------------------------------------------------------------------------
@@ -954,16 +1099,6 @@ __global__ void __launch_bounds__(RED_BLOCK) k_pcg_update(int64_t n, T* __restri
 }
 
 template <class T, class U>
-__global__ void __launch_bounds__(RED_BLOCK) k_pcg_rz(int64_t n, const T* __restrict__ r, const U* __restrict__ z,
-                                                     double* __restrict__ partials, const PcgState* st) {
-    if (st->done) return;
-    double acc = 0.0;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        acc += double(r[i]) * double(z[i]);
-    store_partial(block_sum(acc), partials);
-}
-
-template <class T, class U>
 __global__ void k_pcg_pupdate(int64_t n, T* __restrict__ p, const U* __restrict__ z, const PcgState* st) {
     if (st->done) return;
     const T beta = static_cast<T>(st->beta);
@@ -1015,6 +1150,7 @@ static int pcg_run(const fa_sell_t* A, const fa_mg_t* M, const T* b, T* x, doubl
         hipLaunchKernelGGL(k_pcg_finish<PCG_RZ0>, one, block, 0, s, partials, np, st);
         FA_LAUNCH("k_pcg_rz0");
     }
+    const DotOut rz_out{r, partials, std::is_same<T, double>::value ? 1 : 0};
     PcgState h{};
     int64_t it = 0;
     for (;;) {
@@ -1024,9 +1160,8 @@ static int pcg_run(const fa_sell_t* A, const fa_mg_t* M, const T* b, T* x, doubl
             hipLaunchKernelGGL(k_pcg_finish<PCG_ALPHA>, one, block, 0, s, partials, np, st);
             if (M) {
                 hipLaunchKernelGGL((k_pcg_update<T, U, true>), grid, block, 0, s, n, x, r, ru, p, Ap, partials, st);
-                int e = vcycle_any(M, ru, z, done, s);
+                int e = vcycle_any(M, ru, z, done, s, &rz_out);  // z = M r and the partials of r.z
                 if (e) return e;
-                hipLaunchKernelGGL((k_pcg_rz<T, U>), grid, block, 0, s, n, r, z, partials, st);
                 hipLaunchKernelGGL(k_pcg_finish<PCG_CHECK>, one, block, 0, s, partials, np, st);
                 hipLaunchKernelGGL((k_pcg_pupdate<T, U>), grid, block, 0, s, n, p, z, st);
             } else {

@@ -19,9 +19,11 @@ namespace fa {
 
 constexpr int ROW_BATCH = 16;
 
-template <class T, class X, int B = ROW_BATCH>
-__device__ inline T sell_row_dot(const int32_t* __restrict__ slice_ptr, const int32_t* __restrict__ cols,
-                                 const T* __restrict__ vals, int64_t row, const X* __restrict__ x) {
+// sum_k vals[k] * xf(cols[k]) over the row, xf a gather functor (the vector,
+// or a value computed from several vectors at that column)
+template <class T, int B, class XF>
+__device__ inline T sell_row_dot_f(const int32_t* __restrict__ slice_ptr, const int32_t* __restrict__ cols,
+                                   const T* __restrict__ vals, int64_t row, XF xf) {
     const int64_t s = row >> 6;
     const int i = static_cast<int>(row & 63);
     const int32_t p0 = slice_ptr[s];
@@ -40,7 +42,7 @@ __device__ inline T sell_row_dot(const int32_t* __restrict__ slice_ptr, const in
         }
         T xx[B];
 #pragma unroll
-        for (int u = 0; u < B; ++u) xx[u] = static_cast<T>(x[cc[u]]);
+        for (int u = 0; u < B; ++u) xx[u] = xf(cc[u]);
 #pragma unroll
         for (int u = 0; u < B; ++u)
             if (k0 + u < w) acc += vv[u] * xx[u];
@@ -48,11 +50,17 @@ __device__ inline T sell_row_dot(const int32_t* __restrict__ slice_ptr, const in
     return acc;
 }
 
-// row dot of the compressed format: sum_k vals[k] * x[row + offs[k]], FP32
+template <class T, class X, int B = ROW_BATCH>
+__device__ inline T sell_row_dot(const int32_t* __restrict__ slice_ptr, const int32_t* __restrict__ cols,
+                                 const T* __restrict__ vals, int64_t row, const X* __restrict__ x) {
+    return sell_row_dot_f<T, B>(slice_ptr, cols, vals, row, [=](int32_t j) { return static_cast<T>(x[j]); });
+}
+
+// row dot of the compressed format: sum_k vals[k] * xf(row + offs[k]), FP32
 // accumulation; rows of the last slice past n are not evaluated
-__device__ inline float sell16_row_dot(const int32_t* __restrict__ slice_ptr, const int16_t* __restrict__ offs,
-                                       const _Float16* __restrict__ vals, int64_t row, int64_t n,
-                                       const float* __restrict__ x) {
+template <class XF>
+__device__ inline float sell16_row_dot_f(const int32_t* __restrict__ slice_ptr, const int16_t* __restrict__ offs,
+                                         const _Float16* __restrict__ vals, int64_t row, int64_t n, XF xf) {
     if (row >= n) return 0.f;
     const int64_t s = row >> 6;
     const int i = static_cast<int>(row & 63);
@@ -72,12 +80,18 @@ __device__ inline float sell16_row_dot(const int32_t* __restrict__ slice_ptr, co
         }
         float xx[ROW_BATCH];
 #pragma unroll
-        for (int u = 0; u < ROW_BATCH; ++u) xx[u] = x[row + oo[u]];
+        for (int u = 0; u < ROW_BATCH; ++u) xx[u] = xf(row + oo[u]);
 #pragma unroll
         for (int u = 0; u < ROW_BATCH; ++u)
             if (k0 + u < w) acc += vv[u] * xx[u];
     }
     return acc;
+}
+
+__device__ inline float sell16_row_dot(const int32_t* __restrict__ slice_ptr, const int16_t* __restrict__ offs,
+                                       const _Float16* __restrict__ vals, int64_t row, int64_t n,
+                                       const float* __restrict__ x) {
+    return sell16_row_dot_f(slice_ptr, offs, vals, row, n, [=](int64_t j) { return x[j]; });
 }
 
 }  // namespace fa
