@@ -256,9 +256,33 @@ class GeometricMG:
                                                                      device=self.device), (c.n, c.n))
             c.max_row = int(mrow.value)
             c.As = SellMatrix(c.A.crow, c.A.col, c.A.shape)
+            f.terms = self._rap_terms(f, c, max(cand, 1))
         last = self.levels[-1]
         last.dense = (torch.zeros((last.n, last.n), dtype=torch.float64, device=self.device) if self.coarse_dense
                       else None)
+
+    def _rap_terms(self, f, c, cand):
+        """Term list of the numeric product (fa_rap_terms_*), or None when the
+        transfers are empty or take more than 3 distinct values, or A is too
+        large; the search kernel (fa_rap_values) then forms the product."""
+        L = nat.lib()
+        vals = torch.unique(torch.cat([f.R.values, f.P.values])).cpu().tolist()
+        if not vals or len(vals) > 3 or f.A.nnz >= (1 << 28) or c.A.nnz == 0:
+            return None
+        wset = (ctypes.c_double * len(vals))(*vals)
+        ne = c.A.nnz
+        tslice = torch.empty((ne + 63) // 64 + 1, dtype=torch.int32, device=self.device)
+        ws = nat.workspace(L.fa_rap_terms_work_bytes(ne), self.device)
+        nterms = ctypes.c_int64(0)
+        args = (c.n, nat.ptr(f.R.crow), nat.ptr(f.R.col), nat.ptr(f.R.values), nat.ptr(f.A.crow), nat.ptr(f.A.col),
+                nat.ptr(f.P.crow), nat.ptr(f.P.col), nat.ptr(f.P.values), cand, nat.ptr(c.A.crow), ne, wset,
+                len(vals))
+        nat.check(L.fa_rap_terms_count(*args, nat.ptr(tslice), ctypes.byref(nterms), nat.ptr(ws), ws.numel(),
+                                       nat.stream()), "fa_rap_terms_count")
+        terms = torch.empty(max(int(nterms.value), 1), dtype=torch.int32, device=self.device)
+        nat.check(L.fa_rap_terms_fill(*args, nat.ptr(tslice), int(nterms.value), nat.ptr(terms), nat.ptr(ws),
+                                      ws.numel(), nat.stream()), "fa_rap_terms_fill")
+        return (tslice, terms, wset, len(vals))
 
     def setup(self, A):
         """Numeric setup for the (Dirichlet-applied, FP64) fine matrix A."""
@@ -272,6 +296,12 @@ class GeometricMG:
         self.levels[0].A = A
         for l in range(len(self.levels) - 1):
             f, c = self.levels[l], self.levels[l + 1]
+            if getattr(f, 'terms', None) is not None:
+                tslice, terms, wset, nw = f.terms
+                nat.check(L.fa_rap_values_terms(c.A.nnz, nat.ptr(tslice), nat.ptr(terms), wset, nw,
+                                                nat.ptr(f.A.values), nat.ptr(c.A.values), nat.stream()),
+                          "fa_rap_values_terms")
+                continue
             nat.check(L.fa_rap_values(c.n, nat.ptr(f.R.crow), nat.ptr(f.R.col), nat.ptr(f.R.values),
                                       nat.ptr(f.A.crow), nat.ptr(f.A.col), nat.ptr(f.A.values), nat.ptr(f.P.crow),
                                       nat.ptr(f.P.col), nat.ptr(f.P.values), nat.ptr(c.A.crow), nat.ptr(c.A.col),

@@ -290,6 +290,32 @@ int fa_rap_values(int64_t nrows, const int32_t* r_ptr, const int32_t* r_idx, con
                   const int32_t* c_ptr, const int32_t* c_idx, int32_t max_row, double* c_val,
                   fa_stream_t stream);
 
+/* Numeric R A P from a precomputed term list, for transfers with at most 3
+ * distinct values (nested P1 grids: 1, 1/2; P2 -> P1: 1, 1/2; P3 -> P1: 1,
+ * 2/3, 1/3); same product as fa_rap_values, which it replaces per assembly
+ * (gamg_solver.py:85-101 forms it with SciPy SpGEMM).  Each coarse entry is
+ * sum_k w_k a_val[t_k] over its terms in the order R entry -> A entry -> P
+ * entry, w_k = r p a product of two values of `values` (host array).  Terms
+ * of 64 consecutive coarse entries form a slice (SELL-64 layout): term k of
+ * entry e at tslice[e/64] + 64 k + e%64; one 32-bit word each (A entry index
+ * in bits 0-27, weight code in bits 28-31), so nnz(A) < 2^28.
+ * fa_rap_terms_count: slice offsets tslice (ceil(nnz_c/64) + 1 entries) and
+ * the term count; syncs.  fa_rap_terms_fill: the words; syncs (error flags).
+ * c_ptr: the coarse pattern of fa_rap_count / fa_rap_fill. */
+size_t fa_rap_terms_work_bytes(int64_t nnz_c);
+int fa_rap_terms_count(int64_t nrows, const int32_t* r_ptr, const int32_t* r_idx, const double* r_val,
+                       const int32_t* a_ptr, const int32_t* a_idx, const int32_t* p_ptr, const int32_t* p_idx,
+                       const double* p_val, int32_t max_cand, const int32_t* c_ptr, int64_t nnz_c,
+                       const double* values, int nvalues, int32_t* tslice, int64_t* nterms_host, void* work,
+                       size_t work_bytes, fa_stream_t stream);
+int fa_rap_terms_fill(int64_t nrows, const int32_t* r_ptr, const int32_t* r_idx, const double* r_val,
+                      const int32_t* a_ptr, const int32_t* a_idx, const int32_t* p_ptr, const int32_t* p_idx,
+                      const double* p_val, int32_t max_cand, const int32_t* c_ptr, int64_t nnz_c,
+                      const double* values, int nvalues, const int32_t* tslice, int64_t nterms, uint32_t* terms,
+                      void* work, size_t work_bytes, fa_stream_t stream);
+int fa_rap_values_terms(int64_t nnz_c, const int32_t* tslice, const uint32_t* terms, const double* values,
+                        int nvalues, const double* a_val, double* c_val, fa_stream_t stream);
+
 /* ------------------------------------------------------- multigrid cycle --
  * Geometric multigrid V-cycle used as a CG preconditioner: Chebyshev
  * polynomial smoothing of D^-1 A (pre and post, same polynomial, so the cycle

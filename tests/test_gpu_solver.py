@@ -79,6 +79,123 @@ def test_rap_matches_scipy_triple_product():
     assert diff.max() < 1e-12 * abs(ref).max()
 
 
+def _rap_by_search(f, c):
+    """The same product by the pattern-search kernel (fa_rap_values)."""
+    import ctypes
+    from fealpy_amd import _native as nat
+    out = torch.empty_like(c.A.values)
+    nat.check(nat.lib().fa_rap_values(c.n, nat.ptr(f.R.crow), nat.ptr(f.R.col), nat.ptr(f.R.values),
+                                      nat.ptr(f.A.crow), nat.ptr(f.A.col), nat.ptr(f.A.values), nat.ptr(f.P.crow),
+                                      nat.ptr(f.P.col), nat.ptr(f.P.values), nat.ptr(c.A.crow), nat.ptr(c.A.col),
+                                      c.max_row, nat.ptr(out), nat.stream()), "fa_rap_values")
+    return out
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_rap_term_list(p):
+    """Term-list numeric product equals SciPy's triple product and the search
+    kernel on every level (transfers 1, 1/2: nested P1 grids, P2 -> P1)."""
+    import scipy.sparse as sp
+    n = {1: 8, 2: 6}[p]
+    mesh, space, bc, A, b = _system(n, p=p)
+    k = _lognormal(mesh, seed=11)
+    _, _, _, A, _ = _system(n, k=k, p=p)
+    mg = GeometricMG(space, bc.is_boundary_dof, dtype=torch.float64, coarse_max=1)
+    mg.setup(A)
+    assert mg.levels[0].terms is not None
+    for l in range(len(mg.levels) - 1):
+        f, c = mg.levels[l], mg.levels[l + 1]
+        ref = (f.R.to_scipy() @ f.A.to_scipy() @ f.P.to_scipy()).tocsr()
+        ref = ref + sp.diags((np.diff(ref.indptr) == 0).astype(float))
+        got = c.A.to_scipy()
+        scale = abs(ref).max()
+        assert abs(got - ref).max() < 1e-12 * scale, f"level {l}"
+        other = _rap_by_search(f, c)
+        assert float((other - c.A.values).abs().max()) < 1e-13 * scale, f"level {l}"
+
+
+def _csr_dev(M):
+    M = M.tocsr()
+    M.sort_indices()
+    return (torch.as_tensor(M.indptr, dtype=torch.int32, device=DEV),
+            torch.as_tensor(M.indices, dtype=torch.int32, device=DEV),
+            torch.as_tensor(M.data, dtype=torch.float64, device=DEV))
+
+
+@pytest.mark.parametrize("values", [(1.0,), (1.0, 0.5), (1.0, 2 / 3, 1 / 3)])
+def test_rap_terms_random_transfers(values):
+    """C-ABI term list on random transfers P (1-3 entries a row, values from a
+    set of 1-3) and a random symmetric A, R = P^T, against SciPy; coarse rows
+    with no entries get the unit diagonal; 4 distinct values are refused."""
+    import ctypes
+    import scipy.sparse as sp
+    from fealpy_amd import _native as nat
+    rng = np.random.default_rng(len(values))
+    nf, nc = 700, 150
+    rows, cols, vals = [], [], []
+    for i in range(nf):
+        for J in rng.choice(nc - 10, size=rng.integers(1, 4), replace=False):  # last 10 coarse rows empty
+            rows.append(i), cols.append(J), vals.append(values[rng.integers(0, len(values))])
+    P = sp.csr_matrix((vals, (rows, cols)), shape=(nf, nc))
+    B = sp.random(nf, nf, density=8 / nf, random_state=3, format="csr")
+    A = (B + B.T + sp.eye(nf)).tocsr()
+    R = P.T.tocsr()
+    ref = (R @ A @ P).tocsr()
+    ref = ref + sp.diags((np.diff(ref.indptr) == 0).astype(float))
+    ref.sort_indices()
+    L = nat.lib()
+    r, a, p = _csr_dev(R), _csr_dev(A), _csr_dev(P)
+    cand = int(np.diff(R.indptr).max() * np.diff(A.indptr).max() * np.diff(P.indptr).max())
+    cptr = torch.empty(nc + 1, dtype=torch.int32, device=DEV)
+    ws = nat.workspace(max(L.fa_rap_work_bytes(nc), L.fa_rap_terms_work_bytes(ref.nnz)), DEV)
+    nnz, mrow = ctypes.c_int64(0), ctypes.c_int32(0)
+    nat.check(L.fa_rap_count(nc, nc, nat.ptr(r[0]), nat.ptr(r[1]), nat.ptr(a[0]), nat.ptr(a[1]), nat.ptr(p[0]),
+                             nat.ptr(p[1]), cand, nat.ptr(cptr), ctypes.byref(nnz), ctypes.byref(mrow), nat.ptr(ws),
+                             ws.numel(), nat.stream()), "fa_rap_count")
+    assert nnz.value == ref.nnz
+    cidx = torch.empty(ref.nnz, dtype=torch.int32, device=DEV)
+    nat.check(L.fa_rap_fill(nc, nat.ptr(r[0]), nat.ptr(r[1]), nat.ptr(a[0]), nat.ptr(a[1]), nat.ptr(p[0]),
+                            nat.ptr(p[1]), cand, nat.ptr(cptr), nat.ptr(cidx), nat.stream()), "fa_rap_fill")
+    wset = (ctypes.c_double * len(values))(*values)
+    args = (nc, nat.ptr(r[0]), nat.ptr(r[1]), nat.ptr(r[2]), nat.ptr(a[0]), nat.ptr(a[1]), nat.ptr(p[0]),
+            nat.ptr(p[1]), nat.ptr(p[2]), cand, nat.ptr(cptr), ref.nnz, wset, len(values))
+    tslice = torch.empty((ref.nnz + 63) // 64 + 1, dtype=torch.int32, device=DEV)
+    nt = ctypes.c_int64(0)
+    nat.check(L.fa_rap_terms_count(*args, nat.ptr(tslice), ctypes.byref(nt), nat.ptr(ws), ws.numel(),
+                                   nat.stream()), "fa_rap_terms_count")
+    terms = torch.empty(nt.value, dtype=torch.int32, device=DEV)
+    nat.check(L.fa_rap_terms_fill(*args, nat.ptr(tslice), nt.value, nat.ptr(terms), nat.ptr(ws), ws.numel(),
+                                  nat.stream()), "fa_rap_terms_fill")
+    cval = torch.empty(ref.nnz, dtype=torch.float64, device=DEV)
+    nat.check(L.fa_rap_values_terms(ref.nnz, nat.ptr(tslice), nat.ptr(terms), wset, len(values), nat.ptr(a[2]),
+                                    nat.ptr(cval), nat.stream()), "fa_rap_values_terms")
+    assert np.array_equal(cptr.cpu().numpy(), ref.indptr) and np.array_equal(cidx.cpu().numpy(), ref.indices)
+    np.testing.assert_allclose(cval.cpu().numpy(), ref.data, rtol=0, atol=1e-13 * abs(ref.data).max())
+    four = (ctypes.c_double * 4)(1.0, 0.75, 0.5, 0.25)
+    st = L.fa_rap_values_terms(ref.nnz, nat.ptr(tslice), nat.ptr(terms), four, 4, nat.ptr(a[2]), nat.ptr(cval),
+                               nat.stream())
+    assert st == nat.FA_ECAPACITY
+
+
+def test_rap_term_list_elasticity():
+    """Vector P2 -> P1 transfer (per component) through the term list."""
+    import scipy.sparse as sp
+    n = 4
+    mesh = fa.TetrahedronMesh.from_box([0, 1, 0, 1, 0, 1], nx=n, ny=n, nz=n, device=DEV)
+    ts = fa.TensorFunctionSpace(fa.LagrangeFESpace(mesh, 2), (-1, 3))
+    mat = fa.LinearElasticMaterial("E", lame_lambda=1.0, shear_modulus=0.5)
+    A = fa.BilinearForm(ts).add_integrator(fa.LinearElasticityIntegrator(mat, q=3)).assembly()
+    bc = fa.DirichletBC(ts, gd=0.0)
+    A, _ = bc.apply(A, torch.zeros(A.shape[0], dtype=torch.float64, device=DEV))
+    mg = GeometricMG(ts, bc.is_boundary_dof, dtype=torch.float64, coarse_max=1)
+    mg.setup(A)
+    f, c = mg.levels[0], mg.levels[1]
+    assert f.terms is not None
+    ref = (f.R.to_scipy() @ f.A.to_scipy() @ f.P.to_scipy()).tocsr()
+    ref = ref + sp.diags((np.diff(ref.indptr) == 0).astype(float))
+    assert abs(c.A.to_scipy() - ref).max() < 1e-12 * abs(ref).max()
+
+
 def test_sell_padding_columns_in_range():
     """Padding entries of rectangular transfers (P: fine x coarse) must point at
     a valid column of the operand (regression: they used the row index)."""
