@@ -85,6 +85,7 @@ _SIGNATURES = {
     "fa_dense_spd_inverse": (ctypes.c_int, [_i64, _p, _p]),
     "fa_mg_lmax": (ctypes.c_int, [_p, ctypes.c_int, ctypes.c_int, _pf64, _p, _sz, _p]),
     "fa_mg_vcycle": (ctypes.c_int, [_p, _p, _p, _p]),
+    "fa_smoother_setup": (ctypes.c_int, [ctypes.c_int, _p, _p, _p, _p, _p, _p, _p, _sz, _p]),
     "fa_jacobi_setup": (ctypes.c_int, [ctypes.c_int, _i64, _i64, _p, _p, _p, _p, _pf64, _p, _sz, _p]),
     "fa_pcg_work_bytes": (_sz, [ctypes.c_int, _i64]),
     "fa_pcg": (ctypes.c_int, [ctypes.c_int, _p, _p, _p, _p, _f64, _f64, _i64, _i32, _pi64, _pf64, _p, _sz, _p]),

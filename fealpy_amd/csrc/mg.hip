@@ -595,6 +595,62 @@ __global__ void __launch_bounds__(RED_BLOCK) k_max_finish(const double* __restri
     }
 }
 
+// Smoother data of a level in one pass over its FP64 SELL copy (one row per
+// thread, coalesced across the slice): dinv = 1/a_ii (1 where the diagonal
+// is absent or zero; the diagonal is the row's first entry in column `row`,
+// real entries precede the padding), diag = 1/dinv for the compressed
+// residual, the Gershgorin bound max_i sum_j |a_ij| / |a_ii| (one partial
+// maximum per workgroup), and the cycle's operator values: FP16 D^-1 A on
+// the compressed slices (h16) and/or the FP32 copy of the SELL values (v32).
+template <class T>
+__global__ void __launch_bounds__(RED_BLOCK) k_smoother_setup(int64_t n, const int32_t* __restrict__ sp,
+                                                             const int32_t* __restrict__ cols,
+                                                             const double* __restrict__ v64, T* __restrict__ dinv,
+                                                             T* __restrict__ diag, _Float16* __restrict__ h16,
+                                                             float* __restrict__ v32, double* __restrict__ partials) {
+    __shared__ double ws[RED_BLOCK / WAVE];
+    double m = 0.0;
+    const int64_t npad = (n + 63) & ~63LL;
+    for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < npad;
+         row += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t sl = row >> 6;
+        const int32_t p0 = sp[sl] + static_cast<int32_t>(row & 63);
+        const int w = (sp[sl + 1] - sp[sl]) >> 6;
+        double d = 0.0, sum = 0.0;
+        bool found = false;
+        for (int k = 0; k < w; ++k) {
+            const double v = v64[p0 + 64 * k];
+            sum += fabs(v);
+            if (!found && cols[p0 + 64 * k] == row) {
+                d = v;
+                found = true;
+            }
+        }
+        const double inv = d != 0.0 ? 1.0 / d : 1.0;
+        if (row < n) {
+            const T di = static_cast<T>(inv);
+            dinv[row] = di;
+            if (diag) diag[row] = T(1) / di;
+            if (d != 0.0) m = fmax(m, sum / fabs(d));
+        }
+        if (h16 || v32) {
+            for (int k = 0; k < w; ++k) {
+                const double v = v64[p0 + 64 * k];
+                if (h16) h16[p0 + 64 * k] = static_cast<_Float16>(row < n ? v * inv : 0.0);
+                if (v32) v32[p0 + 64 * k] = static_cast<float>(v);
+            }
+        }
+    }
+    for (int o = WAVE / 2; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if ((threadIdx.x & (WAVE - 1)) == 0) ws[threadIdx.x / WAVE] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = 0.0;
+        for (int w = 0; w < RED_BLOCK / WAVE; ++w) b = fmax(b, ws[w]);
+        partials[blockIdx.x] = b;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Fused coarse cycle: the levels with at most FUSE_ROWS rows run as one
 // workgroup, phases separated by block barriers, instead of ~7 launches per
@@ -1372,6 +1428,38 @@ extern "C" int fa_jacobi_setup(int dtype_out, int64_t n, int64_t nnz, const int3
     FA_HIP(hipMemcpyAsync(&h, out, sizeof(double), hipMemcpyDeviceToHost, s));
     FA_HIP(hipStreamSynchronize(s));
     if (bound_host) *bound_host = h;
+    return FA_OK;
+}
+
+extern "C" int fa_smoother_setup(int dtype, const fa_sell_t* A, void* dinv, void* diag, void* vals_f32,
+                                 const fa_sell16_t* Ah, double* bound_dev, void* work, size_t work_bytes,
+                                 fa_stream_t stream) {
+    FA_REQUIRE(A && A->n > 0 && A->slice_ptr && A->cols && A->vals && dinv && bound_dev,
+               "fa_smoother_setup: bad arguments");
+    FA_REQUIRE(dtype == FA_F64 || dtype == FA_F32, "fa_smoother_setup: dtype must be FP64 or FP32");
+    FA_REQUIRE(!Ah || !Ah->vals || (Ah->n == A->n && Ah->slice_ptr == A->slice_ptr),
+               "fa_smoother_setup: the compressed operator must share the slices of A");
+    hipStream_t s = S(stream);
+    Arena ar(work, work_bytes);
+    double* partials = ar.take<double>(RED_GRID);
+    if (!partials) {
+        set_error("fa_smoother_setup: workspace too small");
+        return FA_ENOMEM;
+    }
+    const int64_t npad = (A->n + 63) & ~63LL;
+    const unsigned grid = red_grid(npad);
+    _Float16* h16 = (Ah && Ah->vals) ? static_cast<_Float16*>(const_cast<void*>(Ah->vals)) : nullptr;
+    const double* v64 = static_cast<const double*>(A->vals);
+    if (dtype == FA_F64)
+        hipLaunchKernelGGL(k_smoother_setup<double>, dim3(grid), dim3(RED_BLOCK), 0, s, A->n, A->slice_ptr, A->cols,
+                           v64, static_cast<double*>(dinv), static_cast<double*>(diag), h16,
+                           static_cast<float*>(vals_f32), partials);
+    else
+        hipLaunchKernelGGL(k_smoother_setup<float>, dim3(grid), dim3(RED_BLOCK), 0, s, A->n, A->slice_ptr, A->cols,
+                           v64, static_cast<float*>(dinv), static_cast<float*>(diag), h16,
+                           static_cast<float*>(vals_f32), partials);
+    hipLaunchKernelGGL(k_max_finish, dim3(1), dim3(RED_BLOCK), 0, s, partials, (int)grid, bound_dev);
+    FA_LAUNCH("k_smoother_setup");
     return FA_OK;
 }
 

@@ -238,6 +238,8 @@ class GeometricMG:
         L = nat.lib()
         self.levels[0].A = A
         self.levels[0].As = sell_of(A)
+        for lv in self.levels:
+            lv.compressed = None        # decided by the next setup (band check of the new pattern)
         for l in range(len(self.levels) - 1):
             f, c = self.levels[l], self.levels[l + 1]
             cand = _max_row(f.R) * _max_row(f.A) * max(_max_row(f.P), 1)
@@ -308,16 +310,27 @@ class GeometricMG:
                                       c.max_row, nat.ptr(c.A.values), nat.stream()), "fa_rap_values")
         code = nat.dtype_code(self.dtype)
         ws = nat.workspace(L.fa_reduce_work_bytes(), self.device)
-        for lv in self.levels:
-            lv.As.set_values(lv.A.values, self.dtype)
-            lam = ctypes.c_double(0.0)
-            nat.check(L.fa_jacobi_setup(code, lv.n, lv.A.nnz, nat.ptr(lv.A.crow), nat.ptr(lv.A.col),
-                                        nat.ptr(lv.A.values), nat.ptr(lv.dinv), ctypes.byref(lam), nat.ptr(ws),
-                                        ws.numel(), nat.stream()), "fa_jacobi_setup")
-            lv.lmax = float(lam.value)
-            lv.compressed = self.compress and lv.n > 1024 and lv.As.set_compressed(lv.A.values, ws)
-            if lv.compressed:
-                lv.diag = torch.reciprocal(lv.dinv)
+        bounds = torch.empty(len(self.levels), dtype=torch.float64, device=self.device)
+        f32 = self.dtype == torch.float32
+        for i, lv in enumerate(self.levels):
+            lv.As.set_values(lv.A.values, torch.float64)          # FP64 SELL (the outer operator on level 0)
+            if getattr(lv, 'compressed', None) is None:            # first setup of this pattern: band check
+                lv.compressed = self.compress and lv.n > 1024 and lv.As.set_compressed(lv.A.values, ws)
+                lv.diag = torch.empty_like(lv.dinv) if lv.compressed else None
+            v32 = None
+            if f32 and not lv.compressed:
+                v32 = lv.As.value_buffer(torch.float32, self.device)
+                lv.As.forget_values(torch.float32)
+            d64 = lv.As.desc(torch.float64)
+            h16 = lv.As.desc16() if lv.compressed else None
+            nat.check(L.fa_smoother_setup(code, ctypes.byref(d64), nat.ptr(lv.dinv),
+                                          nat.ptr(lv.diag) if lv.compressed else None,
+                                          nat.ptr(v32) if v32 is not None else None,
+                                          ctypes.byref(h16) if h16 is not None else None,
+                                          bounds[i:].data_ptr(), nat.ptr(ws), ws.numel(), nat.stream()),
+                      "fa_smoother_setup")
+        for lv, lam in zip(self.levels, bounds.tolist()):           # one copy for every level's bound
+            lv.lmax = float(lam)
         last = self.levels[-1]
         if self.coarse_dense:
             nat.check(L.fa_csr_to_dense(last.n, nat.ptr(last.A.crow), nat.ptr(last.A.col), nat.ptr(last.A.values),
@@ -330,6 +343,8 @@ class GeometricMG:
     def estimate_lmax(self, level, iters=None):
         """Power-iteration estimate of lambda_max(D^-1 A) on a level (diagnostics)."""
         L = nat.lib()
+        lv = self.levels[level]
+        lv.As.set_values(lv.A.values, self.dtype)     # compressed levels keep no current copy in the cycle dtype
         ws = nat.workspace(L.fa_reduce_work_bytes(), self.device)
         lam = ctypes.c_double(0.0)
         nat.check(L.fa_mg_lmax(ctypes.byref(self._desc), level, iters or self.power_iters, ctypes.byref(lam),
@@ -342,6 +357,7 @@ class GeometricMG:
         empty = FaSell(0, 0, None, None, None)
         for i, lv in enumerate(self.levels):
             e = arr[i]
+            lv.As.value_buffer(self.dtype, self.device)
             e.A = lv.As.desc(self.dtype)
             e.dinv = lv.dinv.data_ptr()
             e.R = lv.Rs.desc(self.dtype) if i < n - 1 else empty

@@ -380,6 +380,16 @@ int fa_jacobi_setup(int dtype_out, int64_t n, int64_t nnz, const int32_t* indptr
                     fa_stream_t stream);
 int fa_mg_lmax(const fa_mg_t* mg, int level, int iters, double* lmax_host, void* work, size_t work_bytes,
                fa_stream_t stream);
+/* Smoother data of a level in one pass over its FP64 SELL copy A, no sync:
+ * dinv (dtype) as fa_jacobi_setup, diag = 1/dinv (or NULL), the FP16
+ * D^-1 A values of Ah (offsets already filled by fa_sell16_fill on the same
+ * slices; Ah NULL or Ah->vals NULL: none), the FP32 copy of A's values
+ * (vals_f32, or NULL), and the Gershgorin bound written to bound_dev[0]
+ * (device memory; read it back with the other levels' bounds in one copy).
+ * Replaces the per-level diagonal extraction of GAMGSolver.setup
+ * (fealpy/solver/gamg_solver.py:85-120).  Work: fa_reduce_work_bytes(). */
+int fa_smoother_setup(int dtype, const fa_sell_t* A, void* dinv, void* diag, void* vals_f32, const fa_sell16_t* Ah,
+                      double* bound_dev, void* work, size_t work_bytes, fa_stream_t stream);
 /* z = V(r) on the finest level (r, z: dtype of the cycle). */
 int fa_mg_vcycle(const fa_mg_t* mg, const void* r, void* z, fa_stream_t stream);
 

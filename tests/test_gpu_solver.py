@@ -270,6 +270,46 @@ def test_compressed_level_matches_full_format():
     assert abs(a - bb) < 1e-3 * max(abs(a), abs(bb))
 
 
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_smoother_setup_matches_csr_setup(dtype):
+    """fa_smoother_setup (one pass over the FP64 SELL) gives the dinv of
+    fa_jacobi_setup bit for bit, its Gershgorin bound to rounding, 1/dinv as
+    diag, the FP16 D^-1 A of fa_sell16_fill and the FP32 cast of the values."""
+    import ctypes
+    from fealpy_amd import _native as nat
+    from fealpy_amd.sparse.sell import FaSell16
+    mesh, space, bc, A, b = _system(10)
+    k = _lognormal(mesh, seed=4)
+    _, _, _, A, b = _system(10, k=k)
+    L = nat.lib()
+    n = A.shape[0]
+    code = nat.dtype_code(dtype)
+    ws = nat.workspace(L.fa_reduce_work_bytes(), DEV)
+    dinv_ref = torch.empty(n, dtype=dtype, device=DEV)
+    lam = ctypes.c_double(0.0)
+    nat.check(L.fa_jacobi_setup(code, n, A.nnz, nat.ptr(A.crow), nat.ptr(A.col), nat.ptr(A.values),
+                                nat.ptr(dinv_ref), ctypes.byref(lam), nat.ptr(ws), ws.numel(), nat.stream()), "jacobi")
+    S = SellMatrix(A.crow, A.col, A.shape)
+    S.set_values(A.values, torch.float64)
+    wsc = torch.empty(64, dtype=torch.uint8, device=DEV)
+    assert S.set_compressed(A.values, wsc)
+    h_ref = S.h_vals.clone()
+    S.h_vals.zero_()
+    v32 = torch.full((S.entries,), 7.0, dtype=torch.float32, device=DEV)
+    dinv = torch.empty(n, dtype=dtype, device=DEV)
+    diag = torch.empty(n, dtype=dtype, device=DEV)
+    bound = torch.zeros(1, dtype=torch.float64, device=DEV)
+    d64, h16 = S.desc(torch.float64), S.desc16()
+    nat.check(L.fa_smoother_setup(code, ctypes.byref(d64), nat.ptr(dinv), nat.ptr(diag), nat.ptr(v32),
+                                  ctypes.byref(h16), nat.ptr(bound), nat.ptr(ws), ws.numel(), nat.stream()),
+              "fa_smoother_setup")
+    assert torch.equal(dinv, dinv_ref)
+    assert torch.equal(diag, torch.reciprocal(dinv))
+    assert abs(float(bound[0]) - lam.value) <= 1e-14 * lam.value
+    assert torch.equal(S.h_vals, h_ref)
+    assert torch.equal(v32, S.vals[torch.float64].float())
+
+
 def test_mg_pcg_2d():
     mesh, space, bc, A, b = _system(64, dim=2)
     x_ref = fa.cg(A, b, atol=1e-30, rtol=1e-13, maxit=50000)
