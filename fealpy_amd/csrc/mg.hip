@@ -659,6 +659,7 @@ __global__ void __launch_bounds__(RED_BLOCK) k_smoother_setup(int64_t n, const i
 constexpr int FUSE_MAX_LEVELS = 16;
 constexpr int64_t FUSE_ROWS = 1024;  // one workgroup is latency-bound above ~1 row per thread
 constexpr int FUSE_BLOCK = 1024;
+constexpr size_t FUSE_LDS_MAX = 60 * 1024;  // level data kept in LDS up to this size
 
 struct FusedLevel {
     fa_sell_t A, R, P;
@@ -731,14 +732,71 @@ __device__ inline void blk_apply(const fa_sell_t& M, const T* x, T* y) {
     __syncthreads();
 }
 
+// LDS bytes of the level data the LDS variant keeps on chip: slice offsets
+// of A, R, P, dinv and the five level vectors
 template <class T>
-__global__ void __launch_bounds__(FUSE_BLOCK) k_fused_cycle(FusedCycle fc, const int* __restrict__ done) {
+__host__ __device__ inline size_t fused_level_lds(const FusedLevel& L, bool last) {
+    const int64_t n = L.A.n;
+    size_t b = align_up(((n + 63) / 64 + 1) * sizeof(int32_t), 16) + 6 * align_up(n * sizeof(T), 16);
+    if (!last)
+        b += align_up(((L.R.n + 63) / 64 + 1) * sizeof(int32_t), 16) +
+             align_up(((L.P.n + 63) / 64 + 1) * sizeof(int32_t), 16);
+    return b;
+}
+
+template <class T>
+__device__ inline const int32_t* lds_copy_i32(char*& p, const int32_t* src, int64_t m) {
+    int32_t* dst = reinterpret_cast<int32_t*>(p);
+    for (int64_t i = threadIdx.x; i < m; i += blockDim.x) dst[i] = src[i];
+    p += align_up(m * sizeof(int32_t), 16);
+    return dst;
+}
+
+template <class T>
+__device__ inline T* lds_vec(char*& p, int64_t n, const void* src) {
+    T* dst = reinterpret_cast<T*>(p);
+    if (src)
+        for (int64_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = static_cast<const T*>(src)[i];
+    p += align_up(n * sizeof(T), 16);
+    return dst;
+}
+
+// LDS: every level vector, dinv and the slice offsets live in LDS (the
+// right-hand side of the top level is copied in, its result copied out to
+// `out`), so each phase waits on one global latency (the matrix entries)
+// instead of three; the arithmetic is that of the global variant.
+template <class T, bool LDS>
+__global__ void __launch_bounds__(FUSE_BLOCK) k_fused_cycle(FusedCycle fc, void* out, const int* __restrict__ done) {
     if (done && *done) return;
+    __shared__ FusedLevel slv[LDS ? FUSE_MAX_LEVELS : 1];
+    const int nl = fc.nlev;
+    const FusedLevel* lv = fc.lv;
+    if (LDS) {
+        extern __shared__ char smem[];
+        char* p = smem;
+        for (int l = 0; l < nl; ++l) {
+            FusedLevel L = fc.lv[l];
+            const int64_t n = L.A.n;
+            L.A.slice_ptr = lds_copy_i32<T>(p, L.A.slice_ptr, (n + 63) / 64 + 1);
+            if (l < nl - 1) {
+                L.R.slice_ptr = lds_copy_i32<T>(p, L.R.slice_ptr, (L.R.n + 63) / 64 + 1);
+                L.P.slice_ptr = lds_copy_i32<T>(p, L.P.slice_ptr, (L.P.n + 63) / 64 + 1);
+            }
+            L.dinv = lds_vec<T>(p, n, L.dinv);
+            L.b = lds_vec<T>(p, n, l == 0 ? L.b : nullptr);
+            L.x = lds_vec<T>(p, n, nullptr);
+            L.x2 = lds_vec<T>(p, n, nullptr);
+            L.d = lds_vec<T>(p, n, nullptr);
+            L.r = lds_vec<T>(p, n, nullptr);
+            if (threadIdx.x == 0) slv[l] = L;
+        }
+        __syncthreads();
+        lv = slv;
+    }
     T* cur[FUSE_MAX_LEVELS];
     T* nxt[FUSE_MAX_LEVELS];
-    const int nl = fc.nlev;
     for (int l = 0; l < nl - 1; ++l) {
-        const FusedLevel& L = fc.lv[l];
+        const FusedLevel& L = lv[l];
         const T* b = static_cast<const T*>(L.b);
         cur[l] = static_cast<T*>(L.x);
         nxt[l] = static_cast<T*>(L.x2);
@@ -750,10 +808,10 @@ __global__ void __launch_bounds__(FUSE_BLOCK) k_fused_cycle(FusedCycle fc, const
             if (row < n) r[row] = b[row] - ax;
         }
         __syncthreads();
-        blk_apply<T, false>(L.R, r, static_cast<T*>(fc.lv[l + 1].b));
+        blk_apply<T, false>(L.R, r, static_cast<T*>(lv[l + 1].b));
     }
     {
-        const FusedLevel& C = fc.lv[nl - 1];
+        const FusedLevel& C = lv[nl - 1];
         const int n = static_cast<int>(C.A.n);
         const T* b = static_cast<const T*>(C.b);
         T* x = static_cast<T*>(C.x);
@@ -767,9 +825,13 @@ __global__ void __launch_bounds__(FUSE_BLOCK) k_fused_cycle(FusedCycle fc, const
         __syncthreads();
     }
     for (int l = nl - 2; l >= 0; --l) {
-        const FusedLevel& L = fc.lv[l];
+        const FusedLevel& L = lv[l];
         blk_apply<T, true>(L.P, cur[l + 1], cur[l]);
         blk_smooth<T>(fc, L, static_cast<const T*>(L.b), cur[l], nxt[l], false);
+    }
+    if (LDS) {
+        T* o = static_cast<T*>(out);
+        for (int64_t i = threadIdx.x; i < lv[0].A.n; i += blockDim.x) o[i] = cur[0][i];
     }
 }
 
@@ -916,10 +978,16 @@ struct Cycle {
             const fa_mg_level_t& L = mg->levels[l + k];
             fc.lv[k] = FusedLevel{L.A, L.R, L.P, L.dinv, L.b, L.x, L.x2, L.d, L.r, L.lmax};
         }
-        hipLaunchKernelGGL(k_fused_cycle<T>, dim3(1), dim3(FUSE_BLOCK), 0, s, fc, done);
-        FA_LAUNCH("k_fused_cycle");
         const fa_mg_level_t& L = mg->levels[l];
-        *out = static_cast<T*>(fc.nlev == 1 ? L.x : L.x2);
+        T* res = static_cast<T*>(fc.nlev == 1 ? L.x : L.x2);
+        size_t lds = 0;
+        for (int k = 0; k < fc.nlev; ++k) lds += fused_level_lds<T>(fc.lv[k], k == fc.nlev - 1);
+        if (fc.nlev > 1 && lds <= FUSE_LDS_MAX)
+            hipLaunchKernelGGL((k_fused_cycle<T, true>), dim3(1), dim3(FUSE_BLOCK), lds, s, fc, res, done);
+        else
+            hipLaunchKernelGGL((k_fused_cycle<T, false>), dim3(1), dim3(FUSE_BLOCK), 0, s, fc, res, done);
+        FA_LAUNCH("k_fused_cycle");
+        *out = res;
         return FA_OK;
     }
 
@@ -1115,17 +1183,19 @@ __global__ void __launch_bounds__(RED_BLOCK) k_pcg_rz0(int64_t n, const T* __res
     store_partial(block_sum(acc), partials);
 }
 
-// Ap = A p and partials of p.Ap
+// Ap = A p and partials of p.Ap.  so != NULL: column indices are row + so[k]
+// (int16 offsets on the same slices; 2 bytes an entry instead of 4)
 template <class T>
 __global__ void __launch_bounds__(RED_BLOCK) k_pcg_spmv(int64_t n, const int32_t* __restrict__ sp,
-                                                       const int32_t* __restrict__ sc, const T* __restrict__ sv,
-                                                       const T* __restrict__ p, T* __restrict__ Ap,
-                                                       double* __restrict__ partials, const PcgState* st) {
+                                                       const int32_t* __restrict__ sc, const int16_t* __restrict__ so,
+                                                       const T* __restrict__ sv, const T* __restrict__ p,
+                                                       T* __restrict__ Ap, double* __restrict__ partials,
+                                                       const PcgState* st) {
     if (st->done) return;
     double acc = 0.0;
     for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
          row += (int64_t)gridDim.x * blockDim.x) {
-        const T v = sell_row_dot<T, T>(sp, sc, sv, row, p);
+        const T v = so ? sell_row_dot_off<T>(sp, so, sv, row, n, p) : sell_row_dot<T, T>(sp, sc, sv, row, p);
         if (row < n) {
             Ap[row] = v;
             acc += double(p[row]) * double(v);
@@ -1207,12 +1277,19 @@ static int pcg_run(const fa_sell_t* A, const fa_mg_t* M, const T* b, T* x, doubl
         FA_LAUNCH("k_pcg_rz0");
     }
     const DotOut rz_out{r, partials, std::is_same<T, double>::value ? 1 : 0};
+    // the finest level's compressed smoother operator lies on A's own slices:
+    // its int16 column offsets index A's values as well
+    const int16_t* offs = (M && M->levels[0].Ah.vals && M->levels[0].Ah.offs &&
+                           M->levels[0].Ah.slice_ptr == A->slice_ptr && M->levels[0].Ah.n == n)
+                              ? M->levels[0].Ah.offs
+                              : nullptr;
     PcgState h{};
     int64_t it = 0;
     for (;;) {
         const int64_t chunk = std::min<int64_t>(check_every, cap - it);
         for (int64_t k = 0; k < chunk; ++k) {
-            hipLaunchKernelGGL(k_pcg_spmv<T>, grid, block, 0, s, n, A->slice_ptr, A->cols, sv, p, Ap, partials, st);
+            hipLaunchKernelGGL(k_pcg_spmv<T>, grid, block, 0, s, n, A->slice_ptr, A->cols, offs, sv, p, Ap, partials,
+                               st);
             hipLaunchKernelGGL(k_pcg_finish<PCG_ALPHA>, one, block, 0, s, partials, np, st);
             if (M) {
                 hipLaunchKernelGGL((k_pcg_update<T, U, true>), grid, block, 0, s, n, x, r, ru, p, Ap, partials, st);

@@ -56,6 +56,39 @@ __device__ inline T sell_row_dot(const int32_t* __restrict__ slice_ptr, const in
     return sell_row_dot_f<T, B>(slice_ptr, cols, vals, row, [=](int32_t j) { return static_cast<T>(x[j]); });
 }
 
+// row dot on int16 column offsets (column = row + offs[k]) with full-precision
+// values: the outer operator stored on the slices of the compressed smoother
+// format; rows of the last slice past n are not evaluated
+template <class T, int B = ROW_BATCH>
+__device__ inline T sell_row_dot_off(const int32_t* __restrict__ slice_ptr, const int16_t* __restrict__ offs,
+                                     const T* __restrict__ vals, int64_t row, int64_t n, const T* __restrict__ x) {
+    if (row >= n) return T(0);
+    const int64_t s = row >> 6;
+    const int i = static_cast<int>(row & 63);
+    const int32_t p0 = slice_ptr[s];
+    const int w = (slice_ptr[s + 1] - p0) >> 6;
+    const int16_t* o = offs + p0 + i;
+    const T* v = vals + p0 + i;
+    T acc = T(0);
+    for (int k0 = 0; k0 < w; k0 += B) {
+        int oo[B];
+        T vv[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const int k = min(k0 + u, w - 1);
+            oo[u] = o[64 * k];
+            vv[u] = v[64 * k];
+        }
+        T xx[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) xx[u] = x[row + oo[u]];
+#pragma unroll
+        for (int u = 0; u < B; ++u)
+            if (k0 + u < w) acc += vv[u] * xx[u];
+    }
+    return acc;
+}
+
 // row dot of the compressed format: sum_k vals[k] * xf(row + offs[k]), FP32
 // accumulation; rows of the last slice past n are not evaluated
 template <class XF>
