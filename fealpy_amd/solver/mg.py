@@ -320,7 +320,6 @@ class GeometricMG:
             v32 = None
             if f32 and not lv.compressed:
                 v32 = lv.As.value_buffer(torch.float32, self.device)
-                lv.As.forget_values(torch.float32)
             d64 = lv.As.desc(torch.float64)
             h16 = lv.As.desc16() if lv.compressed else None
             nat.check(L.fa_smoother_setup(code, ctypes.byref(d64), nat.ptr(lv.dinv),
@@ -338,6 +337,7 @@ class GeometricMG:
             nat.check(L.fa_dense_spd_inverse(last.n, nat.ptr(last.dense), nat.stream()), "fa_dense_spd_inverse")
         self._build_desc()
         self._setup_A = A
+        self._setup_version = A.values._version
         return self
 
     def estimate_lmax(self, level, iters=None):
@@ -387,12 +387,17 @@ class GeometricMG:
     def pcg(self, A, b, x0=None, *, atol=1e-12, rtol=1e-8, maxit=10000, check_every=4):
         """Preconditioned CG on the device: A's dtype outside, V-cycle in self.dtype."""
         from ..sparse.sell import sell_of
-        if self._setup_A is not A:
+        fresh = self._setup_A is A and getattr(self, '_setup_version', None) == A.values._version
+        if not fresh:
             self.setup(A if A.dtype == torch.float64 else A.astype(torch.float64))
             self._setup_A = A
+            self._setup_version = A.values._version
         L = nat.lib()
-        S = sell_of(A)
         dt = A.dtype
+        if dt == torch.float64 and self.levels[0].A is A:
+            S = self.levels[0].As          # its FP64 copy was just filled from A by setup()
+        else:
+            S = sell_of(A)
         b = b.to(dt).contiguous()
         x = torch.zeros_like(b) if x0 is None else x0.to(dt).clone().contiguous()
         ws = nat.workspace(L.fa_pcg_work_bytes(nat.dtype_code(dt), b.shape[0]), b.device)

@@ -1,7 +1,6 @@
 """SELL-64 copies of CSR matrices for the solver kernels (see include/fealpy_amd.h)."""
 import ctypes
 import threading
-import weakref
 
 import torch
 
@@ -65,7 +64,6 @@ class SellMatrix:
         self.shape = tuple(shape)
         self.slice_ptr, self.cols, self.entries = (_sell_index if shared else _build_index)(crow, col, self.shape)
         self.vals = {}
-        self._src = {}      # dtype -> (weakref to the CSR values last copied, their version)
         self.h_offs = self.h_vals = None
 
     def set_compressed(self, values, work):
@@ -102,25 +100,15 @@ class SellMatrix:
         return out
 
     def set_values(self, values, dtype=None):
-        """Copy CSR values (FP64 or FP32) into the SELL layout as `dtype`.
-        Skipped when this array was last filled from the same tensor object
-        at the same version (torch bumps it on in-place changes)."""
+        """Copy CSR values (FP64 or FP32) into the SELL layout as `dtype`."""
         dtype = values.dtype if dtype is None else dtype
         out = self.value_buffer(dtype, values.device)
-        src = self._src.get(dtype)
-        if src is not None and src[0]() is values and src[1] == values._version:
-            return out
         if self.entries == 0 or values.numel() == 0:      # no stored entries (e.g. an empty transfer)
             return out
         nat.check(nat.lib().fa_sell_fill(nat.dtype_code(values.dtype), nat.dtype_code(dtype), self.shape[0],
                                          self.shape[1], nat.ptr(self.crow), nat.ptr(self.col), nat.ptr(values.contiguous()),
                                          nat.ptr(self.slice_ptr), None, nat.ptr(out), nat.stream()), "fa_sell_fill")
-        self._src[dtype] = (weakref.ref(values), values._version)
         return out
-
-    def forget_values(self, dtype):
-        """The `dtype` array was written by another kernel: refill on next use."""
-        self._src.pop(dtype, None)
 
     def desc(self, dtype):
         v = self.vals[dtype]

@@ -382,3 +382,29 @@ def test_dense_spd_inverse(n):
     nat.check(nat.lib().fa_dense_spd_inverse(n, nat.ptr(a), nat.stream()), "fa_dense_spd_inverse")
     ref = np.linalg.inv(A)
     assert np.abs(a.cpu().numpy() - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_reused_hierarchy_equals_fresh(dtype):
+    """A hierarchy set up again for a new coefficient equals a fresh one bit
+    for bit: coarse operators, their SELL copies, smoother data and the
+    V-cycle (regression: coarse SELL copies were not refilled after the
+    Galerkin kernel rewrote the coarse values in place)."""
+    mesh, space, bc, A, b = _system(16)
+    k1, k2 = _lognormal(mesh, seed=21), _lognormal(mesh, seed=22)
+    _, _, _, A1, _ = _system(16, k=k1)
+    _, _, _, A2, b2 = _system(16, k=k2)
+    used = GeometricMG(space, bc.is_boundary_dof, dtype=dtype).setup(A1)
+    used.setup(A2)
+    fresh = GeometricMG(space, bc.is_boundary_dof, dtype=dtype).setup(A2)
+    for lu, lf in zip(used.levels, fresh.levels):
+        assert torch.equal(lu.A.values, lf.A.values)
+        assert torch.equal(lu.As.vals[torch.float64], lf.As.vals[torch.float64])
+        assert torch.equal(lu.dinv, lf.dinv) and lu.lmax == lf.lmax
+        if getattr(lf, "compressed", False):
+            assert torch.equal(lu.As.h_vals, lf.As.h_vals)
+    r = b2.to(dtype)
+    assert torch.equal(used @ r, fresh @ r)
+    x1, i1 = fa.cg(A2, b2, M=used, atol=0.0, rtol=1e-10, returninfo=True)
+    x2, i2 = fa.cg(A2, b2, M=fresh, atol=0.0, rtol=1e-10, returninfo=True)
+    assert torch.equal(x1, x2) and i1["niter"] == i2["niter"]
