@@ -85,6 +85,13 @@ _SIGNATURES = {
     "fa_dense_spd_inverse": (ctypes.c_int, [_i64, _p, _p]),
     "fa_mg_lmax": (ctypes.c_int, [_p, ctypes.c_int, ctypes.c_int, _pf64, _p, _sz, _p]),
     "fa_mg_vcycle": (ctypes.c_int, [_p, _p, _p, _p]),
+    "fa_incidence_maps": (ctypes.c_int, [_i64, _i32, _p, _p, _p, _p, _p, _p, _p, _p, _sz, _p]),
+    "fa_p1_incidence_rows": (ctypes.c_int, [ctypes.c_int, _i64, ctypes.c_int, _p, _p, _p, _p, ctypes.c_int, _f64, _p,
+                                            ctypes.c_int, _f64, _p, _p, _p, _p]),
+    "fa_sum_incidence_rows": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64, _p, _p, _p, _p, _i32, _p, _p]),
+    "fa_p1_incidence_vectors": (ctypes.c_int, [ctypes.c_int, _i64, ctypes.c_int, _p, _p, _p, _p, ctypes.c_int, _f64,
+                                               _p, _p, _p, _p]),
+    "fa_sum_incidence_values": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p]),
     "fa_smoother_setup": (ctypes.c_int, [ctypes.c_int, _p, _p, _p, _p, _p, _p, _p, _sz, _p]),
     "fa_jacobi_setup": (ctypes.c_int, [ctypes.c_int, _i64, _i64, _p, _p, _p, _p, _pf64, _p, _sz, _p]),
     "fa_pcg_work_bytes": (_sz, [ctypes.c_int, _i64]),

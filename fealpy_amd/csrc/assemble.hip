@@ -879,3 +879,393 @@ extern "C" int fa_element_vectors_tensor(int dtype, int td, int ldof, int ncomp,
     FA_LAUNCH("k_element_vectors_tensor");
     return FA_OK;
 }
+
+// ---- incidence-ordered two-pass P1 assembly -----------------------------------
+// The fused kernel above computes cell geometry once per incidence, behind
+// three dependent loads (incidence -> cell -> vertex coordinates): it is bound
+// by load latency, not by bytes.  Here pass 1 runs one thread per cell, reads
+// the cell once, and writes element row i of cell c into the slot of
+// incidence (row r = cell2dof[c][i], cell c).  A row's incidences are stored
+// contiguously in ascending cell order (the adjacency), so pass 2 streams
+// them: per CSR row, the element rows of its incidences (ldof doubles each)
+// and their packed column positions (one byte per local DOF), summed through
+// the same LDS table and in the same order as the fused kernel, which makes
+// the values equal bit for bit.  The load vector is the same with one value
+// per incidence and a plain ordered sum per row.
+namespace fa {
+
+// inc[c*ldof + i] = incidence slot of (c, i); pos[k] packs the CSR row
+// positions of the cell's DOFs for incidence k (byte j: position of DOF j)
+__global__ void __launch_bounds__(256) k_incidence_maps(int64_t gdof, int ldof, const int32_t* __restrict__ c2d,
+                                                       const int32_t* __restrict__ adj_ptr,
+                                                       const int32_t* __restrict__ adj,
+                                                       const int32_t* __restrict__ indptr,
+                                                       const int32_t* __restrict__ indices, int32_t* __restrict__ inc,
+                                                       uint32_t* __restrict__ pos, int32_t* __restrict__ bad) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < gdof; r += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t r0 = indptr[r], L = indptr[r + 1] - indptr[r];
+        for (int32_t k = adj_ptr[r]; k < adj_ptr[r + 1]; ++k) {
+            const int32_t e = adj[k];
+            inc[e] = k;
+            const int64_t c = e / ldof;
+            uint32_t packed = 0;
+            for (int j = 0; j < ldof; ++j) {
+                const int32_t col = c2d[c * ldof + j];
+                int lo = 0, hi = L;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (indices[r0 + mid] < col) lo = mid + 1;
+                    else hi = mid;
+                }
+                if (lo >= L || indices[r0 + lo] != col || lo > 255) atomicOr(bad, 1);
+                packed |= static_cast<uint32_t>(lo & 255) << (8 * j);
+            }
+            pos[k] = packed;
+        }
+    }
+}
+
+// Pass 1 stages a block's 256 cells through LDS with coalesced loads (vertex
+// ids, incidence slots, per-cell / per-point coefficients: contiguous per
+// block) and reads vertex coordinates with one wide load per vertex (node
+// rows padded to 4 doubles in 3D, 2 doubles are already 16-byte rows in 2D):
+// the kernels are bound by the texture address unit, so fewer, wider memory
+// instructions are what counts.  Values equal the fused kernel's bit for bit.
+constexpr int P1_BLOCK = 256;
+
+__device__ inline size_t coef_stage_len(int kind, int nq) {
+    return kind == FA_COEF_CELL ? 1 : (kind == FA_COEF_CELLQ ? static_cast<size_t>(nq) : 0);
+}
+
+// stage cells [c0, c0 + P1_BLOCK) of `src` (len doubles per cell)
+__device__ inline void stage_doubles(double* dst, const double* __restrict__ src, int64_t c0, int64_t nc, size_t len) {
+    const int64_t n = (min<int64_t>(nc - c0, P1_BLOCK)) * static_cast<int64_t>(len);
+    const double* s0 = src + c0 * static_cast<int64_t>(len);
+    for (int64_t t = threadIdx.x; t < n; t += blockDim.x) dst[t] = s0[t];
+}
+
+__device__ inline void stage_ints(int32_t* dst, const int32_t* __restrict__ src, int64_t c0, int64_t nc, int len) {
+    const int64_t n = (min<int64_t>(nc - c0, P1_BLOCK)) * len;
+    const int32_t* s0 = src + c0 * len;
+    for (int64_t t = threadIdx.x; t < n; t += blockDim.x) dst[t] = s0[t];
+}
+
+template <int TD>
+__device__ inline void load_vertices_wide(const double* __restrict__ node, const int32_t* vid, double p[TD + 1][TD]) {
+#pragma unroll
+    for (int a = 0; a < TD + 1; ++a) {
+        if (TD == 3) {
+            const double4 q = reinterpret_cast<const double4*>(node)[vid[a]];
+            p[a][0] = q.x;
+            p[a][1] = q.y;
+            p[a][TD - 1] = q.z;
+        } else {
+            const double2 q = reinterpret_cast<const double2*>(node)[vid[a]];
+            p[a][0] = q.x;
+            p[a][TD - 1] = q.y;
+        }
+    }
+}
+
+// pass 1, matrix: rows[inc[c*NV + i]][j] = element entry (i, j) of cell c
+template <int TD>
+__global__ void __launch_bounds__(P1_BLOCK) k_p1_incidence_rows(int64_t nc, const double* __restrict__ nodew,
+                                                               const int32_t* __restrict__ cell, int nq,
+                                                               const double* __restrict__ qw,
+                                                               const double* __restrict__ phi, int dk, double ds,
+                                                               const double* __restrict__ dc, int mk, double ms,
+                                                               const double* __restrict__ mc,
+                                                               const int32_t* __restrict__ inc,
+                                                               double* __restrict__ rows) {
+    constexpr int NV = TD + 1;
+    extern __shared__ double stage[];
+    const size_t ld = coef_stage_len(dk, nq), lm = coef_stage_len(mk, nq);
+    double* dS = stage;
+    double* mS = dS + P1_BLOCK * ld;
+    int32_t* vS = reinterpret_cast<int32_t*>(mS + P1_BLOCK * lm);
+    int32_t* iS = vS + P1_BLOCK * NV;
+    for (int64_t c0 = static_cast<int64_t>(blockIdx.x) * P1_BLOCK; c0 < nc; c0 += static_cast<int64_t>(gridDim.x) * P1_BLOCK) {
+        if (ld) stage_doubles(dS, dc, c0, nc, ld);
+        if (lm) stage_doubles(mS, mc, c0, nc, lm);
+        stage_ints(vS, cell, c0, nc, NV);
+        stage_ints(iS, inc, c0, nc, NV);
+        __syncthreads();
+        const int t = threadIdx.x;
+        if (c0 + t < nc) {
+            double p[NV][TD];
+            load_vertices_wide<TD>(nodew, vS + t * NV, p);
+            P1Cell<TD> g;
+            g.compute(p);
+            // coefficients through the staged block: cell index t, row stride ld / lm
+            const double* dcl = ld ? dS : nullptr;
+            const double* mcl = lm ? mS : nullptr;
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                double v[NV];
+                p1_row<TD>(g, i, t, nq, qw, phi, dk, ds, dcl, mk, ms, mcl, v);
+                double* out = rows + static_cast<int64_t>(iS[t * NV + i]) * NV;
+#pragma unroll
+                for (int j = 0; j < NV; ++j) out[j] = v[j];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// pass 1, load vector: vals[inc[c*NV + i]] = |T| sum_q w_q f_cq phi_i(q)
+template <int TD>
+__global__ void __launch_bounds__(P1_BLOCK) k_p1_incidence_vectors(int64_t nc, const double* __restrict__ nodew,
+                                                                  const int32_t* __restrict__ cell, int nq,
+                                                                  const double* __restrict__ qw,
+                                                                  const double* __restrict__ phi, int sk, double ss,
+                                                                  const double* __restrict__ src,
+                                                                  const int32_t* __restrict__ inc,
+                                                                  double* __restrict__ vals) {
+    constexpr int NV = TD + 1;
+    extern __shared__ double stage[];
+    const size_t ls = coef_stage_len(sk, nq);
+    double* sS = stage;
+    int32_t* vS = reinterpret_cast<int32_t*>(sS + P1_BLOCK * ls);
+    int32_t* iS = vS + P1_BLOCK * NV;
+    for (int64_t c0 = static_cast<int64_t>(blockIdx.x) * P1_BLOCK; c0 < nc; c0 += static_cast<int64_t>(gridDim.x) * P1_BLOCK) {
+        if (ls) stage_doubles(sS, src, c0, nc, ls);
+        stage_ints(vS, cell, c0, nc, NV);
+        stage_ints(iS, inc, c0, nc, NV);
+        __syncthreads();
+        const int t = threadIdx.x;
+        if (c0 + t < nc) {
+            double p[NV][TD];
+            load_vertices_wide<TD>(nodew, vS + t * NV, p);
+            P1Cell<TD> g;
+            g.compute(p);
+            const double* sl = ls ? sS : nullptr;
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                double v = 0.0;
+                for (int q = 0; q < nq; ++q) v += qw[q] * g.meas * coef_value(sk, ss, sl, t, q, nq) * phi[q * NV + i];
+                vals[iS[t * NV + i]] = v;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// pass 2, matrix: the fused kernel's LDS-table sum with the element rows and
+// positions read from the incidence slots instead of computed
+template <int NV, int G, int K, class T>
+__global__ void __launch_bounds__(256) k_sum_incidence_rows(int64_t gdof, const int32_t* __restrict__ adj_ptr,
+                                                           const int32_t* __restrict__ indptr,
+                                                           const uint32_t* __restrict__ pos,
+                                                           const double* __restrict__ rows, int lmax,
+                                                           T* __restrict__ values) {
+    constexpr int GS = G + 1;
+    extern __shared__ double tabs[];
+    const int rpb = blockDim.x / G;
+    const int sub = threadIdx.x / G;
+    const int lane = threadIdx.x % G;
+    double* tab = tabs + (int64_t)sub * lmax * GS;
+    (void)rpb;
+    for (int64_t r = (int64_t)blockIdx.x * (blockDim.x / G) + sub; r < gdof; r += (int64_t)gridDim.x * (blockDim.x / G)) {
+        const int32_t r0 = indptr[r];
+        const int L = indptr[r + 1] - r0;
+        for (int t = 0; t < L; ++t) tab[t * GS + lane] = 0.0;
+        double acc[K];
+#pragma unroll
+        for (int m = 0; m < K; ++m) acc[m] = 0.0;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const int32_t k1 = adj_ptr[r + 1];
+        for (int32_t kb = adj_ptr[r]; kb < k1; kb += G) {
+            const int32_t k = kb + lane;
+            int p[NV];
+            if (k < k1) {
+                const uint32_t pk = pos[k];
+                const double* v = rows + static_cast<int64_t>(k) * NV;
+#pragma unroll
+                for (int j = 0; j < NV; ++j) {
+                    p[j] = static_cast<int>((pk >> (8 * j)) & 255u);
+                    tab[p[j] * GS + lane] = v[j];
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+            for (int m = 0; m < K; ++m) {
+                const int t = lane + m * G;
+                if (t < L) {
+#pragma unroll
+                    for (int l = 0; l < G; ++l) acc[m] += tab[t * GS + l];
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            if (k < k1) {
+#pragma unroll
+                for (int j = 0; j < NV; ++j) tab[p[j] * GS + lane] = 0.0;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < K; ++m) {
+            const int t = lane + m * G;
+            if (t < L) values[r0 + t] = static_cast<T>(acc[m]);
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+}
+
+// pass 2, load vector: out[r] = sum of vals[k] over the row's incidences, in order
+template <class T>
+__global__ void __launch_bounds__(256) k_sum_incidence_values(int64_t gdof, const int32_t* __restrict__ adj_ptr,
+                                                             const double* __restrict__ vals, T* __restrict__ out) {
+    constexpr int B = 8;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < gdof; r += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t k0 = adj_ptr[r], k1 = adj_ptr[r + 1];
+        double acc = 0.0;
+        for (int32_t kb = k0; kb < k1; kb += B) {
+            double v[B];
+#pragma unroll
+            for (int u = 0; u < B; ++u) v[u] = vals[min(kb + u, k1 - 1)];
+#pragma unroll
+            for (int u = 0; u < B; ++u)
+                if (kb + u < k1) acc += v[u];
+        }
+        out[r] = static_cast<T>(acc);
+    }
+}
+
+template <int NV, int G, int K, class T>
+static int launch_sum_rows(int64_t gdof, const int32_t* adj_ptr, const int32_t* indptr, const uint32_t* pos,
+                           const double* rows, int lmax, void* values, hipStream_t s) {
+    const size_t per_row = sizeof(double) * lmax * (G + 1);
+    int rpb = 256 / G;
+    while (rpb > 1 && rpb * per_row > 48 * 1024) rpb >>= 1;
+    const size_t lds = per_row * rpb;
+    if (lds > 64 * 1024) {
+        set_error("fa_sum_incidence_rows: row length %d exceeds the LDS capacity", lmax);
+        return FA_ECAPACITY;
+    }
+    const unsigned grid = grid_for(gdof, rpb, 1 << 20);
+    hipLaunchKernelGGL((k_sum_incidence_rows<NV, G, K, T>), dim3(grid), dim3(rpb * G), lds, s, gdof, adj_ptr, indptr,
+                       pos, rows, lmax, static_cast<T*>(values));
+    FA_LAUNCH("k_sum_incidence_rows");
+    return FA_OK;
+}
+
+}  // namespace fa
+
+extern "C" int fa_incidence_maps(int64_t gdof, int32_t ldof, const int32_t* cell2dof, const int32_t* adj_ptr,
+                                 const int32_t* adj, const int32_t* indptr, const int32_t* indices, int32_t* inc,
+                                 uint32_t* pos, void* work, size_t work_bytes, fa_stream_t stream) {
+    FA_REQUIRE(gdof > 0 && ldof >= 1 && ldof <= 4 && inc && pos, "fa_incidence_maps: bad arguments (ldof <= 4)");
+    FA_REQUIRE(work && work_bytes >= sizeof(int32_t), "fa_incidence_maps: workspace too small");
+    hipStream_t s = S(stream);
+    int32_t* bad = static_cast<int32_t*>(work);
+    FA_HIP(hipMemsetAsync(bad, 0, sizeof(int32_t), s));
+    hipLaunchKernelGGL(k_incidence_maps, dim3(grid_for(gdof, 256, 1 << 16)), dim3(256), 0, s, gdof, ldof, cell2dof,
+                       adj_ptr, adj, indptr, indices, inc, pos, bad);
+    FA_LAUNCH("k_incidence_maps");
+    int32_t h = 0;
+    FA_HIP(hipMemcpyAsync(&h, bad, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    FA_HIP(hipStreamSynchronize(s));
+    if (h) {
+        set_error("fa_incidence_maps: a row is longer than 256 entries or misses a DOF of an incident cell");
+        return FA_ECAPACITY;
+    }
+    return FA_OK;
+}
+
+static size_t p1_stage_bytes(int td, int nq, int k1, int k2) {
+    auto len = [nq](int kind) -> size_t {
+        return kind == FA_COEF_CELL ? 1 : (kind == FA_COEF_CELLQ ? static_cast<size_t>(nq) : 0);
+    };
+    return sizeof(double) * P1_BLOCK * (len(k1) + len(k2)) + 2 * sizeof(int32_t) * P1_BLOCK * (td + 1);
+}
+
+extern "C" int fa_p1_incidence_rows(int td, int64_t nc, int nq, const double* node_wide, const int32_t* cell,
+                                    const double* qw, const double* phi, int diff_kind, double diff_scale,
+                                    const double* diff_coef, int mass_kind, double mass_scale,
+                                    const double* mass_coef, const int32_t* inc, double* rows, fa_stream_t stream) {
+    FA_REQUIRE(td == 2 || td == 3, "fa_p1_incidence_rows: td must be 2 or 3");
+    FA_REQUIRE(nq > 0 && inc && rows && node_wide, "fa_p1_incidence_rows: bad arguments");
+    FA_REQUIRE(!mass_kind || phi, "fa_p1_incidence_rows: mass needs phi");
+    FA_REQUIRE(!(diff_kind >= FA_COEF_CELL && !diff_coef) && !(mass_kind >= FA_COEF_CELL && !mass_coef),
+               "fa_p1_incidence_rows: coefficient array missing");
+    if (nc == 0) return FA_OK;
+    const size_t lds = p1_stage_bytes(td, nq, diff_kind, mass_kind);
+    if (lds > 64 * 1024) {
+        set_error("fa_p1_incidence_rows: %d quadrature points per cell exceed the staging capacity", nq);
+        return FA_ECAPACITY;
+    }
+    const unsigned grid = grid_for(nc, P1_BLOCK, 1 << 20);
+    if (td == 3)
+        hipLaunchKernelGGL(k_p1_incidence_rows<3>, dim3(grid), dim3(P1_BLOCK), lds, S(stream), nc, node_wide, cell, nq,
+                           qw, phi, diff_kind, diff_scale, diff_coef, mass_kind, mass_scale, mass_coef, inc, rows);
+    else
+        hipLaunchKernelGGL(k_p1_incidence_rows<2>, dim3(grid), dim3(P1_BLOCK), lds, S(stream), nc, node_wide, cell, nq,
+                           qw, phi, diff_kind, diff_scale, diff_coef, mass_kind, mass_scale, mass_coef, inc, rows);
+    FA_LAUNCH("k_p1_incidence_rows");
+    return FA_OK;
+}
+
+extern "C" int fa_sum_incidence_rows(int dtype, int td, int64_t gdof, const int32_t* adj_ptr, const int32_t* indptr,
+                                     const uint32_t* pos, const double* rows, int32_t max_row, void* values,
+                                     fa_stream_t stream) {
+    FA_REQUIRE(dtype == FA_F64 || dtype == FA_F32, "fa_sum_incidence_rows: bad dtype %d", dtype);
+    FA_REQUIRE(td == 2 || td == 3, "fa_sum_incidence_rows: td must be 2 or 3");
+    FA_REQUIRE(gdof > 0 && max_row > 0 && max_row <= 256, "fa_sum_incidence_rows: bad sizes");
+    hipStream_t s = S(stream);
+#define FA_SR(NV_, G_, K_)                                                                                         \
+    return dtype == FA_F64 ? launch_sum_rows<NV_, G_, K_, double>(gdof, adj_ptr, indptr, pos, rows, max_row, values, s) \
+                           : launch_sum_rows<NV_, G_, K_, float>(gdof, adj_ptr, indptr, pos, rows, max_row, values, s)
+    if (td == 3) {
+        if (max_row <= 32) FA_SR(4, 16, 2);
+        if (max_row <= 64) FA_SR(4, 16, 4);
+        if (max_row <= 128) FA_SR(4, 32, 4);
+        FA_SR(4, 32, 8);
+    } else {
+        if (max_row <= 16) FA_SR(3, 8, 2);
+        if (max_row <= 32) FA_SR(3, 8, 4);
+        if (max_row <= 128) FA_SR(3, 32, 4);
+        FA_SR(3, 32, 8);
+    }
+#undef FA_SR
+}
+
+extern "C" int fa_p1_incidence_vectors(int td, int64_t nc, int nq, const double* node_wide, const int32_t* cell,
+                                       const double* qw, const double* phi, int src_kind, double src_scale,
+                                       const double* src, const int32_t* inc, double* vals, fa_stream_t stream) {
+    FA_REQUIRE(td == 2 || td == 3, "fa_p1_incidence_vectors: td must be 2 or 3");
+    FA_REQUIRE(nq > 0 && phi && inc && vals && node_wide, "fa_p1_incidence_vectors: bad arguments");
+    FA_REQUIRE(src_kind < FA_COEF_CELL || src, "fa_p1_incidence_vectors: source array missing");
+    if (nc == 0) return FA_OK;
+    const size_t lds = p1_stage_bytes(td, nq, src_kind, FA_COEF_NONE);
+    if (lds > 64 * 1024) {
+        set_error("fa_p1_incidence_vectors: %d quadrature points per cell exceed the staging capacity", nq);
+        return FA_ECAPACITY;
+    }
+    const unsigned grid = grid_for(nc, P1_BLOCK, 1 << 20);
+    if (td == 3)
+        hipLaunchKernelGGL(k_p1_incidence_vectors<3>, dim3(grid), dim3(P1_BLOCK), lds, S(stream), nc, node_wide, cell,
+                           nq, qw, phi, src_kind, src_scale, src, inc, vals);
+    else
+        hipLaunchKernelGGL(k_p1_incidence_vectors<2>, dim3(grid), dim3(P1_BLOCK), lds, S(stream), nc, node_wide, cell,
+                           nq, qw, phi, src_kind, src_scale, src, inc, vals);
+    FA_LAUNCH("k_p1_incidence_vectors");
+    return FA_OK;
+}
+
+extern "C" int fa_sum_incidence_values(int dtype, int64_t gdof, const int32_t* adj_ptr, const double* vals, void* out,
+                                       fa_stream_t stream) {
+    FA_REQUIRE(dtype == FA_F64 || dtype == FA_F32, "fa_sum_incidence_values: bad dtype %d", dtype);
+    FA_REQUIRE(gdof > 0 && adj_ptr && vals && out, "fa_sum_incidence_values: bad arguments");
+    const unsigned grid = grid_for(gdof, 256, 1 << 20);
+    if (dtype == FA_F64)
+        hipLaunchKernelGGL(k_sum_incidence_values<double>, dim3(grid), dim3(256), 0, S(stream), gdof, adj_ptr, vals,
+                           static_cast<double*>(out));
+    else
+        hipLaunchKernelGGL(k_sum_incidence_values<float>, dim3(grid), dim3(256), 0, S(stream), gdof, adj_ptr, vals,
+                           static_cast<float*>(out));
+    FA_LAUNCH("k_sum_incidence_values");
+    return FA_OK;
+}

@@ -137,7 +137,7 @@ class BilinearForm(Form):
     def _p1_fused(self, space, dtype):
         """P1 scalar diffusion/mass groups over all cells: one fused kernel per
         group (fa_assemble_p1_matrix), no element buffer."""
-        if space.p != 1 or getattr(space, 'TD', None) not in (2, 3):
+        if space.p != 1 or getattr(space, 'TD', None) not in (2, 3) or space.mesh.GD != space.TD:
             return None
         groups = [p1_fusable_ops(it) for it in self.integrators.values()]
         if not groups or any(g is None for g in groups):
@@ -182,7 +182,8 @@ class LinearForm(Form):
         full_map = space.cell_to_dof()
         out = None
         ints = list(self.integrators.values())
-        if space.p == 1 and getattr(space, 'TD', None) in (2, 3) and ints and all(map(p1_source_fusable, ints)):
+        if (space.p == 1 and getattr(space, 'TD', None) in (2, 3) and space.mesh.GD == space.TD and ints
+                and all(map(p1_source_fusable, ints))):
             pat = pattern_of(space)
             dt = torch.float64 if dtype is None else dtype
             for it in ints:                         # fused P1 load vector, no element buffer

@@ -324,6 +324,16 @@ def p1_matrix_values(space, ops, pat, dtype):
         spec[op] = coef_spec(it.coef, space, q)
     (dk, ds, dc), (mk, ms, mc) = spec['diff'], spec['mass']
     values = torch.empty(pat.nnz, dtype=dtype, device=space.device)
+    maps = pat.incidence_maps()
+    if maps is not None:         # pass 1 per cell into incidence slots, pass 2 streams each row's slots
+        rows = torch.empty(pat.nc * pat.ldof * pat.ldof, dtype=torch.float64, device=space.device)
+        nat.check(L.fa_p1_incidence_rows(space.TD, pat.nc, qw.shape[0], nat.ptr(mesh.node_rows16()), nat.ptr(cell),
+                                         nat.ptr(qw), nat.ptr(phi), dk, ds, nat.ptr(dc), mk, ms, nat.ptr(mc),
+                                         nat.ptr(maps[0]), nat.ptr(rows), nat.stream()), "fa_p1_incidence_rows")
+        nat.check(L.fa_sum_incidence_rows(nat.dtype_code(dtype), space.TD, pat.gdof, nat.ptr(pat.adj_ptr),
+                                          nat.ptr(pat.indptr), nat.ptr(maps[1]), nat.ptr(rows), pat.max_row,
+                                          nat.ptr(values), nat.stream()), "fa_sum_incidence_rows")
+        return values
     nat.check(L.fa_assemble_p1_matrix(nat.dtype_code(dtype), space.TD, pat.gdof, qw.shape[0], nat.ptr(mesh.node),
                                       nat.ptr(cell), nat.ptr(qw), nat.ptr(phi), dk, ds, nat.ptr(dc), mk, ms,
                                       nat.ptr(mc), nat.ptr(pat.adj_ptr), nat.ptr(pat.adj), nat.ptr(pat.indptr),
@@ -345,6 +355,16 @@ def p1_vector_values(space, it, pat, dtype):
     if it.source is None:
         kind, scale, src = nat.COEF_CONST, 1.0, None
     out = torch.empty(pat.gdof, dtype=dtype, device=space.device)
+    maps = pat.incidence_maps()
+    if maps is not None:         # one value per incidence slot, then an ordered sum per row
+        vals = torch.empty(pat.nc * pat.ldof, dtype=torch.float64, device=space.device)
+        nat.check(L.fa_p1_incidence_vectors(space.TD, pat.nc, qw.shape[0], nat.ptr(mesh.node_rows16()),
+                                            nat.ptr(space.cell_to_dof()), nat.ptr(qw), nat.ptr(phi), kind, scale,
+                                            nat.ptr(src), nat.ptr(maps[0]), nat.ptr(vals), nat.stream()),
+                  "fa_p1_incidence_vectors")
+        nat.check(L.fa_sum_incidence_values(nat.dtype_code(dtype), pat.gdof, nat.ptr(pat.adj_ptr), nat.ptr(vals),
+                                            nat.ptr(out), nat.stream()), "fa_sum_incidence_values")
+        return out
     nat.check(L.fa_assemble_p1_vector(nat.dtype_code(dtype), space.TD, pat.gdof, qw.shape[0], nat.ptr(mesh.node),
                                       nat.ptr(space.cell_to_dof()), nat.ptr(qw), nat.ptr(phi), kind, scale,
                                       nat.ptr(src), nat.ptr(pat.adj_ptr), nat.ptr(pat.adj), nat.ptr(out),

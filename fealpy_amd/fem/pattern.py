@@ -12,7 +12,7 @@ from .. import _native as nat
 
 class Pattern:
     __slots__ = ("cell2dof", "gdof", "ldof", "nc", "adj_ptr", "adj", "max_degree", "indptr", "indices", "nnz",
-                 "max_row")
+                 "max_row", "_maps", "_maps_lock")
 
     def __init__(self, cell2dof, gdof):
         L = nat.lib()
@@ -42,6 +42,36 @@ class Pattern:
                                     self.max_degree, nat.ptr(self.indptr), nat.ptr(self.indices), nat.stream()),
                   "fa_pattern_fill")
         self.indices = self.indices[:self.nnz]
+        self._maps = None
+        self._maps_lock = threading.Lock()
+
+    def incidence_maps(self):
+        """(inc, pos) of the two-pass P1 assembly (fa_incidence_maps): the
+        incidence slot of every (cell, local DOF) and, per incidence, the CSR
+        row positions of the cell's DOFs packed one byte each.  Built once and
+        completed on the building stream before it is shared; None for
+        ldof > 4 or rows longer than 256 entries."""
+        if self._maps is None:
+            with self._maps_lock:
+                if self._maps is None:
+                    self._maps = self._build_maps()
+        return self._maps or None
+
+    def _build_maps(self):
+        if self.ldof > 4 or self.max_row > 256 or self.nnz == 0 or self.nc == 0:
+            return False
+        L = nat.lib()
+        dev = self.indptr.device
+        inc = torch.empty(self.nc * self.ldof, dtype=torch.int32, device=dev)
+        pos = torch.empty(self.nc * self.ldof, dtype=torch.int32, device=dev)
+        ws = nat.workspace(64, dev)
+        st = L.fa_incidence_maps(self.gdof, self.ldof, nat.ptr(self.cell2dof), nat.ptr(self.adj_ptr),
+                                 nat.ptr(self.adj), nat.ptr(self.indptr), nat.ptr(self.indices), nat.ptr(inc),
+                                 nat.ptr(pos), nat.ptr(ws), ws.numel(), nat.stream())
+        if st == nat.FA_ECAPACITY:
+            return False
+        nat.check(st, "fa_incidence_maps")
+        return (inc, pos)
 
     def gather_matrix(self, elem, values=None):
         """CSR values from element matrices (NC, ldof, ldof) in ascending-cell order."""

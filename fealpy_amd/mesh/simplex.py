@@ -95,6 +95,23 @@ class SimplexMesh:
         self._cache = {}
         self.construct()
 
+    def node_rows16(self):
+        """Vertex coordinates as 16-byte-aligned rows for wide loads in the
+        assembly kernels: the node array itself in 2D ((NN, 2) FP64), a copy
+        padded to (NN, 4) in 3D, refreshed when `node` is replaced or changed
+        in place."""
+        node = self.node
+        if self.GD == 2 and node.dtype == torch.float64 and node.is_contiguous():
+            return node
+        key = (id(node), node._version)
+        hit = self._cache.get('_node_rows16')
+        if hit is not None and hit[0] == key and hit[1] is node:
+            return hit[2]
+        wide = torch.zeros((node.shape[0], 4), dtype=torch.float64, device=node.device)
+        wide[:, :node.shape[1]] = node
+        self._cache['_node_rows16'] = (key, node, wide)
+        return wide
+
     # -- topology ---------------------------------------------------------------
     def construct(self):
         cell = self.cell

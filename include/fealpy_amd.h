@@ -190,6 +190,35 @@ int fa_assemble_p1_vector(int dtype, int td, int64_t gdof, int nq, const double*
                           int src_kind, double src_scale, const double* src,
                           const int32_t* adj_ptr, const int32_t* adj, void* out, fa_stream_t stream);
 
+/* Two-pass P1 assembly in incidence order, the same values as
+ * fa_assemble_p1_matrix bit for bit (same element rows, same LDS-table sum in
+ * ascending cell order: COOTensor.coalesce's order, sparse/coo_tensor.py:
+ * 184-213).  Built once per pattern, fa_incidence_maps: inc[c*ldof + i] = the
+ * incidence slot k of (cell c, local DOF i) in the DOF -> incidence list
+ * (adj_ptr, adj), and pos[k] = the CSR row positions of the cell's DOFs, one
+ * byte each (ldof <= 4, rows of at most 256 entries; syncs).  Pass 1, one
+ * thread per cell, vertex coordinates `node_wide` as 16-byte rows (NN x 4
+ * doubles in 3D, the node array itself in 2D) (ScalarDiffusionIntegrator / ScalarMassIntegrator /
+ * ScalarSourceIntegrator .assembly, fem/scalar_*_integrator.py):
+ * fa_p1_incidence_rows writes element row i of cell c at rows[inc*ldof ..],
+ * fa_p1_incidence_vectors the load-vector entry at vals[inc].  Pass 2 streams
+ * each row's slots: fa_sum_incidence_rows (CSR values), fa_sum_incidence_values
+ * (the load vector, an ordered sum per DOF). */
+int fa_incidence_maps(int64_t gdof, int32_t ldof, const int32_t* cell2dof, const int32_t* adj_ptr,
+                      const int32_t* adj, const int32_t* indptr, const int32_t* indices, int32_t* inc, uint32_t* pos,
+                      void* work, size_t work_bytes, fa_stream_t stream);
+int fa_p1_incidence_rows(int td, int64_t nc, int nq, const double* node_wide, const int32_t* cell, const double* qw,
+                         const double* phi, int diff_kind, double diff_scale, const double* diff_coef, int mass_kind,
+                         double mass_scale, const double* mass_coef, const int32_t* inc, double* rows,
+                         fa_stream_t stream);
+int fa_sum_incidence_rows(int dtype, int td, int64_t gdof, const int32_t* adj_ptr, const int32_t* indptr,
+                          const uint32_t* pos, const double* rows, int32_t max_row, void* values, fa_stream_t stream);
+int fa_p1_incidence_vectors(int td, int64_t nc, int nq, const double* node_wide, const int32_t* cell, const double* qw,
+                            const double* phi, int src_kind, double src_scale, const double* src, const int32_t* inc,
+                            double* vals, fa_stream_t stream);
+int fa_sum_incidence_values(int dtype, int64_t gdof, const int32_t* adj_ptr, const double* vals, void* out,
+                            fa_stream_t stream);
+
 /* Physical points of barycentric quadrature points on every cell:
  * out[c][q][d] = sum_v bcs[q][v] node[cell[c][v]][d]  (FP64; (gd, nv) in
  * {(2, 3), (3, 3), (3, 4)}).

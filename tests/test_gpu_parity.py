@@ -263,3 +263,59 @@ def test_fused_p1_matches_element_gather(dim):
     A32 = fa.BilinearForm(space).add_integrator(di, ma).assembly(dtype=torch.float32)
     assert A32.values.dtype == torch.float32
     assert float((A32.values.double() - ref).abs().max()) <= 1e-6 * scale
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_incidence_two_pass_p1(dim):
+    """Two-pass P1 assembly in incidence order: the maps equal a NumPy
+    construction; the CSR values equal the fused single kernel bit for bit
+    (FP64 and FP32 output, diffusion + mass with per-point / per-cell
+    coefficients); the load vector equals the fused kernel to rounding (that
+    kernel adds lane partials by a tree, this one in cell order)."""
+    from fealpy_amd import _native as nat
+    from fealpy_amd.fem.pattern import pattern_of
+    if dim == 2:
+        mesh = fa.TriangleMesh.from_box(nx=7, ny=5, device=DEV)
+    else:
+        mesh = fa.TetrahedronMesh.from_box(nx=4, ny=3, nz=5, device=DEV)
+    space = fa.LagrangeFESpace(mesh, 1)
+    pat = pattern_of(space)
+    inc, pos = pat.incidence_maps()
+    adj = pat.adj.cpu().numpy()
+    c2d = pat.cell2dof.cpu().numpy()
+    crow, col = pat.indptr.cpu().numpy(), pat.indices.cpu().numpy()
+    aptr = pat.adj_ptr.cpu().numpy()
+    inc_h, pos_h = inc.cpu().numpy(), pos.cpu().numpy().view(np.uint32)
+    ld = pat.ldof
+    for r in range(pat.gdof):
+        cols = list(col[crow[r]:crow[r + 1]])
+        for k in range(aptr[r], aptr[r + 1]):
+            c, i = divmod(int(adj[k]), ld)
+            assert inc_h[c * ld + i] == k
+            for j in range(ld):
+                assert (int(pos_h[k]) >> (8 * j)) & 255 == cols.index(c2d[c, j])
+    nc = mesh.number_of_cells()
+    g = np.random.default_rng(12)
+    q = 3
+    _, qw, phi, _ = space.tables(q)
+    nq = qw.shape[0]
+    kq = torch.as_tensor(np.exp(g.standard_normal((nc, nq))), device=DEV)
+    kc = torch.as_tensor(np.exp(g.standard_normal(nc)), device=DEV)
+    L = nat.lib()
+    cell = space.cell_to_dof()
+    for dt in (torch.float64, torch.float32):
+        two = fa.BilinearForm(space).add_integrator(fa.ScalarDiffusionIntegrator(kq, q=q),
+                                                    fa.ScalarMassIntegrator(kc, q=q)).assembly(dtype=dt).values
+        fused = torch.empty(pat.nnz, dtype=dt, device=DEV)
+        nat.check(L.fa_assemble_p1_matrix(nat.dtype_code(dt), dim, pat.gdof, nq, nat.ptr(mesh.node), nat.ptr(cell),
+                                          nat.ptr(qw), nat.ptr(phi), nat.COEF_CELLQ, 1.0, nat.ptr(kq), nat.COEF_CELL,
+                                          1.0, nat.ptr(kc), nat.ptr(pat.adj_ptr), nat.ptr(pat.adj),
+                                          nat.ptr(pat.indptr), nat.ptr(pat.indices), pat.max_row, nat.ptr(fused),
+                                          nat.stream()), "fa_assemble_p1_matrix")
+        assert torch.equal(two, fused), dt
+    F = fa.LinearForm(space).add_integrator(fa.ScalarSourceIntegrator(kc, q=q)).assembly()
+    Ff = torch.empty(pat.gdof, dtype=torch.float64, device=DEV)
+    nat.check(L.fa_assemble_p1_vector(nat.FA_F64, dim, pat.gdof, nq, nat.ptr(mesh.node), nat.ptr(cell), nat.ptr(qw),
+                                      nat.ptr(phi), nat.COEF_CELL, 1.0, nat.ptr(kc), nat.ptr(pat.adj_ptr),
+                                      nat.ptr(pat.adj), nat.ptr(Ff), nat.stream()), "fa_assemble_p1_vector")
+    assert float((F - Ff).abs().max()) <= 1e-14 * float(Ff.abs().max())
