@@ -312,6 +312,12 @@ def p1_fusable_ops(it):
     return None
 
 
+# Meshes of at least this many cells assemble P1 in two passes (per cell, then
+# per row over the incidence slots); smaller ones use the single fused kernel,
+# whose one launch and no slot buffer cost less when the mesh is small.
+TWO_PASS_MIN_CELLS = 65536
+
+
 def p1_matrix_values(space, ops, pat, dtype):
     """CSR values of one fusable group on the space's pattern (P1)."""
     L = nat.lib()
@@ -324,7 +330,7 @@ def p1_matrix_values(space, ops, pat, dtype):
         spec[op] = coef_spec(it.coef, space, q)
     (dk, ds, dc), (mk, ms, mc) = spec['diff'], spec['mass']
     values = torch.empty(pat.nnz, dtype=dtype, device=space.device)
-    maps = pat.incidence_maps()
+    maps = pat.incidence_maps() if pat.nc >= TWO_PASS_MIN_CELLS else None
     if maps is not None:         # pass 1 per cell into incidence slots, pass 2 streams each row's slots
         rows = torch.empty(pat.nc * pat.ldof * pat.ldof, dtype=torch.float64, device=space.device)
         nat.check(L.fa_p1_incidence_rows(space.TD, pat.nc, qw.shape[0], nat.ptr(mesh.node_rows16()), nat.ptr(cell),
@@ -355,7 +361,7 @@ def p1_vector_values(space, it, pat, dtype):
     if it.source is None:
         kind, scale, src = nat.COEF_CONST, 1.0, None
     out = torch.empty(pat.gdof, dtype=dtype, device=space.device)
-    maps = pat.incidence_maps()
+    maps = pat.incidence_maps() if pat.nc >= TWO_PASS_MIN_CELLS else None
     if maps is not None:         # one value per incidence slot, then an ordered sum per row
         vals = torch.empty(pat.nc * pat.ldof, dtype=torch.float64, device=space.device)
         nat.check(L.fa_p1_incidence_vectors(space.TD, pat.nc, qw.shape[0], nat.ptr(mesh.node_rows16()),

@@ -266,14 +266,16 @@ def test_fused_p1_matches_element_gather(dim):
 
 
 @pytest.mark.parametrize("dim", [2, 3])
-def test_incidence_two_pass_p1(dim):
+def test_incidence_two_pass_p1(dim, monkeypatch):
     """Two-pass P1 assembly in incidence order: the maps equal a NumPy
     construction; the CSR values equal the fused single kernel bit for bit
     (FP64 and FP32 output, diffusion + mass with per-point / per-cell
     coefficients); the load vector equals the fused kernel to rounding (that
     kernel adds lane partials by a tree, this one in cell order)."""
     from fealpy_amd import _native as nat
+    from fealpy_amd.fem import integrators
     from fealpy_amd.fem.pattern import pattern_of
+    monkeypatch.setattr(integrators, "TWO_PASS_MIN_CELLS", 0)   # small mesh, two-pass path
     if dim == 2:
         mesh = fa.TriangleMesh.from_box(nx=7, ny=5, device=DEV)
     else:
