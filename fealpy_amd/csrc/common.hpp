@@ -76,6 +76,18 @@ __device__ inline T wave_sum(T v) {
 }
 
 // inclusive scan over a whole wavefront
+// XCD-aware workgroup order.  The dispatcher deals consecutive workgroups to
+// the 8 XCDs in turn, each with its own L2; renumbering workgroup b of a grid
+// of G (G a multiple of 8) as (b % 8) * (G / 8) + b / 8 gives every XCD one
+// contiguous range of rows, so the neighbouring rows whose x entries a
+// sparse row gathers sit in the same L2.  Any other grid keeps its order.
+__device__ inline unsigned xcd_block() {
+    const unsigned G = gridDim.x, b = blockIdx.x;
+    return (G & 7u) ? b : (b & 7u) * (G >> 3) + (b >> 3);
+}
+
+inline unsigned xcd_grid(unsigned g) { return (g + 7u) & ~7u; }
+
 __device__ inline int wave_inclusive_scan(int v) {
     const int lane = threadIdx.x & (WAVE - 1);
 #pragma unroll

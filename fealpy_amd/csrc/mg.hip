@@ -200,7 +200,8 @@ __global__ void __launch_bounds__(256) k_cheb(int64_t n, const int32_t* __restri
                                               DotOut dot = DotOut{}) {
     if (done && *done) return;
     double acc = 0.0;
-    for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
+    const int64_t b0 = DOT ? blockIdx.x : xcd_block();
+    for (int64_t row = b0 * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
          row += (int64_t)gridDim.x * blockDim.x) {
         if (FROM_ZERO) {
             if (row < n) {
@@ -234,7 +235,7 @@ __global__ void __launch_bounds__(256) k_cheb_z(int64_t n, const int32_t* __rest
                                                 T* __restrict__ xout, T* __restrict__ d, T c0, T c1, T c2,
                                                 const int* __restrict__ done) {
     if (done && *done) return;
-    for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
+    for (int64_t row = xcd_block() * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
          row += (int64_t)gridDim.x * blockDim.x) {
         const T ax = sell_row_dot_f<T, ROW_BATCH>(sp, sc, sv, row, [=](int32_t j) { return c0 * dinv[j] * b[j]; });
         if (row < n) {
@@ -260,7 +261,8 @@ __global__ void __launch_bounds__(256) k_cheb16(int64_t n, const int32_t* __rest
                                                const int* __restrict__ done, DotOut dot = DotOut{}) {
     if (done && *done) return;
     double acc = 0.0;
-    for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
+    const int64_t b0 = DOT ? blockIdx.x : xcd_block();
+    for (int64_t row = b0 * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
          row += (int64_t)gridDim.x * blockDim.x) {
         const float ax = sell16_row_dot(sp, so, sv, row, n, xin);
         if (row < n) {
@@ -280,7 +282,7 @@ __global__ void __launch_bounds__(256) k_cheb16_z(int64_t n, const int32_t* __re
                                                  float* __restrict__ xout, float* __restrict__ d, float c0, float c1,
                                                  float c2, const int* __restrict__ done) {
     if (done && *done) return;
-    for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
+    for (int64_t row = xcd_block() * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
          row += (int64_t)gridDim.x * blockDim.x) {
         const float ax = sell16_row_dot_f(sp, so, sv, row, n, [=](int64_t j) { return c0 * dinv[j] * b[j]; });
         if (row < n) {
@@ -298,7 +300,7 @@ __global__ void __launch_bounds__(256) k_residual16(int64_t n, const int32_t* __
                                                    const float* __restrict__ x, float* __restrict__ r,
                                                    const int* __restrict__ done) {
     if (done && *done) return;
-    for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
+    for (int64_t row = xcd_block() * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
          row += (int64_t)gridDim.x * blockDim.x) {
         const float ax = sell16_row_dot(sp, so, sv, row, n, x);
         if (row < n) r[row] = b[row] - diag[row] * ax;
@@ -311,7 +313,7 @@ __global__ void __launch_bounds__(256) k_residual(int64_t n, const int32_t* __re
                                                  const T* __restrict__ b, const T* __restrict__ x, T* __restrict__ r,
                                                  const int* __restrict__ done) {
     if (done && *done) return;
-    for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
+    for (int64_t row = xcd_block() * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
          row += (int64_t)gridDim.x * blockDim.x) {
         const T ax = sell_row_dot<T, T>(sp, sc, sv, row, x);
         if (row < n) r[row] = b[row] - ax;
@@ -325,7 +327,7 @@ __global__ void __launch_bounds__(256) k_sell_apply(int64_t n, const int32_t* __
                                                    const T* __restrict__ x, T* __restrict__ y,
                                                    const int* __restrict__ done) {
     if (done && *done) return;
-    for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
+    for (int64_t row = xcd_block() * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
          row += (int64_t)gridDim.x * blockDim.x) {
         const T v = sell_row_dot<T, T>(sp, sc, sv, row, x);
         if (row < n) y[row] = ADD ? y[row] + v : v;
@@ -843,7 +845,7 @@ struct Cycle {
     hipStream_t s;
     const int* done;
 
-    static unsigned grid(int64_t n) { return grid_for(((n + 63) / 64) * 64, 256, 1 << 16); }
+    static unsigned grid(int64_t n) { return xcd_grid(grid_for(((n + 63) / 64) * 64, 256, 1 << 16)); }
 
     // compressed operator present (FP32 cycles only)
     static bool compressed(const fa_mg_level_t& L) {
