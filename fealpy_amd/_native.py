@@ -62,6 +62,7 @@ _SIGNATURES = {
     "fa_dirichlet_count": (ctypes.c_int, [_i64, _p, _p, _p, _p, _pi64, _p, _sz, _p]),
     "fa_dirichlet_fill": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p, _p, _p, _p, _p]),
     "fa_dirichlet_rhs": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p, _p, _p, _p, _p]),
+    "fa_dirichlet_apply": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
     "fa_spmv": (ctypes.c_int, [ctypes.c_int, _i64, _i64, _p, _p, _p, _p, _p, _p]),
     "fa_reduce_work_bytes": (_sz, []),
     "fa_dot": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p, _sz, _p]),

@@ -117,6 +117,48 @@ __global__ void __launch_bounds__(256) k_bc_rhs(int64_t n, const int32_t* __rest
     }
 }
 
+// k_bc_fill (values only) and k_bc_rhs in one pass over A: the same lane
+// order for both, so the filtered values and f - A g equal the two separate
+// kernels' bit for bit
+template <class T>
+__global__ void __launch_bounds__(256) k_bc_apply(int64_t n, const int32_t* __restrict__ indptr,
+                                                 const int32_t* __restrict__ indices, const T* __restrict__ vals,
+                                                 const uint8_t* __restrict__ bd, const int32_t* __restrict__ nptr,
+                                                 T* __restrict__ nval, const T* __restrict__ uh,
+                                                 const T* __restrict__ f, T* __restrict__ out) {
+    const int lane = threadIdx.x % BCG;
+    for (int64_t r = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / BCG; r < n;
+         r += (int64_t)gridDim.x * blockDim.x / BCG) {
+        const int32_t o0 = nptr[r];
+        if (bd[r]) {
+            if (lane == 0) {
+                nval[o0] = T(1);
+                out[r] = uh[r];
+            }
+            continue;
+        }
+        int32_t base = o0;
+        T acc = T(0);
+        const int32_t k1 = indptr[r + 1];
+        for (int32_t k0 = indptr[r]; k0 < k1; k0 += BCG) {
+            const int32_t k = k0 + lane;
+            bool keep = false;
+            T v = T(0);
+            if (k < k1) {
+                const int32_t c = indices[k];
+                v = vals[k];
+                acc += v * uh[c];
+                keep = !bd[c];
+            }
+            const int inc = group_inclusive_scan<BCG>(keep ? 1 : 0);
+            if (keep) nval[base + inc - 1] = v;
+            base += __shfl(inc, BCG - 1, BCG);
+        }
+        acc = group_sum<BCG>(acc);
+        if (lane == 0) out[r] = f[r] - acc;
+    }
+}
+
 // ---- vector kernels --------------------------------------------------------------
 template <class T>
 __global__ void __launch_bounds__(RED_BLOCK) k_dot(int64_t n, const T* __restrict__ x, const T* __restrict__ y,
@@ -223,6 +265,24 @@ extern "C" int fa_dirichlet_rhs(int dtype, int64_t n, const int32_t* indptr, con
                            static_cast<const float*>(values), is_bd, static_cast<const float*>(uh),
                            static_cast<const float*>(f), static_cast<float*>(f_out));
     FA_LAUNCH("k_bc_rhs");
+    return FA_OK;
+}
+
+extern "C" int fa_dirichlet_apply(int dtype, int64_t n, const int32_t* indptr, const int32_t* indices,
+                                  const void* values, const uint8_t* is_bd, const int32_t* new_indptr,
+                                  void* new_values, const void* uh, const void* f, void* f_out, fa_stream_t stream) {
+    FA_REQUIRE(dtype == FA_F64 || dtype == FA_F32, "fa_dirichlet_apply: bad dtype %d", dtype);
+    hipStream_t s = S(stream);
+    const unsigned grid = grid_for(n * BCG, 256, 1 << 16);
+    if (dtype == FA_F64)
+        hipLaunchKernelGGL(k_bc_apply<double>, dim3(grid), dim3(256), 0, s, n, indptr, indices,
+                           static_cast<const double*>(values), is_bd, new_indptr, static_cast<double*>(new_values),
+                           static_cast<const double*>(uh), static_cast<const double*>(f), static_cast<double*>(f_out));
+    else
+        hipLaunchKernelGGL(k_bc_apply<float>, dim3(grid), dim3(256), 0, s, n, indptr, indices,
+                           static_cast<const float*>(values), is_bd, new_indptr, static_cast<float*>(new_values),
+                           static_cast<const float*>(uh), static_cast<const float*>(f), static_cast<float*>(f_out));
+    FA_LAUNCH("k_bc_apply");
     return FA_OK;
 }
 

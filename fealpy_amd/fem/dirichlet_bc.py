@@ -60,10 +60,35 @@ class DirichletBC:
 
     def apply(self, A, f, uh=None, gd=None, *, check=True):
         """-> (A with boundary rows/columns removed and a unit boundary
-        diagonal, f - A g with f[bd] = g[bd])."""
-        f = self.apply_vector(f, A, uh, gd, check=check)
-        A = self.apply_matrix(A, check=check)
-        return A, f
+        diagonal, f - A g with f[bd] = g[bd]); one pass over A
+        (fa_dirichlet_apply), the results of apply_vector + apply_matrix."""
+        A0 = self.check_matrix(A) if check else A
+        f = self.check_vector(f) if check else f
+        if f.ndim != 1:
+            return self.apply_matrix(A0, check=False), self.apply_vector(f, A0, uh, gd, check=False)
+        gd = self.gd if gd is None else gd
+        if gd is None:
+            raise RuntimeError("The boundary condition is None.")
+        was_coo = isinstance(A0, COOTensor)
+        Ac = self._csr(A0)
+        g = self._boundary_values(f, uh, gd, Ac.dtype)
+        f = f.to(Ac.dtype).contiguous()
+        new_ptr, new_idx = self._filtered_pattern(Ac)
+        new_val = torch.empty(new_idx.shape[0], dtype=Ac.dtype, device=Ac.device)
+        out = torch.empty_like(f)
+        nat.check(nat.lib().fa_dirichlet_apply(nat.dtype_code(Ac.dtype), Ac.shape[0], nat.ptr(Ac.crow),
+                                               nat.ptr(Ac.col), nat.ptr(Ac.values), nat.ptr(self._bd_u8),
+                                               nat.ptr(new_ptr), nat.ptr(new_val), nat.ptr(g), nat.ptr(f),
+                                               nat.ptr(out), nat.stream()), "fa_dirichlet_apply")
+        A2 = CSRTensor(new_ptr, new_idx, new_val, Ac.shape)
+        return (A2.tocoo() if was_coo else A2), out
+
+    def _boundary_values(self, f, uh, gd, dtype):
+        if uh is None:
+            uh = torch.zeros_like(f)
+        uh, _ = self.space.boundary_interpolate(gd=gd, uh=uh, threshold=self.threshold, method=self.method)
+        g = uh.array if hasattr(uh, 'array') else uh
+        return g.to(dtype).contiguous()
 
     def _csr(self, A):
         return A.coalesce().tocsr() if isinstance(A, COOTensor) else A
@@ -118,12 +143,8 @@ class DirichletBC:
         gd = self.gd if gd is None else gd
         if gd is None:
             raise RuntimeError("The boundary condition is None.")
-        if uh is None:
-            uh = torch.zeros_like(f)
-        uh, _ = self.space.boundary_interpolate(gd=gd, uh=uh, threshold=self.threshold, method=self.method)
-        g = uh.array if hasattr(uh, 'array') else uh
         A = self._csr(A)
-        g = g.to(A.dtype).contiguous()
+        g = self._boundary_values(f, uh, gd, A.dtype)
         f = f.to(A.dtype).contiguous()
         out = torch.empty_like(f)
         nat.check(nat.lib().fa_dirichlet_rhs(nat.dtype_code(A.dtype), A.shape[0], nat.ptr(A.crow), nat.ptr(A.col),

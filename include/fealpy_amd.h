@@ -245,6 +245,12 @@ int fa_dirichlet_fill(int dtype, int64_t n, const int32_t* indptr, const int32_t
                       fa_stream_t stream);
 int fa_dirichlet_rhs(int dtype, int64_t n, const int32_t* indptr, const int32_t* indices, const void* values,
                      const uint8_t* is_bd, const void* uh, const void* f, void* f_out, fa_stream_t stream);
+/* fa_dirichlet_fill (values only, on the pattern of fa_dirichlet_count) and
+ * fa_dirichlet_rhs in one pass over A: DirichletBC.apply (fealpy/fem/
+ * dirichlet_bc.py:101-121) with A read once; the same results bit for bit. */
+int fa_dirichlet_apply(int dtype, int64_t n, const int32_t* indptr, const int32_t* indices, const void* values,
+                       const uint8_t* is_bd, const int32_t* new_indptr, void* new_values, const void* uh,
+                       const void* f, void* f_out, fa_stream_t stream);
 
 /* ------------------------------------------------------------------ SpMV --
  * y = A x.  Replaces CSRTensor.matmul -> bm.csr_spmm -> scipy csr_matvec

@@ -321,3 +321,28 @@ def test_incidence_two_pass_p1(dim, monkeypatch):
                                       nat.ptr(phi), nat.COEF_CELL, 1.0, nat.ptr(kc), nat.ptr(pat.adj_ptr),
                                       nat.ptr(pat.adj), nat.ptr(Ff), nat.stream()), "fa_assemble_p1_vector")
     assert float((F - Ff).abs().max()) <= 1e-14 * float(Ff.abs().max())
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_dirichlet_apply_one_pass_equals_two(dim):
+    """DirichletBC.apply (one pass over A) equals apply_vector + apply_matrix
+    bit for bit, FP64 and FP32, CSR and COO input."""
+    if dim == 2:
+        mesh = fa.TriangleMesh.from_box(nx=9, ny=6, device=DEV)
+        u = fa.cartesian(lambda p: torch.sin(np.pi * p[..., 0]) * p[..., 1] + 1.0)
+    else:
+        mesh = fa.TetrahedronMesh.from_box(nx=4, ny=5, nz=3, device=DEV)
+        u = fa.cartesian(lambda p: torch.cos(p[..., 0]) + p[..., 1] * p[..., 2])
+    space = fa.LagrangeFESpace(mesh, 2)
+    k = torch.as_tensor(np.exp(np.random.default_rng(4).standard_normal(mesh.number_of_cells())), device=DEV)
+    for dt in (torch.float64, torch.float32):
+        A = fa.BilinearForm(space).add_integrator(fa.ScalarDiffusionIntegrator(k, q=4)).assembly(dtype=dt)
+        F = fa.LinearForm(space).add_integrator(fa.ScalarSourceIntegrator(1.0, q=4)).assembly(dtype=dt)
+        bc = fa.DirichletBC(space, gd=u)
+        A1, F1 = bc.apply(A, F)
+        F2 = bc.apply_vector(F, A)
+        A2 = bc.apply_matrix(A)
+        assert torch.equal(A1.crow, A2.crow) and torch.equal(A1.col, A2.col)
+        assert torch.equal(A1.values, A2.values) and torch.equal(F1, F2), dt
+        Ac, Fc = bc.apply(A.tocoo(), F)
+        assert torch.equal(Ac.tocsr().values, A2.values) and torch.equal(Fc, F2)
