@@ -254,7 +254,7 @@ __global__ void k_bc_to_point(int64_t nc, int nq, const double* __restrict__ nod
                               const double* __restrict__ bcs, double* __restrict__ out) {
     extern __shared__ double stage[];
     const int per = nq * GD;
-    for (int64_t c0 = (int64_t)xcd_block() * blockDim.x; c0 < nc; c0 += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t c0 = (int64_t)blockIdx.x * blockDim.x; c0 < nc; c0 += (int64_t)gridDim.x * blockDim.x) {
         const int64_t c = c0 + threadIdx.x;
         if (c < nc) {
             double v[NV][GD];
@@ -759,7 +759,7 @@ extern "C" int fa_bc_to_point(int gd, int nv, int64_t nc, int nq, const double* 
     FA_REQUIRE(16 * per <= 64 * 1024, "fa_bc_to_point: %d points per cell exceed the staging capacity", nq);
     unsigned blk = 256;
     while (blk > 16 && blk * per > 48 * 1024) blk >>= 1;
-    const unsigned grid = xcd_grid(grid_for(nc, blk, 1 << 16));
+    const unsigned grid = grid_for(nc, blk, 1 << 16);
 #define FA_BP(GD, NV) \
     hipLaunchKernelGGL((k_bc_to_point<GD, NV>), dim3(grid), dim3(blk), blk * per, s, nc, nq, node, cell, bcs, out)
     if (gd == 2 && nv == 3) FA_BP(2, 3);
@@ -984,7 +984,7 @@ __global__ void __launch_bounds__(P1_BLOCK) k_p1_incidence_rows(int64_t nc, cons
     double* mS = dS + P1_BLOCK * ld;
     int32_t* vS = reinterpret_cast<int32_t*>(mS + P1_BLOCK * lm);
     int32_t* iS = vS + P1_BLOCK * NV;
-    for (int64_t c0 = static_cast<int64_t>(xcd_block()) * P1_BLOCK; c0 < nc; c0 += static_cast<int64_t>(gridDim.x) * P1_BLOCK) {
+    for (int64_t c0 = static_cast<int64_t>(blockIdx.x) * P1_BLOCK; c0 < nc; c0 += static_cast<int64_t>(gridDim.x) * P1_BLOCK) {
         if (ld) stage_doubles(dS, dc, c0, nc, ld);
         if (lm) stage_doubles(mS, mc, c0, nc, lm);
         stage_ints(vS, cell, c0, nc, NV);
@@ -1027,7 +1027,7 @@ __global__ void __launch_bounds__(P1_BLOCK) k_p1_incidence_vectors(int64_t nc, c
     double* sS = stage;
     int32_t* vS = reinterpret_cast<int32_t*>(sS + P1_BLOCK * ls);
     int32_t* iS = vS + P1_BLOCK * NV;
-    for (int64_t c0 = static_cast<int64_t>(xcd_block()) * P1_BLOCK; c0 < nc; c0 += static_cast<int64_t>(gridDim.x) * P1_BLOCK) {
+    for (int64_t c0 = static_cast<int64_t>(blockIdx.x) * P1_BLOCK; c0 < nc; c0 += static_cast<int64_t>(gridDim.x) * P1_BLOCK) {
         if (ls) stage_doubles(sS, src, c0, nc, ls);
         stage_ints(vS, cell, c0, nc, NV);
         stage_ints(iS, inc, c0, nc, NV);
@@ -1197,7 +1197,7 @@ extern "C" int fa_p1_incidence_rows(int td, int64_t nc, int nq, const double* no
         set_error("fa_p1_incidence_rows: %d quadrature points per cell exceed the staging capacity", nq);
         return FA_ECAPACITY;
     }
-    const unsigned grid = xcd_grid(grid_for(nc, P1_BLOCK, 1 << 20));
+    const unsigned grid = grid_for(nc, P1_BLOCK, 1 << 20);
     if (td == 3)
         hipLaunchKernelGGL(k_p1_incidence_rows<3>, dim3(grid), dim3(P1_BLOCK), lds, S(stream), nc, node_wide, cell, nq,
                            qw, phi, diff_kind, diff_scale, diff_coef, mass_kind, mass_scale, mass_coef, inc, rows);
@@ -1244,7 +1244,7 @@ extern "C" int fa_p1_incidence_vectors(int td, int64_t nc, int nq, const double*
         set_error("fa_p1_incidence_vectors: %d quadrature points per cell exceed the staging capacity", nq);
         return FA_ECAPACITY;
     }
-    const unsigned grid = xcd_grid(grid_for(nc, P1_BLOCK, 1 << 20));
+    const unsigned grid = grid_for(nc, P1_BLOCK, 1 << 20);
     if (td == 3)
         hipLaunchKernelGGL(k_p1_incidence_vectors<3>, dim3(grid), dim3(P1_BLOCK), lds, S(stream), nc, node_wide, cell,
                            nq, qw, phi, src_kind, src_scale, src, inc, vals);

@@ -200,8 +200,7 @@ __global__ void __launch_bounds__(256) k_cheb(int64_t n, const int32_t* __restri
                                               DotOut dot = DotOut{}) {
     if (done && *done) return;
     double acc = 0.0;
-    const int64_t b0 = DOT ? blockIdx.x : xcd_block();
-    for (int64_t row = b0 * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
+    for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
          row += (int64_t)gridDim.x * blockDim.x) {
         if (FROM_ZERO) {
             if (row < n) {
@@ -235,7 +234,7 @@ __global__ void __launch_bounds__(256) k_cheb_z(int64_t n, const int32_t* __rest
                                                 T* __restrict__ xout, T* __restrict__ d, T c0, T c1, T c2,
                                                 const int* __restrict__ done) {
     if (done && *done) return;
-    for (int64_t row = xcd_block() * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
+    for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
          row += (int64_t)gridDim.x * blockDim.x) {
         const T ax = sell_row_dot_f<T, ROW_BATCH>(sp, sc, sv, row, [=](int32_t j) { return c0 * dinv[j] * b[j]; });
         if (row < n) {
@@ -252,16 +251,17 @@ __global__ void __launch_bounds__(256) k_cheb_z(int64_t n, const int32_t* __rest
 //   cheb:     d = c1 d + c2 (dinv b - (D^-1 A) x_in); x_out = x_in + d
 //   cheb_z:   the first two steps from zero (x0 = c0 dinv b formed per column)
 //   residual: r = b - diag * ((D^-1 A) x)
-template <bool FIRST, bool DOT = false>
-__global__ void __launch_bounds__(256) k_cheb16(int64_t n, const int32_t* __restrict__ sp,
-                                               const int16_t* __restrict__ so, const _Float16* __restrict__ sv,
+// CS: the column source (int16 offsets, or the full SELL's int32 columns)
+template <bool FIRST, bool DOT, class CS>
+__global__ void __launch_bounds__(256) k_cheb16(int64_t n, const int32_t* __restrict__ sp, CS so,
+                                               const _Float16* __restrict__ sv,
                                                const float* __restrict__ dinv, const float* __restrict__ b,
                                                const float* __restrict__ xin, float* __restrict__ xout,
                                                float* __restrict__ d, float c1, float c2,
                                                const int* __restrict__ done, DotOut dot = DotOut{}) {
     if (done && *done) return;
     double acc = 0.0;
-    const int64_t b0 = DOT ? blockIdx.x : xcd_block();
+    const int64_t b0 = (!DOT && CS::xcd) ? xcd_block() : blockIdx.x;
     for (int64_t row = b0 * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
          row += (int64_t)gridDim.x * blockDim.x) {
         const float ax = sell16_row_dot(sp, so, sv, row, n, xin);
@@ -276,14 +276,15 @@ __global__ void __launch_bounds__(256) k_cheb16(int64_t n, const int32_t* __rest
     if (DOT) store_partial(block_sum(acc), dot.partials);
 }
 
-__global__ void __launch_bounds__(256) k_cheb16_z(int64_t n, const int32_t* __restrict__ sp,
-                                                 const int16_t* __restrict__ so, const _Float16* __restrict__ sv,
+template <class CS>
+__global__ void __launch_bounds__(256) k_cheb16_z(int64_t n, const int32_t* __restrict__ sp, CS so,
+                                                 const _Float16* __restrict__ sv,
                                                  const float* __restrict__ dinv, const float* __restrict__ b,
                                                  float* __restrict__ xout, float* __restrict__ d, float c0, float c1,
                                                  float c2, const int* __restrict__ done) {
     if (done && *done) return;
-    for (int64_t row = xcd_block() * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
-         row += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t row = (CS::xcd ? xcd_block() : blockIdx.x) * (int64_t)blockDim.x + threadIdx.x;
+         row < ((n + 63) & ~63LL); row += (int64_t)gridDim.x * blockDim.x) {
         const float ax = sell16_row_dot_f(sp, so, sv, row, n, [=](int64_t j) { return c0 * dinv[j] * b[j]; });
         if (row < n) {
             const float x0 = c0 * dinv[row] * b[row];
@@ -294,14 +295,15 @@ __global__ void __launch_bounds__(256) k_cheb16_z(int64_t n, const int32_t* __re
     }
 }
 
-__global__ void __launch_bounds__(256) k_residual16(int64_t n, const int32_t* __restrict__ sp,
-                                                   const int16_t* __restrict__ so, const _Float16* __restrict__ sv,
+template <class CS>
+__global__ void __launch_bounds__(256) k_residual16(int64_t n, const int32_t* __restrict__ sp, CS so,
+                                                   const _Float16* __restrict__ sv,
                                                    const float* __restrict__ diag, const float* __restrict__ b,
                                                    const float* __restrict__ x, float* __restrict__ r,
                                                    const int* __restrict__ done) {
     if (done && *done) return;
-    for (int64_t row = xcd_block() * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
-         row += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t row = (CS::xcd ? xcd_block() : blockIdx.x) * (int64_t)blockDim.x + threadIdx.x;
+         row < ((n + 63) & ~63LL); row += (int64_t)gridDim.x * blockDim.x) {
         const float ax = sell16_row_dot(sp, so, sv, row, n, x);
         if (row < n) r[row] = b[row] - diag[row] * ax;
     }
@@ -313,7 +315,7 @@ __global__ void __launch_bounds__(256) k_residual(int64_t n, const int32_t* __re
                                                  const T* __restrict__ b, const T* __restrict__ x, T* __restrict__ r,
                                                  const int* __restrict__ done) {
     if (done && *done) return;
-    for (int64_t row = xcd_block() * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
+    for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
          row += (int64_t)gridDim.x * blockDim.x) {
         const T ax = sell_row_dot<T, T>(sp, sc, sv, row, x);
         if (row < n) r[row] = b[row] - ax;
@@ -327,7 +329,7 @@ __global__ void __launch_bounds__(256) k_sell_apply(int64_t n, const int32_t* __
                                                    const T* __restrict__ x, T* __restrict__ y,
                                                    const int* __restrict__ done) {
     if (done && *done) return;
-    for (int64_t row = xcd_block() * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
+    for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
          row += (int64_t)gridDim.x * blockDim.x) {
         const T v = sell_row_dot<T, T>(sp, sc, sv, row, x);
         if (row < n) y[row] = ADD ? y[row] + v : v;
@@ -852,6 +854,14 @@ struct Cycle {
         return std::is_same<T, float>::value && L.Ah.vals != nullptr && L.diag != nullptr;
     }
 
+    // column source of the FP16 operator: its int16 offsets, or (offs NULL:
+    // a column too far from its row) the level's int32 SELL columns
+    template <class F>
+    static void with_cols(const fa_mg_level_t& L, F f) {
+        if (L.Ah.offs) f(OffsetCols{L.Ah.offs});
+        else f(IndexCols{L.A.cols});
+    }
+
     int coarse(const fa_mg_level_t& L, const T* b, T* x) {
         const int n = static_cast<int>(L.A.n);
         if (n <= COARSE_ONE_BLOCK)
@@ -900,11 +910,13 @@ struct Cycle {
                     const T d1 = static_cast<T>(rho_new * rho), d2 = static_cast<T>(2.0 * rho_new / delta);
                     T* out = (degree == 2 && last_out) ? last_out : nxt;
                     if (cmp)
-                        hipLaunchKernelGGL(k_cheb16_z, dim3(g), dim3(256), 0, s, L.A.n, L.Ah.slice_ptr, L.Ah.offs,
-                                           static_cast<const _Float16*>(L.Ah.vals),
-                                           reinterpret_cast<const float*>(dinv), reinterpret_cast<const float*>(b),
-                                           reinterpret_cast<float*>(out), reinterpret_cast<float*>(d), (float)c0,
-                                           (float)d1, (float)d2, done);
+                        with_cols(L, [&](auto cs) {
+                            hipLaunchKernelGGL(k_cheb16_z<decltype(cs)>, dim3(g), dim3(256), 0, s, L.A.n,
+                                               L.Ah.slice_ptr, cs, static_cast<const _Float16*>(L.Ah.vals),
+                                               reinterpret_cast<const float*>(dinv), reinterpret_cast<const float*>(b),
+                                               reinterpret_cast<float*>(out), reinterpret_cast<float*>(d), (float)c0,
+                                               (float)d1, (float)d2, done);
+                        });
                     else
                         hipLaunchKernelGGL(k_cheb_z<T>, dim3(g), dim3(256), 0, s, L.A.n, L.A.slice_ptr, L.A.cols, sv,
                                            dinv, b, out, d, c0, d1, d2, done);
@@ -928,22 +940,27 @@ struct Cycle {
                 const float* fcur = reinterpret_cast<const float*>(cur);
                 float* fout = reinterpret_cast<float*>(out);
                 float* fd = reinterpret_cast<float*>(d);
-                if (with_dot) {
-                    if (k == 0)
-                        hipLaunchKernelGGL((k_cheb16<true, true>), dim3(gd), dim3(RED_BLOCK), 0, s, L.A.n,
-                                           L.Ah.slice_ptr, L.Ah.offs, hv, fdinv, fb, fcur, fout, fd, (float)c1,
-                                           (float)c2, done, *dot);
-                    else
-                        hipLaunchKernelGGL((k_cheb16<false, true>), dim3(gd), dim3(RED_BLOCK), 0, s, L.A.n,
-                                           L.Ah.slice_ptr, L.Ah.offs, hv, fdinv, fb, fcur, fout, fd, (float)c1,
-                                           (float)c2, done, *dot);
-                } else if (k == 0) {
-                    hipLaunchKernelGGL(k_cheb16<true>, dim3(g), dim3(256), 0, s, L.A.n, L.Ah.slice_ptr, L.Ah.offs, hv,
-                                       fdinv, fb, fcur, fout, fd, (float)c1, (float)c2, done, DotOut{});
-                } else {
-                    hipLaunchKernelGGL(k_cheb16<false>, dim3(g), dim3(256), 0, s, L.A.n, L.Ah.slice_ptr, L.Ah.offs,
-                                       hv, fdinv, fb, fcur, fout, fd, (float)c1, (float)c2, done, DotOut{});
-                }
+                with_cols(L, [&](auto cs) {
+                    using CS = decltype(cs);
+                    if (with_dot) {
+                        if (k == 0)
+                            hipLaunchKernelGGL((k_cheb16<true, true, CS>), dim3(gd), dim3(RED_BLOCK), 0, s, L.A.n,
+                                               L.Ah.slice_ptr, cs, hv, fdinv, fb, fcur, fout, fd, (float)c1,
+                                               (float)c2, done, *dot);
+                        else
+                            hipLaunchKernelGGL((k_cheb16<false, true, CS>), dim3(gd), dim3(RED_BLOCK), 0, s, L.A.n,
+                                               L.Ah.slice_ptr, cs, hv, fdinv, fb, fcur, fout, fd, (float)c1,
+                                               (float)c2, done, *dot);
+                    } else if (k == 0) {
+                        hipLaunchKernelGGL((k_cheb16<true, false, CS>), dim3(g), dim3(256), 0, s, L.A.n,
+                                           L.Ah.slice_ptr, cs, hv, fdinv, fb, fcur, fout, fd, (float)c1, (float)c2,
+                                           done, DotOut{});
+                    } else {
+                        hipLaunchKernelGGL((k_cheb16<false, false, CS>), dim3(g), dim3(256), 0, s, L.A.n,
+                                           L.Ah.slice_ptr, cs, hv, fdinv, fb, fcur, fout, fd, (float)c1, (float)c2,
+                                           done, DotOut{});
+                    }
+                });
             } else if (with_dot) {
                 if (k == 0)
                     hipLaunchKernelGGL((k_cheb<T, false, true, true>), dim3(gd), dim3(RED_BLOCK), 0, s, L.A.n,
@@ -1021,10 +1038,12 @@ struct Cycle {
         T* r = static_cast<T*>(L.r);
         const unsigned g = grid(L.A.n);
         if (compressed(L))
-            hipLaunchKernelGGL(k_residual16, dim3(g), dim3(256), 0, s, L.A.n, L.Ah.slice_ptr, L.Ah.offs,
-                               static_cast<const _Float16*>(L.Ah.vals), static_cast<const float*>(L.diag),
-                               reinterpret_cast<const float*>(b), reinterpret_cast<const float*>(cur),
-                               reinterpret_cast<float*>(r), done);
+            with_cols(L, [&](auto cs) {
+                hipLaunchKernelGGL(k_residual16<decltype(cs)>, dim3(g), dim3(256), 0, s, L.A.n, L.Ah.slice_ptr, cs,
+                                   static_cast<const _Float16*>(L.Ah.vals), static_cast<const float*>(L.diag),
+                                   reinterpret_cast<const float*>(b), reinterpret_cast<const float*>(cur),
+                                   reinterpret_cast<float*>(r), done);
+            });
         else
             hipLaunchKernelGGL(k_residual<T>, dim3(g), dim3(256), 0, s, L.A.n, L.A.slice_ptr, L.A.cols,
                                static_cast<const T*>(L.A.vals), b, cur, r, done);

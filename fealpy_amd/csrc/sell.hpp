@@ -89,31 +89,47 @@ __device__ inline T sell_row_dot_off(const int32_t* __restrict__ slice_ptr, cons
     return acc;
 }
 
-// row dot of the compressed format: sum_k vals[k] * xf(row + offs[k]), FP32
-// accumulation; rows of the last slice past n are not evaluated
-template <class XF>
-__device__ inline float sell16_row_dot_f(const int32_t* __restrict__ slice_ptr, const int16_t* __restrict__ offs,
+// Column sources of the FP16 smoother format: int16 offsets from the row
+// (4 bytes an entry with the FP16 value) or the full SELL's int32 columns
+// when a column lies too far from its row (6 bytes an entry).
+// xcd: whether kernels over this format order workgroups by XCD (measured
+// faster for the short-row operators that fit int16 offsets, slower for the
+// long-row operators that need int32 columns)
+struct OffsetCols {
+    static constexpr bool xcd = true;
+    const int16_t* __restrict__ o;
+    __device__ int64_t operator()(int64_t p, int64_t row) const { return row + o[p]; }
+};
+
+struct IndexCols {
+    static constexpr bool xcd = false;
+    const int32_t* __restrict__ c;
+    __device__ int64_t operator()(int64_t p, int64_t) const { return c[p]; }
+};
+
+// row dot of the FP16 format: sum_k vals[k] * xf(col_k), FP32 accumulation;
+// rows of the last slice past n are not evaluated
+template <class CS, class XF>
+__device__ inline float sell16_row_dot_f(const int32_t* __restrict__ slice_ptr, CS cs,
                                          const _Float16* __restrict__ vals, int64_t row, int64_t n, XF xf) {
     if (row >= n) return 0.f;
     const int64_t s = row >> 6;
     const int i = static_cast<int>(row & 63);
-    const int32_t p0 = slice_ptr[s];
-    const int w = (slice_ptr[s + 1] - p0) >> 6;
-    const int16_t* o = offs + p0 + i;
-    const _Float16* v = vals + p0 + i;
+    const int32_t p0 = slice_ptr[s] + i;
+    const int w = (slice_ptr[s + 1] - slice_ptr[s]) >> 6;
     float acc = 0.f;
     for (int k0 = 0; k0 < w; k0 += ROW_BATCH) {
-        int oo[ROW_BATCH];
+        int64_t cc[ROW_BATCH];
         float vv[ROW_BATCH];
 #pragma unroll
         for (int u = 0; u < ROW_BATCH; ++u) {
-            const int k = min(k0 + u, w - 1);
-            oo[u] = o[64 * k];
-            vv[u] = static_cast<float>(v[64 * k]);
+            const int64_t p = p0 + 64 * min(k0 + u, w - 1);
+            cc[u] = cs(p, row);
+            vv[u] = static_cast<float>(vals[p]);
         }
         float xx[ROW_BATCH];
 #pragma unroll
-        for (int u = 0; u < ROW_BATCH; ++u) xx[u] = xf(row + oo[u]);
+        for (int u = 0; u < ROW_BATCH; ++u) xx[u] = xf(cc[u]);
 #pragma unroll
         for (int u = 0; u < ROW_BATCH; ++u)
             if (k0 + u < w) acc += vv[u] * xx[u];
@@ -121,10 +137,10 @@ __device__ inline float sell16_row_dot_f(const int32_t* __restrict__ slice_ptr, 
     return acc;
 }
 
-__device__ inline float sell16_row_dot(const int32_t* __restrict__ slice_ptr, const int16_t* __restrict__ offs,
-                                       const _Float16* __restrict__ vals, int64_t row, int64_t n,
-                                       const float* __restrict__ x) {
-    return sell16_row_dot_f(slice_ptr, offs, vals, row, n, [=](int64_t j) { return x[j]; });
+template <class CS>
+__device__ inline float sell16_row_dot(const int32_t* __restrict__ slice_ptr, CS cs, const _Float16* __restrict__ vals,
+                                       int64_t row, int64_t n, const float* __restrict__ x) {
+    return sell16_row_dot_f(slice_ptr, cs, vals, row, n, [=](int64_t j) { return x[j]; });
 }
 
 }  // namespace fa

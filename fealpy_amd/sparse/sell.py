@@ -67,29 +67,37 @@ class SellMatrix:
         self.h_offs = self.h_vals = None
 
     def set_compressed(self, values, work):
-        """Compressed FP16 D^-1 A with int16 column offsets (fa_sell16_fill)
-        from FP64 CSR values; False when a column is too far from its row."""
-        if self.entries == 0 or getattr(self, "_h_band_ok", True) is False:
+        """Compressed FP16 D^-1 A on these slices (fa_sell16_fill).  With int16
+        column offsets when every column lies within 32767 of its row; else
+        the FP16 values use this matrix's int32 columns (`h_offs` None) and
+        fa_smoother_setup writes them.  False only for an empty matrix."""
+        if self.entries == 0:
             return False
-        checked = self.h_offs is not None
+        if self.h_vals is None:
+            self.h_vals = torch.zeros(self.entries, dtype=torch.float16, device=values.device)
+        band = getattr(self, "_h_band_ok", None)
+        if band is False:
+            return True
+        checked = band is True
         if not checked:
             self.h_offs = torch.zeros(self.entries, dtype=torch.int16, device=values.device)
-            self.h_vals = torch.zeros(self.entries, dtype=torch.float16, device=values.device)
         st = nat.lib().fa_sell16_fill(self.shape[0], nat.ptr(self.crow), nat.ptr(self.col),
                                       nat.ptr(values.contiguous()), nat.ptr(self.slice_ptr), nat.ptr(self.h_offs),
                                       nat.ptr(self.h_vals), None if checked else nat.ptr(work),
                                       0 if checked else work.numel(), nat.stream())
         if st == nat.FA_ECAPACITY:
-            self.h_offs = self.h_vals = None
+            self.h_offs = None
             self._h_band_ok = False
-            return False
+            return True
         nat.check(st, "fa_sell16_fill")
+        self._h_band_ok = True
         return True
 
     def desc16(self):
         if self.h_vals is None:
             return FaSell16(0, None, None, None)
-        return FaSell16(self.shape[0], self.slice_ptr.data_ptr(), self.h_offs.data_ptr(), self.h_vals.data_ptr())
+        offs = self.h_offs.data_ptr() if self.h_offs is not None else None
+        return FaSell16(self.shape[0], self.slice_ptr.data_ptr(), offs, self.h_vals.data_ptr())
 
     def value_buffer(self, dtype, device):
         """The SELL value array of `dtype` (allocated on first use)."""

@@ -287,7 +287,9 @@ typedef struct {
 /* Compressed SELL-64 for the FP32 smoother: the row-scaled operator
  * D^-1 A (entries in [-1, 1]) in FP16 with int16 column offsets from the row
  * (same slices as the full SELL; padding: offset 0, value 0).  4 bytes per
- * entry instead of 8. */
+ * entry instead of 8.  offs = NULL: a column lies more than 32767 from its
+ * row, and the FP16 values use the level's int32 SELL columns (6 bytes an
+ * entry). */
 typedef struct {
     int64_t n;
     const int32_t* slice_ptr;
