@@ -8,10 +8,12 @@ the preconditioner of CG.  Designed for the GPU:
   (interpolation of P1 into Pp, exact; the p-transfer of
   lagrange_fe_space.py:237-258); vector spaces apply every transfer per
   component;
-* geometric levels are the ``from_box`` meshes with every cell count halved; the Kuhn /
-  diagonal-split triangulations are nested under halving, so P1 prolongation
-  is closed form: a fine node with grid index g takes 1/2 of the coarse
-  nodes floor(g/2) and ceil(g/2) (one node with weight 1 when g is even);
+* geometric levels are the ``from_box`` meshes with every cell count halved
+  (odd counts rounded up); P1 prolongation evaluates the coarse P1 function
+  at the fine nodes in closed form (Freudenthal weights of the Kuhn /
+  diagonal-split cells).  Even counts give nested grids, where a fine node
+  with grid index g takes 1/2 of the coarse nodes floor(g/2) and ceil(g/2)
+  (one node with weight 1 when g is even);
 * coarse operators are Galerkin products R A P formed on the GPU
   (``fa_rap_*``: symbolic once, numeric per setup, fixed summation order);
 * Dirichlet rows stay decoupled: P has no rows for fine Dirichlet nodes and
@@ -68,24 +70,55 @@ def _grid_index(dims, ids):
     return torch.stack(out[::-1], dim=-1)
 
 
+def _coarse_dims(dims):
+    """Cell counts of the next coarser grid: every count >= 2 halved, rounded up."""
+    return [(n + 1) // 2 if n >= 2 else n for n in dims]
+
+
 def _grid_prolongation(dims_f, device):
-    """Scalar P1 prolongation coarse -> fine for nested grids as (rows, cols,
-    vals): a fine node at grid index g takes coarse nodes floor(g/2) and
-    ceil(g/2) per axis with weight 1/2 each (1 when g is even)."""
-    dims_c = [n // 2 for n in dims_f]
+    """Scalar P1 prolongation coarse -> fine as (rows, cols, vals): each fine
+    node takes the value of the coarse P1 function at its position.
+
+    Both grids triangulate the same box with the diagonal-split (2D) / Kuhn
+    (3D) cells of ``from_box``, whose simplices all contain the cube diagonal
+    from the low corner to the high corner.  A point with local coordinates
+    t in the coarse cube at corner c therefore lies in the simplex of the
+    path c -> c + e_s1 -> c + e_s1 + e_s2 -> ... ordered by t_s1 >= t_s2 >= ...,
+    and its barycentric weights are 1 - t_s1, t_s1 - t_s2, ..., t_sd (zero
+    weights dropped).  An even count n (coarse n/2, nested grids) gives
+    t in {0, 1/2}: the fine node takes coarse nodes floor(g/2) and ceil(g/2)
+    with 1/2 each.  An odd count n (coarse (n+1)/2, not nested) gives
+    fractional weights."""
+    dims_c = _coarse_dims(dims_f)
+    d = len(dims_f)
     nf = 1
     for n in dims_f:
         nf *= n + 1
     fine = torch.arange(nf, device=device)
     g = _grid_index(dims_f, fine)
-    lo, hi = g // 2, (g + 1) // 2
-    c_lo, c_hi = _node_ids(dims_c, lo), _node_ids(dims_c, hi)
-    same = c_lo == c_hi
-    rows = torch.cat([fine, fine[~same]])
-    cols = torch.cat([c_lo, c_hi[~same]])
-    vals = torch.cat([torch.where(same, 1.0, 0.5).to(torch.float64), torch.full((int((~same).sum()),), 0.5,
-                                                                             dtype=torch.float64, device=device)])
-    return rows, cols, vals
+    nn = torch.tensor(dims_f, dtype=torch.int64, device=device)
+    mm = torch.tensor(dims_c, dtype=torch.int64, device=device)
+    num = g * mm                                       # coarse coordinate = num / n, exactly
+    base = num // nn
+    t = (num % nn).to(torch.float64) / nn.to(torch.float64)
+    ts, order = torch.sort(t, dim=-1, descending=True, stable=True)
+    ext = torch.cat([torch.ones_like(ts[:, :1]), ts, torch.zeros_like(ts[:, :1])], dim=1)
+    w = ext[:, :-1] - ext[:, 1:]                       # (nf, d+1): 1 - t_s1, t_s1 - t_s2, ..., t_sd
+    step = torch.zeros(nf, d + 1, d, dtype=torch.int64, device=device)
+    step[:, 1:, :] = torch.nn.functional.one_hot(order, d).cumsum(dim=1)
+    vert = base[:, None, :] + step                     # (nf, d+1, d) path vertices
+    cols = _node_ids(dims_c, vert)
+    rows = fine[:, None].expand(nf, d + 1)
+    keep = w != 0
+    return rows[keep], cols[keep], w[keep]
+
+
+def _coarse_to_fine_node(dims_f, dims_c, g):
+    """Fine grid index nearest to coarse grid index g (2g on nested grids;
+    boundary nodes map to boundary nodes)."""
+    nn = torch.tensor(dims_f, dtype=torch.int64, device=g.device)
+    mm = torch.tensor(dims_c, dtype=torch.int64, device=g.device)
+    return (g * nn + mm // 2) // mm
 
 
 def _order_prolongation(space):
@@ -198,23 +231,23 @@ class GeometricMG:
             self.levels.append(nxt)
             lev, bd = nxt, bd_c
         lev.dims = dims
-        while all(n % 2 == 0 and n >= 2 for n in dims) and lev.n > coarse_max:
-            dims_c = [n // 2 for n in dims]
+        while any(n >= 2 for n in dims) and lev.n > coarse_max:
+            dims_c = _coarse_dims(dims)
             nf_s, nc_s = 1, 1
             for n in dims:
                 nf_s *= n + 1
             for n in dims_c:
                 nc_s *= n + 1
             g = _grid_index(dims_c, torch.arange(nc_s, device=dev))
-            bd_c = _select_nodes(bd, k, prio, nf_s, _node_ids(dims, 2 * g))
+            bd_c = _select_nodes(bd, k, prio, nf_s, _node_ids(dims, _coarse_to_fine_node(dims, dims_c, g)))
             self._link(lev, _block_prolongation(*_grid_prolongation(dims, dev), k, prio, nf_s, nc_s, bd, bd_c))
             nxt = _Level()
             nxt.dims, nxt.n, nxt.bd = dims_c, k * nc_s, bd_c
             self.levels.append(nxt)
             lev, dims, bd = nxt, dims_c, bd_c
         # the coarsest level is solved exactly (dense inverse) up to 2048 rows;
-        # above that (an odd grid dimension stops the halving early) it is
-        # smoothed from zero with a Chebyshev polynomial of degree coarse_degree
+        # above that (a coarse_max above 2048) it is smoothed from zero with a
+        # Chebyshev polynomial of degree coarse_degree
         self.coarse_dense = self.levels[-1].n <= 2048
         self.coarse_degree = coarse_degree
         for lv in self.levels:

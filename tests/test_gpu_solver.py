@@ -319,6 +319,31 @@ def test_mg_pcg_2d():
     assert info["niter"] < 40
 
 
+@pytest.mark.parametrize("n,dim,p", [(45, 2, 1), (45, 2, 2), (27, 3, 1), (24, 3, 1)])
+def test_mg_pcg_odd_counts(n, dim, p):
+    """Odd cell counts coarsen to (n+1)/2 on non-nested grids (transfer =
+    coarse P1 function at the fine nodes) down to a dense coarsest level; the
+    iteration count stays that of nested hierarchies."""
+    mesh, space, bc, A, b = _system(n, dim=dim, p=p)
+    x_ref = fa.cg(A, b, atol=1e-30, rtol=1e-13, maxit=50000)
+    mg = GeometricMG(space, bc.is_boundary_dof, dtype=torch.float32)
+    assert mg.coarse_dense and mg.levels[-1].n <= 64
+    x, info = fa.cg(A, b, M=mg, atol=1e-30, rtol=1e-11, maxit=200, returninfo=True)
+    assert float(torch.linalg.norm(x - x_ref)) / float(torch.linalg.norm(x_ref)) < 1e-8
+    assert info["niter"] < 30, info
+
+
+def test_mg_polynomial_coarsest_level():
+    """A coarsest level above 2048 rows (coarse_max above that) is smoothed
+    from zero by a Chebyshev polynomial instead of inverted."""
+    mesh, space, bc, A, b = _system(63, dim=2)
+    x_ref = fa.cg(A, b, atol=1e-30, rtol=1e-13, maxit=50000)
+    mg = GeometricMG(space, bc.is_boundary_dof, dtype=torch.float32, coarse_max=4096)
+    assert not mg.coarse_dense and len(mg.levels) == 1
+    x, info = fa.cg(A, b, M=mg, atol=1e-30, rtol=1e-11, maxit=400, returninfo=True)
+    assert float(torch.linalg.norm(x - x_ref)) / float(torch.linalg.norm(x_ref)) < 1e-8
+
+
 def test_pcg_reproducible():
     mesh, space, bc, A, b = _system(16)
     mg = GeometricMG(space, bc.is_boundary_dof, dtype=torch.float32)

@@ -137,6 +137,35 @@ def test_order_prolongation_interpolates_linears(TD, p):
     assert float((sums - 1).abs().max()) < 1e-15
 
 
+@pytest.mark.parametrize("dims", [(4, 6), (5, 3), (7, 7), (4, 2, 6), (3, 5, 3), (5, 5, 5), (1, 4, 3)])
+def test_grid_prolongation_evaluates_the_coarse_function(dims):
+    """Grid transfer of the multigrid: every fine node takes the value of the
+    coarse P1 function at its position.  Checked by brute-force point location
+    (barycentric coordinates of each fine node in every coarse cell) on the
+    coarse from_box mesh with every count >= 2 halved, rounded up."""
+    from fealpy_amd.solver.mg import _coarse_dims, _grid_prolongation
+    dc = _coarse_dims(list(dims))
+    M = fa.TetrahedronMesh if len(dims) == 3 else fa.TriangleMesh
+    kw = dict(zip(("nx", "ny", "nz"), dims))
+    kc = dict(zip(("nx", "ny", "nz"), dc))
+    fine, coarse = M.from_box(**kw, device=CPU), M.from_box(**kc, device=CPU)
+    v = np.random.default_rng(7).standard_normal(coarse.number_of_nodes())
+    xc, cell = _np(coarse.node), _np(coarse.cell).astype(np.int64)
+    xf = _np(fine.node)
+    T = xc[cell[:, 1:]] - xc[cell[:, :1]]                          # (NC, d, d) edge vectors as rows
+    lam = np.einsum('cij,pcj->pci', np.linalg.inv(np.transpose(T, (0, 2, 1))), xf[:, None, :] - xc[cell[:, 0]][None])
+    lam = np.concatenate([1 - lam.sum(-1, keepdims=True), lam], axis=-1)   # (NP, NC, d+1)
+    inside = (lam > -1e-12).all(-1)
+    c = inside.argmax(1)
+    assert inside.any(1).all()
+    ref = (lam[np.arange(len(xf)), c] * v[cell[c]]).sum(-1)
+    rows, cols, vals = _grid_prolongation(list(dims), CPU)
+    got = np.zeros(len(xf))
+    np.add.at(got, _np(rows), _np(vals) * v[_np(cols)])
+    assert np.abs(got - ref).max() < 1e-13
+    assert (_np(vals) > 0).all() and len(np.unique(_np(rows) * 10**6 + _np(cols))) == len(rows)
+
+
 @pytest.mark.parametrize("prio", [False, True])
 def test_block_prolongation_layout(prio):
     from fealpy_amd.solver.mg import _block_prolongation
