@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-n", type=int, default=40)
     ap.add_argument("--requests", type=int, default=4096, help="serving: number of requests")
-    ap.add_argument("--slots", type=int, default=8, help="serving: concurrent request slots (streams)")
+    ap.add_argument("--slots", type=int, default=4, help="serving: concurrent request slots (streams)")
     ap.add_argument("--rate", type=float, default=0.0,
                     help="serving: Poisson arrival rate per second (0: 0.8 x the measured server throughput)")
     ap.add_argument("--workload", choices=("diffusion", "elasticity", "highorder", "serving"), default="diffusion",
@@ -388,7 +388,13 @@ def serving_main(args):
     saturated (every request queued at t=0) for solves/s, (3) the server under
     Poisson arrivals at `rate` for p50/p99 latency (arrival to result); the
     plain loop's latency at that rate follows from its measured service times
-    (a FIFO single server: start = max(arrival, previous done))."""
+    (a FIFO single server: start = max(arrival, previous done)).
+
+    HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues per process (4 by
+    default); streams beyond that share queues and serialise.  Unless the
+    environment sets it, the serving process asks for 2 x slots queues (the
+    slots' streams, the default stream and torch's side streams each get one)."""
+    os.environ.setdefault("GPU_MAX_HW_QUEUES", str(min(32, max(4, 2 * args.slots))))
     import torch
     from fealpy_amd.serving import SolveServer, make_requests
     n = args.requests
@@ -447,7 +453,8 @@ def serving_main(args):
         "dtype": "f64+f32", "data": "synthetic",
         "config": {"workload": f"{n} requests, TriangleMesh.from_box nx^2 with nx log-uniform in [16, 512], P1/P2, "
                                f"k=exp(N(0,1)) per cell, 8 Fourier modes, u=0 on the boundary",
-                   "slots": args.slots, "rtol": args.rtol, "distinct_sizes": len(sizes),
+                   "slots": args.slots, "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"]), "rtol": args.rtol,
+                   "distinct_sizes": len(sizes),
                    "mean_gdof": round(float(np.mean(gdofs)), 1), "max_gdof": int(max(gdofs))},
         "plain_loop": {"solves_per_s": round(n / t_plain, 2), **_pcts(lat_plain)},
         "server": {"solves_per_s": round(thr, 2), "speedup_vs_plain": round(thr * t_plain / n, 3),

@@ -4,6 +4,8 @@
 #include <climits>
 #include <cmath>
 
+#include <new>
+
 #include "common.hpp"
 #include "reduce.hpp"
 #include "sell.hpp"
@@ -1261,10 +1263,53 @@ __global__ void k_pcg_set(PcgState* st, double atol, double rtol, long long maxi
     st->niter = 0;
 }
 
+// A captured PCG iteration: the kernels of one iteration recorded on a private
+// capture stream (nothing runs there) into a graph, then replayed on the
+// caller's stream.  Every kernel of an iteration reads the convergence state
+// from device memory, so the replayed sequence is the eager one.
+struct PcgGraph {
+    hipStream_t cs = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+// record body(cs) into a graph and make pg->exec run it: an update of the
+// existing executable graph when the topology is unchanged, else a new one
+template <class F>
+static int pcg_capture(PcgGraph* pg, F body) {
+    FA_HIP(hipStreamBeginCapture(pg->cs, hipStreamCaptureModeThreadLocal));
+    const int e = body(pg->cs);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(pg->cs, &g);
+    if (e != FA_OK || ec != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (e != FA_OK) return e;
+        set_error("fa_pcg_graph: stream capture failed: %s", hipGetErrorString(ec));
+        return FA_EHIP;
+    }
+    if (pg->exec) {
+        hipGraphNode_t bad = nullptr;
+        hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+        if (hipGraphExecUpdate(pg->exec, g, &bad, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+            (void)hipGetLastError();
+            (void)hipGraphExecDestroy(pg->exec);
+            pg->exec = nullptr;
+        }
+    }
+    hipError_t ei = hipSuccess;
+    if (!pg->exec) ei = hipGraphInstantiate(&pg->exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        pg->exec = nullptr;
+        set_error("fa_pcg_graph: hipGraphInstantiate failed: %s", hipGetErrorString(ei));
+        return FA_EHIP;
+    }
+    return FA_OK;
+}
+
 template <class T, class U>
 static int pcg_run(const fa_sell_t* A, const fa_mg_t* M, const T* b, T* x, double atol, double rtol, int64_t maxit,
                    int check_every, int64_t* niter_host, double* res_host, void* work, size_t work_bytes,
-                   hipStream_t s) {
+                   hipStream_t s, PcgGraph* pg = nullptr) {
     const int64_t n = A->n;
     Arena ar(work, work_bytes);
     T* r = ar.take<T>(n);
@@ -1304,28 +1349,40 @@ static int pcg_run(const fa_sell_t* A, const fa_mg_t* M, const T* b, T* x, doubl
                            M->levels[0].Ah.slice_ptr == A->slice_ptr && M->levels[0].Ah.n == n)
                               ? M->levels[0].Ah.offs
                               : nullptr;
+    auto iteration = [&](hipStream_t q) -> int {
+        hipLaunchKernelGGL(k_pcg_spmv<T>, grid, block, 0, q, n, A->slice_ptr, A->cols, offs, sv, p, Ap, partials, st);
+        hipLaunchKernelGGL(k_pcg_finish<PCG_ALPHA>, one, block, 0, q, partials, np, st);
+        if (M) {
+            hipLaunchKernelGGL((k_pcg_update<T, U, true>), grid, block, 0, q, n, x, r, ru, p, Ap, partials, st);
+            int e = vcycle_any(M, ru, z, done, q, &rz_out);  // z = M r and the partials of r.z
+            if (e) return e;
+            hipLaunchKernelGGL(k_pcg_finish<PCG_CHECK>, one, block, 0, q, partials, np, st);
+            hipLaunchKernelGGL((k_pcg_pupdate<T, U>), grid, block, 0, q, n, p, z, st);
+        } else {
+            hipLaunchKernelGGL((k_pcg_update<T, T, false>), grid, block, 0, q, n, x, r, (T*)nullptr, p, Ap, partials,
+                               st);
+            hipLaunchKernelGGL(k_pcg_finish<PCG_CHECK>, one, block, 0, q, partials, np, st);
+            hipLaunchKernelGGL((k_pcg_pupdate<T, T>), grid, block, 0, q, n, p, r, st);
+        }
+        FA_LAUNCH("pcg iteration");
+        return FA_OK;
+    };
+    if (pg) {
+        int e = pcg_capture(pg, iteration);
+        if (e) return e;
+    }
     PcgState h{};
     int64_t it = 0;
     for (;;) {
         const int64_t chunk = std::min<int64_t>(check_every, cap - it);
         for (int64_t k = 0; k < chunk; ++k) {
-            hipLaunchKernelGGL(k_pcg_spmv<T>, grid, block, 0, s, n, A->slice_ptr, A->cols, offs, sv, p, Ap, partials,
-                               st);
-            hipLaunchKernelGGL(k_pcg_finish<PCG_ALPHA>, one, block, 0, s, partials, np, st);
-            if (M) {
-                hipLaunchKernelGGL((k_pcg_update<T, U, true>), grid, block, 0, s, n, x, r, ru, p, Ap, partials, st);
-                int e = vcycle_any(M, ru, z, done, s, &rz_out);  // z = M r and the partials of r.z
-                if (e) return e;
-                hipLaunchKernelGGL(k_pcg_finish<PCG_CHECK>, one, block, 0, s, partials, np, st);
-                hipLaunchKernelGGL((k_pcg_pupdate<T, U>), grid, block, 0, s, n, p, z, st);
+            if (pg) {
+                FA_HIP(hipGraphLaunch(pg->exec, s));
             } else {
-                hipLaunchKernelGGL((k_pcg_update<T, T, false>), grid, block, 0, s, n, x, r, (T*)nullptr, p, Ap,
-                                   partials, st);
-                hipLaunchKernelGGL(k_pcg_finish<PCG_CHECK>, one, block, 0, s, partials, np, st);
-                hipLaunchKernelGGL((k_pcg_pupdate<T, T>), grid, block, 0, s, n, p, r, st);
+                int e = iteration(s);
+                if (e) return e;
             }
         }
-        FA_LAUNCH("pcg iteration");
         it += chunk;
         FA_HIP(hipMemcpyAsync(&h, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
         FA_HIP(hipStreamSynchronize(s));
@@ -1576,9 +1633,9 @@ extern "C" size_t fa_pcg_work_bytes(int dtype, int64_t n) {
            align_up(sizeof(PcgState)) + 2048;
 }
 
-extern "C" int fa_pcg(int dtype, const fa_sell_t* A, const fa_mg_t* M, const void* b, void* x, double atol,
-                      double rtol, int64_t maxit, int32_t check_every, int64_t* niter_host, double* residual_host,
-                      void* work, size_t work_bytes, fa_stream_t stream) {
+static int pcg_dispatch(int dtype, const fa_sell_t* A, const fa_mg_t* M, const void* b, void* x, double atol,
+                        double rtol, int64_t maxit, int32_t check_every, int64_t* niter_host, double* residual_host,
+                        void* work, size_t work_bytes, hipStream_t s, PcgGraph* pg) {
     FA_REQUIRE(A && A->n > 0 && check_every > 0, "fa_pcg: bad arguments");
     if (M) {
         FA_REQUIRE(M->nlevels >= 1 && M->levels && (M->coarse_chol || M->coarse_degree >= 1),
@@ -1587,15 +1644,59 @@ extern "C" int fa_pcg(int dtype, const fa_sell_t* A, const fa_mg_t* M, const voi
         FA_REQUIRE(!M->coarse_chol || M->levels[M->nlevels - 1].A.n <= COARSE_MAX,
                    "fa_pcg: coarsest level too large for the dense inverse");
     }
-    hipStream_t s = S(stream);
     const int ud = M ? M->dtype : dtype;
 #define FA_PCG(T, U)                                                                                         \
     return pcg_run<T, U>(A, M, static_cast<const T*>(b), static_cast<T*>(x), atol, rtol, maxit, check_every, \
-                         niter_host, residual_host, work, work_bytes, s)
+                         niter_host, residual_host, work, work_bytes, s, pg)
     if (dtype == FA_F64 && ud == FA_F64) FA_PCG(double, double);
     if (dtype == FA_F64 && ud == FA_F32) FA_PCG(double, float);
     if (dtype == FA_F32 && ud == FA_F32) FA_PCG(float, float);
 #undef FA_PCG
     set_error("fa_pcg: unsupported dtype pair (%d, %d)", dtype, ud);
     return FA_EINVAL;
+}
+
+extern "C" int fa_pcg(int dtype, const fa_sell_t* A, const fa_mg_t* M, const void* b, void* x, double atol,
+                      double rtol, int64_t maxit, int32_t check_every, int64_t* niter_host, double* residual_host,
+                      void* work, size_t work_bytes, fa_stream_t stream) {
+    return pcg_dispatch(dtype, A, M, b, x, atol, rtol, maxit, check_every, niter_host, residual_host, work,
+                        work_bytes, S(stream), nullptr);
+}
+
+struct fa_pcg_graph_s {
+    PcgGraph g;
+};
+
+extern "C" int fa_pcg_graph_create(fa_pcg_graph_t** out) {
+    FA_REQUIRE(out, "fa_pcg_graph_create: out is NULL");
+    *out = nullptr;
+    fa_pcg_graph_t* h = new (std::nothrow) fa_pcg_graph_t();
+    if (!h) {
+        set_error("fa_pcg_graph_create: out of host memory");
+        return FA_ENOMEM;
+    }
+    const hipError_t e = hipStreamCreateWithFlags(&h->g.cs, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete h;
+        set_error("fa_pcg_graph_create: hipStreamCreateWithFlags failed: %s", hipGetErrorString(e));
+        return FA_EHIP;
+    }
+    *out = h;
+    return FA_OK;
+}
+
+extern "C" void fa_pcg_graph_destroy(fa_pcg_graph_t* graph) {
+    if (!graph) return;
+    if (graph->g.exec) (void)hipGraphExecDestroy(graph->g.exec);
+    if (graph->g.cs) (void)hipStreamDestroy(graph->g.cs);
+    delete graph;
+}
+
+extern "C" int fa_pcg_graph(int dtype, const fa_sell_t* A, const fa_mg_t* M, const void* b, void* x, double atol,
+                            double rtol, int64_t maxit, int32_t check_every, int64_t* niter_host,
+                            double* residual_host, void* work, size_t work_bytes, fa_pcg_graph_t* graph,
+                            fa_stream_t stream) {
+    FA_REQUIRE(graph && graph->g.cs, "fa_pcg_graph: no graph handle (fa_pcg_graph_create)");
+    return pcg_dispatch(dtype, A, M, b, x, atol, rtol, maxit, check_every, niter_host, residual_host, work,
+                        work_bytes, S(stream), &graph->g);
 }

@@ -16,6 +16,11 @@ runs alone or in the mix, so its result is bitwise the same.
 
 ``SolveServer.solve`` runs one request on the calling thread's current
 stream; a plain one-at-a-time loop is ``for r in reqs: server.solve(r)``.
+
+HIP maps a process's streams onto GPU_MAX_HW_QUEUES hardware queues (4 by
+default, read when HIP initialises); streams beyond that share queues and
+their kernels serialise, so a process serving with ``slots`` streams should
+set GPU_MAX_HW_QUEUES to at least slots + 2 before its first GPU call.
 """
 import math
 import queue
@@ -102,11 +107,11 @@ class _SizeContext:
 
 
 class SolveServer:
-    def __init__(self, *, slots=8, hbm_budget=64 << 30, rtol=1e-8, maxit=2000, q=3, mg_dtype=torch.float32,
-                 device=None):
+    def __init__(self, *, slots=4, hbm_budget=64 << 30, rtol=1e-8, maxit=2000, q=3, mg_dtype=torch.float32,
+                 graphs=False, device=None):
         self.device = torch.device("cuda") if device is None else torch.device(device)
         self.slots, self.budget, self.rtol, self.maxit, self.q = slots, int(hbm_budget), rtol, maxit, q
-        self.mg_kwargs = {"dtype": mg_dtype}
+        self.mg_kwargs = {"dtype": mg_dtype, "graphs": graphs}
         self._ctx = {}
         self._ctx_lock = threading.Lock()
         self._key_locks = {}

@@ -200,7 +200,7 @@ class GeometricMG:
     the same mesh; vector spaces coarsen every component alike."""
 
     def __init__(self, space, is_bd, *, dtype=torch.float32, degree=2, lower=0.1, upper=1.0, coarse_max=64,
-                 power_iters=12, coarse_degree=12, compress=True):
+                 power_iters=12, coarse_degree=12, compress=True, graphs=False):
         ss = getattr(space, 'scalar_space', space)
         k = int(getattr(space, 'dof_numel', 1))
         prio = bool(getattr(space, 'dof_priority', False))
@@ -213,6 +213,10 @@ class GeometricMG:
         # FP32 cycles smooth with FP16 D^-1 A and int16 column offsets where the band allows
         self.compress = bool(compress) and dtype == torch.float32
         self.power_iters = power_iters
+        # graphs: PCG iterations replayed from a HIP graph (fa_pcg_graph), for
+        # launch-bound solves; the handle is made on first use
+        self.graphs = bool(graphs)
+        self._graph = None
         self.device = dev
         self.levels = []
         bd = is_bd.to(torch.bool)
@@ -409,6 +413,12 @@ class GeometricMG:
         self._desc = FaMg(nat.dtype_code(self.dtype), n, arr, self.degree, self.lower, self.upper,
                           dense.data_ptr() if dense is not None else None, self.coarse_degree)
 
+    def __del__(self):
+        g = getattr(self, '_graph', None)
+        if g is not None and nat._lib is not None:
+            nat._lib.fa_pcg_graph_destroy(g)
+            self._graph = None
+
     # -- use ----------------------------------------------------------------------
     def __matmul__(self, r):
         """z = V(r): one V-cycle (r in the cycle dtype)."""
@@ -436,9 +446,17 @@ class GeometricMG:
         ws = nat.workspace(L.fa_pcg_work_bytes(nat.dtype_code(dt), b.shape[0]), b.device)
         niter, res = ctypes.c_int64(0), ctypes.c_double(0.0)
         d = S.desc(dt)
-        st = L.fa_pcg(nat.dtype_code(dt), ctypes.byref(d), ctypes.byref(self._desc), nat.ptr(b), nat.ptr(x),
-                      float(atol), float(rtol), int(maxit or 0), int(check_every), ctypes.byref(niter),
-                      ctypes.byref(res), nat.ptr(ws), ws.numel(), nat.stream())
+        args = (nat.dtype_code(dt), ctypes.byref(d), ctypes.byref(self._desc), nat.ptr(b), nat.ptr(x), float(atol),
+                float(rtol), int(maxit or 0), int(check_every), ctypes.byref(niter), ctypes.byref(res), nat.ptr(ws),
+                ws.numel())
+        if self.graphs:
+            if self._graph is None:
+                h = ctypes.c_void_p()
+                nat.check(L.fa_pcg_graph_create(ctypes.byref(h)), "fa_pcg_graph_create")
+                self._graph = h
+            st = L.fa_pcg_graph(*args, self._graph, nat.stream())
+        else:
+            st = L.fa_pcg(*args, nat.stream())
         if st != nat.FA_ENOCONV:
             nat.check(st, "fa_pcg")
         return x, {'residual': float(res.value), 'niter': int(niter.value)}

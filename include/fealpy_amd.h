@@ -444,6 +444,24 @@ int fa_pcg(int dtype, const fa_sell_t* A, const fa_mg_t* M, const void* b, void*
            int64_t maxit, int32_t check_every, int64_t* niter_host, double* residual_host,
            void* work, size_t work_bytes, fa_stream_t stream);
 
+/* fa_pcg with its iterations replayed from a HIP graph, for callers whose
+ * solves are bound by kernel-launch cost (small systems, many concurrent
+ * requests).  Same contract, arguments and results as fa_pcg (cg of
+ * fealpy/solver/cg.py:14-123); bitwise the same x, niter and residual.
+ * Each call records one PCG iteration (SpMV, the V-cycle, the scalar
+ * kernels) on the handle's private capture stream into a graph (nothing runs
+ * while recording), updates the handle's executable graph (or instantiates
+ * it when the kernel sequence changed), then launches it on `stream` once
+ * per iteration; the initial residual and first cycle run as in fa_pcg.
+ * A handle serves one caller thread at a time; it holds a HIP stream and an
+ * executable graph until fa_pcg_graph_destroy. */
+typedef struct fa_pcg_graph_s fa_pcg_graph_t;
+int fa_pcg_graph_create(fa_pcg_graph_t** out);
+void fa_pcg_graph_destroy(fa_pcg_graph_t* graph);
+int fa_pcg_graph(int dtype, const fa_sell_t* A, const fa_mg_t* M, const void* b, void* x, double atol, double rtol,
+                 int64_t maxit, int32_t check_every, int64_t* niter_host, double* residual_host,
+                 void* work, size_t work_bytes, fa_pcg_graph_t* graph, fa_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

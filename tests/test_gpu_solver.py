@@ -344,6 +344,35 @@ def test_mg_polynomial_coarsest_level():
     assert float(torch.linalg.norm(x - x_ref)) / float(torch.linalg.norm(x_ref)) < 1e-8
 
 
+@pytest.mark.parametrize("dim,dt", [(2, torch.float32), (3, torch.float32), (3, torch.float64)])
+def test_pcg_graph_replays_equal_eager(dim, dt):
+    """fa_pcg_graph (iterations replayed from a HIP graph) gives bitwise the
+    eager fa_pcg result: over several solves in a row, with eager work and
+    synchronisations between them, after a re-setup with new values (new
+    smoother bounds: the executable graph is updated), and with maxit not a
+    multiple of check_every."""
+    n = 48 if dim == 2 else 12
+    mesh, space, bc, A, b = _system(n, dim=dim)
+    mg_e = GeometricMG(space, bc.is_boundary_dof, dtype=dt)
+    mg_g = GeometricMG(space, bc.is_boundary_dof, dtype=dt, graphs=True)
+    x_e, info_e = fa.cg(A, b, M=mg_e, atol=1e-30, rtol=1e-10, maxit=200, returninfo=True)
+    for rep in range(3):
+        x_g, info_g = fa.cg(A, b, M=mg_g, atol=1e-30, rtol=1e-10, maxit=200, returninfo=True)
+        assert torch.equal(x_g, x_e), rep
+        assert info_g == info_e, (rep, info_g, info_e)
+        y = fa.cg(A, 2 * b, M=mg_e, atol=1e-30, rtol=1e-6, maxit=200)      # eager work between replays
+        torch.cuda.synchronize()
+        assert torch.isfinite(y).all()
+    k = _lognormal(mesh, seed=3)
+    _, _, _, A2, b2 = _system(n, k=k, dim=dim)
+    x_e2, info_e2 = fa.cg(A2, b2, M=mg_e, atol=1e-30, rtol=1e-10, maxit=300, returninfo=True)
+    x_g2, info_g2 = fa.cg(A2, b2, M=mg_g, atol=1e-30, rtol=1e-10, maxit=300, returninfo=True)
+    assert torch.equal(x_g2, x_e2) and info_g2 == info_e2
+    x_e3, info_e3 = fa.cg(A2, b2, M=mg_e, atol=1e-30, rtol=1e-14, maxit=7, returninfo=True)
+    x_g3, info_g3 = fa.cg(A2, b2, M=mg_g, atol=1e-30, rtol=1e-14, maxit=7, returninfo=True)
+    assert info_e3["niter"] == 7 and info_g3 == info_e3 and torch.equal(x_g3, x_e3)
+
+
 def test_pcg_reproducible():
     mesh, space, bc, A, b = _system(16)
     mg = GeometricMG(space, bc.is_boundary_dof, dtype=torch.float32)
