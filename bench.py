@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of k_sell_spmv<float> on the 128^3 operator: FETCH_SIZE
 # and WRITE_SIZE counter passes over `bench.py --spmv-only`, each scaled by the
 # same counter's reading for a 256 MiB device copy (profiles/sell_spmv_f32_128cube_counters.md)
-SPMV_F32_TRAFFIC_128 = 294.1e6
+SPMV_F32_TRAFFIC_128 = 267.3e6
 
 
 def parse():
