@@ -185,8 +185,18 @@ def _transpose(M):
     return _csr(cols[order], rows[order], M.values[order], M.shape[1], M.shape[0])
 
 
+def _identity(n, device):
+    idx = torch.arange(n, dtype=torch.int32, device=device)
+    crow = torch.arange(n + 1, dtype=torch.int32, device=device)
+    return CSRTensor(crow, idx, torch.ones(n, dtype=torch.float64, device=device), (n, n))
+
+
 def _max_row(M):
     return int((M.crow[1:] - M.crow[:-1]).max().item()) if M.shape[0] else 0
+
+
+# candidates per row the symbolic Galerkin kernels hold (fa_rap_count's largest LDS list)
+RAP_CANDIDATES = 8192
 
 
 class _Level:
@@ -272,49 +282,68 @@ class GeometricMG:
     # -- per-matrix setup -----------------------------------------------------
     def _symbolic(self, A):
         from ..sparse.sell import sell_of
-        L = nat.lib()
         self.levels[0].A = A
         self.levels[0].As = sell_of(A)
         for lv in self.levels:
             lv.compressed = None        # decided by the next setup (band check of the new pattern)
         for l in range(len(self.levels) - 1):
             f, c = self.levels[l], self.levels[l + 1]
-            cand = _max_row(f.R) * _max_row(f.A) * max(_max_row(f.P), 1)
-            cptr = torch.empty(c.n + 1, dtype=torch.int32, device=self.device)
-            ws = nat.workspace(L.fa_rap_work_bytes(c.n), self.device)
-            nnz, mrow = ctypes.c_int64(0), ctypes.c_int32(0)
-            nat.check(L.fa_rap_count(c.n, c.n, nat.ptr(f.R.crow), nat.ptr(f.R.col), nat.ptr(f.A.crow),
-                                     nat.ptr(f.A.col), nat.ptr(f.P.crow), nat.ptr(f.P.col), max(cand, 1),
-                                     nat.ptr(cptr), ctypes.byref(nnz), ctypes.byref(mrow), nat.ptr(ws), ws.numel(),
-                                     nat.stream()), "fa_rap_count")
-            cidx = torch.empty(max(int(nnz.value), 1), dtype=torch.int32, device=self.device)
-            nat.check(L.fa_rap_fill(c.n, nat.ptr(f.R.crow), nat.ptr(f.R.col), nat.ptr(f.A.crow), nat.ptr(f.A.col),
-                                    nat.ptr(f.P.crow), nat.ptr(f.P.col), max(cand, 1), nat.ptr(cptr), nat.ptr(cidx),
-                                    nat.stream()), "fa_rap_fill")
-            c.A = CSRTensor(cptr, cidx[:int(nnz.value)], torch.zeros(int(nnz.value), dtype=torch.float64,
-                                                                     device=self.device), (c.n, c.n))
-            c.max_row = int(mrow.value)
+            mr, ma, mp = _max_row(f.R), _max_row(f.A), max(_max_row(f.P), 1)
+            if mr * ma * mp <= RAP_CANDIDATES:
+                c.A, c.max_row = self._rap_pattern(f.R, f.A, f.P, c.n, mr * ma * mp)
+                f.stages = [(f.R, f.A, f.P, c.A, c.max_row, self._rap_terms(f.R, f.A, f.P, c.A, mr * ma * mp))]
+            else:
+                # R (A P) in two products: each row's candidate list stays short
+                # (P3 -> P1 in 3D: 975 against 34,125 for R A P at once)
+                i_f, i_c = _identity(f.n, self.device), _identity(c.n, self.device)
+                ap, ap_max = self._rap_pattern(i_f, f.A, f.P, c.n, ma * mp)
+                c.A, c.max_row = self._rap_pattern(f.R, ap, i_c, c.n, mr * _max_row(ap))
+                f.stages = [(i_f, f.A, f.P, ap, ap_max, self._rap_terms(i_f, f.A, f.P, ap, ma * mp)),
+                            (f.R, ap, i_c, c.A, c.max_row, self._rap_terms(f.R, ap, i_c, c.A, mr * _max_row(ap)))]
             c.As = SellMatrix(c.A.crow, c.A.col, c.A.shape)
-            f.terms = self._rap_terms(f, c, max(cand, 1))
         last = self.levels[-1]
         last.dense = (torch.zeros((last.n, last.n), dtype=torch.float64, device=self.device) if self.coarse_dense
                       else None)
 
-    def _rap_terms(self, f, c, cand):
+    def _rap_pattern(self, R, A, P, ncols, cand):
+        """Symbolic product R A P (fa_rap_count / fa_rap_fill) -> (CSR with
+        zero values, longest row).  A row with no candidates gets a unit
+        diagonal; in the first product of two (A P, rows of fine DOFs) that
+        only happens for Dirichlet rows, which R never reads."""
+        L = nat.lib()
+        nrows = R.shape[0]
+        cand = max(int(cand), 1)
+        if cand > RAP_CANDIDATES:
+            raise NotImplementedError(f"Galerkin product with {cand} candidates per row (at most {RAP_CANDIDATES})")
+        cptr = torch.empty(nrows + 1, dtype=torch.int32, device=self.device)
+        ws = nat.workspace(L.fa_rap_work_bytes(nrows), self.device)
+        nnz, mrow = ctypes.c_int64(0), ctypes.c_int32(0)
+        nat.check(L.fa_rap_count(nrows, ncols, nat.ptr(R.crow), nat.ptr(R.col), nat.ptr(A.crow), nat.ptr(A.col),
+                                 nat.ptr(P.crow), nat.ptr(P.col), cand, nat.ptr(cptr), ctypes.byref(nnz),
+                                 ctypes.byref(mrow), nat.ptr(ws), ws.numel(), nat.stream()), "fa_rap_count")
+        cidx = torch.empty(max(int(nnz.value), 1), dtype=torch.int32, device=self.device)
+        nat.check(L.fa_rap_fill(nrows, nat.ptr(R.crow), nat.ptr(R.col), nat.ptr(A.crow), nat.ptr(A.col),
+                                nat.ptr(P.crow), nat.ptr(P.col), cand, nat.ptr(cptr), nat.ptr(cidx), nat.stream()),
+                  "fa_rap_fill")
+        C = CSRTensor(cptr, cidx[:int(nnz.value)], torch.zeros(int(nnz.value), dtype=torch.float64,
+                                                              device=self.device), (nrows, ncols))
+        return C, int(mrow.value)
+
+    def _rap_terms(self, R, A, P, C, cand):
         """Term list of the numeric product (fa_rap_terms_*), or None when the
         transfers are empty or take more than 3 distinct values, or A is too
         large; the search kernel (fa_rap_values) then forms the product."""
         L = nat.lib()
-        vals = torch.unique(torch.cat([f.R.values, f.P.values])).cpu().tolist()
-        if not vals or len(vals) > 3 or f.A.nnz >= (1 << 28) or c.A.nnz == 0:
+        vals = torch.unique(torch.cat([R.values, P.values])).cpu().tolist()
+        if not vals or len(vals) > 3 or A.nnz >= (1 << 28) or C.nnz == 0:
             return None
         wset = (ctypes.c_double * len(vals))(*vals)
-        ne = c.A.nnz
+        ne = C.nnz
         tslice = torch.empty((ne + 63) // 64 + 1, dtype=torch.int32, device=self.device)
         ws = nat.workspace(L.fa_rap_terms_work_bytes(ne), self.device)
         nterms = ctypes.c_int64(0)
-        args = (c.n, nat.ptr(f.R.crow), nat.ptr(f.R.col), nat.ptr(f.R.values), nat.ptr(f.A.crow), nat.ptr(f.A.col),
-                nat.ptr(f.P.crow), nat.ptr(f.P.col), nat.ptr(f.P.values), cand, nat.ptr(c.A.crow), ne, wset,
+        args = (C.shape[0], nat.ptr(R.crow), nat.ptr(R.col), nat.ptr(R.values), nat.ptr(A.crow), nat.ptr(A.col),
+                nat.ptr(P.crow), nat.ptr(P.col), nat.ptr(P.values), max(int(cand), 1), nat.ptr(C.crow), ne, wset,
                 len(vals))
         nat.check(L.fa_rap_terms_count(*args, nat.ptr(tslice), ctypes.byref(nterms), nat.ptr(ws), ws.numel(),
                                        nat.stream()), "fa_rap_terms_count")
@@ -334,17 +363,18 @@ class GeometricMG:
         L = nat.lib()
         self.levels[0].A = A
         for l in range(len(self.levels) - 1):
-            f, c = self.levels[l], self.levels[l + 1]
-            if getattr(f, 'terms', None) is not None:
-                tslice, terms, wset, nw = f.terms
-                nat.check(L.fa_rap_values_terms(c.A.nnz, nat.ptr(tslice), nat.ptr(terms), wset, nw,
-                                                nat.ptr(f.A.values), nat.ptr(c.A.values), nat.stream()),
-                          "fa_rap_values_terms")
-                continue
-            nat.check(L.fa_rap_values(c.n, nat.ptr(f.R.crow), nat.ptr(f.R.col), nat.ptr(f.R.values),
-                                      nat.ptr(f.A.crow), nat.ptr(f.A.col), nat.ptr(f.A.values), nat.ptr(f.P.crow),
-                                      nat.ptr(f.P.col), nat.ptr(f.P.values), nat.ptr(c.A.crow), nat.ptr(c.A.col),
-                                      c.max_row, nat.ptr(c.A.values), nat.stream()), "fa_rap_values")
+            for R, Af, P, C, cmax, terms in self.levels[l].stages:
+                if terms is not None:
+                    tslice, tw, wset, nw = terms
+                    nat.check(L.fa_rap_values_terms(C.nnz, nat.ptr(tslice), nat.ptr(tw), wset, nw,
+                                                    nat.ptr(Af.values), nat.ptr(C.values), nat.stream()),
+                              "fa_rap_values_terms")
+                else:
+                    nat.check(L.fa_rap_values(C.shape[0], nat.ptr(R.crow), nat.ptr(R.col), nat.ptr(R.values),
+                                              nat.ptr(Af.crow), nat.ptr(Af.col), nat.ptr(Af.values),
+                                              nat.ptr(P.crow), nat.ptr(P.col), nat.ptr(P.values), nat.ptr(C.crow),
+                                              nat.ptr(C.col), cmax, nat.ptr(C.values), nat.stream()),
+                              "fa_rap_values")
         code = nat.dtype_code(self.dtype)
         ws = nat.workspace(L.fa_reduce_work_bytes(), self.device)
         bounds = torch.empty(len(self.levels), dtype=torch.float64, device=self.device)

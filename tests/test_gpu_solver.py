@@ -102,7 +102,7 @@ def test_rap_term_list(p):
     _, _, _, A, _ = _system(n, k=k, p=p)
     mg = GeometricMG(space, bc.is_boundary_dof, dtype=torch.float64, coarse_max=1)
     mg.setup(A)
-    assert mg.levels[0].terms is not None
+    assert mg.levels[0].stages[0][-1] is not None      # term list
     for l in range(len(mg.levels) - 1):
         f, c = mg.levels[l], mg.levels[l + 1]
         ref = (f.R.to_scipy() @ f.A.to_scipy() @ f.P.to_scipy()).tocsr()
@@ -190,10 +190,42 @@ def test_rap_term_list_elasticity():
     mg = GeometricMG(ts, bc.is_boundary_dof, dtype=torch.float64, coarse_max=1)
     mg.setup(A)
     f, c = mg.levels[0], mg.levels[1]
-    assert f.terms is not None
+    assert len(f.stages) == 1 and f.stages[0][-1] is not None
     ref = (f.R.to_scipy() @ f.A.to_scipy() @ f.P.to_scipy()).tocsr()
     ref = ref + sp.diags((np.diff(ref.indptr) == 0).astype(float))
     assert abs(c.A.to_scipy() - ref).max() < 1e-12 * abs(ref).max()
+
+
+@pytest.mark.parametrize("p,n", [(3, 4), (4, 3)])
+def test_mg_pcg_p3_p4_two_products(p, n):
+    """P3/P4 -> P1 in 3D: R A P at once would hold 34,125 / 258,300
+    candidates per row, so the hierarchy forms A P, then R (A P).  Every level
+    equals SciPy's triple product (unit diagonal on empty rows); the
+    MG-preconditioned CG reaches the tight solution."""
+    import scipy.sparse as sp
+    mesh = fa.TetrahedronMesh.from_box(nx=n, ny=n, nz=n, device=DEV)
+    space = fa.LagrangeFESpace(mesh, p)
+    k = _lognormal(mesh, seed=13)
+    A = fa.BilinearForm(space).add_integrator(fa.ScalarDiffusionIntegrator(k, q=p + 2)).assembly()
+    b = fa.LinearForm(space).add_integrator(fa.ScalarSourceIntegrator(1.0, q=p + 2)).assembly()
+    bc = fa.DirichletBC(space, gd=0.0)
+    A, b = bc.apply(A, b)
+    mg = GeometricMG(space, bc.is_boundary_dof, dtype=torch.float64)
+    mg.setup(A)
+    assert len(mg.levels[0].stages) == 2
+    for l in range(len(mg.levels) - 1):
+        f, c = mg.levels[l], mg.levels[l + 1]
+        ref = (f.R.to_scipy() @ f.A.to_scipy() @ f.P.to_scipy()).tocsr()
+        ref = ref + sp.diags((np.diff(ref.indptr) == 0).astype(float))
+        got = c.A.to_scipy()
+        assert abs(got - ref).max() < 1e-12 * abs(ref).max(), l
+    x_ref = fa.cg(A, b, atol=1e-30, rtol=1e-13, maxit=50000)
+    for dt in (torch.float64, torch.float32):
+        m = GeometricMG(space, bc.is_boundary_dof, dtype=dt)
+        x, info = fa.cg(A, b, M=m, atol=1e-30, rtol=1e-11, maxit=300, returninfo=True)
+        err = float(torch.linalg.norm(x - x_ref)) / float(torch.linalg.norm(x_ref))
+        assert err < 1e-8, (dt, err, info)
+        assert info["niter"] < 100, info
 
 
 def test_sell_padding_columns_in_range():
