@@ -1,11 +1,13 @@
 """Conjugate gradients (fealpy/solver/cg.py:14-123).
 
-``cg(A, b)`` with a device CSR matrix runs entirely on the GPU (``fa_cg``:
+``cg(A, b)`` with a device CSR matrix runs entirely on the GPU (``fa_pcg``:
 fused SpMV + dot, update + dot, direction update; the host looks at the
 convergence flag every ``check_every`` iterations).  Any other operator with
 ``__matmul__`` (and any preconditioner ``M``) goes through the same iteration
-written with the HIP vector kernels.  Stopping rule, return values and the
-``info`` dict follow the reference.
+written with the HIP vector kernels.  A 2-D ``b`` holds one right-hand side
+per column (per row with ``batch_first``); the columns iterate in lockstep as
+in the reference.  Stopping rule, return values and the ``info`` dict follow
+the reference.
 """
 import torch
 
@@ -24,7 +26,13 @@ def cg(A, b, x0=None, M=None, *, batch_first=False, atol=1e-12, rtol=1e-8, maxit
     if x0 is not None and x0.shape != b.shape:
         raise ValueError("x0 and b must have the same shape")
     if b.ndim == 2:
-        raise NotImplementedError("multiple right-hand sides: solve them one by one or use fealpy_amd.serving")
+        bt = b.contiguous() if batch_first else b.t().contiguous()
+        x0t = None if x0 is None else (x0.contiguous() if batch_first else x0.t().contiguous())
+        if M is not None and hasattr(M, 'ensure_setup') and isinstance(A, CSRTensor):
+            M.ensure_setup(A)
+        xt, info = _cg_batched(A, bt, x0t, M, atol, rtol, maxit)
+        x = xt if batch_first else xt.t().contiguous()
+        return (x, info) if returninfo else x
     if M is not None and hasattr(M, 'pcg') and isinstance(A, CSRTensor):
         x, info = M.pcg(A, b, x0, atol=atol, rtol=rtol, maxit=maxit)
     elif M is None and isinstance(A, CSRTensor):
@@ -105,3 +113,43 @@ def _cg_generic(A, b, x0, M, atol, rtol, maxit):
         V.axpby(1.0, z, rz_new / rz, p)
         rz = rz_new
     return x, info
+
+
+def _cg_batched(A, bt, x0t, M, atol, rtol, maxit):
+    """Right-hand sides bt[k] (rows of a (B, n) array) in lockstep, as
+    fealpy/solver/cg.py:69-123 runs the columns of b: alpha and beta per
+    column, one stopping test on sqrt(sum_k r_k . z_k) against atol and
+    rtol * ||b||_F.  Every product and vector operation is a HIP kernel on one
+    contiguous row."""
+    info = {'residual': 0.0, 'niter': 0}
+    nb = bt.shape[0]
+    V = _Vec(bt)
+    b_norm = sum(V.dot(bt[k], bt[k]) for k in range(nb)) ** 0.5
+    if b_norm < 1e-15:
+        return torch.zeros_like(bt), info
+    X = torch.zeros_like(bt) if x0t is None else x0t.clone()
+    R = bt.clone()
+    for k in range(nb):
+        V.axpby(-1.0, (A @ X[k]).contiguous(), 1.0, R[k])
+    pre = (lambda v: (M @ v).contiguous()) if M is not None else (lambda v: v)
+    Z = [pre(R[k]) for k in range(nb)]
+    P = torch.stack(Z)
+    rz = [V.dot(R[k], Z[k]) for k in range(nb)]
+    n_iter = 0
+    while True:
+        for k in range(nb):
+            Ap = (A @ P[k]).contiguous()
+            alpha = rz[k] / V.dot(P[k], Ap)
+            V.axpby(alpha, P[k], 1.0, X[k])
+            V.axpby(-alpha, Ap, 1.0, R[k])
+        Z = [pre(R[k]) for k in range(nb)]
+        rz_new = [V.dot(R[k], Z[k]) for k in range(nb)]
+        r_norm = max(sum(rz_new), 0.0) ** 0.5
+        n_iter += 1
+        info['residual'], info['niter'] = r_norm, n_iter
+        if r_norm < atol or r_norm < rtol * b_norm or (maxit is not None and n_iter >= maxit):
+            break
+        for k in range(nb):
+            V.axpby(1.0, Z[k], rz_new[k] / rz[k], P[k])
+        rz = rz_new
+    return X, info

@@ -451,20 +451,29 @@ class GeometricMG:
 
     # -- use ----------------------------------------------------------------------
     def __matmul__(self, r):
-        """z = V(r): one V-cycle (r in the cycle dtype)."""
-        z = torch.empty_like(r)
-        nat.check(nat.lib().fa_mg_vcycle(ctypes.byref(self._desc), nat.ptr(r.contiguous()), nat.ptr(z),
-                                         nat.stream()), "fa_mg_vcycle")
-        return z
+        """z = V(r): one V-cycle of the hierarchy set up last (the cycle runs in
+        self.dtype; z has r's dtype)."""
+        if self._desc is None:
+            raise RuntimeError("GeometricMG: call setup(A) before applying the cycle")
+        rc = r.to(self.dtype).contiguous()
+        z = torch.empty_like(rc)
+        nat.check(nat.lib().fa_mg_vcycle(ctypes.byref(self._desc), nat.ptr(rc), nat.ptr(z), nat.stream()),
+                  "fa_mg_vcycle")
+        return z if z.dtype == r.dtype else z.to(r.dtype)
 
-    def pcg(self, A, b, x0=None, *, atol=1e-12, rtol=1e-8, maxit=10000, check_every=4):
-        """Preconditioned CG on the device: A's dtype outside, V-cycle in self.dtype."""
-        from ..sparse.sell import sell_of
+    def ensure_setup(self, A):
+        """Set up for A unless the last setup was for this matrix and its values
+        have not changed since."""
         fresh = self._setup_A is A and getattr(self, '_setup_version', None) == A.values._version
         if not fresh:
             self.setup(A if A.dtype == torch.float64 else A.astype(torch.float64))
             self._setup_A = A
             self._setup_version = A.values._version
+
+    def pcg(self, A, b, x0=None, *, atol=1e-12, rtol=1e-8, maxit=10000, check_every=4):
+        """Preconditioned CG on the device: A's dtype outside, V-cycle in self.dtype."""
+        from ..sparse.sell import sell_of
+        self.ensure_setup(A)
         L = nat.lib()
         dt = A.dtype
         if dt == torch.float64 and self.levels[0].A is A:

@@ -269,26 +269,34 @@ def dirichlet_apply(A, F, is_bd, ipoints, gd):
 
 
 def cg(A, b, x0=None, M=None, atol=1e-12, rtol=1e-8, maxit=10000):
-    """fealpy/solver/cg.py:14-123 (single vector).  Returns (x, info)."""
+    """fealpy/solver/cg.py:14-123.  b is one vector (n,) or a batch (n, B):
+    the columns iterate in lockstep with their own alpha and beta (sums over
+    axis 0, cg.py:85-110) and stop together on sqrt(sum_k r_k . z_k) against
+    atol and rtol * ||b|| (Frobenius norm for a batch, cg.py:98-113).
+    Returns (x, info)."""
     info = {'residual': 0.0, 'niter': 0}
     if np.linalg.norm(b) < 1e-15:
         return np.zeros_like(b), info
     matvec = A.matvec if hasattr(A, 'matvec') else (lambda v: A @ v)
+    prec = (lambda v: v) if M is None else M
+
+    def apply(f, v):
+        return f(v) if v.ndim == 1 else np.stack([f(v[:, k]) for k in range(v.shape[1])], axis=1)
     x = np.zeros_like(b) if x0 is None else x0.copy()
-    r = b - matvec(x)
-    z = M(r) if M is not None else r
+    r = b - apply(matvec, x)
+    z = apply(prec, r)
     p = z
     b_norm = np.linalg.norm(b)
-    rTr = np.sum(r * z)
+    rTr = np.sum(r * z, axis=0)
     n_iter = 0
     while True:
-        Ap = matvec(p)
-        alpha = rTr / np.sum(p * Ap)
+        Ap = apply(matvec, p)
+        alpha = rTr / np.sum(p * Ap, axis=0)
         x = x + alpha * p
         r_new = r - alpha * Ap
-        z_new = M(r_new) if M is not None else r_new
-        rTr_new = np.sum(r_new * z_new)
-        r_norm = np.sqrt(rTr_new)
+        z_new = apply(prec, r_new)
+        rTr_new = np.sum(r_new * z_new, axis=0)
+        r_norm = np.sqrt(np.sum(rTr_new))
         n_iter += 1
         info['residual'], info['niter'] = r_norm, n_iter
         if r_norm < atol or r_norm < rtol * b_norm or (maxit is not None and n_iter >= maxit):

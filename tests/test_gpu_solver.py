@@ -405,6 +405,47 @@ def test_pcg_graph_replays_equal_eager(dim, dt):
     assert info_e3["niter"] == 7 and info_g3 == info_e3 and torch.equal(x_g3, x_e3)
 
 
+def test_cg_batch_matches_oracle():
+    """Right-hand sides as columns (fealpy/solver/cg.py:69-123): lockstep
+    iterations, alpha and beta per column, one stopping test on the summed
+    r.z -- the oracle's iteration count and iterates; batch_first gives the
+    transposed result."""
+    from oracle import fem as ofem
+    mesh, space, bc, A, b = _system(12, dim=2)
+    Bm = torch.as_tensor(np.random.default_rng(6).standard_normal((A.shape[0], 3)) * np.array([1.0, 10.0, 0.1]),
+                         device=DEV)
+    X, info = fa.cg(A, Bm, atol=0.0, rtol=1e-8, maxit=500, returninfo=True)
+    Xo, info_o = ofem.cg(A.to_scipy().tocsr(), Bm.cpu().numpy(), atol=0.0, rtol=1e-8, maxit=500)
+    assert info["niter"] == info_o["niter"]
+    np.testing.assert_allclose(X.cpu().numpy(), Xo, rtol=0, atol=1e-9 * np.abs(Xo).max())
+    Xt = fa.cg(A, Bm.t().contiguous(), batch_first=True, atol=0.0, rtol=1e-8, maxit=500)
+    assert torch.equal(Xt, X.t())
+
+
+@pytest.mark.parametrize("dt", [torch.float64, torch.float32])
+def test_cg_batch_with_multigrid_and_generic_operator(dt):
+    """A batch through the multigrid preconditioner (either cycle precision)
+    and a 1-D solve through a generic operator (anything with @) with the
+    multigrid cycle: every column reaches its tight single solve."""
+    mesh, space, bc, A, b = _system(16, dim=2)
+    mg = GeometricMG(space, bc.is_boundary_dof, dtype=dt)
+    Bm = torch.stack([b, 2 * b + 1e-3 * torch.arange(b.shape[0], device=DEV, dtype=torch.float64)], dim=1)
+    Bm[bc.is_boundary_dof] = b[bc.is_boundary_dof, None]
+    X, info = fa.cg(A, Bm, M=mg, atol=1e-30, rtol=1e-11, maxit=200, returninfo=True)
+    assert info["niter"] < 40, info
+    for k in range(2):
+        ref = fa.cg(A, Bm[:, k].contiguous(), atol=1e-30, rtol=1e-13, maxit=50000)
+        assert float(torch.linalg.norm(X[:, k] - ref)) < 1e-8 * float(torch.linalg.norm(ref)), k
+
+    class Op:
+        def __matmul__(self, v):
+            return A @ v
+    x, info = fa.cg(Op(), b, M=mg, atol=1e-30, rtol=1e-11, maxit=200, returninfo=True)
+    ref = fa.cg(A, b, atol=1e-30, rtol=1e-13, maxit=50000)
+    assert float(torch.linalg.norm(x - ref)) < 1e-8 * float(torch.linalg.norm(ref))
+    assert info["niter"] < 40, info
+
+
 def test_pcg_reproducible():
     mesh, space, bc, A, b = _system(16)
     mg = GeometricMG(space, bc.is_boundary_dof, dtype=torch.float32)

@@ -169,6 +169,54 @@ def test_cg_matches_direct_solve():
     np.testing.assert_allclose(x, spla.spsolve(S.tocsc(), b), rtol=0, atol=1e-11)
 
 
+def _spd_matrix(n=6):
+    mesh = tri_from_box((0, 1, 0, 1), n, n)
+    c2d = mesh.cell_to_ipoint(1)
+    return fem.assemble_matrix(c2d, [fem.diffusion_element(mesh, 1, None, 2) + fem.mass_element(mesh, 1, 1.0, 2)],
+                               mesh.number_of_global_ipoints(1))
+
+
+def test_cg_batch_columns_reach_direct_solves():
+    """Right-hand sides as columns (cg.py:85-113, lockstep, alpha and beta per
+    column): every column reaches scipy's direct solution."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    A = _spd_matrix()
+    B = np.random.default_rng(2).standard_normal((A.shape[0], 3)) * np.array([1.0, 10.0, 0.1])
+    X, info = fem.cg(A, B, atol=1e-15, rtol=1e-15, maxit=1000)
+    S = sp.csr_matrix((A.data, A.indices, A.indptr), shape=A.shape).tocsc()
+    for k in range(3):
+        np.testing.assert_allclose(X[:, k], spla.spsolve(S, B[:, k]), rtol=0, atol=1e-11 * np.abs(B[:, k]).max())
+
+
+def test_cg_batch_of_identical_columns_equals_single_solve():
+    """Identical columns have the single solve's alpha and beta, and the
+    relative test sqrt(sum_k r.z) < rtol ||B||_F is the single test scaled by
+    sqrt(B) on both sides: same iterates (up to the summation order of the
+    column dots), same iteration count."""
+    A = _spd_matrix()
+    b = np.random.default_rng(3).standard_normal(A.shape[0])
+    x1, i1 = fem.cg(A, b, atol=0.0, rtol=1e-6)
+    X, info = fem.cg(A, np.stack([b] * 3, axis=1), atol=0.0, rtol=1e-6)
+    assert info['niter'] == i1['niter']
+    for k in range(3):
+        np.testing.assert_allclose(X[:, k], x1, rtol=0, atol=1e-8 * np.abs(x1).max())
+
+
+def test_cg_batch_absolute_test_sums_the_columns():
+    """The absolute test sums r.z over the columns: 64 identical columns stop
+    where a single solve with atol / 8 stops (a maximum or mean over the
+    columns would stop where the single solve with atol stops)."""
+    A = _spd_matrix()
+    b = np.random.default_rng(4).standard_normal(A.shape[0])
+    a = 1e-6 * np.linalg.norm(b)
+    _, i_eighth = fem.cg(A, b, atol=a / 8, rtol=0.0)
+    _, i_full = fem.cg(A, b, atol=a, rtol=0.0)
+    X, info = fem.cg(A, np.stack([b] * 64, axis=1), atol=a, rtol=0.0)
+    assert i_full['niter'] < i_eighth['niter']
+    assert info['niter'] == i_eighth['niter']
+
+
 def test_l2_error_converges_second_order():
     u = fem.cartesian(lambda p: np.sin(PI * p[..., 0]) * np.sin(PI * p[..., 1]))
     f = fem.cartesian(lambda p: 2 * PI ** 2 * np.sin(PI * p[..., 0]) * np.sin(PI * p[..., 1]))
