@@ -467,7 +467,7 @@ def serving_main(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: every box in this build has a single MI355X (asked --gpus {args.gpus})",
+        print(f"bench.py measures one MI355X: the multi-GPU path is a design only, not code (asked --gpus {args.gpus})",
               file=sys.stderr)
         sys.exit(2)
     if args.spmv_only:

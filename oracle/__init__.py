@@ -15,7 +15,7 @@ Modules
 
 Pinning (see tests/test_oracle_*.py): every function here is checked against the
 reference's own golden vectors (``tests/golden/ref_*.npz``), against fixtures
-written by running the reference in the survey container
+written by running the reference where its source is present
 (``tests/golden/fixtures_*.npz``) and against closed forms / brute force.
 Functions without such a check are marked "parity unpinned" in their docstring.
 """

@@ -7,8 +7,8 @@ quadrature and coefficient-evaluation order, the linear-elasticity integrator
 counts / residuals and L2 errors.
 
 Runs ONLY in a container that has the reference at ``/root/reference``; the
-outputs are small ``.npz`` files committed next to this script.  Nothing here
-travels to, or runs on, a GPU box.
+outputs are small ``.npz`` files committed next to this script.  The GPU tests
+read only those files; nothing of the reference is needed where they run.
 
 ``fealpy/fem/__init__.py`` and ``fealpy/solver/__init__.py`` import unrelated
 model modules that need ``gmsh`` (absent in this image); the hot-path modules
