@@ -19,17 +19,10 @@ namespace fa {
 
 constexpr int ROW_BATCH = 16;
 
-// sum_k vals[k] * xf(cols[k]) over the row, xf a gather functor (the vector,
-// or a value computed from several vectors at that column)
+// batches of B entries over one row's w entries (c, v: the row's first
+// column / value, stride 64)
 template <class T, int B, class XF>
-__device__ inline T sell_row_dot_f(const int32_t* __restrict__ slice_ptr, const int32_t* __restrict__ cols,
-                                   const T* __restrict__ vals, int64_t row, XF xf) {
-    const int64_t s = row >> 6;
-    const int i = static_cast<int>(row & 63);
-    const int32_t p0 = slice_ptr[s];
-    const int w = (slice_ptr[s + 1] - p0) >> 6;
-    const int32_t* c = cols + p0 + i;
-    const T* v = vals + p0 + i;
+__device__ inline T sell_batches(const int32_t* __restrict__ c, const T* __restrict__ v, int w, XF xf) {
     T acc = T(0);
     for (int k0 = 0; k0 < w; k0 += B) {
         int32_t cc[B];
@@ -48,6 +41,29 @@ __device__ inline T sell_row_dot_f(const int32_t* __restrict__ slice_ptr, const 
             if (k0 + u < w) acc += vv[u] * xx[u];
     }
     return acc;
+}
+
+// Slices at most SHORT_ROW entries wide (transfer operators: 1-4 entries a
+// row) take one batch of SHORT_ROW instead of B, so a 2-entry row issues 4
+// loads, not 16.  The width is that of the slice (one wavefront), so the
+// choice never diverges within a wave, and the sum is the same.
+constexpr int SHORT_ROW = 4;
+
+// sum_k vals[k] * xf(cols[k]) over the row, xf a gather functor (the vector,
+// or a value computed from several vectors at that column)
+template <class T, int B, class XF>
+__device__ inline T sell_row_dot_f(const int32_t* __restrict__ slice_ptr, const int32_t* __restrict__ cols,
+                                   const T* __restrict__ vals, int64_t row, XF xf) {
+    const int64_t s = row >> 6;
+    const int i = static_cast<int>(row & 63);
+    const int32_t p0 = slice_ptr[s];
+    const int w = (slice_ptr[s + 1] - p0) >> 6;
+    const int32_t* c = cols + p0 + i;
+    const T* v = vals + p0 + i;
+    if constexpr (B > SHORT_ROW) {
+        if (w <= SHORT_ROW) return sell_batches<T, SHORT_ROW>(c, v, w, xf);
+    }
+    return sell_batches<T, B>(c, v, w, xf);
 }
 
 template <class T, class X, int B = ROW_BATCH>
