@@ -292,7 +292,18 @@ __global__ void k_bc_to_point(int64_t nc, int nq, const double* __restrict__ nod
 //   a == b: cA A_aa + cB (sum of the other diagonal blocks, in axis order)
 //   a != b: cC A_ab + cD A_ba
 // laid out as i*TD + a (dof_priority = 0) or a*LDOF + i (dof_priority = 1).
-// One thread per (cell, i) writes the TD rows of scalar DOF i.
+// A block takes el_cells() cells at a time: their geometry (gl, |T|) is
+// computed once per cell into LDS, then one thread per (cell, i, j) forms the
+// TD x TD block of the pair (i, j) and stores its TD rows of TD entries.
+// Consecutive threads take consecutive j, so each store of a wavefront covers
+// contiguous runs of a row (one thread per (cell, i) scattered its stores over
+// 64 cache lines).  Same operations in the same order as the per-(cell, i)
+// form: tm[m][b] = sum_n M[i][j][m][n] gl[n][b], A_ab = |T| sum_m gl[m][a] tm[m][b].
+template <int TD, int LDOF>
+constexpr int el_cells() {
+    return (1024 / (LDOF * LDOF)) < 1 ? 1 : ((1024 / (LDOF * LDOF)) > 256 ? 256 : 1024 / (LDOF * LDOF));
+}
+
 template <int TD, int LDOF, class T>
 __global__ void __launch_bounds__(256) k_element_elasticity(const double* __restrict__ node,
                                                            const int32_t* __restrict__ cell, int64_t nc,
@@ -301,56 +312,79 @@ __global__ void __launch_bounds__(256) k_element_elasticity(const double* __rest
                                                            T* __restrict__ elem) {
     constexpr int NV = TD + 1;
     constexpr int N = TD * LDOF;
-    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (t >= nc * LDOF) return;
-    const int64_t c = t / LDOF;
-    const int i = static_cast<int>(t - c * LDOF);
-    double p[NV][TD];
-    load_cell<TD>(node, cell, c, p);
-    Geom<TD> g;
-    g.compute(p);
-    T* out = elem + c * (int64_t)N * N;
-    for (int j = 0; j < LDOF; ++j) {
-        const double* Mij = M + ((int64_t)i * LDOF + j) * NV * NV;
-        double tm[NV][TD];
+    constexpr int CB = el_cells<TD, LDOF>();
+    __shared__ double sgl[CB][NV][TD];
+    __shared__ double smeas[CB];
+    for (int64_t c0 = (int64_t)blockIdx.x * CB; c0 < nc; c0 += (int64_t)gridDim.x * CB) {
+        const int nb = static_cast<int>(nc - c0 < CB ? nc - c0 : CB);
+        if (threadIdx.x < nb) {
+            double p[NV][TD];
+            load_cell<TD>(node, cell, c0 + threadIdx.x, p);
+            Geom<TD> g;
+            g.compute(p);
 #pragma unroll
-        for (int m = 0; m < NV; ++m)
+            for (int m = 0; m < NV; ++m)
 #pragma unroll
-            for (int b = 0; b < TD; ++b) {
-                double s = 0.0;
+                for (int d = 0; d < TD; ++d) sgl[threadIdx.x][m][d] = g.gl[m][d];
+            smeas[threadIdx.x] = g.meas;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < nb * LDOF * LDOF; e += blockDim.x) {
+            const int cl = e / (LDOF * LDOF);
+            const int ij = e - cl * LDOF * LDOF;
+            const int i = ij / LDOF, j = ij - i * LDOF;
+            const double* Mij = M + (int64_t)ij * NV * NV;
+            double mm[NV * NV];
 #pragma unroll
-                for (int n = 0; n < NV; ++n) s += Mij[m * NV + n] * g.gl[n][b];
-                tm[m][b] = s;
-            }
-        double A[TD][TD];
+            for (int q = 0; q < NV * NV; ++q) mm[q] = Mij[q];
+            double gl[NV][TD];
 #pragma unroll
-        for (int a = 0; a < TD; ++a)
+            for (int m = 0; m < NV; ++m)
 #pragma unroll
-            for (int b = 0; b < TD; ++b) {
-                double s = 0.0;
+                for (int d = 0; d < TD; ++d) gl[m][d] = sgl[cl][m][d];
+            const double meas = smeas[cl];
+            double tm[NV][TD];
 #pragma unroll
-                for (int m = 0; m < NV; ++m) s += g.gl[m][a] * tm[m][b];
-                A[a][b] = g.meas * s;
-            }
+            for (int m = 0; m < NV; ++m)
 #pragma unroll
-        for (int a = 0; a < TD; ++a) {
-            const int row = dof_priority ? a * LDOF + i : i * TD + a;
+                for (int bb = 0; bb < TD; ++bb) {
+                    double s = 0.0;
 #pragma unroll
-            for (int b = 0; b < TD; ++b) {
-                double v;
-                if (a == b) {
-                    double others = 0.0;
-#pragma unroll
-                    for (int e = 0; e < TD; ++e)
-                        if (e != a) others += A[e][e];
-                    v = cA * A[a][a] + cB * others;
-                } else {
-                    v = cC * A[a][b] + cD * A[b][a];
+                    for (int n = 0; n < NV; ++n) s += mm[m * NV + n] * gl[n][bb];
+                    tm[m][bb] = s;
                 }
-                const int col = dof_priority ? b * LDOF + j : j * TD + b;
-                out[(int64_t)row * N + col] = static_cast<T>(v);
+            double A[TD][TD];
+#pragma unroll
+            for (int aa = 0; aa < TD; ++aa)
+#pragma unroll
+                for (int bb = 0; bb < TD; ++bb) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int m = 0; m < NV; ++m) s += gl[m][aa] * tm[m][bb];
+                    A[aa][bb] = meas * s;
+                }
+            T* out = elem + (c0 + cl) * (int64_t)N * N;
+#pragma unroll
+            for (int aa = 0; aa < TD; ++aa) {
+                const int row = dof_priority ? aa * LDOF + i : i * TD + aa;
+#pragma unroll
+                for (int bb = 0; bb < TD; ++bb) {
+                    double v;
+                    if (aa == bb) {
+                        double others = 0.0;
+#pragma unroll
+                        for (int f = 0; f < TD; ++f)
+                            if (f != aa) others += A[f][f];
+                        v = cA * A[aa][aa] + cB * others;
+                    } else {
+                        v = cC * A[aa][bb] + cD * A[bb][aa];
+                    }
+                    const int col = dof_priority ? bb * LDOF + j : j * TD + bb;
+                    out[(int64_t)row * N + col] = static_cast<T>(v);
+                }
             }
         }
+        __syncthreads();
     }
 }
 
@@ -840,9 +874,9 @@ extern "C" int fa_element_elasticity(int dtype, int td, int ldof, const double* 
     FA_REQUIRE(M && nc >= 0, "fa_element_elasticity: bad arguments");
     if (nc == 0) return FA_OK;
     hipStream_t s = S(stream);
-    const unsigned grid = (unsigned)((nc * ldof + 255) / 256);
 #define FA_EL(TD_, L_)                                                                                         \
     if (td == TD_ && ldof == L_) {                                                                             \
+        const unsigned grid = grid_for(nc, el_cells<TD_, L_>(), 1 << 20);                                        \
         if (dtype == FA_F64)                                                                                   \
             hipLaunchKernelGGL((k_element_elasticity<TD_, L_, double>), dim3(grid), dim3(256), 0, s, node, cell, \
                                nc, M, cA, cB, cC, cD, dof_priority, static_cast<double*>(elem));               \
