@@ -210,17 +210,52 @@ __global__ void k_gather_matrix(int64_t gdof, int ldof, const int32_t* __restric
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         const int32_t k1 = ptr[r + 1];
-        for (int32_t k = ptr[r]; k < k1; ++k) {
-            const int32_t e = adj[k];
-            const int32_t c = e / ldof;
-            for (int j = lane; j < ldof; j += G) {
-                const int32_t col = c2d[(int64_t)c * ldof + j];
-                const T v = elem[(int64_t)c * cstride + (e - c * ldof) * ldof + j];
-                const int pos = lower_bound_lds(cols, L, col);
-                if (pos < L && cols[pos] == col) acc[pos] += v;
+        if (ldof <= G) {
+            // the row's incidences G at a time in one load, handed out by
+            // shuffles; the next incidence's column and value loads are issued
+            // before the current one's search and add (same adds, same order)
+            for (int32_t kb = ptr[r]; kb < k1; kb += G) {
+                const int32_t mine = kb + lane < k1 ? adj[kb + lane] : 0;
+                const int nk = min(G, k1 - kb);
+                const bool act = lane < ldof;
+                int32_t e = __shfl(mine, 0, G);
+                int32_t c = e / ldof;
+                int32_t col = act ? c2d[(int64_t)c * ldof + lane] : 0;
+                T v = act ? elem[(int64_t)c * cstride + (e - c * ldof) * ldof + lane] : T(0);
+                for (int u = 0; u < nk; ++u) {
+                    int32_t col_n = 0;
+                    T v_n = T(0);
+                    if (u + 1 < nk) {
+                        const int32_t en = __shfl(mine, u + 1, G);
+                        const int32_t cn = en / ldof;
+                        if (act) {
+                            col_n = c2d[(int64_t)cn * ldof + lane];
+                            v_n = elem[(int64_t)cn * cstride + (en - cn * ldof) * ldof + lane];
+                        }
+                    }
+                    if (act) {
+                        const int pos = lower_bound_lds(cols, L, col);
+                        if (pos < L && cols[pos] == col) acc[pos] += v;
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                    col = col_n;
+                    v = v_n;
+                }
             }
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        } else {
+            for (int32_t k = ptr[r]; k < k1; ++k) {
+                const int32_t e = adj[k];
+                const int32_t c = e / ldof;
+                for (int j = lane; j < ldof; j += G) {
+                    const int32_t col = c2d[(int64_t)c * ldof + j];
+                    const T v = elem[(int64_t)c * cstride + (e - c * ldof) * ldof + j];
+                    const int pos = lower_bound_lds(cols, L, col);
+                    if (pos < L && cols[pos] == col) acc[pos] += v;
+                }
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            }
         }
         for (int t = lane; t < L; t += G) values[r0 + t] = acc[t];
         __builtin_amdgcn_wave_barrier();
