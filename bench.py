@@ -391,10 +391,13 @@ def serving_main(args):
     (a FIFO single server: start = max(arrival, previous done)).
 
     HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues per process (4 by
-    default); streams beyond that share queues and serialise.  Unless the
-    environment sets it, the serving process asks for 2 x slots queues (the
-    slots' streams, the default stream and torch's side streams each get one)."""
-    os.environ.setdefault("GPU_MAX_HW_QUEUES", str(min(32, max(4, 2 * args.slots))))
+    default); streams beyond that share queues and serialise.  The serving
+    process asks for at least 2 x slots queues (the slots' streams, the
+    default stream and torch's side streams each get one), set before HIP
+    initialises."""
+    want = min(32, max(4, 2 * args.slots))
+    if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < want:
+        os.environ["GPU_MAX_HW_QUEUES"] = str(want)
     import torch
     from fealpy_amd.serving import SolveServer, make_requests
     n = args.requests

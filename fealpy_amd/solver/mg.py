@@ -289,16 +289,18 @@ class GeometricMG:
         for l in range(len(self.levels) - 1):
             f, c = self.levels[l], self.levels[l + 1]
             mr, ma, mp = _max_row(f.R), _max_row(f.A), max(_max_row(f.P), 1)
+            # stages name the level's own operator None: the numeric setup reads
+            # the current f.A (a new matrix on the same pattern keeps the stages)
             if mr * ma * mp <= RAP_CANDIDATES:
                 c.A, c.max_row = self._rap_pattern(f.R, f.A, f.P, c.n, mr * ma * mp)
-                f.stages = [(f.R, f.A, f.P, c.A, c.max_row, self._rap_terms(f.R, f.A, f.P, c.A, mr * ma * mp))]
+                f.stages = [(f.R, None, f.P, c.A, c.max_row, self._rap_terms(f.R, f.A, f.P, c.A, mr * ma * mp))]
             else:
                 # R (A P) in two products: each row's candidate list stays short
                 # (P3 -> P1 in 3D: 975 against 34,125 for R A P at once)
                 i_f, i_c = _identity(f.n, self.device), _identity(c.n, self.device)
                 ap, ap_max = self._rap_pattern(i_f, f.A, f.P, c.n, ma * mp)
                 c.A, c.max_row = self._rap_pattern(f.R, ap, i_c, c.n, mr * _max_row(ap))
-                f.stages = [(i_f, f.A, f.P, ap, ap_max, self._rap_terms(i_f, f.A, f.P, ap, ma * mp)),
+                f.stages = [(i_f, None, f.P, ap, ap_max, self._rap_terms(i_f, f.A, f.P, ap, ma * mp)),
                             (f.R, ap, i_c, c.A, c.max_row, self._rap_terms(f.R, ap, i_c, c.A, mr * _max_row(ap)))]
             c.As = SellMatrix(c.A.crow, c.A.col, c.A.shape)
         last = self.levels[-1]
@@ -364,6 +366,7 @@ class GeometricMG:
         self.levels[0].A = A
         for l in range(len(self.levels) - 1):
             for R, Af, P, C, cmax, terms in self.levels[l].stages:
+                Af = self.levels[l].A if Af is None else Af
                 if terms is not None:
                     tslice, tw, wset, nw = terms
                     nat.check(L.fa_rap_values_terms(C.nnz, nat.ptr(tslice), nat.ptr(tw), wset, nw,

@@ -537,6 +537,35 @@ def test_reused_hierarchy_equals_fresh(dtype):
     assert torch.equal(x1, x2) and i1["niter"] == i2["niter"]
 
 
+@pytest.mark.parametrize("p", [1, 3])
+def test_reused_hierarchy_same_pattern_equals_fresh(p):
+    """A new matrix on the SAME pattern (same space, new coefficient: the
+    symbolic products are kept) gives the fresh hierarchy's coarse operators
+    and solution bit for bit -- the Galerkin products read the new fine
+    values, with one product (p = 1) and with two (P3 -> P1)."""
+    n = 10 if p == 1 else 4
+    mesh = fa.TetrahedronMesh.from_box(nx=n, ny=n, nz=n, device=DEV)
+    space = fa.LagrangeFESpace(mesh, p)
+    bc = fa.DirichletBC(space, gd=0.0)
+
+    def system(seed):
+        k = _lognormal(mesh, seed=seed)
+        A = fa.BilinearForm(space).add_integrator(fa.ScalarDiffusionIntegrator(k, q=p + 2)).assembly()
+        b = fa.LinearForm(space).add_integrator(fa.ScalarSourceIntegrator(1.0, q=p + 2)).assembly()
+        return bc.apply(A, b)
+    A1, _ = system(31)
+    A2, b2 = system(32)
+    assert A1.crow.data_ptr() == A2.crow.data_ptr()          # one cached pattern
+    used = GeometricMG(space, bc.is_boundary_dof, dtype=torch.float32).setup(A1)
+    used.setup(A2)
+    fresh = GeometricMG(space, bc.is_boundary_dof, dtype=torch.float32).setup(A2)
+    for lu, lf in zip(used.levels[1:], fresh.levels[1:]):
+        assert torch.equal(lu.A.values, lf.A.values)
+    x1, i1 = fa.cg(A2, b2, M=used, atol=0.0, rtol=1e-10, returninfo=True)
+    x2, i2 = fa.cg(A2, b2, M=fresh, atol=0.0, rtol=1e-10, returninfo=True)
+    assert torch.equal(x1, x2) and i1["niter"] == i2["niter"]
+
+
 def test_fp16_operator_on_int32_columns_equals_offsets():
     """The FP16 smoother operator read through the level's int32 columns (the
     form used when a column lies beyond the int16 offset range, e.g. P2 DOF
