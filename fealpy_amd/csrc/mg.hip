@@ -1125,10 +1125,11 @@ struct PcgState {
     int done;  // 0 running, 1 converged, 2 ||b|| < 1e-15, 3 maxit, 4 breakdown
 };
 
-// One PCG iteration = spmv(+p.Ap partials) | alpha | update | [cycle] |
-// r.z partials | check | p update.  The one-block *_finish kernels own
-// every write to the state; the grid kernels only read it.
-enum PcgFinish { PCG_INIT = 0, PCG_RZ0 = 1, PCG_ALPHA = 2, PCG_CHECK = 3 };
+// One PCG iteration = spmv(+p.Ap partials) | alpha + update | [cycle] |
+// r.z partials | check | p update.  The one-block *_finish kernels write the
+// state, and so does workgroup 0 of the update (alpha, p.Ap, which no other
+// workgroup of that kernel reads); the other grid kernels only read it.
+enum PcgFinish { PCG_INIT = 0, PCG_RZ0 = 1, PCG_CHECK = 3 };
 
 template <int WHAT>
 __global__ void __launch_bounds__(RED_BLOCK) k_pcg_finish(const double* __restrict__ partials, int nparts,
@@ -1145,9 +1146,6 @@ __global__ void __launch_bounds__(RED_BLOCK) k_pcg_finish(const double* __restri
         st->done = (st->bnorm < 1e-15) ? 2 : 0;
     } else if (WHAT == PCG_RZ0) {
         st->rz = t;
-    } else if (WHAT == PCG_ALPHA) {
-        st->pAp = t;
-        st->alpha = st->rz / t;
     } else {
         const double rn = sqrt(t);
         st->rnorm = rn;
@@ -1227,15 +1225,28 @@ __global__ void __launch_bounds__(RED_BLOCK) k_pcg_spmv(int64_t n, const int32_t
     store_partial(block_sum(acc), partials);
 }
 
-// x += alpha p; r -= alpha Ap; then either partials of r.r (no
-// preconditioner) or the cast of r for the cycle
+// alpha = r.z / p.Ap from the SpMV's partials (every workgroup sums them in
+// the finish kernel's order, so all get that kernel's alpha; workgroup 0 keeps
+// it in the state), then x += alpha p; r -= alpha Ap; then either partials of
+// r.r (no preconditioner) or the cast of r for the cycle
 template <class T, class U, bool PREC>
 __global__ void __launch_bounds__(RED_BLOCK) k_pcg_update(int64_t n, T* __restrict__ x, T* __restrict__ r,
                                                          U* __restrict__ ru, const T* __restrict__ p,
-                                                         const T* __restrict__ Ap, double* __restrict__ partials,
-                                                         const PcgState* st) {
+                                                         const T* __restrict__ Ap,
+                                                         const double* __restrict__ pap_parts, int np_pap,
+                                                         double* __restrict__ partials, PcgState* st) {
     if (st->done) return;
-    const T alpha = static_cast<T>(st->alpha);
+    __shared__ double s_alpha;
+    const double pAp = finish_sum(pap_parts, np_pap);
+    if (threadIdx.x == 0) {
+        s_alpha = st->rz / pAp;
+        if (blockIdx.x == 0) {
+            st->pAp = pAp;
+            st->alpha = s_alpha;
+        }
+    }
+    __syncthreads();
+    const T alpha = static_cast<T>(s_alpha);
     double acc = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         x[i] += alpha * p[i];
@@ -1349,18 +1360,21 @@ static int pcg_run(const fa_sell_t* A, const fa_mg_t* M, const T* b, T* x, doubl
                            M->levels[0].Ah.slice_ptr == A->slice_ptr && M->levels[0].Ah.n == n)
                               ? M->levels[0].Ah.offs
                               : nullptr;
+    // p.Ap partials in the second partial array: the update reads them while
+    // writing r.r partials (no preconditioner) into the first
+    double* pap = partials + RED_GRID;
     auto iteration = [&](hipStream_t q) -> int {
-        hipLaunchKernelGGL(k_pcg_spmv<T>, grid, block, 0, q, n, A->slice_ptr, A->cols, offs, sv, p, Ap, partials, st);
-        hipLaunchKernelGGL(k_pcg_finish<PCG_ALPHA>, one, block, 0, q, partials, np, st);
+        hipLaunchKernelGGL(k_pcg_spmv<T>, grid, block, 0, q, n, A->slice_ptr, A->cols, offs, sv, p, Ap, pap, st);
         if (M) {
-            hipLaunchKernelGGL((k_pcg_update<T, U, true>), grid, block, 0, q, n, x, r, ru, p, Ap, partials, st);
+            hipLaunchKernelGGL((k_pcg_update<T, U, true>), grid, block, 0, q, n, x, r, ru, p, Ap, pap, np, partials,
+                               st);
             int e = vcycle_any(M, ru, z, done, q, &rz_out);  // z = M r and the partials of r.z
             if (e) return e;
             hipLaunchKernelGGL(k_pcg_finish<PCG_CHECK>, one, block, 0, q, partials, np, st);
             hipLaunchKernelGGL((k_pcg_pupdate<T, U>), grid, block, 0, q, n, p, z, st);
         } else {
-            hipLaunchKernelGGL((k_pcg_update<T, T, false>), grid, block, 0, q, n, x, r, (T*)nullptr, p, Ap, partials,
-                               st);
+            hipLaunchKernelGGL((k_pcg_update<T, T, false>), grid, block, 0, q, n, x, r, (T*)nullptr, p, Ap, pap, np,
+                               partials, st);
             hipLaunchKernelGGL(k_pcg_finish<PCG_CHECK>, one, block, 0, q, partials, np, st);
             hipLaunchKernelGGL((k_pcg_pupdate<T, T>), grid, block, 0, q, n, p, r, st);
         }
