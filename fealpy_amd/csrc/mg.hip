@@ -841,6 +841,9 @@ __global__ void __launch_bounds__(FUSE_BLOCK) k_fused_cycle(FusedCycle fc, void*
     }
 }
 
+// longest row for which the first two pre-smoothing steps run fused
+constexpr int Z01_MAX_WIDTH = 32;
+
 // ---------------------------------------------------------------------------
 // V-cycle driver
 template <class T>
@@ -906,7 +909,10 @@ struct Cycle {
                 c2 = static_cast<T>(2.0 * rho_new / delta);
             }
             if (k == 0 && from_zero) {
-                if (degree >= 2) {  // steps 0 and 1 in one pass
+                // steps 0 and 1 in one pass, forming step 0 at every gathered
+                // column: two gathers an entry instead of one, which pays for
+                // the saved launch on short rows only
+                if (degree >= 2 && L.width > 0 && L.width <= Z01_MAX_WIDTH) {
                     const T c0 = c2;
                     rho_new = 1.0 / (2.0 * sigma - rho);
                     const T d1 = static_cast<T>(rho_new * rho), d2 = static_cast<T>(2.0 * rho_new / delta);

@@ -38,7 +38,7 @@ from ..sparse.tensors import CSRTensor
 class FaMgLevel(ctypes.Structure):
     _fields_ = [("A", FaSell), ("dinv", ctypes.c_void_p), ("R", FaSell), ("P", FaSell), ("lmax", ctypes.c_double),
                 ("b", ctypes.c_void_p), ("x", ctypes.c_void_p), ("x2", ctypes.c_void_p), ("d", ctypes.c_void_p),
-                ("r", ctypes.c_void_p), ("Ah", FaSell16), ("diag", ctypes.c_void_p)]
+                ("r", ctypes.c_void_p), ("Ah", FaSell16), ("diag", ctypes.c_void_p), ("width", ctypes.c_int32)]
 
 
 class FaMg(ctypes.Structure):
@@ -284,6 +284,7 @@ class GeometricMG:
         from ..sparse.sell import sell_of
         self.levels[0].A = A
         self.levels[0].As = sell_of(A)
+        self.levels[0].max_row = _max_row(A)
         for lv in self.levels:
             lv.compressed = None        # decided by the next setup (band check of the new pattern)
         for l in range(len(self.levels) - 1):
@@ -433,6 +434,7 @@ class GeometricMG:
             e.R = lv.Rs.desc(self.dtype) if i < n - 1 else empty
             e.P = lv.Ps.desc(self.dtype) if i < n - 1 else empty
             e.lmax = getattr(lv, 'lmax', 2.0)
+            e.width = getattr(lv, 'max_row', 0)
             e.b, e.x, e.x2, e.d, e.r = (lv.b.data_ptr(), lv.x.data_ptr(), lv.x2.data_ptr(), lv.d.data_ptr(),
                                         lv.r.data_ptr())
             if getattr(lv, 'compressed', False):

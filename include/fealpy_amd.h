@@ -373,6 +373,9 @@ typedef struct {
     void* r;
     fa_sell16_t Ah;            /* compressed D^-1 A for an FP32 cycle, or vals = NULL */
     const void* diag;          /* diag(A) in the cycle dtype (with Ah)          */
+    int32_t width;             /* longest row of A (0: unknown); pre-smoothing
+                                  fuses its first two steps only for rows of at
+                                  most 32 entries                               */
 } fa_mg_level_t;
 
 typedef struct {
