@@ -135,12 +135,12 @@ __device__ inline float sell16_row_dot_f(const int32_t* __restrict__ slice_ptr, 
     const int w = (slice_ptr[s + 1] - slice_ptr[s]) >> 6;
     float acc = 0.f;
     for (int k0 = 0; k0 < w; k0 += ROW_BATCH) {
-        int64_t cc[ROW_BATCH];
+        int32_t cc[ROW_BATCH];  // column indices are int32 (SELL) or row + int16
         float vv[ROW_BATCH];
 #pragma unroll
         for (int u = 0; u < ROW_BATCH; ++u) {
             const int64_t p = p0 + 64 * min(k0 + u, w - 1);
-            cc[u] = cs(p, row);
+            cc[u] = static_cast<int32_t>(cs(p, row));
             vv[u] = static_cast<float>(vals[p]);
         }
         float xx[ROW_BATCH];
