@@ -67,7 +67,20 @@ __global__ void __launch_bounds__(WAVE) k_sell_fill_staged(int64_t n, const int3
         const int32_t r0 = row < n ? indptr[row] - base : 0;
         const int len = row < n ? indptr[row + 1] - base - r0 : 0;
         if (total <= SF_STAGE) {
-            for (int t = lane; t < total; t += WAVE) stage[t] = vin[base + t];
+            constexpr int U = 8;  // loads of a lane in flight before its LDS stores
+            for (int t0 = 0; t0 < total; t0 += U * WAVE) {
+                A v[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int t = t0 + u * WAVE + lane;
+                    v[u] = t < total ? vin[base + t] : A(0);
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int t = t0 + u * WAVE + lane;
+                    if (t < total) stage[t] = v[u];
+                }
+            }
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             for (int k = 0; k < w; ++k) vals[p0 + 64 * k + lane] = k < len ? static_cast<B>(stage[r0 + k]) : B(0);
