@@ -85,8 +85,15 @@ __global__ void __launch_bounds__(WAVE) k_sell_fill_staged(int64_t n, const int3
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             for (int k = 0; k < w; ++k) vals[p0 + 64 * k + lane] = k < len ? static_cast<B>(stage[r0 + k]) : B(0);
         } else {
-            for (int k = 0; k < w; ++k)
-                vals[p0 + 64 * k + lane] = k < len ? static_cast<B>(vin[base + r0 + k]) : B(0);
+            constexpr int U = 8;  // a lane's loads of U entries in flight before its stores
+            for (int k0 = 0; k0 < w; k0 += U) {
+                A v[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) v[u] = k0 + u < len ? vin[base + r0 + k0 + u] : A(0);
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (k0 + u < w) vals[p0 + 64 * (k0 + u) + lane] = static_cast<B>(v[u]);
+            }
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
