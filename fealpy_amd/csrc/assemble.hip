@@ -921,9 +921,9 @@ extern "C" int fa_element_elasticity(int dtype, int td, int ldof, const double* 
         FA_LAUNCH("k_element_elasticity");                                                                     \
         return FA_OK;                                                                                          \
     }
-    FA_EL(2, 3) FA_EL(2, 6) FA_EL(2, 10) FA_EL(3, 4) FA_EL(3, 10) FA_EL(3, 20)
+    FA_EL(2, 3) FA_EL(2, 6) FA_EL(2, 10) FA_EL(2, 15) FA_EL(3, 4) FA_EL(3, 10) FA_EL(3, 20) FA_EL(3, 35)
 #undef FA_EL
-    set_error("fa_element_elasticity: unsupported (td=%d, ldof=%d); p must be 1..3", td, ldof);
+    set_error("fa_element_elasticity: unsupported (td=%d, ldof=%d); p must be 1..4", td, ldof);
     return FA_EINVAL;
 }
 

@@ -135,7 +135,7 @@ int fa_gather_vector(int dtype, int64_t gdof, int32_t ldof,
  * columns (j, b): cA A_aa + cB sum_{e != a} A_ee on diagonal blocks,
  * cC A_ab + cD A_ba off them, with A_ab = |T| sum_q w_q d_a phi_i d_b phi_j.
  * Layout i*td + a (dof_priority 0, TensorFunctionSpace shape (-1, td)) or
- * a*ldof + i (dof_priority 1).  elem: (nc, td*ldof, td*ldof).  p <= 3. */
+ * a*ldof + i (dof_priority 1).  elem: (nc, td*ldof, td*ldof).  p <= 4. */
 int fa_element_elasticity(int dtype, int td, int ldof, const double* node, const int32_t* cell, int64_t nc,
                           const double* M, double cA, double cB, double cC, double cD, int dof_priority,
                           void* elem, fa_stream_t stream);

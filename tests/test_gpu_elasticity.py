@@ -11,7 +11,7 @@ import torch
 
 import fealpy_amd as fa
 from oracle import fem as ofem
-from oracle.mesh import tet_from_box
+from oracle.mesh import tet_from_box, tri_from_box
 
 pytestmark = pytest.mark.gpu
 PI = np.pi
@@ -63,7 +63,7 @@ def test_elasticity_matches_reference_fixture(golden):
     np.testing.assert_allclose(_np(F), g["F"], rtol=0, atol=1e-12 * np.abs(g["F"]).max())
 
 
-@pytest.mark.parametrize("p,prio", [(1, False), (2, False), (2, True), (3, False)])
+@pytest.mark.parametrize("p,prio", [(1, False), (2, False), (2, True), (3, False), (4, False), (4, True)])
 def test_element_matrices_match_oracle(p, prio):
     mesh, ts = _space(2, p, (3, -1) if prio else (-1, 3))
     om = tet_from_box(nx=2, ny=2, nz=2)
@@ -78,6 +78,20 @@ def test_element_matrices_match_oracle(p, prio):
     fe = fa.VectorSourceIntegrator(torch.as_tensor(f, device=DEV), q=p + 2).assembly(ts)
     fref = ofem.vector_source_element(om, p, f, q=p + 2, dof_priority=prio)
     np.testing.assert_allclose(_np(fe), fref, rtol=0, atol=1e-12 * np.abs(fref).max())
+
+
+@pytest.mark.parametrize("p,prio", [(1, False), (2, True), (3, False), (4, False), (4, True)])
+def test_element_matrices_2d_plane_strain_match_oracle(p, prio):
+    mesh = fa.TriangleMesh.from_box([0, 1, 0, 1], nx=3, ny=2, device=DEV)
+    ts = fa.TensorFunctionSpace(fa.LagrangeFESpace(mesh, p), (2, -1) if prio else (-1, 2))
+    om = tri_from_box(nx=3, ny=2)
+    mat = fa.LinearElasticMaterial("E", elastic_modulus=2.5, poisson_ratio=0.3, hypo="plane_strain")
+    ke = fa.LinearElasticityIntegrator(mat, q=p + 2).assembly(ts)
+    ref = ofem.elasticity_element(om, p, mat.lam, mat.mu, q=p + 2, dof_priority=prio)
+    assert ke.shape == ref.shape
+    np.testing.assert_allclose(_np(ke), ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    t2d = ofem.tensor_cell2dof(om.cell_to_ipoint(p), 2, om.number_of_global_ipoints(p), prio)
+    np.testing.assert_array_equal(_np(ts.cell_to_dof()), t2d)
 
 
 def test_clamped_solve_matches_direct_solve():
@@ -96,10 +110,11 @@ def test_clamped_solve_matches_direct_solve():
     assert np.abs(_np(x)[_np(bc.is_boundary_dof)]).max() == 0.0
 
 
-def test_rigid_motions_in_assembled_kernel():
-    mesh, ts = _space(2, 2)
+@pytest.mark.parametrize("p", [2, 4])
+def test_rigid_motions_in_assembled_kernel(p):
+    mesh, ts = _space(2, p)
     mat = fa.LinearElasticMaterial("E", lame_lambda=2.0, shear_modulus=0.75)
-    A = fa.BilinearForm(ts).add_integrator(fa.LinearElasticityIntegrator(mat, q=4)).assembly()
+    A = fa.BilinearForm(ts).add_integrator(fa.LinearElasticityIntegrator(mat, q=p + 2)).assembly()
     ip = ts.interpolation_points()
     for w in torch.eye(3, dtype=torch.float64, device=DEV):
         u = torch.cross(w.expand_as(ip), ip, dim=-1).reshape(-1)

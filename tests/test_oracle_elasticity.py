@@ -49,7 +49,8 @@ def test_elasticity_matches_reference_fixture(golden):
     np.testing.assert_allclose(F, g["F"], rtol=0, atol=1e-12 * np.abs(g["F"]).max())
 
 
-@pytest.mark.parametrize("dim,p,prio", [(3, 1, False), (3, 2, False), (3, 2, True), (2, 2, False), (2, 1, True)])
+@pytest.mark.parametrize("dim,p,prio", [(3, 1, False), (3, 2, False), (3, 2, True), (3, 4, True), (2, 2, False), (2, 1, True),
+                                         (2, 4, False)])
 def test_rigid_motions_are_in_the_kernel(dim, p, prio):
     mesh = tet_from_box(nx=2, ny=1, nz=2) if dim == 3 else tri_from_box(nx=3, ny=2)
     ke = ofem.elasticity_element(mesh, p, 1.3, 0.7, q=p + 2, dof_priority=prio)
@@ -82,3 +83,23 @@ def test_shear_only_blocks():
         for b in range(3):
             if a != b:
                 np.testing.assert_allclose(ke[:, a::3, b::3], A[:, b, a], atol=1e-15)
+
+
+@pytest.mark.parametrize("dim,p,prio", [(3, 1, False), (3, 3, True), (3, 4, False), (2, 1, True), (2, 3, False),
+                                         (2, 4, True)])
+def test_energy_of_linear_displacement(dim, p, prio):
+    """u = G x is represented exactly for every p, so u^T K u summed over the
+    cells is |Omega| (2 mu eps:eps + lam tr(eps)^2) with eps = sym(G)
+    (3D and plane strain): pins the block coefficients and |T| for any p."""
+    mesh = tet_from_box(nx=2, ny=1, nz=1) if dim == 3 else tri_from_box(nx=3, ny=2)
+    lam, mu = 1.3, 0.7
+    ke = ofem.elasticity_element(mesh, p, lam, mu, q=2 * p, dof_priority=prio)
+    ip = mesh.interpolation_points(p)[mesh.cell_to_ipoint(p)]            # (NC, ldof, GD)
+    NC, ldof, GD = ip.shape
+    G = np.random.default_rng(7).standard_normal((GD, GD))
+    u = ip @ G.T
+    v = u.swapaxes(1, 2).reshape(NC, -1) if prio else u.reshape(NC, -1)
+    energy = np.einsum('ci, cij, cj ->', v, ke, v)
+    eps = 0.5 * (G + G.T)
+    exact = 1.0 * (2 * mu * np.sum(eps * eps) + lam * np.trace(eps) ** 2)   # |Omega| = 1
+    assert abs(energy - exact) <= 1e-12 * abs(exact)
