@@ -206,12 +206,11 @@ class SimplexMesh:
         return torch.einsum('cbm, qjb -> cqjm', gl, R)
 
     def bc_to_point(self, bcs, index=_S):
-        """(NC, NQ, GD) physical points of barycentric points bcs (NQ, TD+1);
-        on the GPU through fa_bc_to_point."""
+        """(NC, NQ, GD) physical points of barycentric points bcs (NQ, TD+1),
+        through fa_bc_to_point (a mesh on the host raises, as every kernel
+        wrapper does)."""
         bcs = torch.as_tensor(bcs, dtype=self.ftype, device=self.device)
         cell = self.cell[index]
-        if not self.node.is_cuda:
-            return torch.einsum('cjk, qj -> cqk', self.node[cell.long()], bcs)
         from .. import _native as nat
         cell = cell.to(torch.int32).contiguous()
         nc, nv = cell.shape

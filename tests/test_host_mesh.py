@@ -116,6 +116,9 @@ def test_product_refuses_cpu_tensors():
         pytest.skip("GPU present")
     with pytest.raises(_native.NativeError):
         _native.lib()
+    mesh = fa.TetrahedronMesh.from_box(nx=1, ny=1, nz=1, device=torch.device("cpu"))
+    with pytest.raises(_native.NativeError):
+        mesh.bc_to_point(torch.full((1, 4), 0.25, dtype=torch.float64))
 
 
 @pytest.mark.parametrize("TD,p", [(2, 2), (3, 2), (3, 3)])
