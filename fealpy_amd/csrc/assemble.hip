@@ -1143,13 +1143,29 @@ __global__ void __launch_bounds__(256) k_sum_incidence_rows(int64_t gdof, const 
         for (int m = 0; m < K; ++m) acc[m] = 0.0;
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        const int32_t k1 = adj_ptr[r + 1];
-        for (int32_t kb = adj_ptr[r]; kb < k1; kb += G) {
+        // the next chunk's position word and element row are loaded before
+        // the current chunk is summed (two chunks of loads in flight per wave)
+        const int32_t kbeg = adj_ptr[r], k1 = adj_ptr[r + 1];
+        uint32_t pk_next = 0;
+        double v_next[NV];
+        if (kbeg + lane < k1) {
+            pk_next = pos[kbeg + lane];
+#pragma unroll
+            for (int j = 0; j < NV; ++j) v_next[j] = rows[static_cast<int64_t>(kbeg + lane) * NV + j];
+        }
+        for (int32_t kb = kbeg; kb < k1; kb += G) {
             const int32_t k = kb + lane;
+            const uint32_t pk = pk_next;
+            double v[NV];
+#pragma unroll
+            for (int j = 0; j < NV; ++j) v[j] = v_next[j];
+            if (k + G < k1) {
+                pk_next = pos[k + G];
+#pragma unroll
+                for (int j = 0; j < NV; ++j) v_next[j] = rows[static_cast<int64_t>(k + G) * NV + j];
+            }
             int p[NV];
             if (k < k1) {
-                const uint32_t pk = pos[k];
-                const double* v = rows + static_cast<int64_t>(k) * NV;
 #pragma unroll
                 for (int j = 0; j < NV; ++j) {
                     p[j] = static_cast<int>((pk >> (8 * j)) & 255u);
