@@ -4,21 +4,28 @@ Workload (one "step"): 3D scalar diffusion -div(k grad u) = f on the unit
 cube, TetrahedronMesh.from_box 128^3 (12,582,912 tets, 2,146,689 P1 DOFs),
 per-cell coefficient k = exp(g), g ~ N(0,1) i.i.d. from numpy
 default_rng(20261202), manufactured u = sin(pi x) sin(pi y) sin(pi z),
-source f_c = 3 pi^2 k_c u (barycentric callable), q = 2.  A step assembles
-A and F, applies the Dirichlet condition, sets up the multigrid hierarchy
-(Galerkin products, smoother bounds, coarse factor) and solves with
-multigrid-preconditioned CG.  Inputs (mesh, coefficient, the sparsity
-pattern) are resident in HBM before the timed region.
+source f_c = 3 pi^2 k_c u, q = 2.  A step assembles A and F, applies the
+Dirichlet condition, sets up the multigrid hierarchy (Galerkin products,
+smoother bounds, coarse factor) and solves with multigrid-preconditioned CG.
+Inputs (mesh, coefficient, the sparsity pattern) are resident in HBM before
+the timed region.  The source and the Dirichlet data are separable
+expressions (fealpy_amd.expr), which the load-vector kernel evaluates at
+the quadrature points itself (``--torch-source``: the same source as a
+barycentric torch callable, evaluated on (NC, NQ, 3) points first).
 
-Reported ``value``: mixed precision (FP64 CG, FP32 V-cycle) DOF/s.  The plain
-FP64 GPU path (FP64 CG, FP64 V-cycle) is timed beside it; both solutions are
-checked against a tight FP64 solve (relative 2-norm error <= 1e-8).
+Reported ``value``: mixed precision (FP64 CG, FP32 V-cycle smoothing with
+FP16 D^-1 A on the finest levels) DOF/s.  The plain FP64 GPU path (FP64 CG,
+FP64 V-cycle) is timed beside it; both solutions are checked against a tight
+FP64 solve (relative 2-norm error <= 1e-8).  The same line carries the
+serving figures (solves/s, p50/p99) of the 4096-request mixed-size trace.
 
-    python bench.py [--steps K] [--warmup W] [--n 128]
+    python bench.py [--steps K] [--warmup W] [--n 128] [--no-serving] [--no-cpu]
 """
 import argparse
 import json
 import os
+import platform
+import statistics
 import sys
 import time
 
@@ -32,6 +39,15 @@ HBM_PEAK_GBS = 8000.0
 # and WRITE_SIZE counter passes over `bench.py --spmv-only`, each scaled by the
 # same counter's reading for a 256 MiB device copy (profiles/sell_spmv_f32_128cube_counters.md)
 SPMV_F32_TRAFFIC_128 = 267.3e6
+# HBM bytes per launch of k_pcg_spmv<double> (mixed form) on the 128^3
+# operator, from the same two counter passes over `bench.py --spmv-only`
+# (profiles/pcg_spmv_f64_128cube_counters.md); None until measured
+PCG_SPMV_TRAFFIC_128 = None
+# serving: slots (streams) of the continuous-batching server and the hardware
+# queues the process asks for (the slots' streams, the default stream and
+# torch's side streams each get one); read by HIP when it initialises
+SERVING_SLOTS = 4
+SERVING_HW_QUEUES = 8
 
 
 def parse():
@@ -43,9 +59,13 @@ def parse():
     ap.add_argument("--rtol", type=float, default=1e-8,
                     help="PCG relative residual; every run checks the solution error against 1e-8")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-n", type=int, default=40)
+    ap.add_argument("--cpu-n", type=int, default=48, help="CPU baseline: from_box size of the bounded sample")
+    ap.add_argument("--cpu-runs", type=int, default=3, help="CPU baseline: runs (the median is reported)")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving figures of the headline line")
+    ap.add_argument("--torch-source", action="store_true",
+                    help="source as a barycentric torch callable instead of a separable expression")
     ap.add_argument("--requests", type=int, default=4096, help="serving: number of requests")
-    ap.add_argument("--slots", type=int, default=4, help="serving: concurrent request slots (streams)")
+    ap.add_argument("--slots", type=int, default=SERVING_SLOTS, help="serving: concurrent request slots (streams)")
     ap.add_argument("--rate", type=float, default=0.0,
                     help="serving: Poisson arrival rate per second (0: 0.8 x the measured server throughput)")
     ap.add_argument("--workload", choices=("diffusion", "elasticity", "highorder", "serving"), default="diffusion",
@@ -53,29 +73,27 @@ def parse():
                          "linear elasticity on from_box n^3 (n defaults to 40); highorder: P3/P4 diffusion + "
                          "mass element matrices on the matrix cores, from_box n^3 (n defaults to 48)")
     ap.add_argument("--spmv-only", action="store_true",
-                    help="assemble once, then run only the FP32 SELL SpMV and a plain copy kernel of known size "
-                         "(for hardware-counter runs)")
+                    help="assemble once, then run only the FP32 SELL SpMV, a few mixed PCG iterations and a "
+                         "plain copy kernel of known size (for hardware-counter runs)")
     return ap.parse_args()
 
 
-def make_problem(n, dev):
+def make_problem(n, dev, torch_source=False):
     import torch
     import fealpy_amd as fa
+    from fealpy_amd.expr import cell, sin, x, y, z
     mesh = fa.TetrahedronMesh.from_box([0, 1, 0, 1, 0, 1], nx=n, ny=n, nz=n, device=dev)
     g = np.random.default_rng(20261202).standard_normal(mesh.number_of_cells())
     k = torch.as_tensor(np.exp(g), device=dev)
     pi = np.pi
-
-    @fa.cartesian
-    def u(p):
-        return torch.sin(pi * p).prod(dim=-1)
-
     c3 = 3 * pi ** 2
-
-    @fa.barycentric
-    def f(bcs, index=None):
-        return (c3 * k)[:, None] * u(mesh.bc_to_point(bcs))
-
+    u = sin(pi * x) * sin(pi * y) * sin(pi * z)
+    if torch_source:
+        @fa.barycentric
+        def f(bcs, index=None):
+            return (c3 * k)[:, None] * torch.sin(pi * mesh.bc_to_point(bcs)).prod(dim=-1)
+    else:
+        f = c3 * cell(k) * u
     space = fa.LagrangeFESpace(mesh, 1)
     return mesh, space, k, u, f
 
@@ -125,9 +143,67 @@ def phase_breakdown(space, k, u, f, mg, rtol):
     return ph
 
 
+def in_step_kernels(space, k, u, f, mg, rtol):
+    """One more step of the form with the in-step kernel timer armed: HIP
+    events on the launching stream around every launch of the step's
+    dominant kernels (fa_timer_arm / fa_timer_read).  -> {kind: (launches, ms)}"""
+    import torch
+    from fealpy_amd import _native as nat
+    kinds = {"pcg_spmv": nat.TIMER_PCG_SPMV, "a_pass1": nat.TIMER_P1_ROWS, "a_pass2": nat.TIMER_P1_SUM_ROWS,
+             "f_pass1": nat.TIMER_P1_VECTORS, "f_pass2": nat.TIMER_P1_SUM_VALUES, "dirichlet": nat.TIMER_BC_APPLY,
+             "vcycle": nat.TIMER_VCYCLE}
+    torch.cuda.synchronize()
+    nat.timer_arm(*kinds.values())
+    try:
+        step(space, k, u, f, mg, rtol)
+        torch.cuda.synchronize()
+        return {name: nat.timer_read(kd) for name, kd in kinds.items()}
+    finally:
+        nat.timer_arm()
+
+
+def _roof(kernel, algo_bytes, launches, ms, traffic=None, **extra):
+    avg = ms / max(launches, 1)
+    gbs = algo_bytes / (avg * 1e-3) / 1e9 if avg > 0 else 0.0
+    out = {"bound": "hbm", "kernel": kernel, "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic, "algorithmic_bytes": int(algo_bytes),
+           "avg_us": round(avg * 1e3, 2), "launches": int(launches)}
+    out.update(extra)
+    return out
+
+
+def rooflines(kt, mesh, gdof, nnz_a, nnz_bc, n):
+    """Roofline objects from the in-step kernel times.  Algorithmic bytes
+    are the compulsory traffic of each phase (inputs read once, outputs
+    written once):
+    - CG operator A p, FP64 values, int32 indices: 12 nnz + 4 (n+1) + 16 n;
+    - A values (FP64): 4 NC ldof (cells) + 8 GD NN (coordinates) + 8 NC (k) + 8 nnz;
+    - F (FP64): 4 NC ldof + 8 GD NN + 8 NC (k) + 8 gdof."""
+    nc, nn = mesh.number_of_cells(), mesh.number_of_nodes()
+    b_spmv = 12 * nnz_bc + 4 * (n + 1) + 16 * n
+    pl, pms = kt["pcg_spmv"]
+    tr = PCG_SPMV_TRAFFIC_128 if n == 2146689 else None
+    main = _roof("k_pcg_spmv<double>", b_spmv, pl, pms, tr,
+                 note="outer CG operator of the mixed form (FP64 values on the int16 column offsets of the "
+                      "finest smoother slices: it reads 10 B an entry where the algorithmic count has 12)")
+    b_a = 16 * nc + 24 * nn + 8 * nc + 8 * nnz_a
+    b_f = 16 * nc + 24 * nn + 8 * nc + 8 * gdof
+    (l1, m1), (l2, m2) = kt["a_pass1"], kt["a_pass2"]
+    (l3, m3), (l4, m4) = kt["f_pass1"], kt["f_pass2"]
+    asm_a = _roof("k_p1_incidence_rows<3> + k_sum_incidence_rows", b_a, 1, m1 + m2,
+                  pass1_us=round(m1 * 1e3 / max(l1, 1), 2), pass2_us=round(m2 * 1e3 / max(l2, 1), 2))
+    asm_f = _roof("k_p1_incidence_vectors<3> + k_sum_incidence_values", b_f, 1, m3 + m4,
+                  pass1_us=round(m3 * 1e3 / max(l3, 1), 2), pass2_us=round(m4 * 1e3 / max(l4, 1), 2))
+    vl, vms = kt["vcycle"]
+    dl, dms = kt["dirichlet"]
+    return main, {"A": asm_a, "F": asm_f, "dirichlet_us": round(dms * 1e3 / max(dl, 1), 2),
+                  "vcycle_us": round(vms * 1e3 / max(vl, 1), 2), "vcycles": vl}
+
+
 def spmv_roofline(A, dtype):
-    """Average duration of the SELL SpMV (the kernel under every level-0 cycle
-    step) on torch's current stream, timed with HIP events."""
+    """Average duration of the SELL SpMV on torch's current stream, timed with
+    HIP events over 50 stand-alone launches (an extra figure: the solvers'
+    own operator is k_pcg_spmv)."""
     import torch
     from fealpy_amd.sparse.sell import sell_of
     S = sell_of(A)
@@ -155,49 +231,67 @@ def spmv_roofline(A, dtype):
 
 
 def spmv_only(args):
-    """FP32 SELL SpMV of the assembled 128^3 operator, then a 256 MiB
-    device-to-device copy (torch elementwise copy) as the counter calibration:
-    FETCH_SIZE / WRITE_SIZE of the copy divided by its 256 MiB gives the
-    counter's bytes-per-byte factor."""
+    """For hardware-counter runs: assemble the 128^3 system once, run 50
+    FP32 SELL SpMVs, one mixed-form MG-PCG solve (its k_pcg_spmv<double>
+    launches) and a 256 MiB device-to-device copy (torch elementwise copy)
+    as the counter calibration: FETCH_SIZE / WRITE_SIZE of the copy divided
+    by its 256 MiB gives the counter's bytes-per-byte factor."""
     import torch
     import fealpy_amd as fa
+    from fealpy_amd.solver import GeometricMG
     dev = torch.device("cuda:0")
     mesh, space, k, u, f = make_problem(args.n, dev)
     A = fa.BilinearForm(space).add_integrator(fa.ScalarDiffusionIntegrator(k, q=2)).assembly()
     F = fa.LinearForm(space).add_integrator(fa.ScalarSourceIntegrator(f, q=2)).assembly()
-    A2, _ = fa.DirichletBC(space, gd=u).apply(A, F)
+    bc = fa.DirichletBC(space, gd=u)
+    A2, F2 = bc.apply(A, F)
     roof = spmv_roofline(A2, torch.float32)
+    mg = GeometricMG(space, bc.is_boundary_dof, dtype=torch.float32)
+    mg.setup(A2)
+    x, info = fa.cg(A2, F2, M=mg, atol=0.0, rtol=args.rtol, maxit=400, returninfo=True)
     nbytes = 256 << 20
-    x = torch.rand(nbytes // 4, dtype=torch.float32, device=dev)
-    y = torch.empty_like(x)
+    xb = torch.rand(nbytes // 4, dtype=torch.float32, device=dev)
+    yb = torch.empty_like(xb)
     for _ in range(10):
-        y.copy_(x)
+        yb.copy_(xb)
     torch.cuda.synchronize()
-    print(json.dumps({"spmv": roof, "copy_bytes_each_way": nbytes}))
+    print(json.dumps({"spmv": roof, "pcg_iterations": info["niter"], "copy_bytes_each_way": nbytes}))
 
 
-def cpu_baseline(n):
-    """The oracle (NumPy/SciPy FP64 restatement of the reference path) on the
-    same workload at n^3, single BLAS thread: assemble A and F (einsum,
-    lexsort-coalesce, add.at), Dirichlet filter, and the reference's CG loop
-    (unpreconditioned, rtol 1e-10)."""
+def _cpu_model():
     try:
-        from threadpoolctl import threadpool_limits
-    except ImportError:
-        threadpool_limits = None
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(n, runs):
+    """The oracle (NumPy/SciPy FP64 restatement of the reference path) on a
+    bounded sample of the same workload, from_box n^3: assemble A and F
+    (einsum element tensors, lexsort-coalesce, add.at), Dirichlet filter,
+    and the reference's CG loop (unpreconditioned, rtol 1e-10, which reaches
+    the 1e-8 solution error).  BLAS and OpenMP may use every host core; the
+    sorts, add.at and SciPy's csr_matvec run on one.  Median of `runs`."""
     from oracle import fem as ofem
     from oracle.mesh import tet_from_box
-    ctx = threadpool_limits(limits=1) if threadpool_limits else None
     try:
+        from threadpoolctl import threadpool_info
+        threads = max((int(p.get("num_threads", 1)) for p in threadpool_info()), default=1)
+    except ImportError:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    mesh = tet_from_box(nx=n, ny=n, nz=n)
+    k = np.exp(np.random.default_rng(20261202).standard_normal(mesh.number_of_cells()))
+    u = ofem.cartesian(lambda p: np.sin(np.pi * p[..., 0]) * np.sin(np.pi * p[..., 1]) * np.sin(np.pi * p[..., 2]))
+    f = ofem.barycentric(lambda bcs, index=None: 3 * np.pi ** 2 * k[:, None] * u(mesh.bc_to_point(bcs)))
+    c2d = mesh.cell_to_ipoint(1)
+    gdof = mesh.number_of_global_ipoints(1)
+    times = []
+    for _ in range(runs):
         t0 = time.perf_counter()
-        mesh = tet_from_box(nx=n, ny=n, nz=n)
-        t_mesh = time.perf_counter() - t0
-        k = np.exp(np.random.default_rng(20261202).standard_normal(mesh.number_of_cells()))
-        u = ofem.cartesian(lambda p: np.sin(np.pi * p[..., 0]) * np.sin(np.pi * p[..., 1]) * np.sin(np.pi * p[..., 2]))
-        f = ofem.barycentric(lambda bcs, index=None: 3 * np.pi ** 2 * k[:, None] * u(mesh.bc_to_point(bcs)))
-        t0 = time.perf_counter()
-        c2d = mesh.cell_to_ipoint(1)
-        gdof = mesh.number_of_global_ipoints(1)
         A = ofem.assemble_matrix(c2d, [ofem.diffusion_element(mesh, 1, k, 2)], gdof)
         F = ofem.assemble_vector(c2d, [ofem.source_element(mesh, 1, f, 2)], gdof)
         A2, F2 = ofem.dirichlet_apply(A, F, mesh.is_boundary_dof(1), mesh.interpolation_points(1), u)
@@ -205,13 +299,16 @@ def cpu_baseline(n):
         t0 = time.perf_counter()
         x, info = ofem.cg(A2, F2, atol=0.0, rtol=1e-10, maxit=100000)
         t_solve = time.perf_counter() - t0
-    finally:
-        if ctx is not None:
-            ctx.__exit__(None, None, None)
-    total = t_asm + t_solve
-    return {"value": round(gdof / total, 1), "unit": "DOF/s", "cores": 1, "kind": "port",
-            "sample": f"tet from_box {n}^3 ({gdof} DOFs), assemble+BC {t_asm:.2f}s + CG {info['niter']} it "
-                      f"{t_solve:.2f}s (oracle, 1 thread; mesh build {t_mesh:.2f}s not counted)"}
+        times.append((t_asm + t_solve, t_asm, t_solve, info["niter"]))
+    tot, t_asm, t_solve, it = sorted(times)[len(times) // 2]
+    env = {v: os.environ.get(v) for v in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")}
+    return {"value": round(gdof / tot, 1), "unit": "DOF/s", "cores": threads, "kind": "port",
+            "sample": f"tet from_box {n}^3 ({gdof} DOFs) of the same workload (lognormal k, q=2, Dirichlet, "
+                      f"CG to rtol 1e-10: {it} iterations); median of {runs} runs: assemble+BC {t_asm:.2f} s + "
+                      f"CG {t_solve:.2f} s; the oracle (NumPy/SciPy FP64 restatement of the reference path)",
+            "runs_s": [round(t[0], 3) for t in times], "host_cpus": os.cpu_count(), "cpu_model": _cpu_model(),
+            "thread_env": env, "blas_threads": threads,
+            "single_threaded_steps": "lexsort, np.add.at and SciPy csr_matvec run on one core"}
 
 
 def make_elasticity(n, dev, lam=1.0, mu=0.5):
@@ -378,7 +475,7 @@ def _pcts(lat):
             "mean_ms": round(float(lat.mean()) * 1e3, 2)}
 
 
-def serving_main(args):
+def run_serving(args):
     """Mixed-size serving: `requests` 2D diffusion requests (TriangleMesh.from_box
     nx^2, nx log-uniform in [16, 512], P1/P2 50:50, k = exp(N(0,1)) per cell,
     8 Fourier modes, u = 0 on the boundary), solved to rtol 1e-8 with
@@ -388,16 +485,8 @@ def serving_main(args):
     saturated (every request queued at t=0) for solves/s, (3) the server under
     Poisson arrivals at `rate` for p50/p99 latency (arrival to result); the
     plain loop's latency at that rate follows from its measured service times
-    (a FIFO single server: start = max(arrival, previous done)).
-
-    HIP maps streams onto GPU_MAX_HW_QUEUES hardware queues per process (4 by
-    default); streams beyond that share queues and serialise.  The serving
-    process asks for at least 2 x slots queues (the slots' streams, the
-    default stream and torch's side streams each get one), set before HIP
-    initialises."""
-    want = min(32, max(4, 2 * args.slots))
-    if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < want:
-        os.environ["GPU_MAX_HW_QUEUES"] = str(want)
+    (a FIFO single server: start = max(arrival, previous done)).  Needs
+    GPU_MAX_HW_QUEUES >= 2 x slots, set before HIP initialises (main does)."""
     import torch
     from fealpy_amd.serving import SolveServer, make_requests
     n = args.requests
@@ -406,16 +495,17 @@ def serving_main(args):
     plain = make_requests(n, seed=seed)
     sizes = sorted({(r.nx, r.p) for r in plain})
     t_warm = 0.0
-    for i in range(0, len(sizes), 32):
-        t_warm += srv.warm(sizes[i:i + 32])
-        print(f"# warm {min(i + 32, len(sizes))}/{len(sizes)} sizes, {t_warm:.1f}s", file=sys.stderr, flush=True)
+    for i in range(0, len(sizes), 64):
+        t_warm += srv.warm(sizes[i:i + 64])
+        print(f"# serving warm {min(i + 64, len(sizes))}/{len(sizes)} sizes, {t_warm:.1f}s", file=sys.stderr,
+              flush=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for r in plain:
         srv.solve(r)
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
-    print(f"# plain loop {n / t_plain:.1f} solves/s", file=sys.stderr, flush=True)
+    print(f"# serving plain loop {n / t_plain:.1f} solves/s", file=sys.stderr, flush=True)
     mix = make_requests(n, seed=seed)
     srv.start()
     t0 = time.perf_counter()
@@ -425,7 +515,7 @@ def serving_main(args):
         r.event.wait()
     t_srv = time.perf_counter() - t0
     thr = n / t_srv
-    print(f"# server {thr:.1f} solves/s", file=sys.stderr, flush=True)
+    print(f"# serving server {thr:.1f} solves/s", file=sys.stderr, flush=True)
     same = sum(int(torch.equal(a.x, b.x)) for a, b in zip(plain, mix) if b.error is None)
     errors = sum(1 for r in mix if r.error is not None)
     rate = args.rate if args.rate > 0 else 0.8 * thr
@@ -450,21 +540,29 @@ def serving_main(args):
         done = max(arr, done) + sv
         lat_plain.append(done - arr)
     gdofs = [(r.p * r.nx + 1) ** 2 for r in plain]
-    print(json.dumps({
-        "metric": "serving solves/s and p99 latency, mixed-size 2D diffusion to rtol 1e-8",
-        "value": round(thr, 2), "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
-        "dtype": "f64+f32", "data": "synthetic",
+    iters = [r.info["niter"] for r in plain]
+    return {
+        "solves_per_s": round(thr, 2), "speedup_vs_plain": round(thr * t_plain / n, 3),
+        "p50_ms": _pcts(lat_srv)["p50_ms"], "p99_ms": _pcts(lat_srv)["p99_ms"],
         "config": {"workload": f"{n} requests, TriangleMesh.from_box nx^2 with nx log-uniform in [16, 512], P1/P2, "
                                f"k=exp(N(0,1)) per cell, 8 Fourier modes, u=0 on the boundary",
-                   "slots": args.slots, "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"]), "rtol": args.rtol,
-                   "distinct_sizes": len(sizes),
-                   "mean_gdof": round(float(np.mean(gdofs)), 1), "max_gdof": int(max(gdofs))},
+                   "slots": args.slots, "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
+                   "rtol": args.rtol, "distinct_sizes": len(sizes),
+                   "mean_gdof": round(float(np.mean(gdofs)), 1), "max_gdof": int(max(gdofs)),
+                   "pcg_iterations_mean": round(float(np.mean(iters)), 2), "pcg_iterations_max": int(max(iters))},
         "plain_loop": {"solves_per_s": round(n / t_plain, 2), **_pcts(lat_plain)},
-        "server": {"solves_per_s": round(thr, 2), "speedup_vs_plain": round(thr * t_plain / n, 3),
-                   "arrival_rate": round(rate, 2), **_pcts(lat_srv)},
+        "server": {"solves_per_s": round(thr, 2), "arrival_rate": round(rate, 2), **_pcts(lat_srv)},
         "results_identical_alone_and_in_mix": f"{same}/{n}", "request_errors": errors,
         "cold": {"warm_s": round(t_warm, 2)},
-        "dof_per_s_server": round(sum(gdofs) / t_srv, 1)}))
+        "dof_per_s_server": round(sum(gdofs) / t_srv, 1)}
+
+
+def serving_main(args):
+    s = run_serving(args)
+    print(json.dumps({
+        "metric": "serving solves/s and p99 latency, mixed-size 2D diffusion to rtol 1e-8",
+        "value": s["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f64+f32+f16", "data": "synthetic", "config": s.pop("config"), **s}))
 
 
 def main():
@@ -473,6 +571,10 @@ def main():
         print(f"bench.py measures one MI355X: the multi-GPU path is a design only, not code (asked --gpus {args.gpus})",
               file=sys.stderr)
         sys.exit(2)
+    # hardware queues for the serving streams: HIP reads this when it initialises
+    want = min(32, max(SERVING_HW_QUEUES, 2 * args.slots))
+    if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < want:
+        os.environ["GPU_MAX_HW_QUEUES"] = str(want)
     if args.spmv_only:
         spmv_only(args)
         return
@@ -490,7 +592,7 @@ def main():
     from fealpy_amd.solver import GeometricMG
     dev = torch.device("cuda:0")
     t0 = time.perf_counter()
-    mesh, space, k, u, f = make_problem(args.n, dev)
+    mesh, space, k, u, f = make_problem(args.n, dev, args.torch_source)
     torch.cuda.synchronize()
     t_mesh = time.perf_counter() - t0
     gdof = space.number_of_global_dofs()
@@ -518,13 +620,13 @@ def main():
               file=sys.stderr)
     phases_m = phase_breakdown(space, k, u, f, mg32, args.rtol)
     phases_d = phase_breakdown(space, k, u, f, mg64, args.rtol)
+    kt = in_step_kernels(space, k, u, f, mg32, args.rtol)
     err_l2 = float(mesh.error(u, space.function(x_m)))
-    roof = spmv_roofline(A2, torch.float32)
+    nnz_a = pat.nnz
+    roof, roof_asm = rooflines(kt, mesh, gdof, nnz_a, int(A2.nnz), gdof)
+    roof32 = spmv_roofline(A2, torch.float32)
     if args.n == 128:
-        roof["traffic"] = SPMV_F32_TRAFFIC_128
-    roof64 = spmv_roofline(A2, torch.float64)
-    cpu = None if args.no_cpu else cpu_baseline(args.cpu_n)
-
+        roof32["traffic"] = SPMV_F32_TRAFFIC_128
     out = {
         "metric": "DOF/s assembled+solved to 1e-8 rel err, mixed vs FP64 GPU; serving p99",
         "value": round(gdof / t_mixed, 1),
@@ -536,11 +638,12 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64+f32",
+        "dtype": "f64+f32+f16",
         "data": "synthetic",
         "config": {"workload": f"3D P1 diffusion, TetrahedronMesh.from_box {args.n}^3, lognormal per-cell k, "
                                f"q=2, multigrid-preconditioned CG", "gdof": gdof,
-                   "ncell": mesh.number_of_cells(), "nnz": int(A2.nnz), "rtol": args.rtol},
+                   "ncell": mesh.number_of_cells(), "nnz": int(A2.nnz), "rtol": args.rtol,
+                   "source": "torch callable" if args.torch_source else "separable expression, in-kernel"},
         "fp64": {"value": round(gdof / t_fp64, 1), "ms_per_step": round(t_fp64 * 1e3, 3), "rel_err": err_d,
                  "pcg_iterations": info_d["niter"], "phases": phases_d},
         "rel_err_ok": bool(max(err_m, err_d) <= 1e-8),
@@ -551,9 +654,17 @@ def main():
                  "dof_per_s_with_pattern": round(gdof / (t_mixed + t_pattern), 1)},
         "reference_solve": {"pcg_iterations": info_ref["niter"]},
         "roofline": roof,
-        "roofline_fp64_spmv": roof64,
-        "cpu_baseline": cpu,
+        "roofline_assembly": roof_asm,
+        "roofline_sell_spmv_f32": roof32,
     }
+    del x_ref, x_m, x_d, A2, F2, mg32, mg64, pat, bc, space, mesh, k
+    torch.cuda.empty_cache()
+    out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.cpu_n, args.cpu_runs)
+    if not args.no_serving:
+        try:
+            out["serving"] = run_serving(args)
+        except Exception as e:          # the headline figures stand on their own
+            out["serving"] = {"error": f"{type(e).__name__}: {e}"}
     print(json.dumps(out))
 
 

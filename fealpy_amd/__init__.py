@@ -7,6 +7,7 @@ DirichletBC -> CSRTensor -> cg, on torch device tensors, with every numerical
 step in hand-written HIP kernels for gfx950 (libfealpy_amd.so, C-ABI in
 include/fealpy_amd.h).
 """
+from . import expr
 from .decorator import barycentric, cartesian
 from .fem import BilinearForm, DirichletBC, GroupIntegrator, LinearElasticityIntegrator, LinearForm, \
     ScalarDiffusionIntegrator, ScalarMassIntegrator, ScalarSourceIntegrator, VectorSourceIntegrator

@@ -15,7 +15,9 @@ LIB_PATH = os.path.join(_HERE, "libfealpy_amd.so")
 
 FA_OK, FA_EINVAL, FA_ENOMEM, FA_EHIP, FA_ENOCONV, FA_ECAPACITY = range(6)
 FA_F64, FA_F32 = 0, 1
-COEF_NONE, COEF_CONST, COEF_CELL, COEF_CELLQ = 0, 1, 2, 3
+(TIMER_PCG_SPMV, TIMER_P1_ROWS, TIMER_P1_SUM_ROWS, TIMER_P1_VECTORS, TIMER_P1_SUM_VALUES, TIMER_BC_APPLY,
+ TIMER_SELL_SPMV, TIMER_VCYCLE) = range(1, 9)
+COEF_NONE, COEF_CONST, COEF_CELL, COEF_CELLQ, COEF_SEP = 0, 1, 2, 3, 4
 
 _p = ctypes.c_void_p
 _i32 = ctypes.c_int32
@@ -99,6 +101,8 @@ _SIGNATURES = {
     "fa_pcg": (ctypes.c_int, [ctypes.c_int, _p, _p, _p, _p, _f64, _f64, _i64, _i32, _pi64, _pf64, _p, _sz, _p]),
     "fa_pcg_graph_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
     "fa_pcg_graph_destroy": (None, [_p]),
+    "fa_timer_arm": (ctypes.c_int, [ctypes.c_uint]),
+    "fa_timer_read": (ctypes.c_int, [ctypes.c_int, _pi64, _pf64]),
     "fa_pcg_graph": (ctypes.c_int, [ctypes.c_int, _p, _p, _p, _p, _f64, _f64, _i64, _i32, _pi64, _pf64, _p, _sz, _p,
                                     _p]),
 }
@@ -166,3 +170,19 @@ def dtype_code(dt):
 
 def workspace(nbytes, device):
     return torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+
+
+def timer_arm(*kinds):
+    """Arm the in-step kernel timer for the FA_TIMER_* kinds (none: disarm)."""
+    mask = 0
+    for k in kinds:
+        mask |= 1 << int(k)
+    check(lib().fa_timer_arm(mask), "fa_timer_arm")
+
+
+def timer_read(kind):
+    """(launches, total ms) of one armed kind since the last arming."""
+    n = ctypes.c_int64(0)
+    ms = ctypes.c_double(0.0)
+    check(lib().fa_timer_read(int(kind), ctypes.byref(n), ctypes.byref(ms)), "fa_timer_read")
+    return int(n.value), float(ms.value)

@@ -97,6 +97,65 @@ __device__ inline double coef_value(int kind, double scale, const double* __rest
     }
 }
 
+// ---- separable coefficients (FA_COEF_SEP) -----------------------------------
+// One factor g(s) of a term; every lane runs the same program (the struct is
+// read through uniform loads), so the switch does not diverge.
+__device__ inline double sep_factor(int kind, double a, double b, double s) {
+    switch (kind) {
+        case FA_SEP_SIN: return sin(a * s + b);
+        case FA_SEP_COS: return cos(a * s + b);
+        case FA_SEP_EXP: return exp(a * s + b);
+        case FA_SEP_AFFINE: return a * s + b;
+        default: {  // FA_SEP_POW: s^a, a a whole number
+            const int k = static_cast<int>(a);
+            double r = 1.0;
+            for (int i = 0; i < k; ++i) r *= s;
+            return r;
+        }
+    }
+}
+
+// f(x) at quadrature point q of cell c with vertices p: x = sum_v bcs[q][v] p_v
+// (the order of bc_to_point), then the sum of the terms in order
+template <int TD>
+__device__ inline double sep_value(const fa_sep_t* __restrict__ e, int q, const double p[TD + 1][TD], int64_t c) {
+    double x[TD];
+#pragma unroll
+    for (int d = 0; d < TD; ++d) x[d] = 0.0;
+#pragma unroll
+    for (int v = 0; v < TD + 1; ++v) {
+        const double w = e->bcs[q * (TD + 1) + v];
+#pragma unroll
+        for (int d = 0; d < TD; ++d) x[d] += w * p[v][d];
+    }
+    double acc = 0.0;
+    int k = 0;
+    const int nt = e->nterms;
+    for (int t = 0; t < nt; ++t) {
+        double v = e->coef[t];
+        const double* fld = e->field[t];
+        if (fld) v *= fld[c];
+        const int kend = e->term_end[t];
+        for (; k < kend; ++k) {
+            const int d = e->dim[k];
+            const double s = (TD == 2) ? (d == 0 ? x[0] : x[TD - 1]) : (d == 0 ? x[0] : (d == 1 ? x[1] : x[TD - 1]));
+            v *= sep_factor(e->kind[k], e->a[k], e->b[k], s);
+        }
+        acc += v;
+    }
+    return acc;
+}
+
+// coefficient at quadrature point q of cell c: `ci` indexes the per-cell /
+// per-point arrays (which may be a block's staged copy), `cg` is the global
+// cell id (fields of a separable coefficient)
+template <int TD>
+__device__ inline double coef_at(int kind, double scale, const double* __restrict__ coef, int64_t ci, int64_t cg,
+                                 int q, int nq, const double p[TD + 1][TD]) {
+    if (kind == FA_COEF_SEP) return scale * sep_value<TD>(reinterpret_cast<const fa_sep_t*>(coef), q, p, cg);
+    return coef_value(kind, scale, coef, ci, q, nq);
+}
+
 // One thread per (cell, row i); the row of the element matrix is kept in
 // registers (LDOF is a compile-time constant).
 template <int TD, int LDOF, class T>
@@ -119,7 +178,7 @@ __global__ void __launch_bounds__(256) k_element_matrices(const double* __restri
     for (int q = 0; q < nq; ++q) {
         const double wq = qw[q] * g.meas;
         if (dk) {
-            const double kd = wq * coef_value(dk, ds, dc, c, q, nq);
+            const double kd = wq * coef_at<TD>(dk, ds, dc, c, c, q, nq, p);
             const double* R = gphi + (int64_t)q * LDOF * (TD + 1);
             double gi[TD];
 #pragma unroll
@@ -143,7 +202,7 @@ __global__ void __launch_bounds__(256) k_element_matrices(const double* __restri
             }
         }
         if (mk) {
-            const double km = wq * coef_value(mk, ms, mc, c, q, nq);
+            const double km = wq * coef_at<TD>(mk, ms, mc, c, c, q, nq, p);
             const double* ph = phi + (int64_t)q * LDOF;
             const double pi = ph[i] * km;
 #pragma unroll
@@ -169,7 +228,7 @@ __global__ void __launch_bounds__(256) k_element_vectors(const double* __restric
     Geom<TD> g;
     g.compute(p);
     double acc = 0.0;
-    for (int q = 0; q < nq; ++q) acc += qw[q] * g.meas * coef_value(sk, ss, src, c, q, nq) * phi[(int64_t)q * ldof + i];
+    for (int q = 0; q < nq; ++q) acc += qw[q] * g.meas * coef_at<TD>(sk, ss, src, c, c, q, nq, p) * phi[(int64_t)q * ldof + i];
     elem[t] = static_cast<T>(acc);
 }
 
@@ -509,14 +568,15 @@ __device__ inline void P1Cell<2>::compute(const double p[3][2]) {
 }
 
 template <int TD>
-__device__ inline void p1_row(const P1Cell<TD>& g, int i, int64_t c, int nq, const double* __restrict__ qw,
+__device__ inline void p1_row(const P1Cell<TD>& g, const double p[TD + 1][TD], int64_t cg, int i, int64_t c, int nq,
+                              const double* __restrict__ qw,
                               const double* __restrict__ phi, int dk, double ds, const double* __restrict__ dc, int mk,
                               double ms, const double* __restrict__ mc, double out[TD + 1]) {
 #pragma unroll
     for (int j = 0; j < TD + 1; ++j) out[j] = 0.0;
     if (dk) {
         double kd = 0.0;
-        for (int q = 0; q < nq; ++q) kd += qw[q] * coef_value(dk, ds, dc, c, q, nq);
+        for (int q = 0; q < nq; ++q) kd += qw[q] * coef_at<TD>(dk, ds, dc, c, cg, q, nq, p);
         const double s = kd * g.meas / (g.den * g.den);
         double ni[TD];  // row i selected without a runtime index into g.n (keeps it in registers)
 #pragma unroll
@@ -538,7 +598,7 @@ __device__ inline void p1_row(const P1Cell<TD>& g, int i, int64_t c, int nq, con
     if (mk) {
         for (int q = 0; q < nq; ++q) {
             const double* ph = phi + q * (TD + 1);
-            const double w = qw[q] * g.meas * coef_value(mk, ms, mc, c, q, nq) * ph[i];
+            const double w = qw[q] * g.meas * coef_at<TD>(mk, ms, mc, c, cg, q, nq, p) * ph[i];
 #pragma unroll
             for (int j = 0; j < TD + 1; ++j) out[j] += w * ph[j];
         }
@@ -592,7 +652,7 @@ __global__ void __launch_bounds__(256) k_assemble_p1(int64_t gdof, const double*
                 P1Cell<TD> g;
                 g.compute(p);
                 double v[NV];
-                p1_row<TD>(g, i, c, nq, qw, phi, dk, ds, dc, mk, ms, mc, v);
+                p1_row<TD>(g, p, c, i, c, nq, qw, phi, dk, ds, dc, mk, ms, mc, v);
 #pragma unroll
                 for (int j = 0; j < NV; ++j) {
                     pos[j] = lower_bound_lds(cols, L, cell[c * NV + j]);
@@ -651,7 +711,7 @@ __global__ void __launch_bounds__(256) k_assemble_p1_vec(int64_t gdof, const dou
             P1Cell<TD> g;
             g.compute(p);
             double v = 0.0;
-            for (int q = 0; q < nq; ++q) v += qw[q] * g.meas * coef_value(sk, ss, src, c, q, nq) * phi[q * NV + i];
+            for (int q = 0; q < nq; ++q) v += qw[q] * g.meas * coef_at<TD>(sk, ss, src, c, c, q, nq, p) * phi[q * NV + i];
             s += v;
         }
         s = group_sum<G>(s);
@@ -1001,6 +1061,7 @@ __global__ void __launch_bounds__(256) k_incidence_maps(int64_t gdof, int ldof, 
 // the kernels are bound by the texture address unit, so fewer, wider memory
 // instructions are what counts.  Values equal the fused kernel's bit for bit.
 constexpr int P1_BLOCK = 256;
+constexpr int P1_MAX_NQ = 8;  // source values per cell kept in registers by the load-vector pass
 
 __device__ inline size_t coef_stage_len(int kind, int nq) {
     return kind == FA_COEF_CELL ? 1 : (kind == FA_COEF_CELLQ ? static_cast<size_t>(nq) : 0);
@@ -1066,12 +1127,13 @@ __global__ void __launch_bounds__(P1_BLOCK) k_p1_incidence_rows(int64_t nc, cons
             P1Cell<TD> g;
             g.compute(p);
             // coefficients through the staged block: cell index t, row stride ld / lm
-            const double* dcl = ld ? dS : nullptr;
-            const double* mcl = lm ? mS : nullptr;
+            // (a separable coefficient reads its fields at the global cell id)
+            const double* dcl = ld ? dS : (dk == FA_COEF_SEP ? dc : nullptr);
+            const double* mcl = lm ? mS : (mk == FA_COEF_SEP ? mc : nullptr);
 #pragma unroll
             for (int i = 0; i < NV; ++i) {
                 double v[NV];
-                p1_row<TD>(g, i, t, nq, qw, phi, dk, ds, dcl, mk, ms, mcl, v);
+                p1_row<TD>(g, p, c0 + t, i, t, nq, qw, phi, dk, ds, dcl, mk, ms, mcl, v);
                 double* out = rows + static_cast<int64_t>(iS[t * NV + i]) * NV;
 #pragma unroll
                 for (int j = 0; j < NV; ++j) out[j] = v[j];
@@ -1107,12 +1169,30 @@ __global__ void __launch_bounds__(P1_BLOCK) k_p1_incidence_vectors(int64_t nc, c
             load_vertices_wide<TD>(nodew, vS + t * NV, p);
             P1Cell<TD> g;
             g.compute(p);
-            const double* sl = ls ? sS : nullptr;
+            const double* sl = ls ? sS : (sk == FA_COEF_SEP ? src : nullptr);
+            if (nq <= P1_MAX_NQ) {
+                // w_q |T| f(x_q) once per point (a separable source is evaluated
+                // once, not once per basis function), then the moments
+                double fq[P1_MAX_NQ];
 #pragma unroll
-            for (int i = 0; i < NV; ++i) {
-                double v = 0.0;
-                for (int q = 0; q < nq; ++q) v += qw[q] * g.meas * coef_value(sk, ss, sl, t, q, nq) * phi[q * NV + i];
-                vals[iS[t * NV + i]] = v;
+                for (int q = 0; q < P1_MAX_NQ; ++q)
+                    fq[q] = q < nq ? qw[q] * g.meas * coef_at<TD>(sk, ss, sl, t, c0 + t, q, nq, p) : 0.0;
+#pragma unroll
+                for (int i = 0; i < NV; ++i) {
+                    double v = 0.0;
+#pragma unroll
+                    for (int q = 0; q < P1_MAX_NQ; ++q)
+                        if (q < nq) v += fq[q] * phi[q * NV + i];
+                    vals[iS[t * NV + i]] = v;
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < NV; ++i) {
+                    double v = 0.0;
+                    for (int q = 0; q < nq; ++q)
+                        v += qw[q] * g.meas * coef_at<TD>(sk, ss, sl, t, c0 + t, q, nq, p) * phi[q * NV + i];
+                    vals[iS[t * NV + i]] = v;
+                }
             }
         }
         __syncthreads();
@@ -1231,8 +1311,10 @@ static int launch_sum_rows(int64_t gdof, const int32_t* adj_ptr, const int32_t* 
         return FA_ECAPACITY;
     }
     const unsigned grid = grid_for(gdof, rpb, 1 << 20);
+    timer_begin(FA_TIMER_P1_SUM_ROWS, s);
     hipLaunchKernelGGL((k_sum_incidence_rows<NV, G, K, T>), dim3(grid), dim3(rpb * G), lds, s, gdof, adj_ptr, indptr,
                        pos, rows, lmax, static_cast<T*>(values));
+    timer_end(FA_TIMER_P1_SUM_ROWS, s);
     FA_LAUNCH("k_sum_incidence_rows");
     return FA_OK;
 }
@@ -1283,12 +1365,14 @@ extern "C" int fa_p1_incidence_rows(int td, int64_t nc, int nq, const double* no
         return FA_ECAPACITY;
     }
     const unsigned grid = grid_for(nc, P1_BLOCK, 1 << 20);
+    timer_begin(FA_TIMER_P1_ROWS, S(stream));
     if (td == 3)
         hipLaunchKernelGGL(k_p1_incidence_rows<3>, dim3(grid), dim3(P1_BLOCK), lds, S(stream), nc, node_wide, cell, nq,
                            qw, phi, diff_kind, diff_scale, diff_coef, mass_kind, mass_scale, mass_coef, inc, rows);
     else
         hipLaunchKernelGGL(k_p1_incidence_rows<2>, dim3(grid), dim3(P1_BLOCK), lds, S(stream), nc, node_wide, cell, nq,
                            qw, phi, diff_kind, diff_scale, diff_coef, mass_kind, mass_scale, mass_coef, inc, rows);
+    timer_end(FA_TIMER_P1_ROWS, S(stream));
     FA_LAUNCH("k_p1_incidence_rows");
     return FA_OK;
 }
@@ -1333,12 +1417,14 @@ extern "C" int fa_p1_incidence_vectors(int td, int64_t nc, int nq, const double*
         return FA_ECAPACITY;
     }
     const unsigned grid = grid_for(nc, P1_BLOCK, 1 << 20);
+    timer_begin(FA_TIMER_P1_VECTORS, S(stream));
     if (td == 3)
         hipLaunchKernelGGL(k_p1_incidence_vectors<3>, dim3(grid), dim3(P1_BLOCK), lds, S(stream), nc, node_wide, cell,
                            nq, qw, phi, src_kind, src_scale, src, inc, vals);
     else
         hipLaunchKernelGGL(k_p1_incidence_vectors<2>, dim3(grid), dim3(P1_BLOCK), lds, S(stream), nc, node_wide, cell,
                            nq, qw, phi, src_kind, src_scale, src, inc, vals);
+    timer_end(FA_TIMER_P1_VECTORS, S(stream));
     FA_LAUNCH("k_p1_incidence_vectors");
     return FA_OK;
 }
@@ -1348,12 +1434,14 @@ extern "C" int fa_sum_incidence_values(int dtype, int64_t gdof, const int32_t* a
     FA_REQUIRE(dtype == FA_F64 || dtype == FA_F32, "fa_sum_incidence_values: bad dtype %d", dtype);
     FA_REQUIRE(gdof > 0 && adj_ptr && vals && out, "fa_sum_incidence_values: bad arguments");
     const unsigned grid = grid_for(gdof, 256, 1 << 20);
+    timer_begin(FA_TIMER_P1_SUM_VALUES, S(stream));
     if (dtype == FA_F64)
         hipLaunchKernelGGL(k_sum_incidence_values<double>, dim3(grid), dim3(256), 0, S(stream), gdof, adj_ptr, vals,
                            static_cast<double*>(out));
     else
         hipLaunchKernelGGL(k_sum_incidence_values<float>, dim3(grid), dim3(256), 0, S(stream), gdof, adj_ptr, vals,
                            static_cast<float*>(out));
+    timer_end(FA_TIMER_P1_SUM_VALUES, S(stream));
     FA_LAUNCH("k_sum_incidence_values");
     return FA_OK;
 }

@@ -13,6 +13,12 @@ constexpr int WAVE = 64;
 
 void set_error(const char* fmt, ...);
 
+// In-step kernel timer (fa_timer_arm / fa_timer_read): when the kernel family
+// `kind` (FA_TIMER_*) is armed, timer_begin / timer_end record HIP events on
+// `s` around its launch; otherwise they do nothing.
+void timer_begin(int kind, hipStream_t s);
+void timer_end(int kind, hipStream_t s);
+
 inline hipStream_t S(fa_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
 
 inline int launch_status(const char* what) {

@@ -1278,6 +1278,7 @@ __global__ void k_pcg_set(PcgState* st, double atol, double rtol, long long maxi
     st->maxit = maxit;
     st->done = 0;
     st->niter = 0;
+    st->rnorm = 0.0;  // the residual reported when ||b|| < 1e-15 stops the solve before any iteration
 }
 
 // A captured PCG iteration: the kernels of one iteration recorded on a private
@@ -1370,11 +1371,15 @@ static int pcg_run(const fa_sell_t* A, const fa_mg_t* M, const T* b, T* x, doubl
     // writing r.r partials (no preconditioner) into the first
     double* pap = partials + RED_GRID;
     auto iteration = [&](hipStream_t q) -> int {
+        timer_begin(FA_TIMER_PCG_SPMV, q);
         hipLaunchKernelGGL(k_pcg_spmv<T>, grid, block, 0, q, n, A->slice_ptr, A->cols, offs, sv, p, Ap, pap, st);
+        timer_end(FA_TIMER_PCG_SPMV, q);
         if (M) {
             hipLaunchKernelGGL((k_pcg_update<T, U, true>), grid, block, 0, q, n, x, r, ru, p, Ap, pap, np, partials,
                                st);
+            timer_begin(FA_TIMER_VCYCLE, q);
             int e = vcycle_any(M, ru, z, done, q, &rz_out);  // z = M r and the partials of r.z
+            timer_end(FA_TIMER_VCYCLE, q);
             if (e) return e;
             hipLaunchKernelGGL(k_pcg_finish<PCG_CHECK>, one, block, 0, q, partials, np, st);
             hipLaunchKernelGGL((k_pcg_pupdate<T, U>), grid, block, 0, q, n, p, z, st);

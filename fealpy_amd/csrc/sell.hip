@@ -302,6 +302,7 @@ extern "C" int fa_sell_spmv(int dtype, const fa_sell_t* A, const void* x, void* 
     FA_REQUIRE(A && A->n > 0, "fa_sell_spmv: empty matrix");
     hipStream_t s = S(stream);
     const unsigned grid = grid_for(((A->n + 63) / 64) * 64, 256, 1 << 16);
+    timer_begin(FA_TIMER_SELL_SPMV, s);
     if (dtype == FA_F64)
         hipLaunchKernelGGL(k_sell_spmv<double>, dim3(grid), dim3(256), 0, s, A->n, A->slice_ptr, A->cols,
                            static_cast<const double*>(A->vals), static_cast<const double*>(x),
@@ -313,6 +314,7 @@ extern "C" int fa_sell_spmv(int dtype, const fa_sell_t* A, const void* x, void* 
         set_error("fa_sell_spmv: bad dtype %d", dtype);
         return FA_EINVAL;
     }
+    timer_end(FA_TIMER_SELL_SPMV, s);
     FA_LAUNCH("k_sell_spmv");
     return FA_OK;
 }

@@ -274,6 +274,7 @@ extern "C" int fa_dirichlet_apply(int dtype, int64_t n, const int32_t* indptr, c
     FA_REQUIRE(dtype == FA_F64 || dtype == FA_F32, "fa_dirichlet_apply: bad dtype %d", dtype);
     hipStream_t s = S(stream);
     const unsigned grid = grid_for(n * BCG, 256, 1 << 16);
+    timer_begin(FA_TIMER_BC_APPLY, s);
     if (dtype == FA_F64)
         hipLaunchKernelGGL(k_bc_apply<double>, dim3(grid), dim3(256), 0, s, n, indptr, indices,
                            static_cast<const double*>(values), is_bd, new_indptr, static_cast<double*>(new_values),
@@ -282,6 +283,7 @@ extern "C" int fa_dirichlet_apply(int dtype, int64_t n, const int32_t* indptr, c
         hipLaunchKernelGGL(k_bc_apply<float>, dim3(grid), dim3(256), 0, s, n, indptr, indices,
                            static_cast<const float*>(values), is_bd, new_indptr, static_cast<float*>(new_values),
                            static_cast<const float*>(uh), static_cast<const float*>(f), static_cast<float*>(f_out));
+    timer_end(FA_TIMER_BC_APPLY, s);
     FA_LAUNCH("k_bc_apply");
     return FA_OK;
 }

@@ -85,9 +85,11 @@ class DirichletBC:
 
     def _boundary_values(self, f, uh, gd, dtype):
         if uh is None:
-            uh = torch.zeros_like(f)
+            uh = torch.zeros(f.shape, dtype=dtype, device=f.device)
         uh, _ = self.space.boundary_interpolate(gd=gd, uh=uh, threshold=self.threshold, method=self.method)
         g = uh.array if hasattr(uh, 'array') else uh
+        if g.shape != f.shape:
+            raise ValueError(f"uh of shape {tuple(g.shape)} does not match f of shape {tuple(f.shape)}")
         return g.to(dtype).contiguous()
 
     def _csr(self, A):
@@ -145,7 +147,14 @@ class DirichletBC:
             raise RuntimeError("The boundary condition is None.")
         A = self._csr(A)
         g = self._boundary_values(f, uh, gd, A.dtype)
-        f = f.to(A.dtype).contiguous()
+        f = f.to(A.dtype)
+        if f.ndim == 2:             # one right-hand side per column: f - A uh column by column
+            cols = [self._rhs(A, f[:, j].contiguous(), g[:, j].contiguous()) for j in range(f.shape[1])]
+            return torch.stack(cols, dim=1) if cols else torch.empty_like(f)
+        return self._rhs(A, f.contiguous(), g)
+
+    def _rhs(self, A, f, g):
+        """f - A g with f[bd] = g[bd] for one contiguous column (fa_dirichlet_rhs)."""
         out = torch.empty_like(f)
         nat.check(nat.lib().fa_dirichlet_rhs(nat.dtype_code(A.dtype), A.shape[0], nat.ptr(A.crow), nat.ptr(A.col),
                                              nat.ptr(A.values), nat.ptr(self._bd_u8), nat.ptr(g), nat.ptr(f),

@@ -9,6 +9,7 @@ element kernels (``fa_element_matrices`` / ``fa_element_vectors``).
 import torch
 
 from .. import _native as nat
+from ..expr import Affine, Sep
 
 _S = slice(None)
 
@@ -145,17 +146,29 @@ def _axpby_sum(a, b):
 
 
 # ---- coefficients (fealpy/utils/utils.py:21-53) ---------------------------------
-def coef_spec(coef, space, q, selection=_S):
-    """-> (kind, scale, tensor-or-None) for the element kernels."""
+def coef_spec(coef, space, q, selection=_S, allow_sep=False):
+    """-> (kind, scale, tensor-or-None) for the element kernels.  A separable
+    expression (fealpy_amd.expr.Sep) goes to kernels that evaluate it
+    in registers (`allow_sep`, all cells); elsewhere it is evaluated with
+    torch at the quadrature points like any cartesian callable."""
     if coef is None:
         return nat.COEF_CONST, 1.0, None
     if isinstance(coef, (int, float)):
         return nat.COEF_CONST, float(coef), None
     mesh = space.mesh
+    if isinstance(coef, Affine):
+        coef = Sep.of(coef)
+    if isinstance(coef, Sep):
+        if coef.max_dim() >= mesh.GD:
+            raise ValueError(f"the expression uses coordinate {coef.max_dim()} of a {mesh.GD}-D mesh")
+        if allow_sep and isinstance(selection, slice) and selection == _S:
+            return nat.COEF_SEP, 1.0, coef.device_table(space.bcs_device(q), mesh.number_of_cells())
     if callable(coef) and not torch.is_tensor(coef):
         bcs = space.tables(q)[0]
         if getattr(coef, 'coordtype', 'barycentric') == 'barycentric':
             val = coef(bcs.to(space.device), index=selection)
+        elif isinstance(coef, Sep):
+            val = coef(mesh.bc_to_point(bcs, index=selection), index=None if selection is _S else selection)
         else:
             val = coef(mesh.bc_to_point(bcs, index=selection))
         coef = val
@@ -209,14 +222,15 @@ def _scalar_elements(space, ops, *, indices=None, selection=_S, dtype=None):
         for op, it in ops:
             spec[op] = coef_spec(it.coef, space, q)
         elem = element_matrices_mfma(space, spec['diff'], spec['mass'], q, MFMA_METHODS[next(iter(methods))])
-        return elem if dtype in (None, torch.float32) else elem.to(dtype)
+        # the MFMA tensor is FP32; like every other path the default value type is FP64
+        return elem if dtype == torch.float32 else elem.to(torch.float64 if dtype is None else dtype)
     bcs, qw, phi, gphi = space.tables(q)
     cell = mesh.cell[selection].to(torch.int32).contiguous()
     nc = cell.shape[0]
     ldof = space.number_of_local_dofs()
     spec = {'diff': (nat.COEF_NONE, 0.0, None), 'mass': (nat.COEF_NONE, 0.0, None)}
     for op, it in ops:
-        spec[op] = coef_spec(it.coef, space, q, selection)
+        spec[op] = coef_spec(it.coef, space, q, selection, allow_sep=True)
     dtype = torch.float64 if dtype is None else dtype
     elem = torch.empty((nc, ldof, ldof), dtype=dtype, device=space.device)
     (dk, ds, dc), (mk, ms, mc) = spec['diff'], spec['mass']
@@ -284,7 +298,7 @@ class ScalarSourceIntegrator(_ScalarCellInt):
         cell = mesh.cell[sel].to(torch.int32).contiguous()
         nc = cell.shape[0]
         ldof = space.number_of_local_dofs()
-        kind, scale, src = coef_spec(self.source, space, q, sel)
+        kind, scale, src = coef_spec(self.source, space, q, sel, allow_sep=True)
         if self.source is None:
             kind, scale, src = nat.COEF_CONST, 1.0, None
         dtype = torch.float64 if dtype is None else dtype
@@ -327,7 +341,7 @@ def p1_matrix_values(space, ops, pat, dtype):
     cell = space.cell_to_dof()
     spec = {'diff': (nat.COEF_NONE, 0.0, None), 'mass': (nat.COEF_NONE, 0.0, None)}
     for op, it in ops:
-        spec[op] = coef_spec(it.coef, space, q)
+        spec[op] = coef_spec(it.coef, space, q, allow_sep=True)
     (dk, ds, dc), (mk, ms, mc) = spec['diff'], spec['mass']
     values = torch.empty(pat.nnz, dtype=dtype, device=space.device)
     maps = pat.incidence_maps() if pat.nc >= TWO_PASS_MIN_CELLS else None
@@ -357,7 +371,7 @@ def p1_vector_values(space, it, pat, dtype):
     mesh = space.mesh
     q = it.q if it.q is not None else space.p + 3
     _, qw, phi, _ = space.tables(q)
-    kind, scale, src = coef_spec(it.source, space, q)
+    kind, scale, src = coef_spec(it.source, space, q, allow_sep=True)
     if it.source is None:
         kind, scale, src = nat.COEF_CONST, 1.0, None
     out = torch.empty(pat.gdof, dtype=dtype, device=space.device)

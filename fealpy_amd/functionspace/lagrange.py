@@ -200,5 +200,14 @@ class LagrangeFESpace:
             self._tables[key] = (bcs, qw, phi, gphi)
         return self._tables[key]
 
+    def bcs_device(self, q):
+        """Barycentric points of rule q, (NQ, TD+1) FP64 on the device."""
+        key = ('bcs', int(q))
+        if key not in self._tables:
+            bcs = self.tables(q)[0]
+            self._tables[key] = torch.as_tensor(_as_points_array(bcs), dtype=torch.float64,
+                                                device=self.device).contiguous()
+        return self._tables[key]
+
     def multi_index_matrix(self):
         return multi_index_matrix(self.p, self.TD)

@@ -31,7 +31,8 @@ import numpy as np
 import torch
 
 from .fem import BilinearForm, DirichletBC, LinearForm, ScalarDiffusionIntegrator, ScalarSourceIntegrator
-from .decorator import cartesian
+from . import expr as ex
+from .expr import sin
 from .functionspace import LagrangeFESpace
 from .mesh import TriangleMesh
 from .solver import GeometricMG, cg
@@ -58,14 +59,13 @@ class SolveRequest:
         return None if self.done is None else self.done - self.arrival
 
 
-def fourier_source(modes, device):
-    m = torch.as_tensor(np.asarray(modes, dtype=np.float64), device=device)
-    a, kx, ky = m[:, 0], m[:, 1], m[:, 2]
-
-    @cartesian
-    def f(p):
-        x, y = p[..., 0, None], p[..., 1, None]
-        return (a * torch.sin(kx * PI * x) * torch.sin(ky * PI * y)).sum(dim=-1)
+def fourier_source(modes, device=None):
+    """f = sum_m a_m sin(kx_m pi x) sin(ky_m pi y) as a separable expression:
+    the load-vector kernel evaluates it at the quadrature points itself."""
+    f = None
+    for a, kx, ky in np.asarray(modes, dtype=np.float64):
+        t = float(a) * sin(float(kx) * PI * ex.x) * sin(float(ky) * PI * ex.y)
+        f = t if f is None else f + t
     return f
 
 

@@ -42,6 +42,12 @@ def cg(A, b, x0=None, M=None, *, batch_first=False, atol=1e-12, rtol=1e-8, maxit
     return (x, info) if returninfo else x
 
 
+def maxit_arg(maxit):
+    """maxit for fa_pcg: None -> no cap (-1); 0 stops after one iteration, as
+    the reference loop tests niter >= maxit after its first update."""
+    return -1 if maxit is None else max(int(maxit), 1)
+
+
 def _cg_native(A, b, x0, atol, rtol, maxit, check_every):
     """Plain CG on the SELL copy of A (fa_pcg with no preconditioner)."""
     import ctypes
@@ -57,7 +63,7 @@ def _cg_native(A, b, x0, atol, rtol, maxit, check_every):
     res = ctypes.c_double(0.0)
     d = S.desc(dt)
     st = L.fa_pcg(nat.dtype_code(dt), ctypes.byref(d), None, nat.ptr(b), nat.ptr(x), float(atol), float(rtol),
-                  int(maxit if maxit is not None else 0), int(check_every), ctypes.byref(niter), ctypes.byref(res),
+                  maxit_arg(maxit), int(check_every), ctypes.byref(niter), ctypes.byref(res),
                   nat.ptr(ws), ws.numel(), nat.stream())
     if st != nat.FA_ENOCONV:
         nat.check(st, "fa_pcg")

@@ -33,6 +33,7 @@ import torch
 from .. import _native as nat
 from ..sparse.sell import FaSell, FaSell16, SellMatrix
 from ..sparse.tensors import CSRTensor
+from .cg import maxit_arg
 
 
 class FaMgLevel(ctypes.Structure):
@@ -491,7 +492,7 @@ class GeometricMG:
         niter, res = ctypes.c_int64(0), ctypes.c_double(0.0)
         d = S.desc(dt)
         args = (nat.dtype_code(dt), ctypes.byref(d), ctypes.byref(self._desc), nat.ptr(b), nat.ptr(x), float(atol),
-                float(rtol), int(maxit or 0), int(check_every), ctypes.byref(niter), ctypes.byref(res), nat.ptr(ws),
+                float(rtol), maxit_arg(maxit), int(check_every), ctypes.byref(niter), ctypes.byref(res), nat.ptr(ws),
                 ws.numel())
         if self.graphs:
             if self._graph is None:

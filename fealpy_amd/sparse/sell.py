@@ -17,7 +17,6 @@ class FaSell16(ctypes.Structure):
                 ("vals", ctypes.c_void_p)]
 
 
-_INDEX = {}
 _INDEX_LOCK = threading.Lock()
 
 
@@ -39,17 +38,18 @@ def _build_index(crow, col, shape):
 
 
 def _sell_index(crow, col, shape):
-    """(slice_ptr, cols, entries) of a CSR pattern, built once per pattern
-    (the entry keeps the pattern tensors alive, so their addresses are not
-    reused) and completed on the building stream before it is shared."""
-    key = (crow.data_ptr(), col.data_ptr(), int(col.numel()), tuple(shape))
+    """(slice_ptr, cols, entries) of a CSR pattern, built once per pattern and
+    completed on the building stream before it is shared.  The index is kept
+    on the pattern's column tensor, so it lives exactly as long as the
+    pattern: every matrix of one pattern shares it, and a dropped pattern
+    takes its index with it."""
     with _INDEX_LOCK:
-        ent = _INDEX.get(key)
-        if ent is not None and ent[0] is crow and ent[1] is col:
+        ent = getattr(col, "_fa_sell_index", None)
+        if ent is not None and ent[0] is crow and ent[1] == tuple(shape):
             return ent[2]
         idx = _build_index(crow, col, shape)
         torch.cuda.current_stream(crow.device).synchronize()
-        _INDEX[key] = (crow, col, idx)
+        col._fa_sell_index = (crow, tuple(shape), idx)
         return idx
 
 

@@ -52,8 +52,40 @@ enum {
     FA_COEF_NONE = 0,   /* operator absent                                        */
     FA_COEF_CONST = 1,  /* scalar `scale`                                         */
     FA_COEF_CELL = 2,   /* scale * coef[c],      coef shape (NC,)                 */
-    FA_COEF_CELLQ = 3   /* scale * coef[c*NQ+q], coef shape (NC, NQ)              */
+    FA_COEF_CELLQ = 3,  /* scale * coef[c*NQ+q], coef shape (NC, NQ)              */
+    FA_COEF_SEP = 4     /* scale * f(x_cq), f an fa_sep_t that `coef` points to   */
 };
+
+/* A coefficient or source given as a sum of separable terms, evaluated by
+ * the element kernels at each quadrature point (no point array, no callback):
+ *
+ *   f(x) = sum_t coef[t] * (field[t] ? field[t][c] : 1)
+ *                * prod_{k in [term_end[t-1], term_end[t])} g_k(x[dim[k]])
+ *
+ *   g = sin(a s + b) | cos(a s + b) | exp(a s + b) | a s + b | s^a (a a whole
+ *   number >= 0), s the coordinate dim[k] of the point;
+ *   x = sum_v bcs[q*(TD+1) + v] * node[cell[c*(TD+1) + v]] (bc_to_point,
+ *   fealpy/mesh/mesh_base.py:454-481).
+ *
+ * It stands for the reference's @cartesian coefficient callables
+ * (fealpy/utils/utils.py:21-53, process_coef_func) when they have this form,
+ * e.g. 3 pi^2 k_c sin(pi x) sin(pi y) sin(pi z).  `field` arrays are FP64 per
+ * cell (NC,); `bcs` is the (NQ, TD+1) FP64 table of the rule in use.  The
+ * struct itself lives in device memory. */
+#define FA_SEP_MAX_TERMS 16
+#define FA_SEP_MAX_FACTORS 48
+enum { FA_SEP_SIN = 0, FA_SEP_COS = 1, FA_SEP_EXP = 2, FA_SEP_AFFINE = 3, FA_SEP_POW = 4 };
+typedef struct {
+    int32_t nterms, nfactors;
+    int32_t term_end[FA_SEP_MAX_TERMS];
+    int32_t kind[FA_SEP_MAX_FACTORS];
+    int32_t dim[FA_SEP_MAX_FACTORS];
+    double coef[FA_SEP_MAX_TERMS];
+    double a[FA_SEP_MAX_FACTORS];
+    double b[FA_SEP_MAX_FACTORS];
+    const double* field[FA_SEP_MAX_TERMS];
+    const double* bcs;
+} fa_sep_t;
 
 const char* fa_last_error(void);
 int fa_version(void);
@@ -464,6 +496,29 @@ void fa_pcg_graph_destroy(fa_pcg_graph_t* graph);
 int fa_pcg_graph(int dtype, const fa_sell_t* A, const fa_mg_t* M, const void* b, void* x, double atol, double rtol,
                  int64_t maxit, int32_t check_every, int64_t* niter_host, double* residual_host,
                  void* work, size_t work_bytes, fa_pcg_graph_t* graph, fa_stream_t stream);
+
+/* ------------------------------------------------------ in-step timing --
+ * Measurement only; no counterpart in the reference.  fa_timer_arm(mask)
+ * arms the kernel families whose FA_TIMER_* bits are set (bit 0 must be
+ * clear; mask 0 disarms) and clears their counts: from then on every launch
+ * of an armed family made by this library is bracketed by two HIP events
+ * recorded on the stream it is launched on (at most 128 launches per family
+ * and arming; launches under stream capture are not timed).
+ * fa_timer_read(kind) waits for the recorded events of one family and
+ * returns how many launches were timed and their summed duration (ms).
+ * Process-wide state, not thread-safe: for benchmarks, not for serving. */
+enum {
+    FA_TIMER_PCG_SPMV = 1,       /* k_pcg_spmv: the CG operator A p (+ p.Ap)   */
+    FA_TIMER_P1_ROWS = 2,        /* P1 matrix, pass 1 (element rows per cell)  */
+    FA_TIMER_P1_SUM_ROWS = 3,    /* P1 matrix, pass 2 (ordered CSR row sums)   */
+    FA_TIMER_P1_VECTORS = 4,     /* P1 load vector, pass 1                     */
+    FA_TIMER_P1_SUM_VALUES = 5,  /* P1 load vector, pass 2                     */
+    FA_TIMER_BC_APPLY = 6,       /* Dirichlet filter + right-hand side         */
+    FA_TIMER_SELL_SPMV = 7,      /* fa_sell_spmv                               */
+    FA_TIMER_VCYCLE = 8          /* one whole V-cycle (all its kernels)        */
+};
+int fa_timer_arm(unsigned mask);
+int fa_timer_read(int kind, int64_t* launches, double* total_ms);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,107 @@
+"""Coefficients and sources evaluated inside the element kernels
+(FA_COEF_SEP) against the reference: the same fixtures the torch-callable
+path is checked on (values written by running the reference's numpy
+backend), at the same 1e-12 relative bar, on every kernel that reads the
+table -- the generic element matrices and vectors (P2-P4), the fused P1
+kernels and the two-pass P1 passes."""
+import numpy as np
+import pytest
+import torch
+
+import fealpy_amd as fa
+from fealpy_amd.expr import cell, cos, sin, x, y, z
+from fealpy_amd.mesh import TetrahedronMesh, TriangleMesh
+
+pytestmark = pytest.mark.gpu
+PI = np.pi
+DEV = torch.device("cuda")
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _close(got, ref, rtol=1e-12):
+    ref = np.asarray(ref)
+    np.testing.assert_allclose(_np(got), ref, rtol=0, atol=rtol * np.abs(ref).max())
+
+
+U3 = sin(PI * x) * sin(PI * y) * sin(PI * z)
+
+
+def _k_sep():
+    return 1 + 0.5 * sin(2 * PI * x) * cos(2 * PI * y) * z
+
+
+def _f_highorder_sep():
+    """-div(k grad u) + u for u = sin sin sin, k = 1 + 0.5 sin(2 pi x) cos(2 pi y) z."""
+    k = _k_sep()
+    dk = (PI * cos(2 * PI * x) * cos(2 * PI * y) * z, -PI * sin(2 * PI * x) * sin(2 * PI * y) * z,
+          0.5 * sin(2 * PI * x) * cos(2 * PI * y))
+    du = (PI * cos(PI * x) * sin(PI * y) * sin(PI * z), PI * sin(PI * x) * cos(PI * y) * sin(PI * z),
+          PI * sin(PI * x) * sin(PI * y) * cos(PI * z))
+    return 3 * PI ** 2 * k * U3 - (dk[0] * du[0] + dk[1] * du[1] + dk[2] * du[2]) + U3
+
+
+@pytest.mark.parametrize("p", [3, 4])
+def test_highorder_separable_coefficient_and_source(golden, p):
+    F = golden(f"fixtures_highorder3d_p{p}.npz")
+    n, q = int(F["n"]), int(F["q"])
+    mesh = TetrahedronMesh.from_box(nx=n, ny=n, nz=n, device=DEV)
+    space = fa.LagrangeFESpace(mesh, p)
+    A = fa.BilinearForm(space).add_integrator(fa.ScalarDiffusionIntegrator(_k_sep(), q=q),
+                                              fa.ScalarMassIntegrator(1.0, q=q)).assembly()
+    np.testing.assert_array_equal(_np(A.col), F["A_indices"])
+    _close(A.values, F["A_data"])
+    b = fa.LinearForm(space).add_integrator(fa.ScalarSourceIntegrator(_f_highorder_sep(), q=q)).assembly()
+    _close(b, F["F"])
+
+
+@pytest.mark.parametrize("two_pass", [False, True])
+def test_lognormal_source_in_kernel(golden, two_pass, monkeypatch):
+    """f_c(x) = 3 pi^2 k_c u(x) with the per-cell field read in the kernel:
+    fused P1 kernel and the two-pass passes (forced on the small mesh)."""
+    from fealpy_amd.fem import integrators
+    if two_pass:
+        monkeypatch.setattr(integrators, "TWO_PASS_MIN_CELLS", 0)
+    F = golden("fixtures_diffusion3d_p1_lognormal.npz")
+    n = int(F["n"])
+    mesh = TetrahedronMesh.from_box(nx=n, ny=n, nz=n, device=DEV)
+    space = fa.LagrangeFESpace(mesh, 1)
+    k = torch.as_tensor(F["k"], device=DEV)
+    f = 3 * PI ** 2 * cell(k) * U3
+    b = fa.LinearForm(space).add_integrator(fa.ScalarSourceIntegrator(f, q=2)).assembly()
+    _close(b, F["F"])
+    # the coefficient as an expression through the P1 matrix kernels
+    kx = _k_sep()
+    A1 = fa.BilinearForm(space).add_integrator(fa.ScalarDiffusionIntegrator(kx, q=3),
+                                               fa.ScalarMassIntegrator(cell(k) * (1 + x), q=3)).assembly()
+    A2 = fa.BilinearForm(space).add_integrator(
+        fa.ScalarDiffusionIntegrator(fa.cartesian(lambda pt: kx(pt)), q=3),
+        fa.ScalarMassIntegrator(fa.cartesian(lambda pt: k[:, None] * (1 + pt[..., 0])), q=3)).assembly()
+    assert torch.equal(A1.col, A2.col)
+    _close(A1.values, _np(A2.values), 1e-14)
+
+
+def test_fourier_source_2d_p2(golden):
+    F = golden("fixtures_poisson2d_p2_fourier.npz")
+    n = int(F["n"])
+    mesh = TriangleMesh.from_box([0, 1, 0, 1], n, n, device=DEV)
+    space = fa.LagrangeFESpace(mesh, 2)
+    f = sum(float(F["a"][m]) * sin(float(F["km"][m]) * PI * x) * sin(float(F["lm"][m]) * PI * y) for m in range(8))
+    b = fa.LinearForm(space).add_integrator(fa.ScalarSourceIntegrator(f)).assembly()
+    _close(b, F["F"])
+
+
+def test_expression_as_dirichlet_data_and_error():
+    """The same object serves as cartesian Dirichlet data and in mesh.error."""
+    mesh = TetrahedronMesh.from_box(nx=6, ny=6, nz=6, device=DEV)
+    space = fa.LagrangeFESpace(mesh, 1)
+    A = fa.BilinearForm(space).add_integrator(fa.ScalarDiffusionIntegrator(q=2)).assembly()
+    b = fa.LinearForm(space).add_integrator(fa.ScalarSourceIntegrator(3 * PI ** 2 * U3, q=2)).assembly()
+    A2, b2 = fa.DirichletBC(space, gd=U3).apply(A, b)
+    xh = fa.cg(A2, b2, atol=1e-14, rtol=1e-13)
+    e1 = float(mesh.error(U3, space.function(xh)))
+    u_t = fa.cartesian(lambda pt: torch.sin(PI * pt).prod(-1))
+    e2 = float(mesh.error(u_t, space.function(xh)))
+    assert abs(e1 - e2) <= 1e-12 * e2 and 0 < e1 < 0.1

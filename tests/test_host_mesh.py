@@ -235,3 +235,36 @@ def test_serving_trace_is_deterministic_and_in_range():
     assert request_bytes(32, 2) > request_bytes(32, 1) > 0
     z = make_requests(8, seed=7)
     assert all(r.arrival == 0.0 for r in z)
+
+
+def test_sell_index_lives_with_its_pattern(monkeypatch):
+    """The shared SELL index is kept on the pattern's column tensor: built
+    once per pattern, reused by every matrix of it, and freed with it (no
+    process-wide cache that would pin every pattern ever solved)."""
+    import gc
+    import weakref
+    from fealpy_amd.sparse import sell
+    built = []
+
+    def fake_build(crow, col, shape):
+        idx = (torch.zeros(1), torch.zeros(1), 0)
+        built.append(weakref.ref(idx[0]))
+        return idx
+    monkeypatch.setattr(sell, "_build_index", fake_build)
+    monkeypatch.setattr(torch.cuda, "current_stream", lambda *a, **k: type("S", (), {"synchronize": lambda s: None})())
+    crow = torch.tensor([0, 1, 2], dtype=torch.int32)
+    col = torch.tensor([0, 1], dtype=torch.int32)
+    a = sell._sell_index(crow, col, (2, 2))
+    assert sell._sell_index(crow, col, (2, 2)) is a and len(built) == 1
+    for _ in range(20):                         # distinct patterns, dropped after use
+        c2 = col.clone()
+        sell._sell_index(crow, c2, (2, 2))
+        del c2
+    gc.collect()
+    assert len(built) == 21
+    assert sum(r() is not None for r in built) == 1     # only the live pattern's index remains
+
+
+def test_maxit_argument_follows_the_reference():
+    from fealpy_amd.solver.cg import maxit_arg
+    assert maxit_arg(None) == -1 and maxit_arg(0) == 1 and maxit_arg(1) == 1 and maxit_arg(50) == 50
