@@ -62,6 +62,7 @@ def parse():
     ap.add_argument("--cpu-n", type=int, default=48, help="CPU baseline: from_box size of the bounded sample")
     ap.add_argument("--cpu-runs", type=int, default=3, help="CPU baseline: runs (the median is reported)")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving figures of the headline line")
+    ap.add_argument("--no-solve", action="store_true", help="highorder: contraction figures only")
     ap.add_argument("--torch-source", action="store_true",
                     help="source as a barycentric torch callable instead of a separable expression")
     ap.add_argument("--requests", type=int, default=4096, help="serving: number of requests")
@@ -457,6 +458,14 @@ def highorder_main(args):
         del ref
         results.append(row)
         torch.cuda.empty_cache()
+    solves = []
+    if not args.no_solve:
+        for p, q in ((3, 5), (4, 6)):
+            try:
+                solves.append(highorder_solve(n, p, q, args.rtol, max(1, min(args.steps, 3))))
+            except Exception as e:               # report and keep the contraction figures
+                solves.append({"p": p, "error": f"{type(e).__name__}: {e}"})
+            print(f"# highorder solve p={p}: {solves[-1]}", file=sys.stderr, flush=True)
     best = results[-1]["bf16"]
     print(json.dumps({
         "metric": "MFMA element contraction TFLOP/s (high-order diffusion + mass)", "value": best["tflops"],
@@ -465,7 +474,86 @@ def highorder_main(args):
                                f"cos(2 pi y) z + mass 1"},
         "roofline": {"bound": "mfma", "achieved": best["tflops"], "peak": MFMA_PEAK_TFLOPS["bf16"], "unit": "TFLOP/s",
                      "frac": best["frac"], "traffic": None},
-        "results": results}))
+        "results": results, "solve": solves}))
+
+
+def _highorder_exprs():
+    """k(x) = 1 + 0.5 sin(2 pi x) cos(2 pi y) z, u = sin sin sin and
+    f = -div(k grad u) + u, as separable expressions (in-kernel)."""
+    from fealpy_amd.expr import cos, sin, x, y, z
+    pi = np.pi
+    k = 1 + 0.5 * sin(2 * pi * x) * cos(2 * pi * y) * z
+    u = sin(pi * x) * sin(pi * y) * sin(pi * z)
+    dk = (pi * cos(2 * pi * x) * cos(2 * pi * y) * z, -pi * sin(2 * pi * x) * sin(2 * pi * y) * z,
+          0.5 * sin(2 * pi * x) * cos(2 * pi * y))
+    du = (pi * cos(pi * x) * sin(pi * y) * sin(pi * z), pi * sin(pi * x) * cos(pi * y) * sin(pi * z),
+          pi * sin(pi * x) * sin(pi * y) * cos(pi * z))
+    f = 3 * pi ** 2 * k * u - (dk[0] * du[0] + dk[1] * du[1] + dk[2] * du[2]) + u
+    return k, u, f
+
+
+def highorder_solve(n, p, q, rtol, steps):
+    """cfg3 assembled and solved: diffusion k(x) + mass 1 on from_box n^3 with
+    P p, degree-2p rule q; assemble A (FP64 element kernel + fixed-order
+    gather), F, the Dirichlet condition with g = u, the multigrid hierarchy
+    (Pp -> P1 on the same mesh, then geometric), and MG-preconditioned CG
+    (FP64 outer, FP32 cycle) to rtol.  The error is against a tight FP64
+    solve of the same system."""
+    import torch
+    import fealpy_amd as fa
+    from fealpy_amd.fem.pattern import pattern_of
+    from fealpy_amd.solver import GeometricMG
+    dev = torch.device("cuda:0")
+    k, u, f = _highorder_exprs()
+    mesh = fa.TetrahedronMesh.from_box([0, 1, 0, 1, 0, 1], nx=n, ny=n, nz=n, device=dev)
+    space = fa.LagrangeFESpace(mesh, p)
+    gdof = space.number_of_global_dofs()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pat = pattern_of(space)
+    torch.cuda.synchronize()
+    t_pat = time.perf_counter() - t0
+    bc = fa.DirichletBC(space, gd=u)
+    mg = GeometricMG(space, bc.is_boundary_dof, dtype=torch.float32)
+    ph = {}
+
+    def one(rt, record):
+        def t(name, fn):
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            r = fn()
+            torch.cuda.synchronize()
+            if record:
+                ph[name] = ph.get(name, 0.0) + (time.perf_counter() - t1) * 1e3
+            return r
+        A = t("assemble_A_ms", lambda: fa.BilinearForm(space).add_integrator(
+            fa.ScalarDiffusionIntegrator(k, q=q), fa.ScalarMassIntegrator(1.0, q=q)).assembly())
+        F = t("assemble_F_ms", lambda: fa.LinearForm(space).add_integrator(
+            fa.ScalarSourceIntegrator(f, q=q)).assembly())
+        A2, F2 = t("dirichlet_ms", lambda: bc.apply(A, F))
+        del A
+        t("mg_setup_ms", lambda: mg.setup(A2))
+        xs, info = t("pcg_ms", lambda: fa.cg(A2, F2, M=mg, atol=0.0, rtol=rt, maxit=2000, returninfo=True))
+        return xs, info, A2
+    x_ref, info_ref, A2 = one(1e-13, False)
+    nnz = int(A2.nnz)
+    del A2
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        xs, info, _ = one(rtol, True)
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / steps
+    err = float(torch.linalg.norm(xs - x_ref) / torch.linalg.norm(x_ref))
+    l2 = float(mesh.error(u, space.function(xs), q=q))
+    out = {"p": p, "q": q, "gdof": gdof, "nnz": nnz, "ms_per_step": round(t_step * 1e3, 2),
+           "dof_per_s": round(gdof / t_step, 1), "rel_err": err, "rel_err_ok": bool(err <= 1e-8),
+           "pcg_iterations": info["niter"], "reference_iterations": info_ref["niter"], "l2_error": l2,
+           "phases_ms": {kk: round(v / steps, 3) for kk, v in ph.items()},
+           "cold_pattern_ms": round(t_pat * 1e3, 2), "levels": [lv.n for lv in mg.levels]}
+    del mg, bc, pat, space, mesh, xs, x_ref
+    torch.cuda.empty_cache()
+    return out
 
 
 def _pcts(lat):

@@ -42,6 +42,7 @@ _SIGNATURES = {
                                           _p, _p, ctypes.c_int, _f64, _p, _p, _p]),
     "fa_gather_matrix": (ctypes.c_int, [ctypes.c_int, _i64, _i32, _p, _p, _p, _p, _p, _i32, _p, _i64, _p, _p]),
     "fa_gather_vector": (ctypes.c_int, [ctypes.c_int, _i64, _i32, _p, _p, _p, _p, _p]),
+    "fa_element_apply": (ctypes.c_int, [ctypes.c_int, _i64, _i32, _p, _p, _i64, _p, _p, _p]),
     "fa_element_elasticity": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _p, _p, _i64, _p,
                                              ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                              ctypes.c_int, _p, _p]),
@@ -66,6 +67,7 @@ _SIGNATURES = {
     "fa_dirichlet_rhs": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p, _p, _p, _p, _p]),
     "fa_dirichlet_apply": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
     "fa_spmv": (ctypes.c_int, [ctypes.c_int, _i64, _i64, _p, _p, _p, _p, _p, _p]),
+    "fa_spmm": (ctypes.c_int, [ctypes.c_int, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p]),
     "fa_reduce_work_bytes": (_sz, []),
     "fa_dot": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p, _sz, _p]),
     "fa_axpby": (ctypes.c_int, [ctypes.c_int, _i64, _f64, _p, _f64, _p, _p]),
@@ -101,6 +103,7 @@ _SIGNATURES = {
     "fa_pcg": (ctypes.c_int, [ctypes.c_int, _p, _p, _p, _p, _f64, _f64, _i64, _i32, _pi64, _pf64, _p, _sz, _p]),
     "fa_pcg_graph_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
     "fa_pcg_graph_destroy": (None, [_p]),
+    "fa_pcg_static": (ctypes.c_int, [ctypes.c_int, _p, _p, _p, _p, _f64, _f64, _i64, _i32, _pi64, _pf64, _p]),
     "fa_timer_arm": (ctypes.c_int, [ctypes.c_uint]),
     "fa_timer_read": (ctypes.c_int, [ctypes.c_int, _pi64, _pf64]),
     "fa_pcg_graph": (ctypes.c_int, [ctypes.c_int, _p, _p, _p, _p, _f64, _f64, _i64, _i32, _pi64, _pf64, _p, _sz, _p,

@@ -98,14 +98,86 @@ __device__ inline double coef_value(int kind, double scale, const double* __rest
 }
 
 // ---- separable coefficients (FA_COEF_SEP) -----------------------------------
+// sin(pi x) and cos(pi x): x = k/2 + r with k = rint(2x), |r| <= 1/4 (r = x - k/2
+// is exact for |x| < 2^51), then the Taylor series of sin(pi r) and cos(pi r)
+// at 0 (coefficients (-1)^n pi^(2n+1)/(2n+1)! and (-1)^n pi^(2n)/(2n)!, through
+// r^17 and r^18: the remainder is below 1e-19 for |r| <= 1/4) by Horner's rule,
+// and the quadrant k mod 4.  Few registers, unlike the general library sin.
+__device__ inline void sincospi_quarter(double r, double& s, double& c) {
+    const double z = r * r;
+    double ps = 7.952054001475513e-07;
+    ps = fma(ps, z, -2.1915353447830217e-05);
+    ps = fma(ps, z, 0.00046630280576761255);
+    ps = fma(ps, z, -0.0073704309457143504);
+    ps = fma(ps, z, 0.08214588661112823);
+    ps = fma(ps, z, -0.5992645293207921);
+    ps = fma(ps, z, 2.5501640398773455);
+    ps = fma(ps, z, -5.16771278004997);
+    ps = fma(ps, z, 3.141592653589793);
+    s = r * ps;
+    double pc = -1.3878952462213771e-07;
+    pc = fma(pc, z, 4.303069587032947e-06);
+    pc = fma(pc, z, -0.0001046381049248457);
+    pc = fma(pc, z, 0.0019295743094039231);
+    pc = fma(pc, z, -0.02580689139001406);
+    pc = fma(pc, z, 0.2353306303588932);
+    pc = fma(pc, z, -1.3352627688545895);
+    pc = fma(pc, z, 4.0587121264167685);
+    pc = fma(pc, z, -4.934802200544679);
+    c = fma(pc, z, 1.0);
+}
+
+// sin(pi x) for shift 0, cos(pi x) for shift 1
+__device__ inline double sinpi_shift(double x, int shift) {
+    const double k = rint(2.0 * x);
+    const double r = fma(-0.5, k, x);
+    double s, c;
+    sincospi_quarter(r, s, c);
+    const int q = (static_cast<int>(static_cast<long long>(k) & 3) + shift) & 3;
+    const double v = (q & 1) ? c : s;
+    return (q & 2) ? -v : v;
+}
+
+constexpr double INV_PI = 0.31830988618379067154;
+
+// exp(x) = 2^k e^r: k = rint(x / ln 2), r = x - k ln 2 with ln 2 split in two
+// doubles (Cody and Waite's reduction, the second part applied by FMA), so
+// |r| <= ln(2)/2; e^r from its Taylor series through r^14 (remainder below
+// 1e-19), by Horner's rule; then the exponent is scaled in by ldexp.
+__device__ inline double exp_reduced(double x) {
+    if (x > 709.78) return HUGE_VAL;
+    if (x < -745.2) return 0.0;
+    const double k = rint(x * 1.4426950408889634);
+    double r = fma(-k, 0.6931471805599453, x);
+    r = fma(-k, 2.3190468138462996e-17, r);
+    double e = 1.1470745597729725e-11;
+    e = fma(e, r, 1.6059043836821613e-10);
+    e = fma(e, r, 2.08767569878681e-09);
+    e = fma(e, r, 2.505210838544172e-08);
+    e = fma(e, r, 2.755731922398589e-07);
+    e = fma(e, r, 2.7557319223985893e-06);
+    e = fma(e, r, 2.48015873015873e-05);
+    e = fma(e, r, 0.0001984126984126984);
+    e = fma(e, r, 0.001388888888888889);
+    e = fma(e, r, 0.008333333333333333);
+    e = fma(e, r, 0.041666666666666664);
+    e = fma(e, r, 0.16666666666666666);
+    e = fma(e, r, 0.5);
+    e = fma(e, r, 1.0);
+    e = fma(e, r, 1.0);
+    return ldexp(e, static_cast<int>(k));
+}
+
 // One factor g(s) of a term; every lane runs the same program (the struct is
 // read through uniform loads), so the switch does not diverge.
 __device__ inline double sep_factor(int kind, double a, double b, double s) {
     switch (kind) {
-        case FA_SEP_SIN: return sin(a * s + b);
-        case FA_SEP_COS: return cos(a * s + b);
-        case FA_SEP_EXP: return exp(a * s + b);
+        case FA_SEP_SIN: return sinpi_shift((a * s + b) * INV_PI, 0);
+        case FA_SEP_COS: return sinpi_shift((a * s + b) * INV_PI, 1);
+        case FA_SEP_EXP: return exp_reduced(a * s + b);
         case FA_SEP_AFFINE: return a * s + b;
+        case FA_SEP_SINPI: return sinpi_shift(a * s + b, 0);
+        case FA_SEP_COSPI: return sinpi_shift(a * s + b, 1);
         default: {  // FA_SEP_POW: s^a, a a whole number
             const int k = static_cast<int>(a);
             double r = 1.0;
@@ -149,16 +221,20 @@ __device__ inline double sep_value(const fa_sep_t* __restrict__ e, int q, const 
 // coefficient at quadrature point q of cell c: `ci` indexes the per-cell /
 // per-point arrays (which may be a block's staged copy), `cg` is the global
 // cell id (fields of a separable coefficient)
-template <int TD>
+// SEP: the kernel was instantiated for a separable coefficient; kernels
+// without one do not carry the transcendental code (it costs registers).
+template <int TD, bool SEP>
 __device__ inline double coef_at(int kind, double scale, const double* __restrict__ coef, int64_t ci, int64_t cg,
                                  int q, int nq, const double p[TD + 1][TD]) {
-    if (kind == FA_COEF_SEP) return scale * sep_value<TD>(reinterpret_cast<const fa_sep_t*>(coef), q, p, cg);
+    if constexpr (SEP) {
+        if (kind == FA_COEF_SEP) return scale * sep_value<TD>(reinterpret_cast<const fa_sep_t*>(coef), q, p, cg);
+    }
     return coef_value(kind, scale, coef, ci, q, nq);
 }
 
 // One thread per (cell, row i); the row of the element matrix is kept in
 // registers (LDOF is a compile-time constant).
-template <int TD, int LDOF, class T>
+template <int TD, int LDOF, class T, bool SEP>
 __global__ void __launch_bounds__(256) k_element_matrices(const double* __restrict__ node, const int32_t* __restrict__ cell,
                                                          int64_t nc, int nq, const double* __restrict__ qw,
                                                          const double* __restrict__ phi, const double* __restrict__ gphi,
@@ -178,7 +254,7 @@ __global__ void __launch_bounds__(256) k_element_matrices(const double* __restri
     for (int q = 0; q < nq; ++q) {
         const double wq = qw[q] * g.meas;
         if (dk) {
-            const double kd = wq * coef_at<TD>(dk, ds, dc, c, c, q, nq, p);
+            const double kd = wq * coef_at<TD, SEP>(dk, ds, dc, c, c, q, nq, p);
             const double* R = gphi + (int64_t)q * LDOF * (TD + 1);
             double gi[TD];
 #pragma unroll
@@ -202,7 +278,7 @@ __global__ void __launch_bounds__(256) k_element_matrices(const double* __restri
             }
         }
         if (mk) {
-            const double km = wq * coef_at<TD>(mk, ms, mc, c, c, q, nq, p);
+            const double km = wq * coef_at<TD, SEP>(mk, ms, mc, c, c, q, nq, p);
             const double* ph = phi + (int64_t)q * LDOF;
             const double pi = ph[i] * km;
 #pragma unroll
@@ -214,7 +290,7 @@ __global__ void __launch_bounds__(256) k_element_matrices(const double* __restri
     for (int j = 0; j < LDOF; ++j) out[j] = static_cast<T>(acc[j]);
 }
 
-template <int TD, class T>
+template <int TD, class T, bool SEP>
 __global__ void __launch_bounds__(256) k_element_vectors(const double* __restrict__ node, const int32_t* __restrict__ cell,
                                                         int64_t nc, int ldof, int nq, const double* __restrict__ qw,
                                                         const double* __restrict__ phi, int sk, double ss,
@@ -228,7 +304,7 @@ __global__ void __launch_bounds__(256) k_element_vectors(const double* __restric
     Geom<TD> g;
     g.compute(p);
     double acc = 0.0;
-    for (int q = 0; q < nq; ++q) acc += qw[q] * g.meas * coef_at<TD>(sk, ss, src, c, c, q, nq, p) * phi[(int64_t)q * ldof + i];
+    for (int q = 0; q < nq; ++q) acc += qw[q] * g.meas * coef_at<TD, SEP>(sk, ss, src, c, c, q, nq, p) * phi[(int64_t)q * ldof + i];
     elem[t] = static_cast<T>(acc);
 }
 
@@ -567,16 +643,25 @@ __device__ inline void P1Cell<2>::compute(const double p[3][2]) {
     meas = (a0 * b1 - a1 * b0) / 2.0;
 }
 
-template <int TD>
-__device__ inline void p1_row(const P1Cell<TD>& g, const double p[TD + 1][TD], int64_t cg, int i, int64_t c, int nq,
-                              const double* __restrict__ qw,
+// sum_q w_q k(x_q): the diffusion coefficient's share of a P1 element matrix
+// (grad lambda is constant on the cell)
+template <int TD, bool SEP>
+__device__ inline double p1_kd(int nq, const double* __restrict__ qw, int dk, double ds, const double* __restrict__ dc,
+                               int64_t c, int64_t cg, const double p[TD + 1][TD]) {
+    double kd = 0.0;
+    if (dk)
+        for (int q = 0; q < nq; ++q) kd += qw[q] * coef_at<TD, SEP>(dk, ds, dc, c, cg, q, nq, p);
+    return kd;
+}
+
+template <int TD, bool SEP>
+__device__ inline void p1_row(const P1Cell<TD>& g, const double p[TD + 1][TD], int64_t cg, double kd, int i, int64_t c,
+                              int nq, const double* __restrict__ qw,
                               const double* __restrict__ phi, int dk, double ds, const double* __restrict__ dc, int mk,
                               double ms, const double* __restrict__ mc, double out[TD + 1]) {
 #pragma unroll
     for (int j = 0; j < TD + 1; ++j) out[j] = 0.0;
     if (dk) {
-        double kd = 0.0;
-        for (int q = 0; q < nq; ++q) kd += qw[q] * coef_at<TD>(dk, ds, dc, c, cg, q, nq, p);
         const double s = kd * g.meas / (g.den * g.den);
         double ni[TD];  // row i selected without a runtime index into g.n (keeps it in registers)
 #pragma unroll
@@ -598,7 +683,7 @@ __device__ inline void p1_row(const P1Cell<TD>& g, const double p[TD + 1][TD], i
     if (mk) {
         for (int q = 0; q < nq; ++q) {
             const double* ph = phi + q * (TD + 1);
-            const double w = qw[q] * g.meas * coef_at<TD>(mk, ms, mc, c, cg, q, nq, p) * ph[i];
+            const double w = qw[q] * g.meas * coef_at<TD, SEP>(mk, ms, mc, c, cg, q, nq, p) * ph[i];
 #pragma unroll
             for (int j = 0; j < TD + 1; ++j) out[j] += w * ph[j];
         }
@@ -612,7 +697,7 @@ __device__ inline void p1_row(const P1Cell<TD>& g, const double p[TD + 1][TD], i
 // entry is therefore the sequential sum of its cell contributions in
 // ascending cell order, the order of the reference's stable lexsort followed
 // by sequential np.add.at (coo_tensor.py:184-213).
-template <int TD, int G, int K, class T>
+template <int TD, int G, int K, class T, bool SEP>
 __global__ void __launch_bounds__(256) k_assemble_p1(int64_t gdof, const double* __restrict__ node,
                                                     const int32_t* __restrict__ cell, int nq,
                                                     const double* __restrict__ qw, const double* __restrict__ phi,
@@ -652,7 +737,8 @@ __global__ void __launch_bounds__(256) k_assemble_p1(int64_t gdof, const double*
                 P1Cell<TD> g;
                 g.compute(p);
                 double v[NV];
-                p1_row<TD>(g, p, c, i, c, nq, qw, phi, dk, ds, dc, mk, ms, mc, v);
+                const double kd = p1_kd<TD, SEP>(nq, qw, dk, ds, dc, c, c, p);
+                p1_row<TD, SEP>(g, p, c, kd, i, c, nq, qw, phi, dk, ds, dc, mk, ms, mc, v);
 #pragma unroll
                 for (int j = 0; j < NV; ++j) {
                     pos[j] = lower_bound_lds(cols, L, cell[c * NV + j]);
@@ -688,7 +774,7 @@ __global__ void __launch_bounds__(256) k_assemble_p1(int64_t gdof, const double*
 
 // load vector, P1: b_r = sum over incidences (c, i) of |T_c| sum_q w_q f_cq phi_i(q);
 // G lanes per DOF, lane partials combined by a fixed tree
-template <int TD, int G, class T>
+template <int TD, int G, class T, bool SEP>
 __global__ void __launch_bounds__(256) k_assemble_p1_vec(int64_t gdof, const double* __restrict__ node,
                                                         const int32_t* __restrict__ cell, int nq,
                                                         const double* __restrict__ qw,
@@ -711,7 +797,7 @@ __global__ void __launch_bounds__(256) k_assemble_p1_vec(int64_t gdof, const dou
             P1Cell<TD> g;
             g.compute(p);
             double v = 0.0;
-            for (int q = 0; q < nq; ++q) v += qw[q] * g.meas * coef_at<TD>(sk, ss, src, c, c, q, nq, p) * phi[q * NV + i];
+            for (int q = 0; q < nq; ++q) v += qw[q] * g.meas * coef_at<TD, SEP>(sk, ss, src, c, c, q, nq, p) * phi[q * NV + i];
             s += v;
         }
         s = group_sum<G>(s);
@@ -733,8 +819,12 @@ static int launch_p1(int64_t gdof, const double* node, const int32_t* cell, int 
         return FA_ECAPACITY;
     }
     const unsigned grid = grid_for(gdof, rpb, 1 << 20);
-    hipLaunchKernelGGL((k_assemble_p1<TD, G, K, T>), dim3(grid), dim3(rpb * G), lds, s, gdof, node, cell, nq, qw, phi,
-                       dk, ds, dc, mk, ms, mc, ptr, adj, indptr, indices, lmax, static_cast<T*>(values));
+    if (dk == FA_COEF_SEP || mk == FA_COEF_SEP)
+        hipLaunchKernelGGL((k_assemble_p1<TD, G, K, T, true>), dim3(grid), dim3(rpb * G), lds, s, gdof, node, cell, nq,
+                           qw, phi, dk, ds, dc, mk, ms, mc, ptr, adj, indptr, indices, lmax, static_cast<T*>(values));
+    else
+        hipLaunchKernelGGL((k_assemble_p1<TD, G, K, T, false>), dim3(grid), dim3(rpb * G), lds, s, gdof, node, cell, nq,
+                           qw, phi, dk, ds, dc, mk, ms, mc, ptr, adj, indptr, indices, lmax, static_cast<T*>(values));
     FA_LAUNCH("k_assemble_p1");
     return FA_OK;
 }
@@ -744,8 +834,12 @@ static void launch_elem(const double* node, const int32_t* cell, int64_t nc, int
                         const double* phi, const double* gphi, int dk, double ds, const double* dc, int mk, double ms,
                         const double* mc, void* elem, hipStream_t s) {
     const int64_t n = nc * LDOF;
-    hipLaunchKernelGGL((k_element_matrices<TD, LDOF, T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, node,
-                       cell, nc, nq, qw, phi, gphi, dk, ds, dc, mk, ms, mc, static_cast<T*>(elem));
+    if (dk == FA_COEF_SEP || mk == FA_COEF_SEP)
+        hipLaunchKernelGGL((k_element_matrices<TD, LDOF, T, true>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                           node, cell, nc, nq, qw, phi, gphi, dk, ds, dc, mk, ms, mc, static_cast<T*>(elem));
+    else
+        hipLaunchKernelGGL((k_element_matrices<TD, LDOF, T, false>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                           node, cell, nc, nq, qw, phi, gphi, dk, ds, dc, mk, ms, mc, static_cast<T*>(elem));
 }
 
 template <class T>
@@ -829,21 +923,18 @@ extern "C" int fa_element_vectors(int dtype, int td, int ldof, int nq, const dou
     const int64_t n = nc * ldof;
     const unsigned grid = (unsigned)((n + 255) / 256);
     hipStream_t s = S(stream);
+#define FA_EV(TD_, T_, SEP_)                                                                                     \
+    hipLaunchKernelGGL((k_element_vectors<TD_, T_, SEP_>), dim3(grid), dim3(256), 0, s, node, cell, nc, ldof, nq, qw, \
+                       phi, src_kind, src_scale, src, static_cast<T_*>(elem))
+#define FA_EV2(TD_, T_) \
+    if (src_kind == FA_COEF_SEP) FA_EV(TD_, T_, true); else FA_EV(TD_, T_, false)
     if (dtype == FA_F64) {
-        if (td == 2)
-            hipLaunchKernelGGL((k_element_vectors<2, double>), dim3(grid), dim3(256), 0, s, node, cell, nc, ldof, nq,
-                               qw, phi, src_kind, src_scale, src, static_cast<double*>(elem));
-        else
-            hipLaunchKernelGGL((k_element_vectors<3, double>), dim3(grid), dim3(256), 0, s, node, cell, nc, ldof, nq,
-                               qw, phi, src_kind, src_scale, src, static_cast<double*>(elem));
+        if (td == 2) { FA_EV2(2, double); } else { FA_EV2(3, double); }
     } else {
-        if (td == 2)
-            hipLaunchKernelGGL((k_element_vectors<2, float>), dim3(grid), dim3(256), 0, s, node, cell, nc, ldof, nq,
-                               qw, phi, src_kind, src_scale, src, static_cast<float*>(elem));
-        else
-            hipLaunchKernelGGL((k_element_vectors<3, float>), dim3(grid), dim3(256), 0, s, node, cell, nc, ldof, nq,
-                               qw, phi, src_kind, src_scale, src, static_cast<float*>(elem));
+        if (td == 2) { FA_EV2(2, float); } else { FA_EV2(3, float); }
     }
+#undef FA_EV2
+#undef FA_EV
     FA_LAUNCH("k_element_vectors");
     return FA_OK;
 }
@@ -876,6 +967,43 @@ extern "C" int fa_gather_vector(int dtype, int64_t gdof, int32_t ldof, const int
         hipLaunchKernelGGL(k_gather_vector<float>, dim3(grid), dim3(256), 0, s, gdof, adj_ptr, adj,
                            static_cast<const float*>(elem), static_cast<float*>(out));
     FA_LAUNCH("k_gather_vector");
+    return FA_OK;
+}
+
+// v[c][i] = sum_j elem[c][i][j] x[c2d[c][j]]: one thread per (cell, row),
+// the row read contiguously, the cell's x entries gathered (shared by the
+// cell's rows, so they come from L1 after the first)
+template <class T>
+__global__ void __launch_bounds__(256) k_element_apply(int64_t nc, int ldof, const int32_t* __restrict__ c2d,
+                                                      const T* __restrict__ elem, int64_t stride,
+                                                      const T* __restrict__ x, T* __restrict__ v) {
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nc * ldof;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = t / ldof;
+        const int i = static_cast<int>(t - c * ldof);
+        const T* row = elem + c * stride + static_cast<int64_t>(i) * ldof;
+        const int32_t* dofs = c2d + c * ldof;
+        T s = T(0);
+        for (int j = 0; j < ldof; ++j) s += row[j] * x[dofs[j]];
+        v[t] = s;
+    }
+}
+
+extern "C" int fa_element_apply(int dtype, int64_t nc, int32_t ldof, const int32_t* cell2dof, const void* elem,
+                                int64_t elem_stride, const void* x, void* v, fa_stream_t stream) {
+    FA_REQUIRE(dtype == FA_F64 || dtype == FA_F32, "fa_element_apply: bad dtype %d", dtype);
+    FA_REQUIRE(ldof >= 1 && elem_stride >= (int64_t)ldof * ldof, "fa_element_apply: bad sizes");
+    if (nc == 0) return FA_OK;
+    const unsigned grid = grid_for(nc * ldof, 256, 1 << 20);
+    if (dtype == FA_F64)
+        hipLaunchKernelGGL(k_element_apply<double>, dim3(grid), dim3(256), 0, S(stream), nc, ldof, cell2dof,
+                           static_cast<const double*>(elem), elem_stride, static_cast<const double*>(x),
+                           static_cast<double*>(v));
+    else
+        hipLaunchKernelGGL(k_element_apply<float>, dim3(grid), dim3(256), 0, S(stream), nc, ldof, cell2dof,
+                           static_cast<const float*>(elem), elem_stride, static_cast<const float*>(x),
+                           static_cast<float*>(v));
+    FA_LAUNCH("k_element_apply");
     return FA_OK;
 }
 
@@ -949,15 +1077,18 @@ extern "C" int fa_assemble_p1_vector(int dtype, int td, int64_t gdof, int nq, co
     hipStream_t s = S(stream);
     constexpr int G = 8;
     const unsigned grid = grid_for(gdof * G, 256, 1 << 16);
-#define FA_V(TD_, T_)                                                                                             \
-    hipLaunchKernelGGL((k_assemble_p1_vec<TD_, G, T_>), dim3(grid), dim3(256), 0, s, gdof, node, cell, nq, qw, phi, \
-                       src_kind, src_scale, src, adj_ptr, adj, static_cast<T_*>(out))
+#define FA_V1(TD_, T_, SEP_)                                                                                     \
+    hipLaunchKernelGGL((k_assemble_p1_vec<TD_, G, T_, SEP_>), dim3(grid), dim3(256), 0, s, gdof, node, cell, nq, qw, \
+                       phi, src_kind, src_scale, src, adj_ptr, adj, static_cast<T_*>(out))
+#define FA_V(TD_, T_) \
+    if (src_kind == FA_COEF_SEP) FA_V1(TD_, T_, true); else FA_V1(TD_, T_, false)
     if (td == 3) {
         if (dtype == FA_F64) FA_V(3, double); else FA_V(3, float);
     } else {
         if (dtype == FA_F64) FA_V(2, double); else FA_V(2, float);
     }
 #undef FA_V
+#undef FA_V1
     FA_LAUNCH("k_assemble_p1_vec");
     return FA_OK;
 }
@@ -1061,7 +1192,6 @@ __global__ void __launch_bounds__(256) k_incidence_maps(int64_t gdof, int ldof, 
 // the kernels are bound by the texture address unit, so fewer, wider memory
 // instructions are what counts.  Values equal the fused kernel's bit for bit.
 constexpr int P1_BLOCK = 256;
-constexpr int P1_MAX_NQ = 8;  // source values per cell kept in registers by the load-vector pass
 
 __device__ inline size_t coef_stage_len(int kind, int nq) {
     return kind == FA_COEF_CELL ? 1 : (kind == FA_COEF_CELLQ ? static_cast<size_t>(nq) : 0);
@@ -1098,7 +1228,7 @@ __device__ inline void load_vertices_wide(const double* __restrict__ node, const
 }
 
 // pass 1, matrix: rows[inc[c*NV + i]][j] = element entry (i, j) of cell c
-template <int TD>
+template <int TD, bool SEP>
 __global__ void __launch_bounds__(P1_BLOCK) k_p1_incidence_rows(int64_t nc, const double* __restrict__ nodew,
                                                                const int32_t* __restrict__ cell, int nq,
                                                                const double* __restrict__ qw,
@@ -1127,13 +1257,19 @@ __global__ void __launch_bounds__(P1_BLOCK) k_p1_incidence_rows(int64_t nc, cons
             P1Cell<TD> g;
             g.compute(p);
             // coefficients through the staged block: cell index t, row stride ld / lm
-            // (a separable coefficient reads its fields at the global cell id)
-            const double* dcl = ld ? dS : (dk == FA_COEF_SEP ? dc : nullptr);
-            const double* mcl = lm ? mS : (mk == FA_COEF_SEP ? mc : nullptr);
+            // (a separable coefficient reads its fields at the global cell id;
+            // without one the pointers stay LDS-only, so the loads are LDS reads)
+            const double* dcl = ld ? dS : nullptr;
+            const double* mcl = lm ? mS : nullptr;
+            if constexpr (SEP) {
+                if (dk == FA_COEF_SEP) dcl = dc;
+                if (mk == FA_COEF_SEP) mcl = mc;
+            }
+            const double kd = p1_kd<TD, SEP>(nq, qw, dk, ds, dcl, t, c0 + t, p);
 #pragma unroll
             for (int i = 0; i < NV; ++i) {
                 double v[NV];
-                p1_row<TD>(g, p, c0 + t, i, t, nq, qw, phi, dk, ds, dcl, mk, ms, mcl, v);
+                p1_row<TD, SEP>(g, p, c0 + t, kd, i, t, nq, qw, phi, dk, ds, dcl, mk, ms, mcl, v);
                 double* out = rows + static_cast<int64_t>(iS[t * NV + i]) * NV;
 #pragma unroll
                 for (int j = 0; j < NV; ++j) out[j] = v[j];
@@ -1144,7 +1280,7 @@ __global__ void __launch_bounds__(P1_BLOCK) k_p1_incidence_rows(int64_t nc, cons
 }
 
 // pass 1, load vector: vals[inc[c*NV + i]] = |T| sum_q w_q f_cq phi_i(q)
-template <int TD>
+template <int TD, bool SEP>
 __global__ void __launch_bounds__(P1_BLOCK) k_p1_incidence_vectors(int64_t nc, const double* __restrict__ nodew,
                                                                   const int32_t* __restrict__ cell, int nq,
                                                                   const double* __restrict__ qw,
@@ -1169,31 +1305,22 @@ __global__ void __launch_bounds__(P1_BLOCK) k_p1_incidence_vectors(int64_t nc, c
             load_vertices_wide<TD>(nodew, vS + t * NV, p);
             P1Cell<TD> g;
             g.compute(p);
-            const double* sl = ls ? sS : (sk == FA_COEF_SEP ? src : nullptr);
-            if (nq <= P1_MAX_NQ) {
-                // w_q |T| f(x_q) once per point (a separable source is evaluated
-                // once, not once per basis function), then the moments
-                double fq[P1_MAX_NQ];
-#pragma unroll
-                for (int q = 0; q < P1_MAX_NQ; ++q)
-                    fq[q] = q < nq ? qw[q] * g.meas * coef_at<TD>(sk, ss, sl, t, c0 + t, q, nq, p) : 0.0;
-#pragma unroll
-                for (int i = 0; i < NV; ++i) {
-                    double v = 0.0;
-#pragma unroll
-                    for (int q = 0; q < P1_MAX_NQ; ++q)
-                        if (q < nq) v += fq[q] * phi[q * NV + i];
-                    vals[iS[t * NV + i]] = v;
-                }
-            } else {
-#pragma unroll
-                for (int i = 0; i < NV; ++i) {
-                    double v = 0.0;
-                    for (int q = 0; q < nq; ++q)
-                        v += qw[q] * g.meas * coef_at<TD>(sk, ss, sl, t, c0 + t, q, nq, p) * phi[q * NV + i];
-                    vals[iS[t * NV + i]] = v;
-                }
+            const double* sl = ls ? sS : nullptr;
+            if constexpr (SEP) {
+                if (sk == FA_COEF_SEP) sl = src;
             }
+            // w_q |T| f(x_q) once per point, then its share of every basis
+            // moment; each moment is summed over the points in order
+            double v[NV];
+#pragma unroll
+            for (int i = 0; i < NV; ++i) v[i] = 0.0;
+            for (int q = 0; q < nq; ++q) {
+                const double fq = qw[q] * g.meas * coef_at<TD, SEP>(sk, ss, sl, t, c0 + t, q, nq, p);
+#pragma unroll
+                for (int i = 0; i < NV; ++i) v[i] += fq * phi[q * NV + i];
+            }
+#pragma unroll
+            for (int i = 0; i < NV; ++i) vals[iS[t * NV + i]] = v[i];
         }
         __syncthreads();
     }
@@ -1366,12 +1493,16 @@ extern "C" int fa_p1_incidence_rows(int td, int64_t nc, int nq, const double* no
     }
     const unsigned grid = grid_for(nc, P1_BLOCK, 1 << 20);
     timer_begin(FA_TIMER_P1_ROWS, S(stream));
-    if (td == 3)
-        hipLaunchKernelGGL(k_p1_incidence_rows<3>, dim3(grid), dim3(P1_BLOCK), lds, S(stream), nc, node_wide, cell, nq,
-                           qw, phi, diff_kind, diff_scale, diff_coef, mass_kind, mass_scale, mass_coef, inc, rows);
-    else
-        hipLaunchKernelGGL(k_p1_incidence_rows<2>, dim3(grid), dim3(P1_BLOCK), lds, S(stream), nc, node_wide, cell, nq,
-                           qw, phi, diff_kind, diff_scale, diff_coef, mass_kind, mass_scale, mass_coef, inc, rows);
+#define FA_R(TD_, SEP_)                                                                                            \
+    hipLaunchKernelGGL((k_p1_incidence_rows<TD_, SEP_>), dim3(grid), dim3(P1_BLOCK), lds, S(stream), nc, node_wide, \
+                       cell, nq, qw, phi, diff_kind, diff_scale, diff_coef, mass_kind, mass_scale, mass_coef, inc, rows)
+    const bool sep = diff_kind == FA_COEF_SEP || mass_kind == FA_COEF_SEP;
+    if (td == 3) {
+        if (sep) FA_R(3, true); else FA_R(3, false);
+    } else {
+        if (sep) FA_R(2, true); else FA_R(2, false);
+    }
+#undef FA_R
     timer_end(FA_TIMER_P1_ROWS, S(stream));
     FA_LAUNCH("k_p1_incidence_rows");
     return FA_OK;
@@ -1418,12 +1549,15 @@ extern "C" int fa_p1_incidence_vectors(int td, int64_t nc, int nq, const double*
     }
     const unsigned grid = grid_for(nc, P1_BLOCK, 1 << 20);
     timer_begin(FA_TIMER_P1_VECTORS, S(stream));
-    if (td == 3)
-        hipLaunchKernelGGL(k_p1_incidence_vectors<3>, dim3(grid), dim3(P1_BLOCK), lds, S(stream), nc, node_wide, cell,
-                           nq, qw, phi, src_kind, src_scale, src, inc, vals);
-    else
-        hipLaunchKernelGGL(k_p1_incidence_vectors<2>, dim3(grid), dim3(P1_BLOCK), lds, S(stream), nc, node_wide, cell,
-                           nq, qw, phi, src_kind, src_scale, src, inc, vals);
+#define FA_PV(TD_, SEP_)                                                                                              \
+    hipLaunchKernelGGL((k_p1_incidence_vectors<TD_, SEP_>), dim3(grid), dim3(P1_BLOCK), lds, S(stream), nc, node_wide, \
+                       cell, nq, qw, phi, src_kind, src_scale, src, inc, vals)
+    if (td == 3) {
+        if (src_kind == FA_COEF_SEP) FA_PV(3, true); else FA_PV(3, false);
+    } else {
+        if (src_kind == FA_COEF_SEP) FA_PV(2, true); else FA_PV(2, false);
+    }
+#undef FA_PV
     timer_end(FA_TIMER_P1_VECTORS, S(stream));
     FA_LAUNCH("k_p1_incidence_vectors");
     return FA_OK;

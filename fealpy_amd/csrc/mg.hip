@@ -6,6 +6,8 @@
 
 #include <new>
 
+#include <mutex>
+
 #include "common.hpp"
 #include "reduce.hpp"
 #include "sell.hpp"
@@ -1686,6 +1688,51 @@ extern "C" int fa_pcg(int dtype, const fa_sell_t* A, const fa_mg_t* M, const voi
                       void* work, size_t work_bytes, fa_stream_t stream) {
     return pcg_dispatch(dtype, A, M, b, x, atol, rtol, maxit, check_every, niter_host, residual_host, work,
                         work_bytes, S(stream), nullptr);
+}
+
+// ---- fa_pcg_static: the PCG workspace in a file-scope device array --------
+// For callers that cannot allocate.  The array holds the workspace of
+// fa_pcg_work_bytes(FA_F64, FA_PCG_STATIC_MAX_ROWS); every GPU the library
+// runs on keeps its own copy until the process ends.  In host code the
+// array's name is only a stand-in, so its device address is looked up once
+// per device with hipGetSymbolAddress and only that address is used.
+constexpr size_t PCG_STATIC_BYTES = FA_PCG_STATIC_BYTES;
+static __device__ __align__(256) unsigned char scratch[PCG_STATIC_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+int scratch_on_current_device(void** out) {
+    int dev = -1;
+    FA_HIP(hipGetDevice(&dev));
+    FA_REQUIRE(dev >= 0 && dev < MAX_DEVICES, "fa_pcg_static: device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        FA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return FA_OK;
+}
+}  // namespace
+
+extern "C" int fa_pcg_static(int dtype, const fa_sell_t* A, const fa_mg_t* M, const void* b, void* x, double atol,
+                             double rtol, int64_t maxit, int32_t check_every, int64_t* niter_host,
+                             double* residual_host, fa_stream_t stream) {
+    FA_REQUIRE(A && A->n > 0, "fa_pcg_static: bad arguments");
+    FA_REQUIRE(A->n <= FA_PCG_STATIC_MAX_ROWS, "fa_pcg_static: %lld rows exceed the static workspace (at most %d)",
+               (long long)A->n, FA_PCG_STATIC_MAX_ROWS);
+    const size_t need = fa_pcg_work_bytes(dtype, A->n);
+    FA_REQUIRE(need <= PCG_STATIC_BYTES, "fa_pcg_static: workspace of %zu bytes exceeds the static %zu", need,
+               PCG_STATIC_BYTES);
+    void* work = nullptr;
+    const int e = scratch_on_current_device(&work);
+    if (e != FA_OK) return e;
+    return pcg_dispatch(dtype, A, M, b, x, atol, rtol, maxit, check_every, niter_host, residual_host, work, need,
+                        S(stream), nullptr);
 }
 
 struct fa_pcg_graph_s {

@@ -38,6 +38,37 @@ static int spmv_launch(int64_t n, const int32_t* indptr, const int32_t* indices,
     return FA_OK;
 }
 
+// Y = A X for nv right-hand sides stored row-major (X[col*nv + v]), the
+// layout of SciPy's csr_matvecs; G lanes per row, each entry's columns of X
+// read by one lane, vectors taken VB at a time
+template <class T, int G, int VB>
+__global__ void __launch_bounds__(256) k_spmm(int64_t n, int64_t nv, const int32_t* __restrict__ indptr,
+                                             const int32_t* __restrict__ indices, const T* __restrict__ vals,
+                                             const T* __restrict__ x, T* __restrict__ y) {
+    const int lane = threadIdx.x % G;
+    for (int64_t r = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G; r < n;
+         r += (int64_t)gridDim.x * blockDim.x / G) {
+        const int32_t k0 = indptr[r], k1 = indptr[r + 1];
+        for (int64_t v0 = 0; v0 < nv; v0 += VB) {
+            T s[VB];
+#pragma unroll
+            for (int u = 0; u < VB; ++u) s[u] = T(0);
+            for (int32_t k = k0 + lane; k < k1; k += G) {
+                const T a = vals[k];
+                const T* xr = x + static_cast<int64_t>(indices[k]) * nv + v0;
+#pragma unroll
+                for (int u = 0; u < VB; ++u)
+                    if (v0 + u < nv) s[u] += a * xr[u];
+            }
+#pragma unroll
+            for (int u = 0; u < VB; ++u) {
+                const T t = group_sum<G>(s[u]);
+                if (lane == 0 && v0 + u < nv) y[r * nv + v0 + u] = t;
+            }
+        }
+    }
+}
+
 static int lanes_for(int64_t n, int64_t nnz) {
     const double avg = n > 0 ? double(nnz) / double(n) : 1.0;
     if (avg <= 6) return 4;
@@ -201,6 +232,22 @@ extern "C" int fa_spmv(int dtype, int64_t n, int64_t nnz, const int32_t* indptr,
                                    static_cast<const double*>(x), static_cast<double*>(y), G, S(stream));
     return spmv_launch<float>(n, indptr, indices, static_cast<const float*>(values), static_cast<const float*>(x),
                               static_cast<float*>(y), G, S(stream));
+}
+
+extern "C" int fa_spmm(int dtype, int64_t n, int64_t nnz, int64_t nvec, const int32_t* indptr, const int32_t* indices,
+                       const void* values, const void* x, void* y, fa_stream_t stream) {
+    FA_REQUIRE(dtype == FA_F64 || dtype == FA_F32, "fa_spmm: bad dtype %d", dtype);
+    FA_REQUIRE(nvec >= 1, "fa_spmm: nvec must be >= 1");
+    if (n == 0) return FA_OK;
+    const unsigned grid = grid_for(n * 8, 256, 1 << 20);
+    if (dtype == FA_F64)
+        hipLaunchKernelGGL((k_spmm<double, 8, 4>), dim3(grid), dim3(256), 0, S(stream), n, nvec, indptr, indices,
+                           static_cast<const double*>(values), static_cast<const double*>(x), static_cast<double*>(y));
+    else
+        hipLaunchKernelGGL((k_spmm<float, 8, 4>), dim3(grid), dim3(256), 0, S(stream), n, nvec, indptr, indices,
+                           static_cast<const float*>(values), static_cast<const float*>(x), static_cast<float*>(y));
+    FA_LAUNCH("k_spmm");
+    return FA_OK;
 }
 
 extern "C" size_t fa_dirichlet_work_bytes(int64_t n) {

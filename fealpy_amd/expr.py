@@ -31,7 +31,7 @@ import torch
 
 __all__ = ["x", "y", "z", "sin", "cos", "exp", "cell", "Sep", "Affine"]
 
-SIN, COS, EXP, AFFINE, POW = range(5)
+SIN, COS, EXP, AFFINE, POW, SINPI, COSPI = range(7)
 MAX_TERMS, MAX_FACTORS = 16, 48
 
 
@@ -216,6 +216,8 @@ class Sep:
                 keep.append(fld)
                 st.field[i] = fld.data_ptr()
             for kind, d, a, b in fs:
+                if kind in (SIN, COS):        # sin(a s + b) = sinpi(a/pi s + b/pi): a cheap argument reduction
+                    kind, a, b = (SINPI if kind == SIN else COSPI), a / math.pi, b / math.pi
                 st.kind[k], st.dim[k], st.a[k], st.b[k] = kind, d, a, b
                 k += 1
             st.term_end[i] = k

@@ -153,9 +153,57 @@ class BilinearForm(Form):
         return CSRTensor(pat.indptr, pat.indices, values, self.sparse_shape)
 
     def __matmul__(self, u):
-        if self._M is None:
-            self.assembly()
-        return self._M @ u
+        """self @ u.  With an assembled matrix, its product; otherwise without
+        assembling (bilinear_form.py:126-158): each group's element tensors
+        applied cell by cell to the gathered u, then summed per DOF in
+        ascending cell order; the form keeps no matrix (``_M`` stays None).
+        A 2-D u holds one vector per row (batch first), as in the reference.
+        Element tensors are recomputed on each product unless every
+        integrator keeps its data (``keep_data()``)."""
+        if self._M is not None:
+            return self._M @ u
+        self.check_space()
+        if u.ndim == 2:
+            return torch.stack([self._apply_local(u[b]) for b in range(u.shape[0])])
+        if u.ndim != 1:
+            raise ValueError("u must be a 1-D or 2-D tensor")
+        return self._apply_local(u)
+
+    def _local_tensors(self, dtype):
+        keep = all(getattr(it, '_keep_data', False) for it in self.integrators.values())
+        key = ('local', dtype)
+        cached = getattr(self, '_local', None)
+        if keep and cached is not None and cached[0] == key:
+            return cached[1]
+        space = self._spaces[0]
+        full_map = space.cell_to_dof()
+        groups = []
+        for elem, e2d in self._groups(dtype):
+            same = e2d.shape == full_map.shape and e2d.data_ptr() == full_map.data_ptr()
+            pat = pattern_of(space) if same else pattern_of(space, e2d)
+            groups.append((elem.contiguous(), pat))
+        self._local = (key, groups) if keep else None
+        return groups
+
+    def _apply_local(self, u):
+        L = nat.lib()
+        space = self._spaces[0]
+        gdof = space.number_of_global_dofs()
+        if u.shape[0] != gdof:
+            raise ValueError(f"u of length {u.shape[0]} for a space with {gdof} DOFs")
+        dt = u.dtype if u.dtype in (torch.float64, torch.float32) else torch.float64
+        x = u.to(dt).contiguous()
+        out = None
+        for elem, pat in self._local_tensors(dt):
+            elem = elem.to(dt)
+            v = torch.empty((pat.nc, pat.ldof), dtype=dt, device=x.device)
+            nat.check(L.fa_element_apply(nat.dtype_code(dt), pat.nc, pat.ldof, nat.ptr(pat.cell2dof), nat.ptr(elem),
+                                         elem.stride(0), nat.ptr(x), nat.ptr(v), nat.stream()), "fa_element_apply")
+            y = pat.gather_vector(v)
+            out = y if out is None else _axpby_sum(out, y)
+        if out is None:
+            raise RuntimeError("no integrator was added to the form")
+        return out
 
     @property
     def T(self):

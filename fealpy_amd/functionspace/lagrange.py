@@ -34,6 +34,18 @@ class Function:
     def __call__(self, bcs, index=_S):
         return self.space.value(self.array, bcs, index)
 
+    @property
+    def grad_value(self):
+        """Barycentric callable bcs -> (NC, NQ, GD) gradient of the function
+        (function.py's attribute forwarding of space.grad_value)."""
+        def gv(bcs, index=_S):
+            return self.space.grad_value(self.array, bcs, index)
+        gv.coordtype = 'barycentric'
+        return gv
+
+    def value(self, bcs, index=_S):
+        return self.space.value(self.array, bcs, index)
+
     def __len__(self):
         return self.array.shape[0]
 
@@ -142,7 +154,10 @@ class LagrangeFESpace:
             if torch.is_tensor(gd):
                 arr[idx] = gd[idx].to(arr.dtype)
             elif callable(gd):
-                arr[idx] = torch.as_tensor(gd(pts), dtype=arr.dtype, device=arr.device)
+                val = torch.as_tensor(gd(pts), dtype=arr.dtype, device=arr.device)
+                if arr.ndim == 2 and val.ndim == 1:      # one value per boundary DOF, every column
+                    val = val[:, None]
+                arr[idx] = val
             elif isinstance(gd, (int, float)):
                 arr[idx] = gd
             else:
@@ -173,6 +188,13 @@ class LagrangeFESpace:
         phi = self.basis(bc)[0]                                    # (NQ, ldof)
         c2d = self.cell_to_dof(index).long()
         return torch.einsum('ql, cl -> cq', phi.to(uh.dtype), uh[c2d])
+
+    def grad_value(self, uh, bc, index=_S):
+        """(NC, NQ, GD) gradient of uh at barycentric points
+        (lagrange_fe_space.py:226-235)."""
+        gphi = self.grad_basis(bc, index=index, variable='x')        # (NC, NQ, ldof, GD)
+        c2d = self.cell_to_dof(index).long()
+        return torch.einsum('cqlm, cl -> cqm', gphi.to(uh.dtype), uh[c2d])
 
     def function(self, array=None, dtype=None):
         return Function(self, array, dtype)

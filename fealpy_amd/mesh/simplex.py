@@ -272,18 +272,47 @@ class SimplexMesh:
         return flag
 
     # -- L2 error (reporting; fealpy/mesh/mesh_base.py:545-608) ------------------
+    def to_vtk(self, fname=None, etype='cell', index=_S):
+        """Mesh (and its nodedata / celldata) as a VTK unstructured grid
+        (tetrahedron_mesh.py:1536-1573, triangle_mesh.py's to_vtk).  With no
+        file name, returns (node, flat (NV, v0, ...) cell list, VTK cell type,
+        NC) as the reference does."""
+        from .vtu import write_vtu
+        ent = self.entity(etype)[index]
+        nc, nv = ent.shape
+        cl = torch.cat([torch.full((nc, 1), nv, dtype=ent.dtype, device=ent.device), ent], dim=1)
+        ctype = {2: 3, 3: 5, 4: 10}[nv]            # VTK_LINE, VTK_TRIANGLE, VTK_TETRA
+        data = {'cell': self.celldata, 'face': self.facedata, 'edge': self.edgedata}.get(etype, self.celldata)
+        if fname is None:
+            return self.node, cl.reshape(-1), ctype, nc
+        write_vtu(fname, self.node.detach().cpu().numpy(), cl.reshape(-1).cpu().numpy(), nc, ctype,
+                  nodedata=self.nodedata, celldata=data)
+        return None
+
     def error(self, u, v, q=3, power=2, celltype=False):
+        """(sum_c int_c |u - v|^power)^(1/power) (mesh_base.py:545-608).  u, v:
+        cartesian or barycentric callables or (NC, NQ[, ...]) values; vector
+        values (gradients: mesh.error(grad_u, uh.grad_value) is the H1
+        semi-norm error) are summed over their components."""
         qf = self.quadrature_formula(q)
         bcs, ws = qf.get_quadrature_points_and_weights()
         ws = ws.to(self.device)
         ps = self.bc_to_point(bcs)
         uu = u(bcs) if getattr(u, 'coordtype', None) == 'barycentric' else (u(ps) if callable(u) else u)
         vv = v(bcs) if getattr(v, 'coordtype', None) == 'barycentric' else (v(ps) if callable(v) else v)
+        uu, vv = torch.as_tensor(uu, device=self.device), torch.as_tensor(vv, device=self.device)
+        if uu.ndim == 2 and vv.ndim == 3 and vv.shape[-1] == 1:
+            uu = uu[..., None]
         f = torch.abs(uu - vv) ** power
-        if f.ndim == 1:
-            f = f[:, None]
         cm = self.entity_measure('cell')
-        e = torch.einsum('q, cq..., c -> c...', ws, f, cm)
+        if f.ndim == 0:                             # a constant function
+            e = f * cm
+        elif f.ndim == 1 and f.shape[0] == self.GD:  # a constant vector
+            e = cm[:, None] * f
+        else:
+            if f.ndim == 1:
+                f = f[:, None]
+            e = torch.einsum('q, cq..., c -> c...', ws.to(f.dtype), f, cm)
         if not celltype:
             return torch.sum(e) ** (1 / power)
         return torch.sum(e.reshape(e.shape[0], -1), dim=1) ** (1 / power)

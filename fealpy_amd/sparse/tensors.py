@@ -24,6 +24,22 @@ def _spmv(A, x):
     return y
 
 
+def _spmm(A, X):
+    """A X for a 2-D X (ncols, nvec): one launch over all columns (fa_spmm)."""
+    L = nat.lib()
+    if X.dtype != A.values.dtype:
+        raise TypeError(f"SpMV operand dtype {X.dtype} differs from matrix values {A.values.dtype}")
+    if X.shape[0] != A.shape[1]:
+        raise ValueError(f"SpMV operand of shape {tuple(X.shape)} for a matrix of shape {A.shape}")
+    X = X.contiguous()
+    Y = torch.empty((A.shape[0], X.shape[1]), dtype=X.dtype, device=X.device)
+    if X.shape[1] == 0:
+        return Y
+    nat.check(L.fa_spmm(nat.dtype_code(X.dtype), A.shape[0], A.nnz, X.shape[1], nat.ptr(A.crow), nat.ptr(A.col),
+                        nat.ptr(A.values), nat.ptr(X), nat.ptr(Y), nat.stream()), "fa_spmm")
+    return Y
+
+
 class SparseTensor:
     pass
 
@@ -87,7 +103,7 @@ class CSRTensor(SparseTensor):
         if other.ndim == 1:
             return _spmv(self, other)
         if other.ndim == 2:
-            return torch.stack([_spmv(self, other[:, k]) for k in range(other.shape[1])], dim=1)
+            return _spmm(self, other)
         raise ValueError("`other` must be a 1-D or 2-D array.")
 
     __matmul__ = matmul

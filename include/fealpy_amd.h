@@ -63,7 +63,8 @@ enum {
  *                * prod_{k in [term_end[t-1], term_end[t])} g_k(x[dim[k]])
  *
  *   g = sin(a s + b) | cos(a s + b) | exp(a s + b) | a s + b | s^a (a a whole
- *   number >= 0), s the coordinate dim[k] of the point;
+ *   number >= 0) | sin(pi (a s + b)) | cos(pi (a s + b)), s the coordinate
+ *   dim[k] of the point;
  *   x = sum_v bcs[q*(TD+1) + v] * node[cell[c*(TD+1) + v]] (bc_to_point,
  *   fealpy/mesh/mesh_base.py:454-481).
  *
@@ -74,7 +75,8 @@ enum {
  * struct itself lives in device memory. */
 #define FA_SEP_MAX_TERMS 16
 #define FA_SEP_MAX_FACTORS 48
-enum { FA_SEP_SIN = 0, FA_SEP_COS = 1, FA_SEP_EXP = 2, FA_SEP_AFFINE = 3, FA_SEP_POW = 4 };
+enum { FA_SEP_SIN = 0, FA_SEP_COS = 1, FA_SEP_EXP = 2, FA_SEP_AFFINE = 3, FA_SEP_POW = 4, FA_SEP_SINPI = 5,
+       FA_SEP_COSPI = 6 };
 typedef struct {
     int32_t nterms, nfactors;
     int32_t term_end[FA_SEP_MAX_TERMS];
@@ -158,6 +160,13 @@ int fa_gather_matrix(int dtype, int64_t gdof, int32_t ldof, const int32_t* cell2
 int fa_gather_vector(int dtype, int64_t gdof, int32_t ldof,
                      const int32_t* adj_ptr, const int32_t* adj,
                      const void* elem, void* out, fa_stream_t stream);
+/* Element matrices applied to a global vector, cell by cell:
+ *   v[c][i] = sum_j elem[c*elem_stride + i*ldof + j] * x[cell2dof[c][j]]
+ * (the einsum of BilinearForm.__matmul__ without an assembled matrix,
+ * fealpy/fem/bilinear_form.py:126-158); fa_gather_vector then sums v into
+ * the global result in ascending cell order (its index_add). */
+int fa_element_apply(int dtype, int64_t nc, int32_t ldof, const int32_t* cell2dof, const void* elem,
+                     int64_t elem_stride, const void* x, void* v, fa_stream_t stream);
 
 /* Element matrices of LinearElasticityIntegrator on a vector Lagrange space
  * (fealpy/fem/linear_elasticity_integrator.py:60-186, isotropic D of
@@ -288,6 +297,11 @@ int fa_dirichlet_apply(int dtype, int64_t n, const int32_t* indptr, const int32_
  * y = A x.  Replaces CSRTensor.matmul -> bm.csr_spmm -> scipy csr_matvec
  * (fealpy/sparse/csr_tensor.py:411-452, backend/numpy_backend.py:181-199). */
 int fa_spmv(int dtype, int64_t n, int64_t nnz, const int32_t* indptr, const int32_t* indices,
+            const void* values, const void* x, void* y, fa_stream_t stream);
+/* Y = A X for nvec right-hand sides, X (ncols, nvec) and Y (n, nvec)
+ * row-major: the 2-D branch of bm.csr_spmm, SciPy csr_matvecs
+ * (backend/numpy_backend.py:181-199). */
+int fa_spmm(int dtype, int64_t n, int64_t nnz, int64_t nvec, const int32_t* indptr, const int32_t* indices,
             const void* values, const void* x, void* y, fa_stream_t stream);
 
 /* ------------------------------------------------------------ vector ops --
@@ -478,6 +492,24 @@ size_t fa_pcg_work_bytes(int dtype, int64_t n);
 int fa_pcg(int dtype, const fa_sell_t* A, const fa_mg_t* M, const void* b, void* x, double atol, double rtol,
            int64_t maxit, int32_t check_every, int64_t* niter_host, double* residual_host,
            void* work, size_t work_bytes, fa_stream_t stream);
+
+/* fa_pcg without a workspace argument.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code in a tight serving
+ * loop, harnesses that forbid hipMalloc).  Same contract, arguments and
+ * results as fa_pcg (bitwise the same x, niter and residual); the vectors,
+ * partial sums and state live in a file-scope device array of
+ * FA_PCG_STATIC_BYTES (the workspace of FA_PCG_STATIC_MAX_ROWS FP64 rows),
+ * one per GPU, held until the process ends.  A->n above
+ * FA_PCG_STATIC_MAX_ROWS returns FA_EINVAL.  It calls no allocator.  The
+ * array's device address is looked up on the first call on each device, so
+ * `stream` must belong to the current device.  Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (other streams, other
+ * threads) race and must be serialised by the caller. */
+#define FA_PCG_STATIC_MAX_ROWS 262144
+#define FA_PCG_STATIC_BYTES (5u * 262144u * 8u + 32768u + 4096u)
+int fa_pcg_static(int dtype, const fa_sell_t* A, const fa_mg_t* M, const void* b, void* x, double atol, double rtol,
+                  int64_t maxit, int32_t check_every, int64_t* niter_host, double* residual_host,
+                  fa_stream_t stream);
 
 /* fa_pcg with its iterations replayed from a HIP graph, for callers whose
  * solves are bound by kernel-launch cost (small systems, many concurrent

@@ -39,6 +39,7 @@ SOURCES = {
     "ref_scalar_mass_integrator.npz": ("fem/scalar_mass_integrator_data.py",
                                        ["triangle_mesh_one_box"]),
     "ref_stroud_quadrature.npz": ("quadrature/stroud_quadrature_data.py", ["data"]),
+    "ref_bilinear_form.npz": ("fem/bilinear_form_data.py", ["mesh_data"]),
 }
 
 _DTYPES = {"int32": np.int32, "int64": np.int64, "float64": np.float64,

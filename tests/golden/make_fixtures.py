@@ -138,6 +138,67 @@ def diffusion3d_p1_lognormal(R, n=8, seed=20261202):
     return out
 
 
+def _sha(*arrays):
+    import hashlib
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def diffusion3d_p1_summary(R, n=128, seed=20261202, nsample=65536):
+    """The lognormal P1 problem at full size (from_box 128^3, q=2): too large
+    to store, so the fixture holds the SHA-256 of the canonical CSR patterns
+    (int64 indptr bytes, int32 indices bytes) before and after the Dirichlet
+    condition, value statistics (sum, sum |.|, sum of squares, min, max) of
+    A, F and their post-BC forms, values at seeded random positions, a tight
+    solution's norm and samples, and the L2 error."""
+    mesh = R.TetrahedronMesh.from_box([0, 1, 0, 1, 0, 1], n, n, n)
+    NC = mesh.number_of_cells()
+    k = np.exp(np.random.default_rng(seed).standard_normal(NC))
+
+    @R.cartesian
+    def u(p):
+        return np.sin(PI * p[..., 0]) * np.sin(PI * p[..., 1]) * np.sin(PI * p[..., 2])
+
+    @R.barycentric
+    def f(bcs, index=None):
+        return 3 * PI**2 * k[:, None] * u(mesh.bc_to_point(bcs))
+
+    space = R.LagrangeFESpace(mesh, 1)
+    bform = R.BilinearForm(space)
+    bform.add_integrator(R.ScalarDiffusionIntegrator(k, q=2))
+    A = bform.assembly()
+    lform = R.LinearForm(space)
+    lform.add_integrator(R.ScalarSourceIntegrator(f, q=2))
+    F = np.asarray(lform.assembly())
+    bc = R.DirichletBC(space, gd=u)
+    A2, F2 = bc.apply(A, F)
+    out = {"n": n, "seed": seed, "gdof": space.number_of_global_dofs()}
+    g = np.random.default_rng(seed + 1)
+    for name, M in (("A", A), ("Abc", A2)):
+        ip, ix, d = canon(M)
+        out[name + "_nnz"] = len(ix)
+        out[name + "_pattern_sha256"] = _sha(ip.astype(np.int64), ix.astype(np.int32))
+        out[name + "_stats"] = np.array([d.sum(), np.abs(d).sum(), (d * d).sum(), d.min(), d.max()])
+        pos = np.sort(g.choice(len(ix), nsample, replace=False))
+        out[name + "_sample_pos"], out[name + "_sample_val"] = pos, d[pos]
+        del ip, ix, d
+    for name, v in (("F", F), ("Fbc", np.asarray(F2))):
+        out[name + "_stats"] = np.array([v.sum(), np.abs(v).sum(), (v * v).sum(), v.min(), v.max()])
+        pos = np.sort(g.choice(len(v), nsample, replace=False))
+        out[name + "_sample_pos"], out[name + "_sample_val"] = pos, v[pos]
+    x, info = R.cg(A2, F2, atol=1e-14, rtol=1e-13, maxit=50000, returninfo=True)
+    x = np.asarray(x)
+    out["x_norm"], out["x_niter"], out["x_res"] = np.linalg.norm(x), info["niter"], info["residual"]
+    pos = np.sort(g.choice(len(x), nsample, replace=False))
+    out["x_sample_pos"], out["x_sample_val"] = pos, x[pos]
+    uh = space.function()
+    uh[:] = x
+    out["l2_error"] = mesh.error(u, uh, q=3)
+    return out
+
+
 def _k_smooth(p):
     x, y, z = p[..., 0], p[..., 1], p[..., 2]
     return 1 + 0.5 * np.sin(2 * PI * x) * np.cos(2 * PI * y) * z
@@ -224,7 +285,20 @@ def elasticity3d_p2(R, n=2, lam=1.0, mu=0.5):
 
     lform = R.LinearForm(tspace)
     lform.add_integrator(R.VectorSourceIntegrator(body, q=5))
-    out["F"] = np.asarray(lform.assembly())
+    F = lform.assembly()
+    out["F"] = np.asarray(F)
+
+    @R.cartesian
+    def zero(p):
+        return np.zeros(p.shape[:-1] + (3,))
+
+    bc = R.DirichletBC(tspace, gd=zero)              # clamped on every face
+    out["is_bd"] = np.asarray(bc.is_boundary_dof)
+    A2, F2 = bc.apply(bform.assembly(), F)
+    put(out, "Abc", A2)
+    out["Fbc"] = np.asarray(F2)
+    x, info = R.cg(A2, F2, atol=1e-14, rtol=1e-14, maxit=20000, returninfo=True)
+    out["x"], out["x_niter"], out["x_res"] = np.asarray(x), info["niter"], info["residual"]
     out.update(n=n, lam=lam, mu=mu)
     return out
 
@@ -251,6 +325,8 @@ def main():
     jobs = {
         "fixtures_poisson2d_p1.npz": lambda: poisson2d_p1(R),
         "fixtures_diffusion3d_p1_lognormal.npz": lambda: diffusion3d_p1_lognormal(R),
+        "fixtures_diffusion3d_p1_lognormal_24.npz": lambda: diffusion3d_p1_lognormal(R, n=24),
+        "fixtures_diffusion3d_p1_128_summary.npz": lambda: diffusion3d_p1_summary(R),
         "fixtures_highorder3d_p3.npz": lambda: highorder3d(R, 3, 5, 2),
         "fixtures_highorder3d_p4.npz": lambda: highorder3d(R, 4, 6, 2),
         "fixtures_poisson2d_p2_fourier.npz": lambda: poisson2d_p2_fourier(R),

@@ -61,6 +61,16 @@ def test_elasticity_matches_reference_fixture(golden):
     np.testing.assert_allclose(_np(A.values), g["A_data"], rtol=0, atol=1e-12 * np.abs(g["A_data"]).max())
     F = fa.LinearForm(ts).add_integrator(fa.VectorSourceIntegrator(_body_torch(lam, mu), q=5)).assembly()
     np.testing.assert_allclose(_np(F), g["F"], rtol=0, atol=1e-12 * np.abs(g["F"]).max())
+    # clamped Dirichlet condition of the tensor space: flags, filtered pattern, values, solution
+    bc = fa.DirichletBC(ts, gd=0.0)
+    np.testing.assert_array_equal(_np(bc.is_boundary_dof), g["is_bd"])
+    A2, F2 = bc.apply(A, F)
+    np.testing.assert_array_equal(_np(A2.crow).astype(np.int64), g["Abc_indptr"])
+    np.testing.assert_array_equal(_np(A2.col), g["Abc_indices"])
+    np.testing.assert_allclose(_np(A2.values), g["Abc_data"], rtol=0, atol=1e-12 * np.abs(g["Abc_data"]).max())
+    np.testing.assert_allclose(_np(F2), g["Fbc"], rtol=0, atol=1e-12 * np.abs(g["Fbc"]).max())
+    x = fa.cg(A2, F2, atol=1e-14, rtol=1e-14, maxit=20000)
+    np.testing.assert_allclose(_np(x), g["x"], rtol=0, atol=1e-10 * np.abs(g["x"]).max())
 
 
 @pytest.mark.parametrize("p,prio", [(1, False), (2, False), (2, True), (3, False), (4, False), (4, True)])

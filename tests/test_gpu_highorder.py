@@ -27,9 +27,10 @@ def test_mfma_elements_match_fp64(p, q, precision, tol):
     space = fa.LagrangeFESpace(mesh, p)
     ref = GroupIntegrator(fa.ScalarDiffusionIntegrator(kx, q=q), fa.ScalarMassIntegrator(1.0, q=q)).assembly(space)
     m = f"mfma_{precision}"
-    got = GroupIntegrator(fa.ScalarDiffusionIntegrator(kx, q=q, method=m),
-                          fa.ScalarMassIntegrator(1.0, q=q, method=m)).assembly(space)
+    grp = GroupIntegrator(fa.ScalarDiffusionIntegrator(kx, q=q, method=m), fa.ScalarMassIntegrator(1.0, q=q, method=m))
+    got = grp.assembly(space, dtype=torch.float32)
     assert got.dtype == torch.float32 and got.shape == ref.shape
+    assert torch.equal(grp.assembly(space), got.double())        # default: the same values as FP64
     err = float((got.double() - ref).abs().max()) / float(ref.abs().max())
     assert err < tol, err
 
@@ -40,7 +41,8 @@ def test_mfma_assembled_matrix_and_cell_coefficient():
     space = fa.LagrangeFESpace(mesh, 3)
     kc = torch.as_tensor(np.exp(np.random.default_rng(2).standard_normal(mesh.number_of_cells())), device=DEV)
     A64 = fa.BilinearForm(space).add_integrator(fa.ScalarDiffusionIntegrator(kc, q=5)).assembly()
-    A32 = fa.BilinearForm(space).add_integrator(fa.ScalarDiffusionIntegrator(kc, q=5, method="mfma_fp32")).assembly()
+    A32 = fa.BilinearForm(space).add_integrator(fa.ScalarDiffusionIntegrator(kc, q=5, method="mfma_fp32")).assembly(
+        dtype=torch.float32)
     assert torch.equal(A64.col, A32.col) and A32.values.dtype == torch.float32
     err = float((A32.values.double() - A64.values).abs().max()) / float(A64.values.abs().max())
     assert err < 2e-6, err

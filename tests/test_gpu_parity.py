@@ -93,6 +93,8 @@ def test_diffusion3d_p1_lognormal(golden):
     _assert_vec(b2, F["Fbc"])
     x = fa.cg(A2, b2, atol=1e-14, rtol=1e-14, maxit=20000)
     _assert_vec(x, F["x"], 1e-10)
+    err = float(mesh.error(fa.cartesian(u_sin3), space.function(x)))
+    assert abs(err - float(F["l2_error"])) <= 1e-8 * float(F["l2_error"])
 
 
 def _k_smooth(p):
@@ -126,6 +128,8 @@ def test_highorder3d_diffusion_mass(golden, p):
     _assert_csr(A2, F, "Abc")
     x = fa.cg(A2, b2, atol=1e-14, rtol=1e-14, maxit=20000)
     _assert_vec(x, F["x"], 1e-10)
+    err = float(mesh.error(fa.cartesian(u_sin3), space.function(x), q=q))
+    assert abs(err - float(F["l2_error"])) <= 1e-8 * float(F["l2_error"])
 
 
 def test_poisson2d_p2_fourier(golden):
@@ -149,6 +153,9 @@ def test_poisson2d_p2_fourier(golden):
     _assert_csr(A2, F, "Abc")
     x = fa.cg(A2, b2, atol=1e-14, rtol=1e-14, maxit=20000)
     _assert_vec(x, F["x"], 1e-10)
+    zero = fa.cartesian(lambda p: torch.zeros(p.shape[:-1], dtype=p.dtype, device=p.device))
+    err = float(mesh.error(zero, space.function(x)))
+    assert abs(err - float(F["l2_error"])) <= 1e-8 * float(F["l2_error"])
 
 
 def test_assembly_and_cg_are_bitwise_reproducible():

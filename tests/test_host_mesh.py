@@ -268,3 +268,30 @@ def test_sell_index_lives_with_its_pattern(monkeypatch):
 def test_maxit_argument_follows_the_reference():
     from fealpy_amd.solver.cg import maxit_arg
     assert maxit_arg(None) == -1 and maxit_arg(0) == 1 and maxit_arg(1) == 1 and maxit_arg(50) == 50
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_to_vtk_round_trip(tmp_path, dim):
+    """to_vtk writes points, (NV, v...) connectivity, offsets, VTK types and
+    the mesh's node / cell data; reading the file back gives them again."""
+    from fealpy_amd.mesh.vtu import read_vtu_arrays
+    cpu = torch.device("cpu")
+    mesh = (fa.TriangleMesh.from_box(nx=3, ny=2, device=cpu) if dim == 2
+            else fa.TetrahedronMesh.from_box(nx=2, ny=2, nz=1, device=cpu))
+    nc, nn = mesh.number_of_cells(), mesh.number_of_nodes()
+    mesh.nodedata['u'] = torch.arange(nn, dtype=torch.float64)
+    mesh.celldata['k'] = torch.linspace(1, 2, nc, dtype=torch.float64)
+    node, cl, ctype, ncl = mesh.to_vtk()
+    assert ncl == nc and ctype == (5 if dim == 2 else 10)
+    assert torch.equal(cl.reshape(nc, -1)[:, 1:], mesh.cell.to(cl.dtype))
+    f = tmp_path / "m.vtu"
+    mesh.to_vtk(fname=str(f))
+    a = read_vtu_arrays(str(f))
+    assert a["_counts"] == (nn, nc)
+    pts = a["Points"]
+    np.testing.assert_array_equal(pts[:, :dim], mesh.node.numpy())
+    np.testing.assert_array_equal(a["connectivity"].reshape(nc, -1), mesh.cell.numpy())
+    np.testing.assert_array_equal(a["offsets"], np.arange(1, nc + 1) * (dim + 1))
+    assert np.all(a["types"] == ctype)
+    np.testing.assert_array_equal(a["u"], mesh.nodedata['u'].numpy())
+    np.testing.assert_array_equal(a["k"], mesh.celldata['k'].numpy())
