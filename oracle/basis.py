@@ -2,7 +2,6 @@
 
 TEST INFRASTRUCTURE ONLY (see ``oracle/__init__.py``).
 """
-import importlib.util
 import os
 from itertools import combinations_with_replacement
 from math import factorial
@@ -12,13 +11,13 @@ from scipy.special import roots_jacobi
 
 
 def _load_tables():
-    # The rule tables are pure data shared with the product; load the file
-    # directly so the oracle does not import the product package.
-    path = os.path.join(os.path.dirname(__file__), "..", "fealpy_amd", "quadrature", "tables.py")
-    spec = importlib.util.spec_from_file_location("_oracle_quad_tables", path)
-    mod = importlib.util.module_from_spec(spec)
-    spec.loader.exec_module(mod)
-    return mod.TRIANGLE_RULES, mod.TETRAHEDRON_RULES
+    # quadrature_rules.npz holds the rows of fealpy/quadrature/triangle.py:16-329
+    # and fealpy/quadrature/tetrahedron.py:7-243, parsed from those files by
+    # tests/golden/extract_reference_data.py (nothing here reads the product's copy).
+    d = np.load(os.path.join(os.path.dirname(__file__), "quadrature_rules.npz"))
+    tri = {int(k[4:]): d[k] for k in d.files if k.startswith("tri_")}
+    tet = {int(k[4:]): d[k] for k in d.files if k.startswith("tet_")}
+    return tri, tet
 
 
 _TRI, _TET = _load_tables()

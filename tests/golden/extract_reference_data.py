@@ -128,11 +128,47 @@ def extract(path, names):
     return out
 
 
+RULE_SOURCES = {"tri": "fealpy/quadrature/triangle.py", "tet": "fealpy/quadrature/tetrahedron.py"}
+ORACLE_RULES = os.path.join(HERE, "..", "..", "oracle", "quadrature_rules.npz")
+
+
+def extract_rules(path, key):
+    """The ``if index == k: A = bm.tensor([...])`` branches of a reference
+    rule file's ``make`` method, as ``{f"{key}_{k}": (NQ, TD+2) float64}``."""
+    tree = ast.parse(open(path).read())
+    out = {}
+
+    def visit_if(node):
+        test = node.test
+        if isinstance(test, ast.Compare) and isinstance(test.left, ast.Name) \
+                and test.left.id == "index" and isinstance(test.comparators[0], ast.Constant):
+            k = test.comparators[0].value
+            for st in node.body:
+                if isinstance(st, ast.Assign) and isinstance(st.value, ast.Call) \
+                        and isinstance(st.value.func, ast.Attribute) and st.value.func.attr == "tensor":
+                    out[f"{key}_{k}"] = np.array(_eval(st.value.args[0]), dtype=np.float64)
+        for st in node.orelse:
+            if isinstance(st, ast.If):
+                visit_if(st)
+
+    for node in ast.walk(tree):
+        if isinstance(node, ast.FunctionDef) and node.name == "make":
+            for st in node.body:
+                if isinstance(st, ast.If):
+                    visit_if(st)
+    return out
+
+
 def main():
     for fname, (rel, names) in SOURCES.items():
         data = extract(os.path.join(REF, rel), names)
         np.savez_compressed(os.path.join(HERE, fname), **data)
         print(fname, len(data), "arrays", file=sys.stderr)
+    rules = {}
+    for key, rel in RULE_SOURCES.items():
+        rules.update(extract_rules(os.path.join(os.path.dirname(REF), rel), key))
+    np.savez_compressed(ORACLE_RULES, **rules)
+    print(os.path.basename(ORACLE_RULES), len(rules), "rules", file=sys.stderr)
 
 
 if __name__ == "__main__":

@@ -117,6 +117,18 @@ def test_quadrature_exactness(TD, q):
         assert abs(got - exact) < 5e-13, (a, got, exact)
 
 
+def test_product_rule_tables_equal_the_oracle_copy():
+    """The product's typed-in rule tables against the oracle's copy, which was
+    parsed from fealpy/quadrature/{triangle,tetrahedron}.py: same points, same
+    order, same rounding."""
+    from fealpy_amd.quadrature.tables import TETRAHEDRON_RULES, TRIANGLE_RULES
+    from oracle.basis import _TET, _TRI
+    assert sorted(TRIANGLE_RULES) == sorted(_TRI) and sorted(TETRAHEDRON_RULES) == sorted(_TET)
+    for mine, ref in ((TRIANGLE_RULES, _TRI), (TETRAHEDRON_RULES, _TET)):
+        for q in ref:
+            np.testing.assert_array_equal(np.array(mine[q], dtype=np.float64), ref[q])
+
+
 @pytest.mark.parametrize("TD,p", [(2, p) for p in range(1, 5)] + [(3, p) for p in range(1, 5)])
 def test_shape_function_kronecker_and_gradient(TD, p):
     """phi_i(x_j) = delta_ij at the lattice points m/p; sum_i phi_i = 1; the
