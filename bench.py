@@ -13,11 +13,15 @@ expressions (fealpy_amd.expr), which the load-vector kernel evaluates at
 the quadrature points itself (``--torch-source``: the same source as a
 barycentric torch callable, evaluated on (NC, NQ, 3) points first).
 
-Reported ``value``: mixed precision (FP64 CG, FP32 V-cycle smoothing with
-FP16 D^-1 A on the finest levels) DOF/s.  The plain FP64 GPU path (FP64 CG,
-FP64 V-cycle) is timed beside it; both solutions are checked against a tight
-FP64 solve (relative 2-norm error <= 1e-8).  The same line carries the
-serving figures (solves/s, p50/p99) of the 4096-request mixed-size trace.
+Reported ``value``: DOF/s of the mixed-precision form: A assembled in FP32,
+FP32 Dirichlet filter and multigrid setup, FP32 multigrid-preconditioned CG
+corrections (FP16 D^-1 A smoothing on the finest levels), and FP64 residuals
+b - A x formed matrix-free from the FP64 geometry and coefficient (iterative
+refinement; no FP64 matrix is assembled).  The plain FP64 GPU path (FP64
+assembly, CG and V-cycle) and the FP64-CG form with an FP32 cycle are timed
+beside it; every solution is checked against a tight FP64 solve (relative
+2-norm error <= 1e-8).  The same line carries the serving figures (solves/s,
+p50/p99) of the 4096-request mixed-size trace.
 
     python bench.py [--steps K] [--warmup W] [--n 128] [--no-serving] [--no-cpu]
 """
@@ -47,6 +51,8 @@ PCG_SPMV_TRAFFIC_128 = None
 # queues the process asks for (the slots' streams, the default stream and
 # torch's side streams each get one); read by HIP when it initialises
 SERVING_SLOTS = 4
+# relative tolerance of each FP32 inner solve of the mixed (refinement) form
+INNER_RTOL = 1e-4
 SERVING_HW_QUEUES = 8
 
 
@@ -100,6 +106,8 @@ def make_problem(n, dev, torch_source=False):
 
 
 def step(space, k, u, f, mg, rtol, maxit=400):
+    """The FP64 form (and the FP64-outer form with an FP32 cycle): FP64
+    assembly, Dirichlet condition, multigrid setup, MG-preconditioned CG."""
     import fealpy_amd as fa
     A = fa.BilinearForm(space).add_integrator(fa.ScalarDiffusionIntegrator(k, q=2)).assembly()
     F = fa.LinearForm(space).add_integrator(fa.ScalarSourceIntegrator(f, q=2)).assembly()
@@ -107,6 +115,23 @@ def step(space, k, u, f, mg, rtol, maxit=400):
     mg.setup(A2)
     x, info = fa.cg(A2, F2, M=mg, atol=0.0, rtol=rtol, maxit=maxit, returninfo=True)
     return x, info, A2, F2
+
+
+def step_refine(space, k, u, f, mg, rtol, inner_rtol=INNER_RTOL):
+    """The mixed form: A assembled in FP64 (the operator of the FP64
+    residuals), its Dirichlet-filtered FP32 copy made in the filter pass
+    (each value rounded once), FP32 multigrid setup, FP32 MG-PCG corrections
+    and FP64 residuals b - A x by one SpMV pass (iterative refinement)."""
+    import torch
+    import fealpy_amd as fa
+    from fealpy_amd.solver.refine import cg_refine
+    A = fa.BilinearForm(space).add_integrator(fa.ScalarDiffusionIntegrator(k, q=2)).assembly()
+    F = fa.LinearForm(space).add_integrator(fa.ScalarSourceIntegrator(f, q=2)).assembly()
+    bc = fa.DirichletBC(space, gd=u)
+    A2 = bc.apply_matrix(A, dtype=torch.float32)
+    mg.setup(A2)
+    x, info = cg_refine(A, F, A2, mg, bc, rtol=rtol, inner_rtol=inner_rtol, returninfo=True)
+    return x, info, A2, F
 
 
 def timed(fn, K, W):
@@ -144,7 +169,32 @@ def phase_breakdown(space, k, u, f, mg, rtol):
     return ph
 
 
-def in_step_kernels(space, k, u, f, mg, rtol):
+def phase_breakdown_refine(space, k, u, f, mg, rtol):
+    import torch
+    import fealpy_amd as fa
+    from fealpy_amd.solver.refine import cg_refine
+    ph = {}
+
+    def t(name, fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        ph[name] = round((time.perf_counter() - t0) * 1e3, 3)
+        return r
+    A = t("assemble_A_ms", lambda: fa.BilinearForm(space).add_integrator(
+        fa.ScalarDiffusionIntegrator(k, q=2)).assembly())
+    F = t("assemble_F_ms", lambda: fa.LinearForm(space).add_integrator(fa.ScalarSourceIntegrator(f, q=2)).assembly())
+    bc = fa.DirichletBC(space, gd=u)
+    A2 = t("dirichlet_ms", lambda: bc.apply_matrix(A, dtype=torch.float32))
+    t("mg_setup_ms", lambda: mg.setup(A2))
+    x, info = t("pcg_ms", lambda: cg_refine(A, F, A2, mg, bc, rtol=rtol, inner_rtol=INNER_RTOL, returninfo=True))
+    ph["pcg_iterations"] = info["niter"]
+    ph["refinements"] = info["refinements"]
+    return ph
+
+
+def in_step_kernels(stepfn, space, k, u, f, mg, rtol):
     """One more step of the form with the in-step kernel timer armed: HIP
     events on the launching stream around every launch of the step's
     dominant kernels (fa_timer_arm / fa_timer_read).  -> {kind: (launches, ms)}"""
@@ -152,11 +202,11 @@ def in_step_kernels(space, k, u, f, mg, rtol):
     from fealpy_amd import _native as nat
     kinds = {"pcg_spmv": nat.TIMER_PCG_SPMV, "a_pass1": nat.TIMER_P1_ROWS, "a_pass2": nat.TIMER_P1_SUM_ROWS,
              "f_pass1": nat.TIMER_P1_VECTORS, "f_pass2": nat.TIMER_P1_SUM_VALUES, "dirichlet": nat.TIMER_BC_APPLY,
-             "vcycle": nat.TIMER_VCYCLE}
+             "vcycle": nat.TIMER_VCYCLE, "p1_apply": nat.TIMER_P1_APPLY, "smooth0": nat.TIMER_SMOOTH0}
     torch.cuda.synchronize()
     nat.timer_arm(*kinds.values())
     try:
-        step(space, k, u, f, mg, rtol)
+        stepfn(space, k, u, f, mg, rtol)
         torch.cuda.synchronize()
         return {name: nat.timer_read(kd) for name, kd in kinds.items()}
     finally:
@@ -173,21 +223,40 @@ def _roof(kernel, algo_bytes, launches, ms, traffic=None, **extra):
     return out
 
 
-def rooflines(kt, mesh, gdof, nnz_a, nnz_bc, n):
-    """Roofline objects from the in-step kernel times.  Algorithmic bytes
-    are the compulsory traffic of each phase (inputs read once, outputs
-    written once):
-    - CG operator A p, FP64 values, int32 indices: 12 nnz + 4 (n+1) + 16 n;
-    - A values (FP64): 4 NC ldof (cells) + 8 GD NN (coordinates) + 8 NC (k) + 8 nnz;
-    - F (FP64): 4 NC ldof + 8 GD NN + 8 NC (k) + 8 gdof."""
+def rooflines(kt, mesh, gdof, nnz_a, nnz_bc, n, vb):
+    """Roofline objects from the in-step kernel times of one step.
+    Algorithmic bytes are the compulsory traffic of each kernel or phase
+    (inputs read once, outputs written once); vb = bytes per value of the
+    step's CG operator and assembled matrix (8 FP64, 4 FP32):
+    - CG operator A p on int16 column offsets: (vb + 2) nnz + 4 (n+1) + 2 vb n;
+    - A values: 4 NC ldof (cells) + 8 GD NN (coordinates) + 8 NC (k) + vb nnz;
+    - F (FP64): 4 NC ldof + 8 GD NN + 8 NC (k) + 8 gdof;
+    - matrix-free residual b - A x (FP64 in, FP32 out): 4 NC ldof + 8 GD NN +
+      8 NC + 8 gdof (x) + 8 gdof (b) + gdof (boundary flags) + 4 gdof (r).
+    The dominant kernel (largest in-step time among the CG operator and the
+    matrix-free residual) goes first."""
     nc, nn = mesh.number_of_cells(), mesh.number_of_nodes()
-    b_spmv = 12 * nnz_bc + 4 * (n + 1) + 16 * n
+    tname = "double" if vb == 8 else "float"
+    # the CG operator reads its values through the int16 column offsets of the
+    # finest smoother slices: vb + 2 bytes an entry (the int32-index formula
+    # has vb + 4), plus the row pointer and the vectors p (in) and Ap (out)
+    b_spmv = (vb + 2) * nnz_bc + 4 * (n + 1) + 2 * vb * n
     pl, pms = kt["pcg_spmv"]
-    tr = PCG_SPMV_TRAFFIC_128 if n == 2146689 else None
-    main = _roof("k_pcg_spmv<double>", b_spmv, pl, pms, tr,
-                 note="outer CG operator of the mixed form (FP64 values on the int16 column offsets of the "
-                      "finest smoother slices: it reads 10 B an entry where the algorithmic count has 12)")
-    b_a = 16 * nc + 24 * nn + 8 * nc + 8 * nnz_a
+    tr = PCG_SPMV_TRAFFIC_128 if (n == 2146689 and vb == 8) else None
+    spmv = _roof(f"k_pcg_spmv<{tname}>", b_spmv, pl, pms, tr,
+                 note=f"CG operator: values and int16 column offsets ({vb + 2} B an entry); with int32 indices "
+                      f"the count would be {(vb + 4) * nnz_bc + 4 * (n + 1) + 2 * vb * n} B")
+    b_r = 16 * nc + 24 * nn + 8 * nc + 8 * gdof + 8 * gdof + gdof + 4 * gdof
+    al, ams = kt["p1_apply"]
+    resid = _roof("k_p1_apply<3> (residual form)", b_r, al, ams) if al else None
+    # finest-level smoothing step on FP16 D^-1 A with int16 column offsets:
+    # (2 + 2) nnz + 5 FP32 vectors (dinv, b, x in, x out, d)
+    b_s = 4 * nnz_bc + 20 * n
+    sl, sms = kt["smooth0"]
+    smooth = _roof("k_cheb16* / k_residual16 (finest level)", b_s, sl, sms,
+                   note="FP16 D^-1 A values and int16 column offsets: (2 + 2) B an entry + 5 FP32 vectors") \
+        if sl else None
+    b_a = 16 * nc + 24 * nn + 8 * nc + vb * nnz_a
     b_f = 16 * nc + 24 * nn + 8 * nc + 8 * gdof
     (l1, m1), (l2, m2) = kt["a_pass1"], kt["a_pass2"]
     (l3, m3), (l4, m4) = kt["f_pass1"], kt["f_pass2"]
@@ -197,8 +266,14 @@ def rooflines(kt, mesh, gdof, nnz_a, nnz_bc, n):
                   pass1_us=round(m3 * 1e3 / max(l3, 1), 2), pass2_us=round(m4 * 1e3 / max(l4, 1), 2))
     vl, vms = kt["vcycle"]
     dl, dms = kt["dirichlet"]
-    return main, {"A": asm_a, "F": asm_f, "dirichlet_us": round(dms * 1e3 / max(dl, 1), 2),
-                  "vcycle_us": round(vms * 1e3 / max(vl, 1), 2), "vcycles": vl}
+    cands = [(pms, spmv), (ams, resid), (sms, smooth)]
+    cands = [c for c in cands if c[1] is not None]
+    cands.sort(key=lambda c: -c[0])
+    main = cands[0][1]
+    extra = {"A": asm_a, "F": asm_f, "dirichlet_us": round(dms * 1e3 / max(dl, 1), 2),
+             "vcycle_us": round(vms * 1e3 / max(vl, 1), 2), "vcycles": vl,
+             "other": [c[1] for c in cands[1:]]}
+    return main, extra
 
 
 def spmv_roofline(A, dtype):
@@ -695,23 +770,31 @@ def main():
     mg32 = GeometricMG(space, bc.is_boundary_dof, dtype=torch.float32)
     mg64 = GeometricMG(space, bc.is_boundary_dof, dtype=torch.float64)
 
+    mgr = GeometricMG(space, bc.is_boundary_dof, dtype=torch.float32)     # hierarchy of the FP32 operator
+
     # reference solution: FP64 multigrid CG to a tight tolerance (untimed)
     x_ref, info_ref, A2, F2 = step(space, k, u, f, mg64, 1e-14, maxit=1000)
     nref = float(torch.linalg.norm(x_ref))
 
-    t_mixed, (x_m, info_m, _, _) = timed(lambda: step(space, k, u, f, mg32, args.rtol), args.steps, args.warmup)
+    t_mixed, (x_m, info_m, _, _) = timed(lambda: step_refine(space, k, u, f, mgr, args.rtol), args.steps,
+                                         args.warmup)
     err_m = float(torch.linalg.norm(x_m - x_ref)) / nref
+    t_cyc, (x_c, info_c, _, _) = timed(lambda: step(space, k, u, f, mg32, args.rtol), args.steps, args.warmup)
+    err_c = float(torch.linalg.norm(x_c - x_ref)) / nref
     t_fp64, (x_d, info_d, _, _) = timed(lambda: step(space, k, u, f, mg64, args.rtol), args.steps, args.warmup)
     err_d = float(torch.linalg.norm(x_d - x_ref)) / nref
-    if max(err_m, err_d) > 1e-8:
-        print(f"bench: solution error {max(err_m, err_d):.3e} exceeds 1e-8; the DOF/s figure is not valid",
+    if max(err_m, err_d, err_c) > 1e-8:
+        print(f"bench: solution error {max(err_m, err_d, err_c):.3e} exceeds 1e-8; the DOF/s figure is not valid",
               file=sys.stderr)
-    phases_m = phase_breakdown(space, k, u, f, mg32, args.rtol)
+    phases_m = phase_breakdown_refine(space, k, u, f, mgr, args.rtol)
+    phases_c = phase_breakdown(space, k, u, f, mg32, args.rtol)
     phases_d = phase_breakdown(space, k, u, f, mg64, args.rtol)
-    kt = in_step_kernels(space, k, u, f, mg32, args.rtol)
+    kt = in_step_kernels(step_refine, space, k, u, f, mgr, args.rtol)
+    kt64 = in_step_kernels(step, space, k, u, f, mg64, args.rtol)
     err_l2 = float(mesh.error(u, space.function(x_m)))
     nnz_a = pat.nnz
-    roof, roof_asm = rooflines(kt, mesh, gdof, nnz_a, int(A2.nnz), gdof)
+    roof, roof_asm = rooflines(kt, mesh, gdof, nnz_a, int(A2.nnz), gdof, 4)
+    roof64, roof_asm64 = rooflines(kt64, mesh, gdof, nnz_a, int(A2.nnz), gdof, 8)
     roof32 = spmv_roofline(A2, torch.float32)
     if args.n == 128:
         roof32["traffic"] = SPMV_F32_TRAFFIC_128
@@ -733,10 +816,18 @@ def main():
                    "ncell": mesh.number_of_cells(), "nnz": int(A2.nnz), "rtol": args.rtol,
                    "source": "torch callable" if args.torch_source else "separable expression, in-kernel"},
         "fp64": {"value": round(gdof / t_fp64, 1), "ms_per_step": round(t_fp64 * 1e3, 3), "rel_err": err_d,
-                 "pcg_iterations": info_d["niter"], "phases": phases_d},
-        "rel_err_ok": bool(max(err_m, err_d) <= 1e-8),
-        "mixed": {"rel_err": err_m, "pcg_iterations": info_m["niter"], "phases": phases_m,
-                  "speedup_vs_fp64": round(t_fp64 / t_mixed, 3)},
+                 "pcg_iterations": info_d["niter"], "phases": phases_d, "roofline": roof64,
+                 "roofline_assembly": roof_asm64},
+        "rel_err_ok": bool(max(err_m, err_d, err_c) <= 1e-8),
+        "mixed": {"form": "A assembled in FP64; FP32 Dirichlet-filtered copy (rounded in the filter pass), FP32 "
+                          f"MG setup and MG-PCG corrections to rtol {INNER_RTOL:g}; FP64 residuals b - A x by one "
+                          "SpMV pass (iterative refinement)",
+                  "rel_err": err_m, "pcg_iterations": info_m["niter"], "refinements": info_m["refinements"],
+                  "phases": phases_m, "speedup_vs_fp64": round(t_fp64 / t_mixed, 3)},
+        "fp64_cg_fp32_cycle": {"form": "FP64 assembly and CG, FP32/FP16 V-cycle", "value": round(gdof / t_cyc, 1),
+                               "ms_per_step": round(t_cyc * 1e3, 3), "rel_err": err_c,
+                               "pcg_iterations": info_c["niter"], "phases": phases_c,
+                               "speedup_vs_fp64": round(t_fp64 / t_cyc, 3)},
         "l2_error_vs_manufactured": err_l2,
         "cold": {"pattern_ms": round(t_pattern * 1e3, 3), "mesh_s": round(t_mesh, 3),
                  "dof_per_s_with_pattern": round(gdof / (t_mixed + t_pattern), 1)},
@@ -745,7 +836,7 @@ def main():
         "roofline_assembly": roof_asm,
         "roofline_sell_spmv_f32": roof32,
     }
-    del x_ref, x_m, x_d, A2, F2, mg32, mg64, pat, bc, space, mesh, k
+    del x_ref, x_m, x_d, x_c, A2, F2, mg32, mg64, mgr, pat, bc, space, mesh, k
     torch.cuda.empty_cache()
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.cpu_n, args.cpu_runs)
     if not args.no_serving:

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libfealpy_amd.so")
 FA_OK, FA_EINVAL, FA_ENOMEM, FA_EHIP, FA_ENOCONV, FA_ECAPACITY = range(6)
 FA_F64, FA_F32 = 0, 1
 (TIMER_PCG_SPMV, TIMER_P1_ROWS, TIMER_P1_SUM_ROWS, TIMER_P1_VECTORS, TIMER_P1_SUM_VALUES, TIMER_BC_APPLY,
- TIMER_SELL_SPMV, TIMER_VCYCLE) = range(1, 9)
+ TIMER_SELL_SPMV, TIMER_VCYCLE, TIMER_P1_APPLY, TIMER_SMOOTH0) = range(1, 11)
 COEF_NONE, COEF_CONST, COEF_CELL, COEF_CELLQ, COEF_SEP = 0, 1, 2, 3, 4
 
 _p = ctypes.c_void_p
@@ -48,7 +48,7 @@ _SIGNATURES = {
                                              ctypes.c_int, _p, _p]),
     "fa_element_vectors_tensor": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                  ctypes.c_int, ctypes.c_int, _p, _p, _i64, _p, _p, _p, _p, _p]),
-    "fa_sell16_fill": (ctypes.c_int, [_i64, _p, _p, _p, _p, _p, _p, _p, ctypes.c_size_t, _p]),
+    "fa_sell16_fill": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p, _p, _p, _p, ctypes.c_size_t, _p]),
     "fa_mfma_tiles_per_group": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "fa_mfma_work_bytes": (ctypes.c_size_t, [ctypes.c_int, _i64, ctypes.c_int]),
     "fa_element_matrices_mfma": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -63,7 +63,9 @@ _SIGNATURES = {
     "fa_bc_to_point": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64, ctypes.c_int, _p, _p, _p, _p, _p]),
     "fa_dirichlet_work_bytes": (_sz, [_i64]),
     "fa_dirichlet_count": (ctypes.c_int, [_i64, _p, _p, _p, _p, _pi64, _p, _sz, _p]),
-    "fa_dirichlet_fill": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p, _p, _p, _p, _p]),
+    "fa_dirichlet_fill": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64, _p, _p, _p, _p, _p, _p, _p, _p]),
+    "fa_csr_residual_work_bytes": (_sz, []),
+    "fa_csr_residual": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _sz, _p]),
     "fa_dirichlet_rhs": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p, _p, _p, _p, _p]),
     "fa_dirichlet_apply": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
     "fa_spmv": (ctypes.c_int, [ctypes.c_int, _i64, _i64, _p, _p, _p, _p, _p, _p]),
@@ -79,25 +81,37 @@ _SIGNATURES = {
     "fa_rap_work_bytes": (_sz, [_i64]),
     "fa_rap_count": (ctypes.c_int, [_i64, _i64, _p, _p, _p, _p, _p, _p, _i32, _p, _pi64, _pi32, _p, _sz, _p]),
     "fa_rap_fill": (ctypes.c_int, [_i64, _p, _p, _p, _p, _p, _p, _i32, _p, _p, _p]),
-    "fa_rap_values": (ctypes.c_int, [_i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i32, _p, _p]),
+    "fa_rap_values": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i32, _p,
+                                     _p]),
     "fa_rap_terms_work_bytes": (_sz, [_i64]),
     "fa_rap_terms_count": (ctypes.c_int, [_i64, _p, _p, _p, _p, _p, _p, _p, _p, _i32, _p, _i64, _p, ctypes.c_int, _p,
                                           _pi64, _p, _sz, _p]),
     "fa_rap_terms_fill": (ctypes.c_int, [_i64, _p, _p, _p, _p, _p, _p, _p, _p, _i32, _p, _i64, _p, ctypes.c_int, _p,
                                          _i64, _p, _p, _sz, _p]),
-    "fa_rap_values_terms": (ctypes.c_int, [_i64, _p, _p, _p, ctypes.c_int, _p, _p, _p]),
+    "fa_rap_values_terms": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, ctypes.c_int, _p, _p, _p]),
     "fa_csr_to_dense": (ctypes.c_int, [_i64, _p, _p, _p, _p, _p]),
     "fa_dense_spd_inverse": (ctypes.c_int, [_i64, _p, _p]),
     "fa_mg_lmax": (ctypes.c_int, [_p, ctypes.c_int, ctypes.c_int, _pf64, _p, _sz, _p]),
     "fa_mg_vcycle": (ctypes.c_int, [_p, _p, _p, _p]),
     "fa_incidence_maps": (ctypes.c_int, [_i64, _i32, _p, _p, _p, _p, _p, _p, _p, _p, _sz, _p]),
-    "fa_p1_incidence_rows": (ctypes.c_int, [ctypes.c_int, _i64, ctypes.c_int, _p, _p, _p, _p, ctypes.c_int, _f64, _p,
-                                            ctypes.c_int, _f64, _p, _p, _p, _p]),
-    "fa_sum_incidence_rows": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64, _p, _p, _p, _p, _i32, _p, _p]),
+    "fa_p1_incidence_rows": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64, ctypes.c_int, _p, _p, _p, _p,
+                                            ctypes.c_int, _f64, _p, ctypes.c_int, _f64, _p, _p, _p, _p]),
+    "fa_sum_incidence_rows": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i64, _p, _p, _p, _p, _i32,
+                                             _p, _p]),
     "fa_p1_incidence_vectors": (ctypes.c_int, [ctypes.c_int, _i64, ctypes.c_int, _p, _p, _p, _p, ctypes.c_int, _f64,
                                                _p, _p, _p, _p]),
     "fa_sum_incidence_values": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p]),
-    "fa_smoother_setup": (ctypes.c_int, [ctypes.c_int, _p, _p, _p, _p, _p, _p, _p, _sz, _p]),
+    "fa_refine_update": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p]),
+    "fa_topology_work_bytes": (_sz, [_i64, _i64]),
+    "fa_topology_sort": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _p, ctypes.c_int, _p, _i64, _i64, _pi64, _p, _sz,
+                                        _p]),
+    "fa_topology_fill": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _p, ctypes.c_int, _p, _i64, _i64, _p, _p, _p, _p,
+                                        _sz, _p]),
+    "fa_p1_apply_work_bytes": (_sz, []),
+    "fa_p1_apply": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _i64, ctypes.c_int, _p, _p, _p, _p, ctypes.c_int,
+                                   _f64, _p,
+                                   ctypes.c_int, _f64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _sz, _p]),
+    "fa_smoother_setup": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _p, _p, _p, _p, _p, _p, _p, _sz, _p]),
     "fa_jacobi_setup": (ctypes.c_int, [ctypes.c_int, _i64, _i64, _p, _p, _p, _p, _pf64, _p, _sz, _p]),
     "fa_pcg_work_bytes": (_sz, [ctypes.c_int, _i64]),
     "fa_pcg": (ctypes.c_int, [ctypes.c_int, _p, _p, _p, _p, _f64, _f64, _i64, _i32, _pi64, _pf64, _p, _sz, _p]),

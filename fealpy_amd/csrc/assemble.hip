@@ -1,6 +1,9 @@
 // Kernels "a" and "c": element matrices / vectors on simplices and the
 // fixed-order (ascending cell) gather of element contributions into CSR rows.
 #include "common.hpp"
+#include "reduce.hpp"
+
+#include <algorithm>
 
 namespace fa {
 
@@ -1227,8 +1230,9 @@ __device__ inline void load_vertices_wide(const double* __restrict__ node, const
     }
 }
 
-// pass 1, matrix: rows[inc[c*NV + i]][j] = element entry (i, j) of cell c
-template <int TD, bool SEP>
+// pass 1, matrix: rows[inc[c*NV + i]][j] = element entry (i, j) of cell c,
+// computed in FP64 and stored as R (FP32 rows for an FP32 matrix)
+template <int TD, bool SEP, class R>
 __global__ void __launch_bounds__(P1_BLOCK) k_p1_incidence_rows(int64_t nc, const double* __restrict__ nodew,
                                                                const int32_t* __restrict__ cell, int nq,
                                                                const double* __restrict__ qw,
@@ -1236,7 +1240,7 @@ __global__ void __launch_bounds__(P1_BLOCK) k_p1_incidence_rows(int64_t nc, cons
                                                                const double* __restrict__ dc, int mk, double ms,
                                                                const double* __restrict__ mc,
                                                                const int32_t* __restrict__ inc,
-                                                               double* __restrict__ rows) {
+                                                               R* __restrict__ rows) {
     constexpr int NV = TD + 1;
     extern __shared__ double stage[];
     const size_t ld = coef_stage_len(dk, nq), lm = coef_stage_len(mk, nq);
@@ -1270,9 +1274,9 @@ __global__ void __launch_bounds__(P1_BLOCK) k_p1_incidence_rows(int64_t nc, cons
             for (int i = 0; i < NV; ++i) {
                 double v[NV];
                 p1_row<TD, SEP>(g, p, c0 + t, kd, i, t, nq, qw, phi, dk, ds, dcl, mk, ms, mcl, v);
-                double* out = rows + static_cast<int64_t>(iS[t * NV + i]) * NV;
+                R* out = rows + static_cast<int64_t>(iS[t * NV + i]) * NV;
 #pragma unroll
-                for (int j = 0; j < NV; ++j) out[j] = v[j];
+                for (int j = 0; j < NV; ++j) out[j] = static_cast<R>(v[j]);
             }
         }
         __syncthreads();
@@ -1327,34 +1331,37 @@ __global__ void __launch_bounds__(P1_BLOCK) k_p1_incidence_vectors(int64_t nc, c
 }
 
 // pass 2, matrix: the fused kernel's LDS-table sum with the element rows and
-// positions read from the incidence slots instead of computed
-template <int NV, int G, int K, class T>
+// positions read from the incidence slots instead of computed.  R: the type
+// of the stored rows, of the LDS table and of the sums (FP64 rows give the
+// fused kernel's values bit for bit; FP32 rows an FP32 matrix)
+template <int NV, int G, int K, class T, class R>
 __global__ void __launch_bounds__(256) k_sum_incidence_rows(int64_t gdof, const int32_t* __restrict__ adj_ptr,
                                                            const int32_t* __restrict__ indptr,
                                                            const uint32_t* __restrict__ pos,
-                                                           const double* __restrict__ rows, int lmax,
+                                                           const R* __restrict__ rows, int lmax,
                                                            T* __restrict__ values) {
     constexpr int GS = G + 1;
-    extern __shared__ double tabs[];
+    extern __shared__ double tabs_[];
+    R* tabs = reinterpret_cast<R*>(tabs_);
     const int rpb = blockDim.x / G;
     const int sub = threadIdx.x / G;
     const int lane = threadIdx.x % G;
-    double* tab = tabs + (int64_t)sub * lmax * GS;
+    R* tab = tabs + (int64_t)sub * lmax * GS;
     (void)rpb;
     for (int64_t r = (int64_t)blockIdx.x * (blockDim.x / G) + sub; r < gdof; r += (int64_t)gridDim.x * (blockDim.x / G)) {
         const int32_t r0 = indptr[r];
         const int L = indptr[r + 1] - r0;
-        for (int t = 0; t < L; ++t) tab[t * GS + lane] = 0.0;
-        double acc[K];
+        for (int t = 0; t < L; ++t) tab[t * GS + lane] = R(0);
+        R acc[K];
 #pragma unroll
-        for (int m = 0; m < K; ++m) acc[m] = 0.0;
+        for (int m = 0; m < K; ++m) acc[m] = R(0);
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         // the next chunk's position word and element row are loaded before
         // the current chunk is summed (two chunks of loads in flight per wave)
         const int32_t kbeg = adj_ptr[r], k1 = adj_ptr[r + 1];
         uint32_t pk_next = 0;
-        double v_next[NV];
+        R v_next[NV];
         if (kbeg + lane < k1) {
             pk_next = pos[kbeg + lane];
 #pragma unroll
@@ -1363,7 +1370,7 @@ __global__ void __launch_bounds__(256) k_sum_incidence_rows(int64_t gdof, const 
         for (int32_t kb = kbeg; kb < k1; kb += G) {
             const int32_t k = kb + lane;
             const uint32_t pk = pk_next;
-            double v[NV];
+            R v[NV];
 #pragma unroll
             for (int j = 0; j < NV; ++j) v[j] = v_next[j];
             if (k + G < k1) {
@@ -1393,7 +1400,7 @@ __global__ void __launch_bounds__(256) k_sum_incidence_rows(int64_t gdof, const 
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             if (k < k1) {
 #pragma unroll
-                for (int j = 0; j < NV; ++j) tab[p[j] * GS + lane] = 0.0;
+                for (int j = 0; j < NV; ++j) tab[p[j] * GS + lane] = R(0);
             }
         }
 #pragma unroll
@@ -1426,10 +1433,10 @@ __global__ void __launch_bounds__(256) k_sum_incidence_values(int64_t gdof, cons
     }
 }
 
-template <int NV, int G, int K, class T>
+template <int NV, int G, int K, class T, class R>
 static int launch_sum_rows(int64_t gdof, const int32_t* adj_ptr, const int32_t* indptr, const uint32_t* pos,
-                           const double* rows, int lmax, void* values, hipStream_t s) {
-    const size_t per_row = sizeof(double) * lmax * (G + 1);
+                           const R* rows, int lmax, void* values, hipStream_t s) {
+    const size_t per_row = sizeof(R) * lmax * (G + 1);
     int rpb = 256 / G;
     while (rpb > 1 && rpb * per_row > 48 * 1024) rpb >>= 1;
     const size_t lds = per_row * rpb;
@@ -1439,8 +1446,8 @@ static int launch_sum_rows(int64_t gdof, const int32_t* adj_ptr, const int32_t* 
     }
     const unsigned grid = grid_for(gdof, rpb, 1 << 20);
     timer_begin(FA_TIMER_P1_SUM_ROWS, s);
-    hipLaunchKernelGGL((k_sum_incidence_rows<NV, G, K, T>), dim3(grid), dim3(rpb * G), lds, s, gdof, adj_ptr, indptr,
-                       pos, rows, lmax, static_cast<T*>(values));
+    hipLaunchKernelGGL((k_sum_incidence_rows<NV, G, K, T, R>), dim3(grid), dim3(rpb * G), lds, s, gdof, adj_ptr,
+                       indptr, pos, rows, lmax, static_cast<T*>(values));
     timer_end(FA_TIMER_P1_SUM_ROWS, s);
     FA_LAUNCH("k_sum_incidence_rows");
     return FA_OK;
@@ -1476,10 +1483,11 @@ static size_t p1_stage_bytes(int td, int nq, int k1, int k2) {
     return sizeof(double) * P1_BLOCK * (len(k1) + len(k2)) + 2 * sizeof(int32_t) * P1_BLOCK * (td + 1);
 }
 
-extern "C" int fa_p1_incidence_rows(int td, int64_t nc, int nq, const double* node_wide, const int32_t* cell,
-                                    const double* qw, const double* phi, int diff_kind, double diff_scale,
-                                    const double* diff_coef, int mass_kind, double mass_scale,
-                                    const double* mass_coef, const int32_t* inc, double* rows, fa_stream_t stream) {
+extern "C" int fa_p1_incidence_rows(int dtype_rows, int td, int64_t nc, int nq, const double* node_wide,
+                                    const int32_t* cell, const double* qw, const double* phi, int diff_kind,
+                                    double diff_scale, const double* diff_coef, int mass_kind, double mass_scale,
+                                    const double* mass_coef, const int32_t* inc, void* rows, fa_stream_t stream) {
+    FA_REQUIRE(dtype_rows == FA_F64 || dtype_rows == FA_F32, "fa_p1_incidence_rows: bad dtype %d", dtype_rows);
     FA_REQUIRE(td == 2 || td == 3, "fa_p1_incidence_rows: td must be 2 or 3");
     FA_REQUIRE(nq > 0 && inc && rows && node_wide, "fa_p1_incidence_rows: bad arguments");
     FA_REQUIRE(!mass_kind || phi, "fa_p1_incidence_rows: mass needs phi");
@@ -1493,9 +1501,12 @@ extern "C" int fa_p1_incidence_rows(int td, int64_t nc, int nq, const double* no
     }
     const unsigned grid = grid_for(nc, P1_BLOCK, 1 << 20);
     timer_begin(FA_TIMER_P1_ROWS, S(stream));
-#define FA_R(TD_, SEP_)                                                                                            \
-    hipLaunchKernelGGL((k_p1_incidence_rows<TD_, SEP_>), dim3(grid), dim3(P1_BLOCK), lds, S(stream), nc, node_wide, \
-                       cell, nq, qw, phi, diff_kind, diff_scale, diff_coef, mass_kind, mass_scale, mass_coef, inc, rows)
+#define FA_R1(TD_, SEP_, R_)                                                                                        \
+    hipLaunchKernelGGL((k_p1_incidence_rows<TD_, SEP_, R_>), dim3(grid), dim3(P1_BLOCK), lds, S(stream), nc,         \
+                       node_wide, cell, nq, qw, phi, diff_kind, diff_scale, diff_coef, mass_kind, mass_scale, mass_coef, \
+                       inc, static_cast<R_*>(rows))
+#define FA_R(TD_, SEP_) \
+    if (dtype_rows == FA_F64) FA_R1(TD_, SEP_, double); else FA_R1(TD_, SEP_, float)
     const bool sep = diff_kind == FA_COEF_SEP || mass_kind == FA_COEF_SEP;
     if (td == 3) {
         if (sep) FA_R(3, true); else FA_R(3, false);
@@ -1503,21 +1514,31 @@ extern "C" int fa_p1_incidence_rows(int td, int64_t nc, int nq, const double* no
         if (sep) FA_R(2, true); else FA_R(2, false);
     }
 #undef FA_R
+#undef FA_R1
     timer_end(FA_TIMER_P1_ROWS, S(stream));
     FA_LAUNCH("k_p1_incidence_rows");
     return FA_OK;
 }
 
-extern "C" int fa_sum_incidence_rows(int dtype, int td, int64_t gdof, const int32_t* adj_ptr, const int32_t* indptr,
-                                     const uint32_t* pos, const double* rows, int32_t max_row, void* values,
-                                     fa_stream_t stream) {
+extern "C" int fa_sum_incidence_rows(int dtype, int dtype_rows, int td, int64_t gdof, const int32_t* adj_ptr,
+                                     const int32_t* indptr, const uint32_t* pos, const void* rows, int32_t max_row,
+                                     void* values, fa_stream_t stream) {
+    FA_REQUIRE(dtype_rows == FA_F64 || (dtype_rows == FA_F32 && dtype == FA_F32),
+               "fa_sum_incidence_rows: rows of dtype %d for values of dtype %d", dtype_rows, dtype);
     FA_REQUIRE(dtype == FA_F64 || dtype == FA_F32, "fa_sum_incidence_rows: bad dtype %d", dtype);
     FA_REQUIRE(td == 2 || td == 3, "fa_sum_incidence_rows: td must be 2 or 3");
     FA_REQUIRE(gdof > 0 && max_row > 0 && max_row <= 256, "fa_sum_incidence_rows: bad sizes");
     hipStream_t s = S(stream);
-#define FA_SR(NV_, G_, K_)                                                                                         \
-    return dtype == FA_F64 ? launch_sum_rows<NV_, G_, K_, double>(gdof, adj_ptr, indptr, pos, rows, max_row, values, s) \
-                           : launch_sum_rows<NV_, G_, K_, float>(gdof, adj_ptr, indptr, pos, rows, max_row, values, s)
+#define FA_SR(NV_, G_, K_)                                                                                       \
+    do {                                                                                                         \
+        const double* r64 = static_cast<const double*>(rows);                                                   \
+        const float* r32 = static_cast<const float*>(rows);                                                     \
+        if (dtype_rows == FA_F32)                                                                                \
+            return launch_sum_rows<NV_, G_, K_, float, float>(gdof, adj_ptr, indptr, pos, r32, max_row, values, s); \
+        return dtype == FA_F64                                                                                   \
+                   ? launch_sum_rows<NV_, G_, K_, double, double>(gdof, adj_ptr, indptr, pos, r64, max_row, values, s) \
+                   : launch_sum_rows<NV_, G_, K_, float, double>(gdof, adj_ptr, indptr, pos, r64, max_row, values, s); \
+    } while (0)
     // lanes per row G: the sum runs over the incidences in order for any G, so
     // G only sets the chunking (8 measured fastest on 3D P1 rows of 15 entries
     // and 24 incidences: 32 rows a block, three pipelined chunks a row)
@@ -1577,5 +1598,136 @@ extern "C" int fa_sum_incidence_values(int dtype, int64_t gdof, const int32_t* a
                            static_cast<float*>(out));
     timer_end(FA_TIMER_P1_SUM_VALUES, S(stream));
     FA_LAUNCH("k_sum_incidence_values");
+    return FA_OK;
+}
+
+// ---- matrix-free P1 operator -------------------------------------------------
+// y_r = sum over the incidences (c, i) of DOF r of sum_j A^c_ij x_{cell[c][j]},
+// where A^c is the element matrix whose rows k_assemble_p1 sums into CSR
+// entries (same row formula, computed where it is used and never stored).
+// G lanes per DOF: lane l takes the incidences l, l + G, ... of the row in
+// order and the lane partials are combined by a fixed butterfly, so y is the
+// same run to run.  RES: y_r = b_r - (A x)_r, set to 0 on rows with bd_r != 0,
+// and every workgroup stores the sum of its y_r^2 at part[blockIdx.x] (the
+// grid depends on gdof only, so the norm's summation order is fixed).  Y: the
+// type y is stored in (FP32 for the right-hand side of an FP32 inner solve);
+// the sums and the norm are formed in FP64.
+namespace fa {
+
+template <int TD, int G, bool SEP, bool RES, class Y>
+__global__ void __launch_bounds__(256) k_p1_apply(int64_t gdof, const double* __restrict__ nodew,
+                                                 const int32_t* __restrict__ cell, int nq,
+                                                 const double* __restrict__ qw, const double* __restrict__ phi,
+                                                 int dk, double ds, const double* __restrict__ dc, int mk, double ms,
+                                                 const double* __restrict__ mc, const int32_t* __restrict__ ptr,
+                                                 const int32_t* __restrict__ adj, const double* __restrict__ x,
+                                                 const double* __restrict__ b, const uint8_t* __restrict__ bd,
+                                                 Y* __restrict__ y, double* __restrict__ part) {
+    constexpr int NV = TD + 1;
+    const int lane = threadIdx.x % G;
+    const int64_t stride = (int64_t)gridDim.x * (blockDim.x / G);
+    double nrm = 0.0;
+    for (int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; r < gdof; r += stride) {
+        double s = 0.0;
+        const int32_t k1 = ptr[r + 1];
+        for (int32_t k = ptr[r] + lane; k < k1; k += G) {
+            const int32_t e = adj[k];
+            const int64_t c = e / NV;
+            const int i = e - static_cast<int>(c) * NV;
+            int32_t vid[NV];
+#pragma unroll
+            for (int a = 0; a < NV; ++a) vid[a] = cell[c * NV + a];
+            double xv[NV];
+#pragma unroll
+            for (int a = 0; a < NV; ++a) xv[a] = x[vid[a]];
+            double p[NV][TD];
+            load_vertices_wide<TD>(nodew, vid, p);
+            P1Cell<TD> g;
+            g.compute(p);
+            const double kd = p1_kd<TD, SEP>(nq, qw, dk, ds, dc, c, c, p);
+            double v[NV];
+            p1_row<TD, SEP>(g, p, c, kd, i, c, nq, qw, phi, dk, ds, dc, mk, ms, mc, v);
+            double t = 0.0;
+#pragma unroll
+            for (int j = 0; j < NV; ++j) t += v[j] * xv[j];
+            s += t;
+        }
+        s = group_sum<G>(s);
+        if (RES) {
+            s = bd[r] ? 0.0 : b[r] - s;
+            if (lane == 0) nrm += s * s;
+        }
+        if (lane == 0) y[r] = static_cast<Y>(s);
+    }
+    if (RES) store_partial(block_sum(nrm), part);
+}
+
+__global__ void k_p1_apply_finish(const double* __restrict__ part, int n, double* __restrict__ out) {
+    const double s = finish_sum(part, n);
+    if (threadIdx.x == 0) *out = s;
+}
+
+}  // namespace fa
+
+extern "C" size_t fa_p1_apply_work_bytes(void) { return align_up(RED_GRID * sizeof(double)) + 256; }
+
+extern "C" int fa_p1_apply(int dtype_y, int td, int64_t gdof, int nq, const double* node_wide, const int32_t* cell,
+                           const double* qw, const double* phi, int diff_kind, double diff_scale,
+                           const double* diff_coef, int mass_kind, double mass_scale, const double* mass_coef,
+                           const int32_t* adj_ptr, const int32_t* adj, const double* x, const double* b,
+                           const uint8_t* is_bd, void* y, double* norm2, void* work, size_t work_bytes,
+                           fa_stream_t stream) {
+    FA_REQUIRE(dtype_y == FA_F64 || dtype_y == FA_F32, "fa_p1_apply: bad dtype_y %d", dtype_y);
+    FA_REQUIRE(td == 2 || td == 3, "fa_p1_apply: td must be 2 or 3");
+    FA_REQUIRE(gdof > 0 && nq > 0 && node_wide && cell && adj_ptr && adj && x && y, "fa_p1_apply: bad arguments");
+    FA_REQUIRE(!mass_kind || phi, "fa_p1_apply: mass needs phi");
+    FA_REQUIRE(!(diff_kind >= FA_COEF_CELL && !diff_coef) && !(mass_kind >= FA_COEF_CELL && !mass_coef),
+               "fa_p1_apply: coefficient array missing");
+    const bool res = b != nullptr;
+    FA_REQUIRE(!res || (is_bd && norm2), "fa_p1_apply: the residual form needs is_bd and norm2");
+    Arena ar(work, work_bytes);
+    double* part = res ? ar.take<double>(RED_GRID) : nullptr;
+    if (res && !part) {
+        set_error("fa_p1_apply: workspace too small");
+        return FA_ENOMEM;
+    }
+    hipStream_t s = S(stream);
+    constexpr int G = 8;
+    // rows per workgroup 256 / G; the residual form keeps the grid within the partials' capacity
+    const int64_t rows_per_block = 256 / G;
+    const unsigned grid = res ? static_cast<unsigned>(std::min<int64_t>((gdof + rows_per_block - 1) / rows_per_block,
+                                                                        RED_GRID))
+                              : grid_for(gdof, static_cast<int>(rows_per_block), 1 << 20);
+    const bool sep = diff_kind == FA_COEF_SEP || mass_kind == FA_COEF_SEP;
+    timer_begin(FA_TIMER_P1_APPLY, s);
+#define FA_AP1(TD_, SEP_, RES_, Y_)                                                                                 \
+    hipLaunchKernelGGL((k_p1_apply<TD_, G, SEP_, RES_, Y_>), dim3(grid), dim3(256), 0, s, gdof, node_wide, cell, nq,   \
+                       qw, phi, diff_kind, diff_scale, diff_coef, mass_kind, mass_scale, mass_coef, adj_ptr, adj, x, b, \
+                       is_bd, static_cast<Y_*>(y), part)
+#define FA_AP(TD_, SEP_, RES_)                          \
+    if (dtype_y == FA_F64) FA_AP1(TD_, SEP_, RES_, double); \
+    else FA_AP1(TD_, SEP_, RES_, float)
+#define FA_AP2(TD_)                                   \
+    if (sep) {                                        \
+        if (res) FA_AP(TD_, true, true);              \
+        else FA_AP(TD_, true, false);                 \
+    } else {                                          \
+        if (res) FA_AP(TD_, false, true);             \
+        else FA_AP(TD_, false, false);                \
+    }
+    if (td == 3) {
+        FA_AP2(3)
+    } else {
+        FA_AP2(2)
+    }
+#undef FA_AP2
+#undef FA_AP
+#undef FA_AP1
+    timer_end(FA_TIMER_P1_APPLY, s);
+    FA_LAUNCH("k_p1_apply");
+    if (res) {
+        hipLaunchKernelGGL(k_p1_apply_finish, dim3(1), dim3(RED_BLOCK), 0, s, part, static_cast<int>(grid), norm2);
+        FA_LAUNCH("k_p1_apply_finish");
+    }
     return FA_OK;
 }

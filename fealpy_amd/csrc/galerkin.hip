@@ -181,9 +181,11 @@ __global__ void __launch_bounds__(256) k_term_pad(int64_t n, uint32_t* __restric
 }
 
 // c[e] = sum_k w(code_k) a[t_k] in term order (code TERM_UNIT: the constant 1)
+// (T: the value type of a and c; the sum is formed in FP64 either way)
+template <class T>
 __global__ void __launch_bounds__(256) k_rap_apply_terms(int64_t ne, const int32_t* __restrict__ tslice,
                                                         const uint32_t* __restrict__ terms, WeightTable wt,
-                                                        const double* __restrict__ a, double* __restrict__ c) {
+                                                        const T* __restrict__ a, T* __restrict__ c) {
     __shared__ double tab[16];
     if (threadIdx.x < 16) tab[threadIdx.x] = wt.w[threadIdx.x];
     __syncthreads();
@@ -201,7 +203,7 @@ __global__ void __launch_bounds__(256) k_rap_apply_terms(int64_t ne, const int32
             for (int u = 0; u < B; ++u) ww[u] = tp[64 * min(k0 + u, w - 1)];
             double av[B];
 #pragma unroll
-            for (int u = 0; u < B; ++u) av[u] = a[ww[u] & TERM_IDX_MASK];
+            for (int u = 0; u < B; ++u) av[u] = static_cast<double>(a[ww[u] & TERM_IDX_MASK]);
 #pragma unroll
             for (int u = 0; u < B; ++u) {
                 const uint32_t code = ww[u] >> TERM_IDX_BITS;
@@ -209,7 +211,7 @@ __global__ void __launch_bounds__(256) k_rap_apply_terms(int64_t ne, const int32
                 if (k0 + u < w) acc += v;
             }
         }
-        if (e < ne) c[e] = acc;
+        if (e < ne) c[e] = static_cast<T>(acc);
     }
 }
 
@@ -354,8 +356,10 @@ extern "C" int fa_rap_terms_fill(int64_t nrows, const int32_t* r_ptr, const int3
     return check_flags(flags, s, "fa_rap_terms_fill");
 }
 
-extern "C" int fa_rap_values_terms(int64_t nnz_c, const int32_t* tslice, const uint32_t* terms, const double* values,
-                                   int nvalues, const double* a_val, double* c_val, fa_stream_t stream) {
+extern "C" int fa_rap_values_terms(int dtype, int64_t nnz_c, const int32_t* tslice, const uint32_t* terms,
+                                   const double* values, int nvalues, const void* a_val, void* c_val,
+                                   fa_stream_t stream) {
+    FA_REQUIRE(dtype == FA_F64 || dtype == FA_F32, "fa_rap_values_terms: bad dtype %d", dtype);
     FA_REQUIRE(nnz_c > 0 && tslice && terms && a_val && c_val, "fa_rap_values_terms: bad arguments");
     WeightSet ws;
     int st = weight_set(values, nvalues, &ws);
@@ -365,8 +369,13 @@ extern "C" int fa_rap_values_terms(int64_t nnz_c, const int32_t* tslice, const u
         for (int ip = 0; ip < ws.n; ++ip) wt.w[ir * ws.n + ip] = ws.v[ir] * ws.v[ip];
     wt.w[TERM_UNIT] = 1.0;
     wt.w[TERM_PAD] = 0.0;
-    hipLaunchKernelGGL(k_rap_apply_terms, dim3(grid_for(((nnz_c + 63) / 64) * 64, 256, 1 << 16)), dim3(256), 0,
-                       S(stream), nnz_c, tslice, terms, wt, a_val, c_val);
+    const dim3 grid(grid_for(((nnz_c + 63) / 64) * 64, 256, 1 << 16));
+    if (dtype == FA_F64)
+        hipLaunchKernelGGL(k_rap_apply_terms<double>, grid, dim3(256), 0, S(stream), nnz_c, tslice, terms, wt,
+                           static_cast<const double*>(a_val), static_cast<double*>(c_val));
+    else
+        hipLaunchKernelGGL(k_rap_apply_terms<float>, grid, dim3(256), 0, S(stream), nnz_c, tslice, terms, wt,
+                           static_cast<const float*>(a_val), static_cast<float*>(c_val));
     FA_LAUNCH("k_rap_apply_terms");
     return FA_OK;
 }

@@ -102,13 +102,15 @@ __global__ void __launch_bounds__(WAVE) k_rap_rows(int64_t nrows, const int32_t*
 // lanes round-robin; lane l accumulates r_u a_t p_q into its private LDS row at
 // the position of column J = col(p_q).  The row value is the sum over lanes by
 // a fixed binary tree, so the result does not depend on timing.
+// (T: the value type of A and C; products and sums are formed in FP64)
+template <class T>
 __global__ void __launch_bounds__(WAVE) k_rap_values(int64_t nrows, const int32_t* __restrict__ rp,
                                                      const int32_t* __restrict__ ri, const double* __restrict__ rv,
                                                      const int32_t* __restrict__ ap, const int32_t* __restrict__ ai,
-                                                     const double* __restrict__ av, const int32_t* __restrict__ pp,
+                                                     const T* __restrict__ av, const int32_t* __restrict__ pp,
                                                      const int32_t* __restrict__ pi, const double* __restrict__ pv,
                                                      const int32_t* __restrict__ cptr, const int32_t* __restrict__ cidx,
-                                                     int lmax, double* __restrict__ cval) {
+                                                     int lmax, T* __restrict__ cval) {
     extern __shared__ char smem[];
     int32_t* cols = reinterpret_cast<int32_t*>(smem);
     double* acc = reinterpret_cast<double*>(smem + align_up(sizeof(int32_t) * lmax, 16));  // [WAVE][lmax]
@@ -147,7 +149,7 @@ __global__ void __launch_bounds__(WAVE) k_rap_values(int64_t nrows, const int32_
                 }
                 const int32_t t = s_a0[lo] + (f - s_off[lo]);
                 const int32_t j = ai[t];
-                const double ra = s_r[lo] * av[t];
+                const double ra = s_r[lo] * static_cast<double>(av[t]);
                 for (int32_t q = pp[j]; q < pp[j + 1]; ++q) {
                     const int32_t J = pi[q];
                     int a = 0, b = L;
@@ -169,7 +171,7 @@ __global__ void __launch_bounds__(WAVE) k_rap_values(int64_t nrows, const int32_
             __syncthreads();
         }
         const bool empty = (r0 == r1);
-        for (int t = lane; t < L; t += WAVE) cval[c0 + t] = empty ? 1.0 : acc[t];
+        for (int t = lane; t < L; t += WAVE) cval[c0 + t] = static_cast<T>(empty ? 1.0 : acc[t]);
         __syncthreads();
     }
 }
@@ -610,10 +612,10 @@ __global__ void __launch_bounds__(RED_BLOCK) k_max_finish(const double* __restri
 // residual, the Gershgorin bound max_i sum_j |a_ij| / |a_ii| (one partial
 // maximum per workgroup), and the cycle's operator values: FP16 D^-1 A on
 // the compressed slices (h16) and/or the FP32 copy of the SELL values (v32).
-template <class T>
+template <class T, class V>
 __global__ void __launch_bounds__(RED_BLOCK) k_smoother_setup(int64_t n, const int32_t* __restrict__ sp,
                                                              const int32_t* __restrict__ cols,
-                                                             const double* __restrict__ v64, T* __restrict__ dinv,
+                                                             const V* __restrict__ v64, T* __restrict__ dinv,
                                                              T* __restrict__ diag, _Float16* __restrict__ h16,
                                                              float* __restrict__ v32, double* __restrict__ partials) {
     __shared__ double ws[RED_BLOCK / WAVE];
@@ -627,7 +629,7 @@ __global__ void __launch_bounds__(RED_BLOCK) k_smoother_setup(int64_t n, const i
         double d = 0.0, sum = 0.0;
         bool found = false;
         for (int k = 0; k < w; ++k) {
-            const double v = v64[p0 + 64 * k];
+            const double v = static_cast<double>(v64[p0 + 64 * k]);
             sum += fabs(v);
             if (!found && cols[p0 + 64 * k] == row) {
                 d = v;
@@ -643,7 +645,7 @@ __global__ void __launch_bounds__(RED_BLOCK) k_smoother_setup(int64_t n, const i
         }
         if (h16 || v32) {
             for (int k = 0; k < w; ++k) {
-                const double v = v64[p0 + 64 * k];
+                const double v = static_cast<double>(v64[p0 + 64 * k]);
                 if (h16) h16[p0 + 64 * k] = static_cast<_Float16>(row < n ? v * inv : 0.0);
                 if (v32) v32[p0 + 64 * k] = static_cast<float>(v);
             }
@@ -899,6 +901,8 @@ struct Cycle {
         const unsigned g = grid(L.A.n);
         const unsigned gd = red_grid(L.A.n);
         const bool cmp = compressed(L);
+        // the finest level's compressed steps are timed as one kernel family
+        const int tk = (cmp && &L == &mg->levels[0]) ? FA_TIMER_SMOOTH0 : 0;
         for (int k = 0; k < degree; ++k) {
             T c1, c2;
             double rho_new = rho;
@@ -921,11 +925,13 @@ struct Cycle {
                     T* out = (degree == 2 && last_out) ? last_out : nxt;
                     if (cmp)
                         with_cols(L, [&](auto cs) {
+                            timer_begin(tk, s);
                             hipLaunchKernelGGL(k_cheb16_z<decltype(cs)>, dim3(g), dim3(256), 0, s, L.A.n,
                                                L.Ah.slice_ptr, cs, static_cast<const _Float16*>(L.Ah.vals),
                                                reinterpret_cast<const float*>(dinv), reinterpret_cast<const float*>(b),
                                                reinterpret_cast<float*>(out), reinterpret_cast<float*>(d), (float)c0,
                                                (float)d1, (float)d2, done);
+                            timer_end(tk, s);
                         });
                     else
                         hipLaunchKernelGGL(k_cheb_z<T>, dim3(g), dim3(256), 0, s, L.A.n, L.A.slice_ptr, L.A.cols, sv,
@@ -952,6 +958,7 @@ struct Cycle {
                 float* fd = reinterpret_cast<float*>(d);
                 with_cols(L, [&](auto cs) {
                     using CS = decltype(cs);
+                    timer_begin(tk, s);
                     if (with_dot) {
                         if (k == 0)
                             hipLaunchKernelGGL((k_cheb16<true, true, CS>), dim3(gd), dim3(RED_BLOCK), 0, s, L.A.n,
@@ -970,6 +977,7 @@ struct Cycle {
                                            L.Ah.slice_ptr, cs, hv, fdinv, fb, fcur, fout, fd, (float)c1, (float)c2,
                                            done, DotOut{});
                     }
+                    timer_end(tk, s);
                 });
             } else if (with_dot) {
                 if (k == 0)
@@ -1049,10 +1057,13 @@ struct Cycle {
         const unsigned g = grid(L.A.n);
         if (compressed(L))
             with_cols(L, [&](auto cs) {
+                const int tk = l == 0 ? FA_TIMER_SMOOTH0 : 0;
+                timer_begin(tk, s);
                 hipLaunchKernelGGL(k_residual16<decltype(cs)>, dim3(g), dim3(256), 0, s, L.A.n, L.Ah.slice_ptr, cs,
                                    static_cast<const _Float16*>(L.Ah.vals), static_cast<const float*>(L.diag),
                                    reinterpret_cast<const float*>(b), reinterpret_cast<const float*>(cur),
                                    reinterpret_cast<float*>(r), done);
+                timer_end(tk, s);
             });
         else
             hipLaunchKernelGGL(k_residual<T>, dim3(g), dim3(256), 0, s, L.A.n, L.A.slice_ptr, L.A.cols,
@@ -1498,18 +1509,26 @@ extern "C" int fa_rap_fill(int64_t nrows, const int32_t* r_ptr, const int32_t* r
                     nullptr, S(stream));
 }
 
-extern "C" int fa_rap_values(int64_t nrows, const int32_t* r_ptr, const int32_t* r_idx, const double* r_val,
-                             const int32_t* a_ptr, const int32_t* a_idx, const double* a_val, const int32_t* p_ptr,
-                             const int32_t* p_idx, const double* p_val, const int32_t* c_ptr, const int32_t* c_idx,
-                             int32_t max_row, double* c_val, fa_stream_t stream) {
+extern "C" int fa_rap_values(int dtype, int64_t nrows, const int32_t* r_ptr, const int32_t* r_idx,
+                             const double* r_val, const int32_t* a_ptr, const int32_t* a_idx, const void* a_val,
+                             const int32_t* p_ptr, const int32_t* p_idx, const double* p_val, const int32_t* c_ptr,
+                             const int32_t* c_idx, int32_t max_row, void* c_val, fa_stream_t stream) {
+    FA_REQUIRE(dtype == FA_F64 || dtype == FA_F32, "fa_rap_values: bad dtype %d", dtype);
     FA_REQUIRE(max_row > 0, "fa_rap_values: bad max_row");
     const size_t lds = align_up(sizeof(int32_t) * max_row, 16) + sizeof(double) * WAVE * max_row;
     if (lds > 64 * 1024) {
         set_error("fa_rap_values: coarse rows of %d entries exceed the LDS capacity", max_row);
         return FA_ECAPACITY;
     }
-    hipLaunchKernelGGL(k_rap_values, dim3(grid_for(nrows, 1, 1 << 22)), dim3(WAVE), lds, S(stream), nrows, r_ptr,
-                       r_idx, r_val, a_ptr, a_idx, a_val, p_ptr, p_idx, p_val, c_ptr, c_idx, max_row, c_val);
+    const dim3 grid(grid_for(nrows, 1, 1 << 22));
+    if (dtype == FA_F64)
+        hipLaunchKernelGGL(k_rap_values<double>, grid, dim3(WAVE), lds, S(stream), nrows, r_ptr, r_idx, r_val, a_ptr,
+                           a_idx, static_cast<const double*>(a_val), p_ptr, p_idx, p_val, c_ptr, c_idx, max_row,
+                           static_cast<double*>(c_val));
+    else
+        hipLaunchKernelGGL(k_rap_values<float>, grid, dim3(WAVE), lds, S(stream), nrows, r_ptr, r_idx, r_val, a_ptr,
+                           a_idx, static_cast<const float*>(a_val), p_ptr, p_idx, p_val, c_ptr, c_idx, max_row,
+                           static_cast<float*>(c_val));
     FA_LAUNCH("k_rap_values");
     return FA_OK;
 }
@@ -1613,12 +1632,13 @@ extern "C" int fa_jacobi_setup(int dtype_out, int64_t n, int64_t nnz, const int3
     return FA_OK;
 }
 
-extern "C" int fa_smoother_setup(int dtype, const fa_sell_t* A, void* dinv, void* diag, void* vals_f32,
-                                 const fa_sell16_t* Ah, double* bound_dev, void* work, size_t work_bytes,
-                                 fa_stream_t stream) {
+extern "C" int fa_smoother_setup(int dtype, int dtype_in, const fa_sell_t* A, void* dinv, void* diag,
+                                 void* vals_f32, const fa_sell16_t* Ah, double* bound_dev, void* work,
+                                 size_t work_bytes, fa_stream_t stream) {
     FA_REQUIRE(A && A->n > 0 && A->slice_ptr && A->cols && A->vals && dinv && bound_dev,
                "fa_smoother_setup: bad arguments");
     FA_REQUIRE(dtype == FA_F64 || dtype == FA_F32, "fa_smoother_setup: dtype must be FP64 or FP32");
+    FA_REQUIRE(dtype_in == FA_F64 || dtype_in == FA_F32, "fa_smoother_setup: dtype_in must be FP64 or FP32");
     FA_REQUIRE(!Ah || !Ah->vals || (Ah->n == A->n && Ah->slice_ptr == A->slice_ptr),
                "fa_smoother_setup: the compressed operator must share the slices of A");
     hipStream_t s = S(stream);
@@ -1631,15 +1651,16 @@ extern "C" int fa_smoother_setup(int dtype, const fa_sell_t* A, void* dinv, void
     const int64_t npad = (A->n + 63) & ~63LL;
     const unsigned grid = red_grid(npad);
     _Float16* h16 = (Ah && Ah->vals) ? static_cast<_Float16*>(const_cast<void*>(Ah->vals)) : nullptr;
-    const double* v64 = static_cast<const double*>(A->vals);
-    if (dtype == FA_F64)
-        hipLaunchKernelGGL(k_smoother_setup<double>, dim3(grid), dim3(RED_BLOCK), 0, s, A->n, A->slice_ptr, A->cols,
-                           v64, static_cast<double*>(dinv), static_cast<double*>(diag), h16,
-                           static_cast<float*>(vals_f32), partials);
-    else
-        hipLaunchKernelGGL(k_smoother_setup<float>, dim3(grid), dim3(RED_BLOCK), 0, s, A->n, A->slice_ptr, A->cols,
-                           v64, static_cast<float*>(dinv), static_cast<float*>(diag), h16,
-                           static_cast<float*>(vals_f32), partials);
+#define FA_SS(T_, V_)                                                                                         \
+    hipLaunchKernelGGL((k_smoother_setup<T_, V_>), dim3(grid), dim3(RED_BLOCK), 0, s, A->n, A->slice_ptr, A->cols, \
+                       static_cast<const V_*>(A->vals), static_cast<T_*>(dinv), static_cast<T_*>(diag), h16,     \
+                       static_cast<float*>(vals_f32), partials)
+    if (dtype == FA_F64) {
+        if (dtype_in == FA_F64) FA_SS(double, double); else FA_SS(double, float);
+    } else {
+        if (dtype_in == FA_F64) FA_SS(float, double); else FA_SS(float, float);
+    }
+#undef FA_SS
     hipLaunchKernelGGL(k_max_finish, dim3(1), dim3(RED_BLOCK), 0, s, partials, (int)grid, bound_dev);
     FA_LAUNCH("k_smoother_setup");
     return FA_OK;

@@ -113,10 +113,12 @@ __global__ void __launch_bounds__(256) k_sell_spmv(int64_t n, const int32_t* __r
     }
 }
 
-// compressed fill: offs = col - row, vals = a_ij / a_ii in FP16 (one thread per row)
+// compressed fill: offs = col - row, vals = a_ij / a_ii in FP16 (one thread per row;
+// V: the CSR value type)
+template <class V>
 __global__ void __launch_bounds__(256) k_sell16_fill(int64_t n, const int32_t* __restrict__ indptr,
                                                     const int32_t* __restrict__ indices,
-                                                    const double* __restrict__ vin,
+                                                    const V* __restrict__ vin,
                                                     const int32_t* __restrict__ slice_ptr,
                                                     int16_t* __restrict__ offs, _Float16* __restrict__ vals,
                                                     int32_t* __restrict__ overflow) {
@@ -131,7 +133,7 @@ __global__ void __launch_bounds__(256) k_sell16_fill(int64_t n, const int32_t* _
         const int len = row < n ? indptr[row + 1] - r0 : 0;
         double d = 0.0;
         for (int k = 0; k < len; ++k)
-            if (indices[r0 + k] == row) d = vin[r0 + k];
+            if (indices[r0 + k] == row) d = static_cast<double>(vin[r0 + k]);
         const double inv = d != 0.0 ? 1.0 / d : 1.0;
         bool bad = false;
         for (int k = 0; k < w; ++k) {
@@ -140,7 +142,7 @@ __global__ void __launch_bounds__(256) k_sell16_fill(int64_t n, const int32_t* _
             double v = 0.0;
             if (k < len) {
                 off = static_cast<int64_t>(indices[r0 + k]) - row;
-                v = vin[r0 + k] * inv;
+                v = static_cast<double>(vin[r0 + k]) * inv;
             }
             if (off > 32767 || off < -32767) {
                 bad = true;
@@ -155,9 +157,10 @@ __global__ void __launch_bounds__(256) k_sell16_fill(int64_t n, const int32_t* _
 
 // compressed refill of an already checked pattern, slice staged in LDS
 // (values and columns of the slice's contiguous CSR range)
+template <class V>
 __global__ void __launch_bounds__(WAVE) k_sell16_refill(int64_t n, const int32_t* __restrict__ indptr,
                                                        const int32_t* __restrict__ indices,
-                                                       const double* __restrict__ vin,
+                                                       const V* __restrict__ vin,
                                                        const int32_t* __restrict__ slice_ptr,
                                                        _Float16* __restrict__ vals) {
     __shared__ double sv[SF_STAGE];
@@ -176,7 +179,7 @@ __global__ void __launch_bounds__(WAVE) k_sell16_refill(int64_t n, const int32_t
         const bool staged = total <= SF_STAGE;
         if (staged) {
             for (int t = lane; t < total; t += WAVE) {
-                sv[t] = vin[base + t];
+                sv[t] = static_cast<double>(vin[base + t]);
                 si[t] = indices[base + t];
             }
         }
@@ -185,12 +188,12 @@ __global__ void __launch_bounds__(WAVE) k_sell16_refill(int64_t n, const int32_t
         double d = 0.0;
         for (int k = 0; k < len; ++k) {
             const int32_t c = staged ? si[r0 + k] : indices[base + r0 + k];
-            if (c == row) d = staged ? sv[r0 + k] : vin[base + r0 + k];
+            if (c == row) d = staged ? sv[r0 + k] : static_cast<double>(vin[base + r0 + k]);
         }
         const double inv = d != 0.0 ? 1.0 / d : 1.0;
         for (int k = 0; k < w; ++k) {
             double v = 0.0;
-            if (k < len) v = (staged ? sv[r0 + k] : vin[base + r0 + k]) * inv;
+            if (k < len) v = (staged ? sv[r0 + k] : static_cast<double>(vin[base + r0 + k])) * inv;
             vals[p0 + 64 * k + lane] = static_cast<_Float16>(v);
         }
         __builtin_amdgcn_wave_barrier();
@@ -202,22 +205,33 @@ __global__ void __launch_bounds__(WAVE) k_sell16_refill(int64_t n, const int32_t
 
 using namespace fa;
 
-extern "C" int fa_sell16_fill(int64_t n, const int32_t* indptr, const int32_t* indices, const double* values,
-                              const int32_t* slice_ptr, int16_t* offs, void* vals, void* work, size_t work_bytes,
-                              fa_stream_t stream) {
+extern "C" int fa_sell16_fill(int dtype, int64_t n, const int32_t* indptr, const int32_t* indices,
+                              const void* values, const int32_t* slice_ptr, int16_t* offs, void* vals, void* work,
+                              size_t work_bytes, fa_stream_t stream) {
+    FA_REQUIRE(dtype == FA_F64 || dtype == FA_F32, "fa_sell16_fill: bad dtype %d", dtype);
     FA_REQUIRE(n > 0 && indptr && indices && values && slice_ptr && offs && vals, "fa_sell16_fill: bad arguments");
     FA_REQUIRE(!work || work_bytes >= sizeof(int32_t), "fa_sell16_fill: workspace too small");
     hipStream_t s = S(stream);
     int32_t* flag = static_cast<int32_t*>(work);
     if (!flag) {  // pattern already checked: values only, no sync
-        hipLaunchKernelGGL(k_sell16_refill, dim3(grid_for((n + 63) / 64, 1, 1 << 20)), dim3(WAVE), 0, s, n, indptr,
-                           indices, values, slice_ptr, static_cast<_Float16*>(vals));
+        const dim3 g(grid_for((n + 63) / 64, 1, 1 << 20));
+        if (dtype == FA_F64)
+            hipLaunchKernelGGL(k_sell16_refill<double>, g, dim3(WAVE), 0, s, n, indptr, indices,
+                               static_cast<const double*>(values), slice_ptr, static_cast<_Float16*>(vals));
+        else
+            hipLaunchKernelGGL(k_sell16_refill<float>, g, dim3(WAVE), 0, s, n, indptr, indices,
+                               static_cast<const float*>(values), slice_ptr, static_cast<_Float16*>(vals));
         FA_LAUNCH("k_sell16_refill");
         return FA_OK;
     }
     FA_HIP(hipMemsetAsync(flag, 0, sizeof(int32_t), s));
-    hipLaunchKernelGGL(k_sell16_fill, dim3(grid_for(((n + 63) / 64) * 64, 256, 1 << 16)), dim3(256), 0, s, n, indptr,
-                       indices, values, slice_ptr, offs, static_cast<_Float16*>(vals), flag);
+    const dim3 g(grid_for(((n + 63) / 64) * 64, 256, 1 << 16));
+    if (dtype == FA_F64)
+        hipLaunchKernelGGL(k_sell16_fill<double>, g, dim3(256), 0, s, n, indptr, indices,
+                           static_cast<const double*>(values), slice_ptr, offs, static_cast<_Float16*>(vals), flag);
+    else
+        hipLaunchKernelGGL(k_sell16_fill<float>, g, dim3(256), 0, s, n, indptr, indices,
+                           static_cast<const float*>(values), slice_ptr, offs, static_cast<_Float16*>(vals), flag);
     FA_LAUNCH("k_sell16_fill");
     int32_t h = 0;
     FA_HIP(hipMemcpyAsync(&h, flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));

@@ -95,12 +95,13 @@ __global__ void __launch_bounds__(256) k_bc_count(int64_t n, const int32_t* __re
     }
 }
 
-// kept entries keep their order; nidx == NULL refills values only
-template <class T>
+// kept entries keep their order; nidx == NULL refills values only; U: the
+// type of the filtered values (an FP32 copy of an FP64 matrix is rounded once)
+template <class T, class U>
 __global__ void __launch_bounds__(256) k_bc_fill(int64_t n, const int32_t* __restrict__ indptr,
                                                 const int32_t* __restrict__ indices, const T* __restrict__ vals,
                                                 const uint8_t* __restrict__ bd, const int32_t* __restrict__ nptr,
-                                                int32_t* __restrict__ nidx, T* __restrict__ nval) {
+                                                int32_t* __restrict__ nidx, U* __restrict__ nval) {
     const int lane = threadIdx.x % BCG;
     for (int64_t r = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / BCG; r < n;
          r += (int64_t)gridDim.x * blockDim.x / BCG) {
@@ -108,7 +109,7 @@ __global__ void __launch_bounds__(256) k_bc_fill(int64_t n, const int32_t* __res
         if (bd[r]) {
             if (lane == 0) {
                 if (nidx) nidx[o0] = static_cast<int32_t>(r);
-                nval[o0] = T(1);
+                nval[o0] = U(1);
             }
             continue;
         }
@@ -125,7 +126,7 @@ __global__ void __launch_bounds__(256) k_bc_fill(int64_t n, const int32_t* __res
             const int inc = group_inclusive_scan<BCG>(keep ? 1 : 0);
             if (keep) {
                 if (nidx) nidx[base + inc - 1] = c;
-                nval[base + inc - 1] = vals[k];
+                nval[base + inc - 1] = static_cast<U>(vals[k]);
             }
             base += __shfl(inc, BCG - 1, BCG);  // group total
         }
@@ -212,6 +213,15 @@ __global__ void k_axpby(int64_t n, double a, const T* __restrict__ x, double b, 
         y[i] = static_cast<T>(a * double(x[i]) + b * double(y[i]));
 }
 
+// x += d off the Dirichlet rows (the correction step of iterative refinement:
+// boundary values stay exactly those of the boundary condition)
+template <class D>
+__global__ void k_refine_update(int64_t n, double* __restrict__ x, const D* __restrict__ d,
+                                const uint8_t* __restrict__ bd) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        if (!bd[i]) x[i] += static_cast<double>(d[i]);
+}
+
 template <class A, class B>
 __global__ void k_cast(int64_t n, const A* __restrict__ x, B* __restrict__ y) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -279,18 +289,24 @@ extern "C" int fa_dirichlet_count(int64_t n, const int32_t* indptr, const int32_
     return FA_OK;
 }
 
-extern "C" int fa_dirichlet_fill(int dtype, int64_t n, const int32_t* indptr, const int32_t* indices,
+extern "C" int fa_dirichlet_fill(int dtype, int dtype_out, int64_t n, const int32_t* indptr, const int32_t* indices,
                                  const void* values, const uint8_t* is_bd, const int32_t* new_indptr,
                                  int32_t* new_indices, void* new_values, fa_stream_t stream) {
     FA_REQUIRE(dtype == FA_F64 || dtype == FA_F32, "fa_dirichlet_fill: bad dtype %d", dtype);
+    FA_REQUIRE(dtype_out == dtype || (dtype == FA_F64 && dtype_out == FA_F32),
+               "fa_dirichlet_fill: values of dtype %d cannot be stored as dtype %d", dtype, dtype_out);
     hipStream_t s = S(stream);
     const unsigned grid = grid_for(n * BCG, 256, 1 << 16);
-    if (dtype == FA_F64)
-        hipLaunchKernelGGL(k_bc_fill<double>, dim3(grid), dim3(256), 0, s, n, indptr, indices,
+    if (dtype == FA_F64 && dtype_out == FA_F64)
+        hipLaunchKernelGGL((k_bc_fill<double, double>), dim3(grid), dim3(256), 0, s, n, indptr, indices,
                            static_cast<const double*>(values), is_bd, new_indptr, new_indices,
                            static_cast<double*>(new_values));
+    else if (dtype == FA_F64)
+        hipLaunchKernelGGL((k_bc_fill<double, float>), dim3(grid), dim3(256), 0, s, n, indptr, indices,
+                           static_cast<const double*>(values), is_bd, new_indptr, new_indices,
+                           static_cast<float*>(new_values));
     else
-        hipLaunchKernelGGL(k_bc_fill<float>, dim3(grid), dim3(256), 0, s, n, indptr, indices,
+        hipLaunchKernelGGL((k_bc_fill<float, float>), dim3(grid), dim3(256), 0, s, n, indptr, indices,
                            static_cast<const float*>(values), is_bd, new_indptr, new_indices,
                            static_cast<float*>(new_values));
     FA_LAUNCH("k_bc_fill");
@@ -390,3 +406,80 @@ extern "C" int fa_cast(int dtype_in, int dtype_out, int64_t n, const void* x, vo
     return FA_OK;
 }
 
+
+extern "C" int fa_refine_update(int dtype_d, int64_t n, double* x, const void* d, const uint8_t* is_bd,
+                                fa_stream_t stream) {
+    FA_REQUIRE(dtype_d == FA_F64 || dtype_d == FA_F32, "fa_refine_update: bad dtype %d", dtype_d);
+    FA_REQUIRE(n >= 0 && (n == 0 || (x && d && is_bd)), "fa_refine_update: bad arguments");
+    if (n == 0) return FA_OK;
+    hipStream_t s = S(stream);
+    const unsigned grid = grid_for(n, 256, 1 << 16);
+    if (dtype_d == FA_F64)
+        hipLaunchKernelGGL(k_refine_update<double>, dim3(grid), dim3(256), 0, s, n, x, static_cast<const double*>(d),
+                           is_bd);
+    else
+        hipLaunchKernelGGL(k_refine_update<float>, dim3(grid), dim3(256), 0, s, n, x, static_cast<const float*>(d),
+                           is_bd);
+    FA_LAUNCH("k_refine_update");
+    return FA_OK;
+}
+
+// r = b - A x off the Dirichlet rows, 0 on them, stored as Y, and the
+// workgroup partial sums of r^2 (a grid fixed by n): the FP64 residual of
+// iterative refinement on an assembled FP64 matrix (G lanes per row, lane
+// partials by a fixed butterfly)
+template <int G, class Y>
+__global__ void __launch_bounds__(256) k_csr_residual(int64_t n, const int32_t* __restrict__ indptr,
+                                                     const int32_t* __restrict__ indices,
+                                                     const double* __restrict__ vals, const double* __restrict__ x,
+                                                     const double* __restrict__ b, const uint8_t* __restrict__ bd,
+                                                     Y* __restrict__ y, double* __restrict__ part) {
+    const int lane = threadIdx.x % G;
+    double nrm = 0.0;
+    for (int64_t r = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G; r < n;
+         r += (int64_t)gridDim.x * blockDim.x / G) {
+        const int32_t k1 = indptr[r + 1];
+        double s = 0.0;
+        for (int32_t k = indptr[r] + lane; k < k1; k += G) s += vals[k] * x[indices[k]];
+        s = group_sum<G>(s);
+        s = bd[r] ? 0.0 : b[r] - s;
+        if (lane == 0) {
+            y[r] = static_cast<Y>(s);
+            nrm += s * s;
+        }
+    }
+    store_partial(block_sum(nrm), part);
+}
+
+__global__ void k_residual_finish(const double* __restrict__ part, int n, double* __restrict__ out) {
+    const double s = finish_sum(part, n);
+    if (threadIdx.x == 0) *out = s;
+}
+
+extern "C" size_t fa_csr_residual_work_bytes(void) { return align_up(RED_GRID * sizeof(double)) + 256; }
+
+extern "C" int fa_csr_residual(int dtype_y, int64_t n, const int32_t* indptr, const int32_t* indices,
+                               const double* values, const double* x, const double* b, const uint8_t* is_bd, void* y,
+                               double* norm2, void* work, size_t work_bytes, fa_stream_t stream) {
+    FA_REQUIRE(dtype_y == FA_F64 || dtype_y == FA_F32, "fa_csr_residual: bad dtype %d", dtype_y);
+    FA_REQUIRE(n > 0 && indptr && indices && values && x && b && is_bd && y && norm2, "fa_csr_residual: bad arguments");
+    Arena ar(work, work_bytes);
+    double* part = ar.take<double>(RED_GRID);
+    if (!part) {
+        set_error("fa_csr_residual: workspace too small");
+        return FA_ENOMEM;
+    }
+    hipStream_t s = S(stream);
+    constexpr int G = 8;
+    const unsigned grid = static_cast<unsigned>(std::min<int64_t>((n * G + 255) / 256, RED_GRID));
+    if (dtype_y == FA_F64)
+        hipLaunchKernelGGL((k_csr_residual<G, double>), dim3(grid), dim3(256), 0, s, n, indptr, indices, values, x, b,
+                           is_bd, static_cast<double*>(y), part);
+    else
+        hipLaunchKernelGGL((k_csr_residual<G, float>), dim3(grid), dim3(256), 0, s, n, indptr, indices, values, x, b,
+                           is_bd, static_cast<float*>(y), part);
+    FA_LAUNCH("k_csr_residual");
+    hipLaunchKernelGGL(k_residual_finish, dim3(1), dim3(RED_BLOCK), 0, s, part, static_cast<int>(grid), norm2);
+    FA_LAUNCH("k_residual_finish");
+    return FA_OK;
+}

@@ -95,18 +95,23 @@ class DirichletBC:
     def _csr(self, A):
         return A.coalesce().tocsr() if isinstance(A, COOTensor) else A
 
-    def apply_matrix(self, matrix, *, check=True):
+    def apply_matrix(self, matrix, *, check=True, dtype=None):
+        """The filtered matrix (dirichlet_bc.py:123-147).  ``dtype``
+        (default: the matrix's) may be float32 for a float64 matrix: the
+        filtered operator of a mixed-precision solve, each value rounded once
+        in the same pass."""
         A = self.check_matrix(matrix) if check else matrix
         was_coo = isinstance(A, COOTensor)
         A = self._csr(A)
         L = nat.lib()
         n = A.shape[0]
         dev = A.device
+        dt = A.dtype if dtype is None else dtype
         new_ptr, new_idx = self._filtered_pattern(A)
-        new_val = torch.empty(new_idx.shape[0], dtype=A.dtype, device=dev)
-        nat.check(L.fa_dirichlet_fill(nat.dtype_code(A.dtype), n, nat.ptr(A.crow), nat.ptr(A.col),
-                                      nat.ptr(A.values), nat.ptr(self._bd_u8), nat.ptr(new_ptr), None,
-                                      nat.ptr(new_val), nat.stream()), "fa_dirichlet_fill")
+        new_val = torch.empty(new_idx.shape[0], dtype=dt, device=dev)
+        nat.check(L.fa_dirichlet_fill(nat.dtype_code(A.dtype), nat.dtype_code(dt), n, nat.ptr(A.crow),
+                                      nat.ptr(A.col), nat.ptr(A.values), nat.ptr(self._bd_u8), nat.ptr(new_ptr),
+                                      None, nat.ptr(new_val), nat.stream()), "fa_dirichlet_fill")
         out = CSRTensor(new_ptr, new_idx, new_val, A.shape)
         return out.tocoo() if was_coo else out
 
@@ -131,8 +136,8 @@ class DirichletBC:
                                            nat.stream()), "fa_dirichlet_count")
             new_idx = torch.empty(int(nnz.value), dtype=torch.int32, device=dev)
             scratch = torch.empty(int(nnz.value), dtype=A.dtype, device=dev)
-            nat.check(L.fa_dirichlet_fill(nat.dtype_code(A.dtype), n, nat.ptr(A.crow), nat.ptr(A.col),
-                                          nat.ptr(A.values), nat.ptr(self._bd_u8), nat.ptr(new_ptr),
+            nat.check(L.fa_dirichlet_fill(nat.dtype_code(A.dtype), nat.dtype_code(A.dtype), n, nat.ptr(A.crow),
+                                          nat.ptr(A.col), nat.ptr(A.values), nat.ptr(self._bd_u8), nat.ptr(new_ptr),
                                           nat.ptr(new_idx), nat.ptr(scratch), nat.stream()), "fa_dirichlet_fill")
             torch.cuda.current_stream(dev).synchronize()
             del cache[:-3]                      # keep a few patterns

@@ -9,7 +9,7 @@ import torch
 
 from .. import _native as nat
 from ..sparse.tensors import COOTensor, CSRTensor
-from .integrators import GroupIntegrator, Integrator, _axpby_sum, _call_assembly, p1_fusable_ops, \
+from .integrators import GroupIntegrator, Integrator, _axpby_sum, _call_assembly, coef_spec, p1_fusable_ops, \
     p1_matrix_values, p1_source_fusable, p1_vector_values
 from .pattern import pattern_of
 
@@ -204,6 +204,93 @@ class BilinearForm(Form):
         if out is None:
             raise RuntimeError("no integrator was added to the form")
         return out
+
+    def mult(self, x, out=None):
+        """Matrix-free product self @ x that never reads an assembled matrix
+        (the reference declares ``mult`` without providing it,
+        bilinear_form.py:107-117).  P1 scalar diffusion/mass groups on a
+        simplex mesh recompute their element rows in the kernel
+        (fa_p1_apply, FP64); any other form applies its element tensors
+        (``__matmul__`` with no matrix)."""
+        self.check_space()
+        if x.ndim != 1:
+            raise ValueError("mult takes a 1-D vector")
+        groups = self._p1_groups()
+        if groups is None:
+            y = self._apply_local(x)
+        else:
+            y = None
+            for ops in groups:
+                v = self._p1_apply(ops, x.to(torch.float64).contiguous())
+                y = v if y is None else _axpby_sum(y, v)
+        if out is not None:
+            out.copy_(y)
+            return out
+        return y
+
+    def residual(self, b, x, is_bd, *, dtype=torch.float64):
+        """(b - A x with 0 on the rows flagged by is_bd (uint8), the squared
+        norm of that vector as a 1-element FP64 device tensor).  A is applied
+        matrix-free in FP64; the result is stored as `dtype` (FP32 for the
+        right-hand side of an FP32 inner solve).  For a P1 scalar form this is
+        one kernel (fa_p1_apply, residual form); other forms go through
+        ``mult``."""
+        L = nat.lib()
+        space = self._spaces[0]
+        n = space.number_of_global_dofs()
+        if b.shape != (n,) or x.shape != (n,) or is_bd.shape != (n,):
+            raise ValueError("b, x and is_bd must be vectors over the space's DOFs")
+        groups = self._p1_groups()
+        norm2 = torch.empty(1, dtype=torch.float64, device=x.device)
+        out = torch.empty(n, dtype=dtype, device=x.device)
+        b = b.to(torch.float64).contiguous()
+        x = x.to(torch.float64).contiguous()
+        if groups is not None and len(groups) == 1:
+            self._p1_apply(groups[0], x, b, is_bd, out, norm2)
+            return out, norm2
+        r = b.clone()                            # r = b - A x
+        nat.check(L.fa_axpby(nat.FA_F64, n, -1.0, nat.ptr(self.mult(x).contiguous()), 1.0, nat.ptr(r), nat.stream()),
+                  "fa_axpby")
+        res = torch.zeros(n, dtype=torch.float64, device=x.device)
+        nat.check(L.fa_refine_update(nat.FA_F64, n, nat.ptr(res), nat.ptr(r), nat.ptr(is_bd), nat.stream()),
+                  "fa_refine_update")            # res = r off the Dirichlet rows, 0 on them
+        ws = nat.workspace(L.fa_reduce_work_bytes(), x.device)
+        nat.check(L.fa_dot(nat.FA_F64, n, nat.ptr(res), nat.ptr(res), nat.ptr(norm2), nat.ptr(ws), ws.numel(),
+                           nat.stream()), "fa_dot")
+        if dtype == torch.float64:
+            return res, norm2
+        nat.check(L.fa_cast(nat.FA_F64, nat.dtype_code(dtype), n, nat.ptr(res), nat.ptr(out), nat.stream()),
+                  "fa_cast")
+        return out, norm2
+
+    def _p1_groups(self):
+        space = self._spaces[0]
+        if space.p != 1 or getattr(space, 'TD', None) not in (2, 3) or space.mesh.GD != space.TD:
+            return None
+        groups = [p1_fusable_ops(it) for it in self.integrators.values()]
+        if not groups or any(g is None for g in groups):
+            return None
+        return groups
+
+    def _p1_apply(self, ops, x, b=None, is_bd=None, out=None, norm2=None):
+        L = nat.lib()
+        space = self._spaces[0]
+        mesh = space.mesh
+        pat = pattern_of(space)
+        q = ops[0][1].q if ops[0][1].q is not None else space.p + 3
+        _, qw, phi, _ = space.tables(q)
+        spec = {'diff': (nat.COEF_NONE, 0.0, None), 'mass': (nat.COEF_NONE, 0.0, None)}
+        for op, it in ops:
+            spec[op] = coef_spec(it.coef, space, q, allow_sep=True)
+        (dk, ds, dc), (mk, ms, mc) = spec['diff'], spec['mass']
+        y = torch.empty(pat.gdof, dtype=torch.float64, device=x.device) if out is None else out
+        ws = nat.workspace(L.fa_p1_apply_work_bytes(), x.device) if b is not None else None
+        nat.check(L.fa_p1_apply(nat.dtype_code(y.dtype), space.TD, pat.gdof, qw.shape[0], nat.ptr(mesh.node_rows16()),
+                                nat.ptr(space.cell_to_dof()), nat.ptr(qw), nat.ptr(phi), dk, ds, nat.ptr(dc), mk, ms,
+                                nat.ptr(mc), nat.ptr(pat.adj_ptr), nat.ptr(pat.adj), nat.ptr(x), nat.ptr(b),
+                                nat.ptr(is_bd), nat.ptr(y), nat.ptr(norm2), nat.ptr(ws),
+                                0 if ws is None else ws.numel(), nat.stream()), "fa_p1_apply")
+        return y
 
     @property
     def T(self):

@@ -346,11 +346,14 @@ def p1_matrix_values(space, ops, pat, dtype):
     values = torch.empty(pat.nnz, dtype=dtype, device=space.device)
     maps = pat.incidence_maps() if pat.nc >= TWO_PASS_MIN_CELLS else None
     if maps is not None:         # pass 1 per cell into incidence slots, pass 2 streams each row's slots
-        rows = torch.empty(pat.nc * pat.ldof * pat.ldof, dtype=torch.float64, device=space.device)
-        nat.check(L.fa_p1_incidence_rows(space.TD, pat.nc, qw.shape[0], nat.ptr(mesh.node_rows16()), nat.ptr(cell),
-                                         nat.ptr(qw), nat.ptr(phi), dk, ds, nat.ptr(dc), mk, ms, nat.ptr(mc),
-                                         nat.ptr(maps[0]), nat.ptr(rows), nat.stream()), "fa_p1_incidence_rows")
-        nat.check(L.fa_sum_incidence_rows(nat.dtype_code(dtype), space.TD, pat.gdof, nat.ptr(pat.adj_ptr),
+        # an FP32 matrix keeps its element rows in FP32 (half the slot traffic)
+        rows = torch.empty(pat.nc * pat.ldof * pat.ldof, dtype=dtype, device=space.device)
+        rcode = nat.dtype_code(dtype)
+        nat.check(L.fa_p1_incidence_rows(rcode, space.TD, pat.nc, qw.shape[0], nat.ptr(mesh.node_rows16()),
+                                         nat.ptr(cell), nat.ptr(qw), nat.ptr(phi), dk, ds, nat.ptr(dc), mk, ms,
+                                         nat.ptr(mc), nat.ptr(maps[0]), nat.ptr(rows), nat.stream()),
+                  "fa_p1_incidence_rows")
+        nat.check(L.fa_sum_incidence_rows(nat.dtype_code(dtype), rcode, space.TD, pat.gdof, nat.ptr(pat.adj_ptr),
                                           nat.ptr(pat.indptr), nat.ptr(maps[1]), nat.ptr(rows), pat.max_row,
                                           nat.ptr(values), nat.stream()), "fa_sum_incidence_rows")
         return values

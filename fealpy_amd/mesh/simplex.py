@@ -67,6 +67,30 @@ def flocc(array):
     return i0, i1, j
 
 
+def _device_entities(cell, local, nn):
+    """(entity, cell -> entity, entity -> cell) of the local entities `local`
+    (nloc x m vertex positions) of every cell, by the device topology kernels
+    (fa_topology_sort / fa_topology_fill): the numbering of flocc above."""
+    from .. import _native as nat
+    L = nat.lib()
+    dev = cell.device
+    cell = cell.to(torch.int32).contiguous()
+    loc = local.to(device=dev, dtype=torch.int32).contiguous()
+    nc, nv = cell.shape
+    nloc, m = loc.shape
+    ws = nat.workspace(L.fa_topology_work_bytes(nc * nloc, nn), dev)
+    nu = nat.ctypes.c_int64(0)
+    nat.check(L.fa_topology_sort(m, nloc, nat.ptr(loc), nv, nat.ptr(cell), nc, nn, nat.ctypes.byref(nu),
+                                 nat.ptr(ws), ws.numel(), nat.stream()), "fa_topology_sort")
+    n = int(nu.value)
+    entity = torch.empty((n, m), dtype=torch.int32, device=dev)
+    c2e = torch.empty((nc, nloc), dtype=torch.int32, device=dev)
+    e2c = torch.empty((n, 4), dtype=torch.int32, device=dev)
+    nat.check(L.fa_topology_fill(m, nloc, nat.ptr(loc), nv, nat.ptr(cell), nc, nn, nat.ptr(entity), nat.ptr(c2e),
+                                 nat.ptr(e2c), nat.ptr(ws), ws.numel(), nat.stream()), "fa_topology_fill")
+    return entity, c2e, e2c
+
+
 def _gdof(p, nums):
     """Number of Lagrange DOFs of degree p from entity counts (nodes, edges[, faces], cells)."""
     total = nums[0]
@@ -116,6 +140,18 @@ class SimplexMesh:
     def construct(self):
         cell = self.cell
         NC = cell.shape[0]
+        if cell.is_cuda and NC > 0:
+            it = self.itype
+            self.face, self.cell2face, self.face2cell = (t.to(it) for t in
+                                                         _device_entities(cell, self.localFace, self.node.shape[0]))
+            if self.TD == 3:
+                self.edge, self.cell2edge = (t.to(it) for t in
+                                             _device_entities(cell, self.localEdge, self.node.shape[0])[:2])
+            else:
+                self.edge = self.face
+                self.cell2edge = self.cell2face
+                self.edge2cell = self.face2cell
+            return
         lf = self.localFace.to(self.device)
         NFC = lf.shape[0]
         total_face = cell[:, lf].reshape(-1, lf.shape[1])

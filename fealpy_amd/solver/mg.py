@@ -357,25 +357,32 @@ class GeometricMG:
         return (tslice, terms, wset, len(vals))
 
     def setup(self, A):
-        """Numeric setup for the (Dirichlet-applied, FP64) fine matrix A."""
-        if A.dtype != torch.float64:
-            raise TypeError("GeometricMG.setup expects the FP64 matrix")
+        """Numeric setup for the (Dirichlet-applied) fine matrix A, FP64 or
+        FP32.  The coarse operators keep A's value type (an operator assembled
+        in FP32 gets FP32 Galerkin products, each formed in FP64 and rounded
+        once); the cycle runs in self.dtype either way."""
+        if A.dtype not in (torch.float64, torch.float32):
+            raise TypeError("GeometricMG.setup expects an FP64 or FP32 matrix")
         key = (A.crow.data_ptr(), A.col.data_ptr(), A.nnz)
         if key != self._pattern_key:
             self._symbolic(A)
             self._pattern_key = key
         L = nat.lib()
+        vd = A.dtype
+        vcode = nat.dtype_code(vd)
         self.levels[0].A = A
         for l in range(len(self.levels) - 1):
             for R, Af, P, C, cmax, terms in self.levels[l].stages:
                 Af = self.levels[l].A if Af is None else Af
+                if C.values.dtype != vd:            # value arrays follow the operator's type
+                    C._values = torch.zeros(C.nnz, dtype=vd, device=self.device)
                 if terms is not None:
                     tslice, tw, wset, nw = terms
-                    nat.check(L.fa_rap_values_terms(C.nnz, nat.ptr(tslice), nat.ptr(tw), wset, nw,
+                    nat.check(L.fa_rap_values_terms(vcode, C.nnz, nat.ptr(tslice), nat.ptr(tw), wset, nw,
                                                     nat.ptr(Af.values), nat.ptr(C.values), nat.stream()),
                               "fa_rap_values_terms")
                 else:
-                    nat.check(L.fa_rap_values(C.shape[0], nat.ptr(R.crow), nat.ptr(R.col), nat.ptr(R.values),
+                    nat.check(L.fa_rap_values(vcode, C.shape[0], nat.ptr(R.crow), nat.ptr(R.col), nat.ptr(R.values),
                                               nat.ptr(Af.crow), nat.ptr(Af.col), nat.ptr(Af.values),
                                               nat.ptr(P.crow), nat.ptr(P.col), nat.ptr(P.values), nat.ptr(C.crow),
                                               nat.ptr(C.col), cmax, nat.ptr(C.values), nat.stream()),
@@ -385,16 +392,18 @@ class GeometricMG:
         bounds = torch.empty(len(self.levels), dtype=torch.float64, device=self.device)
         f32 = self.dtype == torch.float32
         for i, lv in enumerate(self.levels):
-            lv.As.set_values(lv.A.values, torch.float64)          # FP64 SELL (the outer operator on level 0)
+            lv.As.set_values(lv.A.values, vd)       # SELL copy in A's type (level 0: the outer operator)
             if getattr(lv, 'compressed', None) is None:            # first setup of this pattern: band check
                 lv.compressed = self.compress and lv.n > 1024 and lv.As.set_compressed(lv.A.values, ws)
                 lv.diag = torch.empty_like(lv.dinv) if lv.compressed else None
             v32 = None
-            if f32 and not lv.compressed:
+            if f32 and not lv.compressed and vd == torch.float64:
                 v32 = lv.As.value_buffer(torch.float32, self.device)
-            d64 = lv.As.desc(torch.float64)
+            if not f32 and vd == torch.float32:     # an FP64 cycle over an FP32 operator: its FP64 copy
+                lv.As.set_values(lv.A.values, torch.float64)
+            din = lv.As.desc(vd)
             h16 = lv.As.desc16() if lv.compressed else None
-            nat.check(L.fa_smoother_setup(code, ctypes.byref(d64), nat.ptr(lv.dinv),
+            nat.check(L.fa_smoother_setup(code, vcode, ctypes.byref(din), nat.ptr(lv.dinv),
                                           nat.ptr(lv.diag) if lv.compressed else None,
                                           nat.ptr(v32) if v32 is not None else None,
                                           ctypes.byref(h16) if h16 is not None else None,
@@ -404,7 +413,13 @@ class GeometricMG:
             lv.lmax = float(lam)
         last = self.levels[-1]
         if self.coarse_dense:
-            nat.check(L.fa_csr_to_dense(last.n, nat.ptr(last.A.crow), nat.ptr(last.A.col), nat.ptr(last.A.values),
+            cv = last.A.values
+            if cv.dtype != torch.float64:
+                c64 = torch.empty(cv.shape[0], dtype=torch.float64, device=self.device)
+                nat.check(L.fa_cast(vcode, nat.FA_F64, cv.shape[0], nat.ptr(cv), nat.ptr(c64), nat.stream()),
+                          "fa_cast")
+                cv = c64
+            nat.check(L.fa_csr_to_dense(last.n, nat.ptr(last.A.crow), nat.ptr(last.A.col), nat.ptr(cv),
                                         nat.ptr(last.dense), nat.stream()), "fa_csr_to_dense")
             nat.check(L.fa_dense_spd_inverse(last.n, nat.ptr(last.dense), nat.stream()), "fa_dense_spd_inverse")
         self._build_desc()
@@ -472,7 +487,7 @@ class GeometricMG:
         have not changed since."""
         fresh = self._setup_A is A and getattr(self, '_setup_version', None) == A.values._version
         if not fresh:
-            self.setup(A if A.dtype == torch.float64 else A.astype(torch.float64))
+            self.setup(A)
             self._setup_A = A
             self._setup_version = A.values._version
 
@@ -482,8 +497,8 @@ class GeometricMG:
         self.ensure_setup(A)
         L = nat.lib()
         dt = A.dtype
-        if dt == torch.float64 and self.levels[0].A is A:
-            S = self.levels[0].As          # its FP64 copy was just filled from A by setup()
+        if self.levels[0].A is A:
+            S = self.levels[0].As          # its copy in A's type was just filled from A by setup()
         else:
             S = sell_of(A)
         b = b.to(dt).contiguous()

@@ -81,9 +81,9 @@ class SellMatrix:
         checked = band is True
         if not checked:
             self.h_offs = torch.zeros(self.entries, dtype=torch.int16, device=values.device)
-        st = nat.lib().fa_sell16_fill(self.shape[0], nat.ptr(self.crow), nat.ptr(self.col),
-                                      nat.ptr(values.contiguous()), nat.ptr(self.slice_ptr), nat.ptr(self.h_offs),
-                                      nat.ptr(self.h_vals), None if checked else nat.ptr(work),
+        st = nat.lib().fa_sell16_fill(nat.dtype_code(values.dtype), self.shape[0], nat.ptr(self.crow),
+                                      nat.ptr(self.col), nat.ptr(values.contiguous()), nat.ptr(self.slice_ptr),
+                                      nat.ptr(self.h_offs), nat.ptr(self.h_vals), None if checked else nat.ptr(work),
                                       0 if checked else work.numel(), nat.stream())
         if st == nat.FA_ECAPACITY:
             self.h_offs = None

@@ -241,24 +241,87 @@ int fa_assemble_p1_vector(int dtype, int td, int64_t gdof, int nq, const double*
  * thread per cell, vertex coordinates `node_wide` as 16-byte rows (NN x 4
  * doubles in 3D, the node array itself in 2D) (ScalarDiffusionIntegrator / ScalarMassIntegrator /
  * ScalarSourceIntegrator .assembly, fem/scalar_*_integrator.py):
- * fa_p1_incidence_rows writes element row i of cell c at rows[inc*ldof ..],
+ * fa_p1_incidence_rows writes element row i of cell c at rows[inc*ldof ..]
+ * (computed in FP64, stored as dtype_rows: FP64, or FP32 for an FP32 matrix,
+ * whose pass 2 then sums in FP32; FP64 rows and values are the fused
+ * kernel's bit for bit),
  * fa_p1_incidence_vectors the load-vector entry at vals[inc].  Pass 2 streams
  * each row's slots: fa_sum_incidence_rows (CSR values), fa_sum_incidence_values
  * (the load vector, an ordered sum per DOF). */
 int fa_incidence_maps(int64_t gdof, int32_t ldof, const int32_t* cell2dof, const int32_t* adj_ptr,
                       const int32_t* adj, const int32_t* indptr, const int32_t* indices, int32_t* inc, uint32_t* pos,
                       void* work, size_t work_bytes, fa_stream_t stream);
-int fa_p1_incidence_rows(int td, int64_t nc, int nq, const double* node_wide, const int32_t* cell, const double* qw,
-                         const double* phi, int diff_kind, double diff_scale, const double* diff_coef, int mass_kind,
-                         double mass_scale, const double* mass_coef, const int32_t* inc, double* rows,
-                         fa_stream_t stream);
-int fa_sum_incidence_rows(int dtype, int td, int64_t gdof, const int32_t* adj_ptr, const int32_t* indptr,
-                          const uint32_t* pos, const double* rows, int32_t max_row, void* values, fa_stream_t stream);
+int fa_p1_incidence_rows(int dtype_rows, int td, int64_t nc, int nq, const double* node_wide, const int32_t* cell,
+                         const double* qw, const double* phi, int diff_kind, double diff_scale,
+                         const double* diff_coef, int mass_kind, double mass_scale, const double* mass_coef,
+                         const int32_t* inc, void* rows, fa_stream_t stream);
+int fa_sum_incidence_rows(int dtype, int dtype_rows, int td, int64_t gdof, const int32_t* adj_ptr,
+                          const int32_t* indptr, const uint32_t* pos, const void* rows, int32_t max_row, void* values,
+                          fa_stream_t stream);
 int fa_p1_incidence_vectors(int td, int64_t nc, int nq, const double* node_wide, const int32_t* cell, const double* qw,
                             const double* phi, int src_kind, double src_scale, const double* src, const int32_t* inc,
                             double* vals, fa_stream_t stream);
 int fa_sum_incidence_values(int dtype, int64_t gdof, const int32_t* adj_ptr, const double* vals, void* out,
                             fa_stream_t stream);
+
+/* Matrix-free P1 operator: y = A x with A the matrix fa_assemble_p1_matrix
+ * would assemble for the same integrators (element rows recomputed from the
+ * cell geometry and coefficients where they are used; nothing is stored).
+ * Replaces BilinearForm.__matmul__ with no assembled matrix
+ * (fealpy/fem/bilinear_form.py:126-158: element tensors from
+ * assembly_local_iterative applied to the gathered u, then index_add).  Each
+ * y_r sums its incidences (adj_ptr, adj) in a fixed order, so the result is
+ * the same run to run.  With b != NULL, the residual form used by iterative
+ * refinement: y = b - A x, y_r = 0 where is_bd[r] != 0, and *norm2 (device) =
+ * sum of y_r^2 in a fixed order; `work` holds fa_p1_apply_work_bytes() bytes.
+ * node_wide as for fa_p1_incidence_rows.  x and b are FP64, y is stored as
+ * dtype_y (FP64, or FP32 for the right-hand side of an FP32 inner solve);
+ * sums and the norm are formed in FP64. */
+size_t fa_p1_apply_work_bytes(void);
+int fa_p1_apply(int dtype_y, int td, int64_t gdof, int nq, const double* node_wide, const int32_t* cell,
+                const double* qw, const double* phi, int diff_kind, double diff_scale, const double* diff_coef,
+                int mass_kind, double mass_scale, const double* mass_coef, const int32_t* adj_ptr,
+                const int32_t* adj, const double* x, const double* b, const uint8_t* is_bd, void* y, double* norm2,
+                void* work, size_t work_bytes, fa_stream_t stream);
+
+/* ------------------------------------------------------------- topology --
+ * Unique faces / edges of a simplex mesh: MeshDS.construct + flocc
+ * (fealpy/mesh/mesh_data_structure.py:428-464, fealpy/mesh/utils.py:81-110).
+ * `local` (nloc x m, m = 2 or 3) lists the vertex positions of each local
+ * entity of a cell (localFace / localEdge); cell is (nc x nv) int32 over nn
+ * vertices.  Entities are numbered in the lexicographic order of their
+ * sorted vertex tuples, and stored with the orientation of their first
+ * occurrence (cell-major order); entity2cell[g] = [c0, c1, l0, l1] from the
+ * first and last occurrence; cell2entity[c][l] = g.  fa_topology_sort
+ * (counting sort by the smallest vertex, then a per-vertex LDS sort; syncs)
+ * returns the number of entities; fa_topology_fill, with the same workspace,
+ * writes entity (n x m), cell2entity (nc x nloc) and entity2cell (n x 4).
+ * FA_ECAPACITY when more than 1024 occurrences share a smallest vertex. */
+size_t fa_topology_work_bytes(int64_t nocc, int64_t nn);
+int fa_topology_sort(int m, int nloc, const int32_t* local, int nv, const int32_t* cell, int64_t nc, int64_t nn,
+                     int64_t* nuniq_host, void* work, size_t work_bytes, fa_stream_t stream);
+int fa_topology_fill(int m, int nloc, const int32_t* local, int nv, const int32_t* cell, int64_t nc, int64_t nn,
+                     int32_t* entity, int32_t* cell2entity, int32_t* entity2cell, void* work, size_t work_bytes,
+                     fa_stream_t stream);
+
+/* FP64 residual of iterative refinement on an assembled FP64 CSR matrix A
+ * (the matrix before the Dirichlet condition): y = b - A x on the rows with
+ * is_bd == 0, 0 on the others, stored as dtype_y (FP64 or FP32), and
+ * *norm2 (device) = sum of y^2 in a fixed order.  With x equal to the
+ * boundary values on the Dirichlet rows this is the residual of the system
+ * DirichletBC.apply produces (fealpy/fem/dirichlet_bc.py:101-121).  Work:
+ * fa_csr_residual_work_bytes().  No counterpart in the reference (its cg
+ * runs in FP64 throughout, fealpy/solver/cg.py:14-123). */
+size_t fa_csr_residual_work_bytes(void);
+int fa_csr_residual(int dtype_y, int64_t n, const int32_t* indptr, const int32_t* indices, const double* values,
+                    const double* x, const double* b, const uint8_t* is_bd, void* y, double* norm2, void* work,
+                    size_t work_bytes, fa_stream_t stream);
+
+/* Correction step of iterative refinement: x += d on every row with
+ * is_bd[i] == 0 (x FP64, d of dtype_d, FP64 or FP32); Dirichlet rows keep
+ * their boundary values exactly.  No counterpart in the reference (its cg
+ * has no refinement); used by the mixed-precision solve. */
+int fa_refine_update(int dtype_d, int64_t n, double* x, const void* d, const uint8_t* is_bd, fa_stream_t stream);
 
 /* Physical points of barycentric quadrature points on every cell:
  * out[c][q][d] = sum_v bcs[q][v] node[cell[c][v]][d]  (FP64; (gd, nv) in
@@ -276,14 +339,16 @@ int fa_bc_to_point(int gd, int nv, int64_t nc, int nq, const double* node, const
  * :123-147); f <- f - A uh, f[bd] <- uh[bd] (apply_vector :149-198).
  * fa_dirichlet_count writes new_indptr[n+1] and returns the new nnz (syncs);
  * fa_dirichlet_fill writes new_indices (skipped when NULL: pattern reuse) and
- * new_values.  is_bd: uint8[n]; uh: boundary values (zeros off the boundary). */
+ * new_values, stored as dtype_out (dtype, or FP32 from FP64 values: the
+ * filtered FP32 operator of a mixed-precision solve, each value rounded once).
+ * is_bd: uint8[n]; uh: boundary values (zeros off the boundary). */
 size_t fa_dirichlet_work_bytes(int64_t n);
 int fa_dirichlet_count(int64_t n, const int32_t* indptr, const int32_t* indices, const uint8_t* is_bd,
                        int32_t* new_indptr, int64_t* new_nnz_host,
                        void* work, size_t work_bytes, fa_stream_t stream);
-int fa_dirichlet_fill(int dtype, int64_t n, const int32_t* indptr, const int32_t* indices, const void* values,
-                      const uint8_t* is_bd, const int32_t* new_indptr, int32_t* new_indices, void* new_values,
-                      fa_stream_t stream);
+int fa_dirichlet_fill(int dtype, int dtype_out, int64_t n, const int32_t* indptr, const int32_t* indices,
+                      const void* values, const uint8_t* is_bd, const int32_t* new_indptr, int32_t* new_indices,
+                      void* new_values, fa_stream_t stream);
 int fa_dirichlet_rhs(int dtype, int64_t n, const int32_t* indptr, const int32_t* indices, const void* values,
                      const uint8_t* is_bd, const void* uh, const void* f, void* f_out, fa_stream_t stream);
 /* fa_dirichlet_fill (values only, on the pattern of fa_dirichlet_count) and
@@ -367,10 +432,13 @@ int fa_rap_count(int64_t nrows, int64_t ncols, const int32_t* r_ptr, const int32
 int fa_rap_fill(int64_t nrows, const int32_t* r_ptr, const int32_t* r_idx, const int32_t* a_ptr,
                 const int32_t* a_idx, const int32_t* p_ptr, const int32_t* p_idx, int32_t max_cand,
                 const int32_t* c_ptr, int32_t* c_idx, fa_stream_t stream);
-int fa_rap_values(int64_t nrows, const int32_t* r_ptr, const int32_t* r_idx, const double* r_val,
-                  const int32_t* a_ptr, const int32_t* a_idx, const double* a_val,
+/* fa_rap_values / fa_rap_values_terms: a_val and c_val are of `dtype` (FP64
+ * or FP32, the type of the operator being set up); products and sums are
+ * formed in FP64. */
+int fa_rap_values(int dtype, int64_t nrows, const int32_t* r_ptr, const int32_t* r_idx, const double* r_val,
+                  const int32_t* a_ptr, const int32_t* a_idx, const void* a_val,
                   const int32_t* p_ptr, const int32_t* p_idx, const double* p_val,
-                  const int32_t* c_ptr, const int32_t* c_idx, int32_t max_row, double* c_val,
+                  const int32_t* c_ptr, const int32_t* c_idx, int32_t max_row, void* c_val,
                   fa_stream_t stream);
 
 /* Numeric R A P from a precomputed term list, for transfers with at most 3
@@ -396,8 +464,8 @@ int fa_rap_terms_fill(int64_t nrows, const int32_t* r_ptr, const int32_t* r_idx,
                       const double* p_val, int32_t max_cand, const int32_t* c_ptr, int64_t nnz_c,
                       const double* values, int nvalues, const int32_t* tslice, int64_t nterms, uint32_t* terms,
                       void* work, size_t work_bytes, fa_stream_t stream);
-int fa_rap_values_terms(int64_t nnz_c, const int32_t* tslice, const uint32_t* terms, const double* values,
-                        int nvalues, const double* a_val, double* c_val, fa_stream_t stream);
+int fa_rap_values_terms(int dtype, int64_t nnz_c, const int32_t* tslice, const uint32_t* terms,
+                        const double* values, int nvalues, const void* a_val, void* c_val, fa_stream_t stream);
 
 /* ------------------------------------------------------- multigrid cycle --
  * Geometric multigrid V-cycle used as a CG preconditioner: Chebyshev
@@ -439,13 +507,14 @@ typedef struct {
     int coarse_degree;
 } fa_mg_t;
 
-/* Fill a compressed SELL (offsets and FP16 values of D^-1 A) from the FP64
- * CSR, on the slices `slice_ptr` of the level's SELL.  With a workspace
+/* Fill a compressed SELL (offsets and FP16 values of D^-1 A) from the CSR
+ * (values of `dtype`, FP64 or FP32), on the slices `slice_ptr` of the level's
+ * SELL.  With a workspace
  * (>= 4 bytes) it checks the band and syncs: FA_ECAPACITY when a column lies
  * more than 32767 from its row (the level then keeps the full format).
  * work = NULL refreshes the values of an already checked pattern without a
  * sync. */
-int fa_sell16_fill(int64_t n, const int32_t* indptr, const int32_t* indices, const double* values,
+int fa_sell16_fill(int dtype, int64_t n, const int32_t* indptr, const int32_t* indices, const void* values,
                    const int32_t* slice_ptr, int16_t* offs, void* vals, void* work, size_t work_bytes,
                    fa_stream_t stream);
 
@@ -466,7 +535,8 @@ int fa_jacobi_setup(int dtype_out, int64_t n, int64_t nnz, const int32_t* indptr
                     fa_stream_t stream);
 int fa_mg_lmax(const fa_mg_t* mg, int level, int iters, double* lmax_host, void* work, size_t work_bytes,
                fa_stream_t stream);
-/* Smoother data of a level in one pass over its FP64 SELL copy A, no sync:
+/* Smoother data of a level in one pass over its SELL copy A (values of
+ * dtype_in: FP64, or FP32 for an operator assembled in FP32), no sync:
  * dinv (dtype) as fa_jacobi_setup, diag = 1/dinv (or NULL), the FP16
  * D^-1 A values of Ah (offsets already filled by fa_sell16_fill on the same
  * slices; Ah NULL or Ah->vals NULL: none), the FP32 copy of A's values
@@ -474,8 +544,8 @@ int fa_mg_lmax(const fa_mg_t* mg, int level, int iters, double* lmax_host, void*
  * (device memory; read it back with the other levels' bounds in one copy).
  * Replaces the per-level diagonal extraction of GAMGSolver.setup
  * (fealpy/solver/gamg_solver.py:85-120).  Work: fa_reduce_work_bytes(). */
-int fa_smoother_setup(int dtype, const fa_sell_t* A, void* dinv, void* diag, void* vals_f32, const fa_sell16_t* Ah,
-                      double* bound_dev, void* work, size_t work_bytes, fa_stream_t stream);
+int fa_smoother_setup(int dtype, int dtype_in, const fa_sell_t* A, void* dinv, void* diag, void* vals_f32,
+                      const fa_sell16_t* Ah, double* bound_dev, void* work, size_t work_bytes, fa_stream_t stream);
 /* z = V(r) on the finest level (r, z: dtype of the cycle). */
 int fa_mg_vcycle(const fa_mg_t* mg, const void* r, void* z, fa_stream_t stream);
 
@@ -547,7 +617,11 @@ enum {
     FA_TIMER_P1_SUM_VALUES = 5,  /* P1 load vector, pass 2                     */
     FA_TIMER_BC_APPLY = 6,       /* Dirichlet filter + right-hand side         */
     FA_TIMER_SELL_SPMV = 7,      /* fa_sell_spmv                               */
-    FA_TIMER_VCYCLE = 8          /* one whole V-cycle (all its kernels)        */
+    FA_TIMER_VCYCLE = 8,         /* one whole V-cycle (all its kernels)        */
+    FA_TIMER_P1_APPLY = 9,       /* matrix-free P1 operator (fa_p1_apply)      */
+    FA_TIMER_SMOOTH0 = 10        /* finest-level FP16 smoothing steps and
+                                    residual of the V-cycle (k_cheb16*,
+                                    k_residual16 on level 0)                   */
 };
 int fa_timer_arm(unsigned mask);
 int fa_timer_read(int kind, int64_t* launches, double* total_ms);

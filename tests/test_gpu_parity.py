@@ -275,10 +275,11 @@ def test_fused_p1_matches_element_gather(dim):
 @pytest.mark.parametrize("dim", [2, 3])
 def test_incidence_two_pass_p1(dim, monkeypatch):
     """Two-pass P1 assembly in incidence order: the maps equal a NumPy
-    construction; the CSR values equal the fused single kernel bit for bit
-    (FP64 and FP32 output, diffusion + mass with per-point / per-cell
-    coefficients); the load vector equals the fused kernel to rounding (that
-    kernel adds lane partials by a tree, this one in cell order)."""
+    construction; the FP64 CSR values equal the fused single kernel bit for
+    bit (diffusion + mass with per-point / per-cell coefficients); FP32
+    values (element rows kept and summed in FP32) within FP32 rounding of
+    them; the load vector equals the fused kernel to rounding (that kernel
+    adds lane partials by a tree, this one in cell order)."""
     from fealpy_amd import _native as nat
     from fealpy_amd.fem import integrators
     from fealpy_amd.fem.pattern import pattern_of
@@ -321,7 +322,13 @@ def test_incidence_two_pass_p1(dim, monkeypatch):
                                           1.0, nat.ptr(kc), nat.ptr(pat.adj_ptr), nat.ptr(pat.adj),
                                           nat.ptr(pat.indptr), nat.ptr(pat.indices), pat.max_row, nat.ptr(fused),
                                           nat.stream()), "fa_assemble_p1_matrix")
-        assert torch.equal(two, fused), dt
+        if dt == torch.float64:
+            assert torch.equal(two, fused)
+            f64 = fused
+        else:
+            scale = float(f64.abs().max())
+            assert float((two.double() - f64).abs().max()) <= 4e-7 * scale
+            assert float((fused.double() - f64).abs().max()) <= 1e-7 * scale
     F = fa.LinearForm(space).add_integrator(fa.ScalarSourceIntegrator(kc, q=q)).assembly()
     Ff = torch.empty(pat.gdof, dtype=torch.float64, device=DEV)
     nat.check(L.fa_assemble_p1_vector(nat.FA_F64, dim, pat.gdof, nq, nat.ptr(mesh.node), nat.ptr(cell), nat.ptr(qw),

@@ -84,7 +84,7 @@ def _rap_by_search(f, c):
     import ctypes
     from fealpy_amd import _native as nat
     out = torch.empty_like(c.A.values)
-    nat.check(nat.lib().fa_rap_values(c.n, nat.ptr(f.R.crow), nat.ptr(f.R.col), nat.ptr(f.R.values),
+    nat.check(nat.lib().fa_rap_values(nat.FA_F64, c.n, nat.ptr(f.R.crow), nat.ptr(f.R.col), nat.ptr(f.R.values),
                                       nat.ptr(f.A.crow), nat.ptr(f.A.col), nat.ptr(f.A.values), nat.ptr(f.P.crow),
                                       nat.ptr(f.P.col), nat.ptr(f.P.values), nat.ptr(c.A.crow), nat.ptr(c.A.col),
                                       c.max_row, nat.ptr(out), nat.stream()), "fa_rap_values")
@@ -167,12 +167,12 @@ def test_rap_terms_random_transfers(values):
     nat.check(L.fa_rap_terms_fill(*args, nat.ptr(tslice), nt.value, nat.ptr(terms), nat.ptr(ws), ws.numel(),
                                   nat.stream()), "fa_rap_terms_fill")
     cval = torch.empty(ref.nnz, dtype=torch.float64, device=DEV)
-    nat.check(L.fa_rap_values_terms(ref.nnz, nat.ptr(tslice), nat.ptr(terms), wset, len(values), nat.ptr(a[2]),
+    nat.check(L.fa_rap_values_terms(nat.FA_F64, ref.nnz, nat.ptr(tslice), nat.ptr(terms), wset, len(values), nat.ptr(a[2]),
                                     nat.ptr(cval), nat.stream()), "fa_rap_values_terms")
     assert np.array_equal(cptr.cpu().numpy(), ref.indptr) and np.array_equal(cidx.cpu().numpy(), ref.indices)
     np.testing.assert_allclose(cval.cpu().numpy(), ref.data, rtol=0, atol=1e-13 * abs(ref.data).max())
     four = (ctypes.c_double * 4)(1.0, 0.75, 0.5, 0.25)
-    st = L.fa_rap_values_terms(ref.nnz, nat.ptr(tslice), nat.ptr(terms), four, 4, nat.ptr(a[2]), nat.ptr(cval),
+    st = L.fa_rap_values_terms(nat.FA_F64, ref.nnz, nat.ptr(tslice), nat.ptr(terms), four, 4, nat.ptr(a[2]), nat.ptr(cval),
                                nat.stream())
     assert st == nat.FA_ECAPACITY
 
@@ -332,7 +332,7 @@ def test_smoother_setup_matches_csr_setup(dtype):
     diag = torch.empty(n, dtype=dtype, device=DEV)
     bound = torch.zeros(1, dtype=torch.float64, device=DEV)
     d64, h16 = S.desc(torch.float64), S.desc16()
-    nat.check(L.fa_smoother_setup(code, ctypes.byref(d64), nat.ptr(dinv), nat.ptr(diag), nat.ptr(v32),
+    nat.check(L.fa_smoother_setup(code, nat.FA_F64, ctypes.byref(d64), nat.ptr(dinv), nat.ptr(diag), nat.ptr(v32),
                                   ctypes.byref(h16), nat.ptr(bound), nat.ptr(ws), ws.numel(), nat.stream()),
               "fa_smoother_setup")
     assert torch.equal(dinv, dinv_ref)
