@@ -766,6 +766,10 @@ def main():
     pat = pattern_of(space)
     torch.cuda.synchronize()
     t_pattern = time.perf_counter() - t0
+    # tile shapes of the step's tunable kernels, measured on this problem (untimed)
+    from fealpy_amd.tune import Autotuner
+    tuner = Autotuner(path=None)
+    tuned_lanes, tuned_ms = tuner.tune_assembly(space, k, q=2)
     bc = fa.DirichletBC(space, gd=u)
     mg32 = GeometricMG(space, bc.is_boundary_dof, dtype=torch.float32)
     mg64 = GeometricMG(space, bc.is_boundary_dof, dtype=torch.float64)
@@ -832,6 +836,8 @@ def main():
         "cold": {"pattern_ms": round(t_pattern * 1e3, 3), "mesh_s": round(t_mesh, 3),
                  "dof_per_s_with_pattern": round(gdof / (t_mixed + t_pattern), 1)},
         "reference_solve": {"pcg_iterations": info_ref["niter"]},
+        "autotune": {"assembly_pass2_lanes_per_row": tuned_lanes,
+                     "candidates_ms": {str(c): round(v, 4) for c, v in tuned_ms.items()}},
         "roofline": roof,
         "roofline_assembly": roof_asm,
         "roofline_sell_spmv_f32": roof32,

@@ -102,6 +102,12 @@ _SIGNATURES = {
                                                _p, _p, _p, _p]),
     "fa_sum_incidence_values": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p]),
     "fa_refine_update": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p]),
+    "fa_tune_set": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "fa_amg_aggregate_work_bytes": (_sz, [_i64, _i64]),
+    "fa_amg_aggregate": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _f64, _p, _pi64, _pi32, _p, _sz, _p]),
+    "fa_amg_smooth_prolongation": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p, _p, _p, _p, _f64, _p, _p]),
+    "fa_tune_get": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "fa_tune_key": (ctypes.c_int, [ctypes.c_int, _i64, _i64]),
     "fa_topology_work_bytes": (_sz, [_i64, _i64]),
     "fa_topology_sort": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _p, ctypes.c_int, _p, _i64, _i64, _pi64, _p, _sz,
                                         _p]),

@@ -1540,16 +1540,25 @@ extern "C" int fa_sum_incidence_rows(int dtype, int dtype_rows, int td, int64_t 
                    : launch_sum_rows<NV_, G_, K_, float, double>(gdof, adj_ptr, indptr, pos, r64, max_row, values, s); \
     } while (0)
     // lanes per row G: the sum runs over the incidences in order for any G, so
-    // G only sets the chunking (8 measured fastest on 3D P1 rows of 15 entries
-    // and 24 incidences: 32 rows a block, three pipelined chunks a row)
+    // G only sets the chunking and the values do not depend on it.  Built-in
+    // choice: 8 for rows of up to 32 entries (3D P1: 15 entries, 24
+    // incidences), else enough lanes for K <= 4 positions each; the autotuner
+    // may set another (FA_TUNE_SUM_ROWS_LANES).  K = positions owned per lane.
+    const int gdef = max_row <= 32 ? 8 : (max_row <= 64 ? 16 : 32);
+    const int G = tuned(FA_TUNE_SUM_ROWS_LANES, fa_tune_key(FA_TUNE_SUM_ROWS_LANES, td, max_row), gdef);
     if (td == 3) {
-        if (max_row <= 32) FA_SR(4, 8, 4);
-        if (max_row <= 64) FA_SR(4, 16, 4);
+        if (G == 4 && max_row <= 32) FA_SR(4, 4, 8);
+        if (G == 16 && max_row <= 32) FA_SR(4, 16, 2);
+        if (G == 32 && max_row <= 32) FA_SR(4, 32, 1);
+        if (G <= 8 && max_row <= 32) FA_SR(4, 8, 4);
+        if (G <= 16 && max_row <= 64) FA_SR(4, 16, 4);
         if (max_row <= 128) FA_SR(4, 32, 4);
         FA_SR(4, 32, 8);
     } else {
-        if (max_row <= 16) FA_SR(3, 8, 2);
-        if (max_row <= 32) FA_SR(3, 8, 4);
+        if (G == 4 && max_row <= 16) FA_SR(3, 4, 4);
+        if (G == 16 && max_row <= 16) FA_SR(3, 16, 1);
+        if (G <= 8 && max_row <= 16) FA_SR(3, 8, 2);
+        if (G <= 8 && max_row <= 32) FA_SR(3, 8, 4);
         if (max_row <= 128) FA_SR(3, 32, 4);
         FA_SR(3, 32, 8);
     }

@@ -13,6 +13,9 @@ constexpr int WAVE = 64;
 
 void set_error(const char* fmt, ...);
 
+// autotuned tile of a kernel family for a shape class (fa_tune_set), or fallback
+int tuned(int family, int key, int fallback);
+
 // In-step kernel timer (fa_timer_arm / fa_timer_read): when the kernel family
 // `kind` (FA_TIMER_*) is armed, timer_begin / timer_end record HIP events on
 // `s` around its launch; otherwise they do nothing.

@@ -15,10 +15,55 @@
 namespace fa {
 
 // ---------------------------------------------------------------------------
+// sort buf[0, m) ascending (padded to a power of two with INT_MAX) and
+// compact its distinct values to the front; returns how many there are.
+// One wavefront; m <= CAP (a power of two).  The compaction writes each value
+// at a position no larger than the one it is read from, and a position is
+// overwritten only after every read of it, so it runs in place.
+template <int CAP>
+__device__ inline int rap_sort_unique(int32_t* buf, int m) {
+    const int lane = threadIdx.x;
+    int P = 2;
+    while (P < m) P <<= 1;
+    for (int t = m + lane; t < P; t += WAVE) buf[t] = INT_MAX;
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = lane; t < P; t += WAVE) {
+                int p = t ^ j;
+                if (p > t) {
+                    int32_t a = buf[t], b = buf[p];
+                    if ((a > b) == ((t & k) == 0)) {
+                        buf[t] = b;
+                        buf[p] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    int count = 0;
+    for (int t0 = 0; t0 < P; t0 += WAVE) {
+        const int t = t0 + lane;
+        const int32_t v = t < P ? buf[t] : INT_MAX;
+        const int32_t prev = (t > 0 && t < P) ? buf[t - 1] : INT_MIN;
+        const bool keep = v != INT_MAX && (t == 0 || prev != v);
+        __syncthreads();
+        const int inc = wave_inclusive_scan(keep ? 1 : 0);
+        if (keep) buf[count + inc - 1] = v;
+        count += __shfl(inc, WAVE - 1);
+        __syncthreads();
+    }
+    return count;
+}
+
 // R A P: symbolic (count / fill) with one wavefront per coarse row.
 // Candidates J are enumerated R-entry by R-entry (sequential), lanes over the
 // entries of A's row, each lane appending the columns of its P row through a
 // wave prefix sum; then bitonic sort + unique in LDS (as the pattern kernel).
+// When the next batch would not fit, the list is first sorted and compacted
+// to its distinct columns, so CAP bounds the distinct columns of the row plus
+// one batch, not all candidates.
 template <int CAP>
 __global__ void __launch_bounds__(WAVE) k_rap_rows(int64_t nrows, const int32_t* __restrict__ rp,
                                                    const int32_t* __restrict__ ri, const int32_t* __restrict__ ap,
@@ -44,12 +89,16 @@ __global__ void __launch_bounds__(WAVE) k_rap_rows(int64_t nrows, const int32_t*
                 }
                 const int inc = wave_inclusive_scan(cnt);
                 const int tot = __shfl(inc, WAVE - 1);
+                if (m + tot > CAP && !ovf) {      // make room: keep the distinct columns only
+                    __syncthreads();
+                    m = rap_sort_unique<CAP>(buf, m);
+                }
                 if (m + tot <= CAP) {
                     for (int c = 0; c < cnt; ++c) buf[m + inc - cnt + c] = pi[pp[j] + c];
+                    m += tot;
                 } else {
                     ovf = true;
                 }
-                m += tot;
             }
         }
         __syncthreads();
@@ -57,35 +106,10 @@ __global__ void __launch_bounds__(WAVE) k_rap_rows(int64_t nrows, const int32_t*
             if (lane == 0) atomicOr(overflow, 1);
             m = 0;
         }
-        int P = 2;
-        while (P < m) P <<= 1;
-        for (int t = m + lane; t < P; t += WAVE) buf[t] = INT_MAX;
-        __syncthreads();
-        for (int k = 2; k <= P; k <<= 1) {
-            for (int j = k >> 1; j > 0; j >>= 1) {
-                for (int t = lane; t < P; t += WAVE) {
-                    int p = t ^ j;
-                    if (p > t) {
-                        int32_t a = buf[t], b = buf[p];
-                        if ((a > b) == ((t & k) == 0)) {
-                            buf[t] = b;
-                            buf[p] = a;
-                        }
-                    }
-                }
-                __syncthreads();
-            }
-        }
-        int32_t count = 0;
+        const int32_t count0 = rap_sort_unique<CAP>(buf, m);
         const int32_t base = cidx ? cptr[I] : 0;
-        for (int t0 = 0; t0 < P; t0 += WAVE) {
-            const int t = t0 + lane;
-            const int32_t v = t < P ? buf[t] : INT_MAX;
-            const bool keep = v != INT_MAX && (t == 0 || buf[t - 1] != v);
-            const int inc = wave_inclusive_scan(keep ? 1 : 0);
-            if (cidx && keep) cidx[base + count + inc - 1] = v;
-            count += __shfl(inc, WAVE - 1);
-        }
+        for (int t = lane; cidx && t < count0; t += WAVE) cidx[base + t] = buf[t];
+        int32_t count = count0;
         if (count == 0) {  // empty coarse row (Dirichlet node): unit diagonal
             if (cidx && lane == 0) cidx[base] = static_cast<int32_t>(I);
             count = 1;
@@ -169,6 +193,62 @@ __global__ void __launch_bounds__(WAVE) k_rap_values(int64_t nrows, const int32_
                 acc[l * lmax + t] += acc[(l + h) * lmax + t];
             }
             __syncthreads();
+        }
+        const bool empty = (r0 == r1);
+        for (int t = lane; t < L; t += WAVE) cval[c0 + t] = static_cast<T>(empty ? 1.0 : acc[t]);
+        __syncthreads();
+    }
+}
+
+// R A P numeric for wide coarse rows (the per-lane tables above would not fit
+// in LDS): one accumulator per coarse entry.  The (R entry u, A entry t)
+// pairs are taken one after another in R order, then A order; the lanes
+// share out the entries of P's row j(t), whose columns are distinct, so no
+// two lanes add to one accumulator in a step, and every entry is summed in
+// the fixed (u, t) order.
+template <class T>
+__global__ void __launch_bounds__(WAVE) k_rap_values_seq(int64_t nrows, const int32_t* __restrict__ rp,
+                                                         const int32_t* __restrict__ ri, const double* __restrict__ rv,
+                                                         const int32_t* __restrict__ ap, const int32_t* __restrict__ ai,
+                                                         const T* __restrict__ av, const int32_t* __restrict__ pp,
+                                                         const int32_t* __restrict__ pi, const double* __restrict__ pv,
+                                                         const int32_t* __restrict__ cptr,
+                                                         const int32_t* __restrict__ cidx, int lmax,
+                                                         T* __restrict__ cval) {
+    extern __shared__ char smem[];
+    int32_t* cols = reinterpret_cast<int32_t*>(smem);
+    double* acc = reinterpret_cast<double*>(smem + align_up(sizeof(int32_t) * lmax, 16));
+    const int lane = threadIdx.x;
+    for (int64_t I = blockIdx.x; I < nrows; I += gridDim.x) {
+        const int32_t c0 = cptr[I];
+        const int L = cptr[I + 1] - c0;
+        for (int t = lane; t < L; t += WAVE) {
+            cols[t] = cidx[c0 + t];
+            acc[t] = 0.0;
+        }
+        __syncthreads();
+        const int32_t r0 = rp[I], r1 = rp[I + 1];
+        for (int32_t u = r0; u < r1; ++u) {
+            const int32_t i = ri[u];
+            const double ru = rv[u];
+            for (int32_t t = ap[i]; t < ap[i + 1]; ++t) {
+                const int32_t j = ai[t];
+                const double ra = ru * static_cast<double>(av[t]);
+                for (int32_t q0 = pp[j]; q0 < pp[j + 1]; q0 += WAVE) {
+                    const int32_t q = q0 + lane;
+                    if (q < pp[j + 1]) {
+                        const int32_t J = pi[q];
+                        int a = 0, b = L;
+                        while (a < b) {
+                            const int mid = (a + b) >> 1;
+                            if (cols[mid] < J) a = mid + 1;
+                            else b = mid;
+                        }
+                        if (a < L && cols[a] == J) acc[a] += ra * pv[q];
+                    }
+                    __syncthreads();
+                }
+            }
         }
         const bool empty = (r0 == r1);
         for (int t = lane; t < L; t += WAVE) cval[c0 + t] = static_cast<T>(empty ? 1.0 : acc[t]);
@@ -1516,11 +1596,24 @@ extern "C" int fa_rap_values(int dtype, int64_t nrows, const int32_t* r_ptr, con
     FA_REQUIRE(dtype == FA_F64 || dtype == FA_F32, "fa_rap_values: bad dtype %d", dtype);
     FA_REQUIRE(max_row > 0, "fa_rap_values: bad max_row");
     const size_t lds = align_up(sizeof(int32_t) * max_row, 16) + sizeof(double) * WAVE * max_row;
-    if (lds > 64 * 1024) {
-        set_error("fa_rap_values: coarse rows of %d entries exceed the LDS capacity", max_row);
-        return FA_ECAPACITY;
-    }
     const dim3 grid(grid_for(nrows, 1, 1 << 22));
+    if (lds > 64 * 1024) {  // wide coarse rows: one accumulator per entry, pairs in sequence
+        const size_t lds1 = align_up(sizeof(int32_t) * max_row, 16) + sizeof(double) * max_row;
+        if (lds1 > 64 * 1024) {
+            set_error("fa_rap_values: coarse rows of %d entries exceed the LDS capacity", max_row);
+            return FA_ECAPACITY;
+        }
+        if (dtype == FA_F64)
+            hipLaunchKernelGGL(k_rap_values_seq<double>, grid, dim3(WAVE), lds1, S(stream), nrows, r_ptr, r_idx,
+                               r_val, a_ptr, a_idx, static_cast<const double*>(a_val), p_ptr, p_idx, p_val, c_ptr,
+                               c_idx, max_row, static_cast<double*>(c_val));
+        else
+            hipLaunchKernelGGL(k_rap_values_seq<float>, grid, dim3(WAVE), lds1, S(stream), nrows, r_ptr, r_idx,
+                               r_val, a_ptr, a_idx, static_cast<const float*>(a_val), p_ptr, p_idx, p_val, c_ptr,
+                               c_idx, max_row, static_cast<float*>(c_val));
+        FA_LAUNCH("k_rap_values_seq");
+        return FA_OK;
+    }
     if (dtype == FA_F64)
         hipLaunchKernelGGL(k_rap_values<double>, grid, dim3(WAVE), lds, S(stream), nrows, r_ptr, r_idx, r_val, a_ptr,
                            a_idx, static_cast<const double*>(a_val), p_ptr, p_idx, p_val, c_ptr, c_idx, max_row,

@@ -236,7 +236,7 @@ extern "C" int fa_spmv(int dtype, int64_t n, int64_t nnz, const int32_t* indptr,
                        const void* values, const void* x, void* y, fa_stream_t stream) {
     FA_REQUIRE(dtype == FA_F64 || dtype == FA_F32, "fa_spmv: bad dtype %d", dtype);
     if (n == 0) return FA_OK;
-    const int G = lanes_for(n, nnz);
+    const int G = tuned(FA_TUNE_SPMV_LANES, fa_tune_key(FA_TUNE_SPMV_LANES, n, nnz), lanes_for(n, nnz));
     if (dtype == FA_F64)
         return spmv_launch<double>(n, indptr, indices, static_cast<const double*>(values),
                                    static_cast<const double*>(x), static_cast<double*>(y), G, S(stream));

@@ -384,7 +384,8 @@ class TriangleMesh(SimplexMesh):
             node, cell = _apply_threshold(node, cell, threshold)
         mesh = cls(node, cell)
         mesh.box = list(box)
-        mesh.meshdata['from_box'] = ('tri', tuple(box), nx, ny)
+        if threshold is None:           # the full structured grid (geometric multigrid reads it)
+            mesh.meshdata['from_box'] = ('tri', tuple(box), nx, ny)
         return mesh
 
     def grad_lambda(self, index=_S, TD=2):
@@ -477,8 +478,8 @@ class TetrahedronMesh(SimplexMesh):
             node, cell = _apply_threshold(node, cell, threshold)
         mesh = cls(node, cell)
         mesh.box = list(box)
-        mesh.meshdata['from_box'] = ('tet', tuple(box), nx, ny, nz)
-        if threshold is None:
+        if threshold is None:           # the full structured grid (geometric multigrid reads it)
+            mesh.meshdata['from_box'] = ('tet', tuple(box), nx, ny, nz)
             mesh._tag_box_faces()
         return mesh
 

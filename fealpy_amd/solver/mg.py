@@ -51,7 +51,8 @@ class FaMg(ctypes.Structure):
 def _grid_shape(mesh):
     meta = mesh.meshdata.get('from_box')
     if meta is None:
-        raise ValueError("GeometricMG needs a mesh made by from_box (nested halving is closed form only there)")
+        raise ValueError("GeometricMG needs a full from_box mesh (nested halving is closed form only there); "
+                         "AlgebraicMG builds its hierarchy from the matrix of any mesh")
     return list(meta[2:])
 
 
@@ -200,6 +201,29 @@ def _max_row(M):
 RAP_CANDIDATES = 8192
 
 
+def _row_sums(M, w):
+    """sum over each row of M of w[col] (int64); a column outside w counts 0
+    (the unit placeholder the symbolic kernel gives a row without
+    candidates, whose column is the row index, which R never reaches)."""
+    col = M.col.long()
+    inside = col < w.shape[0]
+    v = torch.where(inside, w[torch.clamp(col, max=max(w.shape[0] - 1, 0))], torch.zeros_like(col))
+    cs = torch.zeros(v.shape[0] + 1, dtype=torch.int64, device=v.device)
+    cs[1:] = torch.cumsum(v, 0)
+    crow = M.crow.long()
+    return cs[crow[1:]] - cs[crow[:-1]]
+
+
+def _candidates(R, A, P):
+    """Largest number of (R entry, A entry, P entry) triples of one row of
+    R A P: the exact length of the symbolic kernel's candidate list (at
+    least 1)."""
+    plen = (P.crow[1:] - P.crow[:-1]).long()
+    per_a = _row_sums(A, plen)
+    per_r = _row_sums(R, per_a)
+    return max(int(per_r.max().item()) if per_r.numel() else 1, 1)
+
+
 class _Level:
     pass
 
@@ -260,17 +284,20 @@ class GeometricMG:
             nxt.dims, nxt.n, nxt.bd = dims_c, k * nc_s, bd_c
             self.levels.append(nxt)
             lev, dims, bd = nxt, dims_c, bd_c
-        # the coarsest level is solved exactly (dense inverse) up to 2048 rows;
-        # above that (a coarse_max above 2048) it is smoothed from zero with a
-        # Chebyshev polynomial of degree coarse_degree
-        self.coarse_dense = self.levels[-1].n <= 2048
         self.coarse_degree = coarse_degree
-        for lv in self.levels:
-            z = lambda: torch.zeros(lv.n, dtype=dtype, device=dev)
-            lv.b, lv.x, lv.x2, lv.d, lv.r, lv.dinv = z(), z(), z(), z(), z(), z()
+        self._finish_levels()
         self._pattern_key = None
         self._desc = None
         self._setup_A = None
+
+    def _finish_levels(self):
+        """Work vectors of every level; the coarsest level is solved exactly
+        (dense inverse) up to 2048 rows, above that (a coarse_max above 2048)
+        smoothed from zero with a Chebyshev polynomial of degree coarse_degree."""
+        self.coarse_dense = self.levels[-1].n <= 2048
+        for lv in self.levels:
+            z = lambda: torch.zeros(lv.n, dtype=self.dtype, device=self.device)
+            lv.b, lv.x, lv.x2, lv.d, lv.r, lv.dinv = z(), z(), z(), z(), z(), z()
 
     def _link(self, lev, P):
         lev.P = P
@@ -289,25 +316,51 @@ class GeometricMG:
         for lv in self.levels:
             lv.compressed = None        # decided by the next setup (band check of the new pattern)
         for l in range(len(self.levels) - 1):
-            f, c = self.levels[l], self.levels[l + 1]
-            mr, ma, mp = _max_row(f.R), _max_row(f.A), max(_max_row(f.P), 1)
-            # stages name the level's own operator None: the numeric setup reads
-            # the current f.A (a new matrix on the same pattern keeps the stages)
-            if mr * ma * mp <= RAP_CANDIDATES:
-                c.A, c.max_row = self._rap_pattern(f.R, f.A, f.P, c.n, mr * ma * mp)
-                f.stages = [(f.R, None, f.P, c.A, c.max_row, self._rap_terms(f.R, f.A, f.P, c.A, mr * ma * mp))]
-            else:
-                # R (A P) in two products: each row's candidate list stays short
-                # (P3 -> P1 in 3D: 975 against 34,125 for R A P at once)
-                i_f, i_c = _identity(f.n, self.device), _identity(c.n, self.device)
-                ap, ap_max = self._rap_pattern(i_f, f.A, f.P, c.n, ma * mp)
-                c.A, c.max_row = self._rap_pattern(f.R, ap, i_c, c.n, mr * _max_row(ap))
-                f.stages = [(i_f, None, f.P, ap, ap_max, self._rap_terms(i_f, f.A, f.P, ap, ma * mp)),
-                            (f.R, ap, i_c, c.A, c.max_row, self._rap_terms(f.R, ap, i_c, c.A, mr * _max_row(ap)))]
-            c.As = SellMatrix(c.A.crow, c.A.col, c.A.shape)
+            self._symbolic_pair(l)
         last = self.levels[-1]
         last.dense = (torch.zeros((last.n, last.n), dtype=torch.float64, device=self.device) if self.coarse_dense
                       else None)
+
+    def _symbolic_pair(self, l):
+        """Pattern of the Galerkin product of levels l -> l+1 and its stages."""
+        f, c = self.levels[l], self.levels[l + 1]
+        cand = _candidates(f.R, f.A, f.P)
+        # stages name the level's own operator None: the numeric setup reads
+        # the current f.A (a new matrix on the same pattern keeps the stages)
+        if cand <= RAP_CANDIDATES:
+            c.A, c.max_row = self._rap_pattern(f.R, f.A, f.P, c.n, cand)
+            f.stages = [(f.R, None, f.P, c.A, c.max_row, self._rap_terms(f.R, f.A, f.P, c.A, cand))]
+        else:
+            # R (A P) in two products: each row's candidate list stays short
+            # (P3 -> P1 in 3D: 975 against 34,125 for R A P at once)
+            i_f, i_c = _identity(f.n, self.device), _identity(c.n, self.device)
+            c1 = _candidates(i_f, f.A, f.P)
+            ap, ap_max = self._rap_pattern(i_f, f.A, f.P, c.n, c1)
+            c2 = _candidates(f.R, ap, i_c)
+            c.A, c.max_row = self._rap_pattern(f.R, ap, i_c, c.n, c2)
+            f.stages = [(i_f, None, f.P, ap, ap_max, self._rap_terms(i_f, f.A, f.P, ap, c1)),
+                        (f.R, ap, i_c, c.A, c.max_row, self._rap_terms(f.R, ap, i_c, c.A, c2))]
+        c.As = SellMatrix(c.A.crow, c.A.col, c.A.shape)
+
+    def _numeric_pair(self, l, vd):
+        """Values of the Galerkin product of levels l -> l+1 (in type vd)."""
+        L = nat.lib()
+        vcode = nat.dtype_code(vd)
+        for R, Af, P, C, cmax, terms in self.levels[l].stages:
+            Af = self.levels[l].A if Af is None else Af
+            if C.values.dtype != vd:            # value arrays follow the operator's type
+                C._values = torch.zeros(C.nnz, dtype=vd, device=self.device)
+            if terms is not None:
+                tslice, tw, wset, nw = terms
+                nat.check(L.fa_rap_values_terms(vcode, C.nnz, nat.ptr(tslice), nat.ptr(tw), wset, nw,
+                                                nat.ptr(Af.values), nat.ptr(C.values), nat.stream()),
+                          "fa_rap_values_terms")
+            else:
+                nat.check(L.fa_rap_values(vcode, C.shape[0], nat.ptr(R.crow), nat.ptr(R.col), nat.ptr(R.values),
+                                          nat.ptr(Af.crow), nat.ptr(Af.col), nat.ptr(Af.values),
+                                          nat.ptr(P.crow), nat.ptr(P.col), nat.ptr(P.values), nat.ptr(C.crow),
+                                          nat.ptr(C.col), cmax, nat.ptr(C.values), nat.stream()),
+                          "fa_rap_values")
 
     def _rap_pattern(self, R, A, P, ncols, cand):
         """Symbolic product R A P (fa_rap_count / fa_rap_fill) -> (CSR with
@@ -316,9 +369,10 @@ class GeometricMG:
         only happens for Dirichlet rows, which R never reads."""
         L = nat.lib()
         nrows = R.shape[0]
-        cand = max(int(cand), 1)
-        if cand > RAP_CANDIDATES:
-            raise NotImplementedError(f"Galerkin product with {cand} candidates per row (at most {RAP_CANDIDATES})")
+        # the kernel compacts a full candidate list to its distinct columns,
+        # so more candidates than its capacity still fit when the distinct
+        # columns (plus one batch) do; it reports FA_ECAPACITY otherwise
+        cand = min(max(int(cand), 1), RAP_CANDIDATES)
         cptr = torch.empty(nrows + 1, dtype=torch.int32, device=self.device)
         ws = nat.workspace(L.fa_rap_work_bytes(nrows), self.device)
         nnz, mrow = ctypes.c_int64(0), ctypes.c_int32(0)
@@ -338,9 +392,10 @@ class GeometricMG:
         transfers are empty or take more than 3 distinct values, or A is too
         large; the search kernel (fa_rap_values) then forms the product."""
         L = nat.lib()
-        vals = torch.unique(torch.cat([R.values, P.values])).cpu().tolist()
-        if not vals or len(vals) > 3 or A.nnz >= (1 << 28) or C.nnz == 0:
+        uniq = torch.unique(torch.cat([R.values, P.values]))
+        if uniq.numel() == 0 or uniq.numel() > 3 or A.nnz >= (1 << 28) or C.nnz == 0:
             return None
+        vals = uniq.cpu().tolist()
         wset = (ctypes.c_double * len(vals))(*vals)
         ne = C.nnz
         tslice = torch.empty((ne + 63) // 64 + 1, dtype=torch.int32, device=self.device)
@@ -372,21 +427,7 @@ class GeometricMG:
         vcode = nat.dtype_code(vd)
         self.levels[0].A = A
         for l in range(len(self.levels) - 1):
-            for R, Af, P, C, cmax, terms in self.levels[l].stages:
-                Af = self.levels[l].A if Af is None else Af
-                if C.values.dtype != vd:            # value arrays follow the operator's type
-                    C._values = torch.zeros(C.nnz, dtype=vd, device=self.device)
-                if terms is not None:
-                    tslice, tw, wset, nw = terms
-                    nat.check(L.fa_rap_values_terms(vcode, C.nnz, nat.ptr(tslice), nat.ptr(tw), wset, nw,
-                                                    nat.ptr(Af.values), nat.ptr(C.values), nat.stream()),
-                              "fa_rap_values_terms")
-                else:
-                    nat.check(L.fa_rap_values(vcode, C.shape[0], nat.ptr(R.crow), nat.ptr(R.col), nat.ptr(R.values),
-                                              nat.ptr(Af.crow), nat.ptr(Af.col), nat.ptr(Af.values),
-                                              nat.ptr(P.crow), nat.ptr(P.col), nat.ptr(P.values), nat.ptr(C.crow),
-                                              nat.ptr(C.col), cmax, nat.ptr(C.values), nat.stream()),
-                              "fa_rap_values")
+            self._numeric_pair(l, vd)
         code = nat.dtype_code(self.dtype)
         ws = nat.workspace(L.fa_reduce_work_bytes(), self.device)
         bounds = torch.empty(len(self.levels), dtype=torch.float64, device=self.device)

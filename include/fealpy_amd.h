@@ -467,6 +467,27 @@ int fa_rap_terms_fill(int64_t nrows, const int32_t* r_ptr, const int32_t* r_idx,
 int fa_rap_values_terms(int dtype, int64_t nnz_c, const int32_t* tslice, const uint32_t* terms,
                         const double* values, int nvalues, const void* a_val, void* c_val, fa_stream_t stream);
 
+/* ------------------------------------------------ algebraic multigrid --
+ * Smoothed-aggregation transfers (the algebraic branch of GAMGSolver,
+ * fealpy/solver/gamg_solver.py:104-121; fealpy/solver/amg.py).
+ * fa_amg_aggregate: aggregates of the strength graph of A (CSR, values of
+ * dtype; j strong for i when |a_ij| >= theta sqrt(|a_ii a_jj|)) grown from a
+ * distance-2 maximal independent set computed in parallel rounds (keys: a
+ * fixed hash of the row index, the index as tie-break); agg[i] = aggregate of
+ * row i, -1 for rows with no strong off-diagonal entry.  Returns the number
+ * of aggregates and of rounds; syncs once per round.  Deterministic.
+ * fa_amg_smooth_prolongation: values of P = (I - omega D^-1 A) P0 on the
+ * pattern (s_indptr, s_indices) of A P0 with values s_values (P0: one unit
+ * entry per aggregated row): p_ij = [agg(i) == j] - omega / a_ii (A P0)_ij. */
+size_t fa_amg_aggregate_work_bytes(int64_t n, int64_t nnz);
+int fa_amg_aggregate(int dtype, int64_t n, const int32_t* indptr, const int32_t* indices, const void* values,
+                     double theta, int32_t* agg, int64_t* naggregates_host, int32_t* rounds_host, void* work,
+                     size_t work_bytes, fa_stream_t stream);
+int fa_amg_smooth_prolongation(int dtype, int64_t n, const int32_t* a_indptr, const int32_t* a_indices,
+                               const void* a_values, const int32_t* agg, const int32_t* s_indptr,
+                               const int32_t* s_indices, const double* s_values, double omega, double* p_values,
+                               fa_stream_t stream);
+
 /* ------------------------------------------------------- multigrid cycle --
  * Geometric multigrid V-cycle used as a CG preconditioner: Chebyshev
  * polynomial smoothing of D^-1 A (pre and post, same polynomial, so the cycle
@@ -598,6 +619,22 @@ void fa_pcg_graph_destroy(fa_pcg_graph_t* graph);
 int fa_pcg_graph(int dtype, const fa_sell_t* A, const fa_mg_t* M, const void* b, void* x, double atol, double rtol,
                  int64_t maxit, int32_t check_every, int64_t* niter_host, double* residual_host,
                  void* work, size_t work_bytes, fa_pcg_graph_t* graph, fa_stream_t stream);
+
+/* ----------------------------------------------------------- autotuner --
+ * Tile shapes chosen by the project autotuner (fealpy_amd/tune.py), which
+ * times the candidates of a kernel family on the caller's own operator and
+ * stores the fastest here; no counterpart in the reference.  Families:
+ *   FA_TUNE_SPMV_LANES      lanes per CSR row of fa_spmv (4..64), keyed by the
+ *                           average row length class (fa_tune_key(f, n, nnz));
+ *   FA_TUNE_SUM_ROWS_LANES  lanes per CSR row of P1 assembly pass 2
+ *                           (fa_sum_incidence_rows: 4, 8, 16 or 32), keyed by
+ *                           (td, longest row) (fa_tune_key(f, td, max_row)).
+ * A value of 0 restores the kernel's built-in choice.  Process-wide; the
+ * entries are read at each launch. */
+enum { FA_TUNE_SPMV_LANES = 0, FA_TUNE_SUM_ROWS_LANES = 1, FA_TUNE_FAMILIES = 2, FA_TUNE_KEYS = 8 };
+int fa_tune_set(int family, int key, int value);
+int fa_tune_get(int family, int key);
+int fa_tune_key(int family, int64_t a, int64_t b);
 
 /* ------------------------------------------------------ in-step timing --
  * Measurement only; no counterpart in the reference.  fa_timer_arm(mask)
