@@ -103,6 +103,11 @@ _SIGNATURES = {
     "fa_sum_incidence_values": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p]),
     "fa_refine_update": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p]),
     "fa_tune_set": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "fa_pack_sell": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _p, _p, _i64, _i64, _p, _p, _p, _p]),
+    "fa_copy_segments": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _p, _p, _p]),
+    "fa_pcg_batched_work_bytes": (_sz, [_i64, ctypes.c_int]),
+    "fa_pcg_batched": (ctypes.c_int, [_p, _p, ctypes.c_int, _p, _p, _p, _p, _f64, _f64, _i64, _i32, _p, _p, _p, _p,
+                                      _sz, _p]),
     "fa_amg_aggregate_work_bytes": (_sz, [_i64, _i64]),
     "fa_amg_aggregate": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _f64, _p, _pi64, _pi32, _p, _sz, _p]),
     "fa_amg_smooth_prolongation": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p, _p, _p, _p, _f64, _p, _p]),

@@ -16,6 +16,9 @@ void set_error(const char* fmt, ...);
 // autotuned tile of a kernel family for a shape class (fa_tune_set), or fallback
 int tuned(int family, int key, int fallback);
 
+// one V-cycle z = M r on stream s; kernels return at once while *done != 0
+int mg_vcycle(const fa_mg_t* mg, const void* r, void* z, const int* done, hipStream_t s);
+
 // In-step kernel timer (fa_timer_arm / fa_timer_read): when the kernel family
 // `kind` (FA_TIMER_*) is armed, timer_begin / timer_end record HIP events on
 // `s` around its launch; otherwise they do nothing.

@@ -1216,6 +1216,11 @@ static int vcycle_any(const fa_mg_t* mg, const void* r, void* z, const int* done
     return vcycle_t<float>(mg, r, z, done, s, dot);
 }
 
+// one V-cycle for other translation units (the batched PCG)
+int mg_vcycle(const fa_mg_t* mg, const void* r, void* z, const int* done, hipStream_t s) {
+    return vcycle_any(mg, r, z, done, s);
+}
+
 // ---------------------------------------------------------------------------
 // PCG (outer dtype T, cycle dtype U)
 struct PcgState {

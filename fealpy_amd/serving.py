@@ -22,6 +22,7 @@ default, read when HIP initialises); streams beyond that share queues and
 their kernels serialise, so a process serving with ``slots`` streams should
 set GPU_MAX_HW_QUEUES to at least slots + 2 before its first GPU call.
 """
+import ctypes
 import math
 import queue
 import threading
@@ -30,12 +31,16 @@ import time
 import numpy as np
 import torch
 
+from . import _native as nat
 from .fem import BilinearForm, DirichletBC, LinearForm, ScalarDiffusionIntegrator, ScalarSourceIntegrator
 from . import expr as ex
 from .expr import sin
 from .functionspace import LagrangeFESpace
 from .mesh import TriangleMesh
 from .solver import GeometricMG, cg
+from .solver.cg import maxit_arg
+from .solver.mg import FaMg, FaMgLevel
+from .sparse.sell import FaSell, FaSell16
 
 PI = math.pi
 
@@ -93,24 +98,33 @@ class _SizeContext:
         torch.cuda.current_stream(self.mesh.node.device).synchronize()   # resident data complete before sharing
         self.mg_kwargs = mg_kwargs
         self._pool = []
+        self._pool_packed = []
         self._lock = threading.Lock()
 
-    def checkout(self):
+    def checkout(self, packed=False):
+        """A hierarchy from the pool (packed batches need uncompressed ones)."""
+        pool = self._pool_packed if packed else self._pool
         with self._lock:
-            if self._pool:
-                return self._pool.pop()
-        return GeometricMG(self.space, self.is_bd, **self.mg_kwargs)
+            if pool:
+                return pool.pop()
+        kw = dict(self.mg_kwargs, compress=False) if packed else self.mg_kwargs
+        mg = GeometricMG(self.space, self.is_bd, **kw)
+        mg._packed = packed
+        return mg
 
     def checkin(self, mg):
         with self._lock:
-            self._pool.append(mg)
+            (self._pool_packed if getattr(mg, '_packed', False) else self._pool).append(mg)
 
 
 class SolveServer:
     def __init__(self, *, slots=4, hbm_budget=64 << 30, rtol=1e-8, maxit=2000, q=3, mg_dtype=torch.float32,
-                 graphs=False, device=None):
+                 graphs=False, device=None, batch=1):
         self.device = torch.device("cuda") if device is None else torch.device(device)
         self.slots, self.budget, self.rtol, self.maxit, self.q = slots, int(hbm_budget), rtol, maxit, q
+        # batch > 1: a slot takes up to `batch` waiting requests at once and
+        # solves those of equal hierarchy depth in one packed PCG
+        self.batch = max(1, min(int(batch), MAX_BATCH))
         self.mg_kwargs = {"dtype": mg_dtype, "graphs": graphs}
         self._ctx = {}
         self._ctx_lock = threading.Lock()
@@ -138,23 +152,27 @@ class SolveServer:
                     self._ctx[key] = ctx
         return ctx
 
-    def warm(self, sizes):
+    def warm(self, sizes, packed=False):
         """Build the resident data (and one hierarchy with its symbolic
-        products) of every (nx, p) in `sizes`; returns seconds spent."""
+        products; ``packed``: one for packed batches) of every (nx, p) in
+        `sizes`; returns seconds spent."""
         t0 = time.perf_counter()
         for nx, p in sizes:
             ctx = self.context(nx, p)
-            mg = ctx.checkout()
+            mg = ctx.checkout(packed=packed)
             nc = ctx.mesh.number_of_cells()
             r = SolveRequest(-1, nx, p, torch.ones(nc, dtype=torch.float64, device=self.device),
                              [[1.0, 1.0, 1.0]])
-            self._run(r, ctx, mg)
+            if packed:
+                self._system(r, ctx, mg)
+            else:
+                self._run(r, ctx, mg)
             ctx.checkin(mg)
         torch.cuda.synchronize()
         return time.perf_counter() - t0
 
     # -- one request ---------------------------------------------------------------
-    def _run(self, req, ctx, mg):
+    def _system(self, req, ctx, mg):
         space = ctx.space
         k = req.k.to(self.device, torch.float64, non_blocking=True)
         f = fourier_source(req.modes, self.device)
@@ -162,8 +180,47 @@ class SolveServer:
         F = LinearForm(space).add_integrator(ScalarSourceIntegrator(f, q=self.q)).assembly()
         A, F = DirichletBC(space, gd=0.0).apply(A, F)
         mg.setup(A)
+        return A, F
+
+    def _run(self, req, ctx, mg):
+        A, F = self._system(req, ctx, mg)
         x, info = cg(A, F, M=mg, atol=0.0, rtol=self.rtol, maxit=self.maxit, returninfo=True)
         return x, info
+
+    def solve_batch(self, reqs):
+        """Solve requests together on the calling thread's current stream:
+        each one assembled, constrained and set up on its own, then those of
+        equal hierarchy depth in one packed PCG (fa_pcg_batched) per group of
+        at most MAX_BATCH.  Fills every req.x / req.info."""
+        t = time.perf_counter()
+        for r in reqs:
+            r.start = t
+        work = []
+        try:
+            for r in reqs:
+                ctx = self.context(r.nx, r.p)
+                mg = ctx.checkout(packed=True)
+                A, F = self._system(r, ctx, mg)
+                work.append((r, ctx, mg, F))
+            groups = {}
+            for w in work:
+                groups.setdefault(len(w[2].levels), []).append(w)
+            for items in groups.values():
+                for i0 in range(0, len(items), MAX_BATCH):
+                    part = items[i0:i0 + MAX_BATCH]
+                    pb = PackedBatch([w[2] for w in part], self.device)
+                    xs = [torch.zeros(w[3].shape[0], dtype=torch.float64, device=self.device) for w in part]
+                    out = pb.pcg([w[3] for w in part], xs, rtol=self.rtol, maxit=self.maxit)
+                    for w, x, (it, res, st) in zip(part, xs, out):
+                        w[0].x = x
+                        w[0].info = {'niter': it, 'residual': res, 'status': st, 'batch': len(part)}
+        finally:
+            for _, ctx, mg, _ in work:
+                ctx.checkin(mg)
+        t = time.perf_counter()
+        for r in reqs:
+            r.done = t
+        return reqs
 
     def solve(self, req):
         """Solve on the calling thread's current stream; fills req.x / req.info."""
@@ -194,11 +251,14 @@ class SolveServer:
         self._queue.put(req)
         return req
 
-    def _admit(self, req):
+    def _admit(self, reqs):
+        """Admit a slot's requests together once their HBM estimate fits the
+        budget beside what is in flight (or nothing else is in flight)."""
+        need = sum(r.nbytes for r in reqs)
         with self._budget_cv:
-            while self._in_flight > 0 and self._in_flight + req.nbytes > self.budget:
+            while self._in_flight > 0 and self._in_flight + need > self.budget:
                 self._budget_cv.wait()
-            self._in_flight += req.nbytes
+            self._in_flight += need
 
     def _release(self, req):
         with self._budget_cv:
@@ -212,16 +272,34 @@ class SolveServer:
                 req = self._queue.get()
                 if req is None:
                     break
-                self._admit(req)
+                reqs = [req]
+                stop = False
+                while len(reqs) < self.batch:   # the requests waiting now join this batch
+                    try:
+                        nxt = self._queue.get_nowait()
+                    except queue.Empty:
+                        break
+                    if nxt is None:
+                        stop = True
+                        break
+                    reqs.append(nxt)
+                self._admit(reqs)
                 try:
-                    self.solve(req)
-                except Exception as e:          # reported on the request, the slot keeps serving
-                    req.error = e
-                    req.done = time.perf_counter()
+                    if len(reqs) == 1 and self.batch == 1:
+                        self.solve(req)
+                    else:
+                        self.solve_batch(reqs)
+                except Exception as e:          # reported on the requests, the slot keeps serving
+                    for r in reqs:
+                        r.error = e
+                        r.done = time.perf_counter()
                 finally:
-                    stream.synchronize()        # the result is complete before the request is released
-                    self._release(req)
-                    req.event.set()
+                    stream.synchronize()        # the results are complete before the requests are released
+                    for r in reqs:
+                        self._release(r)
+                        r.event.set()
+                if stop:
+                    break
 
     def close(self):
         self._closed = True
@@ -252,3 +330,187 @@ def make_requests(n, *, seed=20261202, nx_range=(16, 512), p_choices=(1, 2), mod
             t += arr_rng.exponential(1.0 / rate)
         out.append(SolveRequest(i, nx, p, k, m.astype(np.float64), arrival=t))
     return out
+
+
+# ---- packed batches: one PCG for many requests ------------------------------------
+class FaPackSell(ctypes.Structure):
+    _fields_ = [("sp", ctypes.c_void_p), ("cols", ctypes.c_void_p), ("vals", ctypes.c_void_p),
+                ("nslices", ctypes.c_int64), ("extra_slices", ctypes.c_int64), ("slice_base", ctypes.c_int64),
+                ("entry_base", ctypes.c_int64), ("entries", ctypes.c_int64), ("col_off", ctypes.c_int32)]
+
+
+class FaCopySeg(ctypes.Structure):
+    _fields_ = [("src", ctypes.c_void_p), ("dst", ctypes.c_void_p), ("len", ctypes.c_int64),
+                ("zero_tail", ctypes.c_int64)]
+
+
+def _upload(items, device):
+    """ctypes array -> (the host array, a device copy of its bytes)."""
+    raw = bytes(memoryview(items).cast("B"))
+    dev = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(device)
+    return items, dev
+
+
+# finest-level rows per PCG workgroup of fa_pcg_batched; coarse levels align to SELL slices
+_SEG0, _SEG = 256, 64
+# requests per packed batch: the packed coarsest level (64 rows a request)
+# keeps its dense inverse within 2048 rows
+MAX_BATCH = 32
+
+
+class PackedBatch:
+    """The hierarchies of B requests (GeometricMG set up, the same number of
+    levels, uncompressed FP32 cycles) packed at offsets into one
+    block-diagonal hierarchy, plus the packed FP64 finest operators: the
+    operand of fa_pcg_batched.  Request q holds rows [off[l][q], off[l][q] +
+    n[l][q]) of level l; the other rows are empty.  The Chebyshev bound of a
+    packed level is the largest of its requests' (an upper bound for each, so
+    every request's smoother stays convergent)."""
+
+    def __init__(self, mgs, device):
+        L = nat.lib()
+        self.device = device
+        self.nreq = len(mgs)
+        nlev = len(mgs[0].levels)
+        if any(len(m.levels) != nlev for m in mgs):
+            raise ValueError("a packed batch needs hierarchies with the same number of levels")
+        if any(getattr(lv, 'compressed', False) for m in mgs for lv in m.levels):
+            raise ValueError("packed batches take uncompressed hierarchies (GeometricMG(compress=False))")
+        if self.nreq > MAX_BATCH:
+            raise ValueError(f"at most {MAX_BATCH} requests per packed batch")
+        self.n, self.off, self.N = [], [], []
+        for l in range(nlev):
+            seg = _SEG0 if l == 0 else _SEG
+            ns = [m.levels[l].n for m in mgs]
+            pads = [(n + seg - 1) // seg * seg for n in ns]
+            offs = np.concatenate([[0], np.cumsum(pads)[:-1]]).astype(np.int64).tolist()
+            self.n.append(ns)
+            self.off.append(offs)
+            self.N.append(int(sum(pads)))
+        if self.N[-1] > 2048 or any(m.levels[-1].dense is None for m in mgs):
+            raise ValueError("the packed coarsest level needs every request's dense inverse and at most 2048 rows")
+        self._keep = []
+        self.levels = []
+        f32 = torch.float32
+        for l in range(nlev):
+            lv = {}
+            lv["A"] = self._pack([m.levels[l].As for m in mgs], f32, l, l)
+            if l < nlev - 1:
+                lv["P"] = self._pack([m.levels[l].Ps for m in mgs], f32, l, l + 1)
+                lv["R"] = self._pack([m.levels[l].Rs for m in mgs], f32, l + 1, l)
+            lv["dinv"] = self._pack_vec([m.levels[l].dinv for m in mgs], l, f32)
+            lv["lmax"] = max(float(m.levels[l].lmax) for m in mgs)
+            lv["width"] = max(int(getattr(m.levels[l], 'max_row', 0)) for m in mgs)
+            lv["work"] = [torch.zeros(self.N[l], dtype=f32, device=device) for _ in range(5)]
+            self.levels.append(lv)
+        self.A64 = self._pack([m.levels[0].As for m in mgs], torch.float64, 0, 0)
+        # coarsest level: the requests' dense inverses on the diagonal blocks
+        Nc = self.N[-1]
+        self.dense = torch.zeros((Nc, Nc), dtype=torch.float64, device=device)
+        segs = (FaCopySeg * sum(self.n[-1]))()
+        k = 0
+        for q, m in enumerate(mgs):
+            d, nc, o = m.levels[-1].dense, self.n[-1][q], self.off[-1][q]
+            for i in range(nc):
+                segs[k] = FaCopySeg(d.data_ptr() + 8 * i * nc, self.dense.data_ptr() + 8 * ((o + i) * Nc + o), nc, 0)
+                k += 1
+        host, dev = _upload(segs, device)
+        nat.check(L.fa_copy_segments(8, k, host, ctypes.c_void_p(dev.data_ptr()), nat.stream()), "fa_copy_segments")
+        self._keep.append(dev)
+        # the finest level's segments of 256 rows -> request
+        nb = [((n + _SEG0 - 1) // _SEG0) for n in self.n[0]]
+        self.block_req = torch.as_tensor(np.repeat(np.arange(self.nreq, dtype=np.int32), nb), device=device)
+        self.block_off = torch.as_tensor(np.concatenate([[0], np.cumsum(nb)]).astype(np.int32), device=device)
+        self._build_desc(mgs[0])
+
+    def _pack(self, sells, dtype, lrow, lcol):
+        """Pack SellMatrix objects (rows on level lrow, columns on level lcol)."""
+        L = nat.lib()
+        seg = _SEG0 if lrow == 0 else _SEG
+        items = (FaPackSell * self.nreq)()
+        sbase = ebase = 0
+        for q, S in enumerate(sells):
+            ns = (S.shape[0] + 63) // 64
+            pad_rows = (S.shape[0] + seg - 1) // seg * seg
+            extra = pad_rows // 64 - ns
+            vals = S.vals[dtype]
+            items[q] = FaPackSell(S.slice_ptr.data_ptr(), S.cols.data_ptr(), vals.data_ptr(), ns, extra, sbase, ebase,
+                                  S.entries, self.off[lcol][q])
+            sbase += ns + extra
+            ebase += S.entries
+        sp = torch.empty(sbase + 1, dtype=torch.int32, device=self.device)
+        cols = torch.zeros(max(ebase, 1), dtype=torch.int32, device=self.device)
+        vals = torch.zeros(max(ebase, 1), dtype=dtype, device=self.device)
+        host, dev = _upload(items, self.device)
+        nat.check(L.fa_pack_sell(vals.element_size(), self.nreq, host, ctypes.c_void_p(dev.data_ptr()), sbase, ebase,
+                                 nat.ptr(sp), nat.ptr(cols), nat.ptr(vals), nat.stream()), "fa_pack_sell")
+        self._keep.append(dev)
+        return FaSell(self.N[lrow], self.N[lcol], sp.data_ptr(), cols.data_ptr(), vals.data_ptr()), (sp, cols, vals)
+
+    def _pack_vec(self, vecs, l, dtype):
+        out = torch.empty(self.N[l], dtype=dtype, device=self.device)
+        self.copy_in(vecs, l, out)
+        return out
+
+    def copy_in(self, vecs, l, out):
+        """Vectors of the requests (level l) into the packed vector `out` (zeros between)."""
+        L = nat.lib()
+        segs = (FaCopySeg * self.nreq)()
+        pads = [self.off[l][q + 1] - self.off[l][q] for q in range(self.nreq - 1)] + \
+            [self.N[l] - self.off[l][-1]]
+        for q, v in enumerate(vecs):
+            segs[q] = FaCopySeg(v.data_ptr(), out.data_ptr() + out.element_size() * self.off[l][q], v.shape[0],
+                                pads[q] - v.shape[0])
+        host, dev = _upload(segs, self.device)
+        nat.check(L.fa_copy_segments(out.element_size(), self.nreq, host, ctypes.c_void_p(dev.data_ptr()),
+                                     nat.stream()), "fa_copy_segments")
+        self._keep.append(dev)
+
+    def copy_out(self, packed, outs, l=0):
+        L = nat.lib()
+        segs = (FaCopySeg * self.nreq)()
+        for q, v in enumerate(outs):
+            segs[q] = FaCopySeg(packed.data_ptr() + packed.element_size() * self.off[l][q], v.data_ptr(), v.shape[0],
+                                0)
+        host, dev = _upload(segs, self.device)
+        nat.check(L.fa_copy_segments(packed.element_size(), self.nreq, host, ctypes.c_void_p(dev.data_ptr()),
+                                     nat.stream()), "fa_copy_segments")
+        self._keep.append(dev)
+
+    def _build_desc(self, mg0):
+        n = len(self.levels)
+        arr = (FaMgLevel * n)()
+        empty = FaSell(0, 0, None, None, None)
+        for i, lv in enumerate(self.levels):
+            e = arr[i]
+            e.A = lv["A"][0]
+            e.dinv = lv["dinv"].data_ptr()
+            e.R = lv["R"][0] if i < n - 1 else empty
+            e.P = lv["P"][0] if i < n - 1 else empty
+            e.lmax = lv["lmax"]
+            e.width = lv["width"]
+            e.b, e.x, e.x2, e.d, e.r = (w.data_ptr() for w in lv["work"])
+            e.Ah = FaSell16(0, None, None, None)
+            e.diag = None
+        self._arr = arr
+        self.desc = FaMg(nat.FA_F32, n, arr, mg0.degree, mg0.lower, mg0.upper, self.dense.data_ptr(),
+                         mg0.coarse_degree)
+
+    def pcg(self, bs, xs, *, atol=0.0, rtol=1e-8, maxit=2000, check_every=8):
+        """Solve every request's A x = b (b: the Dirichlet-applied FP64 right-
+        hand sides, in order) with one batched PCG; fills xs (FP64, zeros in)
+        and returns [(niter, residual, status)] per request."""
+        L = nat.lib()
+        b = torch.empty(self.N[0], dtype=torch.float64, device=self.device)
+        self.copy_in(bs, 0, b)
+        x = torch.zeros(self.N[0], dtype=torch.float64, device=self.device)
+        ws = nat.workspace(L.fa_pcg_batched_work_bytes(self.N[0], self.nreq), self.device)
+        it = (ctypes.c_int64 * self.nreq)()
+        res = (ctypes.c_double * self.nreq)()
+        stat = (ctypes.c_int32 * self.nreq)()
+        nat.check(L.fa_pcg_batched(ctypes.byref(self.A64[0]), ctypes.byref(self.desc), self.nreq,
+                                   nat.ptr(self.block_req), nat.ptr(self.block_off), nat.ptr(b), nat.ptr(x),
+                                   float(atol), float(rtol), maxit_arg(maxit), int(check_every), it, res, stat,
+                                   nat.ptr(ws), ws.numel(), nat.stream()), "fa_pcg_batched")
+        self.copy_out(x, xs, 0)
+        return [(int(it[q]), float(res[q]), int(stat[q])) for q in range(self.nreq)]

@@ -636,6 +636,50 @@ int fa_tune_set(int family, int key, int value);
 int fa_tune_get(int family, int key);
 int fa_tune_key(int family, int64_t a, int64_t b);
 
+/* --------------------------------------------------------- batched solves --
+ * Many independent Dirichlet problems packed at offsets into one block-
+ * diagonal system, so each kernel of a PCG iteration and of its V-cycle runs
+ * once for the whole batch (serving).  The reference's counterpart is the
+ * batch axis of Form and cg (fealpy/fem/form.py:39-52, fealpy/solver/cg.py:
+ * 69-123: columns in lockstep, each with its own alpha and beta).
+ * fa_pack_sell: the SELL-64 matrices of the requests into one (items in host
+ * and device memory, the same contents): request q's slices go to
+ * slice_base.., its entries to entry_base.. with columns shifted by col_off,
+ * followed by extra_slices empty slices; slice_ptr gets nslices_total + 1
+ * entries.  fa_copy_segments: dst[0, len) = src (or 0 when src is NULL),
+ * then zero_tail zeros, for every segment, in one launch.
+ * fa_pcg_batched: PCG on the packed system A (FP64 values) with the FP32
+ * V-cycle M over the packed hierarchy; the rows come in segments of 256
+ * (block_req[k] = request of rows [256k, 256k + 256), block_off[q] = first
+ * segment of request q, nreq + 1 entries, both in device memory); b.b, r.z and
+ * p.Ap are summed per request in a fixed order, alpha, beta and the stop test
+ * (the cg rule: sqrt(r.z) < atol or < rtol ||b_q||) are per request, and a
+ * stopped request's x is frozen at its converging iterate.  Per request:
+ * iterations, final sqrt(r.z) and status (1 converged, 2 ||b|| = 0, 3 maxit,
+ * 4 breakdown).  Syncs every check_every iterations. */
+typedef struct {
+    const int32_t* sp;      /* the request's slice pointers (nslices + 1)          */
+    const int32_t* cols;
+    const void* vals;
+    int64_t nslices, extra_slices, slice_base, entry_base, entries;
+    int32_t col_off;
+} fa_pack_sell_t;
+typedef struct {
+    const void* src;        /* NULL: zeros                                         */
+    void* dst;
+    int64_t len, zero_tail;
+} fa_copy_seg_t;
+int fa_pack_sell(int elem_bytes, int nreq, const fa_pack_sell_t* items_host, const fa_pack_sell_t* items_dev,
+                 int64_t nslices_total, int64_t entries_total, int32_t* slice_ptr, int32_t* cols, void* vals,
+                 fa_stream_t stream);
+int fa_copy_segments(int elem_bytes, int nseg, const fa_copy_seg_t* segs_host, const fa_copy_seg_t* segs_dev,
+                     fa_stream_t stream);
+size_t fa_pcg_batched_work_bytes(int64_t n, int nreq);
+int fa_pcg_batched(const fa_sell_t* A, const fa_mg_t* M, int nreq, const int32_t* block_req,
+                   const int32_t* block_off, const double* b, double* x, double atol, double rtol, int64_t maxit,
+                   int32_t check_every, int64_t* niter_host, double* residual_host, int32_t* status_host,
+                   void* work, size_t work_bytes, fa_stream_t stream);
+
 /* ------------------------------------------------------ in-step timing --
  * Measurement only; no counterpart in the reference.  fa_timer_arm(mask)
  * arms the kernel families whose FA_TIMER_* bits are set (bit 0 must be

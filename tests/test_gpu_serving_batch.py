@@ -1,0 +1,98 @@
+"""Packed batches (fa_pcg_batched): many requests' systems packed into one
+block-diagonal system, one PCG with per-request alpha, beta and stopping.
+
+- every result solves its own system to the requested tolerance (against a
+  tight FP64 CG of the same assembled system), with per-request iteration
+  counts;
+- a request's result is the same alone (a batch of one) and in a mix of
+  other requests, within the solve tolerance (the packed smoother uses the
+  largest Chebyshev bound of the batch, an upper bound for every request);
+- the continuous-batching server with batches equals the one-at-a-time
+  results within the same tolerance, repeated sizes reuse resident data;
+- the packing kernels reproduce each request's operator (SpMV of a packed
+  operator equals the requests' own SpMVs, segment by segment)."""
+import numpy as np
+import pytest
+import torch
+
+import fealpy_amd as fa
+from fealpy_amd.serving import MAX_BATCH, PackedBatch, SolveServer, fourier_source, make_requests
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda")
+
+
+def _tight(srv, r):
+    space = srv.context(r.nx, r.p).space
+    k = r.k.to("cuda")
+    A = fa.BilinearForm(space).add_integrator(fa.ScalarDiffusionIntegrator(k, q=3)).assembly()
+    F = fa.LinearForm(space).add_integrator(fa.ScalarSourceIntegrator(fourier_source(r.modes, "cuda"), q=3)).assembly()
+    A, F = fa.DirichletBC(space, gd=0.0).apply(A, F)
+    return fa.cg(A, F, atol=1e-30, rtol=1e-13, maxit=100000)
+
+
+def _rel(a, b):
+    return float(torch.linalg.norm(a - b)) / max(float(torch.linalg.norm(b)), 1e-300)
+
+
+def test_batch_accuracy_and_alone_equals_mix():
+    srv = SolveServer(slots=1, rtol=1e-10)
+    reqs = make_requests(12, seed=21, nx_range=(16, 96))
+    srv.solve_batch(reqs)
+    for r in reqs:
+        assert r.info["status"] == 1, r.info
+        assert _rel(r.x, _tight(srv, r)) < 1e-8, (r.nx, r.p, r.info)
+    assert max(r.info["batch"] for r in reqs) >= 2           # some share a packed PCG
+    for r in reqs[:4]:
+        alone = make_requests(12, seed=21, nx_range=(16, 96))[r.rid]
+        srv.solve_batch([alone])
+        assert alone.info["batch"] == 1
+        assert _rel(alone.x, r.x) < 1e-8, (r.nx, r.p)
+
+
+def test_packed_operator_equals_request_operators():
+    srv = SolveServer(slots=1, rtol=1e-8)
+    reqs = [r for r in make_requests(10, seed=3, nx_range=(20, 60)) if r.p == 1][:4]
+    mgs, Fs = [], []
+    for r in reqs:
+        ctx = srv.context(r.nx, r.p)
+        mg = ctx.checkout(packed=True)
+        A, F = srv._system(r, ctx, mg)
+        mgs.append(mg)
+        Fs.append(F)
+    depth = len(mgs[0].levels)
+    same = [i for i, m in enumerate(mgs) if len(m.levels) == depth]
+    mgs = [mgs[i] for i in same]
+    pb = PackedBatch(mgs, DEV)
+    xs = [torch.rand(m.levels[0].n, dtype=torch.float64, device=DEV) for m in mgs]
+    xp = torch.empty(pb.N[0], dtype=torch.float64, device=DEV)
+    pb.copy_in(xs, 0, xp)
+    from fealpy_amd.sparse.sell import SellMatrix
+    S = SellMatrix.__new__(SellMatrix)
+    sp, cols, vals = pb.A64[1]
+    S.shape, S.slice_ptr, S.cols, S.entries, S.vals = (pb.N[0], pb.N[0]), sp, cols, int(cols.numel()), \
+        {torch.float64: vals}
+    yp = S.matvec(xp)
+    outs = [torch.empty_like(x) for x in xs]
+    pb.copy_out(yp, outs, 0)
+    for m, x, y in zip(mgs, xs, outs):
+        ref = m.levels[0].As.matvec(x)
+        assert torch.equal(y, ref)
+    for r, m in zip(reqs, mgs):
+        srv.context(r.nx, r.p).checkin(m)
+
+
+def test_server_batches_match_one_at_a_time():
+    reqs = make_requests(40, seed=9, nx_range=(16, 128))
+    srv = SolveServer(slots=2, rtol=1e-10, batch=MAX_BATCH)
+    alone = [srv.solve(r).x.clone() for r in make_requests(40, seed=9, nx_range=(16, 128))]
+    srv.start()
+    for r in reqs:
+        srv.submit(r)
+    for r in reqs:
+        assert r.event.wait(120), "request did not finish"
+        assert r.error is None, r.error
+    srv.close()
+    for a, r in zip(alone, reqs):
+        assert _rel(r.x, a) < 1e-8, (r.nx, r.p)
+    assert max(r.info.get("batch", 1) for r in reqs) >= 2
