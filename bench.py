@@ -72,7 +72,7 @@ def parse():
     ap.add_argument("--torch-source", action="store_true",
                     help="source as a barycentric torch callable instead of a separable expression")
     ap.add_argument("--requests", type=int, default=4096, help="serving: number of requests")
-    ap.add_argument("--serving-batch", type=int, default=32,
+    ap.add_argument("--serving-batch", type=int, default=1,
                     help="serving: most requests a slot packs into one PCG (1: no packed batches)")
     ap.add_argument("--slots", type=int, default=SERVING_SLOTS, help="serving: concurrent request slots (streams)")
     ap.add_argument("--rate", type=float, default=0.0,
@@ -640,6 +640,14 @@ def _pcts(lat):
             "mean_ms": round(float(lat.mean()) * 1e3, 2)}
 
 
+def _wait_all(reqs, label):
+    """Wait for every request in order, with a progress line per 1024 done."""
+    for i, r in enumerate(reqs):
+        r.event.wait()
+        if (i + 1) % 1024 == 0:
+            print(f"# serving {label}: {i + 1}/{len(reqs)} done", file=sys.stderr, flush=True)
+
+
 def run_serving(args):
     """Mixed-size serving: `requests` 2D diffusion requests (TriangleMesh.from_box
     nx^2, nx log-uniform in [16, 512], P1/P2 50:50, k = exp(N(0,1)) per cell,
@@ -653,7 +661,7 @@ def run_serving(args):
     (a FIFO single server: start = max(arrival, previous done)).  Needs
     GPU_MAX_HW_QUEUES >= 2 x slots, set before HIP initialises (main does)."""
     import torch
-    from fealpy_amd.serving import MAX_BATCH, SolveServer, make_requests
+    from fealpy_amd.serving import MAX_BATCH, PACKED_BATCHES_ENABLED, PACKED_BATCHES_OFF, SolveServer, make_requests
     n = args.requests
     seed = 20261202
     srv = SolveServer(slots=args.slots, rtol=args.rtol)
@@ -666,8 +674,10 @@ def run_serving(args):
               flush=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for r in plain:
+    for i, r in enumerate(plain):
         srv.solve(r)
+        if (i + 1) % 1024 == 0:
+            print(f"# serving plain loop {i + 1}/{n} solved", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
     print(f"# serving plain loop {n / t_plain:.1f} solves/s", file=sys.stderr, flush=True)
@@ -676,8 +686,7 @@ def run_serving(args):
     t0 = time.perf_counter()
     for r in mix:
         srv.submit(r)
-    for r in mix:
-        r.event.wait()
+    _wait_all(mix, "server")
     t_srv = time.perf_counter() - t0
     thr = n / t_srv
     print(f"# serving server {thr:.1f} solves/s", file=sys.stderr, flush=True)
@@ -694,14 +703,16 @@ def run_serving(args):
             time.sleep(min(r.arrival - now, 0.002))
         r.arrival += t0
         srv.submit(r)
-    for r in timed_reqs:
-        r.event.wait()
+    _wait_all(timed_reqs, "server, Poisson arrivals")
     srv.close()
     lat_srv = [r.done - r.arrival for r in timed_reqs]
     # the same server with packed batches: a slot takes the requests waiting
     # when it frees up and solves those of equal depth in one PCG
     batched = None
-    if args.serving_batch > 1:
+    if args.serving_batch > 1 and not PACKED_BATCHES_ENABLED:
+        batched = {"skipped": PACKED_BATCHES_OFF}
+    elif args.serving_batch > 1:
+        srv.release_pools(packed=False)
         t_wb = 0.0
         for i in range(0, len(sizes), 64):
             t_wb += srv.warm(sizes[i:i + 64], packed=True)
@@ -712,8 +723,7 @@ def run_serving(args):
         t0 = time.perf_counter()
         for r in mixb:
             srv.submit(r)
-        for r in mixb:
-            r.event.wait()
+        _wait_all(mixb, "server, packed batches")
         t_b = time.perf_counter() - t0
         thr_b = n / t_b
         print(f"# serving server, packed batches {thr_b:.1f} solves/s", file=sys.stderr, flush=True)
@@ -731,8 +741,7 @@ def run_serving(args):
                 time.sleep(min(r.arrival - now, 0.002))
             r.arrival += t0
             srv.submit(r)
-        for r in timed_b:
-            r.event.wait()
+        _wait_all(timed_b, "server, packed batches, Poisson arrivals")
         srv.close()
         lat_b = [r.done - r.arrival for r in timed_b]
         sizes_b = [r.info.get("batch", 1) for r in mixb if r.info]
@@ -767,7 +776,7 @@ def run_serving(args):
         "server": {"solves_per_s": round(thr, 2), "arrival_rate": round(rate, 2), **_pcts(lat_srv)},
         "server_packed_batches": batched,
         "results_identical_alone_and_in_mix": f"{same}/{n}", "request_errors": errors,
-        "cold": {"warm_s": round(t_warm, 2)},
+        "cold": {"warm_s": round(t_warm, 2)}, "resident_evictions": srv.evictions,
         "dof_per_s_server": round(sum(gdofs) / t_srv, 1)}
 
 

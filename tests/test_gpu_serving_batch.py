@@ -1,4 +1,6 @@
-"""Packed batches (fa_pcg_batched): many requests' systems packed into one
+"""Packed batches (fa_pcg_batched), switched off with their tests skipped
+(see PACKED_BATCHES_OFF in fealpy_amd/serving.py), and the resident-size LRU
+budget of the server.  Packed batches: many requests' systems packed into one
 block-diagonal system, one PCG with per-request alpha, beta and stopping.
 
 - every result solves its own system to the requested tolerance (against a
@@ -16,10 +18,14 @@ import pytest
 import torch
 
 import fealpy_amd as fa
-from fealpy_amd.serving import MAX_BATCH, PackedBatch, SolveServer, fourier_source, make_requests
+from fealpy_amd.serving import (MAX_BATCH, PACKED_BATCHES_ENABLED, PACKED_BATCHES_OFF, PackedBatch, SolveServer,
+                                fourier_source, make_requests)
 
 pytestmark = pytest.mark.gpu
 DEV = torch.device("cuda")
+# the packed path is switched off until the cause of its stall on the
+# saturated 4096-request trace is found; its tests stay, skipped
+packed = pytest.mark.skipif(not PACKED_BATCHES_ENABLED, reason=PACKED_BATCHES_OFF)
 
 
 def _tight(srv, r):
@@ -35,6 +41,7 @@ def _rel(a, b):
     return float(torch.linalg.norm(a - b)) / max(float(torch.linalg.norm(b)), 1e-300)
 
 
+@packed
 def test_batch_accuracy_and_alone_equals_mix():
     srv = SolveServer(slots=1, rtol=1e-10)
     reqs = make_requests(12, seed=21, nx_range=(16, 96))
@@ -50,6 +57,7 @@ def test_batch_accuracy_and_alone_equals_mix():
         assert _rel(alone.x, r.x) < 1e-8, (r.nx, r.p)
 
 
+@packed
 def test_packed_operator_equals_request_operators():
     srv = SolveServer(slots=1, rtol=1e-8)
     reqs = [r for r in make_requests(10, seed=3, nx_range=(20, 60)) if r.p == 1][:4]
@@ -82,6 +90,7 @@ def test_packed_operator_equals_request_operators():
         srv.context(r.nx, r.p).checkin(m)
 
 
+@packed
 def test_server_batches_match_one_at_a_time():
     reqs = make_requests(40, seed=9, nx_range=(16, 128))
     srv = SolveServer(slots=2, rtol=1e-10, batch=MAX_BATCH)
@@ -96,3 +105,18 @@ def test_server_batches_match_one_at_a_time():
     for a, r in zip(alone, reqs):
         assert _rel(r.x, a) < 1e-8, (r.nx, r.p)
     assert max(r.info.get("batch", 1) for r in reqs) >= 2
+
+
+def test_resident_budget_evicts_least_recently_used():
+    """A small resident budget drops the least recently used sizes (never one
+    in use); a dropped size is rebuilt on its next request with the same result."""
+    from fealpy_amd.serving import request_bytes
+    reqs = make_requests(8, seed=4, nx_range=(24, 80))
+    budget = 3 * max(request_bytes(r.nx, r.p) for r in reqs)
+    srv = SolveServer(slots=1, rtol=1e-10, resident_budget=budget)
+    first = [srv.solve(r).x.clone() for r in reqs]
+    assert srv.evictions > 0
+    assert srv.resident_bytes() <= budget + 2 * max(request_bytes(r.nx, r.p) for r in reqs)
+    again = [srv.solve(r).x for r in make_requests(8, seed=4, nx_range=(24, 80))]
+    for a, b in zip(first, again):
+        assert torch.equal(a, b)

@@ -237,6 +237,21 @@ def test_serving_trace_is_deterministic_and_in_range():
     assert all(r.arrival == 0.0 for r in z)
 
 
+def test_switched_off_packed_batches_raise():
+    """While packed batches are switched off, asking for them raises before
+    any device work (no silent fall-back to one-at-a-time solves)."""
+    from fealpy_amd import serving
+    if serving.PACKED_BATCHES_ENABLED:
+        pytest.skip("packed batches are switched on")
+    with pytest.raises(ValueError, match="switched off"):
+        serving.SolveServer(slots=1, batch=2)
+    srv = serving.SolveServer(slots=1, batch=1)
+    with pytest.raises(RuntimeError, match="switched off"):
+        srv.solve_batch(serving.make_requests(2, seed=1))
+    with pytest.raises(RuntimeError, match="switched off"):
+        serving.PackedBatch([], torch.device("cpu"))
+
+
 def test_sell_index_lives_with_its_pattern(monkeypatch):
     """The shared SELL index is kept on the pattern's column tensor: built
     once per pattern, reused by every matrix of it, and freed with it (no
