@@ -225,13 +225,13 @@ def _roof(kernel, algo_bytes, launches, ms, traffic=None, **extra):
     return out
 
 
-def rooflines(kt, mesh, gdof, nnz_a, nnz_bc, n, vb):
+def rooflines(kt, mesh, gdof, nnz_a, nnz_bc, n, vb, vb_a=8):
     """Roofline objects from the in-step kernel times of one step.
     Algorithmic bytes are the compulsory traffic of each kernel or phase
     (inputs read once, outputs written once); vb = bytes per value of the
-    step's CG operator and assembled matrix (8 FP64, 4 FP32):
+    step's CG operator, vb_a of the assembled matrix (8 FP64, 4 FP32):
     - CG operator A p on int16 column offsets: (vb + 2) nnz + 4 (n+1) + 2 vb n;
-    - A values: 4 NC ldof (cells) + 8 GD NN (coordinates) + 8 NC (k) + vb nnz;
+    - A values: 4 NC ldof (cells) + 8 GD NN (coordinates) + 8 NC (k) + vb_a nnz;
     - F (FP64): 4 NC ldof + 8 GD NN + 8 NC (k) + 8 gdof;
     - matrix-free residual b - A x (FP64 in, FP32 out): 4 NC ldof + 8 GD NN +
       8 NC + 8 gdof (x) + 8 gdof (b) + gdof (boundary flags) + 4 gdof (r).
@@ -258,7 +258,7 @@ def rooflines(kt, mesh, gdof, nnz_a, nnz_bc, n, vb):
     smooth = _roof("k_cheb16* / k_residual16 (finest level)", b_s, sl, sms,
                    note="FP16 D^-1 A values and int16 column offsets: (2 + 2) B an entry + 5 FP32 vectors") \
         if sl else None
-    b_a = 16 * nc + 24 * nn + 8 * nc + vb * nnz_a
+    b_a = 16 * nc + 24 * nn + 8 * nc + vb_a * nnz_a
     b_f = 16 * nc + 24 * nn + 8 * nc + 8 * gdof
     (l1, m1), (l2, m2) = kt["a_pass1"], kt["a_pass2"]
     (l3, m3), (l4, m4) = kt["f_pass1"], kt["f_pass2"]
@@ -857,7 +857,7 @@ def main():
     kt64 = in_step_kernels(step, space, k, u, f, mg64, args.rtol)
     err_l2 = float(mesh.error(u, space.function(x_m)))
     nnz_a = pat.nnz
-    roof, roof_asm = rooflines(kt, mesh, gdof, nnz_a, int(A2.nnz), gdof, 4)
+    roof, roof_asm = rooflines(kt, mesh, gdof, nnz_a, int(A2.nnz), gdof, 4)     # A itself is FP64 in both forms
     roof64, roof_asm64 = rooflines(kt64, mesh, gdof, nnz_a, int(A2.nnz), gdof, 8)
     roof32 = spmv_roofline(A2, torch.float32)
     if args.n == 128:

@@ -138,7 +138,7 @@ class _SizeContext:
 
 class SolveServer:
     def __init__(self, *, slots=4, hbm_budget=64 << 30, rtol=1e-8, maxit=2000, q=3, mg_dtype=torch.float32,
-                 graphs=False, device=None, batch=1, resident_budget=128 << 30):
+                 graphs=False, device=None, batch=1, resident_budget=200 << 30):
         self.device = torch.device("cuda") if device is None else torch.device(device)
         self.slots, self.budget, self.rtol, self.maxit, self.q = slots, int(hbm_budget), rtol, maxit, q
         # batch > 1: a slot takes up to `batch` waiting requests at once and
@@ -149,7 +149,9 @@ class SolveServer:
         self.mg_kwargs = {"dtype": mg_dtype, "graphs": graphs}
         # resident per-size data (meshes, patterns, pooled hierarchies) is
         # kept within resident_budget bytes (estimate), least recently used
-        # sizes dropped first
+        # sizes dropped first.  The default holds every size of the
+        # 874-size serving trace (about 188 GB by the estimate) beside the
+        # 64 GB in-flight budget on a 288 GB device.
         self.resident_budget = int(resident_budget)
         self.evictions = 0
         self._ctx = collections.OrderedDict()
