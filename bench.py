@@ -73,8 +73,11 @@ def parse():
                     help="source as a barycentric torch callable instead of a separable expression")
     ap.add_argument("--requests", type=int, default=4096, help="serving: number of requests")
     ap.add_argument("--serving-batch", type=int, default=1,
-                    help="serving: most requests a slot packs into one PCG (1: no packed batches)")
+                    help="serving: most waiting requests a slot takes at once, the small ones solved in one launch "
+                         "with one workgroup each (1: one request per slot)")
     ap.add_argument("--slots", type=int, default=SERVING_SLOTS, help="serving: concurrent request slots (streams)")
+    ap.add_argument("--resident-gb", type=float, default=200.0,
+                    help="serving: device memory (estimate) kept for resident per-size data")
     ap.add_argument("--rate", type=float, default=0.0,
                     help="serving: Poisson arrival rate per second (0: 0.8 x the measured server throughput)")
     ap.add_argument("--workload", choices=("diffusion", "elasticity", "highorder", "serving"), default="diffusion",
@@ -204,7 +207,8 @@ def in_step_kernels(stepfn, space, k, u, f, mg, rtol):
     from fealpy_amd import _native as nat
     kinds = {"pcg_spmv": nat.TIMER_PCG_SPMV, "a_pass1": nat.TIMER_P1_ROWS, "a_pass2": nat.TIMER_P1_SUM_ROWS,
              "f_pass1": nat.TIMER_P1_VECTORS, "f_pass2": nat.TIMER_P1_SUM_VALUES, "dirichlet": nat.TIMER_BC_APPLY,
-             "vcycle": nat.TIMER_VCYCLE, "p1_apply": nat.TIMER_P1_APPLY, "smooth0": nat.TIMER_SMOOTH0}
+             "vcycle": nat.TIMER_VCYCLE, "p1_apply": nat.TIMER_P1_APPLY, "smooth0": nat.TIMER_SMOOTH0,
+             "a_box": nat.TIMER_P1_BOX}
     torch.cuda.synchronize()
     nat.timer_arm(*kinds.values())
     try:
@@ -262,9 +266,20 @@ def rooflines(kt, mesh, gdof, nnz_a, nnz_bc, n, vb, vb_a=8):
     b_f = 16 * nc + 24 * nn + 8 * nc + 8 * gdof
     (l1, m1), (l2, m2) = kt["a_pass1"], kt["a_pass2"]
     (l3, m3), (l4, m4) = kt["f_pass1"], kt["f_pass2"]
-    asm_a = _roof("k_p1_incidence_rows<3> + k_sum_incidence_rows", b_a, 1, m1 + m2,
-                  pass1_us=round(m1 * 1e3 / max(l1, 1), 2), pass2_us=round(m2 * 1e3 / max(l2, 1), 2))
-    asm_f = _roof("k_p1_incidence_vectors<3> + k_sum_incidence_values", b_f, 1, m3 + m4,
+    lb, mb = kt["a_box"]
+    if lb:      # the structured from_box kernel: no cell array (cells come from the grid position)
+        b_box = 24 * nn + 8 * nc + 4 * (gdof + 1) + vb_a * nnz_a
+        asm_a = _roof("k_p1_box_rows", b_box, lb, mb,
+                      note="one pass: coordinates, k and the row pointer in, CSR values out (no cell array or "
+                           f"incidence buffer); the generic kernels' compulsory count with cells would be {b_a} B",
+                      generic_algorithmic_bytes=int(b_a))
+    else:
+        asm_a = _roof("k_p1_incidence_rows<3> + k_sum_incidence_rows", b_a, 1, m1 + m2,
+                      pass1_us=round(m1 * 1e3 / max(l1, 1), 2), pass2_us=round(m2 * 1e3 / max(l2, 1), 2))
+    fname = ("k_p1_box_cell_vectors + k_p1_box_sum_vectors" if lb else
+             "k_p1_incidence_vectors<3> + k_sum_incidence_values")
+    # the structured kernels read no cell array: coordinates, k, the vector out
+    asm_f = _roof(fname, 24 * nn + 8 * nc + 8 * gdof if lb else b_f, 1, m3 + m4,
                   pass1_us=round(m3 * 1e3 / max(l3, 1), 2), pass2_us=round(m4 * 1e3 / max(l4, 1), 2))
     vl, vms = kt["vcycle"]
     dl, dms = kt["dirichlet"]
@@ -519,6 +534,7 @@ def highorder_main(args):
         g64 = GroupIntegrator(fa.ScalarDiffusionIntegrator(kq, q=q), fa.ScalarMassIntegrator(1.0, q=q))
         ms64, ref = ev_time(lambda: g64.assembly(space), reps=2)
         flops = mfma_flops(nc, nq, ldof)
+        print(f"# highorder contraction p={p}: FP64 element kernel {ms64:.2f} ms", file=sys.stderr, flush=True)
         row = {"p": p, "q": q, "nq": nq, "ldof": ldof, "ncell": nc, "gflop": round(flops / 1e9, 2),
                "fp64_element_ms": round(ms64, 3),
                "fp64_tflops": round(flops / (ms64 * 1e-3) / 1e12, 2)}
@@ -612,7 +628,11 @@ def highorder_solve(n, p, q, rtol, steps):
         t("mg_setup_ms", lambda: mg.setup(A2))
         xs, info = t("pcg_ms", lambda: fa.cg(A2, F2, M=mg, atol=0.0, rtol=rt, maxit=2000, returninfo=True))
         return xs, info, A2
+    print(f"# highorder solve p={p}: {gdof} DOFs, pattern {t_pat * 1e3:.1f} ms; reference solve", file=sys.stderr,
+          flush=True)
     x_ref, info_ref, A2 = one(1e-13, False)
+    print(f"# highorder solve p={p}: reference solve {info_ref['niter']} iterations; timed steps", file=sys.stderr,
+          flush=True)
     nnz = int(A2.nnz)
     del A2
     torch.cuda.synchronize()
@@ -641,10 +661,10 @@ def _pcts(lat):
 
 
 def _wait_all(reqs, label):
-    """Wait for every request in order, with a progress line per 1024 done."""
+    """Wait for every request in order, with a progress line per 256 done."""
     for i, r in enumerate(reqs):
         r.event.wait()
-        if (i + 1) % 1024 == 0:
+        if (i + 1) % 256 == 0:
             print(f"# serving {label}: {i + 1}/{len(reqs)} done", file=sys.stderr, flush=True)
 
 
@@ -661,10 +681,10 @@ def run_serving(args):
     (a FIFO single server: start = max(arrival, previous done)).  Needs
     GPU_MAX_HW_QUEUES >= 2 x slots, set before HIP initialises (main does)."""
     import torch
-    from fealpy_amd.serving import MAX_BATCH, PACKED_BATCHES_ENABLED, PACKED_BATCHES_OFF, SolveServer, make_requests
+    from fealpy_amd.serving import MAX_BATCH, WG_MAX_GDOF, SolveServer, make_requests
     n = args.requests
     seed = 20261202
-    srv = SolveServer(slots=args.slots, rtol=args.rtol)
+    srv = SolveServer(slots=args.slots, rtol=args.rtol, resident_budget=int(args.resident_gb * (1 << 30)))
     plain = make_requests(n, seed=seed)
     sizes = sorted({(r.nx, r.p) for r in plain})
     t_warm = 0.0
@@ -676,7 +696,7 @@ def run_serving(args):
     t0 = time.perf_counter()
     for i, r in enumerate(plain):
         srv.solve(r)
-        if (i + 1) % 1024 == 0:
+        if (i + 1) % 256 == 0:
             print(f"# serving plain loop {i + 1}/{n} solved", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
@@ -706,27 +726,26 @@ def run_serving(args):
     _wait_all(timed_reqs, "server, Poisson arrivals")
     srv.close()
     lat_srv = [r.done - r.arrival for r in timed_reqs]
-    # the same server with packed batches: a slot takes the requests waiting
-    # when it frees up and solves those of equal depth in one PCG
+    # the same server with batches: a slot takes the requests waiting when
+    # it frees up and solves the small ones in one launch, one workgroup each
     batched = None
-    if args.serving_batch > 1 and not PACKED_BATCHES_ENABLED:
-        batched = {"skipped": PACKED_BATCHES_OFF}
-    elif args.serving_batch > 1:
-        srv.release_pools(packed=False)
+    if args.serving_batch > 1:
+        # small sizes also keep an uncompressed hierarchy for workgroup solves
+        small = [(nx, p) for nx, p in sizes if (p * nx + 1) ** 2 <= WG_MAX_GDOF]
         t_wb = 0.0
-        for i in range(0, len(sizes), 64):
-            t_wb += srv.warm(sizes[i:i + 64], packed=True)
-        print(f"# serving warm (packed hierarchies) {t_wb:.1f}s", file=sys.stderr, flush=True)
+        for i in range(0, len(small), 64):
+            t_wb += srv.warm(small[i:i + 64], packed=True)
+        print(f"# serving warm (workgroup hierarchies, {len(small)} sizes) {t_wb:.1f}s", file=sys.stderr, flush=True)
         srv.batch = min(args.serving_batch, MAX_BATCH)
         mixb = make_requests(n, seed=seed)
         srv.start()
         t0 = time.perf_counter()
         for r in mixb:
             srv.submit(r)
-        _wait_all(mixb, "server, packed batches")
+        _wait_all(mixb, "server, batches")
         t_b = time.perf_counter() - t0
         thr_b = n / t_b
-        print(f"# serving server, packed batches {thr_b:.1f} solves/s", file=sys.stderr, flush=True)
+        print(f"# serving server, batches {thr_b:.1f} solves/s", file=sys.stderr, flush=True)
         close = sum(int(float(torch.linalg.norm(b.x - a.x)) <= 1e-8 * float(torch.linalg.norm(a.x)))
                     for a, b in zip(plain, mixb) if b.error is None)
         errors_b = sum(1 for r in mixb if r.error is not None)
@@ -741,15 +760,15 @@ def run_serving(args):
                 time.sleep(min(r.arrival - now, 0.002))
             r.arrival += t0
             srv.submit(r)
-        _wait_all(timed_b, "server, packed batches, Poisson arrivals")
+        _wait_all(timed_b, "server, batches, Poisson arrivals")
         srv.close()
         lat_b = [r.done - r.arrival for r in timed_b]
         sizes_b = [r.info.get("batch", 1) for r in mixb if r.info]
         batched = {"solves_per_s": round(thr_b, 2), "speedup_vs_plain": round(thr_b * t_plain / n, 3),
                    "arrival_rate": round(rate_b, 2), **_pcts(lat_b), "max_batch": srv.batch,
-                   "mean_requests_per_packed_pcg": round(float(np.mean(sizes_b)), 2) if sizes_b else None,
+                   "mean_requests_per_workgroup_launch": round(float(np.mean(sizes_b)), 2) if sizes_b else None,
                    "results_within_1e-8_of_one_at_a_time": f"{close}/{n}", "request_errors": errors_b,
-                   "cold_packed_warm_s": round(t_wb, 2)}
+                   "cold_workgroup_warm_s": round(t_wb, 2)}
     service = [r.done - r.start for r in plain]
     done, lat_plain = 0.0, []
     for r, sv in zip(timed_reqs, service):
@@ -761,7 +780,7 @@ def run_serving(args):
     head = {"mode": "streams", "solves_per_s": round(thr, 2), "speedup_vs_plain": round(thr * t_plain / n, 3),
             "p50_ms": _pcts(lat_srv)["p50_ms"], "p99_ms": _pcts(lat_srv)["p99_ms"]}
     if batched is not None and batched["solves_per_s"] > thr:
-        head = {"mode": "packed batches", "solves_per_s": batched["solves_per_s"],
+        head = {"mode": "batches (workgroup solves)", "solves_per_s": batched["solves_per_s"],
                 "speedup_vs_plain": batched["speedup_vs_plain"], "p50_ms": batched["p50_ms"],
                 "p99_ms": batched["p99_ms"]}
     return {
@@ -769,12 +788,13 @@ def run_serving(args):
         "config": {"workload": f"{n} requests, TriangleMesh.from_box nx^2 with nx log-uniform in [16, 512], P1/P2, "
                                f"k=exp(N(0,1)) per cell, 8 Fourier modes, u=0 on the boundary",
                    "slots": args.slots, "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")),
-                   "rtol": args.rtol, "distinct_sizes": len(sizes),
+                   "rtol": args.rtol, "distinct_sizes": len(sizes), "resident_gb": args.resident_gb,
+                   "batch": args.serving_batch, "workgroup_max_gdof": WG_MAX_GDOF,
                    "mean_gdof": round(float(np.mean(gdofs)), 1), "max_gdof": int(max(gdofs)),
                    "pcg_iterations_mean": round(float(np.mean(iters)), 2), "pcg_iterations_max": int(max(iters))},
         "plain_loop": {"solves_per_s": round(n / t_plain, 2), **_pcts(lat_plain)},
         "server": {"solves_per_s": round(thr, 2), "arrival_rate": round(rate, 2), **_pcts(lat_srv)},
-        "server_packed_batches": batched,
+        "server_batches": batched,
         "results_identical_alone_and_in_mix": f"{same}/{n}", "request_errors": errors,
         "cold": {"warm_s": round(t_warm, 2)}, "resident_evictions": srv.evictions,
         "dof_per_s_server": round(sum(gdofs) / t_srv, 1)}

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libfealpy_amd.so")
 FA_OK, FA_EINVAL, FA_ENOMEM, FA_EHIP, FA_ENOCONV, FA_ECAPACITY = range(6)
 FA_F64, FA_F32 = 0, 1
 (TIMER_PCG_SPMV, TIMER_P1_ROWS, TIMER_P1_SUM_ROWS, TIMER_P1_VECTORS, TIMER_P1_SUM_VALUES, TIMER_BC_APPLY,
- TIMER_SELL_SPMV, TIMER_VCYCLE, TIMER_P1_APPLY, TIMER_SMOOTH0) = range(1, 11)
+ TIMER_SELL_SPMV, TIMER_VCYCLE, TIMER_P1_APPLY, TIMER_SMOOTH0, TIMER_P1_BOX) = range(1, 12)
 COEF_NONE, COEF_CONST, COEF_CELL, COEF_CELLQ, COEF_SEP = 0, 1, 2, 3, 4
 
 _p = ctypes.c_void_p
@@ -68,6 +68,7 @@ _SIGNATURES = {
     "fa_csr_residual": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _sz, _p]),
     "fa_dirichlet_rhs": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p, _p, _p, _p, _p]),
     "fa_dirichlet_apply": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
+    "fa_dirichlet_apply_keep": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
     "fa_spmv": (ctypes.c_int, [ctypes.c_int, _i64, _i64, _p, _p, _p, _p, _p, _p]),
     "fa_spmm": (ctypes.c_int, [ctypes.c_int, _i64, _i64, _i64, _p, _p, _p, _p, _p, _p]),
     "fa_reduce_work_bytes": (_sz, []),
@@ -98,6 +99,14 @@ _SIGNATURES = {
                                             ctypes.c_int, _f64, _p, ctypes.c_int, _f64, _p, _p, _p, _p]),
     "fa_sum_incidence_rows": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i64, _p, _p, _p, _p, _i32,
                                              _p, _p]),
+    "fa_p1_box_matrix": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _p, ctypes.c_int, _p,
+                                        _p, ctypes.c_int, _f64, _p, ctypes.c_int, _f64, _p, _p, _i64, _p, _p]),
+    "fa_p1_box_vector_work_bytes": (_sz, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "fa_p1_box_pattern_work_bytes": (_sz, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "fa_pcg_wg_batched": (ctypes.c_int, [ctypes.c_int, _p, _p, _p]),
+    "fa_p1_box_pattern": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _p, _p, _i64, _p, _sz, _p]),
+    "fa_p1_box_vector": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _p, ctypes.c_int, _p,
+                                        _p, ctypes.c_int, _f64, _p, _p, _p, _sz, _p]),
     "fa_p1_incidence_vectors": (ctypes.c_int, [ctypes.c_int, _i64, ctypes.c_int, _p, _p, _p, _p, ctypes.c_int, _f64,
                                                _p, _p, _p, _p]),
     "fa_sum_incidence_values": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p]),

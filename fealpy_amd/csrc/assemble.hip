@@ -1610,6 +1610,243 @@ extern "C" int fa_sum_incidence_values(int dtype, int64_t gdof, const int32_t* a
     return FA_OK;
 }
 
+// ---- P1 matrix on a from_box tetrahedral grid -------------------------------
+// TetrahedronMesh.from_box (tetrahedron_mesh.py:1017-1086) numbers node
+// (ix, iy, iz) as (ix (ny+1) + iy) (nz+1) + iz, cube (cx, cy, cz) as
+// (cx ny + cy) nz + cz, and cell 6 cube + t, the Kuhn tetrahedron t over the
+// cube corners BOX_KUHN[t] (corner k at offset BOX_CORNER[k]).  So the cells
+// around a node, in ascending order, are known from its grid position: the
+// cubes (ix - a, iy - b, iz - c) in the order (a, b, c) = (1,1,1), (1,1,0),
+// ..., (0,0,0) (lexicographic cube index), and in each cube the tetrahedra
+// that hold the node's corner, ascending t.  One thread per CSR row walks
+// them in that order, computes each element row with the code of the
+// two-pass kernels (P1Cell, p1_kd, p1_row on the cell's own vertex order) and
+// adds it into 15 register accumulators, one per neighbour offset: every
+// entry is the sequential sum of its cell contributions in ascending cell
+// order, the order of the reference's lexsort + np.add.at, so the values
+// equal those of the two-pass kernels bit for bit.  No incidence list, cell
+// array or element-row buffer is read or written: the thread's loads are
+// vertex coordinates (one 32-byte row each) and coefficients, all addressed
+// from the grid position, so they issue back to back.
+//
+// The neighbour offsets of a node are the 14 Kuhn edge directions
+// +-(1,0,0), +-(0,1,0), +-(0,0,1), +-(1,1,0), +-(1,0,1), +-(0,1,1),
+// +-(1,1,1) and the node itself; a neighbour inside the grid is always in the
+// CSR row (some cube holds the pair as a Kuhn edge), and ascending column
+// order is lexicographic offset order, so the entry's position in the row is
+// the number of present neighbours before it.
+constexpr int box_cdx(int k) { return (k == 1 || k == 2 || k == 5 || k == 6) ? 1 : 0; }
+constexpr int box_cdy(int k) { return (k == 2 || k == 3 || k == 6 || k == 7) ? 1 : 0; }
+constexpr int box_cdz(int k) { return (k >= 4) ? 1 : 0; }
+constexpr int box_corner_of(int a, int b, int c) {
+    return c == 0 ? (a == 0 ? (b == 0 ? 0 : 3) : (b == 0 ? 1 : 2)) : (a == 0 ? (b == 0 ? 4 : 7) : (b == 0 ? 5 : 6));
+}
+constexpr int box_kuhn(int t, int v) {
+    constexpr int K[6][4] = {{0, 1, 2, 6}, {0, 5, 1, 6}, {0, 4, 5, 6}, {0, 7, 4, 6}, {0, 3, 7, 6}, {0, 2, 3, 6}};
+    return K[t][v];
+}
+// local index of corner k in tetrahedron t, or -1
+constexpr int box_local(int t, int k) {
+    return box_kuhn(t, 0) == k ? 0 : box_kuhn(t, 1) == k ? 1 : box_kuhn(t, 2) == k ? 2 : box_kuhn(t, 3) == k ? 3 : -1;
+}
+// accumulator slot of offset (dx, dy, dz): rank among the 15 offsets in
+// lexicographic order, -1 for an offset that is no Kuhn edge
+constexpr int box_slot(int dx, int dy, int dz) {
+    constexpr int S[27] = {0, 1, -1, 2, 3, -1, -1, -1, -1, 4, 5, -1, 6, 7, 8, -1, 9, 10,
+                           -1, -1, -1, -1, 11, 12, -1, 13, 14};
+    return S[(dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)];
+}
+constexpr int BOX_SLOTS = 15;
+
+// the contributions of the tetrahedra of cube (ix - A, iy - B, iz - C) that
+// hold the node, in ascending cell order
+template <int A, int B, int C, bool SEP>
+__device__ inline void box_cube(int ix, int iy, int iz, int nx, int ny, int nz, const double* __restrict__ nodew,
+                                int nq, const double* __restrict__ qw, const double* __restrict__ phi, int dk,
+                                double ds, const double* __restrict__ dc, int mk, double ms,
+                                const double* __restrict__ mc, double acc[BOX_SLOTS]) {
+    const int cx = ix - A, cy = iy - B, cz = iz - C;
+    if (cx < 0 || cy < 0 || cz < 0 || cx >= nx || cy >= ny || cz >= nz) return;
+    constexpr int KAP = box_corner_of(A, B, C);
+    const int64_t cube = (static_cast<int64_t>(cx) * ny + cy) * nz + cz;
+    const int64_t nz1 = nz + 1, nyz = static_cast<int64_t>(ny + 1) * nz1;
+    const int64_t n0 = static_cast<int64_t>(cx) * nyz + static_cast<int64_t>(cy) * nz1 + cz;
+    double P[8][3];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {  // loads of corners no tetrahedron below uses are dropped by the compiler
+        const double4 q = reinterpret_cast<const double4*>(nodew)[n0 + box_cdx(k) * nyz + box_cdy(k) * nz1 + box_cdz(k)];
+        P[k][0] = q.x;
+        P[k][1] = q.y;
+        P[k][2] = q.z;
+    }
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+        const int i = box_local(t, KAP);
+        if (i < 0) continue;
+        double p[4][3];
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) p[v][d] = P[box_kuhn(t, v)][d];
+        P1Cell<3> g;
+        g.compute(p);
+        const int64_t c = 6 * cube + t;
+        const double kd = p1_kd<3, SEP>(nq, qw, dk, ds, dc, c, c, p);
+        double v[4];
+        p1_row<3, SEP>(g, p, c, kd, i, c, nq, qw, phi, dk, ds, dc, mk, ms, mc, v);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = box_kuhn(t, j);
+            acc[box_slot(box_cdx(k) - A, box_cdy(k) - B, box_cdz(k) - C)] += v[j];
+        }
+    }
+}
+
+template <class T, bool SEP>
+__global__ void __launch_bounds__(256) k_p1_box_rows(int nx, int ny, int nz, const double* __restrict__ nodew, int nq,
+                                                    const double* __restrict__ qw, const double* __restrict__ phi,
+                                                    int dk, double ds, const double* __restrict__ dc, int mk,
+                                                    double ms, const double* __restrict__ mc,
+                                                    const int32_t* __restrict__ indptr, T* __restrict__ values) {
+    const int64_t nz1 = nz + 1, nyz = static_cast<int64_t>(ny + 1) * nz1, nn = static_cast<int64_t>(nx + 1) * nyz;
+    // XCD-aware order: each XCD's L2 holds one contiguous range of rows, so
+    // the coordinate rows shared by neighbouring rows are read into one L2
+    for (int64_t r = xcd_block() * (int64_t)blockDim.x + threadIdx.x; r < nn; r += (int64_t)gridDim.x * blockDim.x) {
+        const int ix = static_cast<int>(r / nyz);
+        const int64_t rem = r - ix * nyz;
+        const int iy = static_cast<int>(rem / nz1);
+        const int iz = static_cast<int>(rem - iy * nz1);
+        double acc[BOX_SLOTS];
+#pragma unroll
+        for (int s = 0; s < BOX_SLOTS; ++s) acc[s] = 0.0;
+        box_cube<1, 1, 1, SEP>(ix, iy, iz, nx, ny, nz, nodew, nq, qw, phi, dk, ds, dc, mk, ms, mc, acc);
+        box_cube<1, 1, 0, SEP>(ix, iy, iz, nx, ny, nz, nodew, nq, qw, phi, dk, ds, dc, mk, ms, mc, acc);
+        box_cube<1, 0, 1, SEP>(ix, iy, iz, nx, ny, nz, nodew, nq, qw, phi, dk, ds, dc, mk, ms, mc, acc);
+        box_cube<1, 0, 0, SEP>(ix, iy, iz, nx, ny, nz, nodew, nq, qw, phi, dk, ds, dc, mk, ms, mc, acc);
+        box_cube<0, 1, 1, SEP>(ix, iy, iz, nx, ny, nz, nodew, nq, qw, phi, dk, ds, dc, mk, ms, mc, acc);
+        box_cube<0, 1, 0, SEP>(ix, iy, iz, nx, ny, nz, nodew, nq, qw, phi, dk, ds, dc, mk, ms, mc, acc);
+        box_cube<0, 0, 1, SEP>(ix, iy, iz, nx, ny, nz, nodew, nq, qw, phi, dk, ds, dc, mk, ms, mc, acc);
+        box_cube<0, 0, 0, SEP>(ix, iy, iz, nx, ny, nz, nodew, nq, qw, phi, dk, ds, dc, mk, ms, mc, acc);
+        const int32_t r0 = indptr[r], len = indptr[r + 1] - r0;
+        int pos = 0;
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx)
+#pragma unroll
+            for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                for (int dz = -1; dz <= 1; ++dz) {
+                    const int s = box_slot(dx, dy, dz);
+                    if (s < 0) continue;
+                    const bool in = ix + dx >= 0 && ix + dx <= nx && iy + dy >= 0 && iy + dy <= ny && iz + dz >= 0 &&
+                                    iz + dz <= nz;
+                    if (in) {
+                        if (pos < len) values[r0 + pos] = static_cast<T>(acc[s]);
+                        ++pos;
+                    }
+                }
+    }
+}
+
+// Load vector on the same grid, two passes without an incidence list:
+// pass 1, one thread per cell in cell order, vals[4c + i] = |T| sum_q w_q
+// f(x_q) phi_i(q) (the arithmetic of k_p1_incidence_vectors), one 32-byte
+// store per cell; pass 2, one thread per node, the sum of vals[4c + i] over
+// the node's cells in ascending order, the addresses computed from the grid
+// position (the order of the two-pass kernels' incidence list, so the values
+// are equal bit for bit).
+template <bool SEP>
+__global__ void __launch_bounds__(256) k_p1_box_cell_vectors(int nx, int ny, int nz, const double* __restrict__ nodew,
+                                                            int nq, const double* __restrict__ qw,
+                                                            const double* __restrict__ phi, int sk, double ss,
+                                                            const double* __restrict__ src,
+                                                            double* __restrict__ vals) {
+    const int64_t nz1 = nz + 1, nyz = static_cast<int64_t>(ny + 1) * nz1;
+    const int64_t nc = 6 * static_cast<int64_t>(nx) * ny * nz;
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t cube = c / 6;
+        const int t = static_cast<int>(c - 6 * cube);
+        const int cx = static_cast<int>(cube / (static_cast<int64_t>(ny) * nz));
+        const int64_t rem = cube - static_cast<int64_t>(cx) * ny * nz;
+        const int cy = static_cast<int>(rem / nz);
+        const int cz = static_cast<int>(rem - static_cast<int64_t>(cy) * nz);
+        const int64_t n0 = static_cast<int64_t>(cx) * nyz + static_cast<int64_t>(cy) * nz1 + cz;
+        double p[4][3];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            // corner of vertex v of tetrahedron t, t a runtime index: the
+            // table is small enough to select without memory
+            int k = 0;
+#pragma unroll
+            for (int tt = 0; tt < 6; ++tt)
+                if (t == tt) k = box_kuhn(tt, v);
+            const double4 q = reinterpret_cast<const double4*>(nodew)[n0 + box_cdx(k) * nyz + box_cdy(k) * nz1 +
+                                                                    box_cdz(k)];
+            p[v][0] = q.x;
+            p[v][1] = q.y;
+            p[v][2] = q.z;
+        }
+        P1Cell<3> g;
+        g.compute(p);
+        double v[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int q = 0; q < nq; ++q) {
+            const double fq = qw[q] * g.meas * coef_at<3, SEP>(sk, ss, src, c, c, q, nq, p);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) v[i] += fq * phi[q * 4 + i];
+        }
+        reinterpret_cast<double4*>(vals)[c] = double4{v[0], v[1], v[2], v[3]};
+    }
+}
+
+template <int A, int B, int C>
+__device__ inline void box_cube_sum(int ix, int iy, int iz, int nx, int ny, int nz, const double* __restrict__ vals,
+                                    double& acc) {
+    const int cx = ix - A, cy = iy - B, cz = iz - C;
+    if (cx < 0 || cy < 0 || cz < 0 || cx >= nx || cy >= ny || cz >= nz) return;
+    constexpr int KAP = box_corner_of(A, B, C);
+    const int64_t cube = (static_cast<int64_t>(cx) * ny + cy) * nz + cz;
+    double v[6];
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {  // the loads first, then the adds in order
+        const int i = box_local(t, KAP);
+        v[t] = i < 0 ? 0.0 : vals[(6 * cube + t) * 4 + i];
+    }
+#pragma unroll
+    for (int t = 0; t < 6; ++t)
+        if (box_local(t, KAP) >= 0) acc += v[t];
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_p1_box_sum_vectors(int nx, int ny, int nz, const double* __restrict__ vals,
+                                                           T* __restrict__ out) {
+    const int64_t nz1 = nz + 1, nyz = static_cast<int64_t>(ny + 1) * nz1, nn = static_cast<int64_t>(nx + 1) * nyz;
+    for (int64_t r = xcd_block() * (int64_t)blockDim.x + threadIdx.x; r < nn; r += (int64_t)gridDim.x * blockDim.x) {
+        const int ix = static_cast<int>(r / nyz);
+        const int64_t rem = r - ix * nyz;
+        const int iy = static_cast<int>(rem / nz1);
+        const int iz = static_cast<int>(rem - iy * nz1);
+        double acc = 0.0;
+        box_cube_sum<1, 1, 1>(ix, iy, iz, nx, ny, nz, vals, acc);
+        box_cube_sum<1, 1, 0>(ix, iy, iz, nx, ny, nz, vals, acc);
+        box_cube_sum<1, 0, 1>(ix, iy, iz, nx, ny, nz, vals, acc);
+        box_cube_sum<1, 0, 0>(ix, iy, iz, nx, ny, nz, vals, acc);
+        box_cube_sum<0, 1, 1>(ix, iy, iz, nx, ny, nz, vals, acc);
+        box_cube_sum<0, 1, 0>(ix, iy, iz, nx, ny, nz, vals, acc);
+        box_cube_sum<0, 0, 1>(ix, iy, iz, nx, ny, nz, vals, acc);
+        box_cube_sum<0, 0, 0>(ix, iy, iz, nx, ny, nz, vals, acc);
+        out[r] = static_cast<T>(acc);
+    }
+}
+
+// nonzeros of the P1 pattern of an nx x ny x nz Kuhn grid: the nodes plus
+// both directions of every edge
+static int64_t box_nnz(int64_t nx, int64_t ny, int64_t nz) {
+    const int64_t nn = (nx + 1) * (ny + 1) * (nz + 1);
+    const int o[7][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+    int64_t edges = 0;
+    for (const auto& e : o) edges += (nx + 1 - e[0]) * (ny + 1 - e[1]) * (nz + 1 - e[2]);
+    return nn + 2 * edges;
+}
+
 // ---- matrix-free P1 operator -------------------------------------------------
 // y_r = sum over the incidences (c, i) of DOF r of sum_j A^c_ij x_{cell[c][j]},
 // where A^c is the element matrix whose rows k_assemble_p1 sums into CSR
@@ -1738,5 +1975,82 @@ extern "C" int fa_p1_apply(int dtype_y, int td, int64_t gdof, int nq, const doub
         hipLaunchKernelGGL(k_p1_apply_finish, dim3(1), dim3(RED_BLOCK), 0, s, part, static_cast<int>(grid), norm2);
         FA_LAUNCH("k_p1_apply_finish");
     }
+    return FA_OK;
+}
+
+extern "C" int fa_p1_box_matrix(int dtype, int nx, int ny, int nz, const double* node_wide, int nq, const double* qw,
+                                const double* phi, int diff_kind, double diff_scale, const double* diff_coef,
+                                int mass_kind, double mass_scale, const double* mass_coef, const int32_t* indptr,
+                                int64_t nnz, void* values, fa_stream_t stream) {
+    FA_REQUIRE(dtype == FA_F64 || dtype == FA_F32, "fa_p1_box_matrix: bad dtype %d", dtype);
+    FA_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1 && nq > 0 && node_wide && qw && indptr && values,
+               "fa_p1_box_matrix: bad arguments");
+    FA_REQUIRE(!mass_kind || phi, "fa_p1_box_matrix: mass needs phi");
+    FA_REQUIRE(!(diff_kind >= FA_COEF_CELL && !diff_coef) && !(mass_kind >= FA_COEF_CELL && !mass_coef),
+               "fa_p1_box_matrix: coefficient array missing");
+    const int64_t nn = static_cast<int64_t>(nx + 1) * (ny + 1) * (nz + 1);
+    FA_REQUIRE(nn < INT32_MAX && 6 * static_cast<int64_t>(nx) * ny * nz < INT32_MAX,
+               "fa_p1_box_matrix: grid too large for int32 indices");
+    FA_REQUIRE(nnz == box_nnz(nx, ny, nz),
+               "fa_p1_box_matrix: %lld nonzeros, not the P1 pattern of a %d x %d x %d tetrahedral box (%lld)",
+               (long long)nnz, nx, ny, nz, (long long)box_nnz(nx, ny, nz));
+    const unsigned grid = xcd_grid(grid_for(nn, 256, 1 << 20));
+    const bool sep = diff_kind == FA_COEF_SEP || mass_kind == FA_COEF_SEP;
+    timer_begin(FA_TIMER_P1_BOX, S(stream));
+#define FA_BOX(T_, SEP_)                                                                                            \
+    hipLaunchKernelGGL((k_p1_box_rows<T_, SEP_>), dim3(grid), dim3(256), 0, S(stream), nx, ny, nz, node_wide, nq, qw,  \
+                       phi, diff_kind, diff_scale, diff_coef, mass_kind, mass_scale, mass_coef, indptr,               \
+                       static_cast<T_*>(values))
+    if (dtype == FA_F64) {
+        if (sep) FA_BOX(double, true); else FA_BOX(double, false);
+    } else {
+        if (sep) FA_BOX(float, true); else FA_BOX(float, false);
+    }
+#undef FA_BOX
+    timer_end(FA_TIMER_P1_BOX, S(stream));
+    FA_LAUNCH("k_p1_box_rows");
+    return FA_OK;
+}
+
+extern "C" size_t fa_p1_box_vector_work_bytes(int nx, int ny, int nz) {
+    return sizeof(double) * 4 * 6 * static_cast<size_t>(nx) * ny * nz;
+}
+
+extern "C" int fa_p1_box_vector(int dtype, int nx, int ny, int nz, const double* node_wide, int nq, const double* qw,
+                                const double* phi, int src_kind, double src_scale, const double* src, void* out,
+                                void* work, size_t work_bytes, fa_stream_t stream) {
+    FA_REQUIRE(dtype == FA_F64 || dtype == FA_F32, "fa_p1_box_vector: bad dtype %d", dtype);
+    FA_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1 && nq > 0 && node_wide && qw && phi && out && work,
+               "fa_p1_box_vector: bad arguments");
+    FA_REQUIRE(!(src_kind >= FA_COEF_CELL && !src), "fa_p1_box_vector: source array missing");
+    const int64_t nn = static_cast<int64_t>(nx + 1) * (ny + 1) * (nz + 1);
+    const int64_t nc = 6 * static_cast<int64_t>(nx) * ny * nz;
+    FA_REQUIRE(nn < INT32_MAX && nc < INT32_MAX, "fa_p1_box_vector: grid too large for int32 indices");
+    FA_REQUIRE(work_bytes >= fa_p1_box_vector_work_bytes(nx, ny, nz) && (reinterpret_cast<uintptr_t>(work) & 31) == 0,
+               "fa_p1_box_vector: workspace of %zu bytes (32-byte aligned) too small, %zu needed", work_bytes,
+               fa_p1_box_vector_work_bytes(nx, ny, nz));
+    double* vals = static_cast<double*>(work);
+    hipStream_t s = S(stream);
+    const bool sep = src_kind == FA_COEF_SEP;
+    timer_begin(FA_TIMER_P1_VECTORS, s);
+    const unsigned g1 = grid_for(nc, 256, 1 << 20);
+    if (sep)
+        hipLaunchKernelGGL(k_p1_box_cell_vectors<true>, dim3(g1), dim3(256), 0, s, nx, ny, nz, node_wide, nq, qw, phi,
+                           src_kind, src_scale, src, vals);
+    else
+        hipLaunchKernelGGL(k_p1_box_cell_vectors<false>, dim3(g1), dim3(256), 0, s, nx, ny, nz, node_wide, nq, qw,
+                           phi, src_kind, src_scale, src, vals);
+    timer_end(FA_TIMER_P1_VECTORS, s);
+    FA_LAUNCH("k_p1_box_cell_vectors");
+    const unsigned g2 = xcd_grid(grid_for(nn, 256, 1 << 20));
+    timer_begin(FA_TIMER_P1_SUM_VALUES, s);
+    if (dtype == FA_F64)
+        hipLaunchKernelGGL(k_p1_box_sum_vectors<double>, dim3(g2), dim3(256), 0, s, nx, ny, nz, vals,
+                           static_cast<double*>(out));
+    else
+        hipLaunchKernelGGL(k_p1_box_sum_vectors<float>, dim3(g2), dim3(256), 0, s, nx, ny, nz, vals,
+                           static_cast<float*>(out));
+    timer_end(FA_TIMER_P1_SUM_VALUES, s);
+    FA_LAUNCH("k_p1_box_sum_vectors");
     return FA_OK;
 }

@@ -851,38 +851,14 @@ __device__ inline T* lds_vec(char*& p, int64_t n, const void* src) {
     return dst;
 }
 
-// LDS: every level vector, dinv and the slice offsets live in LDS (the
-// right-hand side of the top level is copied in, its result copied out to
-// `out`), so each phase waits on one global latency (the matrix entries)
-// instead of three; the arithmetic is that of the global variant.
-template <class T, bool LDS>
-__global__ void __launch_bounds__(FUSE_BLOCK) k_fused_cycle(FusedCycle fc, void* out, const int* __restrict__ done) {
-    if (done && *done) return;
-    __shared__ FusedLevel slv[LDS ? FUSE_MAX_LEVELS : 1];
+// One V-cycle over the levels lv[0 .. fc.nlev) by the calling workgroup:
+// pre-smoothing, residual and restriction down, the dense coarse solve,
+// prolongation and post-smoothing up, phases separated by block barriers.
+// Returns the level-0 vector that holds the result (L.x2, or L.x when level
+// 0 is the coarsest).
+template <class T>
+__device__ inline T* fused_vcycle(const FusedCycle& fc, const FusedLevel* lv) {
     const int nl = fc.nlev;
-    const FusedLevel* lv = fc.lv;
-    if (LDS) {
-        extern __shared__ char smem[];
-        char* p = smem;
-        for (int l = 0; l < nl; ++l) {
-            FusedLevel L = fc.lv[l];
-            const int64_t n = L.A.n;
-            L.A.slice_ptr = lds_copy_i32<T>(p, L.A.slice_ptr, (n + 63) / 64 + 1);
-            if (l < nl - 1) {
-                L.R.slice_ptr = lds_copy_i32<T>(p, L.R.slice_ptr, (L.R.n + 63) / 64 + 1);
-                L.P.slice_ptr = lds_copy_i32<T>(p, L.P.slice_ptr, (L.P.n + 63) / 64 + 1);
-            }
-            L.dinv = lds_vec<T>(p, n, L.dinv);
-            L.b = lds_vec<T>(p, n, l == 0 ? L.b : nullptr);
-            L.x = lds_vec<T>(p, n, nullptr);
-            L.x2 = lds_vec<T>(p, n, nullptr);
-            L.d = lds_vec<T>(p, n, nullptr);
-            L.r = lds_vec<T>(p, n, nullptr);
-            if (threadIdx.x == 0) slv[l] = L;
-        }
-        __syncthreads();
-        lv = slv;
-    }
     T* cur[FUSE_MAX_LEVELS];
     T* nxt[FUSE_MAX_LEVELS];
     for (int l = 0; l < nl - 1; ++l) {
@@ -919,9 +895,157 @@ __global__ void __launch_bounds__(FUSE_BLOCK) k_fused_cycle(FusedCycle fc, void*
         blk_apply<T, true>(L.P, cur[l + 1], cur[l]);
         blk_smooth<T>(fc, L, static_cast<const T*>(L.b), cur[l], nxt[l], false);
     }
+    return cur[0];
+}
+
+// LDS: every level vector, dinv and the slice offsets live in LDS (the
+// right-hand side of the top level is copied in, its result copied out to
+// `out`), so each phase waits on one global latency (the matrix entries)
+// instead of three; the arithmetic is that of the global variant.
+template <class T, bool LDS>
+__global__ void __launch_bounds__(FUSE_BLOCK) k_fused_cycle(FusedCycle fc, void* out, const int* __restrict__ done) {
+    if (done && *done) return;
+    __shared__ FusedLevel slv[LDS ? FUSE_MAX_LEVELS : 1];
+    const int nl = fc.nlev;
+    const FusedLevel* lv = fc.lv;
+    if (LDS) {
+        extern __shared__ char smem[];
+        char* p = smem;
+        for (int l = 0; l < nl; ++l) {
+            FusedLevel L = fc.lv[l];
+            const int64_t n = L.A.n;
+            L.A.slice_ptr = lds_copy_i32<T>(p, L.A.slice_ptr, (n + 63) / 64 + 1);
+            if (l < nl - 1) {
+                L.R.slice_ptr = lds_copy_i32<T>(p, L.R.slice_ptr, (L.R.n + 63) / 64 + 1);
+                L.P.slice_ptr = lds_copy_i32<T>(p, L.P.slice_ptr, (L.P.n + 63) / 64 + 1);
+            }
+            L.dinv = lds_vec<T>(p, n, L.dinv);
+            L.b = lds_vec<T>(p, n, l == 0 ? L.b : nullptr);
+            L.x = lds_vec<T>(p, n, nullptr);
+            L.x2 = lds_vec<T>(p, n, nullptr);
+            L.d = lds_vec<T>(p, n, nullptr);
+            L.r = lds_vec<T>(p, n, nullptr);
+            if (threadIdx.x == 0) slv[l] = L;
+        }
+        __syncthreads();
+        lv = slv;
+    }
+    T* res = fused_vcycle<T>(fc, lv);
     if (LDS) {
         T* o = static_cast<T*>(out);
-        for (int64_t i = threadIdx.x; i < lv[0].A.n; i += blockDim.x) o[i] = cur[0][i];
+        for (int64_t i = threadIdx.x; i < lv[0].A.n; i += blockDim.x) o[i] = res[i];
+    }
+}
+
+// ---- one-workgroup PCG solves (fa_pcg_wg_batched) ------------------------------
+// sum over the workgroup (FUSE_BLOCK threads) in a fixed tree; every thread
+// gets the result
+__device__ inline double wg_sum(double v, double* red) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    __syncthreads();
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    if (threadIdx.x < WAVE) {
+        double t = threadIdx.x < FUSE_BLOCK / WAVE ? red[threadIdx.x] : 0.0;
+        t = wave_sum(t);
+        if (threadIdx.x == 0) red[FUSE_BLOCK / WAVE] = t;
+    }
+    __syncthreads();
+    return red[FUSE_BLOCK / WAVE];
+}
+
+__global__ void __launch_bounds__(FUSE_BLOCK) k_pcg_wg(fa_wg_solve_t* __restrict__ reqs) {
+    __shared__ FusedCycle sfc;
+    __shared__ double red[FUSE_BLOCK / WAVE + 1];
+    fa_wg_solve_t& Q = reqs[blockIdx.x];
+    if (threadIdx.x == 0) {
+        sfc.nlev = Q.nlevels;
+        sfc.degree = Q.degree;
+        sfc.lower = Q.lower;
+        sfc.upper = Q.upper;
+        sfc.ainv = Q.coarse_inv;
+        for (int l = 0; l < Q.nlevels; ++l) {
+            const fa_wg_level_t& W = Q.levels[l];
+            sfc.lv[l] = FusedLevel{W.A, W.R, W.P, W.dinv, W.b, W.x, W.x2, W.d, W.r, W.lmax};
+        }
+    }
+    __syncthreads();
+    const fa_sell_t A = Q.A;
+    const int64_t n = A.n, npad = (n + 63) & ~63LL;
+    const double* sv = static_cast<const double*>(A.vals);
+    const double* b = Q.b;
+    double* x = Q.x;
+    double* r = Q.r;
+    double* p = Q.p;
+    double* Ap = Q.Ap;
+    float* ru = static_cast<float*>(sfc.lv[0].b);
+    const double atol = Q.atol, rtol = Q.rtol;
+    const long long maxit = Q.maxit;
+    // r = b - A x, its FP32 copy, ||b||
+    double bb = 0.0;
+    for (int64_t row = threadIdx.x; row < npad; row += blockDim.x) {
+        const double ax = sell_row_dot<double, double>(A.slice_ptr, A.cols, sv, row, x);
+        if (row < n) {
+            const double bv = b[row], rv = bv - ax;
+            r[row] = rv;
+            ru[row] = static_cast<float>(rv);
+            bb += bv * bv;
+        }
+    }
+    const double bnorm = sqrt(wg_sum(bb, red));
+    long long niter = 0;
+    double rn = 0.0;
+    int status = 0;
+    if (bnorm < 1e-15) {
+        for (int64_t i = threadIdx.x; i < n; i += blockDim.x) x[i] = 0.0;
+        status = 2;
+    } else {
+        const float* z = fused_vcycle<float>(sfc, sfc.lv);
+        double s = 0.0;
+        for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+            const double zi = static_cast<double>(z[i]);
+            p[i] = zi;
+            s += r[i] * zi;
+        }
+        double rz = wg_sum(s, red);
+        for (;;) {
+            // Ap = A p and p . Ap
+            s = 0.0;
+            for (int64_t row = threadIdx.x; row < npad; row += blockDim.x) {
+                const double v = sell_row_dot<double, double>(A.slice_ptr, A.cols, sv, row, p);
+                if (row < n) {
+                    Ap[row] = v;
+                    s += p[row] * v;
+                }
+            }
+            const double alpha = rz / wg_sum(s, red);
+            for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+                x[i] += alpha * p[i];
+                const double rv = r[i] - alpha * Ap[i];
+                r[i] = rv;
+                ru[i] = static_cast<float>(rv);
+            }
+            __syncthreads();
+            z = fused_vcycle<float>(sfc, sfc.lv);
+            s = 0.0;
+            for (int64_t i = threadIdx.x; i < n; i += blockDim.x) s += r[i] * static_cast<double>(z[i]);
+            const double rzn = wg_sum(s, red);
+            ++niter;
+            rn = sqrt(rzn);
+            if (rn < atol || rn < rtol * bnorm) { status = 1; break; }
+            if (niter >= maxit) { status = 3; break; }
+            if (!isfinite(rn) || !isfinite(alpha)) { status = 4; break; }
+            const double beta = rzn / rz;
+            rz = rzn;
+            for (int64_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = static_cast<double>(z[i]) + beta * p[i];
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x == 0) {
+        Q.niter = niter;
+        Q.residual = rn;
+        Q.status = status;
     }
 }
 
@@ -1225,6 +1349,7 @@ int mg_vcycle(const fa_mg_t* mg, const void* r, void* z, const int* done, hipStr
 // PCG (outer dtype T, cycle dtype U)
 struct PcgState {
     double rz, pAp, alpha, beta, rnorm, bnorm, atol, rtol;
+    double rnorm0;  // sqrt(r.z) of the first direction: the start of the host's rate estimate
     long long niter, maxit;
     int done;  // 0 running, 1 converged, 2 ||b|| < 1e-15, 3 maxit, 4 breakdown
 };
@@ -1245,11 +1370,13 @@ __global__ void __launch_bounds__(RED_BLOCK) k_pcg_finish(const double* __restri
     if (threadIdx.x != 0) return;
     if (WHAT == PCG_INIT) {
         st->rz = t;  // replaced by r.z when preconditioned
+        st->rnorm0 = sqrt(t);
         st->bnorm = sqrt(t2);
         st->niter = 0;
         st->done = (st->bnorm < 1e-15) ? 2 : 0;
     } else if (WHAT == PCG_RZ0) {
         st->rz = t;
+        st->rnorm0 = sqrt(t);
     } else {
         const double rn = sqrt(t);
         st->rnorm = rn;
@@ -1422,6 +1549,17 @@ static int pcg_capture(PcgGraph* pg, F body) {
     return FA_OK;
 }
 
+// iterations until sqrt(r.z) reaches max(atol, rtol ||b||) at the mean rate
+// since the first direction, clamped to [1, most]
+static int64_t predicted_chunk(const PcgState& h, int64_t most) {
+    const double tol = std::max(h.atol, h.rtol * h.bnorm);
+    if (!(h.niter > 0 && h.rnorm > 0.0 && h.rnorm0 > h.rnorm && tol > 0.0)) return most;
+    const double rate = std::log(h.rnorm / h.rnorm0) / static_cast<double>(h.niter);  // < 0
+    const double need = std::log(tol / h.rnorm) / rate;
+    if (!std::isfinite(need)) return most;
+    return std::max<int64_t>(1, std::min<int64_t>(most, static_cast<int64_t>(std::ceil(need))));
+}
+
 template <class T, class U>
 static int pcg_run(const fa_sell_t* A, const fa_mg_t* M, const T* b, T* x, double atol, double rtol, int64_t maxit,
                    int check_every, int64_t* niter_host, double* res_host, void* work, size_t work_bytes,
@@ -1494,10 +1632,16 @@ static int pcg_run(const fa_sell_t* A, const fa_mg_t* M, const T* b, T* x, doubl
         int e = pcg_capture(pg, iteration);
         if (e) return e;
     }
+    // The host looks at the state after `next` more iterations: check_every
+    // at first, then the count that the mean reduction rate so far predicts
+    // to reach the tolerance (at least 1, at most check_every).  Iterations
+    // launched past convergence return at once but still cost their launches
+    // (about 30 a V-cycle), so a good guess saves them; the iterates and the
+    // result do not depend on when the host looks.
     PcgState h{};
-    int64_t it = 0;
+    int64_t it = 0, next = check_every;
     for (;;) {
-        const int64_t chunk = std::min<int64_t>(check_every, cap - it);
+        const int64_t chunk = std::min<int64_t>(next, cap - it);
         for (int64_t k = 0; k < chunk; ++k) {
             if (pg) {
                 FA_HIP(hipGraphLaunch(pg->exec, s));
@@ -1510,6 +1654,7 @@ static int pcg_run(const fa_sell_t* A, const fa_mg_t* M, const T* b, T* x, doubl
         FA_HIP(hipMemcpyAsync(&h, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
         FA_HIP(hipStreamSynchronize(s));
         if (h.done || it >= cap) break;
+        next = predicted_chunk(h, check_every);
     }
     if (niter_host) *niter_host = h.niter;
     if (res_host) *res_host = h.rnorm;
@@ -1890,4 +2035,33 @@ extern "C" int fa_pcg_graph(int dtype, const fa_sell_t* A, const fa_mg_t* M, con
     FA_REQUIRE(graph && graph->g.cs, "fa_pcg_graph: no graph handle (fa_pcg_graph_create)");
     return pcg_dispatch(dtype, A, M, b, x, atol, rtol, maxit, check_every, niter_host, residual_host, work,
                         work_bytes, S(stream), &graph->g);
+}
+
+extern "C" int fa_pcg_wg_batched(int nreq, const fa_wg_solve_t* reqs_host, fa_wg_solve_t* reqs_dev,
+                                 fa_stream_t stream) {
+    FA_REQUIRE(nreq > 0 && reqs_host && reqs_dev, "fa_pcg_wg_batched: bad arguments");
+    static_assert(FA_WG_MAX_LEVELS == FUSE_MAX_LEVELS, "level capacity");
+    for (int q = 0; q < nreq; ++q) {
+        const fa_wg_solve_t& Q = reqs_host[q];
+        FA_REQUIRE(Q.A.n > 0 && Q.A.slice_ptr && Q.A.cols && Q.A.vals && Q.b && Q.x && Q.r && Q.p && Q.Ap &&
+                       Q.maxit > 0 && Q.nlevels >= 1 && Q.nlevels <= FA_WG_MAX_LEVELS && Q.degree >= 1 &&
+                       Q.coarse_inv,
+                   "fa_pcg_wg_batched: request %d: bad descriptor", q);
+        FA_REQUIRE(Q.levels[0].A.n == Q.A.n, "fa_pcg_wg_batched: request %d: the finest level has %lld rows, A %lld",
+                   q, (long long)Q.levels[0].A.n, (long long)Q.A.n);
+        for (int l = 0; l < Q.nlevels; ++l) {
+            const fa_wg_level_t& W = Q.levels[l];
+            FA_REQUIRE(W.A.n > 0 && W.A.slice_ptr && W.A.vals && W.dinv && W.b && W.x && W.x2 && W.d && W.r,
+                       "fa_pcg_wg_batched: request %d: level %d incomplete", q, l);
+            if (l + 1 < Q.nlevels)
+                FA_REQUIRE(W.R.n == Q.levels[l + 1].A.n && W.R.ncols == W.A.n && W.P.n == W.A.n &&
+                               W.P.ncols == Q.levels[l + 1].A.n && W.R.vals && W.P.vals,
+                           "fa_pcg_wg_batched: request %d: transfers of level %d do not match", q, l);
+        }
+        FA_REQUIRE(Q.levels[Q.nlevels - 1].A.n <= COARSE_MAX,
+                   "fa_pcg_wg_batched: request %d: coarsest level above %d rows", q, COARSE_MAX);
+    }
+    hipLaunchKernelGGL(k_pcg_wg, dim3(nreq), dim3(FUSE_BLOCK), 0, S(stream), reqs_dev);
+    FA_LAUNCH("k_pcg_wg");
+    return FA_OK;
 }

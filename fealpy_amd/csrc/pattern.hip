@@ -235,3 +235,83 @@ extern "C" int fa_pattern_fill(const int32_t* cell2dof, int64_t nc, int32_t ldof
     return dispatch_rows((int64_t)max_degree * ldof, cell2dof, ldof, gdof, adj_ptr, adj, nullptr, indptr, indices,
                          nullptr, S(stream));
 }
+
+// ---- P1 pattern of a from_box tetrahedral grid ------------------------------
+// TetrahedronMesh.from_box numbers node (ix, iy, iz) as (ix (ny+1) + iy)(nz+1)
+// + iz; two nodes share a Kuhn cell exactly when their offset is 0 or one of
+// the 14 directions +-(1,0,0), +-(0,1,0), +-(0,0,1), +-(1,1,0), +-(1,0,1),
+// +-(0,1,1), +-(1,1,1) and both lie in the grid, and lexicographic offset
+// order is ascending column order.  So the CSR pattern (the one the generic
+// kernels build from the incidence lists) follows from the grid alone.
+namespace fa {
+__device__ inline bool box_edge(int dx, int dy, int dz) {
+    // 0 or a Kuhn edge direction: all nonzero components of one sign
+    const bool pos = dx >= 0 && dy >= 0 && dz >= 0, neg = dx <= 0 && dy <= 0 && dz <= 0;
+    return pos || neg;
+}
+
+template <bool FILL>
+__global__ void __launch_bounds__(256) k_box_pattern_rows(int nx, int ny, int nz, int32_t* __restrict__ rowcount,
+                                                         const int32_t* __restrict__ indptr,
+                                                         int32_t* __restrict__ indices, int64_t nnz) {
+    const int64_t nz1 = nz + 1, nyz = static_cast<int64_t>(ny + 1) * nz1, nn = static_cast<int64_t>(nx + 1) * nyz;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nn; r += (int64_t)gridDim.x * blockDim.x) {
+        const int ix = static_cast<int>(r / nyz);
+        const int64_t rem = r - ix * nyz;
+        const int iy = static_cast<int>(rem / nz1);
+        const int iz = static_cast<int>(rem - iy * nz1);
+        int32_t k = FILL ? indptr[r] : 0;
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx)
+#pragma unroll
+            for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                for (int dz = -1; dz <= 1; ++dz) {
+                    if (!box_edge(dx, dy, dz)) continue;
+                    if (ix + dx < 0 || ix + dx > nx || iy + dy < 0 || iy + dy > ny || iz + dz < 0 || iz + dz > nz)
+                        continue;
+                    if (FILL && k < nnz) indices[k] = static_cast<int32_t>(r + dx * nyz + dy * nz1 + dz);
+                    ++k;
+                }
+        if (!FILL) rowcount[r] = k;
+    }
+}
+}  // namespace fa
+
+extern "C" size_t fa_p1_box_pattern_work_bytes(int nx, int ny, int nz) {
+    const int64_t nn = static_cast<int64_t>(nx + 1) * (ny + 1) * (nz + 1);
+    return align_up((nn + 1) * sizeof(int32_t)) + exclusive_scan_work_bytes(nn) + 1024;
+}
+
+extern "C" int fa_p1_box_pattern(int nx, int ny, int nz, int32_t* indptr, int32_t* indices, int64_t nnz, void* work,
+                                 size_t work_bytes, fa_stream_t stream) {
+    FA_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1 && indptr && indices, "fa_p1_box_pattern: bad arguments");
+    const int64_t nn = static_cast<int64_t>(nx + 1) * (ny + 1) * (nz + 1);
+    FA_REQUIRE(nn < INT32_MAX, "fa_p1_box_pattern: grid too large for int32 indices");
+    int64_t want = nn;
+    {
+        const int o[7][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+        for (const auto& e : o) want += 2 * static_cast<int64_t>(nx + 1 - e[0]) * (ny + 1 - e[1]) * (nz + 1 - e[2]);
+    }
+    FA_REQUIRE(nnz == want && want < INT32_MAX, "fa_p1_box_pattern: %lld nonzeros given, the grid has %lld",
+               (long long)nnz, (long long)want);
+    hipStream_t s = S(stream);
+    Arena ar(work, work_bytes);
+    int32_t* rowcount = ar.take<int32_t>(nn + 1);
+    const size_t scan_bytes = exclusive_scan_work_bytes(nn);
+    void* scan_ws = ar.take<char>(scan_bytes);
+    if (!rowcount || !scan_ws) {
+        set_error("fa_p1_box_pattern: workspace too small (%zu bytes)", work_bytes);
+        return FA_ENOMEM;
+    }
+    const unsigned grid = grid_for(nn, 256, 1 << 20);
+    hipLaunchKernelGGL(k_box_pattern_rows<false>, dim3(grid), dim3(256), 0, s, nx, ny, nz, rowcount, nullptr, nullptr,
+                       nnz);
+    FA_LAUNCH("k_box_pattern_rows");
+    int st = exclusive_scan_i32(rowcount, indptr, nn, scan_ws, scan_bytes, s);
+    if (st != FA_OK) return st;
+    hipLaunchKernelGGL(k_box_pattern_rows<true>, dim3(grid), dim3(256), 0, s, nx, ny, nz, nullptr, indptr, indices,
+                       nnz);
+    FA_LAUNCH("k_box_pattern_rows");
+    return FA_OK;
+}

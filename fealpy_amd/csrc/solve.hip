@@ -191,6 +191,40 @@ __global__ void __launch_bounds__(256) k_bc_apply(int64_t n, const int32_t* __re
     }
 }
 
+// Pattern-preserving form of k_bc_apply: the values stay on A's pattern, a
+// boundary row becomes the unit row (1 on the diagonal, 0 elsewhere) and a
+// boundary column of an interior row becomes 0; f - A g is formed in the same
+// lane order as k_bc_apply, so it equals that kernel's bit for bit.  Each
+// entry is read and written by the same lane, so nval may be vals itself.
+template <class T>
+__global__ void __launch_bounds__(256) k_bc_keep(int64_t n, const int32_t* __restrict__ indptr,
+                                                const int32_t* __restrict__ indices, const T* vals,
+                                                const uint8_t* __restrict__ bd, T* nval, const T* __restrict__ uh,
+                                                const T* __restrict__ f, T* __restrict__ out) {
+    const int lane = threadIdx.x % BCG;
+    for (int64_t r = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / BCG; r < n;
+         r += (int64_t)gridDim.x * blockDim.x / BCG) {
+        const int32_t k1 = indptr[r + 1];
+        if (bd[r]) {
+            for (int32_t k = indptr[r] + lane; k < k1; k += BCG) nval[k] = indices[k] == r ? T(1) : T(0);
+            if (lane == 0) out[r] = uh[r];
+            continue;
+        }
+        T acc = T(0);
+        for (int32_t k0 = indptr[r]; k0 < k1; k0 += BCG) {
+            const int32_t k = k0 + lane;
+            if (k < k1) {
+                const int32_t c = indices[k];
+                const T v = vals[k];
+                acc += v * uh[c];
+                nval[k] = bd[c] ? T(0) : v;
+            }
+        }
+        acc = group_sum<BCG>(acc);
+        if (lane == 0) out[r] = f[r] - acc;
+    }
+}
+
 // ---- vector kernels --------------------------------------------------------------
 template <class T>
 __global__ void __launch_bounds__(RED_BLOCK) k_dot(int64_t n, const T* __restrict__ x, const T* __restrict__ y,
@@ -481,5 +515,28 @@ extern "C" int fa_csr_residual(int dtype_y, int64_t n, const int32_t* indptr, co
     FA_LAUNCH("k_csr_residual");
     hipLaunchKernelGGL(k_residual_finish, dim3(1), dim3(RED_BLOCK), 0, s, part, static_cast<int>(grid), norm2);
     FA_LAUNCH("k_residual_finish");
+    return FA_OK;
+}
+
+extern "C" int fa_dirichlet_apply_keep(int dtype, int64_t n, const int32_t* indptr, const int32_t* indices,
+                                       const void* values, const uint8_t* is_bd, void* new_values, const void* uh,
+                                       const void* f, void* f_out, fa_stream_t stream) {
+    FA_REQUIRE(dtype == FA_F64 || dtype == FA_F32, "fa_dirichlet_apply_keep: bad dtype %d", dtype);
+    FA_REQUIRE(n >= 0 && indptr && indices && values && is_bd && new_values && uh && f && f_out,
+               "fa_dirichlet_apply_keep: bad arguments");
+    if (n == 0) return FA_OK;
+    hipStream_t s = S(stream);
+    const unsigned grid = grid_for(n * BCG, 256, 1 << 16);
+    timer_begin(FA_TIMER_BC_APPLY, s);
+    if (dtype == FA_F64)
+        hipLaunchKernelGGL(k_bc_keep<double>, dim3(grid), dim3(256), 0, s, n, indptr, indices,
+                           static_cast<const double*>(values), is_bd, static_cast<double*>(new_values),
+                           static_cast<const double*>(uh), static_cast<const double*>(f), static_cast<double*>(f_out));
+    else
+        hipLaunchKernelGGL(k_bc_keep<float>, dim3(grid), dim3(256), 0, s, n, indptr, indices,
+                           static_cast<const float*>(values), is_bd, static_cast<float*>(new_values),
+                           static_cast<const float*>(uh), static_cast<const float*>(f), static_cast<float*>(f_out));
+    timer_end(FA_TIMER_BC_APPLY, s);
+    FA_LAUNCH("k_bc_keep");
     return FA_OK;
 }

@@ -58,13 +58,23 @@ class DirichletBC:
             raise ValueError('The vector size must match the gdof of the space.')
         return vector
 
-    def apply(self, A, f, uh=None, gd=None, *, check=True):
+    def apply(self, A, f, uh=None, gd=None, *, check=True, keep_pattern=False):
         """-> (A with boundary rows/columns removed and a unit boundary
         diagonal, f - A g with f[bd] = g[bd]); one pass over A
-        (fa_dirichlet_apply), the results of apply_vector + apply_matrix."""
+        (fa_dirichlet_apply), the results of apply_vector + apply_matrix.
+
+        keep_pattern=True (not in the reference): the pattern-preserving
+        mode.  The returned CSR matrix keeps A's row pointer and column
+        tensors, with the unit row on boundary rows and 0 on boundary columns
+        (fa_dirichlet_apply_keep); same solution and bitwise the same right-
+        hand side as the filtered form, and a later boundary condition on the
+        same pattern reuses everything built on it (multigrid symbolic
+        products, SELL index)."""
         A0 = self.check_matrix(A) if check else A
         f = self.check_vector(f) if check else f
         if f.ndim != 1:
+            if keep_pattern:
+                raise ValueError("keep_pattern takes a 1-D right-hand side")
             return self.apply_matrix(A0, check=False), self.apply_vector(f, A0, uh, gd, check=False)
         gd = self.gd if gd is None else gd
         if gd is None:
@@ -73,6 +83,16 @@ class DirichletBC:
         Ac = self._csr(A0)
         g = self._boundary_values(f, uh, gd, Ac.dtype)
         f = f.to(Ac.dtype).contiguous()
+        if keep_pattern:
+            vals = Ac.values.contiguous()
+            new_val = torch.empty_like(vals)
+            out = torch.empty_like(f)
+            nat.check(nat.lib().fa_dirichlet_apply_keep(nat.dtype_code(Ac.dtype), Ac.shape[0], nat.ptr(Ac.crow),
+                                                        nat.ptr(Ac.col), nat.ptr(vals), nat.ptr(self._bd_u8),
+                                                        nat.ptr(new_val), nat.ptr(g), nat.ptr(f), nat.ptr(out),
+                                                        nat.stream()), "fa_dirichlet_apply_keep")
+            A2 = CSRTensor(Ac.crow, Ac.col, new_val, Ac.shape)
+            return (A2.tocoo() if was_coo else A2), out
         new_ptr, new_idx = self._filtered_pattern(Ac)
         new_val = torch.empty(new_idx.shape[0], dtype=Ac.dtype, device=Ac.device)
         out = torch.empty_like(f)

@@ -330,6 +330,23 @@ def p1_fusable_ops(it):
 # per row over the incidence slots); smaller ones use the single fused kernel,
 # whose one launch and no slot buffer cost less when the mesh is small.
 TWO_PASS_MIN_CELLS = 65536
+# P1 matrices and load vectors of TetrahedronMesh.from_box meshes take the
+# structured kernels (fa_p1_box_matrix, fa_p1_box_vector); False routes them
+# to the fused / two-pass kernels like any other mesh
+BOX_ASSEMBLY = True
+
+
+def _box_grid(space, pat):
+    """(nx, ny, nz) when the P1 matrix can take the structured one-pass kernel:
+    a TetrahedronMesh.from_box mesh (no threshold) and the space's own pattern."""
+    mesh = space.mesh
+    fb = mesh.meshdata.get('from_box')
+    if not BOX_ASSEMBLY or fb is None or fb[0] != 'tet' or space.p != 1:
+        return None
+    nx, ny, nz = fb[2], fb[3], fb[4]
+    if pat.gdof != (nx + 1) * (ny + 1) * (nz + 1) or pat.nc != 6 * nx * ny * nz or pat.nc != mesh.number_of_cells():
+        return None
+    return nx, ny, nz
 
 
 def p1_matrix_values(space, ops, pat, dtype):
@@ -344,6 +361,12 @@ def p1_matrix_values(space, ops, pat, dtype):
         spec[op] = coef_spec(it.coef, space, q, allow_sep=True)
     (dk, ds, dc), (mk, ms, mc) = spec['diff'], spec['mass']
     values = torch.empty(pat.nnz, dtype=dtype, device=space.device)
+    grid = _box_grid(space, pat)
+    if grid is not None:         # from_box tetrahedra: one pass, cells found from each node's grid position
+        nat.check(L.fa_p1_box_matrix(nat.dtype_code(dtype), *grid, nat.ptr(mesh.node_rows16()), qw.shape[0],
+                                     nat.ptr(qw), nat.ptr(phi), dk, ds, nat.ptr(dc), mk, ms, nat.ptr(mc),
+                                     nat.ptr(pat.indptr), pat.nnz, nat.ptr(values), nat.stream()), "fa_p1_box_matrix")
+        return values
     maps = pat.incidence_maps() if pat.nc >= TWO_PASS_MIN_CELLS else None
     if maps is not None:         # pass 1 per cell into incidence slots, pass 2 streams each row's slots
         # an FP32 matrix keeps its element rows in FP32 (half the slot traffic)
@@ -378,6 +401,13 @@ def p1_vector_values(space, it, pat, dtype):
     if it.source is None:
         kind, scale, src = nat.COEF_CONST, 1.0, None
     out = torch.empty(pat.gdof, dtype=dtype, device=space.device)
+    grid = _box_grid(space, pat)
+    if grid is not None:         # from_box tetrahedra: cell-order element vectors, then a gather per node by grid position
+        work = nat.workspace(L.fa_p1_box_vector_work_bytes(*grid), space.device)
+        nat.check(L.fa_p1_box_vector(nat.dtype_code(dtype), *grid, nat.ptr(mesh.node_rows16()), qw.shape[0],
+                                     nat.ptr(qw), nat.ptr(phi), kind, scale, nat.ptr(src), nat.ptr(out),
+                                     nat.ptr(work), work.numel(), nat.stream()), "fa_p1_box_vector")
+        return out
     maps = pat.incidence_maps() if pat.nc >= TWO_PASS_MIN_CELLS else None
     if maps is not None:         # one value per incidence slot, then an ordered sum per row
         vals = torch.empty(pat.nc * pat.ldof, dtype=torch.float64, device=space.device)

@@ -10,24 +10,51 @@ import torch
 from .. import _native as nat
 
 
-class Pattern:
-    __slots__ = ("cell2dof", "gdof", "ldof", "nc", "adj_ptr", "adj", "max_degree", "indptr", "indices", "nnz",
-                 "max_row", "_maps", "_maps_lock")
+def _box_counts(nx, ny, nz):
+    """(nnz, longest row) of the P1 pattern of an nx x ny x nz Kuhn grid: a
+    node's row holds itself and its neighbours along the 14 Kuhn edge
+    directions that lie in the grid (each axis: low end, inside, high end)."""
+    dirs = [(a, b, c) for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1)
+            if all(v >= 0 for v in (a, b, c)) or all(v <= 0 for v in (a, b, c))]
+    nnz, longest = 0, 0
+    for kx in ((0, 1), (1, nx - 1), (nx, 1)) if nx > 1 else ((0, 1), (1, 1)):
+        for ky in ((0, 1), (1, ny - 1), (ny, 1)) if ny > 1 else ((0, 1), (1, 1)):
+            for kz in ((0, 1), (1, nz - 1), (nz, 1)) if nz > 1 else ((0, 1), (1, 1)):
+                row = sum(1 for a, b, c in dirs if 0 <= kx[0] + a <= nx and 0 <= ky[0] + b <= ny
+                          and 0 <= kz[0] + c <= nz)
+                nnz += row * kx[1] * ky[1] * kz[1]
+                longest = max(longest, row)
+    return nnz, longest
 
-    def __init__(self, cell2dof, gdof):
+
+class Pattern:
+    __slots__ = ("cell2dof", "gdof", "ldof", "nc", "_adj_ptr", "_adj", "_max_degree", "indptr", "indices", "nnz",
+                 "max_row", "_maps", "_maps_lock", "_adj_lock")
+
+    def __init__(self, cell2dof, gdof, box=None):
+        """box: (nx, ny, nz) of a TetrahedronMesh.from_box P1 map, whose CSR
+        pattern comes from the grid (fa_p1_box_pattern); the incidence lists
+        are then built on first use."""
         L = nat.lib()
         c2d = cell2dof.to(torch.int32).contiguous()
         nc, ldof = c2d.shape
         dev = c2d.device
         self.cell2dof, self.gdof, self.ldof, self.nc = c2d, int(gdof), int(ldof), int(nc)
-        self.adj_ptr = torch.empty(gdof + 1, dtype=torch.int32, device=dev)
-        self.adj = torch.empty(max(nc * ldof, 1), dtype=torch.int32, device=dev)
-        ws = nat.workspace(L.fa_adjacency_work_bytes(nc, ldof, gdof), dev)
-        maxdeg = nat.ctypes.c_int32(0)
-        nat.check(L.fa_adjacency_build(nat.ptr(c2d), nc, ldof, gdof, nat.ptr(self.adj_ptr), nat.ptr(self.adj),
-                                       nat.ctypes.byref(maxdeg), nat.ptr(ws), ws.numel(), nat.stream()),
-                  "fa_adjacency_build")
-        self.max_degree = int(maxdeg.value)
+        self._adj_ptr = self._adj = self._max_degree = None
+        self._adj_lock = threading.Lock()
+        self._maps = None
+        self._maps_lock = threading.Lock()
+        if box is not None:
+            nx, ny, nz = box
+            self.nnz, self.max_row = _box_counts(nx, ny, nz)
+            self.indptr = torch.empty(gdof + 1, dtype=torch.int32, device=dev)
+            self.indices = torch.empty(max(self.nnz, 1), dtype=torch.int32, device=dev)
+            ws = nat.workspace(L.fa_p1_box_pattern_work_bytes(nx, ny, nz), dev)
+            nat.check(L.fa_p1_box_pattern(nx, ny, nz, nat.ptr(self.indptr), nat.ptr(self.indices), self.nnz,
+                                          nat.ptr(ws), ws.numel(), nat.stream()), "fa_p1_box_pattern")
+            self.indices = self.indices[:self.nnz]
+            return
+        self._build_adj()
         self.indptr = torch.empty(gdof + 1, dtype=torch.int32, device=dev)
         ws = nat.workspace(L.fa_pattern_work_bytes(gdof), dev)
         nnz = nat.ctypes.c_int64(0)
@@ -42,8 +69,41 @@ class Pattern:
                                     self.max_degree, nat.ptr(self.indptr), nat.ptr(self.indices), nat.stream()),
                   "fa_pattern_fill")
         self.indices = self.indices[:self.nnz]
-        self._maps = None
-        self._maps_lock = threading.Lock()
+
+    def _build_adj(self):
+        L = nat.lib()
+        c2d, nc, ldof, gdof = self.cell2dof, self.nc, self.ldof, self.gdof
+        dev = c2d.device
+        adj_ptr = torch.empty(gdof + 1, dtype=torch.int32, device=dev)
+        adj = torch.empty(max(nc * ldof, 1), dtype=torch.int32, device=dev)
+        ws = nat.workspace(L.fa_adjacency_work_bytes(nc, ldof, gdof), dev)
+        maxdeg = nat.ctypes.c_int32(0)
+        nat.check(L.fa_adjacency_build(nat.ptr(c2d), nc, ldof, gdof, nat.ptr(adj_ptr), nat.ptr(adj),
+                                       nat.ctypes.byref(maxdeg), nat.ptr(ws), ws.numel(), nat.stream()),
+                  "fa_adjacency_build")
+        self._adj_ptr, self._adj, self._max_degree = adj_ptr, adj, int(maxdeg.value)
+
+    def _adjacency(self):
+        """Incidence lists on first use (patterns built from the grid), complete
+        on the building stream before any other stream or thread sees them."""
+        if self._adj is None:
+            with self._adj_lock:
+                if self._adj is None:
+                    self._build_adj()
+                    torch.cuda.current_stream(self.cell2dof.device).synchronize()
+        return self._adj_ptr, self._adj, self._max_degree
+
+    @property
+    def adj_ptr(self):
+        return self._adjacency()[0]
+
+    @property
+    def adj(self):
+        return self._adjacency()[1]
+
+    @property
+    def max_degree(self):
+        return self._adjacency()[2]
 
     def incidence_maps(self):
         """(inc, pos) of the two-pass P1 assembly (fa_incidence_maps): the
@@ -102,6 +162,20 @@ class Pattern:
 
 
 _LOCK = threading.Lock()
+# P1 patterns of TetrahedronMesh.from_box meshes come from the grid
+# (fa_p1_box_pattern); False builds them from the incidence lists like any other
+BOX_PATTERN = True
+
+
+def _box_grid_of(space):
+    mesh = space.mesh
+    fb = mesh.meshdata.get('from_box')
+    if not BOX_PATTERN or fb is None or fb[0] != 'tet' or space.p != 1:
+        return None
+    nx, ny, nz = fb[2], fb[3], fb[4]
+    if mesh.number_of_cells() != 6 * nx * ny * nz or space.number_of_global_dofs() != (nx + 1) * (ny + 1) * (nz + 1):
+        return None
+    return nx, ny, nz
 
 
 def pattern_of(space, cell2dof=None):
@@ -112,7 +186,7 @@ def pattern_of(space, cell2dof=None):
         if getattr(space, "_pattern", None) is None:
             with _LOCK:
                 if getattr(space, "_pattern", None) is None:
-                    pat = Pattern(space.cell_to_dof(), space.number_of_global_dofs())
+                    pat = Pattern(space.cell_to_dof(), space.number_of_global_dofs(), box=_box_grid_of(space))
                     torch.cuda.current_stream(pat.indptr.device).synchronize()
                     space._pattern = pat
         return space._pattern

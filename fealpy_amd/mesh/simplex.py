@@ -13,6 +13,7 @@ global DOF index downstream depends on it:
   cell-interior points; local order = multi-index order
   (triangle_mesh.py:218-270, tetrahedron_mesh.py:348-441, mesh_base.py:188-211).
 """
+import threading
 from math import comb
 
 import numpy as np
@@ -99,6 +100,66 @@ def _gdof(p, nums):
     return total
 
 
+_TOPOLOGY = frozenset(('face', 'edge', 'cell2face', 'face2cell', 'cell2edge', 'edge2cell'))
+_BOX_FACE_TAGS = ('leftface', 'rightface', 'frontface', 'backface', 'bottomface', 'upface')
+
+
+class _MeshData(dict):
+    """mesh.meshdata: a dict whose from_box face tags (the boundary faces of
+    each side of the box, which need the face numbering) are computed on
+    first access by any dict operation, so a mesh that never asks for them
+    never builds its faces."""
+
+    def __init__(self):
+        super().__init__()
+        self._tagger = None
+
+    def _fill(self):
+        tag, self._tagger = self._tagger, None
+        if tag is not None:
+            tag()
+
+    def __missing__(self, key):
+        if key in _BOX_FACE_TAGS and self._tagger is not None:
+            self._fill()
+            return dict.__getitem__(self, key)
+        raise KeyError(key)
+
+    def get(self, key, default=None):
+        if key in _BOX_FACE_TAGS:
+            self._fill()
+        return dict.get(self, key, default)
+
+    def __contains__(self, key):
+        if key in _BOX_FACE_TAGS:
+            self._fill()
+        return dict.__contains__(self, key)
+
+    def keys(self):
+        self._fill()
+        return dict.keys(self)
+
+    def items(self):
+        self._fill()
+        return dict.items(self)
+
+    def values(self):
+        self._fill()
+        return dict.values(self)
+
+    def __iter__(self):
+        self._fill()
+        return dict.__iter__(self)
+
+    def __len__(self):
+        self._fill()
+        return dict.__len__(self)
+
+    def __repr__(self):
+        self._fill()
+        return dict.__repr__(self)
+
+
 class SimplexMesh:
     TD = None
     localEdge = None
@@ -115,9 +176,25 @@ class SimplexMesh:
         self.itype = cell.dtype
         self.ftype = node.dtype
         self.GD = node.shape[1]
-        self.nodedata, self.edgedata, self.facedata, self.celldata, self.meshdata = {}, {}, {}, {}, {}
+        self.nodedata, self.edgedata, self.facedata, self.celldata = {}, {}, {}, {}
+        self.meshdata = _MeshData()
         self._cache = {}
-        self.construct()
+        self._topo_lock = threading.Lock()
+
+    def __getattr__(self, name):
+        # faces, edges and their cell maps are built on first use (construct);
+        # a P1 solve on a from_box grid needs none of them
+        d = self.__dict__
+        if name in _TOPOLOGY and '_topo_lock' in d:
+            with d['_topo_lock']:
+                if not d.get('_constructed', False):
+                    self.construct()
+                    d['_constructed'] = True
+                    if self.cell.is_cuda:   # complete before another stream or thread reads it
+                        torch.cuda.current_stream(self.cell.device).synchronize()
+            if name in d:
+                return d[name]
+        raise AttributeError(f"{type(self).__name__!r} object has no attribute {name!r}")
 
     def node_rows16(self):
         """Vertex coordinates as 16-byte-aligned rows for wide loads in the
@@ -138,6 +215,7 @@ class SimplexMesh:
 
     # -- topology ---------------------------------------------------------------
     def construct(self):
+        self.__dict__['_constructed'] = True
         cell = self.cell
         NC = cell.shape[0]
         if cell.is_cuda and NC > 0:
@@ -269,6 +347,8 @@ class SimplexMesh:
         return 1 if d == 0 else comb(p + d, d)
 
     def number_of_global_ipoints(self, p):
+        if p == 1:
+            return self.number_of_nodes()
         nums = [self.number_of_nodes(), self.number_of_edges()]
         if self.TD == 3:
             nums.append(self.number_of_faces())
@@ -301,11 +381,26 @@ class SimplexMesh:
         return torch.cat(parts, dim=0)[index]
 
     def is_boundary_dof(self, p):
+        grid = self._box_grid()
+        if p == 1 and grid is not None:
+            # the nodes on the box's surface: those on a boundary face of the
+            # full grid, from the grid position (no face numbering needed)
+            idx = [torch.arange(n + 1, device=self.device) for n in grid]
+            axes = torch.meshgrid(*idx, indexing='ij')
+            flag = torch.zeros(axes[0].shape, dtype=torch.bool, device=self.device)
+            for a, n in zip(axes, grid):
+                flag |= (a == 0) | (a == n)
+            return flag.reshape(-1)
         gdof = self.number_of_global_ipoints(p)
         f2d = self.face_to_ipoint(p)[self.boundary_face_index()]
         flag = torch.zeros(gdof, dtype=torch.bool, device=self.device)
         flag[f2d.reshape(-1).long()] = True
         return flag
+
+    def _box_grid(self):
+        """Cell counts per axis of a full from_box grid, else None."""
+        fb = dict.get(self.meshdata, 'from_box')
+        return None if fb is None else tuple(fb[2:])
 
     # -- L2 error (reporting; fealpy/mesh/mesh_base.py:545-608) ------------------
     def to_vtk(self, fname=None, etype='cell', index=_S):
@@ -480,7 +575,7 @@ class TetrahedronMesh(SimplexMesh):
         mesh.box = list(box)
         if threshold is None:           # the full structured grid (geometric multigrid reads it)
             mesh.meshdata['from_box'] = ('tet', tuple(box), nx, ny, nz)
-            mesh._tag_box_faces()
+            mesh.meshdata._tagger = mesh._tag_box_faces       # the side tags on first access
         return mesh
 
     def _tag_box_faces(self):
@@ -488,7 +583,7 @@ class TetrahedronMesh(SimplexMesh):
         n = self.face_unit_normal(bd)
         for name, (k, s) in dict(leftface=(0, -1), rightface=(0, 1), frontface=(1, -1), backface=(1, 1),
                                  bottomface=(2, -1), upface=(2, 1)).items():
-            self.meshdata[name] = bd[torch.abs(n[:, k] - s) < 1e-14]
+            dict.__setitem__(self.meshdata, name, bd[torch.abs(n[:, k] - s) < 1e-14])
 
     def face_unit_normal(self, index=_S):
         f = self.face[index].long()

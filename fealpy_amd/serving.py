@@ -40,7 +40,7 @@ from .functionspace import LagrangeFESpace
 from .mesh import TriangleMesh
 from .solver import GeometricMG, cg
 from .solver.cg import maxit_arg
-from .solver.mg import FaMg, FaMgLevel
+from .solver.mg import FaMg, FaMgLevel, pcg_workgroups
 from .sparse.sell import FaSell, FaSell16
 
 PI = math.pi
@@ -119,7 +119,9 @@ class _SizeContext:
                 self._pool = []
 
     def checkout(self, packed=False):
-        """A hierarchy from the pool (packed batches need uncompressed ones)."""
+        """A hierarchy from the pool; packed=True: an uncompressed one (FP32
+        operators on every level), as workgroup solves and packed batches
+        need."""
         pool = self._pool_packed if packed else self._pool
         with self._lock:
             self.inuse += 1
@@ -142,9 +144,7 @@ class SolveServer:
         self.device = torch.device("cuda") if device is None else torch.device(device)
         self.slots, self.budget, self.rtol, self.maxit, self.q = slots, int(hbm_budget), rtol, maxit, q
         # batch > 1: a slot takes up to `batch` waiting requests at once and
-        # solves those of equal hierarchy depth in one packed PCG
-        if int(batch) > 1 and not PACKED_BATCHES_ENABLED:
-            raise ValueError(PACKED_BATCHES_OFF)
+        # solves the small ones in one launch, one workgroup each
         self.batch = max(1, min(int(batch), MAX_BATCH))
         self.mg_kwargs = {"dtype": mg_dtype, "graphs": graphs}
         # resident per-size data (meshes, patterns, pooled hierarchies) is
@@ -246,40 +246,38 @@ class SolveServer:
         return x, info
 
     def solve_batch(self, reqs):
-        """Solve requests together on the calling thread's current stream:
-        each one assembled, constrained and set up on its own, then those of
-        equal hierarchy depth in one packed PCG (fa_pcg_batched) per group of
-        at most MAX_BATCH.  Fills every req.x / req.info."""
-        if not PACKED_BATCHES_ENABLED:
-            raise RuntimeError(PACKED_BATCHES_OFF)
+        """Solve requests together on the calling thread's current stream.
+        Requests of at most WG_MAX_GDOF DOFs are each assembled, constrained
+        and set up on their own (uncompressed FP32 hierarchies), then solved
+        in one launch with one workgroup per request (pcg_workgroups: the
+        whole MG-PCG of a request inside its workgroup); larger requests are
+        solved one at a time.  Fills every req.x / req.info."""
         t = time.perf_counter()
-        for r in reqs:
+        small = [r for r in reqs if (r.p * r.nx + 1) ** 2 <= WG_MAX_GDOF]
+        big = [r for r in reqs if (r.p * r.nx + 1) ** 2 > WG_MAX_GDOF]
+        for r in small:
             r.start = t
-        work = []
+        work, systems = [], []
         try:
-            for r in reqs:
+            for r in small:
                 ctx = self.context(r.nx, r.p)
                 mg = ctx.checkout(packed=True)
+                work.append((r, ctx, mg))
                 A, F = self._system(r, ctx, mg)
-                work.append((r, ctx, mg, F))
-            groups = {}
-            for w in work:
-                groups.setdefault(len(w[2].levels), []).append(w)
-            for items in groups.values():
-                for i0 in range(0, len(items), MAX_BATCH):
-                    part = items[i0:i0 + MAX_BATCH]
-                    pb = PackedBatch([w[2] for w in part], self.device)
-                    xs = [torch.zeros(w[3].shape[0], dtype=torch.float64, device=self.device) for w in part]
-                    out = pb.pcg([w[3] for w in part], xs, rtol=self.rtol, maxit=self.maxit)
-                    for w, x, (it, res, st) in zip(part, xs, out):
-                        w[0].x = x
-                        w[0].info = {'niter': it, 'residual': res, 'status': st, 'batch': len(part)}
+                systems.append((mg, A, F))
+            if systems:
+                out = pcg_workgroups(systems, atol=0.0, rtol=self.rtol, maxit=self.maxit)
+                for (r, _, _), (x, info) in zip(work, out):
+                    r.x = x
+                    r.info = dict(info, batch=len(systems))
         finally:
-            for _, ctx, mg, _ in work:
+            for _, ctx, mg in work:
                 ctx.checkin(mg)
         t = time.perf_counter()
-        for r in reqs:
+        for r in small:
             r.done = t
+        for r in big:
+            self.solve(r)
         return reqs
 
     def solve(self, req):
@@ -414,16 +412,19 @@ def _upload(items, device):
 
 # finest-level rows per PCG workgroup of fa_pcg_batched; coarse levels align to SELL slices
 _SEG0, _SEG = 256, 64
-# requests per packed batch: the packed coarsest level (64 rows a request)
-# keeps its dense inverse within 2048 rows
+# requests a slot takes at once (packed batches: the packed coarsest level,
+# 64 rows a request, keeps its dense inverse within 2048 rows)
 MAX_BATCH = 32
+# requests of at most this many DOFs are solved by one workgroup each in a
+# slot's batch; larger ones have enough rows to fill the GPU on their own
+WG_MAX_GDOF = 20000
 # Packed batches are switched off.  They agree with one-at-a-time solves on
 # small traces (40 requests, nx <= 128), but the saturated 4096-request trace
 # with 4 slots and 32-request batches stalled the GPU with no kernel fault,
 # and no cause has been found in the code yet (every loop of the batched PCG
 # and of the packed V-cycle is bounded; memory use near the device's capacity
 # is suspected but not shown).  Until a cause is shown and fixed, PackedBatch
-# and SolveServer(batch > 1) raise instead of running.
+# raises instead of running; a server's batches use workgroup solves.
 PACKED_BATCHES_ENABLED = False
 PACKED_BATCHES_OFF = ("packed batches are switched off: the saturated 4096-request serving trace stalled the GPU "
                       "and the cause is not found yet")

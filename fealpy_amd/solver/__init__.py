@@ -1,9 +1,9 @@
 from .cg import cg
 
 __all__ = ["cg"]
-from .mg import GeometricMG
+from .mg import GeometricMG, pcg_workgroups
 
-__all__.append("GeometricMG")
+__all__ += ["GeometricMG", "pcg_workgroups"]
 from .amg import AlgebraicMG
 
 __all__.append("AlgebraicMG")

@@ -561,3 +561,77 @@ class GeometricMG:
         if st != nat.FA_ENOCONV:
             nat.check(st, "fa_pcg")
         return x, {'residual': float(res.value), 'niter': int(niter.value)}
+
+
+# ---- many small solves in one launch: one workgroup per system -------------------
+class FaWgLevel(ctypes.Structure):
+    _fields_ = [("A", FaSell), ("R", FaSell), ("P", FaSell), ("dinv", ctypes.c_void_p), ("b", ctypes.c_void_p),
+                ("x", ctypes.c_void_p), ("x2", ctypes.c_void_p), ("d", ctypes.c_void_p), ("r", ctypes.c_void_p),
+                ("lmax", ctypes.c_double)]
+
+
+WG_MAX_LEVELS = 16
+
+
+class FaWgSolve(ctypes.Structure):
+    _fields_ = [("A", FaSell), ("b", ctypes.c_void_p), ("x", ctypes.c_void_p), ("r", ctypes.c_void_p),
+                ("p", ctypes.c_void_p), ("Ap", ctypes.c_void_p), ("atol", ctypes.c_double), ("rtol", ctypes.c_double),
+                ("maxit", ctypes.c_int64), ("nlevels", ctypes.c_int32), ("degree", ctypes.c_int32),
+                ("lower", ctypes.c_double), ("upper", ctypes.c_double), ("coarse_inv", ctypes.c_void_p),
+                ("levels", FaWgLevel * WG_MAX_LEVELS), ("niter", ctypes.c_int64), ("residual", ctypes.c_double),
+                ("status", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+def pcg_workgroups(systems, *, atol=0.0, rtol=1e-8, maxit=10000):
+    """Solve several small FP64 systems at once (fa_pcg_wg_batched), one
+    1024-thread workgroup per system running its whole MG-preconditioned CG:
+    the FP64 recurrence on A and the V-cycle of its hierarchy, with block
+    barriers between phases and no kernel launch per step.
+
+    systems: [(mg, A, b)], mg a GeometricMG with an FP32 cycle built with
+    compress=False and set up for A (FP64, Dirichlet-applied), b FP64.
+    Returns [(x, {'niter', 'residual', 'status'})] in order; each result
+    depends only on its own system (the same alone and in any batch)."""
+    L = nat.lib()
+    n = len(systems)
+    if n == 0:
+        return []
+    reqs = (FaWgSolve * n)()
+    keep, outs = [], []
+    for q, (mg, A, b) in enumerate(systems):
+        mg.ensure_setup(A)
+        if mg.dtype != torch.float32 or any(getattr(lv, 'compressed', False) for lv in mg.levels):
+            raise ValueError("pcg_workgroups needs an FP32 cycle built with compress=False")
+        if A.dtype != torch.float64 or mg.levels[0].A is not A:
+            raise ValueError("pcg_workgroups needs the FP64 operator the hierarchy was set up for")
+        if len(mg.levels) > WG_MAX_LEVELS or mg.levels[-1].dense is None:
+            raise ValueError("pcg_workgroups needs at most 16 levels and a dense coarsest inverse")
+        nrow = A.shape[0]
+        dev = b.device
+        bb = b.to(torch.float64).contiguous()
+        x = torch.zeros(nrow, dtype=torch.float64, device=dev)
+        work = torch.empty((3, nrow), dtype=torch.float64, device=dev)
+        keep.append((bb, work))
+        outs.append(x)
+        e = reqs[q]
+        e.A = mg.levels[0].As.desc(torch.float64)
+        e.b, e.x = bb.data_ptr(), x.data_ptr()
+        e.r, e.p, e.Ap = (work[i].data_ptr() for i in range(3))
+        e.atol, e.rtol, e.maxit = float(atol), float(rtol), max(int(maxit), 1)
+        e.nlevels, e.degree, e.lower, e.upper = len(mg.levels), mg.degree, mg.lower, mg.upper
+        e.coarse_inv = mg.levels[-1].dense.data_ptr()
+        for l, src in enumerate(mg._lv_array):
+            w = e.levels[l]
+            w.A, w.R, w.P, w.dinv = src.A, src.R, src.P, src.dinv
+            w.b, w.x, w.x2, w.d, w.r, w.lmax = src.b, src.x, src.x2, src.d, src.r, src.lmax
+    raw = bytes(memoryview(reqs).cast("B"))
+    dev = systems[0][2].device
+    dreq = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(dev)
+    nat.check(L.fa_pcg_wg_batched(n, reqs, ctypes.c_void_p(dreq.data_ptr()), nat.stream()), "fa_pcg_wg_batched")
+    back = dreq.cpu()                                   # waits for the stream
+    done = (FaWgSolve * n).from_buffer_copy(back.numpy().tobytes())
+    res = []
+    for q in range(n):
+        info = {'niter': int(done[q].niter), 'residual': float(done[q].residual), 'status': int(done[q].status)}
+        res.append((outs[q], info))
+    return res

@@ -127,8 +127,11 @@ class Autotuner:
 
     def tune_assembly(self, space, coef=None, q=None, dtype=torch.float64):
         """Lanes per row of P1 assembly pass 2 (fa_sum_incidence_rows) on the
-        space's pattern, timing a whole assembly of a diffusion form."""
-        from .fem import BilinearForm, ScalarDiffusionIntegrator
+        space's pattern, timing a whole two-pass assembly of a diffusion form
+        (also on a from_box mesh, whose own matrices take the structured
+        kernel: the choice serves the meshes with this row length that do
+        not).  Not thread-safe: the two-pass route is forced for the duration."""
+        from .fem import BilinearForm, ScalarDiffusionIntegrator, integrators
         from .fem.pattern import pattern_of
         L = nat.lib()
         pat = pattern_of(space)
@@ -137,7 +140,12 @@ class Autotuner:
 
         def run():
             form.assembly(dtype=dtype)
-        return self.tune(SUM_ROWS_LANES, key, run, device=space.device)
+        box, two = integrators.BOX_ASSEMBLY, integrators.TWO_PASS_MIN_CELLS
+        integrators.BOX_ASSEMBLY, integrators.TWO_PASS_MIN_CELLS = False, 0
+        try:
+            return self.tune(SUM_ROWS_LANES, key, run, device=space.device)
+        finally:
+            integrators.BOX_ASSEMBLY, integrators.TWO_PASS_MIN_CELLS = box, two
 
 
 _default = None

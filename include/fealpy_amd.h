@@ -118,6 +118,17 @@ int fa_pattern_fill(const int32_t* cell2dof, int64_t nc, int32_t ldof, int64_t g
                     const int32_t* adj_ptr, const int32_t* adj, int32_t max_degree,
                     const int32_t* indptr, int32_t* indices, fa_stream_t stream);
 
+/* The CSR pattern of the P1 space of TetrahedronMesh.from_box(nx, ny, nz)
+ * (what fa_pattern_count + fa_pattern_fill build from the incidence lists,
+ * coo_tensor.py:137-213) straight from the grid: node (ix, iy, iz) =
+ * (ix (ny+1) + iy)(nz+1) + iz shares a Kuhn cell with its neighbours along
+ * the 14 Kuhn edge directions, whose lexicographic order is ascending column
+ * order.  indptr: NN + 1 entries; indices: nnz, which must be the grid's
+ * count (FA_EINVAL otherwise; the caller knows it from the grid). */
+size_t fa_p1_box_pattern_work_bytes(int nx, int ny, int nz);
+int fa_p1_box_pattern(int nx, int ny, int nz, int32_t* indptr, int32_t* indices, int64_t nnz, void* work,
+                      size_t work_bytes, fa_stream_t stream);
+
 /* ------------------------------------------------------- element matrices --
  * Scalar Lagrange element matrices of ScalarDiffusionIntegrator (default
  * variant, fealpy/fem/scalar_diffusion_integrator.py:54-63) plus
@@ -230,6 +241,35 @@ int fa_assemble_p1_vector(int dtype, int td, int64_t gdof, int nq, const double*
                           const int32_t* cell, const double* qw, const double* phi,
                           int src_kind, double src_scale, const double* src,
                           const int32_t* adj_ptr, const int32_t* adj, void* out, fa_stream_t stream);
+
+/* P1 scalar diffusion/mass matrix of TetrahedronMesh.from_box(nx, ny, nz)
+ * (tetrahedron_mesh.py:1017-1086) straight into its CSR values, one pass, no
+ * incidence list and no element-row buffer: one thread per row walks the
+ * cells around its node in ascending cell order, found from the node's grid
+ * position, and sums each entry's contributions in that order (the order of
+ * BilinearForm.assembly's COO coalesce, bilinear_form.py:46-105,
+ * coo_tensor.py:184-213).  FP64 values equal those of fa_p1_incidence_rows +
+ * fa_sum_incidence_rows bit for bit; FP32 values are the FP64 sums rounded
+ * once.  node_wide: NN x 4 coordinate rows (x, y, z, pad) in from_box node
+ * order; indptr: the space's CSR row pointer; nnz must be the P1 pattern's
+ * (checked: FA_EINVAL otherwise).  Coefficient arguments as fa_p1_incidence_rows. */
+int fa_p1_box_matrix(int dtype, int nx, int ny, int nz, const double* node_wide, int nq, const double* qw,
+                     const double* phi, int diff_kind, double diff_scale, const double* diff_coef, int mass_kind,
+                     double mass_scale, const double* mass_coef, const int32_t* indptr, int64_t nnz, void* values,
+                     fa_stream_t stream);
+
+/* P1 load vector of a ScalarSourceIntegrator on TetrahedronMesh.from_box(nx,
+ * ny, nz) (scalar_source_integrator.py, LinearForm.assembly,
+ * linear_form.py:36-86): pass 1 writes each cell's element vector in cell
+ * order into `work` (fa_p1_box_vector_work_bytes, 32-byte aligned), pass 2
+ * sums each node's contributions over its cells in ascending cell order,
+ * found from its grid position.  FP64 values equal those of
+ * fa_p1_incidence_vectors + fa_sum_incidence_values bit for bit.  Source
+ * arguments as fa_p1_incidence_vectors. */
+size_t fa_p1_box_vector_work_bytes(int nx, int ny, int nz);
+int fa_p1_box_vector(int dtype, int nx, int ny, int nz, const double* node_wide, int nq, const double* qw,
+                     const double* phi, int src_kind, double src_scale, const double* src, void* out, void* work,
+                     size_t work_bytes, fa_stream_t stream);
 
 /* Two-pass P1 assembly in incidence order, the same values as
  * fa_assemble_p1_matrix bit for bit (same element rows, same LDS-table sum in
@@ -351,6 +391,19 @@ int fa_dirichlet_fill(int dtype, int dtype_out, int64_t n, const int32_t* indptr
                       void* new_values, fa_stream_t stream);
 int fa_dirichlet_rhs(int dtype, int64_t n, const int32_t* indptr, const int32_t* indices, const void* values,
                      const uint8_t* is_bd, const void* uh, const void* f, void* f_out, fa_stream_t stream);
+/* DirichletBC.apply with the pattern kept: the pattern-preserving mode of
+ * the Dirichlet condition (the reference's apply, dirichlet_bc.py:101-147,
+ * drops boundary rows and columns and so changes the pattern).  new_values
+ * (same length as values, may be values itself) get the unit row on every
+ * boundary row and 0 on boundary columns of interior rows; f_out = f - A uh
+ * on interior rows, uh on boundary rows, bit for bit the right-hand side of
+ * fa_dirichlet_apply.  The matrix has the same solution as the filtered one,
+ * and a new boundary condition on it keeps the pattern (and everything built
+ * on the pattern, such as multigrid symbolic products). */
+int fa_dirichlet_apply_keep(int dtype, int64_t n, const int32_t* indptr, const int32_t* indices, const void* values,
+                            const uint8_t* is_bd, void* new_values, const void* uh, const void* f, void* f_out,
+                            fa_stream_t stream);
+
 /* fa_dirichlet_fill (values only, on the pattern of fa_dirichlet_count) and
  * fa_dirichlet_rhs in one pass over A: DirichletBC.apply (fealpy/fem/
  * dirichlet_bc.py:101-121) with A read once; the same results bit for bit. */
@@ -528,6 +581,57 @@ typedef struct {
     int coarse_degree;
 } fa_mg_t;
 
+/* ------------------------------------------- one-workgroup PCG solves --
+ * Many small independent systems in one launch, one 1024-thread workgroup
+ * each: the whole MG-preconditioned CG of a request (FP64 outer recurrence
+ * on its FP64 SELL operator, the V-cycle of its FP32 hierarchy) runs inside
+ * one workgroup with block barriers between phases, so a request costs one
+ * workgroup's time instead of ~400 dependent kernel launches.  The role of
+ * the reference's cg (fealpy/solver/cg.py:14-123) with GAMGSolver as M, per
+ * request; the batch axis of Form / cg (fem/form.py:39-52, solver/cg.py:
+ * 69-123) is served by one workgroup per request instead of lockstep columns,
+ * so every request stops on its own test.  Workgroups never wait on each
+ * other.  A request's result depends only on its own data (bitwise the same
+ * alone and in any batch); it agrees with fa_pcg to the solve tolerance (the
+ * reductions are summed in a different fixed order). */
+typedef struct {
+    fa_sell_t A, R, P;         /* level operator and transfers (FP32 values)    */
+    const void* dinv;          /* FP32                                          */
+    void* b;                   /* level vectors (FP32, length A.n); the finest  */
+    void* x;                   /* level's b is the cycle's input (the FP32 copy */
+    void* x2;                  /* of the residual)                              */
+    void* d;
+    void* r;
+    double lmax;
+} fa_wg_level_t;
+
+#define FA_WG_MAX_LEVELS 16
+typedef struct {
+    fa_sell_t A;               /* FP64 operator (SELL-64, FP64 values)          */
+    const double* b;           /* right-hand side                               */
+    double* x;                 /* in: initial guess; out: solution              */
+    double* r;                 /* FP64 work vectors, length A.n each            */
+    double* p;
+    double* Ap;
+    double atol, rtol;
+    int64_t maxit;             /* > 0                                           */
+    int32_t nlevels;           /* 1 .. FA_WG_MAX_LEVELS                         */
+    int32_t degree;            /* Chebyshev degree                              */
+    double lower, upper;       /* smoothing interval [lower, upper] * lmax      */
+    const double* coarse_inv;  /* inverse of the coarsest operator, row-major   */
+    fa_wg_level_t levels[FA_WG_MAX_LEVELS];
+    int64_t niter;             /* out                                           */
+    double residual;           /* out: sqrt(r . z) at the last test             */
+    int32_t status;            /* out: 1 converged, 2 ||b|| < 1e-15, 3 maxit,
+                                  4 breakdown                                   */
+    int32_t pad;
+} fa_wg_solve_t;
+
+/* reqs_host: the descriptors (checked on the host); reqs_dev: a device copy
+ * of them, whose out fields the kernel writes (copy them back after the
+ * stream has run).  Asynchronous. */
+int fa_pcg_wg_batched(int nreq, const fa_wg_solve_t* reqs_host, fa_wg_solve_t* reqs_dev, fa_stream_t stream);
+
 /* Fill a compressed SELL (offsets and FP16 values of D^-1 A) from the CSR
  * (values of `dtype`, FP64 or FP32), on the slices `slice_ptr` of the level's
  * SELL.  With a workspace
@@ -700,9 +804,10 @@ enum {
     FA_TIMER_SELL_SPMV = 7,      /* fa_sell_spmv                               */
     FA_TIMER_VCYCLE = 8,         /* one whole V-cycle (all its kernels)        */
     FA_TIMER_P1_APPLY = 9,       /* matrix-free P1 operator (fa_p1_apply)      */
-    FA_TIMER_SMOOTH0 = 10        /* finest-level FP16 smoothing steps and
+    FA_TIMER_SMOOTH0 = 10,       /* finest-level FP16 smoothing steps and
                                     residual of the V-cycle (k_cheb16*,
                                     k_residual16 on level 0)                   */
+    FA_TIMER_P1_BOX = 11         /* P1 matrix on a from_box grid, one pass     */
 };
 int fa_timer_arm(unsigned mask);
 int fa_timer_read(int kind, int64_t* launches, double* total_ms);

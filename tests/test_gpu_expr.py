@@ -57,13 +57,16 @@ def test_highorder_separable_coefficient_and_source(golden, p):
     _close(b, F["F"])
 
 
-@pytest.mark.parametrize("two_pass", [False, True])
-def test_lognormal_source_in_kernel(golden, two_pass, monkeypatch):
+@pytest.mark.parametrize("path", ["fused", "two_pass", "box"])
+def test_lognormal_source_in_kernel(golden, path, monkeypatch):
     """f_c(x) = 3 pi^2 k_c u(x) with the per-cell field read in the kernel:
-    fused P1 kernel and the two-pass passes (forced on the small mesh)."""
+    fused P1 kernel, the two-pass passes (forced on the small mesh) and, for
+    matrix and vector, the structured from_box kernels."""
     from fealpy_amd.fem import integrators
-    if two_pass:
+    if path == "two_pass":
         monkeypatch.setattr(integrators, "TWO_PASS_MIN_CELLS", 0)
+    if path != "box":
+        monkeypatch.setattr(integrators, "BOX_ASSEMBLY", False)
     F = golden("fixtures_diffusion3d_p1_lognormal.npz")
     n = int(F["n"])
     mesh = TetrahedronMesh.from_box(nx=n, ny=n, nz=n, device=DEV)
@@ -145,6 +148,7 @@ def test_every_factor_kind_matches_torch(p, monkeypatch):
     from fealpy_amd.expr import exp
     from fealpy_amd.fem import integrators
     monkeypatch.setattr(integrators, "TWO_PASS_MIN_CELLS", 0)
+    monkeypatch.setattr(integrators, "BOX_ASSEMBLY", False)
     mesh = TetrahedronMesh.from_box([-3, 5, 0, 2, -1, 1], nx=5, ny=4, nz=3, device=DEV)
     space = fa.LagrangeFESpace(mesh, p)
     k = torch.rand(mesh.number_of_cells(), dtype=torch.float64, device=DEV) + 0.5

@@ -239,12 +239,15 @@ def test_boundary_data_cached_per_space():
     assert torch.equal(space.is_boundary_dof(), ref_flag)
 
 
-@pytest.mark.parametrize("dim", [2, 3])
-def test_fused_p1_matches_element_gather(dim):
-    """The fused P1 kernels (element rows computed inside the CSR-row sum)
-    against the element-tensor + fixed-order gather path, diffusion + mass with
-    a per-(cell, point) coefficient and a per-cell source."""
+@pytest.mark.parametrize("dim,box", [(2, False), (3, False), (3, True)])
+def test_fused_p1_matches_element_gather(dim, box, monkeypatch):
+    """The fused P1 kernels (element rows computed inside the CSR-row sum),
+    and for tetrahedra the structured from_box kernel, against the
+    element-tensor + fixed-order gather path, diffusion + mass with a
+    per-(cell, point) coefficient and a per-cell source."""
+    from fealpy_amd.fem import integrators
     from fealpy_amd.fem.pattern import pattern_of
+    monkeypatch.setattr(integrators, "BOX_ASSEMBLY", box)
     if dim == 2:
         mesh = fa.TriangleMesh.from_box(nx=9, ny=7, device=DEV)
     else:
@@ -284,6 +287,7 @@ def test_incidence_two_pass_p1(dim, monkeypatch):
     from fealpy_amd.fem import integrators
     from fealpy_amd.fem.pattern import pattern_of
     monkeypatch.setattr(integrators, "TWO_PASS_MIN_CELLS", 0)   # small mesh, two-pass path
+    monkeypatch.setattr(integrators, "BOX_ASSEMBLY", False)     # not the structured from_box kernel
     if dim == 2:
         mesh = fa.TriangleMesh.from_box(nx=7, ny=5, device=DEV)
     else:

@@ -55,9 +55,14 @@ def _lognormal(n, k, source):
     return mesh, space, bc, A, F, A2, F2
 
 
+@pytest.mark.parametrize("path", ["two_pass", "box"])
 @pytest.mark.parametrize("source", ["expr", "callable"])
-def test_lognormal_24cube_two_pass(golden, source):
+def test_lognormal_24cube_two_pass(golden, source, path, monkeypatch):
+    """The matrix through the two-pass kernels and through the structured
+    from_box kernel, each against the reference's values."""
     from fealpy_amd.fem import integrators
+    if path == "two_pass":
+        monkeypatch.setattr(integrators, "BOX_ASSEMBLY", False)
     g = golden("fixtures_diffusion3d_p1_lognormal_24.npz")
     n = int(g["n"])
     k = torch.as_tensor(g["k"], device=DEV)

@@ -1,18 +1,19 @@
-"""Packed batches (fa_pcg_batched), switched off with their tests skipped
-(see PACKED_BATCHES_OFF in fealpy_amd/serving.py), and the resident-size LRU
-budget of the server.  Packed batches: many requests' systems packed into one
-block-diagonal system, one PCG with per-request alpha, beta and stopping.
+"""Batched solves for serving.
 
+Workgroup solves (pcg_workgroups, fa_pcg_wg_batched): many small systems in
+one launch, one workgroup per system running its whole MG-preconditioned CG:
 - every result solves its own system to the requested tolerance (against a
-  tight FP64 CG of the same assembled system), with per-request iteration
-  counts;
-- a request's result is the same alone (a batch of one) and in a mix of
-  other requests, within the solve tolerance (the packed smoother uses the
-  largest Chebyshev bound of the batch, an upper bound for every request);
-- the continuous-batching server with batches equals the one-at-a-time
-  results within the same tolerance, repeated sizes reuse resident data;
-- the packing kernels reproduce each request's operator (SpMV of a packed
-  operator equals the requests' own SpMVs, segment by segment)."""
+  tight FP64 CG of the same assembled system) and agrees with the ordinary
+  multi-kernel MG-PCG on the same hierarchy;
+- a system's result is bitwise the same alone and in a batch of others;
+- the server with batches equals the one-at-a-time results within the
+  solve tolerance; a small resident budget evicts and rebuilds sizes with
+  the same result.
+
+Packed batches (PackedBatch, fa_pcg_batched) are switched off with their
+test skipped (PACKED_BATCHES_OFF in fealpy_amd/serving.py): the packing
+kernels reproduce each request's operator (SpMV of a packed operator equals
+the requests' own SpMVs, segment by segment)."""
 import numpy as np
 import pytest
 import torch
@@ -41,20 +42,49 @@ def _rel(a, b):
     return float(torch.linalg.norm(a - b)) / max(float(torch.linalg.norm(b)), 1e-300)
 
 
-@packed
-def test_batch_accuracy_and_alone_equals_mix():
+def test_workgroup_batch_accuracy_and_alone_equals_mix():
+    from fealpy_amd.serving import WG_MAX_GDOF
     srv = SolveServer(slots=1, rtol=1e-10)
-    reqs = make_requests(12, seed=21, nx_range=(16, 96))
+    reqs = [r for r in make_requests(40, seed=21, nx_range=(8, 96)) if (r.p * r.nx + 1) ** 2 <= WG_MAX_GDOF][:12]
+    assert len(reqs) >= 6
     srv.solve_batch(reqs)
     for r in reqs:
-        assert r.info["status"] == 1, r.info
+        assert r.info["status"] == 1 and r.info["batch"] == len(reqs), r.info
         assert _rel(r.x, _tight(srv, r)) < 1e-8, (r.nx, r.p, r.info)
-    assert max(r.info["batch"] for r in reqs) >= 2           # some share a packed PCG
+    by_id = {r.rid: r for r in reqs}
     for r in reqs[:4]:
-        alone = make_requests(12, seed=21, nx_range=(16, 96))[r.rid]
+        alone = make_requests(40, seed=21, nx_range=(8, 96))[r.rid]
         srv.solve_batch([alone])
         assert alone.info["batch"] == 1
-        assert _rel(alone.x, r.x) < 1e-8, (r.nx, r.p)
+        assert torch.equal(alone.x, by_id[r.rid].x), (r.nx, r.p)          # bitwise: its own workgroup
+
+
+def test_workgroup_solve_matches_multi_kernel_pcg():
+    """pcg_workgroups against GeometricMG.pcg on the same hierarchy and
+    system: the same solution to the tolerance, iteration counts within one;
+    2D P1/P2 and a small 3D P1 system in one batch."""
+    from fealpy_amd.solver import GeometricMG, pcg_workgroups
+    systems, refs = [], []
+    g = np.random.default_rng(5)
+    for mesh, p in ((fa.TriangleMesh.from_box([0, 1, 0, 1], nx=40, ny=40, device=DEV), 1),
+                    (fa.TriangleMesh.from_box([0, 1, 0, 1], nx=23, ny=23, device=DEV), 2),
+                    (fa.TetrahedronMesh.from_box(nx=10, ny=9, nz=11, device=DEV), 1)):
+        space = fa.LagrangeFESpace(mesh, p)
+        k = torch.as_tensor(np.exp(g.standard_normal(mesh.number_of_cells())), device=DEV)
+        A = fa.BilinearForm(space).add_integrator(fa.ScalarDiffusionIntegrator(k, q=p + 2)).assembly()
+        F = fa.LinearForm(space).add_integrator(fa.ScalarSourceIntegrator(1.0, q=p + 2)).assembly()
+        bc = fa.DirichletBC(space, gd=fa.cartesian(lambda x: x[..., 0] + 2 * x[..., 1]))
+        A, F = bc.apply(A, F)
+        mg = GeometricMG(space, bc.is_boundary_dof, dtype=torch.float32, compress=False)
+        mg.setup(A)
+        x_ref, info_ref = mg.pcg(A, F, atol=0.0, rtol=1e-10, maxit=500)
+        systems.append((mg, A, F))
+        refs.append((x_ref, info_ref))
+    out = pcg_workgroups(systems, atol=0.0, rtol=1e-10, maxit=500)
+    for (x, info), (x_ref, info_ref) in zip(out, refs):
+        assert info["status"] == 1, info
+        assert abs(info["niter"] - info_ref["niter"]) <= 1, (info, info_ref)
+        assert _rel(x, x_ref) < 1e-8
 
 
 @packed
@@ -90,7 +120,6 @@ def test_packed_operator_equals_request_operators():
         srv.context(r.nx, r.p).checkin(m)
 
 
-@packed
 def test_server_batches_match_one_at_a_time():
     reqs = make_requests(40, seed=9, nx_range=(16, 128))
     srv = SolveServer(slots=2, rtol=1e-10, batch=MAX_BATCH)

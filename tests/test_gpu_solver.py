@@ -458,6 +458,23 @@ def test_pcg_reproducible():
     assert torch.equal(x1, x2)
 
 
+def test_pcg_result_independent_of_host_checks():
+    """The host looks at the PCG state after a predicted number of
+    iterations (at most check_every): the iterate, the iteration count and
+    the residual are the same whatever the interval, with and without the
+    preconditioner, and maxit still stops at exactly maxit."""
+    mesh, space, bc, A, b = _system(16)
+    mg = GeometricMG(space, bc.is_boundary_dof, dtype=torch.float32)
+    runs = [mg.pcg(A, b, atol=0.0, rtol=1e-10, maxit=200, check_every=c) for c in (1, 3, 16, 64)]
+    for x, info in runs[1:]:
+        assert torch.equal(x, runs[0][0]) and info == runs[0][1]
+    plain = [fa.cg(A, b, atol=0.0, rtol=1e-9, maxit=2000, returninfo=True, check_every=c) for c in (1, 7, 50)]
+    for x, info in plain[1:]:
+        assert torch.equal(x, plain[0][0]) and info == plain[0][1]
+    x, info = mg.pcg(A, b, atol=0.0, rtol=1e-14, maxit=5, check_every=16)
+    assert info["niter"] == 5
+
+
 def test_cg_maxit_and_trivial_rhs():
     mesh, space, bc, A, b = _system(8)
     x, info = fa.cg(A, b, maxit=3, returninfo=True)

@@ -16,7 +16,9 @@ pytestmark = pytest.mark.gpu
 DEV = torch.device("cuda")
 
 
-def test_assembly_lanes_bitwise_and_cached(tmp_path):
+def test_assembly_lanes_bitwise_and_cached(tmp_path, monkeypatch):
+    from fealpy_amd.fem import integrators
+    monkeypatch.setattr(integrators, "BOX_ASSEMBLY", False)     # the two-pass kernels, not the from_box one
     L = nat.lib()
     mesh = fa.TetrahedronMesh.from_box(nx=24, ny=24, nz=20, device=DEV)     # 69,120 cells: two-pass path
     space = fa.LagrangeFESpace(mesh, 1)

@@ -238,18 +238,15 @@ def test_serving_trace_is_deterministic_and_in_range():
 
 
 def test_switched_off_packed_batches_raise():
-    """While packed batches are switched off, asking for them raises before
-    any device work (no silent fall-back to one-at-a-time solves)."""
+    """While packed batches are switched off, building one raises before any
+    device work (no silent fall-back); a server still takes batches (solved
+    by workgroup solves)."""
     from fealpy_amd import serving
     if serving.PACKED_BATCHES_ENABLED:
         pytest.skip("packed batches are switched on")
-    with pytest.raises(ValueError, match="switched off"):
-        serving.SolveServer(slots=1, batch=2)
-    srv = serving.SolveServer(slots=1, batch=1)
-    with pytest.raises(RuntimeError, match="switched off"):
-        srv.solve_batch(serving.make_requests(2, seed=1))
     with pytest.raises(RuntimeError, match="switched off"):
         serving.PackedBatch([], torch.device("cpu"))
+    assert serving.SolveServer(slots=1, batch=8).batch == 8
 
 
 def test_sell_index_lives_with_its_pattern(monkeypatch):
@@ -310,3 +307,76 @@ def test_to_vtk_round_trip(tmp_path, dim):
     assert np.all(a["types"] == ctype)
     np.testing.assert_array_equal(a["u"], mesh.nodedata['u'].numpy())
     np.testing.assert_array_equal(a["k"], mesh.celldata['k'].numpy())
+
+
+@pytest.mark.parametrize("dims", [(1, 1, 1), (2, 1, 3), (3, 4, 2), (1, 1, 7), (5, 3, 6)])
+def test_box_grid_pattern_counts_and_cell_walk(dims):
+    """The structured from_box kernels' host-side facts against the oracle's
+    Kuhn grid: the nonzero count and longest row of the P1 pattern
+    (pattern._box_counts, which sizes the CSR arrays), and the walk the
+    kernels make around a node (cubes in the order (a, b, c) = (1,1,1) ...
+    (0,0,0), tetrahedra holding the node's corner in ascending t) visits
+    exactly the node's cells in ascending order with the right local index,
+    the vertices it derives are the cell array's, and the neighbour offsets
+    in lexicographic order are the sorted CSR columns."""
+    from fealpy_amd.fem.pattern import _box_counts
+    nx, ny, nz = dims
+    o = tet_from_box(nx=nx, ny=ny, nz=nz)
+    cell = np.asarray(o.cell)
+    nn = (nx + 1) * (ny + 1) * (nz + 1)
+    cols = [set() for _ in range(nn)]
+    inc = [[] for _ in range(nn)]
+    for c in range(cell.shape[0]):
+        for i in range(4):
+            inc[cell[c, i]].append((c, i))
+            cols[cell[c, i]].update(int(v) for v in cell[c])
+    assert _box_counts(nx, ny, nz) == (sum(map(len, cols)), max(map(len, cols)))
+    corner = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+    kuhn = [[0, 1, 2, 6], [0, 5, 1, 6], [0, 4, 5, 6], [0, 7, 4, 6], [0, 3, 7, 6], [0, 2, 3, 6]]
+    nz1, nyz = nz + 1, (ny + 1) * (nz + 1)
+    for r in range(nn):
+        ix, iy, iz = r // nyz, (r % nyz) // nz1, r % nz1
+        walk = []
+        for a, b, c in [(1, 1, 1), (1, 1, 0), (1, 0, 1), (1, 0, 0), (0, 1, 1), (0, 1, 0), (0, 0, 1), (0, 0, 0)]:
+            cx, cy, cz = ix - a, iy - b, iz - c
+            if not (0 <= cx < nx and 0 <= cy < ny and 0 <= cz < nz):
+                continue
+            kap = corner.index((a, b, c))
+            for t in range(6):
+                if kap in kuhn[t]:
+                    cid = 6 * ((cx * ny + cy) * nz + cz) + t
+                    verts = [(cx + corner[k][0]) * nyz + (cy + corner[k][1]) * nz1 + cz + corner[k][2]
+                             for k in kuhn[t]]
+                    assert list(cell[cid]) == verts
+                    walk.append((cid, kuhn[t].index(kap)))
+        assert walk == inc[r]
+        offs = [(dx, dy, dz) for dx in (-1, 0, 1) for dy in (-1, 0, 1) for dz in (-1, 0, 1)
+                if (min(dx, dy, dz) >= 0 or max(dx, dy, dz) <= 0)
+                and 0 <= ix + dx <= nx and 0 <= iy + dy <= ny and 0 <= iz + dz <= nz]
+        assert [r + dx * nyz + dy * nz1 + dz for dx, dy, dz in offs] == sorted(cols[r])
+
+
+@pytest.mark.parametrize("cls,dims", [("tet", (3, 2, 4)), ("tri", (5, 3))])
+def test_topology_on_first_use(cls, dims):
+    """Faces, edges and the from_box side tags are built on first use: a P1
+    count and the P1 boundary of a full box grid (from grid positions) need
+    none of them, and equal what the face numbering gives; the tags appear
+    on any dict access."""
+    if cls == "tet":
+        mesh = TetrahedronMesh.from_box(nx=dims[0], ny=dims[1], nz=dims[2], device="cpu")
+    else:
+        mesh = TriangleMesh.from_box([0, 1, 0, 1], nx=dims[0], ny=dims[1], device="cpu")
+    assert "_constructed" not in mesh.__dict__
+    assert mesh.number_of_global_ipoints(1) == mesh.number_of_nodes()
+    flag = mesh.is_boundary_dof(1)
+    assert "_constructed" not in mesh.__dict__
+    ref = torch.zeros_like(flag)
+    ref[mesh.face[mesh.boundary_face_flag()].reshape(-1).long()] = True      # builds the topology
+    assert mesh.__dict__["_constructed"] and torch.equal(flag, ref)
+    if cls == "tet":
+        assert "upface" in mesh.meshdata and set(mesh.meshdata) >= {"from_box", "leftface", "bottomface"}
+        tags = torch.cat([mesh.meshdata[k] for k in ("leftface", "rightface", "frontface", "backface",
+                                                     "bottomface", "upface")])
+        assert torch.equal(torch.sort(tags).values, mesh.boundary_face_index())
+    with pytest.raises(AttributeError):
+        mesh.no_such_attribute
