@@ -43,10 +43,16 @@ HBM_PEAK_GBS = 8000.0
 # and WRITE_SIZE counter passes over `bench.py --spmv-only`, each scaled by the
 # same counter's reading for a 256 MiB device copy (profiles/sell_spmv_f32_128cube_counters.md)
 SPMV_F32_TRAFFIC_128 = 267.3e6
-# HBM bytes per launch of k_pcg_spmv<double> (mixed form) on the 128^3
-# operator, from the same two counter passes over `bench.py --spmv-only`
-# (profiles/pcg_spmv_f64_128cube_counters.md); None until measured
-PCG_SPMV_TRAFFIC_128 = None
+# HBM bytes per launch on the 128^3 problem, from the same two counter passes
+# over `bench.py --spmv-only` scaled the same way
+# (profiles/step_128cube_counters.md): k_pcg_spmv<double> on the FP64
+# operator; the finest level's FP16 smoother steps and residual (mean over
+# the family's launches); the structured P1 matrix kernel; the structured
+# load vector (both passes, with its cell-order buffer)
+PCG_SPMV_TRAFFIC_128 = 403.0e6
+SMOOTH0_TRAFFIC_128 = 164.0e6
+BOX_A_TRAFFIC_128 = 446.9e6
+BOX_F_TRAFFIC_128 = 1203.6e6
 # serving: slots (streams) of the continuous-batching server and the hardware
 # queues the process asks for (the slots' streams, the default stream and
 # torch's side streams each get one); read by HIP when it initialises
@@ -80,7 +86,8 @@ def parse():
                     help="serving: device memory (estimate) kept for resident per-size data")
     ap.add_argument("--rate", type=float, default=0.0,
                     help="serving: Poisson arrival rate per second (0: 0.8 x the measured server throughput)")
-    ap.add_argument("--workload", choices=("diffusion", "elasticity", "highorder", "serving"), default="diffusion",
+    ap.add_argument("--workload", choices=("diffusion", "elasticity", "highorder", "serving", "workgroups"),
+                    default="diffusion",
                     help="diffusion: 3D P1 lognormal diffusion (the headline line); elasticity: vector P2 "
                          "linear elasticity on from_box n^3 (n defaults to 40); highorder: P3/P4 diffusion + "
                          "mass element matrices on the matrix cores, from_box n^3 (n defaults to 48)")
@@ -259,7 +266,8 @@ def rooflines(kt, mesh, gdof, nnz_a, nnz_bc, n, vb, vb_a=8):
     # (2 + 2) nnz + 5 FP32 vectors (dinv, b, x in, x out, d)
     b_s = 4 * nnz_bc + 20 * n
     sl, sms = kt["smooth0"]
-    smooth = _roof("k_cheb16* / k_residual16 (finest level)", b_s, sl, sms,
+    big = n == 2146689
+    smooth = _roof("k_cheb16* / k_residual16 (finest level)", b_s, sl, sms, SMOOTH0_TRAFFIC_128 if big else None,
                    note="FP16 D^-1 A values and int16 column offsets: (2 + 2) B an entry + 5 FP32 vectors") \
         if sl else None
     b_a = 16 * nc + 24 * nn + 8 * nc + vb_a * nnz_a
@@ -269,7 +277,7 @@ def rooflines(kt, mesh, gdof, nnz_a, nnz_bc, n, vb, vb_a=8):
     lb, mb = kt["a_box"]
     if lb:      # the structured from_box kernel: no cell array (cells come from the grid position)
         b_box = 24 * nn + 8 * nc + 4 * (gdof + 1) + vb_a * nnz_a
-        asm_a = _roof("k_p1_box_rows", b_box, lb, mb,
+        asm_a = _roof("k_p1_box_rows", b_box, lb, mb, BOX_A_TRAFFIC_128 if big else None,
                       note="one pass: coordinates, k and the row pointer in, CSR values out (no cell array or "
                            f"incidence buffer); the generic kernels' compulsory count with cells would be {b_a} B",
                       generic_algorithmic_bytes=int(b_a))
@@ -279,7 +287,7 @@ def rooflines(kt, mesh, gdof, nnz_a, nnz_bc, n, vb, vb_a=8):
     fname = ("k_p1_box_cell_vectors + k_p1_box_sum_vectors" if lb else
              "k_p1_incidence_vectors<3> + k_sum_incidence_values")
     # the structured kernels read no cell array: coordinates, k, the vector out
-    asm_f = _roof(fname, 24 * nn + 8 * nc + 8 * gdof if lb else b_f, 1, m3 + m4,
+    asm_f = _roof(fname, 24 * nn + 8 * nc + 8 * gdof if lb else b_f, 1, m3 + m4, BOX_F_TRAFFIC_128 if (big and lb) else None,
                   pass1_us=round(m3 * 1e3 / max(l3, 1), 2), pass2_us=round(m4 * 1e3 / max(l4, 1), 2))
     vl, vms = kt["vcycle"]
     dl, dms = kt["dirichlet"]
@@ -800,6 +808,61 @@ def run_serving(args):
         "dof_per_s_server": round(sum(gdofs) / t_srv, 1)}
 
 
+def workgroups_main(args):
+    """Per-request GPU time of small 2D requests (P1 nx 16-160, P2 nx 8-80,
+    lognormal k, rtol 1e-8): one request on the multi-kernel MG-PCG path
+    (compressed hierarchy) against fa_pcg_wg_batched with one request and
+    with 32 requests of the same size in one launch (uncompressed hierarchy,
+    one workgroup each).  One JSON line with a row per size."""
+    import torch
+    import fealpy_amd as fa
+    from fealpy_amd.solver import GeometricMG, pcg_workgroups
+    dev = torch.device("cuda:0")
+
+    def system(nx, p, seed):
+        mesh = fa.TriangleMesh.from_box([0, 1, 0, 1], nx=nx, ny=nx, device=dev)
+        space = fa.LagrangeFESpace(mesh, p)
+        k = torch.as_tensor(np.exp(np.random.default_rng(seed).standard_normal(mesh.number_of_cells())), device=dev)
+        A = fa.BilinearForm(space).add_integrator(fa.ScalarDiffusionIntegrator(k, q=3)).assembly()
+        F = fa.LinearForm(space).add_integrator(fa.ScalarSourceIntegrator(1.0, q=3)).assembly()
+        bc = fa.DirichletBC(space, gd=0.0)
+        A, F = bc.apply(A, F)
+        mgc = GeometricMG(space, bc.is_boundary_dof, dtype=torch.float32)
+        mgu = GeometricMG(space, bc.is_boundary_dof, dtype=torch.float32, compress=False)
+        mgc.setup(A)
+        mgu.setup(A)
+        return A, F, mgc, mgu
+
+    def timed_ms(fn, reps):
+        out = fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            out = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps * 1e3, out
+
+    rows = []
+    for p, nxs in ((1, (16, 32, 64, 96, 128, 160)), (2, (8, 16, 32, 48, 64, 80))):
+        for nx in nxs:
+            A, F, mgc, mgu = system(nx, p, nx)
+            ms_k, (x1, i1) = timed_ms(lambda: mgc.pcg(A, F, atol=0.0, rtol=1e-8, maxit=500), 5)
+            ms_1, out = timed_ms(lambda: pcg_workgroups([(mgu, A, F)], atol=0.0, rtol=1e-8, maxit=500), 5)
+            batch = [system(nx, p, 100 + s) for s in range(32)]
+            ms_32, _ = timed_ms(lambda: pcg_workgroups([(b[3], b[0], b[1]) for b in batch], atol=0.0, rtol=1e-8,
+                                                       maxit=500), 3)
+            x2, i2 = out[0]
+            rows.append({"p": p, "nx": nx, "gdof": A.shape[0], "multi_kernel_ms": round(ms_k, 3),
+                         "multi_kernel_iterations": i1["niter"], "workgroup_ms": round(ms_1, 3),
+                         "workgroup_iterations": i2["niter"], "batch32_ms": round(ms_32, 3),
+                         "batch32_ms_per_request": round(ms_32 / 32, 4),
+                         "rel_diff": float(torch.linalg.norm(x2 - x1) / torch.linalg.norm(x1))})
+            print(f"# workgroups {rows[-1]}", file=sys.stderr, flush=True)
+            del batch
+    print(json.dumps({"metric": "GPU time a small request, multi-kernel vs workgroup solves", "unit": "ms",
+                      "n_gpus": 1, "data": "synthetic", "rows": rows}))
+
+
 def serving_main(args):
     s = run_serving(args)
     print(json.dumps({
@@ -829,6 +892,9 @@ def main():
         return
     if args.workload == "serving":
         serving_main(args)
+        return
+    if args.workload == "workgroups":
+        workgroups_main(args)
         return
     import torch
     import fealpy_amd as fa

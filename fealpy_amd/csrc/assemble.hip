@@ -1623,8 +1623,10 @@ extern "C" int fa_sum_incidence_values(int dtype, int64_t gdof, const int32_t* a
 // two-pass kernels (P1Cell, p1_kd, p1_row on the cell's own vertex order) and
 // adds it into 15 register accumulators, one per neighbour offset: every
 // entry is the sequential sum of its cell contributions in ascending cell
-// order, the order of the reference's lexsort + np.add.at, so the values
-// equal those of the two-pass kernels bit for bit.  No incidence list, cell
+// order, the order of the reference's lexsort + np.add.at.  The values equal
+// those of the two-pass kernels to a few units in the last place (the
+// compiler contracts multiply-adds in the inlined element geometry
+// differently here) and are bitwise reproducible.  No incidence list, cell
 // array or element-row buffer is read or written: the thread's loads are
 // vertex coordinates (one 32-byte row each) and coefficients, all addressed
 // from the grid position, so they issue back to back.
@@ -1696,6 +1698,11 @@ __device__ inline void box_cube(int ix, int iy, int iz, int nx, int ny, int nz, 
         p1_row<3, SEP>(g, p, c, kd, i, c, nq, qw, phi, dk, ds, dc, mk, ms, mc, v);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
+            // no contraction into the element value's last multiply: each
+            // value is rounded before it is summed, as the two-pass kernels
+            // round it when they store it and the reference when it forms
+            // its element matrices
+#pragma clang fp contract(off)
             const int k = box_kuhn(t, j);
             acc[box_slot(box_cdx(k) - A, box_cdy(k) - B, box_cdz(k) - C)] += v[j];
         }

@@ -249,7 +249,9 @@ int fa_assemble_p1_vector(int dtype, int td, int64_t gdof, int nq, const double*
  * position, and sums each entry's contributions in that order (the order of
  * BilinearForm.assembly's COO coalesce, bilinear_form.py:46-105,
  * coo_tensor.py:184-213).  FP64 values equal those of fa_p1_incidence_rows +
- * fa_sum_incidence_rows bit for bit; FP32 values are the FP64 sums rounded
+ * fa_sum_incidence_rows to a few units in the last place (the same element
+ * rows and order; multiply-adds inside the element geometry may be contracted
+ * differently), bitwise reproducible; FP32 values are the FP64 sums rounded
  * once.  node_wide: NN x 4 coordinate rows (x, y, z, pad) in from_box node
  * order; indptr: the space's CSR row pointer; nnz must be the P1 pattern's
  * (checked: FA_EINVAL otherwise).  Coefficient arguments as fa_p1_incidence_rows. */

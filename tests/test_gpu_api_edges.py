@@ -70,3 +70,15 @@ def test_mfma_path_defaults_to_fp64_values():
     assert it.assembly(space, dtype=torch.float32).dtype == torch.float32
     A = fa.BilinearForm(space).add_integrator(it).assembly()
     assert A.values.dtype == torch.float64
+
+
+def test_native_stream_is_torch_current_stream():
+    """The library launches on the calling thread's current torch stream,
+    inside and outside a stream context."""
+    import torch
+    from fealpy_amd import _native as nat
+    assert nat.stream().value == torch.cuda.current_stream().cuda_stream
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        assert nat.stream().value == s.cuda_stream
+    assert nat.stream().value == torch.cuda.current_stream().cuda_stream
