@@ -1637,9 +1637,11 @@ extern "C" int fa_sum_incidence_values(int dtype, int64_t gdof, const int32_t* a
 // CSR row (some cube holds the pair as a Kuhn edge), and ascending column
 // order is lexicographic offset order, so the entry's position in the row is
 // the number of present neighbours before it.
-constexpr int box_cdx(int k) { return (k == 1 || k == 2 || k == 5 || k == 6) ? 1 : 0; }
-constexpr int box_cdy(int k) { return (k == 2 || k == 3 || k == 6 || k == 7) ? 1 : 0; }
-constexpr int box_cdz(int k) { return (k >= 4) ? 1 : 0; }
+// offsets of corner k as bit masks over k (corners 1, 2, 5, 6 have x + 1;
+// 2, 3, 6, 7 have y + 1; 4-7 have z + 1): branch-free for a runtime k
+constexpr int box_cdx(int k) { return (0x66 >> k) & 1; }
+constexpr int box_cdy(int k) { return (0xCC >> k) & 1; }
+constexpr int box_cdz(int k) { return (k >> 2) & 1; }
 constexpr int box_corner_of(int a, int b, int c) {
     return c == 0 ? (a == 0 ? (b == 0 ? 0 : 3) : (b == 0 ? 1 : 2)) : (a == 0 ? (b == 0 ? 4 : 7) : (b == 0 ? 5 : 6));
 }
@@ -1755,52 +1757,311 @@ __global__ void __launch_bounds__(256) k_p1_box_rows(int nx, int ny, int nz, con
 }
 
 // Load vector on the same grid, two passes without an incidence list:
-// pass 1, one thread per cell in cell order, vals[4c + i] = |T| sum_q w_q
-// f(x_q) phi_i(q) (the arithmetic of k_p1_incidence_vectors), one 32-byte
+// pass 1, one thread per cube, vals[4c + i] = |T| sum_q w_q f(x_q) phi_i(q)
+// for its 6 cells (the arithmetic of k_p1_incidence_vectors), a 32-byte
 // store per cell; pass 2, one thread per node, the sum of vals[4c + i] over
 // the node's cells in ascending order, the addresses computed from the grid
 // position (the order of the two-pass kernels' incidence list, so the values
 // are equal bit for bit).
+//
+// A separable source on a tensor grid (node (ix, iy, iz) at (X[ix], Y[iy],
+// Z[iz])): coordinate d of quadrature point q of tetrahedron t in a cube whose
+// index along d is i is sum_v bcs[q][v] A_d[i + o_d(v)], o_d(v) the offset
+// along d of the tetrahedron's vertex v, so it depends on (i, t, q) only, and
+// so does every factor g_k of dimension d.  k_box_sep_table evaluates each
+// factor once per (i, t, q), the same sum in the same order as the per-cell
+// evaluation, and pass 1 multiplies table entries instead of evaluating
+// sin/cos/exp per cell (12 transcendental factors a cell for the usual
+// 3-factor source at 4 points).  Pass 1 uses the table for a cell only when
+// its four vertices lie exactly on the grid lines the table was built from;
+// any other cell evaluates the source directly.
+
+// axes[0 .. nx] = X, axes[nx+1 .. nx+ny+1] = Y, then Z, from the nodes on the
+// three grid axes; tab[((k * nmax + i) * 6 + t) * nq + q] = g_k at coordinate
+// dim[k] of point q of tetrahedron t in the cubes of index i along dim[k]
+__global__ void __launch_bounds__(256) k_box_sep_table(int nx, int ny, int nz, const double* __restrict__ nodew,
+                                                      const fa_sep_t* __restrict__ e, int nq, int nmax,
+                                                      double* __restrict__ axes, double* __restrict__ tab) {
+    const int64_t nz1 = nz + 1, nyz = static_cast<int64_t>(ny + 1) * nz1;
+    const double4* __restrict__ node = reinterpret_cast<const double4*>(nodew);
+    const int64_t naxes = static_cast<int64_t>(nx) + ny + nz + 3;
+    const int64_t ntab = static_cast<int64_t>(e->nfactors) * nmax * 6 * nq;
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < naxes + ntab;
+         j += (int64_t)gridDim.x * blockDim.x) {
+        if (j < naxes) {
+            if (j <= nx) axes[j] = node[j * nyz].x;
+            else if (j <= nx + 1 + ny) axes[j] = node[(j - nx - 1) * nz1].y;
+            else axes[j] = node[j - nx - ny - 2].z;
+            continue;
+        }
+        const int64_t idx = j - naxes;
+        const int q = static_cast<int>(idx % nq);
+        int64_t rest = idx / nq;
+        const int t = static_cast<int>(rest % 6);
+        rest /= 6;
+        const int i = static_cast<int>(rest % nmax);
+        const int k = static_cast<int>(rest / nmax);
+        const int d = e->dim[k];
+        const int n = d == 0 ? nx : (d == 1 ? ny : nz);
+        if (i >= n) continue;
+        double lo, hi;
+        if (d == 0) {
+            lo = node[i * nyz].x;
+            hi = node[(i + 1) * nyz].x;
+        } else if (d == 1) {
+            lo = node[i * nz1].y;
+            hi = node[(i + 1) * nz1].y;
+        } else {
+            lo = node[i].z;
+            hi = node[i + 1].z;
+        }
+        double x = 0.0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            int kv = 0;
+#pragma unroll
+            for (int tt = 0; tt < 6; ++tt)
+                if (t == tt) kv = box_kuhn(tt, v);
+            const int o = d == 0 ? box_cdx(kv) : (d == 1 ? box_cdy(kv) : box_cdz(kv));
+            x += e->bcs[q * 4 + v] * (o ? hi : lo);
+        }
+        tab[idx] = sep_factor(e->kind[k], e->a[k], e->b[k], x);
+    }
+}
+
+// sep_value at the NQ points of each of the 6 tetrahedra of cube (cx, cy,
+// cz) (cells c0 .. c0 + 5), the factors read from the table of
+// k_box_sep_table: per factor one block load (6 x NQ entries, contiguous),
+// per term the 6 cells' field values, so a cube waits on a few independent
+// batches of loads rather than a chain of one load per point and factor; per
+// point the operations and their order are those of sep_value
+template <int NQ>
+__device__ inline void sep_table_cube(const fa_sep_t* __restrict__ e, const double* __restrict__ tab, int nmax,
+                                      int cx, int cy, int cz, int64_t c0, double f[6][NQ]) {
+#pragma unroll
+    for (int t = 0; t < 6; ++t)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) f[t][q] = 0.0;
+    int k = 0;
+    const int nt = e->nterms;
+    for (int m = 0; m < nt; ++m) {
+        double v[6][NQ];
+        const double cm = e->coef[m];
+#pragma unroll
+        for (int t = 0; t < 6; ++t)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) v[t][q] = cm;
+        const double* fld = e->field[m];
+        if (fld) {
+            double fc[6];
+#pragma unroll
+            for (int t = 0; t < 6; ++t) fc[t] = fld[c0 + t];
+#pragma unroll
+            for (int t = 0; t < 6; ++t)
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) v[t][q] *= fc[t];
+        }
+        const int kend = e->term_end[m];
+        for (; k < kend; ++k) {
+            const int d = e->dim[k];
+            const int i = d == 0 ? cx : (d == 1 ? cy : cz);
+            const double* row = tab + (static_cast<int64_t>(k) * nmax + i) * 6 * NQ;
+            double r[6][NQ];
+            if constexpr (NQ == 4) {  // rows of 32 bytes, 32-byte aligned
+#pragma unroll
+                for (int t = 0; t < 6; ++t) {
+                    const double4 w = reinterpret_cast<const double4*>(row)[t];
+                    r[t][0] = w.x;
+                    r[t][1] = w.y;
+                    r[t][2] = w.z;
+                    r[t][3] = w.w;
+                }
+            } else {
+#pragma unroll
+                for (int t = 0; t < 6; ++t)
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) r[t][q] = row[t * NQ + q];
+            }
+#pragma unroll
+            for (int t = 0; t < 6; ++t)
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) v[t][q] *= r[t][q];
+        }
+#pragma unroll
+        for (int t = 0; t < 6; ++t)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) f[t][q] += v[t][q];
+    }
+}
+
+// element vectors of the 6 cells of a cube from the source values at NQ
+// points: v_i = sum_q (w_q |T| s f_q) phi_i(q), the arithmetic of the per-cell
+// kernels
+template <int NQ>
+__device__ inline void box_cube_cell_vectors(const double f[6][NQ], const double meas[6], double ss,
+                                             const double* __restrict__ qw, const double* __restrict__ phi,
+                                             double4* __restrict__ out) {
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+        double v[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const double fq = qw[q] * meas[t] * (ss * f[t][q]);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) v[i] += fq * phi[q * 4 + i];
+        }
+        out[t] = double4{v[0], v[1], v[2], v[3]};
+    }
+}
+
+// the same for one point of a rule with more than 4 points
+__device__ inline double sep_table_value(const fa_sep_t* __restrict__ e, const double* __restrict__ tab, int nmax,
+                                         int nq, int t, int q, int cx, int cy, int cz, int64_t c) {
+    double acc = 0.0;
+    int k = 0;
+    const int nt = e->nterms;
+    for (int m = 0; m < nt; ++m) {
+        double v = e->coef[m];
+        const double* fld = e->field[m];
+        if (fld) v *= fld[c];
+        const int kend = e->term_end[m];
+        for (; k < kend; ++k) {
+            const int d = e->dim[k];
+            const int i = d == 0 ? cx : (d == 1 ? cy : cz);
+            v *= tab[((static_cast<int64_t>(k) * nmax + i) * 6 + t) * nq + q];
+        }
+        acc += v;
+    }
+    return acc;
+}
+
+// Pass 1, one thread per cube: its 8 corners loaded once, the 6 cells'
+// measures, and for a separable source on the grid lines the 6 x NQ source
+// values from the table; the 6 element vectors are stored at the cells'
+// positions (cell 6 cube + t).  Other sources, rules and cubes off the grid
+// lines take the per-cell arithmetic cell by cell.
 template <bool SEP>
 __global__ void __launch_bounds__(256) k_p1_box_cell_vectors(int nx, int ny, int nz, const double* __restrict__ nodew,
                                                             int nq, const double* __restrict__ qw,
                                                             const double* __restrict__ phi, int sk, double ss,
                                                             const double* __restrict__ src,
+                                                            const double* __restrict__ axes,
+                                                            const double* __restrict__ tab, int nmax,
                                                             double* __restrict__ vals) {
     const int64_t nz1 = nz + 1, nyz = static_cast<int64_t>(ny + 1) * nz1;
-    const int64_t nc = 6 * static_cast<int64_t>(nx) * ny * nz;
-    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nc; c += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t cube = c / 6;
-        const int t = static_cast<int>(c - 6 * cube);
+    const int64_t ncube = static_cast<int64_t>(nx) * ny * nz;
+    const double4* __restrict__ node = reinterpret_cast<const double4*>(nodew);
+    for (int64_t cube = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; cube < ncube;
+         cube += (int64_t)gridDim.x * blockDim.x) {
         const int cx = static_cast<int>(cube / (static_cast<int64_t>(ny) * nz));
         const int64_t rem = cube - static_cast<int64_t>(cx) * ny * nz;
         const int cy = static_cast<int>(rem / nz);
         const int cz = static_cast<int>(rem - static_cast<int64_t>(cy) * nz);
         const int64_t n0 = static_cast<int64_t>(cx) * nyz + static_cast<int64_t>(cy) * nz1 + cz;
-        double p[4][3];
+        const int64_t c0 = 6 * cube;
+        double4* out = reinterpret_cast<double4*>(vals) + c0;
+        if constexpr (SEP) {
+            double P[8][3];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            // corner of vertex v of tetrahedron t, t a runtime index: the
-            // table is small enough to select without memory
-            int k = 0;
+            for (int k = 0; k < 8; ++k) {
+                const double4 w = node[n0 + box_cdx(k) * nyz + box_cdy(k) * nz1 + box_cdz(k)];
+                P[k][0] = w.x;
+                P[k][1] = w.y;
+                P[k][2] = w.z;
+            }
+            // all six grid-line coordinates loaded at once, compared without branches
+            const double x0 = axes[cx], x1 = axes[cx + 1];
+            const double y0 = axes[nx + 1 + cy], y1 = axes[nx + 2 + cy];
+            const double z0 = axes[nx + ny + 2 + cz], z1 = axes[nx + ny + 3 + cz];
+            int ok = 1;
 #pragma unroll
-            for (int tt = 0; tt < 6; ++tt)
-                if (t == tt) k = box_kuhn(tt, v);
-            const double4 q = reinterpret_cast<const double4*>(nodew)[n0 + box_cdx(k) * nyz + box_cdy(k) * nz1 +
-                                                                    box_cdz(k)];
-            p[v][0] = q.x;
-            p[v][1] = q.y;
-            p[v][2] = q.z;
+            for (int k = 0; k < 8; ++k)
+                ok &= static_cast<int>(P[k][0] == (box_cdx(k) ? x1 : x0)) &
+                      static_cast<int>(P[k][1] == (box_cdy(k) ? y1 : y0)) &
+                      static_cast<int>(P[k][2] == (box_cdz(k) ? z1 : z0));
+            if (ok && (nq == 4 || nq == 1)) {
+                double meas[6];
+#pragma unroll
+                for (int t = 0; t < 6; ++t) {
+                    double p[4][3];
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) p[v][d] = P[box_kuhn(t, v)][d];
+                    P1Cell<3> g;
+                    g.compute(p);
+                    meas[t] = g.meas;
+                }
+                const fa_sep_t* e = reinterpret_cast<const fa_sep_t*>(src);
+                if (nq == 4) {  // the usual rule (q = 2)
+                    double f[6][4];
+                    sep_table_cube<4>(e, tab, nmax, cx, cy, cz, c0, f);
+                    box_cube_cell_vectors<4>(f, meas, ss, qw, phi, out);
+                } else {
+                    double f[6][1];
+                    sep_table_cube<1>(e, tab, nmax, cx, cy, cz, c0, f);
+                    box_cube_cell_vectors<1>(f, meas, ss, qw, phi, out);
+                }
+                continue;
+            }
+            // a cube on the grid lines with another rule: table entries point by point
+            if (ok) {
+#pragma unroll 1
+                for (int t = 0; t < 6; ++t) {
+                    double p[4][3];
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        int k = 0;
+#pragma unroll
+                        for (int tt = 0; tt < 6; ++tt)
+                            if (t == tt) k = box_kuhn(tt, v);
+                        const double4 w = node[n0 + box_cdx(k) * nyz + box_cdy(k) * nz1 + box_cdz(k)];
+                        p[v][0] = w.x;
+                        p[v][1] = w.y;
+                        p[v][2] = w.z;
+                    }
+                    P1Cell<3> g;
+                    g.compute(p);
+                    double v[4] = {0.0, 0.0, 0.0, 0.0};
+                    for (int q = 0; q < nq; ++q) {
+                        const double f = ss * sep_table_value(reinterpret_cast<const fa_sep_t*>(src), tab, nmax, nq,
+                                                              t, q, cx, cy, cz, c0 + t);
+                        const double fq = qw[q] * g.meas * f;
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) v[i] += fq * phi[q * 4 + i];
+                    }
+                    out[t] = double4{v[0], v[1], v[2], v[3]};
+                }
+                continue;
+            }
         }
-        P1Cell<3> g;
-        g.compute(p);
-        double v[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int q = 0; q < nq; ++q) {
-            const double fq = qw[q] * g.meas * coef_at<3, SEP>(sk, ss, src, c, c, q, nq, p);
+        // any other source, or a cube off the grid lines: cell by cell
+#pragma unroll 1
+        for (int t = 0; t < 6; ++t) {
+            double p[4][3];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) v[i] += fq * phi[q * 4 + i];
+            for (int v = 0; v < 4; ++v) {
+                // corner of vertex v of tetrahedron t, t a runtime index: the
+                // table is small enough to select without memory
+                int k = 0;
+#pragma unroll
+                for (int tt = 0; tt < 6; ++tt)
+                    if (t == tt) k = box_kuhn(tt, v);
+                const double4 w = node[n0 + box_cdx(k) * nyz + box_cdy(k) * nz1 + box_cdz(k)];
+                p[v][0] = w.x;
+                p[v][1] = w.y;
+                p[v][2] = w.z;
+            }
+            P1Cell<3> g;
+            g.compute(p);
+            double v[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int q = 0; q < nq; ++q) {
+                const double fq = qw[q] * g.meas * coef_at<3, SEP>(sk, ss, src, c0 + t, c0 + t, q, nq, p);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) v[i] += fq * phi[q * 4 + i];
+            }
+            out[t] = double4{v[0], v[1], v[2], v[3]};
         }
-        reinterpret_cast<double4*>(vals)[c] = double4{v[0], v[1], v[2], v[3]};
     }
 }
 
@@ -2019,8 +2280,20 @@ extern "C" int fa_p1_box_matrix(int dtype, int nx, int ny, int nz, const double*
     return FA_OK;
 }
 
-extern "C" size_t fa_p1_box_vector_work_bytes(int nx, int ny, int nz) {
+// work: the cell-order element vectors (NC x 4 doubles), then the axis
+// coordinates and the separable-source table of k_box_sep_table
+static size_t box_vector_axes_offset(int nx, int ny, int nz) {
     return sizeof(double) * 4 * 6 * static_cast<size_t>(nx) * ny * nz;
+}
+static size_t box_vector_table_offset(int nx, int ny, int nz) {
+    const size_t naxes = static_cast<size_t>(nx) + ny + nz + 3;
+    return box_vector_axes_offset(nx, ny, nz) + ((naxes * sizeof(double) + 255) & ~static_cast<size_t>(255));
+}
+
+extern "C" size_t fa_p1_box_vector_work_bytes(int nx, int ny, int nz, int nq) {
+    const size_t nmax = static_cast<size_t>(std::max(nx, std::max(ny, nz)));
+    return box_vector_table_offset(nx, ny, nz) +
+           sizeof(double) * FA_SEP_MAX_FACTORS * nmax * 6 * static_cast<size_t>(std::max(nq, 0));
 }
 
 extern "C" int fa_p1_box_vector(int dtype, int nx, int ny, int nz, const double* node_wide, int nq, const double* qw,
@@ -2033,20 +2306,30 @@ extern "C" int fa_p1_box_vector(int dtype, int nx, int ny, int nz, const double*
     const int64_t nn = static_cast<int64_t>(nx + 1) * (ny + 1) * (nz + 1);
     const int64_t nc = 6 * static_cast<int64_t>(nx) * ny * nz;
     FA_REQUIRE(nn < INT32_MAX && nc < INT32_MAX, "fa_p1_box_vector: grid too large for int32 indices");
-    FA_REQUIRE(work_bytes >= fa_p1_box_vector_work_bytes(nx, ny, nz) && (reinterpret_cast<uintptr_t>(work) & 31) == 0,
-               "fa_p1_box_vector: workspace of %zu bytes (32-byte aligned) too small, %zu needed", work_bytes,
-               fa_p1_box_vector_work_bytes(nx, ny, nz));
+    const size_t need = fa_p1_box_vector_work_bytes(nx, ny, nz, nq);
+    FA_REQUIRE(work_bytes >= need && (reinterpret_cast<uintptr_t>(work) & 31) == 0,
+               "fa_p1_box_vector: workspace of %zu bytes (32-byte aligned) too small, %zu needed", work_bytes, need);
     double* vals = static_cast<double*>(work);
+    double* axes = reinterpret_cast<double*>(static_cast<char*>(work) + box_vector_axes_offset(nx, ny, nz));
+    double* tab = reinterpret_cast<double*>(static_cast<char*>(work) + box_vector_table_offset(nx, ny, nz));
+    const int nmax = std::max(nx, std::max(ny, nz));
     hipStream_t s = S(stream);
     const bool sep = src_kind == FA_COEF_SEP;
     timer_begin(FA_TIMER_P1_VECTORS, s);
-    const unsigned g1 = grid_for(nc, 256, 1 << 20);
+    if (sep) {  // the table's size follows the struct's factor count, read on the device
+        const int64_t most = static_cast<int64_t>(nx) + ny + nz + 3 +
+                             static_cast<int64_t>(FA_SEP_MAX_FACTORS) * nmax * 6 * nq;
+        hipLaunchKernelGGL(k_box_sep_table, dim3(grid_for(most, 256, 1 << 14)), dim3(256), 0, s, nx, ny, nz,
+                           node_wide, reinterpret_cast<const fa_sep_t*>(src), nq, nmax, axes, tab);
+        FA_LAUNCH("k_box_sep_table");
+    }
+    const unsigned g1 = grid_for(nc / 6, 256, 1 << 20);
     if (sep)
         hipLaunchKernelGGL(k_p1_box_cell_vectors<true>, dim3(g1), dim3(256), 0, s, nx, ny, nz, node_wide, nq, qw, phi,
-                           src_kind, src_scale, src, vals);
+                           src_kind, src_scale, src, axes, tab, nmax, vals);
     else
         hipLaunchKernelGGL(k_p1_box_cell_vectors<false>, dim3(g1), dim3(256), 0, s, nx, ny, nz, node_wide, nq, qw,
-                           phi, src_kind, src_scale, src, vals);
+                           phi, src_kind, src_scale, src, axes, tab, nmax, vals);
     timer_end(FA_TIMER_P1_VECTORS, s);
     FA_LAUNCH("k_p1_box_cell_vectors");
     const unsigned g2 = xcd_grid(grid_for(nn, 256, 1 << 20));

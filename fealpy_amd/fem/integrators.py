@@ -403,7 +403,7 @@ def p1_vector_values(space, it, pat, dtype):
     out = torch.empty(pat.gdof, dtype=dtype, device=space.device)
     grid = _box_grid(space, pat)
     if grid is not None:         # from_box tetrahedra: cell-order element vectors, then a gather per node by grid position
-        work = nat.workspace(L.fa_p1_box_vector_work_bytes(*grid), space.device)
+        work = nat.workspace(L.fa_p1_box_vector_work_bytes(*grid, qw.shape[0]), space.device)
         nat.check(L.fa_p1_box_vector(nat.dtype_code(dtype), *grid, nat.ptr(mesh.node_rows16()), qw.shape[0],
                                      nat.ptr(qw), nat.ptr(phi), kind, scale, nat.ptr(src), nat.ptr(out),
                                      nat.ptr(work), work.numel(), nat.stream()), "fa_p1_box_vector")

@@ -263,12 +263,16 @@ int fa_p1_box_matrix(int dtype, int nx, int ny, int nz, const double* node_wide,
 /* P1 load vector of a ScalarSourceIntegrator on TetrahedronMesh.from_box(nx,
  * ny, nz) (scalar_source_integrator.py, LinearForm.assembly,
  * linear_form.py:36-86): pass 1 writes each cell's element vector in cell
- * order into `work` (fa_p1_box_vector_work_bytes, 32-byte aligned), pass 2
- * sums each node's contributions over its cells in ascending cell order,
- * found from its grid position.  FP64 values equal those of
- * fa_p1_incidence_vectors + fa_sum_incidence_values bit for bit.  Source
- * arguments as fa_p1_incidence_vectors. */
-size_t fa_p1_box_vector_work_bytes(int nx, int ny, int nz);
+ * order into `work` (fa_p1_box_vector_work_bytes for the rule's nq points,
+ * 32-byte aligned), pass 2 sums each node's contributions over its cells in
+ * ascending cell order, found from its grid position.  A separable source
+ * (FA_COEF_SEP) is first tabulated per grid line: each factor once per (cube
+ * index along its axis, tetrahedron, point), used for the cells whose
+ * vertices lie on the grid lines (a tensor grid: all of them).  The values
+ * are those of fa_p1_incidence_vectors + fa_sum_incidence_values to a few
+ * units in the last place (bit for bit where the compiler contracts the
+ * products alike).  Source arguments as fa_p1_incidence_vectors. */
+size_t fa_p1_box_vector_work_bytes(int nx, int ny, int nz, int nq);
 int fa_p1_box_vector(int dtype, int nx, int ny, int nz, const double* node_wide, int nq, const double* qw,
                      const double* phi, int src_kind, double src_scale, const double* src, void* out, void* work,
                      size_t work_bytes, fa_stream_t stream);

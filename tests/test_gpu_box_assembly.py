@@ -12,8 +12,10 @@ run both paths against the reference):
   per-point and per-cell coefficients, and a separable expression evaluated
   in the kernel;
 - FP32 values are the FP64 sums rounded once;
-- load vectors (per-cell, per-point, constant and separable sources) bit for
-  bit equal to the two-pass kernels;
+- load vectors bit for bit equal to the two-pass kernels for per-cell,
+  per-point and constant sources; separable sources, taken from per-grid-line
+  tables, to a few units in the last place, also on a grid whose nodes were
+  moved off the grid lines (those cells evaluate the source directly);
 - the CSR pattern built from the grid equals the incidence-list pattern;
 - a CSR array that is not the grid's P1 pattern is refused before any launch;
 - meshes that are not a full from_box grid (threshold) keep the other kernels."""
@@ -80,17 +82,51 @@ def test_box_load_vector_equals_two_pass(dims, monkeypatch):
     nq = space.tables(q)[1].shape[0]
     kq = torch.as_tensor(g.standard_normal((nc, nq)), device=DEV)
     kc = torch.as_tensor(np.exp(g.standard_normal(nc)), device=DEV)
+    for src in (kc, kq, 2.5) + _separable_sources(kc):
+        _check_load_vector(space, src, q, monkeypatch)
+
+
+def _separable_sources(kc):
     from fealpy_amd.expr import cell
-    for src in (kc, kq, 2.5, 3 * np.pi ** 2 * cell(kc) * sin(np.pi * x) * sin(np.pi * y) * sin(np.pi * z)):
-        form = fa.LinearForm(space).add_integrator(fa.ScalarSourceIntegrator(src, q=q))
-        b64 = form.assembly()
-        b32 = form.assembly(dtype=torch.float32)
-        monkeypatch.setattr(integrators, "BOX_ASSEMBLY", False)
-        monkeypatch.setattr(integrators, "TWO_PASS_MIN_CELLS", 0)
-        two = fa.LinearForm(space).add_integrator(fa.ScalarSourceIntegrator(src, q=q)).assembly()
-        monkeypatch.setattr(integrators, "BOX_ASSEMBLY", True)
+    return (3 * np.pi ** 2 * cell(kc) * sin(np.pi * x) * sin(np.pi * y) * sin(np.pi * z),
+            sin(2 * x + 0.3) * cos(y) * exp(-0.5 * z) + 2 * x ** 2 * (y - 0.25) + 0.5 * cell(kc) * cos(3 * z) - 1.25)
+
+
+def _check_load_vector(space, src, q, monkeypatch):
+    form = fa.LinearForm(space).add_integrator(fa.ScalarSourceIntegrator(src, q=q))
+    b64 = form.assembly()
+    b32 = form.assembly(dtype=torch.float32)
+    monkeypatch.setattr(integrators, "BOX_ASSEMBLY", False)
+    monkeypatch.setattr(integrators, "TWO_PASS_MIN_CELLS", 0)
+    two = fa.LinearForm(space).add_integrator(fa.ScalarSourceIntegrator(src, q=q)).assembly()
+    monkeypatch.setattr(integrators, "BOX_ASSEMBLY", True)
+    if isinstance(src, (torch.Tensor, float)):
         assert torch.equal(b64, two), float((b64 - two).abs().max())
-        assert torch.equal(b32, b64.to(torch.float32))
+    else:     # separable: the same points and factors, products possibly contracted differently
+        assert float((b64 - two).abs().max()) <= 1e-15 * float(two.abs().max()), float((b64 - two).abs().max())
+        assert torch.equal(form.assembly(), b64)              # run to run: bitwise
+    assert torch.equal(b32, b64.to(torch.float32))
+
+
+@pytest.mark.parametrize("moved", ["none", "one", "many"])
+def test_box_load_vector_off_the_grid_lines(moved, monkeypatch):
+    """Nodes moved off the grid lines after from_box: the cells touching
+    them evaluate the separable source directly, the others from the
+    tables; either way the two-pass kernels' values."""
+    dims = (5, 4, 6)
+    mesh = TetrahedronMesh.from_box([0, 1, 0, 2, -1, 1], nx=dims[0], ny=dims[1], nz=dims[2], device=DEV)
+    node = mesh.node
+    g = np.random.default_rng(7)
+    if moved == "one":
+        node[37, 1] += 0.01
+    elif moved == "many":
+        pick = torch.as_tensor(g.choice(node.shape[0], node.shape[0] // 3, replace=False), device=DEV)
+        node[pick] += torch.as_tensor(0.02 * g.standard_normal((pick.numel(), 3)), device=DEV)
+    space = fa.LagrangeFESpace(mesh, 1)
+    assert integrators._box_grid(space, pattern_of(space)) == dims
+    kc = torch.as_tensor(np.exp(g.standard_normal(mesh.number_of_cells())), device=DEV)
+    for src in _separable_sources(kc):
+        _check_load_vector(space, src, 3, monkeypatch)
 
 
 @pytest.mark.parametrize("dims", [(1, 1, 1), (1, 6, 2), (4, 3, 5), (9, 7, 8)])
