@@ -90,6 +90,9 @@ _SIGNATURES = {
     "fa_rap_terms_fill": (ctypes.c_int, [_i64, _p, _p, _p, _p, _p, _p, _p, _p, _i32, _p, _i64, _p, ctypes.c_int, _p,
                                          _i64, _p, _p, _sz, _p]),
     "fa_rap_values_terms": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, ctypes.c_int, _p, _p, _p]),
+    "fa_rap_terms_fill_staged": (ctypes.c_int, [_i64, _p, _p, _p, _p, _p, _p, _p, _p, _i32, _p, _i64, _p, ctypes.c_int,
+                                                _p, _i64, _p, _p, _sz, _p]),
+    "fa_rap_values_staged": (ctypes.c_int, [ctypes.c_int, _i64, _p, _p, _p, _p, _p, _p, _p, ctypes.c_int, _p, _p, _p]),
     "fa_csr_to_dense": (ctypes.c_int, [_i64, _p, _p, _p, _p, _p]),
     "fa_dense_spd_inverse": (ctypes.c_int, [_i64, _p, _p]),
     "fa_mg_lmax": (ctypes.c_int, [_p, ctypes.c_int, ctypes.c_int, _pf64, _p, _sz, _p]),

@@ -102,33 +102,88 @@ __global__ void __launch_bounds__(256) k_bc_fill(int64_t n, const int32_t* __res
                                                 const int32_t* __restrict__ indices, const T* __restrict__ vals,
                                                 const uint8_t* __restrict__ bd, const int32_t* __restrict__ nptr,
                                                 int32_t* __restrict__ nidx, U* __restrict__ nval) {
+    // two of a group's rows at a time, every load of a row issued before any
+    // of its results is needed (row bounds with the boundary flag, values
+    // with the column indices): three rounds of loads per pair of rows
+    constexpr int R = 2;
     const int lane = threadIdx.x % BCG;
-    for (int64_t r = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / BCG; r < n;
-         r += (int64_t)gridDim.x * blockDim.x / BCG) {
-        const int32_t o0 = nptr[r];
-        if (bd[r]) {
-            if (lane == 0) {
-                if (nidx) nidx[o0] = static_cast<int32_t>(r);
-                nval[o0] = U(1);
-            }
-            continue;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x / BCG;
+    for (int64_t r0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / BCG; r0 < n; r0 += R * stride) {
+        // row data of both rows (a row past n reads row n - 1 and is skipped
+        // below): all loads issued, then used
+        int32_t o0[R], k0[R], k1[R];
+        bool rb[R];
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const int64_t r = min(r0 + u * stride, n - 1);
+            o0[u] = nptr[r];
+            rb[u] = bd[r] != 0;
+            k0[u] = indptr[r];
+            k1[u] = indptr[r + 1];
         }
-        int32_t base = o0;
-        const int32_t k1 = indptr[r + 1];
-        for (int32_t k0 = indptr[r]; k0 < k1; k0 += BCG) {
-            const int32_t k = k0 + lane;
-            int32_t c = 0;
-            bool keep = false;
-            if (k < k1) {
-                c = indices[k];
-                keep = !bd[c];
+        // the first BCG entries of both rows: column indices and values, then
+        // the columns' boundary flags, all loads issued before the results
+        // are used (positions clamped into the row; none for an empty row)
+        int32_t c[R];
+        T v[R];
+        bool all = true;
+#pragma unroll
+        for (int u = 0; u < R; ++u) all = all && k1[u] > k0[u];
+        if (all) {  // the usual case: no empty row, the loads in one straight run
+#pragma unroll
+            for (int u = 0; u < R; ++u) {
+                const int32_t k = k0[u] + min(lane, k1[u] - k0[u] - 1);
+                c[u] = indices[k];
+                v[u] = vals[k];
             }
-            const int inc = group_inclusive_scan<BCG>(keep ? 1 : 0);
-            if (keep) {
-                if (nidx) nidx[base + inc - 1] = c;
-                nval[base + inc - 1] = static_cast<U>(vals[k]);
+        } else {
+#pragma unroll
+            for (int u = 0; u < R; ++u) {
+                const int32_t k = k0[u] + min(lane, k1[u] - k0[u] - 1);
+                c[u] = 0;
+                v[u] = T(0);
+                if (k1[u] > k0[u]) {
+                    c[u] = indices[k];
+                    v[u] = vals[k];
+                }
             }
-            base += __shfl(inc, BCG - 1, BCG);  // group total
+        }
+        bool cb[R];
+#pragma unroll
+        for (int u = 0; u < R; ++u) cb[u] = bd[c[u]] != 0;
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const int64_t r = r0 + u * stride;
+            if (r >= n) break;  // the same for the group's lanes
+            if (rb[u]) {
+                if (lane == 0) {
+                    if (nidx) nidx[o0[u]] = static_cast<int32_t>(r);
+                    nval[o0[u]] = U(1);
+                }
+                continue;
+            }
+            int32_t base = o0[u];
+            for (int32_t kb = k0[u]; kb < k1[u]; kb += BCG) {
+                const int32_t k = kb + lane;
+                int32_t cc = c[u];
+                T vv = v[u];
+                bool keep = false;
+                if (k < k1[u]) {
+                    if (kb != k0[u]) {  // entries past the first BCG: loaded here
+                        cc = indices[k];
+                        vv = vals[k];
+                        keep = !bd[cc];
+                    } else {
+                        keep = !cb[u];
+                    }
+                }
+                const int inc = group_inclusive_scan<BCG>(keep ? 1 : 0);
+                if (keep) {
+                    if (nidx) nidx[base + inc - 1] = cc;
+                    nval[base + inc - 1] = static_cast<U>(vv);
+                }
+                base += __shfl(inc, BCG - 1, BCG);  // group total
+            }
         }
     }
 }
@@ -468,18 +523,69 @@ __global__ void __launch_bounds__(256) k_csr_residual(int64_t n, const int32_t* 
                                                      const double* __restrict__ vals, const double* __restrict__ x,
                                                      const double* __restrict__ b, const uint8_t* __restrict__ bd,
                                                      Y* __restrict__ y, double* __restrict__ part) {
+    // a group's rows r, r + stride, ... taken U at a time with their loads in
+    // flight together (the grid is capped, so a group walks many rows); the
+    // rows, the lane order within a row and the order of the r^2 sum are
+    // those of one row at a time
+    constexpr int U = 4;
     const int lane = threadIdx.x % G;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x / G;
     double nrm = 0.0;
-    for (int64_t r = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G; r < n;
-         r += (int64_t)gridDim.x * blockDim.x / G) {
-        const int32_t k1 = indptr[r + 1];
-        double s = 0.0;
-        for (int32_t k = indptr[r] + lane; k < k1; k += G) s += vals[k] * x[indices[k]];
-        s = group_sum<G>(s);
-        s = bd[r] ? 0.0 : b[r] - s;
-        if (lane == 0) {
-            y[r] = static_cast<Y>(s);
-            nrm += s * s;
+    for (int64_t r0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G; r0 < n; r0 += U * stride) {
+        // row data of the U rows (a row past n reads row n - 1 and counts
+        // as empty): all loads issued, then used
+        int32_t k0[U], k1[U];
+        double bv[U];
+        uint8_t dv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t r = min(r0 + u * stride, n - 1);
+            k0[u] = indptr[r];
+            k1[u] = indptr[r + 1];
+            bv[u] = b[r];
+            dv[u] = bd[r];
+        }
+        int len[U];
+        int m = 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            len[u] = r0 + u * stride < n ? k1[u] - k0[u] : 0;
+            m = max(m, len[u]);
+        }
+        double s[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) s[u] = 0.0;
+        for (int j = lane; j < m; j += G) {
+            // loads for all U rows before any is used (positions clamped
+            // into the row; none for an empty row), the sums masked
+            double v[U], xv[U];
+            int32_t c[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int32_t k = k0[u] + min(j, len[u] - 1);
+                v[u] = 0.0;
+                c[u] = 0;
+                if (len[u] > 0) {
+                    v[u] = vals[k];
+                    c[u] = indices[k];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) xv[u] = x[c[u]];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (j < len[u]) s[u] += v[u] * xv[u];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t r = r0 + u * stride;
+            if (r >= n) break;  // the same for the group's lanes
+            double t = group_sum<G>(s[u]);
+            t = dv[u] ? 0.0 : bv[u] - t;
+            if (lane == 0) {
+                y[r] = static_cast<Y>(t);
+                nrm += t * t;
+            }
         }
     }
     store_partial(block_sum(nrm), part);

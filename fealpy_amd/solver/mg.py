@@ -199,6 +199,12 @@ def _max_row(M):
 
 # candidates per row the symbolic Galerkin kernels hold (fa_rap_count's largest LDS list)
 RAP_CANDIDATES = 8192
+# Galerkin products whose coarse rows' fine rows hold at most this many
+# entries use the staged term words (fa_rap_values_staged); False keeps the
+# 32-bit words with A entry indices (fa_rap_values_terms), the same values
+RAP_STAGED = True
+RAP_STAGE_MAX = 512
+RAP_ROW_WORDS = 1024
 
 
 def _row_sums(M, w):
@@ -212,6 +218,19 @@ def _row_sums(M, w):
     cs[1:] = torch.cumsum(v, 0)
     crow = M.crow.long()
     return cs[crow[1:]] - cs[crow[:-1]]
+
+
+def _stage_fits(R, A, C, tptr):
+    """Whether every coarse row of R A P is within the staged form's bounds:
+    at most 64 R entries and 63 coarse entries, at most RAP_ROW_WORDS terms,
+    and at most RAP_STAGE_MAX entries in the A rows R row I names."""
+    if R.nnz == 0 or C.nnz == 0:
+        return False
+    per_r = _row_sums(R, (A.crow[1:] - A.crow[:-1]).long())
+    crow = C.crow.long()
+    words = tptr.long()[crow[1:]] - tptr.long()[crow[:-1]]
+    return (int(per_r.max()) <= RAP_STAGE_MAX and int((R.crow[1:] - R.crow[:-1]).max()) <= 64
+            and int((C.crow[1:] - C.crow[:-1]).max()) <= 63 and int(words.max()) <= RAP_ROW_WORDS)
 
 
 def _candidates(R, A, P):
@@ -351,10 +370,16 @@ class GeometricMG:
             if C.values.dtype != vd:            # value arrays follow the operator's type
                 C._values = torch.zeros(C.nnz, dtype=vd, device=self.device)
             if terms is not None:
-                tslice, tw, wset, nw = terms
-                nat.check(L.fa_rap_values_terms(vcode, C.nnz, nat.ptr(tslice), nat.ptr(tw), wset, nw,
-                                                nat.ptr(Af.values), nat.ptr(C.values), nat.stream()),
-                          "fa_rap_values_terms")
+                tptr, tw, wset, nw, staged = terms
+                if staged:
+                    nat.check(L.fa_rap_values_staged(vcode, C.shape[0], nat.ptr(R.crow), nat.ptr(R.col),
+                                                     nat.ptr(Af.crow), nat.ptr(C.crow), nat.ptr(tptr), nat.ptr(tw),
+                                                     wset, nw, nat.ptr(Af.values), nat.ptr(C.values),
+                                                     nat.stream()), "fa_rap_values_staged")
+                else:
+                    nat.check(L.fa_rap_values_terms(vcode, C.nnz, nat.ptr(tptr), nat.ptr(tw), wset, nw,
+                                                    nat.ptr(Af.values), nat.ptr(C.values), nat.stream()),
+                              "fa_rap_values_terms")
             else:
                 nat.check(L.fa_rap_values(vcode, C.shape[0], nat.ptr(R.crow), nat.ptr(R.col), nat.ptr(R.values),
                                           nat.ptr(Af.crow), nat.ptr(Af.col), nat.ptr(Af.values),
@@ -398,18 +423,24 @@ class GeometricMG:
         vals = uniq.cpu().tolist()
         wset = (ctypes.c_double * len(vals))(*vals)
         ne = C.nnz
-        tslice = torch.empty((ne + 63) // 64 + 1, dtype=torch.int32, device=self.device)
+        tptr = torch.empty(ne + 1, dtype=torch.int32, device=self.device)
         ws = nat.workspace(L.fa_rap_terms_work_bytes(ne), self.device)
         nterms = ctypes.c_int64(0)
         args = (C.shape[0], nat.ptr(R.crow), nat.ptr(R.col), nat.ptr(R.values), nat.ptr(A.crow), nat.ptr(A.col),
                 nat.ptr(P.crow), nat.ptr(P.col), nat.ptr(P.values), max(int(cand), 1), nat.ptr(C.crow), ne, wset,
                 len(vals))
-        nat.check(L.fa_rap_terms_count(*args, nat.ptr(tslice), ctypes.byref(nterms), nat.ptr(ws), ws.numel(),
+        nat.check(L.fa_rap_terms_count(*args, nat.ptr(tptr), ctypes.byref(nterms), nat.ptr(ws), ws.numel(),
                                        nat.stream()), "fa_rap_terms_count")
-        terms = torch.empty(max(int(nterms.value), 1), dtype=torch.int32, device=self.device)
-        nat.check(L.fa_rap_terms_fill(*args, nat.ptr(tslice), int(nterms.value), nat.ptr(terms), nat.ptr(ws),
-                                      ws.numel(), nat.stream()), "fa_rap_terms_fill")
-        return (tslice, terms, wset, len(vals))
+        staged = RAP_STAGED and _stage_fits(R, A, C, tptr)
+        if staged:   # 16-bit words naming the A entry's slot in the coarse row's staged fine rows
+            terms = torch.empty(max(int(nterms.value), 1), dtype=torch.int16, device=self.device)
+            nat.check(L.fa_rap_terms_fill_staged(*args, nat.ptr(tptr), int(nterms.value), nat.ptr(terms),
+                                                 nat.ptr(ws), ws.numel(), nat.stream()), "fa_rap_terms_fill_staged")
+        else:
+            terms = torch.empty(max(int(nterms.value), 1), dtype=torch.int32, device=self.device)
+            nat.check(L.fa_rap_terms_fill(*args, nat.ptr(tptr), int(nterms.value), nat.ptr(terms), nat.ptr(ws),
+                                          ws.numel(), nat.stream()), "fa_rap_terms_fill")
+        return (tptr, terms, wset, len(vals), staged)
 
     def setup(self, A):
         """Numeric setup for the (Dirichlet-applied) fine matrix A, FP64 or

@@ -505,26 +505,48 @@ int fa_rap_values(int dtype, int64_t nrows, const int32_t* r_ptr, const int32_t*
  * 2/3, 1/3); same product as fa_rap_values, which it replaces per assembly
  * (gamg_solver.py:85-101 forms it with SciPy SpGEMM).  Each coarse entry is
  * sum_k w_k a_val[t_k] over its terms in the order R entry -> A entry -> P
- * entry, w_k = r p a product of two values of `values` (host array).  Terms
- * of 64 consecutive coarse entries form a slice (SELL-64 layout): term k of
- * entry e at tslice[e/64] + 64 k + e%64; one 32-bit word each (A entry index
- * in bits 0-27, weight code in bits 28-31), so nnz(A) < 2^28.
- * fa_rap_terms_count: slice offsets tslice (ceil(nnz_c/64) + 1 entries) and
- * the term count; syncs.  fa_rap_terms_fill: the words; syncs (error flags).
+ * entry, w_k = r p a product of two values of `values` (host array).  The
+ * terms of coarse entry e are words tptr[e] .. tptr[e+1] - 1 (CSR over the
+ * coarse entries, no padding); one 32-bit word each (A entry index in bits
+ * 0-27, weight code in bits 28-31), so nnz(A) < 2^28.  The numeric sum of an
+ * entry is formed by 16 lanes, lane h adding terms h, h + 16, ... in order,
+ * combined by a fixed butterfly: the same value run to run.
+ * fa_rap_terms_count: the entry offsets tptr (nnz_c + 1 entries) and the term
+ * count; syncs.  fa_rap_terms_fill: the words; syncs (error flags).
  * c_ptr: the coarse pattern of fa_rap_count / fa_rap_fill. */
 size_t fa_rap_terms_work_bytes(int64_t nnz_c);
 int fa_rap_terms_count(int64_t nrows, const int32_t* r_ptr, const int32_t* r_idx, const double* r_val,
                        const int32_t* a_ptr, const int32_t* a_idx, const int32_t* p_ptr, const int32_t* p_idx,
                        const double* p_val, int32_t max_cand, const int32_t* c_ptr, int64_t nnz_c,
-                       const double* values, int nvalues, int32_t* tslice, int64_t* nterms_host, void* work,
+                       const double* values, int nvalues, int32_t* tptr, int64_t* nterms_host, void* work,
                        size_t work_bytes, fa_stream_t stream);
 int fa_rap_terms_fill(int64_t nrows, const int32_t* r_ptr, const int32_t* r_idx, const double* r_val,
                       const int32_t* a_ptr, const int32_t* a_idx, const int32_t* p_ptr, const int32_t* p_idx,
                       const double* p_val, int32_t max_cand, const int32_t* c_ptr, int64_t nnz_c,
-                      const double* values, int nvalues, const int32_t* tslice, int64_t nterms, uint32_t* terms,
+                      const double* values, int nvalues, const int32_t* tptr, int64_t nterms, uint32_t* terms,
                       void* work, size_t work_bytes, fa_stream_t stream);
-int fa_rap_values_terms(int dtype, int64_t nnz_c, const int32_t* tslice, const uint32_t* terms,
+int fa_rap_values_terms(int dtype, int64_t nnz_c, const int32_t* tptr, const uint32_t* terms,
                         const double* values, int nvalues, const void* a_val, void* c_val, fa_stream_t stream);
+/* Staged form of the same product, for patterns whose every coarse row I
+ * has at most 64 R entries, at most 63 coarse entries, at most 1024 terms,
+ * and whose fine rows (the A rows of R row I) hold at most 512 entries
+ * together (nested P1 grids: 15 fine rows of 15 entries): the terms are
+ * 16-bit words, weight code in bits 9-12 and in bits 0-8 the A entry's slot
+ * in the concatenation of R row I's A rows (in R's order), with the same
+ * offsets tptr and term order as fa_rap_terms_fill.  fa_rap_values_staged
+ * copies those A rows and the row's words into LDS per coarse row and sums
+ * each entry from there: the values of fa_rap_values_terms bit for bit,
+ * without gathers from A.  fa_rap_terms_fill_staged returns FA_ECAPACITY
+ * when a coarse row is past these bounds.  a_ptr and c_ptr must be the
+ * patterns the words were made for. */
+int fa_rap_terms_fill_staged(int64_t nrows, const int32_t* r_ptr, const int32_t* r_idx, const double* r_val,
+                             const int32_t* a_ptr, const int32_t* a_idx, const int32_t* p_ptr, const int32_t* p_idx,
+                             const double* p_val, int32_t max_cand, const int32_t* c_ptr, int64_t nnz_c,
+                             const double* values, int nvalues, const int32_t* tptr, int64_t nterms, uint16_t* terms,
+                             void* work, size_t work_bytes, fa_stream_t stream);
+int fa_rap_values_staged(int dtype, int64_t nrows, const int32_t* r_ptr, const int32_t* r_idx, const int32_t* a_ptr,
+                         const int32_t* c_ptr, const int32_t* tptr, const uint16_t* terms, const double* values,
+                         int nvalues, const void* a_val, void* c_val, fa_stream_t stream);
 
 /* ------------------------------------------------ algebraic multigrid --
  * Smoothed-aggregation transfers (the algebraic branch of GAMGSolver,

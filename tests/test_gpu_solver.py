@@ -159,22 +159,172 @@ def test_rap_terms_random_transfers(values):
     wset = (ctypes.c_double * len(values))(*values)
     args = (nc, nat.ptr(r[0]), nat.ptr(r[1]), nat.ptr(r[2]), nat.ptr(a[0]), nat.ptr(a[1]), nat.ptr(p[0]),
             nat.ptr(p[1]), nat.ptr(p[2]), cand, nat.ptr(cptr), ref.nnz, wset, len(values))
-    tslice = torch.empty((ref.nnz + 63) // 64 + 1, dtype=torch.int32, device=DEV)
+    tptr = torch.empty(ref.nnz + 1, dtype=torch.int32, device=DEV)
     nt = ctypes.c_int64(0)
-    nat.check(L.fa_rap_terms_count(*args, nat.ptr(tslice), ctypes.byref(nt), nat.ptr(ws), ws.numel(),
+    nat.check(L.fa_rap_terms_count(*args, nat.ptr(tptr), ctypes.byref(nt), nat.ptr(ws), ws.numel(),
                                    nat.stream()), "fa_rap_terms_count")
     terms = torch.empty(nt.value, dtype=torch.int32, device=DEV)
-    nat.check(L.fa_rap_terms_fill(*args, nat.ptr(tslice), nt.value, nat.ptr(terms), nat.ptr(ws), ws.numel(),
+    nat.check(L.fa_rap_terms_fill(*args, nat.ptr(tptr), nt.value, nat.ptr(terms), nat.ptr(ws), ws.numel(),
                                   nat.stream()), "fa_rap_terms_fill")
     cval = torch.empty(ref.nnz, dtype=torch.float64, device=DEV)
-    nat.check(L.fa_rap_values_terms(nat.FA_F64, ref.nnz, nat.ptr(tslice), nat.ptr(terms), wset, len(values), nat.ptr(a[2]),
+    nat.check(L.fa_rap_values_terms(nat.FA_F64, ref.nnz, nat.ptr(tptr), nat.ptr(terms), wset, len(values), nat.ptr(a[2]),
                                     nat.ptr(cval), nat.stream()), "fa_rap_values_terms")
     assert np.array_equal(cptr.cpu().numpy(), ref.indptr) and np.array_equal(cidx.cpu().numpy(), ref.indices)
     np.testing.assert_allclose(cval.cpu().numpy(), ref.data, rtol=0, atol=1e-13 * abs(ref.data).max())
+    # entry offsets: contiguous lists, at least one term an entry (the unit
+    # diagonal of an empty row), the last offset the term count
+    tp = tptr.cpu().numpy()
+    assert tp[0] == 0 and tp[-1] == nt.value and (np.diff(tp) >= 1).all()
+    again = torch.empty_like(cval)
+    nat.check(L.fa_rap_values_terms(nat.FA_F64, ref.nnz, nat.ptr(tptr), nat.ptr(terms), wset, len(values),
+                                    nat.ptr(a[2]), nat.ptr(again), nat.stream()), "fa_rap_values_terms")
+    assert torch.equal(again, cval)                                   # run to run: bitwise
+    # coarse rows of 64 or more entries: past the staged form's bounds
+    terms16 = torch.empty(nt.value, dtype=torch.int16, device=DEV)
+    assert np.diff(ref.indptr).max() >= 64
+    st = L.fa_rap_terms_fill_staged(*args, nat.ptr(tptr), nt.value, nat.ptr(terms16), nat.ptr(ws), ws.numel(),
+                                    nat.stream())
+    assert st == nat.FA_ECAPACITY
     four = (ctypes.c_double * 4)(1.0, 0.75, 0.5, 0.25)
-    st = L.fa_rap_values_terms(nat.FA_F64, ref.nnz, nat.ptr(tslice), nat.ptr(terms), four, 4, nat.ptr(a[2]), nat.ptr(cval),
+    st = L.fa_rap_values_terms(nat.FA_F64, ref.nnz, nat.ptr(tptr), nat.ptr(terms), four, 4, nat.ptr(a[2]), nat.ptr(cval),
                                nat.stream())
     assert st == nat.FA_ECAPACITY
+
+
+@pytest.mark.parametrize("values", [(1.0,), (1.0, 0.5), (1.0, 2 / 3, 1 / 3)])
+def test_rap_staged_random_banded(values):
+    """Staged words (fa_rap_terms_fill_staged / fa_rap_values_staged) on a
+    random banded A and local random transfers, against SciPy and bit for bit
+    against the 32-bit words; FP64 and FP32 operators."""
+    import ctypes
+    import scipy.sparse as sp
+    from fealpy_amd import _native as nat
+    rng = np.random.default_rng(10 + len(values))
+    nf, nc = 600, 120
+    rows, cols, vals = [], [], []
+    for i in range(nf):
+        near = [J for J in (i // 5 - 1, i // 5, i // 5 + 1) if 0 <= J < nc - 4]    # last coarse rows empty
+        for J in rng.choice(near, size=min(len(near), int(rng.integers(1, 4))), replace=False):
+            rows.append(i), cols.append(J), vals.append(values[rng.integers(0, len(values))])
+    P = sp.csr_matrix((vals, (rows, cols)), shape=(nf, nc))
+    B = sp.diags([rng.standard_normal(nf - k) for k in range(5)], list(range(5)), shape=(nf, nf))
+    A = (B + B.T).tocsr()
+    R = P.T.tocsr()
+    ref = (R @ A @ P).tocsr()
+    ref = ref + sp.diags((np.diff(ref.indptr) == 0).astype(float))
+    ref.sort_indices()
+    L = nat.lib()
+    r, a, p = _csr_dev(R), _csr_dev(A), _csr_dev(P)
+    cand = int(np.diff(R.indptr).max() * np.diff(A.indptr).max() * np.diff(P.indptr).max())
+    cptr = torch.empty(nc + 1, dtype=torch.int32, device=DEV)
+    ws = nat.workspace(max(L.fa_rap_work_bytes(nc), L.fa_rap_terms_work_bytes(ref.nnz)), DEV)
+    nnz, mrow = ctypes.c_int64(0), ctypes.c_int32(0)
+    nat.check(L.fa_rap_count(nc, nc, nat.ptr(r[0]), nat.ptr(r[1]), nat.ptr(a[0]), nat.ptr(a[1]), nat.ptr(p[0]),
+                             nat.ptr(p[1]), cand, nat.ptr(cptr), ctypes.byref(nnz), ctypes.byref(mrow), nat.ptr(ws),
+                             ws.numel(), nat.stream()), "fa_rap_count")
+    assert nnz.value == ref.nnz and np.diff(ref.indptr).max() < 64
+    cidx = torch.empty(ref.nnz, dtype=torch.int32, device=DEV)
+    nat.check(L.fa_rap_fill(nc, nat.ptr(r[0]), nat.ptr(r[1]), nat.ptr(a[0]), nat.ptr(a[1]), nat.ptr(p[0]),
+                            nat.ptr(p[1]), cand, nat.ptr(cptr), nat.ptr(cidx), nat.stream()), "fa_rap_fill")
+    wset = (ctypes.c_double * len(values))(*values)
+    args = (nc, nat.ptr(r[0]), nat.ptr(r[1]), nat.ptr(r[2]), nat.ptr(a[0]), nat.ptr(a[1]), nat.ptr(p[0]),
+            nat.ptr(p[1]), nat.ptr(p[2]), cand, nat.ptr(cptr), ref.nnz, wset, len(values))
+    tptr = torch.empty(ref.nnz + 1, dtype=torch.int32, device=DEV)
+    nt = ctypes.c_int64(0)
+    nat.check(L.fa_rap_terms_count(*args, nat.ptr(tptr), ctypes.byref(nt), nat.ptr(ws), ws.numel(),
+                                   nat.stream()), "fa_rap_terms_count")
+    terms = torch.empty(nt.value, dtype=torch.int32, device=DEV)
+    nat.check(L.fa_rap_terms_fill(*args, nat.ptr(tptr), nt.value, nat.ptr(terms), nat.ptr(ws), ws.numel(),
+                                  nat.stream()), "fa_rap_terms_fill")
+    terms16 = torch.empty(nt.value, dtype=torch.int16, device=DEV)
+    nat.check(L.fa_rap_terms_fill_staged(*args, nat.ptr(tptr), nt.value, nat.ptr(terms16), nat.ptr(ws), ws.numel(),
+                                         nat.stream()), "fa_rap_terms_fill_staged")
+    for dt, code in ((torch.float64, nat.FA_F64), (torch.float32, nat.FA_F32)):
+        av = a[2].to(dt)
+        glob = torch.empty(ref.nnz, dtype=dt, device=DEV)
+        nat.check(L.fa_rap_values_terms(code, ref.nnz, nat.ptr(tptr), nat.ptr(terms), wset, len(values), nat.ptr(av),
+                                        nat.ptr(glob), nat.stream()), "fa_rap_values_terms")
+        staged = torch.empty_like(glob)
+        nat.check(L.fa_rap_values_staged(code, nc, nat.ptr(r[0]), nat.ptr(r[1]), nat.ptr(a[0]), nat.ptr(cptr),
+                                         nat.ptr(tptr), nat.ptr(terms16), wset, len(values), nat.ptr(av),
+                                         nat.ptr(staged), nat.stream()), "fa_rap_values_staged")
+        assert torch.equal(staged, glob), dt
+    np.testing.assert_allclose(staged.double().cpu().numpy(), ref.data, rtol=0,
+                               atol=1e-6 * abs(ref.data).max())
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_rap_staged_equals_global_terms(p, monkeypatch):
+    """Every level's Galerkin product through the staged words (LDS slots)
+    equals the one through 32-bit words with A entry indices, bit for bit,
+    in FP64 and FP32."""
+    from fealpy_amd.solver import mg as mgmod
+    n = {1: 10, 2: 6}[p]
+    mesh, space, bc, A, b = _system(n, p=p)
+    k = _lognormal(mesh, seed=5)
+    _, _, _, A, _ = _system(n, k=k, p=p)
+    for dt in (torch.float64, torch.float32):
+        Ad = A if dt == torch.float64 else _as_type(A, dt)
+        out = {}
+        for staged in (True, False):
+            monkeypatch.setattr(mgmod, "RAP_STAGED", staged)
+            mg = GeometricMG(space, bc.is_boundary_dof, dtype=torch.float64, coarse_max=1)
+            mg.setup(Ad)
+            kinds = [s[-1][-1] for lv in mg.levels[:-1] for s in lv.stages if s[-1] is not None]
+            assert all(kk == staged for kk in kinds) and kinds
+            out[staged] = [lv.A.values.clone() for lv in mg.levels[1:]]
+        for l, (x1, x2) in enumerate(zip(out[True], out[False])):
+            assert x1.dtype == dt and torch.equal(x1, x2), f"level {l + 1} {dt}"
+
+
+def _as_type(A, dt):
+    from fealpy_amd.sparse import CSRTensor
+    return CSRTensor(A.crow, A.col, A.values.to(dt), A.shape)
+
+
+def test_rap_staged_refuses_wide_rows():
+    """A coarse row whose fine rows hold more than 512 entries is refused by
+    the staged fill (FA_ECAPACITY); the 32-bit words take it."""
+    import ctypes
+    import scipy.sparse as sp
+    from fealpy_amd import _native as nat
+    nf, nc = 200, 4
+    P = sp.csr_matrix((np.ones(nf), (np.arange(nf), np.arange(nf) % nc)), shape=(nf, nc))
+    A = (sp.random(nf, nf, density=0.2, random_state=1, format="csr") + sp.eye(nf)).tocsr()
+    R = P.T.tocsr()
+    L = nat.lib()
+    r, a, p = _csr_dev(R), _csr_dev(A), _csr_dev(P)
+    cand = int(np.diff(R.indptr).max() * np.diff(A.indptr).max() * np.diff(P.indptr).max())
+    assert cand <= 8192
+    cptr = torch.empty(nc + 1, dtype=torch.int32, device=DEV)
+    ref = (R @ A @ P).tocsr()
+    ws = nat.workspace(max(L.fa_rap_work_bytes(nc), L.fa_rap_terms_work_bytes(ref.nnz)), DEV)
+    nnz, mrow = ctypes.c_int64(0), ctypes.c_int32(0)
+    nat.check(L.fa_rap_count(nc, nc, nat.ptr(r[0]), nat.ptr(r[1]), nat.ptr(a[0]), nat.ptr(a[1]), nat.ptr(p[0]),
+                             nat.ptr(p[1]), cand, nat.ptr(cptr), ctypes.byref(nnz), ctypes.byref(mrow), nat.ptr(ws),
+                             ws.numel(), nat.stream()), "fa_rap_count")
+    cidx = torch.empty(nnz.value, dtype=torch.int32, device=DEV)
+    nat.check(L.fa_rap_fill(nc, nat.ptr(r[0]), nat.ptr(r[1]), nat.ptr(a[0]), nat.ptr(a[1]), nat.ptr(p[0]),
+                            nat.ptr(p[1]), cand, nat.ptr(cptr), nat.ptr(cidx), nat.stream()), "fa_rap_fill")
+    wset = (ctypes.c_double * 1)(1.0)
+    args = (nc, nat.ptr(r[0]), nat.ptr(r[1]), nat.ptr(r[2]), nat.ptr(a[0]), nat.ptr(a[1]), nat.ptr(p[0]),
+            nat.ptr(p[1]), nat.ptr(p[2]), cand, nat.ptr(cptr), nnz.value, wset, 1)
+    tptr = torch.empty(nnz.value + 1, dtype=torch.int32, device=DEV)
+    nt = ctypes.c_int64(0)
+    nat.check(L.fa_rap_terms_count(*args, nat.ptr(tptr), ctypes.byref(nt), nat.ptr(ws), ws.numel(), nat.stream()),
+              "fa_rap_terms_count")
+    terms16 = torch.empty(nt.value, dtype=torch.int16, device=DEV)
+    st = L.fa_rap_terms_fill_staged(*args, nat.ptr(tptr), nt.value, nat.ptr(terms16), nat.ptr(ws), ws.numel(),
+                                    nat.stream())
+    assert st == nat.FA_ECAPACITY and b"staged" in L.fa_last_error()
+    terms = torch.empty(nt.value, dtype=torch.int32, device=DEV)
+    nat.check(L.fa_rap_terms_fill(*args, nat.ptr(tptr), nt.value, nat.ptr(terms), nat.ptr(ws), ws.numel(),
+                                  nat.stream()), "fa_rap_terms_fill")
+    cval = torch.empty(nnz.value, dtype=torch.float64, device=DEV)
+    nat.check(L.fa_rap_values_terms(nat.FA_F64, nnz.value, nat.ptr(tptr), nat.ptr(terms), wset, 1, nat.ptr(a[2]),
+                                    nat.ptr(cval), nat.stream()), "fa_rap_values_terms")
+    ref.sort_indices()
+    np.testing.assert_allclose(cval.cpu().numpy(), ref.data, rtol=1e-13)
 
 
 def test_rap_term_list_elasticity():
