@@ -1662,27 +1662,88 @@ constexpr int box_slot(int dx, int dy, int dz) {
 }
 constexpr int BOX_SLOTS = 15;
 
+// Kuhn tetrahedra as paths: the corners of tetrahedron t, ordered by their
+// number of +1 offsets, run from corner 0 to corner 6 one axis at a time.
+// box_pos: path position of local vertex v; box_axis: the axis of path step
+// p (1..3); box_sign: orientation of the local vertex order (the sign of
+// det(o1 - o0, o2 - o0, o3 - o0) on the unit cube)
+constexpr int box_pos(int t, int v) {
+    return box_cdx(box_kuhn(t, v)) + box_cdy(box_kuhn(t, v)) + box_cdz(box_kuhn(t, v));
+}
+constexpr int box_corner_at(int t, int p) {
+    return box_pos(t, 0) == p ? box_kuhn(t, 0)
+           : box_pos(t, 1) == p ? box_kuhn(t, 1)
+           : box_pos(t, 2) == p ? box_kuhn(t, 2)
+                                : box_kuhn(t, 3);
+}
+constexpr int box_axis(int t, int p) {
+    return box_cdx(box_corner_at(t, p)) != box_cdx(box_corner_at(t, p - 1))   ? 0
+           : box_cdy(box_corner_at(t, p)) != box_cdy(box_corner_at(t, p - 1)) ? 1
+                                                                               : 2;
+}
+constexpr int box_off(int t, int v, int d) {
+    return (d == 0 ? box_cdx(box_kuhn(t, v)) : d == 1 ? box_cdy(box_kuhn(t, v)) : box_cdz(box_kuhn(t, v))) -
+           (d == 0 ? box_cdx(box_kuhn(t, 0)) : d == 1 ? box_cdy(box_kuhn(t, 0)) : box_cdz(box_kuhn(t, 0)));
+}
+constexpr int box_sign(int t) {
+    return box_off(t, 1, 0) * (box_off(t, 2, 1) * box_off(t, 3, 2) - box_off(t, 2, 2) * box_off(t, 3, 1)) -
+           box_off(t, 1, 1) * (box_off(t, 2, 0) * box_off(t, 3, 2) - box_off(t, 2, 2) * box_off(t, 3, 0)) +
+           box_off(t, 1, 2) * (box_off(t, 2, 0) * box_off(t, 3, 1) - box_off(t, 2, 1) * box_off(t, 3, 0));
+}
+
 // the contributions of the tetrahedra of cube (ix - A, iy - B, iz - C) that
 // hold the node, in ascending cell order
-template <int A, int B, int C, bool SEP>
+template <int A, int B, int C, bool SEP, bool TENSOR>
 __device__ inline void box_cube(int ix, int iy, int iz, int nx, int ny, int nz, const double* __restrict__ nodew,
-                                int nq, const double* __restrict__ qw, const double* __restrict__ phi, int dk,
-                                double ds, const double* __restrict__ dc, int mk, double ms,
-                                const double* __restrict__ mc, double acc[BOX_SLOTS]) {
+                                const double* __restrict__ axes, int nq, const double* __restrict__ qw,
+                                const double* __restrict__ phi, int dk, double ds, const double* __restrict__ dc,
+                                int mk, double ms, const double* __restrict__ mc, double acc[BOX_SLOTS]) {
     const int cx = ix - A, cy = iy - B, cz = iz - C;
     if (cx < 0 || cy < 0 || cz < 0 || cx >= nx || cy >= ny || cz >= nz) return;
     constexpr int KAP = box_corner_of(A, B, C);
     const int64_t cube = (static_cast<int64_t>(cx) * ny + cy) * nz + cz;
-    const int64_t nz1 = nz + 1, nyz = static_cast<int64_t>(ny + 1) * nz1;
-    const int64_t n0 = static_cast<int64_t>(cx) * nyz + static_cast<int64_t>(cy) * nz1 + cz;
     double P[8][3];
+    if constexpr (TENSOR) {  // corner coordinates from the grid lines (node (ix, iy, iz) at (X, Y, Z)[.])
+        const double x0 = axes[cx], x1 = axes[cx + 1];
+        const double y0 = axes[nx + 1 + cy], y1 = axes[nx + 2 + cy];
+        const double z0 = axes[nx + ny + 2 + cz], z1 = axes[nx + ny + 3 + cz];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {  // loads of corners no tetrahedron below uses are dropped by the compiler
-        const double4 q = reinterpret_cast<const double4*>(nodew)[n0 + box_cdx(k) * nyz + box_cdy(k) * nz1 + box_cdz(k)];
-        P[k][0] = q.x;
-        P[k][1] = q.y;
-        P[k][2] = q.z;
+        for (int k = 0; k < 8; ++k) {
+            P[k][0] = box_cdx(k) ? x1 : x0;
+            P[k][1] = box_cdy(k) ? y1 : y0;
+            P[k][2] = box_cdz(k) ? z1 : z0;
+        }
+    } else {
+        const int64_t nz1 = nz + 1, nyz = static_cast<int64_t>(ny + 1) * nz1;
+        const int64_t n0 = static_cast<int64_t>(cx) * nyz + static_cast<int64_t>(cy) * nz1 + cz;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const double4 q =
+                reinterpret_cast<const double4*>(nodew)[n0 + box_cdx(k) * nyz + box_cdy(k) * nz1 + box_cdz(k)];
+            P[k][0] = q.x;
+            P[k][1] = q.y;
+            P[k][2] = q.z;
+        }
     }
+    // an axis-aligned box (every corner on the box of corners 0 and 6): the
+    // closed form of a Kuhn tetrahedron's geometry.  Along its path with
+    // steps of lengths h_1, h_2, h_3 (u_k = 1 / h_k^2) the barycentric
+    // gradients are -e_1/h_1, e_1/h_1 - e_2/h_2, e_2/h_2 - e_3/h_3, e_3/h_3,
+    // so grad(lambda_i).grad(lambda_j) is u_1, u_1 + u_2, u_2 + u_3, u_3 on
+    // the diagonal (path positions 0..3), -u_{p+1} between positions p and
+    // p + 1 and 0 otherwise, and |T| = sign * h_x h_y h_z / 6: the values of
+    // P1Cell's formula to rounding, with no cross products per visit
+    int ok = 1;
+    if constexpr (!TENSOR) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            ok &= static_cast<int>(P[k][0] == (box_cdx(k) ? P[6][0] : P[0][0])) &
+                  static_cast<int>(P[k][1] == (box_cdy(k) ? P[6][1] : P[0][1])) &
+                  static_cast<int>(P[k][2] == (box_cdz(k) ? P[6][2] : P[0][2]));
+    }
+    const double hx = P[6][0] - P[0][0], hy = P[6][1] - P[0][1], hz = P[6][2] - P[0][2];
+    const double ux = 1.0 / (hx * hx), uy = 1.0 / (hy * hy), uz = 1.0 / (hz * hz);
+    const double vol6 = hx * hy * hz / 6.0;
 #pragma unroll
     for (int t = 0; t < 6; ++t) {
         const int i = box_local(t, KAP);
@@ -1692,12 +1753,38 @@ __device__ inline void box_cube(int ix, int iy, int iz, int nx, int ny, int nz, 
         for (int v = 0; v < 4; ++v)
 #pragma unroll
             for (int d = 0; d < 3; ++d) p[v][d] = P[box_kuhn(t, v)][d];
-        P1Cell<3> g;
-        g.compute(p);
         const int64_t c = 6 * cube + t;
         const double kd = p1_kd<3, SEP>(nq, qw, dk, ds, dc, c, c, p);
         double v[4];
-        p1_row<3, SEP>(g, p, c, kd, i, c, nq, qw, phi, dk, ds, dc, mk, ms, mc, v);
+        if (ok) {
+            const double meas = box_sign(t) > 0 ? vol6 : -vol6;
+            const double u[4] = {0.0, box_axis(t, 1) == 0 ? ux : box_axis(t, 1) == 1 ? uy : uz,
+                                 box_axis(t, 2) == 0 ? ux : box_axis(t, 2) == 1 ? uy : uz,
+                                 box_axis(t, 3) == 0 ? ux : box_axis(t, 3) == 1 ? uy : uz};
+            const int pi = box_pos(t, i);
+            const double km = kd * meas;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int pj = box_pos(t, j);
+                double gij = 0.0;
+                if (pi == pj) gij = pi == 0 ? u[1] : pi == 3 ? u[3] : u[pi] + u[pi + 1];
+                else if (pj == pi + 1) gij = -u[pj];
+                else if (pi == pj + 1) gij = -u[pi];
+                v[j] = dk ? km * gij : 0.0;
+            }
+            if (mk) {
+                for (int q = 0; q < nq; ++q) {
+                    const double* ph = phi + q * 4;
+                    const double w = qw[q] * meas * coef_at<3, SEP>(mk, ms, mc, c, c, q, nq, p) * ph[i];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) v[j] += w * ph[j];
+                }
+            }
+        } else if constexpr (!TENSOR) {
+            P1Cell<3> g;
+            g.compute(p);
+            p1_row<3, SEP>(g, p, c, kd, i, c, nq, qw, phi, dk, ds, dc, mk, ms, mc, v);
+        }
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             // no contraction into the element value's last multiply: each
@@ -1711,8 +1798,13 @@ __device__ inline void box_cube(int ix, int iy, int iz, int nx, int ny, int nz, 
     }
 }
 
-template <class T, bool SEP>
-__global__ void __launch_bounds__(256) k_p1_box_rows(int nx, int ny, int nz, const double* __restrict__ nodew, int nq,
+// TENSOR: the caller has checked that node (ix, iy, iz) sits at (X[ix],
+// Y[iy], Z[iz]) with axes = X, Y, Z (fa_box_axes), so every cube is an
+// axis-aligned box whose spacings come from the three short axis arrays and
+// no node row is read
+template <class T, bool SEP, bool TENSOR>
+__global__ void __launch_bounds__(256) k_p1_box_rows(int nx, int ny, int nz, const double* __restrict__ nodew,
+                                                    const double* __restrict__ axes, int nq,
                                                     const double* __restrict__ qw, const double* __restrict__ phi,
                                                     int dk, double ds, const double* __restrict__ dc, int mk,
                                                     double ms, const double* __restrict__ mc,
@@ -1728,14 +1820,14 @@ __global__ void __launch_bounds__(256) k_p1_box_rows(int nx, int ny, int nz, con
         double acc[BOX_SLOTS];
 #pragma unroll
         for (int s = 0; s < BOX_SLOTS; ++s) acc[s] = 0.0;
-        box_cube<1, 1, 1, SEP>(ix, iy, iz, nx, ny, nz, nodew, nq, qw, phi, dk, ds, dc, mk, ms, mc, acc);
-        box_cube<1, 1, 0, SEP>(ix, iy, iz, nx, ny, nz, nodew, nq, qw, phi, dk, ds, dc, mk, ms, mc, acc);
-        box_cube<1, 0, 1, SEP>(ix, iy, iz, nx, ny, nz, nodew, nq, qw, phi, dk, ds, dc, mk, ms, mc, acc);
-        box_cube<1, 0, 0, SEP>(ix, iy, iz, nx, ny, nz, nodew, nq, qw, phi, dk, ds, dc, mk, ms, mc, acc);
-        box_cube<0, 1, 1, SEP>(ix, iy, iz, nx, ny, nz, nodew, nq, qw, phi, dk, ds, dc, mk, ms, mc, acc);
-        box_cube<0, 1, 0, SEP>(ix, iy, iz, nx, ny, nz, nodew, nq, qw, phi, dk, ds, dc, mk, ms, mc, acc);
-        box_cube<0, 0, 1, SEP>(ix, iy, iz, nx, ny, nz, nodew, nq, qw, phi, dk, ds, dc, mk, ms, mc, acc);
-        box_cube<0, 0, 0, SEP>(ix, iy, iz, nx, ny, nz, nodew, nq, qw, phi, dk, ds, dc, mk, ms, mc, acc);
+        box_cube<1, 1, 1, SEP, TENSOR>(ix, iy, iz, nx, ny, nz, nodew, axes, nq, qw, phi, dk, ds, dc, mk, ms, mc, acc);
+        box_cube<1, 1, 0, SEP, TENSOR>(ix, iy, iz, nx, ny, nz, nodew, axes, nq, qw, phi, dk, ds, dc, mk, ms, mc, acc);
+        box_cube<1, 0, 1, SEP, TENSOR>(ix, iy, iz, nx, ny, nz, nodew, axes, nq, qw, phi, dk, ds, dc, mk, ms, mc, acc);
+        box_cube<1, 0, 0, SEP, TENSOR>(ix, iy, iz, nx, ny, nz, nodew, axes, nq, qw, phi, dk, ds, dc, mk, ms, mc, acc);
+        box_cube<0, 1, 1, SEP, TENSOR>(ix, iy, iz, nx, ny, nz, nodew, axes, nq, qw, phi, dk, ds, dc, mk, ms, mc, acc);
+        box_cube<0, 1, 0, SEP, TENSOR>(ix, iy, iz, nx, ny, nz, nodew, axes, nq, qw, phi, dk, ds, dc, mk, ms, mc, acc);
+        box_cube<0, 0, 1, SEP, TENSOR>(ix, iy, iz, nx, ny, nz, nodew, axes, nq, qw, phi, dk, ds, dc, mk, ms, mc, acc);
+        box_cube<0, 0, 0, SEP, TENSOR>(ix, iy, iz, nx, ny, nz, nodew, axes, nq, qw, phi, dk, ds, dc, mk, ms, mc, acc);
         const int32_t r0 = indptr[r], len = indptr[r + 1] - r0;
         int pos = 0;
 #pragma unroll
@@ -2246,10 +2338,238 @@ extern "C" int fa_p1_apply(int dtype_y, int td, int64_t gdof, int nq, const doub
     return FA_OK;
 }
 
-extern "C" int fa_p1_box_matrix(int dtype, int nx, int ny, int nz, const double* node_wide, int nq, const double* qw,
-                                const double* phi, int diff_kind, double diff_scale, const double* diff_coef,
-                                int mass_kind, double mass_scale, const double* mass_coef, const int32_t* indptr,
-                                int64_t nnz, void* values, fa_stream_t stream) {
+// Tensor grid, coefficients of the kinds none / constant / per cell: every
+// load of a row issued up front (the nine grid-line coordinates around the
+// node, the coefficient of each of its 24 cells, the row pointer), then the
+// closed-form rows of box_cube in the same cell order and arithmetic (so the
+// same values bit for bit as k_p1_box_rows<T, false, true>).  One template
+// instance per incident cube keeps each cube's coefficients in registers.
+struct BoxCoefs {
+    double kd[6], km[6];
+    bool valid;
+};
+
+// the coefficient of a cell as coef_at gives it for the kinds none, constant
+// and per cell, the load issued for every kind (from `any`, a valid array,
+// when the kind has no per-cell array) so no load waits behind a branch
+__device__ inline double box_cell_coef(int kind, double scale, const double* __restrict__ coef,
+                                       const double* __restrict__ any, int64_t c) {
+    const bool cell = kind == FA_COEF_CELL;
+    const double v = (cell ? coef : any)[cell ? c : 0];
+    return cell ? scale * v : (kind ? scale : 0.0);
+}
+
+template <int A, int B, int C, bool DCELL, bool MASS>
+__device__ inline BoxCoefs box_load(int ix, int iy, int iz, int nx, int ny, int nz, int dk, double ds,
+                                    const double* __restrict__ dc, int mk, double ms, const double* __restrict__ mc,
+                                    const double* __restrict__ any) {
+    BoxCoefs o;
+    const int cx = ix - A, cy = iy - B, cz = iz - C;
+    o.valid = cx >= 0 && cy >= 0 && cz >= 0 && cx < nx && cy < ny && cz < nz;
+    const int64_t cube = o.valid ? (static_cast<int64_t>(cx) * ny + cy) * nz + cz : 0;
+    constexpr int KAP = box_corner_of(A, B, C);
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+        o.kd[t] = 0.0;
+        o.km[t] = 0.0;
+        if (box_local(t, KAP) < 0) continue;
+        // DCELL: a per-cell diffusion coefficient, loaded with no branch
+        o.kd[t] = DCELL ? ds * dc[6 * cube + t] : (dk ? ds : 0.0);
+        if (MASS) o.km[t] = box_cell_coef(mk, ms, mc, any, 6 * cube + t);
+    }
+    return o;
+}
+
+// the loaded coefficients held in registers from here on: the compiler
+// otherwise sinks each load to its use, one round trip per cell
+template <int A, int B, int C, bool MASS>
+__device__ inline void box_pin(const BoxCoefs& o) {
+    constexpr int KAP = box_corner_of(A, B, C);
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+        if (box_local(t, KAP) < 0) continue;
+        asm volatile("" : : "v"(o.kd[t]));
+        if (MASS) asm volatile("" : : "v"(o.km[t]));
+    }
+}
+
+template <int A, int B, int C>
+__device__ inline void box_rows_tensor(const BoxCoefs& cf, double xm, double xc, double xp, double ym, double yc,
+                                       double yp, double zm, double zc, double zp, int nq,
+                                       const double* __restrict__ qw, const double* __restrict__ phi, int dk, int mk,
+                                       double acc[BOX_SLOTS]) {
+    if (!cf.valid) return;
+    const double x0 = A ? xm : xc, x1 = A ? xc : xp;
+    const double y0 = B ? ym : yc, y1 = B ? yc : yp;
+    const double z0 = C ? zm : zc, z1 = C ? zc : zp;
+    const double hx = x1 - x0, hy = y1 - y0, hz = z1 - z0;
+    const double ux = 1.0 / (hx * hx), uy = 1.0 / (hy * hy), uz = 1.0 / (hz * hz);
+    const double vol6 = hx * hy * hz / 6.0;
+    constexpr int KAP = box_corner_of(A, B, C);
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+        const int i = box_local(t, KAP);
+        if (i < 0) continue;
+        double kd = 0.0;
+        if (dk)
+            for (int q = 0; q < nq; ++q) kd += qw[q] * cf.kd[t];
+        const double meas = box_sign(t) > 0 ? vol6 : -vol6;
+        const double u[4] = {0.0, box_axis(t, 1) == 0 ? ux : box_axis(t, 1) == 1 ? uy : uz,
+                             box_axis(t, 2) == 0 ? ux : box_axis(t, 2) == 1 ? uy : uz,
+                             box_axis(t, 3) == 0 ? ux : box_axis(t, 3) == 1 ? uy : uz};
+        const int pi = box_pos(t, i);
+        const double km = kd * meas;
+        double v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int pj = box_pos(t, j);
+            double gij = 0.0;
+            if (pi == pj) gij = pi == 0 ? u[1] : pi == 3 ? u[3] : u[pi] + u[pi + 1];
+            else if (pj == pi + 1) gij = -u[pj];
+            else if (pi == pj + 1) gij = -u[pi];
+            v[j] = dk ? km * gij : 0.0;
+        }
+        if (mk) {
+            for (int q = 0; q < nq; ++q) {
+                const double* ph = phi + q * 4;
+                const double w = qw[q] * meas * cf.km[t] * ph[i];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) v[j] += w * ph[j];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+#pragma clang fp contract(off)
+            const int k = box_kuhn(t, j);
+            acc[box_slot(box_cdx(k) - A, box_cdy(k) - B, box_cdz(k) - C)] += v[j];
+        }
+    }
+}
+
+template <class T, bool DCELL, bool MASS>
+__global__ void __launch_bounds__(256) k_p1_box_rows_tensor(int nx, int ny, int nz, const double* __restrict__ axes,
+                                                           int nq, const double* __restrict__ qw,
+                                                           const double* __restrict__ phi, int dk, double ds,
+                                                           const double* __restrict__ dc, int mk, double ms,
+                                                           const double* __restrict__ mc,
+                                                           const int32_t* __restrict__ indptr,
+                                                           T* __restrict__ values) {
+    const int64_t nz1 = nz + 1, nyz = static_cast<int64_t>(ny + 1) * nz1, nn = static_cast<int64_t>(nx + 1) * nyz;
+    for (int64_t r = xcd_block() * (int64_t)blockDim.x + threadIdx.x; r < nn; r += (int64_t)gridDim.x * blockDim.x) {
+        const int ix = static_cast<int>(r / nyz);
+        const int64_t rem = r - ix * nyz;
+        const int iy = static_cast<int>(rem / nz1);
+        const int iz = static_cast<int>(rem - iy * nz1);
+        // grid lines (indices clamped into range), the row pointer and the
+        // cells' coefficients: all issued before any is used
+        const double xm = axes[max(ix - 1, 0)], xc = axes[ix], xp = axes[min(ix + 1, nx)];
+        const double ym = axes[nx + 1 + max(iy - 1, 0)], yc = axes[nx + 1 + iy], yp = axes[nx + 1 + min(iy + 1, ny)];
+        const double zm = axes[nx + ny + 2 + max(iz - 1, 0)], zc = axes[nx + ny + 2 + iz],
+                     zp = axes[nx + ny + 2 + min(iz + 1, nz)];
+        const int32_t r0 = indptr[r], len = indptr[r + 1] - r0;
+#define FA_BL(A_, B_, C_) box_load<A_, B_, C_, DCELL, MASS>(ix, iy, iz, nx, ny, nz, dk, ds, dc, mk, ms, mc, qw)
+        const BoxCoefs c0 = FA_BL(1, 1, 1), c1 = FA_BL(1, 1, 0), c2 = FA_BL(1, 0, 1), c3 = FA_BL(1, 0, 0);
+        const BoxCoefs c4 = FA_BL(0, 1, 1), c5 = FA_BL(0, 1, 0), c6 = FA_BL(0, 0, 1), c7 = FA_BL(0, 0, 0);
+#undef FA_BL
+        box_pin<1, 1, 1, MASS>(c0);
+        box_pin<1, 1, 0, MASS>(c1);
+        box_pin<1, 0, 1, MASS>(c2);
+        box_pin<1, 0, 0, MASS>(c3);
+        box_pin<0, 1, 1, MASS>(c4);
+        box_pin<0, 1, 0, MASS>(c5);
+        box_pin<0, 0, 1, MASS>(c6);
+        box_pin<0, 0, 0, MASS>(c7);
+        double acc[BOX_SLOTS];
+#pragma unroll
+        for (int s = 0; s < BOX_SLOTS; ++s) acc[s] = 0.0;
+#define FA_BR(A_, B_, C_, CF_) \
+    box_rows_tensor<A_, B_, C_>(CF_, xm, xc, xp, ym, yc, yp, zm, zc, zp, nq, qw, phi, dk, MASS ? mk : 0, acc)
+        FA_BR(1, 1, 1, c0);
+        FA_BR(1, 1, 0, c1);
+        FA_BR(1, 0, 1, c2);
+        FA_BR(1, 0, 0, c3);
+        FA_BR(0, 1, 1, c4);
+        FA_BR(0, 1, 0, c5);
+        FA_BR(0, 0, 1, c6);
+        FA_BR(0, 0, 0, c7);
+#undef FA_BR
+        int pos = 0;
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx)
+#pragma unroll
+            for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                for (int dz = -1; dz <= 1; ++dz) {
+                    const int s = box_slot(dx, dy, dz);
+                    if (s < 0) continue;
+                    const bool in = ix + dx >= 0 && ix + dx <= nx && iy + dy >= 0 && iy + dy <= ny && iz + dz >= 0 &&
+                                    iz + dz <= nz;
+                    if (in) {
+                        if (pos < len) values[r0 + pos] = static_cast<T>(acc[s]);
+                        ++pos;
+                    }
+                }
+    }
+}
+
+// axes[0 .. nx] = X (node (i, 0, 0)), then Y (node (0, j, 0)), then Z (node
+// (0, 0, k)); flag[0] = 1 when some node is not at (X[ix], Y[iy], Z[iz])
+__global__ void __launch_bounds__(256) k_box_axes(int nx, int ny, int nz, const double* __restrict__ nodew,
+                                                 double* __restrict__ axes, int32_t* __restrict__ flag) {
+    const int64_t nz1 = nz + 1, nyz = static_cast<int64_t>(ny + 1) * nz1;
+    const double4* __restrict__ node = reinterpret_cast<const double4*>(nodew);
+    const int64_t naxes = static_cast<int64_t>(nx) + ny + nz + 3;
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < naxes; j += (int64_t)gridDim.x * blockDim.x) {
+        if (j <= nx) axes[j] = node[j * nyz].x;
+        else if (j <= nx + 1 + ny) axes[j] = node[(j - nx - 1) * nz1].y;
+        else axes[j] = node[j - nx - ny - 2].z;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) flag[0] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_box_axes_check(int nx, int ny, int nz, const double* __restrict__ nodew,
+                                                       const double* __restrict__ axes, int32_t* __restrict__ flag) {
+    const int64_t nz1 = nz + 1, nyz = static_cast<int64_t>(ny + 1) * nz1, nn = static_cast<int64_t>(nx + 1) * nyz;
+    const double4* __restrict__ node = reinterpret_cast<const double4*>(nodew);
+    bool bad = false;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nn; r += (int64_t)gridDim.x * blockDim.x) {
+        const int ix = static_cast<int>(r / nyz);
+        const int64_t rem = r - ix * nyz;
+        const int iy = static_cast<int>(rem / nz1);
+        const int iz = static_cast<int>(rem - iy * nz1);
+        const double4 q = node[r];
+        bad = bad || !(q.x == axes[ix] && q.y == axes[nx + 1 + iy] && q.z == axes[nx + ny + 2 + iz]);
+    }
+    if (bad) flag[0] = 1;  // any writer writes 1
+}
+
+extern "C" size_t fa_box_axes_work_bytes(void) { return 256; }
+
+extern "C" int fa_box_axes(int nx, int ny, int nz, const double* node_wide, double* axes, int32_t* on_grid_host,
+                           void* work, size_t work_bytes, fa_stream_t stream) {
+    FA_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1 && node_wide && axes && on_grid_host, "fa_box_axes: bad arguments");
+    FA_REQUIRE(work && work_bytes >= fa_box_axes_work_bytes(), "fa_box_axes: workspace too small");
+    const int64_t nn = static_cast<int64_t>(nx + 1) * (ny + 1) * (nz + 1);
+    FA_REQUIRE(nn < INT32_MAX, "fa_box_axes: grid too large for int32 indices");
+    hipStream_t s = S(stream);
+    int32_t* flag = static_cast<int32_t*>(work);
+    const int64_t naxes = static_cast<int64_t>(nx) + ny + nz + 3;
+    hipLaunchKernelGGL(k_box_axes, dim3(grid_for(naxes, 256, 64)), dim3(256), 0, s, nx, ny, nz, node_wide, axes, flag);
+    FA_LAUNCH("k_box_axes");
+    hipLaunchKernelGGL(k_box_axes_check, dim3(grid_for(nn, 256, 1 << 14)), dim3(256), 0, s, nx, ny, nz, node_wide,
+                       axes, flag);
+    FA_LAUNCH("k_box_axes_check");
+    int32_t h = 1;
+    FA_HIP(hipMemcpyAsync(&h, flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    FA_HIP(hipStreamSynchronize(s));
+    *on_grid_host = h == 0;
+    return FA_OK;
+}
+
+extern "C" int fa_p1_box_matrix(int dtype, int nx, int ny, int nz, const double* node_wide, const double* axes,
+                                int nq, const double* qw, const double* phi, int diff_kind, double diff_scale,
+                                const double* diff_coef, int mass_kind, double mass_scale, const double* mass_coef,
+                                const int32_t* indptr, int64_t nnz, void* values, fa_stream_t stream) {
     FA_REQUIRE(dtype == FA_F64 || dtype == FA_F32, "fa_p1_box_matrix: bad dtype %d", dtype);
     FA_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1 && nq > 0 && node_wide && qw && indptr && values,
                "fa_p1_box_matrix: bad arguments");
@@ -2265,15 +2585,33 @@ extern "C" int fa_p1_box_matrix(int dtype, int nx, int ny, int nz, const double*
     const unsigned grid = xcd_grid(grid_for(nn, 256, 1 << 20));
     const bool sep = diff_kind == FA_COEF_SEP || mass_kind == FA_COEF_SEP;
     timer_begin(FA_TIMER_P1_BOX, S(stream));
-#define FA_BOX(T_, SEP_)                                                                                            \
-    hipLaunchKernelGGL((k_p1_box_rows<T_, SEP_>), dim3(grid), dim3(256), 0, S(stream), nx, ny, nz, node_wide, nq, qw,  \
-                       phi, diff_kind, diff_scale, diff_coef, mass_kind, mass_scale, mass_coef, indptr,               \
+#define FA_BOX(T_, SEP_, TEN_)                                                                                      \
+    hipLaunchKernelGGL((k_p1_box_rows<T_, SEP_, TEN_>), dim3(grid), dim3(256), 0, S(stream), nx, ny, nz, node_wide,     \
+                       axes, nq, qw, phi, diff_kind, diff_scale, diff_coef, mass_kind, mass_scale, mass_coef, indptr, \
                        static_cast<T_*>(values))
-    if (dtype == FA_F64) {
-        if (sep) FA_BOX(double, true); else FA_BOX(double, false);
+#define FA_BOX_T(T_, SEP_) \
+    if (axes) FA_BOX(T_, SEP_, true); else FA_BOX(T_, SEP_, false)
+    const auto simple = [](int kind) { return kind == FA_COEF_NONE || kind == FA_COEF_CONST || kind == FA_COEF_CELL; };
+    if (axes && simple(diff_kind) && simple(mass_kind)) {
+#define FA_BT(T_, D_, M_)                                                                                           \
+    hipLaunchKernelGGL((k_p1_box_rows_tensor<T_, D_, M_>), dim3(grid), dim3(256), 0, S(stream), nx, ny, nz, axes, nq, \
+                       qw, phi, diff_kind, diff_scale, diff_coef, mass_kind, mass_scale, mass_coef, indptr,           \
+                       static_cast<T_*>(values))
+#define FA_BT2(T_, M_) \
+    if (diff_kind == FA_COEF_CELL) FA_BT(T_, true, M_); else FA_BT(T_, false, M_)
+        if (dtype == FA_F64) {
+            if (mass_kind) { FA_BT2(double, true); } else { FA_BT2(double, false); }
+        } else {
+            if (mass_kind) { FA_BT2(float, true); } else { FA_BT2(float, false); }
+        }
+#undef FA_BT2
+#undef FA_BT
+    } else if (dtype == FA_F64) {
+        if (sep) { FA_BOX_T(double, true); } else { FA_BOX_T(double, false); }
     } else {
-        if (sep) FA_BOX(float, true); else FA_BOX(float, false);
+        if (sep) { FA_BOX_T(float, true); } else { FA_BOX_T(float, false); }
     }
+#undef FA_BOX_T
 #undef FA_BOX
     timer_end(FA_TIMER_P1_BOX, S(stream));
     FA_LAUNCH("k_p1_box_rows");

@@ -363,7 +363,8 @@ def p1_matrix_values(space, ops, pat, dtype):
     values = torch.empty(pat.nnz, dtype=dtype, device=space.device)
     grid = _box_grid(space, pat)
     if grid is not None:         # from_box tetrahedra: one pass, cells found from each node's grid position
-        nat.check(L.fa_p1_box_matrix(nat.dtype_code(dtype), *grid, nat.ptr(mesh.node_rows16()), qw.shape[0],
+        nat.check(L.fa_p1_box_matrix(nat.dtype_code(dtype), *grid, nat.ptr(mesh.node_rows16()),
+                                     nat.ptr(mesh.box_axes()), qw.shape[0],
                                      nat.ptr(qw), nat.ptr(phi), dk, ds, nat.ptr(dc), mk, ms, nat.ptr(mc),
                                      nat.ptr(pat.indptr), pat.nnz, nat.ptr(values), nat.stream()), "fa_p1_box_matrix")
         return values

@@ -13,6 +13,7 @@ global DOF index downstream depends on it:
   cell-interior points; local order = multi-index order
   (triangle_mesh.py:218-270, tetrahedron_mesh.py:348-441, mesh_base.py:188-211).
 """
+import ctypes
 import threading
 from math import comb
 
@@ -212,6 +213,30 @@ class SimplexMesh:
         wide[:, :node.shape[1]] = node
         self._cache['_node_rows16'] = (key, node, wide)
         return wide
+
+    def box_axes(self):
+        """The grid lines (X, Y, Z concatenated, FP64 on the device) of a 3D
+        from_box mesh whose every node sits on them (fa_box_axes), else None;
+        checked once per node array and its in-place changes."""
+        grid = self._box_grid()
+        if grid is None or self.GD != 3:
+            return None
+        node = self.node
+        key = (id(node), node._version)
+        hit = self._cache.get('_box_axes')
+        if hit is not None and hit[0] == key and hit[1] is node:
+            return hit[2]
+        from .. import _native as nat
+        L = nat.lib()
+        nx, ny, nz = grid
+        axes = torch.empty(nx + ny + nz + 3, dtype=torch.float64, device=node.device)
+        ws = nat.workspace(L.fa_box_axes_work_bytes(), node.device)
+        on = ctypes.c_int32(0)
+        nat.check(L.fa_box_axes(nx, ny, nz, nat.ptr(self.node_rows16()), nat.ptr(axes), ctypes.byref(on),
+                                nat.ptr(ws), ws.numel(), nat.stream()), "fa_box_axes")
+        out = axes if on.value else None
+        self._cache['_box_axes'] = (key, node, out)
+        return out
 
     # -- topology ---------------------------------------------------------------
     def construct(self):

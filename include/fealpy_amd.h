@@ -250,15 +250,27 @@ int fa_assemble_p1_vector(int dtype, int td, int64_t gdof, int nq, const double*
  * BilinearForm.assembly's COO coalesce, bilinear_form.py:46-105,
  * coo_tensor.py:184-213).  FP64 values equal those of fa_p1_incidence_rows +
  * fa_sum_incidence_rows to a few units in the last place (the same element
- * rows and order; multiply-adds inside the element geometry may be contracted
- * differently), bitwise reproducible; FP32 values are the FP64 sums rounded
- * once.  node_wide: NN x 4 coordinate rows (x, y, z, pad) in from_box node
- * order; indptr: the space's CSR row pointer; nnz must be the P1 pattern's
- * (checked: FA_EINVAL otherwise).  Coefficient arguments as fa_p1_incidence_rows. */
-int fa_p1_box_matrix(int dtype, int nx, int ny, int nz, const double* node_wide, int nq, const double* qw,
-                     const double* phi, int diff_kind, double diff_scale, const double* diff_coef, int mass_kind,
-                     double mass_scale, const double* mass_coef, const int32_t* indptr, int64_t nnz, void* values,
-                     fa_stream_t stream);
+ * rows and order; a cube that is an axis-aligned box takes the closed form of
+ * the Kuhn tetrahedron's gradients and volume, any other the cross products),
+ * bitwise reproducible; FP32 values are the FP64 sums rounded once.
+ * node_wide: NN x 4 coordinate rows (x, y, z, pad) in from_box node order;
+ * axes: null, or the X, Y, Z arrays of fa_box_axes for a grid it reported on
+ * its lines (the nodes then are not read); indptr: the space's CSR row
+ * pointer; nnz must be the P1 pattern's (checked: FA_EINVAL otherwise).
+ * Coefficient arguments as fa_p1_incidence_rows. */
+int fa_p1_box_matrix(int dtype, int nx, int ny, int nz, const double* node_wide, const double* axes, int nq,
+                     const double* qw, const double* phi, int diff_kind, double diff_scale, const double* diff_coef,
+                     int mass_kind, double mass_scale, const double* mass_coef, const int32_t* indptr, int64_t nnz,
+                     void* values, fa_stream_t stream);
+
+/* Grid lines of a from_box node array: axes[0 .. nx] = x of nodes (i, 0, 0),
+ * then y of (0, j, 0), then z of (0, 0, k) (nx + ny + nz + 3 doubles), and
+ * *on_grid_host = 1 when every node (i, j, k) is at (X[i], Y[j], Z[k]) (a
+ * tensor grid, as from_box makes it), 0 otherwise; syncs.  No counterpart in
+ * the reference: it lets fa_p1_box_matrix skip the node rows. */
+size_t fa_box_axes_work_bytes(void);
+int fa_box_axes(int nx, int ny, int nz, const double* node_wide, double* axes, int32_t* on_grid_host, void* work,
+                size_t work_bytes, fa_stream_t stream);
 
 /* P1 load vector of a ScalarSourceIntegrator on TetrahedronMesh.from_box(nx,
  * ny, nz) (scalar_source_integrator.py, LinearForm.assembly,

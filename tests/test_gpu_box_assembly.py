@@ -6,8 +6,9 @@ run both paths against the reference):
 
 - FP64 values equal to the two-pass kernels' to a few units in the last
   place (every entry is the sum of its cell contributions in ascending cell
-  order in both; the compiler may contract multiply-adds in the element
-  geometry differently) and bitwise run to run: one-cube-thick and uneven
+  order in both; the element geometry of an axis-aligned cube is the Kuhn
+  closed form, elsewhere the same cross products, which the compiler may
+  contract differently) and bitwise run to run: one-cube-thick and uneven
   grids, box extents other than the unit cube, diffusion + mass with
   per-point and per-cell coefficients, and a separable expression evaluated
   in the kernel;
@@ -70,6 +71,59 @@ def test_box_equals_two_pass(dims, box, monkeypatch):
         assert torch.equal(box32, box64.to(torch.float32))
         again = _assemble(space, its)
         assert torch.equal(again, box64)                      # run to run: bitwise
+
+
+@pytest.mark.parametrize("moved", ["none", "one", "many"])
+def test_box_matrix_off_the_grid_lines(moved, monkeypatch):
+    """Cubes that are axis-aligned boxes take the closed-form Kuhn geometry,
+    cubes with a moved corner the cross-product geometry; either way the
+    two-pass kernels' values to a few units in the last place."""
+    dims = (5, 4, 6)
+    mesh = TetrahedronMesh.from_box([0, 1, -1, 2, 0, 0.5], nx=dims[0], ny=dims[1], nz=dims[2], device=DEV)
+    node = mesh.node
+    g = np.random.default_rng(3)
+    if moved == "one":
+        node[37, 2] += 0.01
+    elif moved == "many":
+        pick = torch.as_tensor(g.choice(node.shape[0], node.shape[0] // 3, replace=False), device=DEV)
+        node[pick] += torch.as_tensor(0.02 * g.standard_normal((pick.numel(), 3)), device=DEV)
+    space = fa.LagrangeFESpace(mesh, 1)
+    assert integrators._box_grid(space, pattern_of(space)) == dims
+    nc = mesh.number_of_cells()
+    kc = torch.as_tensor(np.exp(g.standard_normal(nc)), device=DEV)
+    for its in ([fa.ScalarDiffusionIntegrator(kc, q=2)],
+                [fa.ScalarDiffusionIntegrator(kc, q=3), fa.ScalarMassIntegrator(0.5, q=3)]):
+        box64 = _assemble(space, its)
+        monkeypatch.setattr(integrators, "BOX_ASSEMBLY", False)
+        monkeypatch.setattr(integrators, "TWO_PASS_MIN_CELLS", 0)
+        two = _assemble(space, its)
+        monkeypatch.setattr(integrators, "BOX_ASSEMBLY", True)
+        assert float((box64 - two).abs().max()) <= 1e-15 * float(two.abs().max())
+        assert torch.equal(_assemble(space, its), box64)
+
+
+def test_box_axes_follow_the_nodes(monkeypatch):
+    """fa_box_axes: the grid lines of a from_box mesh, None once a node
+    leaves them (checked again after an in-place change of the nodes), and
+    the matrix assembled through either path equals the two-pass one."""
+    mesh = TetrahedronMesh.from_box([0, 2, -1, 1, 0, 3], nx=4, ny=3, nz=5, device=DEV)
+    ax = mesh.box_axes()
+    assert ax is not None and ax.shape[0] == 5 + 4 + 6
+    ref = torch.cat([torch.linspace(0, 2, 5, dtype=torch.float64), torch.linspace(-1, 1, 4, dtype=torch.float64),
+                     torch.linspace(0, 3, 6, dtype=torch.float64)])
+    assert torch.allclose(ax.cpu(), ref, rtol=0, atol=1e-15)
+    assert mesh.box_axes() is ax                                   # cached
+    space = fa.LagrangeFESpace(mesh, 1)
+    its = [fa.ScalarDiffusionIntegrator(1.5, q=2)]
+    on_grid = _assemble(space, its)
+    mesh.node[17, 0] += 0.05                                       # in place: the check runs again
+    assert mesh.box_axes() is None
+    moved = _assemble(space, its)
+    monkeypatch.setattr(integrators, "BOX_ASSEMBLY", False)
+    monkeypatch.setattr(integrators, "TWO_PASS_MIN_CELLS", 0)
+    two = _assemble(space, its)
+    assert float((moved - two).abs().max()) <= 1e-15 * float(two.abs().max())
+    assert not torch.equal(moved, on_grid)
 
 
 @pytest.mark.parametrize("dims", [(1, 1, 1), (4, 3, 5), (9, 7, 8)])
@@ -156,12 +210,12 @@ def test_box_refuses_a_foreign_pattern():
     _, qw, phi, _ = space.tables(2)
     L = nat.lib()
     vals = torch.zeros(pat.nnz, dtype=torch.float64, device=DEV)
-    st = L.fa_p1_box_matrix(nat.FA_F64, 3, 3, 3, nat.ptr(mesh.node_rows16()), qw.shape[0], nat.ptr(qw),
+    st = L.fa_p1_box_matrix(nat.FA_F64, 3, 3, 3, nat.ptr(mesh.node_rows16()), None, qw.shape[0], nat.ptr(qw),
                             nat.ptr(phi), nat.COEF_CONST, 1.0, None, nat.COEF_NONE, 0.0, None, nat.ptr(pat.indptr),
                             pat.nnz - 1, nat.ptr(vals), nat.stream())
     assert st == nat.FA_EINVAL
     assert b"not the P1 pattern" in L.fa_last_error()
-    st = L.fa_p1_box_matrix(nat.FA_F64, 3, 4, 3, nat.ptr(mesh.node_rows16()), qw.shape[0], nat.ptr(qw),
+    st = L.fa_p1_box_matrix(nat.FA_F64, 3, 4, 3, nat.ptr(mesh.node_rows16()), None, qw.shape[0], nat.ptr(qw),
                             nat.ptr(phi), nat.COEF_CONST, 1.0, None, nat.COEF_NONE, 0.0, None, nat.ptr(pat.indptr),
                             pat.nnz, nat.ptr(vals), nat.stream())
     assert st == nat.FA_EINVAL
