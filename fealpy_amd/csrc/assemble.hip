@@ -2157,19 +2157,37 @@ __global__ void __launch_bounds__(256) k_p1_box_cell_vectors(int nx, int ny, int
     }
 }
 
+// the values of cube (ix - A, iy - B, iz - C) at the node: loads only (cell
+// 0 for a cube outside the grid, masked when summed)
 template <int A, int B, int C>
-__device__ inline void box_cube_sum(int ix, int iy, int iz, int nx, int ny, int nz, const double* __restrict__ vals,
-                                    double& acc) {
+__device__ inline void box_cube_load(int ix, int iy, int iz, int nx, int ny, int nz, const double* __restrict__ vals,
+                                     double v[6], bool& valid) {
     const int cx = ix - A, cy = iy - B, cz = iz - C;
-    if (cx < 0 || cy < 0 || cz < 0 || cx >= nx || cy >= ny || cz >= nz) return;
+    valid = cx >= 0 && cy >= 0 && cz >= 0 && cx < nx && cy < ny && cz < nz;
     constexpr int KAP = box_corner_of(A, B, C);
-    const int64_t cube = (static_cast<int64_t>(cx) * ny + cy) * nz + cz;
-    double v[6];
+    const int64_t cube = valid ? (static_cast<int64_t>(cx) * ny + cy) * nz + cz : 0;
 #pragma unroll
-    for (int t = 0; t < 6; ++t) {  // the loads first, then the adds in order
+    for (int t = 0; t < 6; ++t) {
         const int i = box_local(t, KAP);
         v[t] = i < 0 ? 0.0 : vals[(6 * cube + t) * 4 + i];
     }
+}
+
+// the loaded values held in registers from here on (placed after all the
+// loads of a row, so they are in flight together; the compiler otherwise
+// sinks each load to its use)
+template <int A, int B, int C>
+__device__ inline void box_cube_pin(const double v[6]) {
+    constexpr int KAP = box_corner_of(A, B, C);
+#pragma unroll
+    for (int t = 0; t < 6; ++t)
+        if (box_local(t, KAP) >= 0) asm volatile("" : : "v"(v[t]));
+}
+
+template <int A, int B, int C>
+__device__ inline void box_cube_add(const double v[6], bool valid, double& acc) {
+    constexpr int KAP = box_corner_of(A, B, C);
+    if (!valid) return;
 #pragma unroll
     for (int t = 0; t < 6; ++t)
         if (box_local(t, KAP) >= 0) acc += v[t];
@@ -2184,15 +2202,33 @@ __global__ void __launch_bounds__(256) k_p1_box_sum_vectors(int nx, int ny, int 
         const int64_t rem = r - ix * nyz;
         const int iy = static_cast<int>(rem / nz1);
         const int iz = static_cast<int>(rem - iy * nz1);
-        double acc = 0.0;
-        box_cube_sum<1, 1, 1>(ix, iy, iz, nx, ny, nz, vals, acc);
-        box_cube_sum<1, 1, 0>(ix, iy, iz, nx, ny, nz, vals, acc);
-        box_cube_sum<1, 0, 1>(ix, iy, iz, nx, ny, nz, vals, acc);
-        box_cube_sum<1, 0, 0>(ix, iy, iz, nx, ny, nz, vals, acc);
-        box_cube_sum<0, 1, 1>(ix, iy, iz, nx, ny, nz, vals, acc);
-        box_cube_sum<0, 1, 0>(ix, iy, iz, nx, ny, nz, vals, acc);
-        box_cube_sum<0, 0, 1>(ix, iy, iz, nx, ny, nz, vals, acc);
-        box_cube_sum<0, 0, 0>(ix, iy, iz, nx, ny, nz, vals, acc);
+        double v[8][6];
+        bool ok[8];
+        box_cube_load<1, 1, 1>(ix, iy, iz, nx, ny, nz, vals, v[0], ok[0]);
+        box_cube_load<1, 1, 0>(ix, iy, iz, nx, ny, nz, vals, v[1], ok[1]);
+        box_cube_load<1, 0, 1>(ix, iy, iz, nx, ny, nz, vals, v[2], ok[2]);
+        box_cube_load<1, 0, 0>(ix, iy, iz, nx, ny, nz, vals, v[3], ok[3]);
+        box_cube_load<0, 1, 1>(ix, iy, iz, nx, ny, nz, vals, v[4], ok[4]);
+        box_cube_load<0, 1, 0>(ix, iy, iz, nx, ny, nz, vals, v[5], ok[5]);
+        box_cube_load<0, 0, 1>(ix, iy, iz, nx, ny, nz, vals, v[6], ok[6]);
+        box_cube_load<0, 0, 0>(ix, iy, iz, nx, ny, nz, vals, v[7], ok[7]);
+        box_cube_pin<1, 1, 1>(v[0]);
+        box_cube_pin<1, 1, 0>(v[1]);
+        box_cube_pin<1, 0, 1>(v[2]);
+        box_cube_pin<1, 0, 0>(v[3]);
+        box_cube_pin<0, 1, 1>(v[4]);
+        box_cube_pin<0, 1, 0>(v[5]);
+        box_cube_pin<0, 0, 1>(v[6]);
+        box_cube_pin<0, 0, 0>(v[7]);
+        double acc = 0.0;  // the cells in ascending order
+        box_cube_add<1, 1, 1>(v[0], ok[0], acc);
+        box_cube_add<1, 1, 0>(v[1], ok[1], acc);
+        box_cube_add<1, 0, 1>(v[2], ok[2], acc);
+        box_cube_add<1, 0, 0>(v[3], ok[3], acc);
+        box_cube_add<0, 1, 1>(v[4], ok[4], acc);
+        box_cube_add<0, 1, 0>(v[5], ok[5], acc);
+        box_cube_add<0, 0, 1>(v[6], ok[6], acc);
+        box_cube_add<0, 0, 0>(v[7], ok[7], acc);
         out[r] = static_cast<T>(acc);
     }
 }
