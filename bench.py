@@ -47,12 +47,12 @@ SPMV_F32_TRAFFIC_128 = 267.3e6
 # over `bench.py --spmv-only` scaled the same way
 # (profiles/step_128cube_counters.md): k_pcg_spmv<double> on the FP64
 # operator; the finest level's FP16 smoother steps and residual (mean over
-# the family's launches); the structured P1 matrix kernel; the structured
-# load vector (both passes, with its cell-order buffer)
+# the family's launches); the structured P1 matrix kernel on the tensor grid;
+# the structured load vector (both passes, with its cell-order buffer)
 PCG_SPMV_TRAFFIC_128 = 403.0e6
 SMOOTH0_TRAFFIC_128 = 164.0e6
-BOX_A_TRAFFIC_128 = 446.9e6
-BOX_F_TRAFFIC_128 = 1203.6e6
+BOX_A_TRAFFIC_128 = 403.8e6
+BOX_F_TRAFFIC_128 = 1261.2e6
 # serving: slots (streams) of the continuous-batching server and the hardware
 # queues the process asks for (the slots' streams, the default stream and
 # torch's side streams each get one); read by HIP when it initialises
@@ -276,11 +276,21 @@ def rooflines(kt, mesh, gdof, nnz_a, nnz_bc, n, vb, vb_a=8):
     (l3, m3), (l4, m4) = kt["f_pass1"], kt["f_pass2"]
     lb, mb = kt["a_box"]
     if lb:      # the structured from_box kernel: no cell array (cells come from the grid position)
-        b_box = 24 * nn + 8 * nc + 4 * (gdof + 1) + vb_a * nnz_a
-        asm_a = _roof("k_p1_box_rows", b_box, lb, mb, BOX_A_TRAFFIC_128 if big else None,
-                      note="one pass: coordinates, k and the row pointer in, CSR values out (no cell array or "
-                           f"incidence buffer); the generic kernels' compulsory count with cells would be {b_a} B",
-                      generic_algorithmic_bytes=int(b_a))
+        axes = mesh.box_axes()
+        if axes is not None:    # tensor grid: the grid lines instead of the node rows
+            b_box = 8 * axes.numel() + 8 * nc + 4 * (gdof + 1) + vb_a * nnz_a
+            asm_a = _roof("k_p1_box_rows_tensor", b_box, lb, mb, BOX_A_TRAFFIC_128 if big else None,
+                          note="one pass on a tensor grid: the grid lines, k and the row pointer in, CSR values out "
+                               "(no node rows, cell array or incidence buffer); with the node coordinates the count "
+                               f"would be {24 * nn + 8 * nc + 4 * (gdof + 1) + vb_a * nnz_a} B, the generic kernels' "
+                               f"with cells {b_a} B",
+                          generic_algorithmic_bytes=int(b_a))
+        else:
+            b_box = 24 * nn + 8 * nc + 4 * (gdof + 1) + vb_a * nnz_a
+            asm_a = _roof("k_p1_box_rows", b_box, lb, mb, None,
+                          note="one pass: coordinates, k and the row pointer in, CSR values out (no cell array or "
+                               f"incidence buffer); the generic kernels' compulsory count with cells would be {b_a} B",
+                          generic_algorithmic_bytes=int(b_a))
     else:
         asm_a = _roof("k_p1_incidence_rows<3> + k_sum_incidence_rows", b_a, 1, m1 + m2,
                       pass1_us=round(m1 * 1e3 / max(l1, 1), 2), pass2_us=round(m2 * 1e3 / max(l2, 1), 2))
