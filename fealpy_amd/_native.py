@@ -140,6 +140,8 @@ _SIGNATURES = {
     "fa_jacobi_setup": (ctypes.c_int, [ctypes.c_int, _i64, _i64, _p, _p, _p, _p, _pf64, _p, _sz, _p]),
     "fa_pcg_work_bytes": (_sz, [ctypes.c_int, _i64]),
     "fa_pcg": (ctypes.c_int, [ctypes.c_int, _p, _p, _p, _p, _f64, _f64, _i64, _i32, _pi64, _pf64, _p, _sz, _p]),
+    "fa_pcg_from_zero": (ctypes.c_int, [ctypes.c_int, _p, _p, _p, _p, _f64, _f64, _i64, _i32, _pi64, _pf64, _p, _sz,
+                                        _p]),
     "fa_pcg_graph_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
     "fa_pcg_graph_destroy": (None, [_p]),
     "fa_pcg_static": (ctypes.c_int, [ctypes.c_int, _p, _p, _p, _p, _f64, _f64, _i64, _i32, _pi64, _pf64, _p]),

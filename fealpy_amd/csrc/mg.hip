@@ -1413,6 +1413,28 @@ __global__ void __launch_bounds__(RED_BLOCK) k_pcg_init(int64_t n, const int32_t
     store_partials<2>(v, partials);
 }
 
+// the same for x0 = 0: x = 0, r = b (b - A 0 is b exactly), with no product
+template <class T, class U>
+__global__ void __launch_bounds__(RED_BLOCK) k_pcg_init_zero(int64_t n, const T* __restrict__ b, T* __restrict__ x,
+                                                            T* __restrict__ r, U* __restrict__ ru, T* __restrict__ p,
+                                                            double* __restrict__ partials) {
+    double rr = 0.0, bb = 0.0;
+    for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
+         row += (int64_t)gridDim.x * blockDim.x) {
+        if (row < n) {
+            const T bv = b[row];
+            x[row] = T(0);
+            r[row] = bv;
+            if (ru) ru[row] = static_cast<U>(bv);
+            else p[row] = bv;
+            rr += double(bv) * double(bv);
+            bb += double(bv) * double(bv);
+        }
+    }
+    const double v[2] = {block_sum(rr), block_sum(bb)};
+    store_partials<2>(v, partials);
+}
+
 template <class T>
 __global__ void k_pcg_trivial(int64_t n, T* __restrict__ x, const PcgState* st) {
     if (st->done != 2) return;
@@ -1563,7 +1585,7 @@ static int64_t predicted_chunk(const PcgState& h, int64_t most) {
 template <class T, class U>
 static int pcg_run(const fa_sell_t* A, const fa_mg_t* M, const T* b, T* x, double atol, double rtol, int64_t maxit,
                    int check_every, int64_t* niter_host, double* res_host, void* work, size_t work_bytes,
-                   hipStream_t s, PcgGraph* pg = nullptr) {
+                   hipStream_t s, PcgGraph* pg = nullptr, bool x0_zero = false) {
     const int64_t n = A->n;
     Arena ar(work, work_bytes);
     T* r = ar.take<T>(n);
@@ -1584,8 +1606,11 @@ static int pcg_run(const fa_sell_t* A, const fa_mg_t* M, const T* b, T* x, doubl
     const T* sv = static_cast<const T*>(A->vals);
     const int64_t cap = maxit > 0 ? maxit : (1LL << 62);
     hipLaunchKernelGGL(k_pcg_set, one, one, 0, s, st, atol, rtol, (long long)cap);
-    hipLaunchKernelGGL((k_pcg_init<T, U>), grid, block, 0, s, n, A->slice_ptr, A->cols, sv, b, x, r, ru, p,
-                       partials);
+    if (x0_zero)
+        hipLaunchKernelGGL((k_pcg_init_zero<T, U>), grid, block, 0, s, n, b, x, r, ru, p, partials);
+    else
+        hipLaunchKernelGGL((k_pcg_init<T, U>), grid, block, 0, s, n, A->slice_ptr, A->cols, sv, b, x, r, ru, p,
+                           partials);
     hipLaunchKernelGGL(k_pcg_finish<PCG_INIT>, one, block, 0, s, partials, np, st);
     hipLaunchKernelGGL(k_pcg_trivial<T>, grid, block, 0, s, n, x, st);
     FA_LAUNCH("k_pcg_init");
@@ -1926,7 +1951,7 @@ extern "C" size_t fa_pcg_work_bytes(int dtype, int64_t n) {
 
 static int pcg_dispatch(int dtype, const fa_sell_t* A, const fa_mg_t* M, const void* b, void* x, double atol,
                         double rtol, int64_t maxit, int32_t check_every, int64_t* niter_host, double* residual_host,
-                        void* work, size_t work_bytes, hipStream_t s, PcgGraph* pg) {
+                        void* work, size_t work_bytes, hipStream_t s, PcgGraph* pg, bool x0_zero = false) {
     FA_REQUIRE(A && A->n > 0 && check_every > 0, "fa_pcg: bad arguments");
     if (M) {
         FA_REQUIRE(M->nlevels >= 1 && M->levels && (M->coarse_chol || M->coarse_degree >= 1),
@@ -1938,7 +1963,7 @@ static int pcg_dispatch(int dtype, const fa_sell_t* A, const fa_mg_t* M, const v
     const int ud = M ? M->dtype : dtype;
 #define FA_PCG(T, U)                                                                                         \
     return pcg_run<T, U>(A, M, static_cast<const T*>(b), static_cast<T*>(x), atol, rtol, maxit, check_every, \
-                         niter_host, residual_host, work, work_bytes, s, pg)
+                         niter_host, residual_host, work, work_bytes, s, pg, x0_zero)
     if (dtype == FA_F64 && ud == FA_F64) FA_PCG(double, double);
     if (dtype == FA_F64 && ud == FA_F32) FA_PCG(double, float);
     if (dtype == FA_F32 && ud == FA_F32) FA_PCG(float, float);
@@ -1952,6 +1977,14 @@ extern "C" int fa_pcg(int dtype, const fa_sell_t* A, const fa_mg_t* M, const voi
                       void* work, size_t work_bytes, fa_stream_t stream) {
     return pcg_dispatch(dtype, A, M, b, x, atol, rtol, maxit, check_every, niter_host, residual_host, work,
                         work_bytes, S(stream), nullptr);
+}
+
+extern "C" int fa_pcg_from_zero(int dtype, const fa_sell_t* A, const fa_mg_t* M, const void* b, void* x,
+                                double atol, double rtol, int64_t maxit, int32_t check_every, int64_t* niter_host,
+                                double* residual_host, void* work, size_t work_bytes, fa_stream_t stream) {
+    FA_REQUIRE(x, "fa_pcg_from_zero: bad arguments");
+    return pcg_dispatch(dtype, A, M, b, x, atol, rtol, maxit, check_every, niter_host, residual_host, work,
+                        work_bytes, S(stream), nullptr, true);
 }
 
 // ---- fa_pcg_static: the PCG workspace in a file-scope device array --------

@@ -574,7 +574,10 @@ class GeometricMG:
         else:
             S = sell_of(A)
         b = b.to(dt).contiguous()
-        x = torch.zeros_like(b) if x0 is None else x0.to(dt).clone().contiguous()
+        # from zero (no x0, no graph): fa_pcg_from_zero writes x = 0 itself and
+        # skips the product A x0
+        from_zero = x0 is None and not self.graphs
+        x = torch.empty_like(b) if from_zero else (torch.zeros_like(b) if x0 is None else x0.to(dt).clone().contiguous())
         ws = nat.workspace(L.fa_pcg_work_bytes(nat.dtype_code(dt), b.shape[0]), b.device)
         niter, res = ctypes.c_int64(0), ctypes.c_double(0.0)
         d = S.desc(dt)
@@ -588,7 +591,7 @@ class GeometricMG:
                 self._graph = h
             st = L.fa_pcg_graph(*args, self._graph, nat.stream())
         else:
-            st = L.fa_pcg(*args, nat.stream())
+            st = (L.fa_pcg_from_zero if from_zero else L.fa_pcg)(*args, nat.stream())
         if st != nat.FA_ENOCONV:
             nat.check(st, "fa_pcg")
         return x, {'residual': float(res.value), 'niter': int(niter.value)}

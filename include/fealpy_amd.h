@@ -727,6 +727,13 @@ size_t fa_pcg_work_bytes(int dtype, int64_t n);
 int fa_pcg(int dtype, const fa_sell_t* A, const fa_mg_t* M, const void* b, void* x, double atol, double rtol,
            int64_t maxit, int32_t check_every, int64_t* niter_host, double* residual_host,
            void* work, size_t work_bytes, fa_stream_t stream);
+/* fa_pcg with x0 = 0: x is output only (set to 0 before the first update),
+ * and the initial residual is b itself, with no product A x0; the same
+ * iterates and result as fa_pcg on a zero x, one SpMV pass fewer (the inner
+ * solves of iterative refinement start from zero). */
+int fa_pcg_from_zero(int dtype, const fa_sell_t* A, const fa_mg_t* M, const void* b, void* x, double atol,
+                     double rtol, int64_t maxit, int32_t check_every, int64_t* niter_host, double* residual_host,
+                     void* work, size_t work_bytes, fa_stream_t stream);
 
 /* fa_pcg without a workspace argument.  No counterpart in the reference: it
  * is for callers that cannot allocate device memory (code in a tight serving

@@ -625,6 +625,22 @@ def test_pcg_result_independent_of_host_checks():
     assert info["niter"] == 5
 
 
+@pytest.mark.parametrize("dt", [torch.float64, torch.float32])
+def test_pcg_from_zero_equals_zero_start(dt):
+    """fa_pcg_from_zero (MG.pcg with no x0) against fa_pcg started from an
+    explicit zero vector: the same iterate, iteration count and residual,
+    bit for bit, whatever x held before; a zero right-hand side gives x = 0."""
+    mesh, space, bc, A, b = _system(12)
+    from fealpy_amd.sparse import CSRTensor
+    Ad = A if dt == torch.float64 else CSRTensor(A.crow, A.col, A.values.to(dt), A.shape)
+    mg = GeometricMG(space, bc.is_boundary_dof, dtype=torch.float32)
+    x1, i1 = mg.pcg(Ad, b, atol=0.0, rtol=1e-9, maxit=200)
+    x2, i2 = mg.pcg(Ad, b, x0=torch.zeros_like(b), atol=0.0, rtol=1e-9, maxit=200)
+    assert torch.equal(x1, x2) and i1 == i2
+    z, iz = mg.pcg(Ad, torch.zeros_like(b), atol=0.0, rtol=1e-9, maxit=200)
+    assert float(z.abs().max()) == 0.0 and iz["niter"] == 0
+
+
 def test_cg_maxit_and_trivial_rhs():
     mesh, space, bc, A, b = _system(8)
     x, info = fa.cg(A, b, maxit=3, returninfo=True)
