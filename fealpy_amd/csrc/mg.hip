@@ -422,6 +422,104 @@ __global__ void __launch_bounds__(256) k_sell_apply(int64_t n, const int32_t* __
     }
 }
 
+// ---- transfers of nested from_box grids, from the grid -------------------------
+// Fine grid (nx, ny, nz) (even counts), coarse grid (nx/2, ny/2, nz/2).  The
+// P1 prolongation gives fine node f the coarse node f/2 (weight 1) when every
+// coordinate of f is even, else 1/2 of each end of the Kuhn edge through f,
+// floor(f/2) and floor(f/2) + (f mod 2); rows of Dirichlet fine nodes and
+// columns of Dirichlet coarse nodes are absent.  R = P^T.  Each sum adds its
+// terms in the SELL rows' column order (ascending node index), and the
+// weights 1/2 and 1 make every product exact, so the values equal the SELL
+// transfers'.
+__device__ inline int64_t grid_node(int x, int y, int z, int ny, int nz) {
+    return (static_cast<int64_t>(x) * (ny + 1) + y) * (nz + 1) + z;
+}
+
+// b_c = R r over the coarse nodes: the fine nodes 2I + o, o = 0 (weight 1)
+// or a Kuhn edge direction (weight 1/2), in lexicographic order of o
+template <class T>
+__global__ void __launch_bounds__(256) k_box_restrict(int mx, int my, int mz, const uint8_t* __restrict__ bd_f,
+                                                     const uint8_t* __restrict__ bd_c, const T* __restrict__ r,
+                                                     T* __restrict__ bc, const int* __restrict__ done) {
+    if (done && *done) return;
+    const int nx = 2 * mx, ny = 2 * my, nz = 2 * mz;
+    // node counts of the levels are below 2^31 (int32 SELL indices): 32-bit
+    // index arithmetic
+    const uint32_t nmz = mz + 1, nmyz = static_cast<uint32_t>(my + 1) * nmz, nc = static_cast<uint32_t>(mx + 1) * nmyz;
+    for (uint32_t I = xcd_block() * blockDim.x + threadIdx.x; I < nc; I += gridDim.x * blockDim.x) {
+        const int ix = static_cast<int>(I / nmyz);
+        const uint32_t rem = I - ix * nmyz;
+        const int iy = static_cast<int>(rem / nmz);
+        const int iz = static_cast<int>(rem - iy * nmz);
+        const int fx = 2 * ix, fy = 2 * iy, fz = 2 * iz;
+        T v[15];
+        uint8_t fb[15];
+        bool in[15];
+        int q = 0;
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx)
+#pragma unroll
+            for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                for (int dz = -1; dz <= 1; ++dz) {
+                    const bool pos = dx >= 0 && dy >= 0 && dz >= 0, neg = dx <= 0 && dy <= 0 && dz <= 0;
+                    if (!(pos || neg)) continue;
+                    const int x = fx + dx, y = fy + dy, z = fz + dz;
+                    in[q] = x >= 0 && x <= nx && y >= 0 && y <= ny && z >= 0 && z <= nz;
+                    const int64_t f = in[q] ? grid_node(x, y, z, ny, nz) : grid_node(fx, fy, fz, ny, nz);
+                    v[q] = r[f];
+                    fb[q] = bd_f[f];
+                    ++q;
+                }
+        const uint8_t cb = bd_c[I];
+        T acc = T(0);
+        q = 0;
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx)
+#pragma unroll
+            for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                for (int dz = -1; dz <= 1; ++dz) {
+                    const bool pos = dx >= 0 && dy >= 0 && dz >= 0, neg = dx <= 0 && dy <= 0 && dz <= 0;
+                    if (!(pos || neg)) continue;
+                    const T w = (dx == 0 && dy == 0 && dz == 0) ? T(1) : T(0.5);
+                    if (in[q] && !fb[q]) acc += w * v[q];
+                    ++q;
+                }
+        bc[I] = cb ? T(0) : acc;
+    }
+}
+
+// x_f += P x_c over the fine nodes
+template <class T>
+__global__ void __launch_bounds__(256) k_box_prolong(int nx, int ny, int nz, const uint8_t* __restrict__ bd_f,
+                                                    const uint8_t* __restrict__ bd_c, const T* __restrict__ xc,
+                                                    T* __restrict__ xf, const int* __restrict__ done) {
+    if (done && *done) return;
+    const int my = ny / 2, mz = nz / 2;
+    // node counts below 2^31: 32-bit index arithmetic
+    const uint32_t nz1 = nz + 1, nyz = static_cast<uint32_t>(ny + 1) * nz1, nf = static_cast<uint32_t>(nx + 1) * nyz;
+    for (uint32_t f = xcd_block() * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x) {
+        const int x = static_cast<int>(f / nyz);
+        const uint32_t rem = f - x * nyz;
+        const int y = static_cast<int>(rem / nz1);
+        const int z = static_cast<int>(rem - y * nz1);
+        const int ox = x & 1, oy = y & 1, oz = z & 1;
+        const bool two = ox | oy | oz;
+        const int64_t c0 = grid_node(x >> 1, y >> 1, z >> 1, my, mz);
+        const int64_t c1 = two ? grid_node((x >> 1) + ox, (y >> 1) + oy, (z >> 1) + oz, my, mz) : c0;
+        const uint8_t fb = bd_f[f], b0 = bd_c[c0], b1 = bd_c[c1];
+        const T x0 = xc[c0], x1 = xc[c1], y0 = xf[f];
+        const T w = two ? T(0.5) : T(1);
+        T acc = T(0);
+        if (!fb) {
+            if (!b0) acc += w * x0;
+            if (two && !b1) acc += w * x1;
+        }
+        xf[f] = y0 + acc;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // coarsest level: dense Cholesky (one workgroup)
 __global__ void __launch_bounds__(256) k_csr_to_dense(int64_t n, const int32_t* __restrict__ ptr,
@@ -1272,14 +1370,28 @@ struct Cycle {
         else
             hipLaunchKernelGGL(k_residual<T>, dim3(g), dim3(256), 0, s, L.A.n, L.A.slice_ptr, L.A.cols,
                                static_cast<const T*>(L.A.vals), b, cur, r, done);
-        hipLaunchKernelGGL((k_sell_apply<T, false>), dim3(grid(L.R.n)), dim3(256), 0, s, L.R.n, L.R.slice_ptr,
-                           L.R.cols, static_cast<const T*>(L.R.vals), r, static_cast<T*>(C.b), done);
+        // the stencils only where both levels are the grids' node sets
+        const bool structured =
+            L.grid[0] > 0 && L.grid[1] > 0 && L.grid[2] > 0 && L.bd_fine && L.bd_coarse &&
+            (L.grid[0] | L.grid[1] | L.grid[2]) % 2 == 0 &&
+            L.A.n < INT32_MAX && static_cast<int64_t>(L.grid[0] + 1) * (L.grid[1] + 1) * (L.grid[2] + 1) == L.A.n &&
+            static_cast<int64_t>(L.grid[0] / 2 + 1) * (L.grid[1] / 2 + 1) * (L.grid[2] / 2 + 1) == C.A.n;
+        if (structured)
+            hipLaunchKernelGGL(k_box_restrict<T>, dim3(xcd_grid(grid(C.A.n))), dim3(256), 0, s, L.grid[0] / 2,
+                               L.grid[1] / 2, L.grid[2] / 2, L.bd_fine, L.bd_coarse, r, static_cast<T*>(C.b), done);
+        else
+            hipLaunchKernelGGL((k_sell_apply<T, false>), dim3(grid(L.R.n)), dim3(256), 0, s, L.R.n, L.R.slice_ptr,
+                               L.R.cols, static_cast<const T*>(L.R.vals), r, static_cast<T*>(C.b), done);
         FA_LAUNCH("restrict");
         T* xc = nullptr;
         st = run(l + 1, static_cast<const T*>(C.b), &xc);
         if (st) return st;
-        hipLaunchKernelGGL((k_sell_apply<T, true>), dim3(grid(L.P.n)), dim3(256), 0, s, L.P.n, L.P.slice_ptr,
-                           L.P.cols, static_cast<const T*>(L.P.vals), xc, cur, done);
+        if (structured)
+            hipLaunchKernelGGL(k_box_prolong<T>, dim3(xcd_grid(grid(L.A.n))), dim3(256), 0, s, L.grid[0], L.grid[1],
+                               L.grid[2], L.bd_fine, L.bd_coarse, xc, cur, done);
+        else
+            hipLaunchKernelGGL((k_sell_apply<T, true>), dim3(grid(L.P.n)), dim3(256), 0, s, L.P.n, L.P.slice_ptr,
+                               L.P.cols, static_cast<const T*>(L.P.vals), xc, cur, done);
         FA_LAUNCH("prolongate");
         const bool direct = z && mg->degree >= 1 && z != cur && z != nxt;
         st = smooth(L, b, cur, nxt, false, direct ? z : nullptr, direct ? dot : nullptr);

@@ -39,7 +39,13 @@ from .cg import maxit_arg
 class FaMgLevel(ctypes.Structure):
     _fields_ = [("A", FaSell), ("dinv", ctypes.c_void_p), ("R", FaSell), ("P", FaSell), ("lmax", ctypes.c_double),
                 ("b", ctypes.c_void_p), ("x", ctypes.c_void_p), ("x2", ctypes.c_void_p), ("d", ctypes.c_void_p),
-                ("r", ctypes.c_void_p), ("Ah", FaSell16), ("diag", ctypes.c_void_p), ("width", ctypes.c_int32)]
+                ("r", ctypes.c_void_p), ("Ah", FaSell16), ("diag", ctypes.c_void_p), ("width", ctypes.c_int32),
+                ("grid", ctypes.c_int32 * 3), ("bd_fine", ctypes.c_void_p), ("bd_coarse", ctypes.c_void_p)]
+
+
+# nested 3D P1 grid levels apply R and P from the grid (k_box_restrict,
+# k_box_prolong) instead of their SELL matrices: the same values
+GRID_TRANSFERS = True
 
 
 class FaMg(ctypes.Structure):
@@ -273,7 +279,7 @@ class GeometricMG:
         self._graph = None
         self.device = dev
         self.levels = []
-        bd = is_bd.to(torch.bool)
+        bd = is_bd.to(torch.bool).contiguous()
         if bd.shape[0] != space.number_of_global_dofs():
             raise ValueError("is_bd must flag every DOF of the space")
         lev = _Level()
@@ -299,6 +305,8 @@ class GeometricMG:
             g = _grid_index(dims_c, torch.arange(nc_s, device=dev))
             bd_c = _select_nodes(bd, k, prio, nf_s, _node_ids(dims, _coarse_to_fine_node(dims, dims_c, g)))
             self._link(lev, _block_prolongation(*_grid_prolongation(dims, dev), k, prio, nf_s, nc_s, bd, bd_c))
+            # nested scalar 3D grids: the transfers are the Kuhn-edge stencils
+            lev.grid_transfer = tuple(dims) if (k == 1 and len(dims) == 3 and all(n % 2 == 0 for n in dims)) else None
             nxt = _Level()
             nxt.dims, nxt.n, nxt.bd = dims_c, k * nc_s, bd_c
             self.levels.append(nxt)
@@ -531,6 +539,14 @@ class GeometricMG:
             else:
                 e.Ah = FaSell16(0, None, None, None)
                 e.diag = None
+            gt = getattr(lv, 'grid_transfer', None) if (GRID_TRANSFERS and i < n - 1) else None
+            if gt is not None:     # Dirichlet flags as bytes (torch.bool is one byte a flag)
+                e.grid = (ctypes.c_int32 * 3)(*gt)
+                e.bd_fine = lv.bd.data_ptr()
+                e.bd_coarse = self.levels[i + 1].bd.data_ptr()
+            else:
+                e.grid = (ctypes.c_int32 * 3)(0, 0, 0)
+                e.bd_fine = e.bd_coarse = None
         self._lv_array = arr
         dense = self.levels[-1].dense
         self._desc = FaMg(nat.dtype_code(self.dtype), n, arr, self.degree, self.lower, self.upper,

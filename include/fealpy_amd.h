@@ -604,6 +604,16 @@ typedef struct {
     int32_t width;             /* longest row of A (0: unknown); pre-smoothing
                                   fuses its first two steps only for rows of at
                                   most 32 entries                               */
+    int32_t grid[3];           /* cell counts of this level's TetrahedronMesh
+                                  from_box grid when R and P are the P1
+                                  transfers to the nested grid of half the
+                                  counts (all counts even, one component); then
+                                  the V-cycle applies R and P from the grid
+                                  (fine node 2I + o, o a Kuhn edge direction,
+                                  weight 1/2; o = 0, weight 1) with the values
+                                  of the SELL transfers.  {0, 0, 0} otherwise  */
+    const uint8_t* bd_fine;    /* with grid: Dirichlet flags of this level's    */
+    const uint8_t* bd_coarse;  /* nodes and of the next coarser level's         */
 } fa_mg_level_t;
 
 typedef struct {

@@ -641,6 +641,33 @@ def test_pcg_from_zero_equals_zero_start(dt):
     assert float(z.abs().max()) == 0.0 and iz["niter"] == 0
 
 
+@pytest.mark.parametrize("dt", [torch.float32, torch.float64])
+def test_grid_transfers_equal_sell_transfers(dt, monkeypatch):
+    """Nested 3D levels apply R and P from the grid (k_box_restrict,
+    k_box_prolong): the PCG iterate, iteration count and residual equal those
+    with the SELL transfers bit for bit, with a Dirichlet boundary, a
+    lognormal coefficient and a level of odd counts (16 x 8 x 12 -> 8 x 4 x 6
+    -> 4 x 2 x 3, whose next transfer keeps its SELL matrix)."""
+    from fealpy_amd.solver import mg as mgmod
+    mesh = fa.TetrahedronMesh.from_box(nx=16, ny=8, nz=12, device=DEV)
+    u = fa.cartesian(lambda x: torch.sin(PI * x[..., 0]) * torch.sin(PI * x[..., 1]) * x[..., 2])
+    space = fa.LagrangeFESpace(mesh, 1)
+    k = _lognormal(mesh, seed=9)
+    A = fa.BilinearForm(space).add_integrator(fa.ScalarDiffusionIntegrator(k, q=2)).assembly()
+    b = fa.LinearForm(space).add_integrator(fa.ScalarSourceIntegrator(1.0, q=2)).assembly()
+    bc = fa.DirichletBC(space, gd=u)
+    A, b = bc.apply(A, b)
+    out = {}
+    for grid in (True, False):
+        monkeypatch.setattr(mgmod, "GRID_TRANSFERS", grid)
+        mg = GeometricMG(space, bc.is_boundary_dof, dtype=dt, coarse_max=8)
+        assert mg.levels[0].grid_transfer == (16, 8, 12) and mg.levels[1].grid_transfer == (8, 4, 6)
+        assert mg.levels[2].grid_transfer is None
+        out[grid] = mg.pcg(A, b, atol=0.0, rtol=1e-10, maxit=200)
+    (x1, i1), (x2, i2) = out[True], out[False]
+    assert torch.equal(x1, x2) and i1 == i2, (i1, i2)
+
+
 def test_cg_maxit_and_trivial_rhs():
     mesh, space, bc, A, b = _system(8)
     x, info = fa.cg(A, b, maxit=3, returninfo=True)
