@@ -1986,6 +1986,52 @@ __device__ inline void sep_table_cube(const fa_sep_t* __restrict__ e, const doub
     }
 }
 
+// sep_table_cube for one term of at most 3 factors at 4 points, the usual
+// source (a coefficient, perhaps a cell field, and one factor per axis): the
+// field values and the factors' table blocks loaded in one batch, then the
+// products of sep_value in its order (coefficient, field, factors in turn)
+__device__ inline void sep_table_cube_short(const fa_sep_t* __restrict__ e, const double* __restrict__ tab, int nmax,
+                                            int cx, int cy, int cz, int64_t c0, double f[6][4]) {
+    const int nf = e->term_end[0];
+    const double cm = e->coef[0];
+    const double* fld = e->field[0];
+    double fc[6];
+    const double* rows[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int kk = k < nf ? k : 0;
+        const int d = e->dim[kk];
+        const int i = d == 0 ? cx : (d == 1 ? cy : cz);
+        rows[k] = tab + (static_cast<int64_t>(kk) * nmax + i) * 24;
+    }
+    double4 r[3][6];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int t = 0; t < 6; ++t) r[k][t] = reinterpret_cast<const double4*>(rows[k])[t];
+#pragma unroll
+    for (int t = 0; t < 6; ++t) fc[t] = fld ? fld[c0 + t] : 1.0;
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+        double v[4] = {cm, cm, cm, cm};
+        if (fld) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[q] *= fc[t];
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (k >= nf) break;
+            v[0] *= r[k][t].x;
+            v[1] *= r[k][t].y;
+            v[2] *= r[k][t].z;
+            v[3] *= r[k][t].w;
+        }
+        // a single term: acc = 0 + v, as sep_value adds it
+#pragma unroll
+        for (int q = 0; q < 4; ++q) f[t][q] = 0.0 + v[q];
+    }
+}
+
 // element vectors of the 6 cells of a cube from the source values at NQ
 // points: v_i = sum_q (w_q |T| s f_q) phi_i(q), the arithmetic of the per-cell
 // kernels
@@ -2087,7 +2133,10 @@ __global__ void __launch_bounds__(256) k_p1_box_cell_vectors(int nx, int ny, int
                 const fa_sep_t* e = reinterpret_cast<const fa_sep_t*>(src);
                 if (nq == 4) {  // the usual rule (q = 2)
                     double f[6][4];
-                    sep_table_cube<4>(e, tab, nmax, cx, cy, cz, c0, f);
+                    if (e->nterms == 1 && e->term_end[0] <= 3)
+                        sep_table_cube_short(e, tab, nmax, cx, cy, cz, c0, f);
+                    else
+                        sep_table_cube<4>(e, tab, nmax, cx, cy, cz, c0, f);
                     box_cube_cell_vectors<4>(f, meas, ss, qw, phi, out);
                 } else {
                     double f[6][1];
