@@ -347,7 +347,7 @@ __global__ void __launch_bounds__(256) k_cheb16(int64_t n, const int32_t* __rest
                                                const int* __restrict__ done, DotOut dot = DotOut{}) {
     if (done && *done) return;
     double acc = 0.0;
-    const int64_t b0 = (!DOT && CS::xcd) ? xcd_block() : blockIdx.x;
+    const unsigned b0 = CS::xcd ? xcd_block() : blockIdx.x;
     for (int64_t row = b0 * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
          row += (int64_t)gridDim.x * blockDim.x) {
         const float ax = sell16_row_dot(sp, so, sv, row, n, xin);
@@ -359,7 +359,7 @@ __global__ void __launch_bounds__(256) k_cheb16(int64_t n, const int32_t* __rest
             if (DOT) acc += dot_term(dot, row, double(xn));
         }
     }
-    if (DOT) store_partial(block_sum(acc), dot.partials);
+    if (DOT) store_partial_at(block_sum(acc), dot.partials, b0);
 }
 
 template <class CS>
@@ -490,6 +490,18 @@ __global__ void __launch_bounds__(256) k_box_restrict(int mx, int my, int mz, co
     }
 }
 
+// an empty asm that takes the values as register operands: the loads that
+// produce them are all issued before this point, so none is sunk into a
+// branch behind another's wait
+template <class V>
+__device__ inline void pin_load(V& v) {
+    asm volatile("" : "+v"(v));
+}
+template <class... V>
+__device__ inline void pin_loads(V&... v) {
+    (pin_load(v), ...);
+}
+
 // x_f += P x_c over the fine nodes
 template <class T>
 __global__ void __launch_bounds__(256) k_box_prolong(int nx, int ny, int nz, const uint8_t* __restrict__ bd_f,
@@ -508,8 +520,12 @@ __global__ void __launch_bounds__(256) k_box_prolong(int nx, int ny, int nz, con
         const bool two = ox | oy | oz;
         const int64_t c0 = grid_node(x >> 1, y >> 1, z >> 1, my, mz);
         const int64_t c1 = two ? grid_node((x >> 1) + ox, (y >> 1) + oy, (z >> 1) + oz, my, mz) : c0;
-        const uint8_t fb = bd_f[f], b0 = bd_c[c0], b1 = bd_c[c1];
-        const T x0 = xc[c0], x1 = xc[c1], y0 = xf[f];
+        int fb = bd_f[f], b0 = bd_c[c0], b1 = bd_c[c1];
+        T x0 = xc[c0], x1 = xc[c1];
+        const T y0 = xf[f];
+        // every address is valid: the six loads go out together, before the
+        // branches on the flags (which would otherwise wait on bd_f first)
+        pin_loads(fb, b0, b1, x0, x1);
         const T w = two ? T(0.5) : T(1);
         T acc = T(0);
         if (!fb) {
@@ -849,6 +865,12 @@ constexpr int64_t FUSE_ROWS = 1024;  // one workgroup is latency-bound above ~1 
 constexpr int FUSE_BLOCK = 1024;
 constexpr size_t FUSE_LDS_MAX = 60 * 1024;  // level data kept in LDS up to this size
 
+// entries a row dot loads in one batch in the fused coarse cycle: a whole
+// coarse row (up to 16 entries) per round trip in FP32; 8 in FP64, and in the
+// workgroup solves, where 16 would spill registers
+template <class T>
+constexpr int FUSE_BATCH = sizeof(T) == 4 ? 16 : 8;
+
 struct FusedLevel {
     fa_sell_t A, R, P;
     const void* dinv;
@@ -863,9 +885,16 @@ struct FusedCycle {
     FusedLevel lv[FUSE_MAX_LEVELS];
 };
 
-template <class T>
-__device__ inline void blk_smooth(const FusedCycle& fc, const FusedLevel& L, const T* b, T*& cur, T*& nxt,
+// Smoothing of one level from the iterate in L.x (odd = false) or L.x2 (odd =
+// true); returns where the result lies.  The level's two iterate buffers swap
+// roles each step, so the caller keeps one bit a level instead of an array of
+// pointers (a pointer array indexed by the level lives in scratch memory and
+// puts one more memory round trip in front of every phase).
+template <class T, int B>
+__device__ inline bool blk_smooth(const FusedCycle& fc, const FusedLevel& L, const T* b, bool odd,
                                   bool from_zero) {
+    T* cur = static_cast<T*>(odd ? L.x2 : L.x);
+    T* nxt = static_cast<T*>(odd ? L.x : L.x2);
     const int64_t n = L.A.n, npad = (n + 63) & ~63LL;
     const T* sv = static_cast<const T*>(L.A.vals);
     const T* dinv = static_cast<const T*>(L.dinv);
@@ -892,7 +921,7 @@ __device__ inline void blk_smooth(const FusedCycle& fc, const FusedLevel& L, con
             }
         } else {
             for (int64_t row = threadIdx.x; row < npad; row += blockDim.x) {
-                const T ax = sell_row_dot<T, T, 8>(L.A.slice_ptr, L.A.cols, sv, row, cur);
+                const T ax = sell_row_dot<T, T, B>(L.A.slice_ptr, L.A.cols, sv, row, cur);
                 if (row < n) {
                     const T res = b[row] - ax;
                     const T dn = (k == 0 ? T(0) : c1 * d[row]) + c2 * dinv[row] * res;
@@ -903,18 +932,20 @@ __device__ inline void blk_smooth(const FusedCycle& fc, const FusedLevel& L, con
             T* t = cur;
             cur = nxt;
             nxt = t;
+            odd = !odd;
         }
         rho = rho_new;
         __syncthreads();
     }
+    return odd;
 }
 
-template <class T, bool ADD>
+template <class T, bool ADD, int B>
 __device__ inline void blk_apply(const fa_sell_t& M, const T* x, T* y) {
     const int64_t npad = (M.n + 63) & ~63LL;
     const T* mv = static_cast<const T*>(M.vals);
     for (int64_t row = threadIdx.x; row < npad; row += blockDim.x) {
-        const T v = sell_row_dot<T, T, 8>(M.slice_ptr, M.cols, mv, row, x);
+        const T v = sell_row_dot<T, T, B>(M.slice_ptr, M.cols, mv, row, x);
         if (row < M.n) y[row] = ADD ? y[row] + v : v;
     }
     __syncthreads();
@@ -954,25 +985,26 @@ __device__ inline T* lds_vec(char*& p, int64_t n, const void* src) {
 // prolongation and post-smoothing up, phases separated by block barriers.
 // Returns the level-0 vector that holds the result (L.x2, or L.x when level
 // 0 is the coarsest).
-template <class T>
+template <class T, int B = FUSE_BATCH<T>>
 __device__ inline T* fused_vcycle(const FusedCycle& fc, const FusedLevel* lv) {
     const int nl = fc.nlev;
-    T* cur[FUSE_MAX_LEVELS];
-    T* nxt[FUSE_MAX_LEVELS];
+    // bit l: level l's current iterate is L.x2 (else L.x); the coarsest
+    // level's solution is written to its x
+    uint32_t odd = 0;
+    auto cur = [&](int l) { return static_cast<T*>(((odd >> l) & 1u) ? lv[l].x2 : lv[l].x); };
     for (int l = 0; l < nl - 1; ++l) {
         const FusedLevel& L = lv[l];
         const T* b = static_cast<const T*>(L.b);
-        cur[l] = static_cast<T*>(L.x);
-        nxt[l] = static_cast<T*>(L.x2);
-        blk_smooth<T>(fc, L, b, cur[l], nxt[l], true);
+        if (blk_smooth<T, B>(fc, L, b, false, true)) odd |= 1u << l;
+        const T* xl = cur(l);
         T* r = static_cast<T*>(L.r);
         const int64_t n = L.A.n, npad = (n + 63) & ~63LL;
         for (int64_t row = threadIdx.x; row < npad; row += blockDim.x) {
-            const T ax = sell_row_dot<T, T, 8>(L.A.slice_ptr, L.A.cols, static_cast<const T*>(L.A.vals), row, cur[l]);
+            const T ax = sell_row_dot<T, T, B>(L.A.slice_ptr, L.A.cols, static_cast<const T*>(L.A.vals), row, xl);
             if (row < n) r[row] = b[row] - ax;
         }
         __syncthreads();
-        blk_apply<T, false>(L.R, r, static_cast<T*>(lv[l + 1].b));
+        blk_apply<T, false, B>(L.R, r, static_cast<T*>(lv[l + 1].b));
     }
     {
         const FusedLevel& C = lv[nl - 1];
@@ -985,15 +1017,15 @@ __device__ inline T* fused_vcycle(const FusedCycle& fc, const FusedLevel* lv) {
             for (int j = 0; j < n; ++j) s += row[j] * static_cast<double>(b[j]);
             x[i] = static_cast<T>(s);
         }
-        cur[nl - 1] = x;
         __syncthreads();
     }
     for (int l = nl - 2; l >= 0; --l) {
         const FusedLevel& L = lv[l];
-        blk_apply<T, true>(L.P, cur[l + 1], cur[l]);
-        blk_smooth<T>(fc, L, static_cast<const T*>(L.b), cur[l], nxt[l], false);
+        blk_apply<T, true, B>(L.P, cur(l + 1), cur(l));
+        const bool o = blk_smooth<T, B>(fc, L, static_cast<const T*>(L.b), ((odd >> l) & 1u) != 0, false);
+        odd = (odd & ~(1u << l)) | (static_cast<uint32_t>(o) << l);
     }
-    return cur[0];
+    return cur(0);
 }
 
 // LDS: every level vector, dinv and the slice offsets live in LDS (the
@@ -1099,7 +1131,7 @@ __global__ void __launch_bounds__(FUSE_BLOCK) k_pcg_wg(fa_wg_solve_t* __restrict
         for (int64_t i = threadIdx.x; i < n; i += blockDim.x) x[i] = 0.0;
         status = 2;
     } else {
-        const float* z = fused_vcycle<float>(sfc, sfc.lv);
+        const float* z = fused_vcycle<float, 8>(sfc, sfc.lv);
         double s = 0.0;
         for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
             const double zi = static_cast<double>(z[i]);
@@ -1125,7 +1157,7 @@ __global__ void __launch_bounds__(FUSE_BLOCK) k_pcg_wg(fa_wg_solve_t* __restrict
                 ru[i] = static_cast<float>(rv);
             }
             __syncthreads();
-            z = fused_vcycle<float>(sfc, sfc.lv);
+            z = fused_vcycle<float, 8>(sfc, sfc.lv);
             s = 0.0;
             for (int64_t i = threadIdx.x; i < n; i += blockDim.x) s += r[i] * static_cast<double>(z[i]);
             const double rzn = wg_sum(s, red);
@@ -1579,7 +1611,10 @@ __global__ void __launch_bounds__(RED_BLOCK) k_pcg_spmv(int64_t n, const int32_t
                                                        const PcgState* st) {
     if (st->done) return;
     double acc = 0.0;
-    for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
+    // short rows on int16 offsets: rows dealt to the XCDs in contiguous
+    // ranges, partials stored under the block index of the rows
+    const unsigned b0 = so ? xcd_block() : blockIdx.x;
+    for (int64_t row = b0 * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
          row += (int64_t)gridDim.x * blockDim.x) {
         const T v = so ? sell_row_dot_off<T>(sp, so, sv, row, n, p) : sell_row_dot<T, T>(sp, sc, sv, row, p);
         if (row < n) {
@@ -1587,7 +1622,7 @@ __global__ void __launch_bounds__(RED_BLOCK) k_pcg_spmv(int64_t n, const int32_t
             acc += double(p[row]) * double(v);
         }
     }
-    store_partial(block_sum(acc), partials);
+    store_partial_at(block_sum(acc), partials, b0);
 }
 
 // alpha = r.z / p.Ap from the SpMV's partials (every workgroup sums them in

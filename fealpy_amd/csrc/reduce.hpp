@@ -47,6 +47,14 @@ __device__ inline void store_partial(double v, double* partials) {
     if (threadIdx.x == 0) partials[blockIdx.x] = v;
 }
 
+// the same with the partial stored under the block index b whose rows the
+// block summed: a kernel that deals the rows of block b = xcd_block() to
+// physical block blockIdx.x stores each partial where the kernel with the
+// plain order stores it, so the partials and their sum are bitwise the same
+__device__ inline void store_partial_at(double v, double* partials, unsigned b) {
+    if (threadIdx.x == 0) partials[b] = v;
+}
+
 // sum of partials[0..n) in a fixed order by one block; valid in thread 0
 __device__ inline double finish_sum(const double* partials, int n) {
     constexpr int U = RED_GRID / RED_BLOCK;
