@@ -1538,7 +1538,7 @@ __global__ void __launch_bounds__(RED_BLOCK) k_pcg_init(int64_t n, const int32_t
                                                        const int32_t* __restrict__ sc, const T* __restrict__ sv,
                                                        const T* __restrict__ b, const T* __restrict__ x,
                                                        T* __restrict__ r, U* __restrict__ ru, T* __restrict__ p,
-                                                       double* __restrict__ partials) {
+                                                       double* __restrict__ partials, bool prec) {
     double rr = 0.0, bb = 0.0;
     for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
          row += (int64_t)gridDim.x * blockDim.x) {
@@ -1547,8 +1547,8 @@ __global__ void __launch_bounds__(RED_BLOCK) k_pcg_init(int64_t n, const int32_t
             const T bv = b[row];
             const T rv = bv - ax;
             r[row] = rv;
-            if (ru) ru[row] = static_cast<U>(rv);
-            else p[row] = rv;
+            if (!prec) p[row] = rv;
+            else if (ru) ru[row] = static_cast<U>(rv);
             rr += double(rv) * double(rv);
             bb += double(bv) * double(bv);
         }
@@ -1561,7 +1561,7 @@ __global__ void __launch_bounds__(RED_BLOCK) k_pcg_init(int64_t n, const int32_t
 template <class T, class U>
 __global__ void __launch_bounds__(RED_BLOCK) k_pcg_init_zero(int64_t n, const T* __restrict__ b, T* __restrict__ x,
                                                             T* __restrict__ r, U* __restrict__ ru, T* __restrict__ p,
-                                                            double* __restrict__ partials) {
+                                                            double* __restrict__ partials, bool prec) {
     double rr = 0.0, bb = 0.0;
     for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < ((n + 63) & ~63LL);
          row += (int64_t)gridDim.x * blockDim.x) {
@@ -1569,8 +1569,8 @@ __global__ void __launch_bounds__(RED_BLOCK) k_pcg_init_zero(int64_t n, const T*
             const T bv = b[row];
             x[row] = T(0);
             r[row] = bv;
-            if (ru) ru[row] = static_cast<U>(bv);
-            else p[row] = bv;
+            if (!prec) p[row] = bv;
+            else if (ru) ru[row] = static_cast<U>(bv);
             rr += double(bv) * double(bv);
             bb += double(bv) * double(bv);
         }
@@ -1628,7 +1628,8 @@ __global__ void __launch_bounds__(RED_BLOCK) k_pcg_spmv(int64_t n, const int32_t
 // alpha = r.z / p.Ap from the SpMV's partials (every workgroup sums them in
 // the finish kernel's order, so all get that kernel's alpha; workgroup 0 keeps
 // it in the state), then x += alpha p; r -= alpha Ap; then either partials of
-// r.r (no preconditioner) or the cast of r for the cycle
+// r.r (no preconditioner) or the cast of r for the cycle (none when ru is
+// null: the cycle then reads r itself)
 template <class T, class U, bool PREC>
 __global__ void __launch_bounds__(RED_BLOCK) k_pcg_update(int64_t n, T* __restrict__ x, T* __restrict__ r,
                                                          U* __restrict__ ru, const T* __restrict__ p,
@@ -1652,8 +1653,11 @@ __global__ void __launch_bounds__(RED_BLOCK) k_pcg_update(int64_t n, T* __restri
         x[i] += alpha * p[i];
         const T rv = r[i] - alpha * Ap[i];
         r[i] = rv;
-        if (PREC) ru[i] = static_cast<U>(rv);
-        else acc += double(rv) * double(rv);
+        if (PREC) {
+            if (ru) ru[i] = static_cast<U>(rv);
+        } else {
+            acc += double(rv) * double(rv);
+        }
     }
     if (!PREC) store_partial(block_sum(acc), partials);
 }
@@ -1738,7 +1742,10 @@ static int pcg_run(const fa_sell_t* A, const fa_mg_t* M, const T* b, T* x, doubl
     T* r = ar.take<T>(n);
     T* p = ar.take<T>(n);
     T* Ap = ar.take<T>(n);
-    U* ru = M ? ar.take<U>(n) : nullptr;
+    // the cycle's right-hand side: r itself when the cycle runs in r's type
+    // (the update then writes no copy), else r's cast
+    constexpr bool same = std::is_same<T, U>::value;
+    U* ru = M ? (same ? reinterpret_cast<U*>(r) : ar.take<U>(n)) : nullptr;
     U* z = M ? ar.take<U>(n) : nullptr;
     double* partials = ar.take<double>(2 * RED_GRID);
     PcgState* st = ar.take<PcgState>(1);
@@ -1754,10 +1761,11 @@ static int pcg_run(const fa_sell_t* A, const fa_mg_t* M, const T* b, T* x, doubl
     const int64_t cap = maxit > 0 ? maxit : (1LL << 62);
     hipLaunchKernelGGL(k_pcg_set, one, one, 0, s, st, atol, rtol, (long long)cap);
     if (x0_zero)
-        hipLaunchKernelGGL((k_pcg_init_zero<T, U>), grid, block, 0, s, n, b, x, r, ru, p, partials);
+        hipLaunchKernelGGL((k_pcg_init_zero<T, U>), grid, block, 0, s, n, b, x, r, same ? nullptr : ru, p, partials,
+                           M != nullptr);
     else
-        hipLaunchKernelGGL((k_pcg_init<T, U>), grid, block, 0, s, n, A->slice_ptr, A->cols, sv, b, x, r, ru, p,
-                           partials);
+        hipLaunchKernelGGL((k_pcg_init<T, U>), grid, block, 0, s, n, A->slice_ptr, A->cols, sv, b, x, r,
+                           same ? nullptr : ru, p, partials, M != nullptr);
     hipLaunchKernelGGL(k_pcg_finish<PCG_INIT>, one, block, 0, s, partials, np, st);
     hipLaunchKernelGGL(k_pcg_trivial<T>, grid, block, 0, s, n, x, st);
     FA_LAUNCH("k_pcg_init");
@@ -1783,8 +1791,8 @@ static int pcg_run(const fa_sell_t* A, const fa_mg_t* M, const T* b, T* x, doubl
         hipLaunchKernelGGL(k_pcg_spmv<T>, grid, block, 0, q, n, A->slice_ptr, A->cols, offs, sv, p, Ap, pap, st);
         timer_end(FA_TIMER_PCG_SPMV, q);
         if (M) {
-            hipLaunchKernelGGL((k_pcg_update<T, U, true>), grid, block, 0, q, n, x, r, ru, p, Ap, pap, np, partials,
-                               st);
+            hipLaunchKernelGGL((k_pcg_update<T, U, true>), grid, block, 0, q, n, x, r, same ? nullptr : ru, p, Ap, pap,
+                               np, partials, st);
             timer_begin(FA_TIMER_VCYCLE, q);
             int e = vcycle_any(M, ru, z, done, q, &rz_out);  // z = M r and the partials of r.z
             timer_end(FA_TIMER_VCYCLE, q);

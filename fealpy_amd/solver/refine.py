@@ -61,9 +61,11 @@ def cg_refine(op, b, A, M, bc, *, x0=None, atol=0.0, rtol=1e-8, inner_rtol=1e-3,
     M:  preconditioner of the inner solves (a GeometricMG set up for A);
     bc: the DirichletBC (boundary flags and values).
 
-    inner_check: the inner solves look at their convergence flag every this
-    many iterations (one host synchronisation each); the iterations past
-    convergence before the next look return at once on the device.
+    inner_check: the inner solves look at their convergence flag at most every
+    this many iterations (one host synchronisation each), the first time after
+    the iteration count of the last inner solve to the same ``inner_rtol``
+    with ``M``; the iterations past convergence before a look return at once
+    on the device.
 
     Returns x (FP64), and with ``returninfo`` a dict with the final FP64
     residual norm, the inner PCG iterations in total (``niter``) and the
@@ -108,8 +110,17 @@ def cg_refine(op, b, A, M, bc, *, x0=None, atol=0.0, rtol=1e-8, inner_rtol=1e-3,
         x.zero_()
         return (x, info) if returninfo else x
     tol = max(atol, rtol * bnorm)
+    # The inner solves first look at their flag after as many iterations as
+    # the last inner solve to the same tolerance with this preconditioner took
+    # (at most inner_check): each look past convergence costs the iterations
+    # launched before it, which return at once but still cost their launches.
+    # The iterates and counts do not depend on when the host looks.
+    hints = M.__dict__.setdefault('_inner_niter_hint', {})
     while rnorm >= tol and info['refinements'] < maxit:
-        d, inner = M.pcg(A, r, atol=0.0, rtol=inner_rtol, maxit=inner_maxit, check_every=inner_check)
+        check = max(1, min(inner_check, hints.get(inner_rtol, inner_check)))
+        d, inner = M.pcg(A, r, atol=0.0, rtol=inner_rtol, maxit=inner_maxit, check_every=check)
+        if inner['niter'] > 0:
+            hints[inner_rtol] = inner['niter']
         nat.check(L.fa_refine_update(nat.dtype_code(d.dtype), n, nat.ptr(x), nat.ptr(d), nat.ptr(bd), nat.stream()),
                   "fa_refine_update")
         info['niter'] += inner['niter']
