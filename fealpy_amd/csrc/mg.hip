@@ -882,6 +882,7 @@ struct FusedCycle {
     int nlev, degree;
     double lower, upper;
     const double* ainv;  // inverse of the coarsest operator, row-major
+    int ainv_lds;        // the LDS variant copies ainv into LDS after the level data
     FusedLevel lv[FUSE_MAX_LEVELS];
 };
 
@@ -986,7 +987,7 @@ __device__ inline T* lds_vec(char*& p, int64_t n, const void* src) {
 // Returns the level-0 vector that holds the result (L.x2, or L.x when level
 // 0 is the coarsest).
 template <class T, int B = FUSE_BATCH<T>>
-__device__ inline T* fused_vcycle(const FusedCycle& fc, const FusedLevel* lv) {
+__device__ inline T* fused_vcycle(const FusedCycle& fc, const FusedLevel* lv, const double* ainv) {
     const int nl = fc.nlev;
     // bit l: level l's current iterate is L.x2 (else L.x); the coarsest
     // level's solution is written to its x
@@ -1012,9 +1013,21 @@ __device__ inline T* fused_vcycle(const FusedCycle& fc, const FusedLevel* lv) {
         const T* b = static_cast<const T*>(C.b);
         T* x = static_cast<T*>(C.x);
         for (int i = threadIdx.x; i < n; i += blockDim.x) {
-            const double* row = fc.ainv + (int64_t)i * n;
+            const double* row = ainv + (int64_t)i * n;
             double s = 0.0;
-            for (int j = 0; j < n; ++j) s += row[j] * static_cast<double>(b[j]);
+            int j = 0;
+            // eight loads in flight, then the products added in column order
+            for (; j + 8 <= n; j += 8) {
+                double a[8], c[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    a[u] = row[j + u];
+                    c[u] = static_cast<double>(b[j + u]);
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) s += a[u] * c[u];
+            }
+            for (; j < n; ++j) s += row[j] * static_cast<double>(b[j]);
             x[i] = static_cast<T>(s);
         }
         __syncthreads();
@@ -1038,6 +1051,7 @@ __global__ void __launch_bounds__(FUSE_BLOCK) k_fused_cycle(FusedCycle fc, void*
     __shared__ FusedLevel slv[LDS ? FUSE_MAX_LEVELS : 1];
     const int nl = fc.nlev;
     const FusedLevel* lv = fc.lv;
+    const double* ainv = fc.ainv;
     if (LDS) {
         extern __shared__ char smem[];
         char* p = smem;
@@ -1057,10 +1071,16 @@ __global__ void __launch_bounds__(FUSE_BLOCK) k_fused_cycle(FusedCycle fc, void*
             L.r = lds_vec<T>(p, n, nullptr);
             if (threadIdx.x == 0) slv[l] = L;
         }
+        if (fc.ainv_lds) {
+            const int64_t nn = fc.lv[nl - 1].A.n * fc.lv[nl - 1].A.n;
+            double* a = reinterpret_cast<double*>(p);
+            for (int64_t i = threadIdx.x; i < nn; i += blockDim.x) a[i] = fc.ainv[i];
+            ainv = a;
+        }
         __syncthreads();
         lv = slv;
     }
-    T* res = fused_vcycle<T>(fc, lv);
+    T* res = fused_vcycle<T>(fc, lv, ainv);
     if (LDS) {
         T* o = static_cast<T*>(out);
         for (int64_t i = threadIdx.x; i < lv[0].A.n; i += blockDim.x) o[i] = res[i];
@@ -1131,7 +1151,7 @@ __global__ void __launch_bounds__(FUSE_BLOCK) k_pcg_wg(fa_wg_solve_t* __restrict
         for (int64_t i = threadIdx.x; i < n; i += blockDim.x) x[i] = 0.0;
         status = 2;
     } else {
-        const float* z = fused_vcycle<float, 8>(sfc, sfc.lv);
+        const float* z = fused_vcycle<float, 8>(sfc, sfc.lv, sfc.ainv);
         double s = 0.0;
         for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
             const double zi = static_cast<double>(z[i]);
@@ -1157,7 +1177,7 @@ __global__ void __launch_bounds__(FUSE_BLOCK) k_pcg_wg(fa_wg_solve_t* __restrict
                 ru[i] = static_cast<float>(rv);
             }
             __syncthreads();
-            z = fused_vcycle<float, 8>(sfc, sfc.lv);
+            z = fused_vcycle<float, 8>(sfc, sfc.lv, sfc.ainv);
             s = 0.0;
             for (int64_t i = threadIdx.x; i < n; i += blockDim.x) s += r[i] * static_cast<double>(z[i]);
             const double rzn = wg_sum(s, red);
@@ -1353,6 +1373,11 @@ struct Cycle {
         T* res = static_cast<T*>(fc.nlev == 1 ? L.x : L.x2);
         size_t lds = 0;
         for (int k = 0; k < fc.nlev; ++k) lds += fused_level_lds<T>(fc.lv[k], k == fc.nlev - 1);
+        // the coarsest inverse joins the level data in LDS when it fits
+        const int64_t nc = fc.lv[fc.nlev - 1].A.n;
+        const size_t ainv_bytes = static_cast<size_t>(nc * nc) * sizeof(double);
+        fc.ainv_lds = (lds + ainv_bytes <= FUSE_LDS_MAX) ? 1 : 0;
+        if (fc.ainv_lds) lds += ainv_bytes;
         if (fc.nlev > 1 && lds <= FUSE_LDS_MAX)
             hipLaunchKernelGGL((k_fused_cycle<T, true>), dim3(1), dim3(FUSE_BLOCK), lds, s, fc, res, done);
         else
