@@ -104,6 +104,7 @@ _SIGNATURES = {
                                              _p, _p]),
     "fa_p1_box_matrix": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _p, _p, ctypes.c_int,
                                         _p, _p, ctypes.c_int, _f64, _p, ctypes.c_int, _f64, _p, _p, _i64, _p, _p]),
+    "fa_sep_points": (ctypes.c_int, [_p, ctypes.c_int64, ctypes.c_int, _p, _p, _p]),
     "fa_box_axes_work_bytes": (_sz, []),
     "fa_box_axes": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _p, _p, _p, _p, _sz, _p]),
     "fa_p1_box_vector_work_bytes": (_sz, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),

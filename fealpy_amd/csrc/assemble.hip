@@ -2628,6 +2628,53 @@ __global__ void __launch_bounds__(256) k_box_axes_check(int nx, int ny, int nz, 
     if (bad) flag[0] = 1;  // any writer writes 1
 }
 
+// One factor at a given point, computed as the torch evaluation of the same
+// expression computes it: the argument a s + b rounded after the product and
+// after the sum, then the device library's sin, cos or exp of that argument
+// (the element kernels' sinpi form instead divides by pi first, which is
+// exact where the argument is a multiple of pi, e.g. 0 at x = 1 for sin(pi x),
+// where FP64 sin of the rounded argument gives 1.2e-16)
+__device__ inline double point_factor(int kind, double a, double b, double s) {
+    const double t = __dadd_rn(__dmul_rn(a, s), b);
+    switch (kind) {
+        case FA_SEP_SIN: return ::sin(t);
+        case FA_SEP_COS: return ::cos(t);
+        case FA_SEP_EXP: return ::exp(t);
+        case FA_SEP_AFFINE: return t;
+        default: return sep_factor(kind, a, b, s);
+    }
+}
+
+// f at n points of dimension gd (row-major): the terms of sep_value in the
+// same order, the coordinate dim[k] of the point itself in place of a
+// quadrature point (fields are not read: a field-free f)
+__global__ void __launch_bounds__(256) k_sep_points(const fa_sep_t* __restrict__ e, int64_t n, int gd,
+                                                   const double* __restrict__ pts, double* __restrict__ out) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double* x = pts + i * gd;
+        double acc = 0.0;
+        int k = 0;
+        const int nt = e->nterms;
+        for (int t = 0; t < nt; ++t) {
+            double v = e->coef[t];
+            const int kend = e->term_end[t];
+            for (; k < kend; ++k) v *= point_factor(e->kind[k], e->a[k], e->b[k], x[e->dim[k]]);
+            acc += v;
+        }
+        out[i] = acc;
+    }
+}
+
+extern "C" int fa_sep_points(const void* sep, int64_t n, int gd, const double* points, double* out,
+                             fa_stream_t stream) {
+    FA_REQUIRE(sep && n >= 0 && gd >= 1 && gd <= 3 && (n == 0 || (points && out)), "fa_sep_points: bad arguments");
+    if (n == 0) return FA_OK;
+    hipLaunchKernelGGL(k_sep_points, dim3(grid_for(n, 256, 1 << 14)), dim3(256), 0, S(stream),
+                       static_cast<const fa_sep_t*>(sep), n, gd, points, out);
+    FA_LAUNCH("k_sep_points");
+    return FA_OK;
+}
+
 extern "C" size_t fa_box_axes_work_bytes(void) { return 256; }
 
 extern "C" int fa_box_axes(int nx, int ny, int nz, const double* node_wide, double* axes, int32_t* on_grid_host,

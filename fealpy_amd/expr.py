@@ -167,6 +167,17 @@ class Sep:
         (NC, GD)), or the cells `index` selects."""
         p = torch.as_tensor(p)
         lead = p.shape[:-1]
+        if (p.is_cuda and p.dtype == torch.float64 and 1 <= p.shape[-1] <= 3 and self.max_dim() < p.shape[-1]
+                and not self.fields() and len(self.terms) <= 16 and sum(len(fs) for _, _, fs in self.terms) <= 48):
+            # field-free f at device points: one kernel (fa_sep_points), each
+            # factor's argument rounded as below, then sin / cos / exp of it
+            from . import _native as nat
+            L = nat.lib()
+            q = p.reshape(-1, p.shape[-1]).contiguous()
+            out = torch.empty(q.shape[0], dtype=torch.float64, device=p.device)
+            nat.check(L.fa_sep_points(nat.ptr(self.point_table(p.device)), q.shape[0], q.shape[1], nat.ptr(q),
+                                      nat.ptr(out), nat.stream()), "fa_sep_points")
+            return out.reshape(lead)
         out = torch.zeros(lead, dtype=torch.float64, device=p.device)
         for c, fld, fs in self.terms:
             v = torch.full(lead, c, dtype=torch.float64, device=p.device)
@@ -195,7 +206,7 @@ class Sep:
     def fields(self):
         return [f for _, f, _ in self.terms if f is not None]
 
-    def device_table(self, bcs, nc=None):
+    def device_table(self, bcs, nc=None, sinpi=True):
         """fa_sep_t in device memory for quadrature points `bcs` (NQ, TD+1,
         FP64, on the device): a uint8 tensor that owns the copy (cached per
         bcs table)."""
@@ -216,7 +227,7 @@ class Sep:
                 keep.append(fld)
                 st.field[i] = fld.data_ptr()
             for kind, d, a, b in fs:
-                if kind in (SIN, COS):        # sin(a s + b) = sinpi(a/pi s + b/pi): a cheap argument reduction
+                if sinpi and kind in (SIN, COS):  # sin(a s + b) = sinpi(a/pi s + b/pi): a cheap argument reduction
                     kind, a, b = (SINPI if kind == SIN else COSPI), a / math.pi, b / math.pi
                 st.kind[k], st.dim[k], st.a[k], st.b[k] = kind, d, a, b
                 k += 1
@@ -227,6 +238,20 @@ class Sep:
         t = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(bcs.device)
         t._fa_keep = (keep, bcs)          # the table points into these tensors
         self._dev[key] = t
+        return t
+
+    def point_table(self, device):
+        """fa_sep_t in device memory for evaluation at given points (no
+        quadrature table, no fields), cached per device."""
+        key = ('points', str(device))
+        t = self._dev.get(key)
+        if t is None:
+            if self.fields():
+                raise ValueError("point evaluation takes a field-free expression")
+            z = torch.zeros((1, 1), dtype=torch.float64, device=device)
+            t = self.device_table(z, sinpi=False)
+            self._dev.pop((z.data_ptr(), z.device), None)
+            self._dev[key] = t
         return t
 
     def max_dim(self):

@@ -263,6 +263,14 @@ int fa_p1_box_matrix(int dtype, int nx, int ny, int nz, const double* node_wide,
                      int mass_kind, double mass_scale, const double* mass_coef, const int32_t* indptr, int64_t nnz,
                      void* values, fa_stream_t stream);
 
+/* f at n cartesian points (points[n*gd], row-major, FP64) for a field-free
+ * fa_sep_t in device memory (its field pointers and bcs are not read):
+ * out[i] = sum_t coef[t] prod_k g_k(points[i*gd + dim[k]]).  Replaces the
+ * evaluation of a @cartesian callable at interpolation points, as
+ * LagrangeFESpace.boundary_interpolate does with gd
+ * (fealpy/functionspace/lagrange_fe_space.py:109-140), in one launch. */
+int fa_sep_points(const void* sep, int64_t n, int gd, const double* points, double* out, fa_stream_t stream);
+
 /* Grid lines of a from_box node array: axes[0 .. nx] = x of nodes (i, 0, 0),
  * then y of (0, j, 0), then z of (0, 0, k) (nx + ny + nz + 3 doubles), and
  * *on_grid_host = 1 when every node (i, j, k) is at (X[i], Y[j], Z[k]) (a
