@@ -489,8 +489,7 @@ class GeometricMG:
                                           ctypes.byref(h16) if h16 is not None else None,
                                           bounds[i:].data_ptr(), nat.ptr(ws), ws.numel(), nat.stream()),
                       "fa_smoother_setup")
-        for lv, lam in zip(self.levels, bounds.tolist()):           # one copy for every level's bound
-            lv.lmax = float(lam)
+        # the coarsest inverse is queued before the host waits for the bounds
         last = self.levels[-1]
         if self.coarse_dense:
             cv = last.A.values
@@ -502,6 +501,8 @@ class GeometricMG:
             nat.check(L.fa_csr_to_dense(last.n, nat.ptr(last.A.crow), nat.ptr(last.A.col), nat.ptr(cv),
                                         nat.ptr(last.dense), nat.stream()), "fa_csr_to_dense")
             nat.check(L.fa_dense_spd_inverse(last.n, nat.ptr(last.dense), nat.stream()), "fa_dense_spd_inverse")
+        for lv, lam in zip(self.levels, bounds.tolist()):           # one copy for every level's bound
+            lv.lmax = float(lam)
         self._build_desc()
         self._setup_A = A
         self._setup_version = A.values._version
